@@ -1,0 +1,330 @@
+"""ctypes front end for the CPU oracle (liboracle64/32.so) and, where it has
+been built, hvd-core's own FFT-free engine (oracle/_ref/libhvdref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests, the smoke check and the
+benchmark's CPU-baseline leg, never by the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import pathlib
+import subprocess
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB64 = HERE / "lib" / "liboracle64.so"
+LIB32 = HERE / "lib" / "liboracle32.so"
+REFLIB = HERE / "_ref" / "libhvdref.so"
+REF_SRC = pathlib.Path("/root/reference/hvd-core/src")
+
+
+def build(with_ref: bool | None = None) -> None:
+    """Compile the oracle (and the hvd-core shim when the reference checkout
+    is present)."""
+    subprocess.run(["make", "-s", "-C", str(HERE), "all"], check=True)
+    if with_ref is None:
+        with_ref = REF_SRC.is_dir()
+    if with_ref:
+        subprocess.run(["make", "-s", "-C", str(HERE), "ref"], check=True)
+
+
+class Geom(C.Structure):
+    _fields_ = [(n, C.c_int) for n in (
+        "field_width", "field_height", "active_video_start", "active_video_end",
+        "colour_burst_start", "colour_burst_end", "first_active_field_line",
+        "last_active_field_line")] + [("sample_rate", C.c_double), ("subcarrier_hz", C.c_double),
+                                      ("standard", C.c_int)]
+
+    @classmethod
+    def of(cls, g) -> "Geom":
+        return cls(g.field_width, g.field_height, g.active_video_start, g.active_video_end,
+                   g.colour_burst_start, g.colour_burst_end, g.first_active_field_line,
+                   g.last_active_field_line, g.sample_rate, g.subcarrier_hz, g.standard)
+
+
+class Cfg(C.Structure):
+    _fields_ = [("lambda_c", C.c_double), ("chroma_aniso", C.c_double),
+                ("charbonnier_eps", C.c_double), ("chroma_eps", C.c_double),
+                ("structure_coupling", C.c_double), ("diag_prior", C.c_double),
+                ("cg_iterations", C.c_int), ("irls_outer", C.c_int), ("cg_tol", C.c_double),
+                ("fast", C.c_int), ("is_pal", C.c_int), ("lambda_y", C.c_double)]
+
+    @classmethod
+    def make(cls, **kw) -> "Cfg":
+        d = dict(lambda_c=1.0, chroma_aniso=0.0, charbonnier_eps=0.5, chroma_eps=1.0,
+                 structure_coupling=0.25, diag_prior=0.0, cg_iterations=2, irls_outer=4,
+                 cg_tol=0.0, fast=1, is_pal=0, lambda_y=1.0)
+        d.update(kw)
+        return cls(**d)
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Oracle:
+    """The restatement, FP64 (`fp32=False`) or FP32 with FP64 reductions."""
+
+    def __init__(self, fp32: bool = False):
+        path = LIB32 if fp32 else LIB64
+        if not path.exists():
+            build(with_ref=False)
+        self.lib = C.CDLL(str(path))
+        self.dt = np.float32 if fp32 else np.float64
+        self.pre = "or32_" if fp32 else "or64_"
+        self.R = C.c_float if fp32 else C.c_double
+        for name, res in (("phase_per_sample", C.c_double), ("line_advance", C.c_double),
+                          ("burst_amplitude_ire", self.R), ("acc_gain", self.R),
+                          ("resolve_chroma_aniso", self.R), ("variational_refine", self.R),
+                          ("decode_field", self.R)):
+            getattr(self.lib, self.pre + name).restype = res
+
+    def f(self, name):
+        return getattr(self.lib, self.pre + name)
+
+    def arr(self, a, dtype=None):
+        return np.ascontiguousarray(a, dtype=dtype or self.dt)
+
+    # -- lock-in -------------------------------------------------------------
+    def burst_lockin(self, field, x0, x1, w0):
+        field = self.arr(field)
+        h, w = field.shape
+        z = np.zeros(2 * h, self.dt)
+        amp = np.zeros(h, self.dt)
+        self.f("burst_lockin")(_p(field), h, w, x0, x1, C.c_double(w0), _p(z), _p(amp))
+        return z[0::2] + 1j * z[1::2], amp
+
+    def tridiag_smooth(self, d, a, lam):
+        d, a = self.arr(d), self.arr(a)
+        x = np.zeros_like(d)
+        self.f("tridiag_smooth")(_p(d), _p(a), len(d), self.R(lam), _p(x))
+        return x
+
+    def burst_lockin_phase(self, field, g):
+        field = self.arr(field)
+        th = np.zeros(g.field_height, self.dt)
+        self.f("burst_lockin_phase")(_p(field), C.byref(Geom.of(g)), _p(th))
+        return th
+
+    def burst_lockin_phase_pal(self, field, g):
+        field = self.arr(field)
+        th = np.zeros(g.field_height, self.dt)
+        vs = np.zeros(g.field_height, np.int8)
+        self.f("burst_lockin_phase_pal")(_p(field), C.byref(Geom.of(g)), _p(th), _p(vs))
+        return th, vs
+
+    def burst_amplitude_ire(self, field, g):
+        return float(self.f("burst_amplitude_ire")(_p(self.arr(field)), C.byref(Geom.of(g))))
+
+    def acc_gain(self, measured, nominal):
+        return float(self.f("acc_gain")(self.R(measured), self.R(nominal)))
+
+    def make_carrier(self, theta_active, vswitch_active, g):
+        th = self.arr(theta_active)
+        lines = len(th)
+        out = np.zeros((lines, g.width * 2), self.dt)
+        vs = None if vswitch_active is None else np.ascontiguousarray(vswitch_active, np.int8)
+        self.f("make_carrier")(_p(th), None if vs is None else _p(vs), lines, C.byref(Geom.of(g)), _p(out))
+        return out[:, 0::2] + 1j * out[:, 1::2]
+
+    # -- FFT / operators -----------------------------------------------------
+    def fft2d(self, a, sign):
+        a = np.asarray(a)
+        h, w = a.shape
+        buf = np.zeros((h, 2 * w), self.dt)
+        buf[:, 0::2], buf[:, 1::2] = a.real, a.imag
+        self.f("fft2d")(_p(buf), h, w, sign)
+        return buf[:, 0::2] + 1j * buf[:, 1::2]
+
+    def real_op(self, op, a):
+        a = self.arr(a)
+        h, w = a.shape
+        d = np.zeros_like(a)
+        self.f(("dx", "dy", "dxt", "dyt")[op])(_p(a), h, w, _p(d))
+        return d
+
+    def cplx_op(self, op, a):
+        a = np.asarray(a)
+        h, w = a.shape
+        buf = self.cbuf(a)
+        out = np.zeros_like(buf)
+        self.f("cplx_op")(op, _p(buf), h, w, _p(out))
+        return self.uncbuf(out)
+
+    def cbuf(self, a):
+        a = np.asarray(a)
+        buf = np.zeros(a.shape[:-1] + (2 * a.shape[-1],), self.dt)
+        buf[..., 0::2], buf[..., 1::2] = a.real, a.imag
+        return buf
+
+    @staticmethod
+    def uncbuf(b):
+        return b[..., 0::2] + 1j * b[..., 1::2]
+
+    # -- init / solve ----------------------------------------------------------
+    def holographic_init(self, s, carrier, sample_rate, symmetry_variant=False):
+        s = self.arr(s)
+        h, w = s.shape
+        cb = self.cbuf(carrier)
+        y0 = np.zeros_like(s)
+        chi = np.zeros((h, 2 * w), self.dt)
+        self.f("holographic_init")(_p(s), _p(cb), h, w, C.c_double(sample_rate), int(symmetry_variant),
+                                   _p(y0), _p(chi))
+        return y0, self.uncbuf(chi)
+
+    def resolve_chroma_aniso(self, chi0, is_pal, forced=0.0):
+        chi0 = np.asarray(chi0)
+        h, w = chi0.shape
+        return float(self.f("resolve_chroma_aniso")(_p(self.cbuf(chi0)), h, w, int(is_pal), self.R(forced)))
+
+    def variational_refine(self, s, carrier, chi0, cfg: Cfg):
+        s = self.arr(s)
+        h, w = s.shape
+        cb = self.cbuf(carrier)
+        chi = self.cbuf(chi0)
+        y = np.zeros_like(s)
+        iters = np.zeros(max(cfg.irls_outer, 1), np.int32)
+        aniso = self.f("variational_refine")(_p(s), _p(cb), h, w, C.byref(cfg), _p(chi), _p(y), _p(iters))
+        return y, self.uncbuf(chi), float(aniso), iters
+
+    def prepare_field(self, field_ire, g, chroma_phase_deg=0.0):
+        field = self.arr(field_ire)
+        s = np.zeros((g.lines, g.width), self.dt)
+        car = np.zeros((g.lines, 2 * g.width), self.dt)
+        th = np.zeros(g.field_height, self.dt)
+        vs = np.zeros(g.field_height, np.int8)
+        self.f("prepare_field")(_p(field), C.byref(Geom.of(g)), self.R(chroma_phase_deg), _p(s), _p(car),
+                                _p(th), _p(vs))
+        return s, self.uncbuf(car), th, vs
+
+    def decode_field(self, field_ire, g, cfg: Cfg, chroma_phase_deg=0.0):
+        field = self.arr(field_ire)
+        y = np.zeros((g.lines, g.width), self.dt)
+        chi = np.zeros((g.lines, 2 * g.width), self.dt)
+        th = np.zeros(g.field_height, self.dt)
+        vs = np.zeros(g.field_height, np.int8)
+        iters = np.zeros(max(cfg.irls_outer, 1), np.int32)
+        aniso = self.f("decode_field")(_p(field), C.byref(Geom.of(g)), C.byref(cfg), self.R(chroma_phase_deg),
+                                       _p(y), _p(chi), _p(th), _p(vs), _p(iters))
+        return dict(y=y, chi=self.uncbuf(chi), theta=th, vswitch=vs, aniso=float(aniso), iters=iters)
+
+    # -- motion ---------------------------------------------------------------
+    def box_blur(self, a, r):
+        a = self.arr(a).copy()
+        h, w = a.shape
+        self.f("box_blur_prefix")(_p(a), h, w, r)
+        return a
+
+    def estimate_motion(self, y_ref, y_cur, tile=32, search=8):
+        y_ref, y_cur = self.arr(y_ref), self.arr(y_cur)
+        h, w = y_ref.shape
+        th, tw = -(-h // tile), -(-w // tile)
+        dy, dx, cf = (np.zeros((th, tw), self.dt) for _ in range(3))
+        self.f("estimate_motion")(_p(y_ref), _p(y_cur), h, w, tile, search, _p(dy), _p(dx), _p(cf))
+        return dy, dx, cf
+
+    def block_match_sad_u16(self, ref, cur, bh=8, bw=16, sy=4, sx=16):
+        ref = np.ascontiguousarray(ref, np.uint16)
+        cur = np.ascontiguousarray(cur, np.uint16)
+        h, w = ref.shape
+        nby, nbx = -(-h // bh), -(-w // bw)
+        vy = np.zeros((nby, nbx), np.int16)
+        vx = np.zeros((nby, nbx), np.int16)
+        sad = np.zeros((nby, nbx), np.int32)
+        self.f("block_match_sad_u16")(_p(ref), _p(cur), h, w, bh, bw, sy, sx, _p(vy), _p(vx), _p(sad))
+        return vy, vx, sad
+
+
+class Reference:
+    """hvd-core's own engine code (FFT-free units), FP32 as it ships."""
+
+    def __init__(self):
+        if not REFLIB.exists():
+            raise FileNotFoundError(str(REFLIB))
+        self.lib = C.CDLL(str(REFLIB))
+        self.lib.ref_burst_amplitude_ire.restype = C.c_float
+        self.lib.ref_acc_gain.restype = C.c_float
+
+    @staticmethod
+    def available() -> bool:
+        return REFLIB.exists()
+
+    @staticmethod
+    def f32(a):
+        return np.ascontiguousarray(a, np.float32)
+
+    def burst_lockin_phase(self, field, g):
+        field = self.f32(field)
+        th = np.zeros(g.field_height, np.float32)
+        self.lib.ref_burst_lockin_phase(_p(field), C.byref(Geom.of(g)), _p(th))
+        return th
+
+    def burst_lockin_phase_pal(self, field, g):
+        field = self.f32(field)
+        th = np.zeros(g.field_height, np.float32)
+        vs = np.zeros(g.field_height, np.int8)
+        self.lib.ref_burst_lockin_phase_pal(_p(field), C.byref(Geom.of(g)), _p(th), _p(vs))
+        return th, vs
+
+    def burst_amplitude_ire(self, field, g):
+        return float(self.lib.ref_burst_amplitude_ire(_p(self.f32(field)), C.byref(Geom.of(g))))
+
+    def acc_gain(self, measured, nominal):
+        return float(self.lib.ref_acc_gain(C.c_float(measured), C.c_float(nominal)))
+
+    def tridiag_smooth(self, d, a, lam):
+        d, a = self.f32(d), self.f32(a)
+        x = np.zeros_like(d)
+        self.lib.ref_tridiag_smooth(_p(d), _p(a), len(d), C.c_float(lam), _p(x))
+        return x
+
+    def make_carrier(self, theta_active, vswitch_active, g):
+        th = self.f32(theta_active)
+        out = np.zeros((len(th), 2 * g.width), np.float32)
+        vs = None if vswitch_active is None else np.ascontiguousarray(vswitch_active, np.int8)
+        self.lib.ref_make_carrier(_p(th), None if vs is None else _p(vs), len(th), C.byref(Geom.of(g)), _p(out))
+        return out[:, 0::2] + 1j * out[:, 1::2]
+
+    def real_op(self, op, a):
+        a = self.f32(a)
+        d = np.zeros_like(a)
+        self.lib.ref_real_op(op, _p(a), a.shape[0], a.shape[1], _p(d))
+        return d
+
+    def cplx_op(self, op, a):
+        a = np.asarray(a)
+        buf = np.zeros(a.shape[:-1] + (2 * a.shape[-1],), np.float32)
+        buf[..., 0::2], buf[..., 1::2] = a.real, a.imag
+        out = np.zeros_like(buf)
+        self.lib.ref_cplx_op(op, _p(buf), a.shape[0], a.shape[1], _p(out))
+        return out[..., 0::2] + 1j * out[..., 1::2]
+
+    def variational_refine(self, s, carrier, chi0, cfg: Cfg):
+        s = self.f32(s)
+        h, w = s.shape
+        cb = np.zeros((h, 2 * w), np.float32)
+        cb[:, 0::2], cb[:, 1::2] = carrier.real, carrier.imag
+        chi = np.zeros((h, 2 * w), np.float32)
+        chi[:, 0::2], chi[:, 1::2] = chi0.real, chi0.imag
+        y = np.zeros_like(s)
+        self.lib.ref_variational_refine(_p(s), _p(cb), h, w, C.byref(cfg), _p(chi), _p(y))
+        return y, chi[:, 0::2] + 1j * chi[:, 1::2]
+
+    def box_blur(self, a, r):
+        a = self.f32(a).copy()
+        self.lib.ref_box_blur(_p(a), a.shape[0], a.shape[1], r)
+        return a
+
+    def estimate_motion(self, y_ref, y_cur, tile=32, search=8):
+        y_ref, y_cur = self.f32(y_ref), self.f32(y_cur)
+        h, w = y_ref.shape
+        th, tw = -(-h // tile), -(-w // tile)
+        dy, dx, cf = (np.zeros((th, tw), np.float32) for _ in range(3))
+        self.lib.ref_estimate_motion(_p(y_ref), _p(y_cur), h, w, tile, search, _p(dy), _p(dx), _p(cf))
+        return dy, dx, cf
+
+
+def threads_hint() -> int:
+    return int(os.environ.get("OMP_NUM_THREADS", "1"))

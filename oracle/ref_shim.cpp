@@ -1,0 +1,169 @@
+// ref_shim.cpp -- extern "C" entry points over hvd-core's own engine code.
+//
+// TEST INFRASTRUCTURE ONLY.  Compiled by oracle/Makefile together with the
+// FFT-free hvd-core translation units that sit in the reference checkout
+// (engine/ntsc_geometry.cpp, lockin.cpp, variational.cpp, motion.cpp,
+// colour.cpp) into oracle/_ref/libhvdref.so.  Nothing from the reference is
+// copied here: this file only marshals plain pointers into hvd-core's own
+// Plane / FieldGeometry / HvdConfig types and calls its functions, so the
+// oracle and the GPU path can be compared with the real engine.
+//
+// engine/fft2d.cpp (and with it HolographicInit, DecodeFrame and the field
+// driver) needs FFTW3, which this image does not have; those functions are
+// not exported here.
+
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "engine/colour.h"
+#include "engine/gradients.h"
+#include "engine/lockin.h"
+#include "engine/motion.h"
+#include "engine/ntsc_geometry.h"
+#include "engine/plane.h"
+#include "engine/variational.h"
+
+namespace {
+
+struct ShimGeom {
+  int field_width, field_height, active_video_start, active_video_end;
+  int colour_burst_start, colour_burst_end, first_active_field_line, last_active_field_line;
+  double sample_rate, subcarrier_hz;
+  int standard;
+};
+
+struct ShimCfg {
+  double lambda_c, chroma_aniso, charbonnier_eps, chroma_eps, structure_coupling, diag_prior;
+  int cg_iterations, irls_outer;
+  double cg_tol;
+  int fast, is_pal;
+  double lambda_y;
+};
+
+hvd::FieldGeometry ToGeom(const ShimGeom* s) {
+  hvd::FieldGeometry g;
+  g.field_width = s->field_width;
+  g.field_height = s->field_height;
+  g.active_video_start = s->active_video_start;
+  g.active_video_end = s->active_video_end;
+  g.colour_burst_start = s->colour_burst_start;
+  g.colour_burst_end = s->colour_burst_end;
+  g.first_active_field_line = s->first_active_field_line;
+  g.last_active_field_line = s->last_active_field_line;
+  g.sample_rate = s->sample_rate;
+  g.subcarrier_hz = s->subcarrier_hz;
+  g.standard = s->standard == 1   ? hvd::VideoStandard::kPal
+               : s->standard == 2 ? hvd::VideoStandard::kPalM
+                                  : hvd::VideoStandard::kNtsc;
+  return g;
+}
+
+hvd::Plane ToPlane(const float* p, int h, int w) {
+  hvd::Plane out(h, w);
+  std::memcpy(out.data(), p, sizeof(float) * static_cast<size_t>(h) * w);
+  return out;
+}
+
+hvd::ComplexPlane ToCPlane(const float* p, int h, int w) {
+  hvd::ComplexPlane out(h, w);
+  std::memcpy(out.data(), p, sizeof(float) * 2 * static_cast<size_t>(h) * w);
+  return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+void ref_burst_lockin_phase(const float* field, const ShimGeom* sg, float* theta) {
+  const hvd::FieldGeometry g = ToGeom(sg);
+  const std::vector<float> t = hvd::BurstLockinPhase(ToPlane(field, g.field_height, g.field_width), g);
+  std::memcpy(theta, t.data(), sizeof(float) * t.size());
+}
+
+void ref_burst_lockin_phase_pal(const float* field, const ShimGeom* sg, float* theta, int8_t* vswitch) {
+  const hvd::FieldGeometry g = ToGeom(sg);
+  const hvd::PalBurstLockin l = hvd::BurstLockinPhasePal(ToPlane(field, g.field_height, g.field_width), g);
+  std::memcpy(theta, l.theta.data(), sizeof(float) * l.theta.size());
+  std::memcpy(vswitch, l.vswitch.data(), l.vswitch.size());
+}
+
+float ref_burst_amplitude_ire(const float* field, const ShimGeom* sg) {
+  const hvd::FieldGeometry g = ToGeom(sg);
+  return hvd::BurstAmplitudeIre(ToPlane(field, g.field_height, g.field_width), g);
+}
+
+float ref_acc_gain(float measured, float nominal) { return hvd::AccGain(measured, nominal); }
+
+void ref_tridiag_smooth(const float* d, const float* a, int n, float lam, float* x) {
+  const std::vector<float> r = hvd::TridiagSmooth(std::vector<float>(d, d + n), std::vector<float>(a, a + n), lam);
+  std::memcpy(x, r.data(), sizeof(float) * r.size());
+}
+
+void ref_make_carrier(const float* theta, const int8_t* vswitch, int lines, const ShimGeom* sg, float* carrier) {
+  const hvd::FieldGeometry g = ToGeom(sg);
+  const std::vector<float> t(theta, theta + lines);
+  hvd::ComplexPlane c = vswitch ? hvd::MakeCarrierPal(t, std::vector<int8_t>(vswitch, vswitch + lines), g)
+                                : hvd::MakeCarrier(t, g);
+  std::memcpy(carrier, c.data(), sizeof(float) * 2 * c.size());
+}
+
+// op: 0 Dx, 1 Dy, 2 DxT, 3 DyT over a real plane.
+void ref_real_op(int op, const float* a, int h, int w, float* d) {
+  const hvd::Plane p = ToPlane(a, h, w);
+  hvd::Plane r = op == 0 ? hvd::Dx(p) : op == 1 ? hvd::Dy(p) : op == 2 ? hvd::DxT(p) : hvd::DyT(p);
+  std::memcpy(d, r.data(), sizeof(float) * r.size());
+}
+
+// op: 0 Dx, 1 Dy, 2 DxT, 3 DyT, 4 D1, 5 D1T, 6 D2, 7 D2T over a complex plane.
+void ref_cplx_op(int op, const float* a, int h, int w, float* d) {
+  const hvd::ComplexPlane p = ToCPlane(a, h, w);
+  hvd::ComplexPlane r(h, w);
+  switch (op) {
+    case 0: hvd::DxInto(p, r); break;
+    case 1: hvd::DyInto(p, r); break;
+    case 2: hvd::DxTInto(p, r); break;
+    case 3: hvd::DyTInto(p, r); break;
+    case 4: hvd::D1Into(p, r); break;
+    case 5: hvd::D1TInto(p, r); break;
+    case 6: hvd::D2Into(p, r); break;
+    default: hvd::D2TInto(p, r); break;
+  }
+  std::memcpy(d, r.data(), sizeof(float) * 2 * r.size());
+}
+
+// VariationalRefine, 2-D (no neighbours).  chi: chi0 in, chi out.
+void ref_variational_refine(const float* s, const float* carrier, int h, int w, const ShimCfg* sc, float* chi,
+                            float* y) {
+  hvd::HvdConfig cfg;
+  cfg.lambda_c = static_cast<float>(sc->lambda_c);
+  cfg.chroma_aniso = static_cast<float>(sc->chroma_aniso);
+  cfg.charbonnier_eps = static_cast<float>(sc->charbonnier_eps);
+  cfg.chroma_eps = static_cast<float>(sc->chroma_eps);
+  cfg.structure_coupling = static_cast<float>(sc->structure_coupling);
+  cfg.diag_prior = static_cast<float>(sc->diag_prior);
+  cfg.cg_iterations = sc->cg_iterations;
+  cfg.irls_outer = sc->irls_outer;
+  cfg.cg_tol = static_cast<float>(sc->cg_tol);
+  cfg.fast = sc->fast != 0;
+  cfg.is_pal = sc->is_pal != 0;
+  const hvd::RefineResult r =
+      hvd::VariationalRefine(ToPlane(s, h, w), ToCPlane(carrier, h, w), ToCPlane(chi, h, w), cfg);
+  std::memcpy(chi, r.chroma.data(), sizeof(float) * 2 * r.chroma.size());
+  std::memcpy(y, r.luma.data(), sizeof(float) * r.luma.size());
+}
+
+void ref_box_blur(float* a, int h, int w, int r) {
+  hvd::Plane p = hvd::BoxBlur(ToPlane(a, h, w), r);
+  std::memcpy(a, p.data(), sizeof(float) * p.size());
+}
+
+void ref_estimate_motion(const float* y_ref, const float* y_cur, int h, int w, int tile, int search, float* dy,
+                         float* dx, float* conf) {
+  const hvd::MotionField m = hvd::EstimateMotion(ToPlane(y_ref, h, w), ToPlane(y_cur, h, w), tile, search);
+  std::memcpy(dy, m.dy.data(), sizeof(float) * m.dy.size());
+  std::memcpy(dx, m.dx.data(), sizeof(float) * m.dx.size());
+  std::memcpy(conf, m.confidence.data(), sizeof(float) * m.confidence.size());
+}
+
+}  // extern "C"
