@@ -1,0 +1,247 @@
+"""Throughput of the Y/C-separation hot path on one MI355X.
+
+One step = one batched decode through the C ABI (hvd_gpu_decode_fields_u16)
+of 64 NTSC 910x263 uint16 fields already resident in HBM: IRE unpack ->
+burst lock-in -> active crop + carrier -> FFT holographic init -> IRLS 4 x
+CG 25 (cg_tol 1e-12, so every iteration runs) -> Y and chi.  `value` is
+fields/s of that batched path.  Also reported: the plain path (one field per
+call), the mixed 32 NTSC + 32 PAL batch (stop at relative residual 1e-6 or
+40 CG iterations per stage), integer-SAD block matching, a live CG roofline
+from HIP events recorded on the kernels' own stream, and the hvd-core CPU
+baseline (its own lock-in + VariationalRefine compiled from source) timed on
+this host, which doubles as a check that the GPU solve matches hvd-core on
+identical inputs.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import pathlib
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "hvd-cvbs-decoding_amd"))
+
+HBM_PEAK_GBS = 8000.0
+CG_BYTES_PER_PX_ITER = 64  # algorithmic: chi r+w, g r+w, d r+w, 4 weights read
+BATCH = 64
+CFG0 = dict(cg_iterations=100, irls_outer=4, fast=0, cg_tol=1e-12)
+CFG2 = dict(cg_iterations=160, irls_outer=4, fast=0, cg_tol=1e-6)
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_fields(torch, synth, n, pal_every=0, seed0=1000):
+    raws, geoms = [], []
+    for k in range(n):
+        pal = pal_every and (k % pal_every == pal_every - 1)
+        raw = (synth.pal_field if pal else synth.ntsc_field)(seed0 + k)[0]
+        raws.append(torch.from_numpy(raw.view(np.int16)).to("cuda:0"))
+        geoms.append(synth.PAL if pal else synth.NTSC)
+    return raws, geoms
+
+
+def traffic_from_profile():
+    """HBM bytes per CG iteration (both launches) measured with rocprofv3 --pmc
+    FETCH_SIZE / WRITE_SIZE, as recorded in profiles/; None if absent."""
+    p = ROOT / "profiles" / "cg_traffic_ntsc910x263_batch64.md"
+    if not p.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_cg_iteration:\s*([0-9.eE+]+)", p.read_text())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(torch, gpu, synth, eng, n_fields=6):
+    """hvd-core's own BurstLockinPhase + VariationalRefine (compiled from the
+    reference sources with OpenMP, oracle/_ref) on n config-0 fields.  The
+    holographic init needs FFTW, which this image lacks, so the init is not
+    part of the timed CPU work (it is ~4% of the field's algorithmic bytes);
+    its chi0 comes from the GPU init and feeds both solvers, which makes the
+    comparison of their outputs exact-input."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+    if not oracle.REFLIB_OMP.exists():
+        return None
+    ref = oracle.Reference(openmp=True)
+    g = synth.NTSC
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    fields, chi0s, ss = [], [], []
+    for k in range(n_fields):
+        raw = synth.ntsc_field(5000 + k)[0]
+        f = synth.ire_from_raw(raw)
+        fields.append(f)
+    # GPU lock-in + init give the shared solver input
+    dfs = [torch.from_numpy(f).to("cuda:0") for f in fields]
+    th_d, _ = eng.burst_lockin_phase(dfs, [g] * n_fields)
+    fal = g.first_active_field_line
+    planes = []
+    for f, th in zip(dfs, th_d):
+        s = f[fal:fal + g.lines, g.active_video_start:g.active_video_end].contiguous()
+        chi = torch.zeros((g.lines, g.width, 2), device="cuda:0")
+        planes.append(eng.plane(s, chi, None, theta=th[fal:fal + g.lines].contiguous(), a0=g.active_video_start,
+                                sample_rate=g.sample_rate))
+    eng.holographic_init(planes)
+    torch.cuda.synchronize()
+    for p in planes:
+        s, chi = p._keep[0], p._keep[1]
+        ss.append(s.cpu().numpy())
+        c = chi.cpu().numpy()
+        chi0s.append(c[..., 0].astype(np.float64) + 1j * c[..., 1])
+    cfg = oracle.Cfg.make(**CFG0)
+    t0 = time.perf_counter()
+    outs, cars = [], []
+    for f, s, chi0 in zip(fields, ss, chi0s):
+        th = ref.burst_lockin_phase(f, g)
+        car = ref.make_carrier(th[fal:fal + g.lines], None, g)
+        outs.append(ref.variational_refine(s, car, chi0, cfg))
+        cars.append(car)
+    dt = time.perf_counter() - t0
+    # the same solve on the GPU from the same chi0 and the same hvd-core carrier
+    errs = []
+    for (y_ref, chi_ref), s, chi0, car in zip(outs, ss, chi0s, cars):
+        ds = torch.from_numpy(s).to("cuda:0")
+        dchi = torch.from_numpy(np.stack([chi0.real, chi0.imag], -1).astype(np.float32)).to("cuda:0")
+        dcar = torch.from_numpy(np.stack([car.real, car.imag], -1).astype(np.float32)).to("cuda:0")
+        dy = torch.zeros_like(ds)
+        eng.variational_refine([eng.plane(ds, dchi, dy, carrier=dcar)], gpu.SolverParams.make(**CFG0))
+        torch.cuda.synchronize()
+        c = dchi.cpu().numpy()
+        errs.append(max(float(np.abs(dy.cpu().numpy() - y_ref).max()),
+                        float(np.abs(c[..., 0] + 1j * c[..., 1] - chi_ref).max())))
+    return {"value": round(n_fields / dt, 3), "unit": "fields/s", "cores": cores, "kind": "reference",
+            "sample": f"{n_fields} NTSC 910x263 fields: hvd-core BurstLockinPhase + VariationalRefine "
+                      f"(IRLS 4 x CG 25), OpenMP {cores} threads, one field at a time; init excluded (FFTW absent)",
+            "gpu_vs_reference_max_abs_err_ire": float(max(errs))}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only-batched", action="store_true", help="time the headline workload only (profiling)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU layout is described in "
+              "DESIGN.md and is not measured.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from hvd_amd import gpu, synth
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    eng = gpu.Engine(0)
+    g = synth.NTSC
+    px_field = g.lines * g.width
+
+    log(f"[bench] generating {BATCH} NTSC fields")
+    raws, geoms = make_fields(torch, synth, BATCH)
+    p0 = gpu.SolverParams.make(**CFG0)
+    ys = [torch.empty((g.lines, g.width), device="cuda:0") for _ in range(BATCH)]
+    chis = [torch.empty((g.lines, g.width, 2), device="cuda:0") for _ in range(BATCH)]
+
+    def step():
+        eng.decode_fields_u16(raws, geoms, p0, outputs=(ys, chis))
+
+    log(f"[bench] warmup {args.warmup}")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    eng.set_profiling(True)
+    log(f"[bench] timing {args.steps} batched steps")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    cg_ms, cg_launches, pxit = eng.cg_timing()
+    eng.set_profiling(False)
+    fields_s = BATCH * args.steps / dt
+    ms_step = 1000.0 * dt / args.steps
+    pairs = cg_launches / 2
+    avg_pair_ms = cg_ms / pairs
+    px_per_pair = pxit / pairs  # pixels actually iterated per CG launch pair
+    achieved = CG_BYTES_PER_PX_ITER * px_per_pair / (avg_pair_ms * 1e-3) / 1e9
+    log(f"[bench] batched: {fields_s:.1f} fields/s, {ms_step:.2f} ms/step; CG pair {avg_pair_ms*1e3:.1f} us, "
+        f"{achieved:.0f} GB/s algorithmic")
+    traffic = traffic_from_profile()
+    result = {
+        "metric": "NTSC fields/s at <=1e-4 err, batched vs plain on 1 GPU; % of HBM peak",
+        "value": round(fields_s, 2), "unit": "fields/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (f64 reductions)", "data": "synthetic",
+        "config": {"workload": "ntsc_910x263_batched64_lockin_fftinit_irls4x25", "fields_per_step": BATCH,
+                   "field": "NTSC 4fsc 910x263 uint16, active 760x242", "irls_outer": 4, "cg_inner": 25,
+                   "cg_tol": 1e-12, "motion_estimation": False, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": "CG iteration (k_cg_curv + k_cg_step)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "avg_us_per_iteration": round(avg_pair_ms * 1e3, 2),
+                     "algorithmic_bytes_per_iteration": int(CG_BYTES_PER_PX_ITER * px_per_pair)},
+    }
+    if not args.only_batched:
+        # plain path: one field per call
+        nplain = 16
+        for k in range(2):
+            eng.decode_fields_u16([raws[k]], [g], p0, outputs=([ys[k]], [chis[k]]))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(nplain):
+            eng.decode_fields_u16([raws[k]], [g], p0, outputs=([ys[k]], [chis[k]]))
+        torch.cuda.synchronize()
+        plain = nplain / (time.perf_counter() - t0)
+        result["plain_fields_per_s"] = round(plain, 2)
+        result["batched_over_plain"] = round(fields_s / plain, 3)
+        log(f"[bench] plain: {plain:.1f} fields/s")
+        # mixed batch: 32 NTSC + 32 PAL, relative residual 1e-6 or 40 iterations per stage
+        mraws, mgeoms = make_fields(torch, synth, 64, pal_every=2, seed0=3000)
+        mys = [torch.empty((gg.lines, gg.width), device="cuda:0") for gg in mgeoms]
+        mchis = [torch.empty((gg.lines, gg.width, 2), device="cuda:0") for gg in mgeoms]
+        p2 = gpu.SolverParams.make(**CFG2)
+        eng.decode_fields_u16(mraws, mgeoms, p2, outputs=(mys, mchis))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            eng.decode_fields_u16(mraws, mgeoms, p2, outputs=(mys, mchis))
+        torch.cuda.synchronize()
+        result["mixed_32ntsc_32pal_fields_per_s"] = round(3 * 64 / (time.perf_counter() - t0), 2)
+        log(f"[bench] mixed: {result['mixed_32ntsc_32pal_fields_per_s']} fields/s")
+        # integer-SAD block matching: 8 consecutive fields -> 7 pairs, 16x8 blocks, +/-16 x +/-4
+        seq, _ = synth.motion_sequence(77, n_fields=8)
+        dseq = [torch.from_numpy(r.view(np.int16)).to("cuda:0") for r in seq]
+        eng.block_match_sad_u16(dseq[:-1], dseq[1:])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(20):
+            eng.block_match_sad_u16(dseq[:-1], dseq[1:])
+        torch.cuda.synchronize()
+        sad_pairs = 20 * 7 / (time.perf_counter() - t0)
+        result["sad_bm_field_pairs_per_s"] = round(sad_pairs, 1)
+        result["sad_bm_gops"] = round(sad_pairs * 910 * 263 * 33 * 9 * 3 / 1e9, 1)  # sub, abs, add per candidate px
+        log(f"[bench] SAD BM: {sad_pairs:.0f} field pairs/s")
+    if not args.no_cpu_baseline:
+        log("[bench] CPU baseline (hvd-core reference build)")
+        cb = cpu_baseline(torch, gpu, synth, eng)
+        result["cpu_baseline"] = cb
+        if cb:
+            log(f"[bench] CPU: {cb['value']} fields/s on {cb['cores']} threads; GPU vs hvd-core "
+                f"max |d| {cb['gpu_vs_reference_max_abs_err_ire']:.2e} IRE")
+    else:
+        result["cpu_baseline"] = None
+    eng.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,1048 @@
+// abi.hip -- host implementation of the C ABI declared in include/hvd_gpu.h.
+//
+// A context owns: one grow-only device workspace carved per call, a ring of
+// pinned-host/device descriptor slots (each guarded by an event so a slot is
+// never rewritten while kernels of an earlier call still read it), and the
+// FFT root tables per transform length.  Every entry point validates its
+// arguments on the host before queueing any work.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "hvd_gpu.h"
+#include "hvd_kernels.h"
+
+using namespace hvdk;
+
+namespace {
+
+constexpr int kSlots = 8;
+constexpr double kPi = 3.14159265358979323846;
+
+struct Slot {
+  void* host = nullptr;
+  void* dev = nullptr;
+  size_t cap = 0;
+  hipEvent_t ev = nullptr;
+  bool armed = false;
+};
+
+}  // namespace
+
+struct hvd_gpu_ctx {
+  int device = 0;
+  std::string err;
+  char* ws = nullptr;
+  size_t ws_cap = 0;
+  Slot slots[kSlots];
+  int next_slot = 0;
+  std::map<int, std::vector<float2>> roots;
+  int max_dyn_lds = 0;
+  // profiling (diagnostics): event pairs around every CG loop and pinned
+  // copies of the per-field solver state, read by hvd_gpu_cg_timing().
+  bool profiling = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> cg_events;
+  std::vector<long long> cg_launches;
+  std::vector<std::pair<void*, int>> cg_states;  // pinned FieldCG copies, count
+};
+
+namespace {
+
+struct Fail {
+  int code;
+};
+
+void set_err(hvd_gpu_ctx* c, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  c->err = buf;
+}
+
+#define HIPCHK(ctx, expr)                                                                     \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess) {                                                                   \
+      set_err(ctx, "%s failed: %s", #expr, hipGetErrorString(e_));                            \
+      throw Fail{HVD_GPU_ERR_HIP};                                                            \
+    }                                                                                         \
+  } while (0)
+
+#define REQUIRE(ctx, cond, ...)            \
+  do {                                     \
+    if (!(cond)) {                         \
+      set_err(ctx, __VA_ARGS__);           \
+      throw Fail{HVD_GPU_ERR_INVALID};     \
+    }                                      \
+  } while (0)
+
+#define LAUNCH_CHECK(ctx) HIPCHK(ctx, hipGetLastError())
+
+template <typename F>
+int guarded(hvd_gpu_ctx* ctx, F&& fn) {
+  if (!ctx) return HVD_GPU_ERR_INVALID;
+  int prev = 0;
+  if (hipGetDevice(&prev) != hipSuccess) prev = ctx->device;
+  if (hipSetDevice(ctx->device) != hipSuccess) {
+    set_err(ctx, "hipSetDevice(%d) failed", ctx->device);
+    return HVD_GPU_ERR_HIP;
+  }
+  int rc = HVD_GPU_OK;
+  try {
+    fn();
+    ctx->err.clear();
+  } catch (const Fail& f) {
+    rc = f.code;
+  } catch (const std::bad_alloc&) {
+    set_err(ctx, "host allocation failed");
+    rc = HVD_GPU_ERR_NOMEM;
+  }
+  if (prev != ctx->device) (void)hipSetDevice(prev);
+  return rc;
+}
+
+inline hipStream_t S(void* s) { return static_cast<hipStream_t>(s); }
+inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// ---- workspace ------------------------------------------------------------
+struct Carve {
+  size_t off = 0;
+  size_t take(size_t bytes) {
+    const size_t o = off;
+    off += align256(bytes);
+    return o;
+  }
+};
+
+char* ensure_ws(hvd_gpu_ctx* ctx, size_t bytes, hipStream_t st) {
+  if (bytes <= ctx->ws_cap) return ctx->ws;
+  HIPCHK(ctx, hipStreamSynchronize(st));
+  HIPCHK(ctx, hipDeviceSynchronize());
+  if (ctx->ws) HIPCHK(ctx, hipFree(ctx->ws));
+  ctx->ws = nullptr;
+  ctx->ws_cap = 0;
+  const size_t want = std::max(bytes + bytes / 4, size_t(64) << 20);
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, want);
+  if (e != hipSuccess) {
+    set_err(ctx, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+    throw Fail{HVD_GPU_ERR_NOMEM};
+  }
+  ctx->ws = static_cast<char*>(p);
+  ctx->ws_cap = want;
+  return ctx->ws;
+}
+
+// ---- descriptor staging ----------------------------------------------------
+struct Stage {
+  std::vector<char> buf;
+  template <typename T>
+  size_t put(const T* p, size_t n) {
+    const size_t o = align256(buf.size());
+    buf.resize(o + sizeof(T) * n);
+    if (n) std::memcpy(buf.data() + o, p, sizeof(T) * n);
+    return o;
+  }
+  template <typename T>
+  size_t put(const std::vector<T>& v) {
+    return put(v.data(), v.size());
+  }
+};
+
+// Uploads the staged bytes through the next slot; returns the device base.
+char* upload(hvd_gpu_ctx* ctx, const Stage& stg, hipStream_t st, Slot** used) {
+  Slot& sl = ctx->slots[ctx->next_slot];
+  ctx->next_slot = (ctx->next_slot + 1) % kSlots;
+  if (sl.armed) HIPCHK(ctx, hipEventSynchronize(sl.ev));
+  const size_t need = std::max<size_t>(stg.buf.size(), 1);
+  if (need > sl.cap) {
+    if (sl.host) HIPCHK(ctx, hipHostFree(sl.host));
+    if (sl.dev) HIPCHK(ctx, hipFree(sl.dev));
+    sl.host = sl.dev = nullptr;
+    const size_t cap = std::max(need * 2, size_t(1) << 16);
+    HIPCHK(ctx, hipHostMalloc(&sl.host, cap, hipHostMallocDefault));
+    HIPCHK(ctx, hipMalloc(&sl.dev, cap));
+    sl.cap = cap;
+  }
+  std::memcpy(sl.host, stg.buf.data(), stg.buf.size());
+  HIPCHK(ctx, hipMemcpyAsync(sl.dev, sl.host, stg.buf.size(), hipMemcpyHostToDevice, st));
+  *used = &sl;
+  return static_cast<char*>(sl.dev);
+}
+
+void arm(hvd_gpu_ctx* ctx, Slot* sl, hipStream_t st) {
+  HIPCHK(ctx, hipEventRecord(sl->ev, st));
+  sl->armed = true;
+}
+
+// ---- FFT plans --------------------------------------------------------------
+std::vector<int> factorize(int n) {
+  std::vector<int> f;
+  for (int r : {8, 4, 2, 3, 5, 7})
+    while (n > 1 && n % r == 0) {
+      f.push_back(r);
+      n /= r;
+    }
+  for (int p = 11; n > 1; p += 2)
+    while (n % p == 0) {
+      f.push_back(p);
+      n /= p;
+    }
+  return f;
+}
+
+const std::vector<float2>& root_table(hvd_gpu_ctx* ctx, int n) {
+  auto it = ctx->roots.find(n);
+  if (it != ctx->roots.end()) return it->second;
+  std::vector<float2> r(n);
+  for (int j = 0; j < n; ++j) {
+    const double a = -2.0 * kPi * (double)j / (double)n;
+    r[j] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  return ctx->roots.emplace(n, std::move(r)).first->second;
+}
+
+struct PlanSet {
+  std::vector<FftPlan> plans;
+  std::vector<float2> roots;
+  std::map<int, int> index;
+  int get(hvd_gpu_ctx* ctx, int n) {
+    auto it = index.find(n);
+    if (it != index.end()) return it->second;
+    FftPlan p{};
+    p.n = n;
+    const std::vector<int> f = factorize(n);
+    REQUIRE(ctx, (int)f.size() <= kMaxRadix, "FFT length %d has too many factors", n);
+    p.nst = (int)f.size();
+    for (size_t i = 0; i < f.size(); ++i) p.radix[i] = f[i];
+    p.root_off = (int)roots.size();
+    const std::vector<float2>& r = root_table(ctx, n);
+    roots.insert(roots.end(), r.begin(), r.end());
+    plans.push_back(p);
+    const int id = (int)plans.size() - 1;
+    index[n] = id;
+    return id;
+  }
+};
+
+// ---- geometry helpers --------------------------------------------------------
+double phase_per_sample(const hvd_field_geometry& g) {
+  if (g.subcarrier_hz <= 0.0) return 0.5 * kPi;
+  return 2.0 * kPi * g.subcarrier_hz / g.sample_rate;
+}
+double line_advance(const hvd_field_geometry& g) {
+  if (g.subcarrier_hz <= 0.0) {
+    if (g.standard == 1) return 2.0 * kPi * 0.7516;
+    if (g.standard == 2) return 0.5 * kPi;
+    return kPi;
+  }
+  const double cyc = g.subcarrier_hz * (double)g.field_width / g.sample_rate;
+  return 2.0 * kPi * (cyc - std::floor(cyc));
+}
+bool uses_vswitch(const hvd_field_geometry& g) { return g.standard == 1 || g.standard == 2; }
+int last_line(const hvd_field_geometry& g) {
+  return g.last_active_field_line != 0 ? g.last_active_field_line : g.field_height;
+}
+
+void check_geom(hvd_gpu_ctx* ctx, const hvd_field_geometry& g, int i) {
+  REQUIRE(ctx, g.field_width > 0 && g.field_height > 0, "field %d: empty geometry", i);
+  REQUIRE(ctx, g.field_height <= kMaxLines, "field %d: %d lines exceeds %d", i, g.field_height, kMaxLines);
+  REQUIRE(ctx, g.colour_burst_start >= 0 && g.colour_burst_end <= g.field_width, "field %d: burst window outside the line", i);
+  REQUIRE(ctx, g.colour_burst_end - g.colour_burst_start <= 67, "field %d: burst window longer than 67 samples", i);
+  REQUIRE(ctx, g.standard >= 0 && g.standard <= 2, "field %d: unknown standard %d", i, g.standard);
+  REQUIRE(ctx, !(g.subcarrier_hz > 0.0) || g.sample_rate > 0.0, "field %d: custom subcarrier needs a sample rate", i);
+}
+
+void check_active(hvd_gpu_ctx* ctx, const hvd_field_geometry& g, int i) {
+  const int lal = last_line(g);
+  REQUIRE(ctx, g.first_active_field_line >= 0 && lal <= g.field_height && lal - g.first_active_field_line >= 1,
+          "field %d: bad active line range", i);
+  REQUIRE(ctx, g.active_video_start >= 0 && g.active_video_end <= g.field_width &&
+                   g.active_video_end - g.active_video_start >= 1,
+          "field %d: bad active sample range", i);
+}
+
+LockinGeom lockin_geom(const hvd_field_geometry& g) {
+  LockinGeom L{};
+  L.fw = g.field_width;
+  L.fh = g.field_height;
+  L.x0 = g.colour_burst_start;
+  L.x1 = g.colour_burst_end;
+  L.std_phase = g.subcarrier_hz <= 0.0;
+  L.vswitch = uses_vswitch(g);
+  L.w0 = phase_per_sample(g);
+  L.line_advance = line_advance(g);
+  L.nominal_burst = g.standard == 1 ? 21.43f : 20.0f;
+  return L;
+}
+
+// ---- launch sequences -------------------------------------------------------------
+
+void run_lockin(hvd_gpu_ctx* ctx, const std::vector<LockinGeom>& G, const std::vector<const float*>& fields,
+                const std::vector<float*>& theta, const std::vector<int8_t*>& vsw, hipStream_t st) {
+  const int n = (int)G.size();
+  Stage stg;
+  const size_t oG = stg.put(G), oF = stg.put(fields), oT = stg.put(theta), oV = stg.put(vsw);
+  Slot* sl;
+  char* base = upload(ctx, stg, st, &sl);
+  hipLaunchKernelGGL(k_lockin, dim3(n), dim3(256), 0, st, (const LockinGeom*)(base + oG),
+                     (const float* const*)(base + oF), (float* const*)(base + oT), (int8_t* const*)(base + oV));
+  LAUNCH_CHECK(ctx);
+  arm(ctx, sl, st);
+}
+
+// Tables (and optionally the active crop) for a batch.
+void run_prepare(hvd_gpu_ctx* ctx, const std::vector<PrepDesc>& P, bool crop, hipStream_t st) {
+  if (P.empty()) return;
+  Stage stg;
+  const size_t oP = stg.put(P);
+  Slot* sl;
+  char* base = upload(ctx, stg, st, &sl);
+  int mx = 1, mlines = 1, mwidth = 1;
+  for (const PrepDesc& p : P) {
+    mx = std::max(mx, std::max(p.lines, p.width));
+    mlines = std::max(mlines, p.lines);
+    mwidth = std::max(mwidth, p.width);
+  }
+  hipLaunchKernelGGL(k_prepare, dim3((mx + 255) / 256, (unsigned)P.size()), dim3(256), 0, st,
+                     (const PrepDesc*)(base + oP));
+  LAUNCH_CHECK(ctx);
+  if (crop) {
+    hipLaunchKernelGGL(k_crop, dim3((mwidth + 255) / 256, mlines, (unsigned)P.size()), dim3(256), 0, st,
+                       (const PrepDesc*)(base + oP));
+    LAUNCH_CHECK(ctx);
+  }
+  arm(ctx, sl, st);
+}
+
+size_t fft_rows_lds(int n) { return sizeof(float2) * 3 * (size_t)n; }
+size_t fft_cols_lds(int n) { return sizeof(float2) * ((size_t)2 * n * kColGroup + n); }
+
+void run_init(hvd_gpu_ctx* ctx, std::vector<InitField>& F, PlanSet& ps, hipStream_t st) {
+  const int nf = (int)F.size();
+  std::vector<int> rows_fwd(nf + 1, 0), grp(nf + 1, 0), rows_inv(nf + 1, 0), tiles(nf + 1, 0);
+  int max_row = 1, max_col = 1;
+  for (int i = 0; i < nf; ++i) {
+    const InitField& f = F[i];
+    rows_fwd[i + 1] = rows_fwd[i] + f.hp;
+    grp[i + 1] = grp[i] + (f.w + kColGroup - 1) / kColGroup;
+    rows_inv[i + 1] = rows_inv[i] + f.h;
+    tiles[i + 1] = tiles[i] + ((f.w + kBlendW - 1) / kBlendW) * ((f.h + kBlendH - 1) / kBlendH);
+    max_row = std::max(max_row, f.w);
+    max_col = std::max(max_col, f.hp);
+  }
+  const size_t lds_r = fft_rows_lds(max_row), lds_c = fft_cols_lds(max_col);
+  REQUIRE(ctx, (int)lds_r <= ctx->max_dyn_lds && (int)lds_c <= ctx->max_dyn_lds,
+          "FFT sizes %d x %d exceed the LDS budget", max_col, max_row);
+  Stage stg;
+  const size_t oF = stg.put(F), oP = stg.put(ps.plans), oR = stg.put(ps.roots);
+  const size_t o1 = stg.put(rows_fwd), o2 = stg.put(grp), o3 = stg.put(rows_inv), o4 = stg.put(tiles);
+  Slot* sl;
+  char* base = upload(ctx, stg, st, &sl);
+  const InitField* dF = (const InitField*)(base + oF);
+  const FftPlan* dP = (const FftPlan*)(base + oP);
+  const float2* dR = (const float2*)(base + oR);
+  hipLaunchKernelGGL(k_fft_rows_fwd, dim3(rows_fwd[nf]), dim3(256), lds_r, st, dF, dP, dR, (const int*)(base + o1), nf);
+  LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_fft_cols_fwd, dim3(grp[nf]), dim3(256), lds_c, st, dF, dP, dR, (const int*)(base + o2), nf);
+  LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_fft_cols_inv, dim3(grp[nf]), dim3(256), lds_c, st, dF, dP, dR, (const int*)(base + o2), nf);
+  LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_fft_rows_inv, dim3(rows_inv[nf]), dim3(256), lds_r, st, dF, dP, dR, (const int*)(base + o3), nf);
+  LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_holo_blend, dim3(tiles[nf]), dim3(256), 0, st, dF, (const int*)(base + o4), nf);
+  LAUNCH_CHECK(ctx);
+  arm(ctx, sl, st);
+}
+
+struct RefineOut {
+  int* stage_iters = nullptr;
+  float* aniso = nullptr;
+};
+
+void run_refine(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_params& p, double* partials,
+                float* gx, float* gy, const std::vector<int>& sample_prefix, FieldCG* dstate, hipStream_t st,
+                RefineOut out) {
+  const int nf = (int)F.size();
+  std::vector<int> tp(nf + 1, 0);
+  for (int i = 0; i < nf; ++i) {
+    F[i].tiles_x = (F[i].w + kTileW - 1) / kTileW;
+    F[i].tile0 = tp[i];
+    tp[i + 1] = tp[i] + F[i].tiles_x * ((F[i].h + kTileH - 1) / kTileH);
+  }
+  const int nb = tp[nf];
+  const int outer = std::max(1, p.irls_outer);
+  const int cg_total = p.fast ? p.cg_iterations * 2 / 3 : p.cg_iterations;
+  const int n_inner = std::max(1, cg_total / outer);
+  const float tolf = p.cg_tol > 0.0 ? (float)p.cg_tol : (p.fast ? 0.10f : 0.02f);
+  std::vector<FieldCG> init(nf);
+  for (int i = 0; i < nf; ++i) {
+    FieldCG s{};
+    s.npix = (double)F[i].h * (double)F[i].w;
+    s.tol2 = (double)tolf * (double)tolf;
+    s.lam_y = (float)(p.lambda_y > 0.0 ? p.lambda_y : 1.0);
+    s.eps = (float)p.charbonnier_eps;
+    s.eps_c = (float)p.chroma_eps;
+    s.k = (float)p.structure_coupling;
+    s.aniso_forced = (float)p.chroma_aniso;
+    s.is_pal = p.is_pal;
+    s.n_inner = n_inner;
+    s.done = 1;
+    init[i] = s;
+  }
+  Stage stg;
+  const size_t oF = stg.put(F), oT = stg.put(tp), oI = stg.put(init), oS = stg.put(sample_prefix);
+  Slot* sl;
+  char* base = upload(ctx, stg, st, &sl);
+  const DevField* dF = (const DevField*)(base + oF);
+  const int* dtp = (const int*)(base + oT);
+  const int* dsp = (const int*)(base + oS);
+  hipLaunchKernelGGL(k_refine_begin, dim3((nf + 63) / 64), dim3(64), 0, st, dstate, (const FieldCG*)(base + oI), nf);
+  LAUNCH_CHECK(ctx);
+  if (!(p.chroma_aniso > 0.0) && sample_prefix[nf] > 0) {
+    const int blocks = std::min(4096, (sample_prefix[nf] + 255) / 256);
+    hipLaunchKernelGGL(k_aniso_samples, dim3(blocks), dim3(256), 0, st, dF, (const FieldCG*)dstate, dsp, nf, gx, gy);
+    LAUNCH_CHECK(ctx);
+  }
+  hipLaunchKernelGGL(k_aniso_select, dim3(nf), dim3(256), 0, st, dstate, dsp, (const float*)gx, (const float*)gy,
+                     (float)p.lambda_c);
+  LAUNCH_CHECK(ctx);
+  for (int o = 0; o < outer; ++o) {
+    hipLaunchKernelGGL(k_irls_weights, dim3(nb), dim3(kBlock), 0, st, dF, dstate, dtp, nf, nb);
+    LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_grad_init, dim3(nb), dim3(kBlock), 0, st, dF, dstate, dtp, nf, nb, partials);
+    LAUNCH_CHECK(ctx);
+    std::pair<hipEvent_t, hipEvent_t> evs{nullptr, nullptr};
+    if (ctx->profiling) {
+      HIPCHK(ctx, hipEventCreate(&evs.first));
+      HIPCHK(ctx, hipEventCreate(&evs.second));
+      HIPCHK(ctx, hipEventRecord(evs.first, st));
+    }
+    for (int it = 0; it < n_inner; ++it) {
+      hipLaunchKernelGGL(k_cg_curv, dim3(nb), dim3(kBlock), 0, st, dF, dstate, dtp, nf, nb, partials, it);
+      LAUNCH_CHECK(ctx);
+      hipLaunchKernelGGL(k_cg_step, dim3(nb), dim3(kBlock), 0, st, dF, dstate, dtp, nf, nb, partials);
+      LAUNCH_CHECK(ctx);
+    }
+    if (ctx->profiling) {
+      HIPCHK(ctx, hipEventRecord(evs.second, st));
+      ctx->cg_events.push_back(evs);
+      ctx->cg_launches.push_back(2LL * n_inner);
+    }
+    hipLaunchKernelGGL(k_stage_end, dim3((nf + 63) / 64), dim3(64), 0, st, dstate, nf, o);
+    LAUNCH_CHECK(ctx);
+  }
+  hipLaunchKernelGGL(k_refine_finalize, dim3(nb), dim3(kBlock), 0, st, dF, (const FieldCG*)dstate, dtp, nf, nb);
+  LAUNCH_CHECK(ctx);
+  arm(ctx, sl, st);
+  if (ctx->profiling) {
+    void* h = nullptr;
+    HIPCHK(ctx, hipHostMalloc(&h, sizeof(FieldCG) * nf, hipHostMallocDefault));
+    ctx->cg_states.emplace_back(h, nf);
+    HIPCHK(ctx, hipMemcpyAsync(h, dstate, sizeof(FieldCG) * nf, hipMemcpyDeviceToHost, st));
+  }
+  if (out.stage_iters || out.aniso) {
+    std::vector<FieldCG> hs(nf);
+    HIPCHK(ctx, hipMemcpyAsync(hs.data(), dstate, sizeof(FieldCG) * nf, hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    for (int i = 0; i < nf; ++i) {
+      if (out.stage_iters)
+        for (int o = 0; o < outer; ++o) out.stage_iters[i * outer + o] = o < 16 ? hs[i].iters[o] : -1;
+      if (out.aniso) out.aniso[i] = (float)(hs[i].mu_h / (p.lambda_c != 0.0 ? p.lambda_c : 1.0));
+    }
+  }
+}
+
+int aniso_samples(int h, int w, int is_pal) {
+  const int group = is_pal ? 4 : 2;
+  const int hh = h / group;
+  const int nt = hh / 2, nx = (w - 1 + 2) / 3;
+  return (w >= 2 && nt > 0) ? nt * nx : 0;
+}
+
+// End offset of the solver workspace refine_planes carves after `cv`.
+size_t refine_ws_end(Carve cv, const std::vector<DevField>& F, int is_pal) {
+  size_t tiles = 0, samples = 0;
+  for (const DevField& f : F) {
+    const size_t np = (size_t)f.h * f.w;
+    cv.take(sizeof(float2) * np);
+    cv.take(sizeof(float2) * np);
+    cv.take(sizeof(float2) * np * 2);
+    cv.take(sizeof(float4) * np);
+    tiles += (size_t)((f.w + kTileW - 1) / kTileW) * ((f.h + kTileH - 1) / kTileH);
+    samples += aniso_samples(f.h, f.w, is_pal);
+  }
+  cv.take(sizeof(double) * 2 * tiles);
+  cv.take(sizeof(float) * (samples + 1));
+  cv.take(sizeof(float) * (samples + 1));
+  cv.take(sizeof(FieldCG) * F.size());
+  return cv.off;
+}
+
+// Refine workspace for planes already described by DevField (chi[0], s, y and
+// the carrier set); fills the remaining pointers from the carve.
+void check_params(hvd_gpu_ctx* ctx, const hvd_solver_params* p) {
+  REQUIRE(ctx, p != nullptr, "null solver parameters");
+  REQUIRE(ctx, p->diag_prior == 0.0, "diag_prior > 0 is not supported by this build");
+  REQUIRE(ctx, p->charbonnier_eps > 0.0 && p->chroma_eps > 0.0, "Charbonnier scales must be positive");
+  REQUIRE(ctx, p->cg_iterations >= 0, "negative cg_iterations");
+}
+
+void check_plane(hvd_gpu_ctx* ctx, const hvd_gpu_plane& pl, int i) {
+  REQUIRE(ctx, pl.lines >= 1 && pl.width >= 1, "plane %d: empty", i);
+  REQUIRE(ctx, pl.s && pl.chi, "plane %d: null s or chi", i);
+  REQUIRE(ctx, pl.carrier || pl.theta, "plane %d: needs a carrier or theta", i);
+  REQUIRE(ctx, pl.width <= 2048 && pl.lines + 32 <= 1024, "plane %d: %d x %d exceeds the FFT limits (1024 - 32 lines, 2048 samples)",
+          i, pl.lines, pl.width);
+}
+
+PrepDesc plane_tables(const hvd_gpu_plane& pl, float2* rowph, int8_t* vsw, float2* colph) {
+  PrepDesc d{};
+  d.theta_field = pl.theta;
+  d.vswitch_field = pl.vswitch;
+  d.rowph = rowph;
+  d.vsw_active = pl.vswitch ? vsw : nullptr;
+  d.colph = colph;
+  d.fal = 0;
+  d.a0 = pl.active_video_start;
+  d.lines = pl.lines;
+  d.width = pl.width;
+  d.std_phase = pl.phase_per_sample == 0.5 * kPi;
+  d.w0 = pl.phase_per_sample;
+  return d;
+}
+
+}  // namespace
+
+namespace {
+void clear_profile(hvd_gpu_ctx* ctx) {
+  for (auto& e : ctx->cg_events) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  for (auto& p : ctx->cg_states) (void)hipHostFree(p.first);
+  ctx->cg_events.clear();
+  ctx->cg_launches.clear();
+  ctx->cg_states.clear();
+}
+}  // namespace
+
+// ============================================================================
+extern "C" {
+
+int hvd_gpu_abi_version(void) { return 100; }
+
+int hvd_gpu_create(int device, hvd_gpu_ctx** out) {
+  if (!out) return HVD_GPU_ERR_INVALID;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return HVD_GPU_ERR_INVALID;
+  hvd_gpu_ctx* ctx = new hvd_gpu_ctx;
+  ctx->device = device;
+  const int rc = guarded(ctx, [&] {
+    for (Slot& s : ctx->slots) HIPCHK(ctx, hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
+    int lds = 0;
+    HIPCHK(ctx, hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    ctx->max_dyn_lds = lds;
+    for (const void* k : {(const void*)k_fft_rows_fwd, (const void*)k_fft_cols_fwd, (const void*)k_fft_cols_inv,
+                          (const void*)k_fft_rows_inv})
+      HIPCHK(ctx, hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  });
+  if (rc != HVD_GPU_OK) {
+    hvd_gpu_destroy(ctx);
+    return rc;
+  }
+  *out = ctx;
+  return HVD_GPU_OK;
+}
+
+void hvd_gpu_destroy(hvd_gpu_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipDeviceSynchronize();
+  for (Slot& s : ctx->slots) {
+    if (s.ev) (void)hipEventDestroy(s.ev);
+    if (s.host) (void)hipHostFree(s.host);
+    if (s.dev) (void)hipFree(s.dev);
+  }
+  if (ctx->ws) (void)hipFree(ctx->ws);
+  clear_profile(ctx);
+  delete ctx;
+}
+
+const char* hvd_gpu_last_error(const hvd_gpu_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int hvd_gpu_synchronize(hvd_gpu_ctx* ctx, void* stream) {
+  return guarded(ctx, [&] { HIPCHK(ctx, hipStreamSynchronize(S(stream))); });
+}
+
+int hvd_gpu_set_profiling(hvd_gpu_ctx* ctx, int enable) {
+  return guarded(ctx, [&] {
+    HIPCHK(ctx, hipDeviceSynchronize());
+    clear_profile(ctx);
+    ctx->profiling = enable != 0;
+  });
+}
+
+int hvd_gpu_cg_timing(hvd_gpu_ctx* ctx, double* ms, long long* launches, double* pixel_iterations) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, ms && launches && pixel_iterations, "null output");
+    HIPCHK(ctx, hipDeviceSynchronize());
+    double tot = 0.0, pxit = 0.0;
+    long long nl = 0;
+    for (size_t i = 0; i < ctx->cg_events.size(); ++i) {
+      float t = 0.f;
+      HIPCHK(ctx, hipEventElapsedTime(&t, ctx->cg_events[i].first, ctx->cg_events[i].second));
+      tot += t;
+      nl += ctx->cg_launches[i];
+    }
+    for (auto& p : ctx->cg_states) {
+      const FieldCG* st = static_cast<const FieldCG*>(p.first);
+      for (int f = 0; f < p.second; ++f) {
+        long long it = 0;
+        for (int o = 0; o < 16; ++o) it += st[f].iters[o];
+        pxit += st[f].npix * (double)it;
+      }
+    }
+    *ms = tot;
+    *launches = nl;
+    *pixel_iterations = pxit;
+    clear_profile(ctx);
+  });
+}
+
+int hvd_gpu_sample_to_ire_u16(hvd_gpu_ctx* ctx, const uint16_t* raw, size_t n, float blanking, float white,
+                              float* out, void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, raw && out, "null buffer");
+    REQUIRE(ctx, white != blanking, "white level equals blanking");
+    if (n == 0) return;
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_u16_to_ire, dim3(blocks), dim3(256), 0, S(stream), raw, n, blanking,
+                       (white - blanking) / 100.0f, out);
+    LAUNCH_CHECK(ctx);
+  });
+}
+
+int hvd_gpu_burst_lockin_phase(hvd_gpu_ctx* ctx, const float* const* fields, int n, const hvd_field_geometry* geom,
+                               float* const* theta, int8_t* const* vswitch, void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, n >= 0 && (n == 0 || (fields && geom && theta)), "null argument");
+    if (n == 0) return;
+    std::vector<LockinGeom> G(n);
+    std::vector<const float*> F(n);
+    std::vector<float*> T(n);
+    std::vector<int8_t*> V(n);
+    for (int i = 0; i < n; ++i) {
+      check_geom(ctx, geom[i], i);
+      REQUIRE(ctx, fields[i] && theta[i], "field %d: null pointer", i);
+      G[i] = lockin_geom(geom[i]);
+      F[i] = fields[i];
+      T[i] = theta[i];
+      V[i] = vswitch ? vswitch[i] : nullptr;
+      REQUIRE(ctx, !G[i].vswitch || V[i], "field %d: PAL-family field needs a vswitch buffer", i);
+    }
+    run_lockin(ctx, G, F, T, V, S(stream));
+  });
+}
+
+int hvd_gpu_burst_amplitude_ire(hvd_gpu_ctx* ctx, const float* const* fields, int n, const hvd_field_geometry* geom,
+                                float* amp, void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, n >= 0 && (n == 0 || (fields && geom && amp)), "null argument");
+    if (n == 0) return;
+    std::vector<LockinGeom> G(n);
+    std::vector<const float*> F(n);
+    for (int i = 0; i < n; ++i) {
+      check_geom(ctx, geom[i], i);
+      REQUIRE(ctx, fields[i], "field %d: null pointer", i);
+      G[i] = lockin_geom(geom[i]);
+      F[i] = fields[i];
+    }
+    Stage stg;
+    const size_t oG = stg.put(G), oF = stg.put(F);
+    Slot* sl;
+    char* base = upload(ctx, stg, S(stream), &sl);
+    hipLaunchKernelGGL(k_burst_amp, dim3(n), dim3(256), 0, S(stream), (const LockinGeom*)(base + oG),
+                       (const float* const*)(base + oF), amp);
+    LAUNCH_CHECK(ctx);
+    arm(ctx, sl, S(stream));
+  });
+}
+
+int hvd_gpu_holographic_init(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, int symmetry_variant,
+                             void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, n >= 0 && (n == 0 || planes), "null argument");
+    if (n == 0) return;
+    hipStream_t st = S(stream);
+    Carve cv;
+    std::vector<size_t> o_row(n), o_col(n), o_vsw(n), o_spec(n), o_var(n);
+    const int nvar = symmetry_variant ? 3 : 2;
+    for (int i = 0; i < n; ++i) {
+      const hvd_gpu_plane& pl = planes[i];
+      check_plane(ctx, pl, i);
+      const int pad = pl.lines > 1 ? std::min(16, pl.lines - 1) : 0;
+      const size_t np = (size_t)pl.lines * pl.width;
+      o_row[i] = cv.take(sizeof(float2) * pl.lines);
+      o_vsw[i] = cv.take(pl.lines);
+      o_col[i] = cv.take(sizeof(float2) * pl.width);
+      o_spec[i] = cv.take(sizeof(float2) * (size_t)(pl.lines + 2 * pad) * pl.width);
+      o_var[i] = cv.take(sizeof(float2) * np * nvar);
+    }
+    char* ws = ensure_ws(ctx, cv.off, st);
+    PlanSet ps;
+    std::vector<PrepDesc> prep;
+    std::vector<InitField> F(n);
+    for (int i = 0; i < n; ++i) {
+      const hvd_gpu_plane& pl = planes[i];
+      InitField f{};
+      f.h = pl.lines;
+      f.w = pl.width;
+      f.pad = pl.lines > 1 ? std::min(16, pl.lines - 1) : 0;
+      f.hp = f.h + 2 * f.pad;
+      f.s = pl.s;
+      f.carrier = reinterpret_cast<const float2*>(pl.carrier);
+      if (!pl.carrier) {
+        prep.push_back(plane_tables(pl, (float2*)(ws + o_row[i]), (int8_t*)(ws + o_vsw[i]), (float2*)(ws + o_col[i])));
+        f.rowph = (const float2*)(ws + o_row[i]);
+        f.colph = (const float2*)(ws + o_col[i]);
+        f.vsw = pl.vswitch ? (const int8_t*)(ws + o_vsw[i]) : nullptr;
+      }
+      f.spec = (float2*)(ws + o_spec[i]);
+      for (int v = 0; v < nvar; ++v) f.var[v] = (float2*)(ws + o_var[i]) + (size_t)v * f.h * f.w;
+      f.chi0 = reinterpret_cast<float2*>(pl.chi);
+      f.y0 = pl.y;
+      f.row_plan = ps.get(ctx, f.w);
+      f.col_plan = ps.get(ctx, f.hp);
+      f.nvar = nvar;
+      f.sample_rate = (float)pl.sample_rate;
+      F[i] = f;
+    }
+    run_prepare(ctx, prep, false, st);
+    run_init(ctx, F, ps, st);
+  });
+}
+
+namespace {
+
+// Shared by the plane-level refine and the field driver: carve the solver
+// workspace after `cv` and run.
+void refine_planes(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_params& p, hipStream_t st,
+                   Carve& cv, RefineOut out) {
+  const int nf = (int)F.size();
+  std::vector<size_t> o_chi1(nf), o_g(nf), o_d(nf), o_w(nf);
+  std::vector<int> sp(nf + 1, 0);
+  size_t tiles = 0;
+  for (int i = 0; i < nf; ++i) {
+    const size_t np = (size_t)F[i].h * F[i].w;
+    o_chi1[i] = cv.take(sizeof(float2) * np);
+    o_g[i] = cv.take(sizeof(float2) * np);
+    o_d[i] = cv.take(sizeof(float2) * np * 2);
+    o_w[i] = cv.take(sizeof(float4) * np);
+    tiles += (size_t)((F[i].w + kTileW - 1) / kTileW) * ((F[i].h + kTileH - 1) / kTileH);
+    sp[i + 1] = sp[i] + aniso_samples(F[i].h, F[i].w, p.is_pal);
+  }
+  const size_t o_part = cv.take(sizeof(double) * 2 * tiles);
+  const size_t o_gx = cv.take(sizeof(float) * (sp[nf] + 1));
+  const size_t o_gy = cv.take(sizeof(float) * (sp[nf] + 1));
+  const size_t o_st = cv.take(sizeof(FieldCG) * nf);
+  char* ws = ensure_ws(ctx, cv.off, st);
+  // table pointers were expressed as offsets until the workspace existed
+  for (int i = 0; i < nf; ++i) {
+    F[i].chi[1] = (float2*)(ws + o_chi1[i]);
+    F[i].g = (float2*)(ws + o_g[i]);
+    F[i].d[0] = (float2*)(ws + o_d[i]);
+    F[i].d[1] = F[i].d[0] + (size_t)F[i].h * F[i].w;
+    F[i].wts = (float4*)(ws + o_w[i]);
+  }
+  run_refine(ctx, F, p, (double*)(ws + o_part), (float*)(ws + o_gx), (float*)(ws + o_gy), sp,
+             (FieldCG*)(ws + o_st), st, out);
+}
+
+}  // namespace
+
+int hvd_gpu_variational_refine(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hvd_solver_params* p,
+                               int* stage_iters, float* aniso, void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, n >= 0 && (n == 0 || planes), "null argument");
+    check_params(ctx, p);
+    if (n == 0) return;
+    hipStream_t st = S(stream);
+    Carve cv;
+    std::vector<size_t> o_row(n), o_col(n), o_vsw(n);
+    for (int i = 0; i < n; ++i) {
+      check_plane(ctx, planes[i], i);
+      o_row[i] = cv.take(sizeof(float2) * planes[i].lines);
+      o_vsw[i] = cv.take(planes[i].lines);
+      o_col[i] = cv.take(sizeof(float2) * planes[i].width);
+    }
+    std::vector<DevField> F(n);
+    for (int i = 0; i < n; ++i) {
+      DevField f{};
+      f.h = planes[i].lines;
+      f.w = planes[i].width;
+      f.s = planes[i].s;
+      f.carrier = reinterpret_cast<const float2*>(planes[i].carrier);
+      f.chi[0] = reinterpret_cast<float2*>(planes[i].chi);
+      f.y = planes[i].y;
+      F[i] = f;
+    }
+    // size the whole call (tables + solver) before any pointer is taken
+    ensure_ws(ctx, refine_ws_end(cv, F, p->is_pal), st);
+    char* ws = ctx->ws;
+    std::vector<PrepDesc> prep;
+    for (int i = 0; i < n; ++i) {
+      if (!planes[i].carrier) {
+        prep.push_back(plane_tables(planes[i], (float2*)(ws + o_row[i]), (int8_t*)(ws + o_vsw[i]),
+                                    (float2*)(ws + o_col[i])));
+        F[i].rowph = (const float2*)(ws + o_row[i]);
+        F[i].colph = (const float2*)(ws + o_col[i]);
+        F[i].vsw = planes[i].vswitch ? (const int8_t*)(ws + o_vsw[i]) : nullptr;
+      }
+    }
+    run_prepare(ctx, prep, false, st);
+    refine_planes(ctx, F, *p, st, cv, RefineOut{stage_iters, aniso});
+  });
+}
+
+namespace {
+
+int decode_fields(hvd_gpu_ctx* ctx, const uint16_t* const* raw, const float* const* field_ire, int n,
+                  const hvd_field_geometry* geom, float blanking, float white, const hvd_solver_params* p,
+                  float chroma_phase_deg, float* const* y_out, float* const* chi_out, float* const* theta_out,
+                  void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, n >= 0 && (n == 0 || (geom && y_out && chi_out && (raw || field_ire))), "null argument");
+    check_params(ctx, p);
+    if (raw) REQUIRE(ctx, white != blanking, "white level equals blanking");
+    if (n == 0) return;
+    hipStream_t st = S(stream);
+    Carve cv;
+    std::vector<size_t> o_ire(n), o_th(n), o_vs(n), o_s(n), o_row(n), o_vsa(n), o_col(n), o_spec(n), o_var(n);
+    for (int i = 0; i < n; ++i) {
+      const hvd_field_geometry& g = geom[i];
+      check_geom(ctx, g, i);
+      check_active(ctx, g, i);
+      REQUIRE(ctx, (raw ? (const void*)raw[i] : (const void*)field_ire[i]) && y_out[i] && chi_out[i],
+              "field %d: null pointer", i);
+      const int lines = last_line(g) - g.first_active_field_line, width = g.active_video_end - g.active_video_start;
+      REQUIRE(ctx, width <= 2048 && lines + 32 <= 1024, "field %d: active area exceeds the FFT limits", i);
+      const int pad = lines > 1 ? std::min(16, lines - 1) : 0;
+      const size_t np = (size_t)lines * width;
+      o_ire[i] = raw ? cv.take(sizeof(float) * (size_t)g.field_width * g.field_height) : 0;
+      o_th[i] = cv.take(sizeof(float) * g.field_height);
+      o_vs[i] = cv.take(g.field_height);
+      o_s[i] = cv.take(sizeof(float) * np);
+      o_row[i] = cv.take(sizeof(float2) * lines);
+      o_vsa[i] = cv.take(lines);
+      o_col[i] = cv.take(sizeof(float2) * width);
+      o_spec[i] = cv.take(sizeof(float2) * (size_t)(lines + 2 * pad) * width);
+      o_var[i] = cv.take(sizeof(float2) * np * 2);
+    }
+    // size for the solver's part too, then carve it in refine_planes
+    {
+      Carve full = cv;
+      size_t tiles = 0, samples = 0;
+      for (int i = 0; i < n; ++i) {
+        const hvd_field_geometry& g = geom[i];
+        const int lines = last_line(g) - g.first_active_field_line, width = g.active_video_end - g.active_video_start;
+        const size_t np = (size_t)lines * width;
+        full.take(sizeof(float2) * np);
+        full.take(sizeof(float2) * np);
+        full.take(sizeof(float2) * np * 2);
+        full.take(sizeof(float4) * np);
+        tiles += (size_t)((width + kTileW - 1) / kTileW) * ((lines + kTileH - 1) / kTileH);
+        samples += aniso_samples(lines, width, uses_vswitch(g));
+      }
+      full.take(sizeof(double) * 2 * tiles);
+      full.take(sizeof(float) * (samples + 1));
+      full.take(sizeof(float) * (samples + 1));
+      full.take(sizeof(FieldCG) * n);
+      ensure_ws(ctx, full.off, st);
+    }
+    char* ws = ctx->ws;
+    // (1) codes -> IRE
+    std::vector<const float*> fields(n);
+    for (int i = 0; i < n; ++i) {
+      if (raw) {
+        const hvd_field_geometry& g = geom[i];
+        const size_t cnt = (size_t)g.field_width * g.field_height;
+        float* dst = (float*)(ws + o_ire[i]);
+        hipLaunchKernelGGL(k_u16_to_ire, dim3((unsigned)std::min<size_t>((cnt + 255) / 256, 65536)), dim3(256), 0, st,
+                           raw[i], cnt, blanking, (white - blanking) / 100.0f, dst);
+        LAUNCH_CHECK(ctx);
+        fields[i] = dst;
+      } else {
+        fields[i] = field_ire[i];
+      }
+    }
+    // (2) lock-in
+    std::vector<LockinGeom> G(n);
+    std::vector<float*> T(n);
+    std::vector<int8_t*> V(n);
+    for (int i = 0; i < n; ++i) {
+      G[i] = lockin_geom(geom[i]);
+      T[i] = (float*)(ws + o_th[i]);
+      V[i] = (int8_t*)(ws + o_vs[i]);
+    }
+    run_lockin(ctx, G, fields, T, V, st);
+    // (3) active crop + carrier tables (PrepareFieldObs + ApplyChromaPhase)
+    std::vector<PrepDesc> prep(n);
+    for (int i = 0; i < n; ++i) {
+      const hvd_field_geometry& g = geom[i];
+      PrepDesc d{};
+      d.field = fields[i];
+      d.theta_field = T[i];
+      d.vswitch_field = uses_vswitch(g) ? V[i] : nullptr;
+      d.s = (float*)(ws + o_s[i]);
+      d.rowph = (float2*)(ws + o_row[i]);
+      d.vsw_active = uses_vswitch(g) ? (int8_t*)(ws + o_vsa[i]) : nullptr;
+      d.colph = (float2*)(ws + o_col[i]);
+      d.fw = g.field_width;
+      d.fal = g.first_active_field_line;
+      d.a0 = g.active_video_start;
+      d.lines = last_line(g) - g.first_active_field_line;
+      d.width = g.active_video_end - g.active_video_start;
+      d.std_phase = g.subcarrier_hz <= 0.0;
+      d.signed_by_parity = uses_vswitch(g);
+      d.phase_offset = chroma_phase_deg == 0.0f ? 0.0f : -chroma_phase_deg * 3.14159265358979323846f / 180.0f;
+      d.w0 = phase_per_sample(g);
+      prep[i] = d;
+    }
+    run_prepare(ctx, prep, true, st);
+    if (theta_out)
+      for (int i = 0; i < n; ++i)
+        if (theta_out[i])
+          HIPCHK(ctx, hipMemcpyAsync(theta_out[i], T[i], sizeof(float) * geom[i].field_height,
+                                     hipMemcpyDeviceToDevice, st));
+    // (4) holographic init into the caller's chi / y
+    PlanSet ps;
+    std::vector<InitField> IF(n);
+    for (int i = 0; i < n; ++i) {
+      InitField f{};
+      f.h = prep[i].lines;
+      f.w = prep[i].width;
+      f.pad = f.h > 1 ? std::min(16, f.h - 1) : 0;
+      f.hp = f.h + 2 * f.pad;
+      f.s = prep[i].s;
+      f.rowph = prep[i].rowph;
+      f.vsw = prep[i].vsw_active;
+      f.colph = prep[i].colph;
+      f.spec = (float2*)(ws + o_spec[i]);
+      f.var[0] = (float2*)(ws + o_var[i]);
+      f.var[1] = f.var[0] + (size_t)f.h * f.w;
+      f.chi0 = reinterpret_cast<float2*>(chi_out[i]);
+      f.y0 = y_out[i];
+      f.row_plan = ps.get(ctx, f.w);
+      f.col_plan = ps.get(ctx, f.hp);
+      f.nvar = 2;
+      f.sample_rate = (float)geom[i].sample_rate;
+      IF[i] = f;
+    }
+    run_init(ctx, IF, ps, st);
+    // (5) solve (sequence.cpp:1046: always, with is_pal from the geometry)
+    std::vector<DevField> F(n);
+    bool any_pal = false, any_ntsc = false;
+    for (int i = 0; i < n; ++i) {
+      DevField f{};
+      f.h = IF[i].h;
+      f.w = IF[i].w;
+      f.s = IF[i].s;
+      f.rowph = IF[i].rowph;
+      f.vsw = IF[i].vsw;
+      f.colph = IF[i].colph;
+      f.chi[0] = IF[i].chi0;
+      f.y = y_out[i];
+      F[i] = f;
+      (uses_vswitch(geom[i]) ? any_pal : any_ntsc) = true;
+    }
+    hvd_solver_params pp = *p;
+    if (any_pal && any_ntsc) {
+      // Mixed batch: the AUTO aniso line grouping differs per standard, so
+      // solve each standard as its own batch on the same stream.
+      std::vector<DevField> Fp, Fn;
+      for (int i = 0; i < n; ++i) (uses_vswitch(geom[i]) ? Fp : Fn).push_back(F[i]);
+      Carve c1 = cv;
+      pp.is_pal = 1;
+      refine_planes(ctx, Fp, pp, st, c1, RefineOut{});
+      Carve c2 = cv;
+      pp.is_pal = 0;
+      refine_planes(ctx, Fn, pp, st, c2, RefineOut{});
+    } else {
+      pp.is_pal = any_pal ? 1 : 0;
+      refine_planes(ctx, F, pp, st, cv, RefineOut{});
+    }
+  });
+}
+
+}  // namespace
+
+int hvd_gpu_decode_fields_u16(hvd_gpu_ctx* ctx, const uint16_t* const* raw, int n, const hvd_field_geometry* geom,
+                              float blanking, float white, const hvd_solver_params* p, float chroma_phase_deg,
+                              float* const* y_out, float* const* chi_out, float* const* theta_out, void* stream) {
+  if (!raw && n > 0) return HVD_GPU_ERR_INVALID;
+  return decode_fields(ctx, raw, nullptr, n, geom, blanking, white, p, chroma_phase_deg, y_out, chi_out, theta_out,
+                       stream);
+}
+
+int hvd_gpu_decode_fields_ire(hvd_gpu_ctx* ctx, const float* const* field_ire, int n, const hvd_field_geometry* geom,
+                              const hvd_solver_params* p, float chroma_phase_deg, float* const* y_out,
+                              float* const* chi_out, float* const* theta_out, void* stream) {
+  if (!field_ire && n > 0) return HVD_GPU_ERR_INVALID;
+  return decode_fields(ctx, nullptr, field_ire, n, geom, 0.0f, 100.0f, p, chroma_phase_deg, y_out, chi_out,
+                       theta_out, stream);
+}
+
+int hvd_gpu_block_match_sad_u16(hvd_gpu_ctx* ctx, const uint16_t* const* ref, const uint16_t* const* cur, int n,
+                                int h, int w, int bh, int bw, int sy, int sx, int16_t* const* vy, int16_t* const* vx,
+                                int32_t* const* sad, void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, n >= 0 && (n == 0 || (ref && cur && vy && vx && sad)), "null argument");
+    REQUIRE(ctx, h >= 1 && w >= 1, "empty plane");
+    REQUIRE(ctx, bw >= 1 && bw <= 32 && bh >= 1 && bh <= 16, "block size %d x %d outside 1..32 x 1..16", bw, bh);
+    REQUIRE(ctx, sx >= 0 && sx <= 32 && sy >= 0 && sy <= 16, "search range %d x %d outside 0..32 x 0..16", sx, sy);
+    if (n == 0) return;
+    std::vector<SadDesc> D(n);
+    std::vector<int> pre(n + 1, 0);
+    const int nby = (h + bh - 1) / bh, nbx = (w + bw - 1) / bw;
+    for (int i = 0; i < n; ++i) {
+      REQUIRE(ctx, ref[i] && cur[i] && vy[i] && vx[i] && sad[i], "pair %d: null pointer", i);
+      SadDesc d{};
+      d.ref = ref[i];
+      d.cur = cur[i];
+      d.vy = vy[i];
+      d.vx = vx[i];
+      d.sad = sad[i];
+      d.h = h;
+      d.w = w;
+      d.bh = bh;
+      d.bw = bw;
+      d.sy = sy;
+      d.sx = sx;
+      d.nby = nby;
+      d.nbx = nbx;
+      D[i] = d;
+      pre[i + 1] = pre[i] + nby * ((nbx + 3) / 4);
+    }
+    Stage stg;
+    const size_t oD = stg.put(D), oP = stg.put(pre);
+    Slot* sl;
+    char* base = upload(ctx, stg, S(stream), &sl);
+    hipLaunchKernelGGL(k_sad_bm, dim3(pre[n]), dim3(256), 0, S(stream), (const SadDesc*)(base + oD),
+                       (const int*)(base + oP), n);
+    LAUNCH_CHECK(ctx);
+    arm(ctx, sl, S(stream));
+  });
+}
+
+}  // extern "C"

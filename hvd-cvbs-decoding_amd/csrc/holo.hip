@@ -1,0 +1,292 @@
+// holo.hip -- holographic chroma init (engine/holographic_init.cpp:200-317)
+// with project-written exact-size FFTs.
+//
+// The 2-D transforms keep hvd-core's sizes exactly (numpy semantics of
+// engine/fft2d.h:30-33, no zero padding to friendly sizes -- that would
+// change the circular convolution): NTSC 274 x 760, PAL 320 x 922.  Each 1-D
+// transform runs in LDS as a mixed-radix Stockham autosort pass sequence in
+// which every output of a radix-R stage is a direct R-term sum
+//     y[o] = sum_r x[ib + r M] W_n^{r E(o)},
+// so awkward primes (19, 137, 461) are just long stages; every twiddle is a
+// lookup into one table of n-th roots computed in FP64 on the host.
+//
+// Pipeline per field (5 launches for the whole batch):
+//   k_fft_rows_fwd  demod S conj(c) + reflect101 row padding + row FFT
+//   k_fft_cols_fwd  column FFT in place (8 columns per block)
+//   k_fft_cols_inv  per variant: spectrum x Gaussian crop, inverse column
+//                   FFT, only the rows that survive the crop are written
+//   k_fft_rows_inv  inverse row FFT, x 2 / (hp w)
+//   k_holo_blend    residual luma, edge energy (7x7 box in LDS), weighted
+//                   blend, Y0 = S - Re[chi0 c]
+#include "hvd_device.h"
+#include "hvd_kernels.h"
+
+namespace hvdk {
+
+namespace {
+
+__device__ inline float2 carrier_of(const InitField& f, int y, int x) {
+  if (f.carrier) return f.carrier[(size_t)y * f.w + x];
+  float2 v = cmul(f.rowph[y], f.colph[x]);
+  if (f.vsw && f.vsw[y] < 0) v.x = -v.x;
+  return v;
+}
+
+// reflect101 (holographic_init.cpp:126-138).
+__device__ inline int mirror_row(int y, int h) {
+  if (h <= 1) return 0;
+  const int period = 2 * h - 2;
+  y = ((y % period) + period) % period;
+  return y < h ? y : period - y;
+}
+
+// In-LDS transform of `ncols` interleaved sequences of length n:
+// element (j, c) lives at buf[j * ncols + c].  Returns the buffer holding the
+// result (a or b).  sign < 0 forward (e^{-2 pi i/n}), > 0 inverse, unscaled.
+__device__ float2* fft_lds(float2* a, float2* b, const float2* root, const FftPlan& P, int sign, int ncols) {
+  const int n = P.n;
+  float2* src = a;
+  float2* dst = b;
+  int p = 1;
+  for (int s = 0; s < P.nst; ++s) {
+    const int R = P.radix[s];
+    const int M = n / R;
+    const int step1 = n / (p * R);
+    for (int e = threadIdx.x; e < n * ncols; e += blockDim.x) {
+      const int c = e % ncols, o = e / ncols;
+      const int k = o % p, t = o / p;
+      const int q = t % R, ib = (t / R) * p + k;
+      int E = k * step1 + q * M;
+      if (E >= n) E -= n;
+      float ar = 0.f, ai = 0.f;
+      int ex = 0;
+      for (int r = 0; r < R; ++r) {
+        const float2 x = src[(ib + r * M) * ncols + c];
+        const float2 w = root[ex];
+        const float wi = sign < 0 ? w.y : -w.y;
+        ar = __fmaf_rn(x.x, w.x, __fmaf_rn(-x.y, wi, ar));
+        ai = __fmaf_rn(x.x, wi, __fmaf_rn(x.y, w.x, ai));
+        ex += E;
+        if (ex >= n) ex -= n;
+      }
+      dst[o * ncols + c] = make_float2(ar, ai);
+    }
+    __syncthreads();
+    float2* t2 = src;
+    src = dst;
+    dst = t2;
+    p *= R;
+  }
+  return src;
+}
+
+__device__ inline void load_root(float2* lds, const float2* root, int n) {
+  for (int j = threadIdx.x; j < n; j += blockDim.x) lds[j] = root[j];
+}
+
+// numpy fftfreq(n, d) bin i (holographic_init.cpp:16-25), cycles per unit.
+__device__ inline float fftfreq(int i, int n, float inv_nd) {
+  const int half = (n % 2 == 0) ? n / 2 : (n + 1) / 2;
+  const int k = i < half ? i : i - n;
+  return (float)k * inv_nd;
+}
+
+}  // namespace
+
+// (1) demod + reflect101 padding + forward row FFT.  One block per padded row.
+__global__ void __launch_bounds__(256) k_fft_rows_fwd(const InitField* F, const FftPlan* plans, const float2* roots,
+                                                      const int* row_prefix, int nf) {
+  extern __shared__ float2 lds[];
+  const int fi = field_of_tile(row_prefix, nf, blockIdx.x);
+  const InitField& f = F[fi];
+  const int yp = blockIdx.x - row_prefix[fi];
+  const FftPlan& P = plans[f.row_plan];
+  float2* a = lds;
+  float2* b = lds + P.n;
+  float2* rt = lds + 2 * P.n;
+  load_root(rt, roots + P.root_off, P.n);
+  const int sy = mirror_row(yp - f.pad, f.h);
+  for (int x = threadIdx.x; x < f.w; x += blockDim.x) {
+    const float sv = f.s[(size_t)sy * f.w + x];
+    const float2 c = carrier_of(f, sy, x);
+    a[x] = make_float2(sv * c.x, -sv * c.y);  // S * conj(c)
+  }
+  __syncthreads();
+  const float2* r = fft_lds(a, b, rt, P, -1, 1);
+  for (int x = threadIdx.x; x < f.w; x += blockDim.x) f.spec[(size_t)yp * f.w + x] = r[x];
+}
+
+// (2) forward column FFT in place, kColGroup columns per block.
+__global__ void __launch_bounds__(256) k_fft_cols_fwd(const InitField* F, const FftPlan* plans, const float2* roots,
+                                                      const int* grp_prefix, int nf) {
+  extern __shared__ float2 lds[];
+  const int fi = field_of_tile(grp_prefix, nf, blockIdx.x);
+  const InitField& f = F[fi];
+  const int x0 = (blockIdx.x - grp_prefix[fi]) * kColGroup;
+  const int nc = min(kColGroup, f.w - x0);
+  const FftPlan& P = plans[f.col_plan];
+  float2* a = lds;
+  float2* b = lds + P.n * kColGroup;
+  float2* rt = lds + 2 * P.n * kColGroup;
+  load_root(rt, roots + P.root_off, P.n);
+  for (int e = threadIdx.x; e < P.n * kColGroup; e += blockDim.x) {
+    const int c = e % kColGroup, y = e / kColGroup;
+    a[e] = c < nc ? f.spec[(size_t)y * f.w + x0 + c] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  const float2* r = fft_lds(a, b, rt, P, -1, kColGroup);
+  for (int e = threadIdx.x; e < P.n * kColGroup; e += blockDim.x) {
+    const int c = e % kColGroup, y = e / kColGroup;
+    if (c < nc) f.spec[(size_t)y * f.w + x0 + c] = r[e];
+  }
+}
+
+// (3) per variant: spectrum x separable Gaussian (GaussianLpf :33-56, the
+// vertical cutoff defined against the unpadded height), optional spectral
+// symmetry certificate (:378-397), inverse column FFT; rows [pad, pad + h)
+// are stored.
+__global__ void __launch_bounds__(256) k_fft_cols_inv(const InitField* F, const FftPlan* plans, const float2* roots,
+                                                      const int* grp_prefix, int nf) {
+  extern __shared__ float2 lds[];
+  const int fi = field_of_tile(grp_prefix, nf, blockIdx.x);
+  const InitField& f = F[fi];
+  const int x0 = (blockIdx.x - grp_prefix[fi]) * kColGroup;
+  const int nc = min(kColGroup, f.w - x0);
+  const FftPlan& P = plans[f.col_plan];
+  const int hp = P.n;
+  float2* a = lds;
+  float2* b = lds + hp * kColGroup;
+  float2* rt = lds + 2 * hp * kColGroup;
+  load_root(rt, roots + P.root_off, hp);
+  const float vh[3] = {0.8e6f, 1.8e6f, 1.3e6f};
+  const float vv[3] = {120.f, 30.f, 60.f};
+  const float inv_w = f.sample_rate / (float)f.w;  // fftfreq(w, 1/fs): k / (w / fs)
+  const float inv_hp = 1.f / (float)hp;
+  for (int v = 0; v < f.nvar; ++v) {
+    const float cx = vh[v];
+    const float cy = vv[v] / (2.f * (float)f.h);
+    for (int e = threadIdx.x; e < hp * kColGroup; e += blockDim.x) {
+      const int c = e % kColGroup, y = e / kColGroup;
+      float2 val = make_float2(0.f, 0.f);
+      if (c < nc) {
+        const int x = x0 + c;
+        const float rx = fftfreq(x, f.w, inv_w) / cx;
+        const float ry = fftfreq(y, hp, inv_hp) / cy;
+        float k = expf(-0.5f * ry * ry) * expf(-0.5f * rx * rx);
+        const float2 sp = f.spec[(size_t)y * f.w + x];
+        if (v == 2) {
+          const int ry_ = (hp - y) % hp, rx_ = (f.w - x) % f.w;
+          const float2 sr = f.spec[(size_t)ry_ * f.w + rx_];
+          const float m = sqrtf(sp.x * sp.x + sp.y * sp.y), mr = sqrtf(sr.x * sr.x + sr.y * sr.y);
+          k *= fminf(m, mr) / (m + 1e-6f);
+        }
+        val = make_float2(sp.x * k, sp.y * k);
+      }
+      a[e] = val;
+    }
+    __syncthreads();
+    const float2* r = fft_lds(a, b, rt, P, +1, kColGroup);
+    for (int e = threadIdx.x; e < hp * kColGroup; e += blockDim.x) {
+      const int c = e % kColGroup, y = e / kColGroup - f.pad;
+      if (c < nc && y >= 0 && y < f.h) f.var[v][(size_t)y * f.w + x0 + c] = r[e];
+    }
+    __syncthreads();
+  }
+}
+
+// (4) inverse row FFT of every variant, scaled by 2 / (hp w) (the numpy
+// inverse normalisation of fft2d.cpp:112-118 and the x2 of :358).
+__global__ void __launch_bounds__(256) k_fft_rows_inv(const InitField* F, const FftPlan* plans, const float2* roots,
+                                                      const int* row_prefix, int nf) {
+  extern __shared__ float2 lds[];
+  const int fi = field_of_tile(row_prefix, nf, blockIdx.x);
+  const InitField& f = F[fi];
+  const int y = blockIdx.x - row_prefix[fi];
+  const FftPlan& P = plans[f.row_plan];
+  float2* a = lds;
+  float2* b = lds + P.n;
+  float2* rt = lds + 2 * P.n;
+  load_root(rt, roots + P.root_off, P.n);
+  const float scale = 2.f / ((float)f.hp * (float)f.w);
+  for (int v = 0; v < f.nvar; ++v) {
+    float2* row = f.var[v] + (size_t)y * f.w;
+    for (int x = threadIdx.x; x < f.w; x += blockDim.x) a[x] = row[x];
+    __syncthreads();
+    const float2* r = fft_lds(a, b, rt, P, +1, 1);
+    for (int x = threadIdx.x; x < f.w; x += blockDim.x) row[x] = make_float2(r[x].x * scale, r[x].y * scale);
+    __syncthreads();
+  }
+}
+
+// (5) blend (holographic_init.cpp:171-196, :360-369, :415-422).  Tiles of
+// kBlendW x kBlendH pixels; each variant's residual luma is staged with the
+// 3 + 1 halo the edge-energy box blur needs.
+__global__ void __launch_bounds__(256) k_holo_blend(const InitField* F, const int* tp, int nf) {
+  constexpr int RW = kBlendW + 7, RH = kBlendH + 7;  // Y_v region (3 left/up, 4 right/down)
+  constexpr int EW = kBlendW + 6, EH = kBlendH + 6;  // edge-energy region
+  __shared__ float ys[RH][RW];
+  __shared__ float es[EH][EW];
+  __shared__ float hs[EH][kBlendW];
+  const int fi = field_of_tile(tp, nf, blockIdx.x);
+  const InitField& f = F[fi];
+  const int lt = blockIdx.x - tp[fi];
+  const int tiles_x = (f.w + kBlendW - 1) / kBlendW;
+  const int bx0 = (lt % tiles_x) * kBlendW, by0 = (lt / tiles_x) * kBlendH;
+  const int tx = threadIdx.x % kBlendW, ty = threadIdx.x / kBlendW;
+  const int px = bx0 + tx, py = by0 + ty;
+  const bool in = px < f.w && py < f.h;
+  float2 num = make_float2(0.f, 0.f);
+  float den = 0.f;
+  for (int v = 0; v < f.nvar; ++v) {
+    const float2* chv = f.var[v];
+    for (int e = threadIdx.x; e < RH * RW; e += blockDim.x) {
+      const int ry = e / RW, rx = e % RW;
+      const int y = by0 - 3 + ry, x = bx0 - 3 + rx;
+      float val = 0.f;
+      if (y >= 0 && y < f.h && x >= 0 && x < f.w) {
+        const size_t i = (size_t)y * f.w + x;
+        val = f.s[i] - re_mul(chv[i], carrier_of(f, y, x));
+      }
+      ys[ry][rx] = val;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < EH * EW; e += blockDim.x) {
+      const int ry = e / EW, rx = e % EW;
+      const int y = by0 - 3 + ry, x = bx0 - 3 + rx;
+      float val = 0.f;
+      if (y >= 0 && y < f.h && x >= 0 && x < f.w) {
+        const float gx = x + 1 < f.w ? ys[ry][rx + 1] - ys[ry][rx] : 0.f;
+        const float gy = y + 1 < f.h ? ys[ry + 1][rx] - ys[ry][rx] : 0.f;
+        val = fabsf(gx) + fabsf(gy);
+      }
+      es[ry][rx] = val;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < EH * kBlendW; e += blockDim.x) {
+      const int ry = e / kBlendW, cx = e % kBlendW;
+      float acc = 0.f;
+      for (int k = 0; k < 7; ++k) acc += es[ry][cx + k];
+      hs[ry][cx] = acc;
+    }
+    __syncthreads();
+    if (in) {
+      float acc = 0.f;
+      for (int k = 0; k < 7; ++k) acc += hs[ty + k][tx];
+      const float energy = acc * (1.f / 49.f);
+      const float wgt = 1.f / (energy + 0.5f);
+      const float2 c = chv[(size_t)py * f.w + px];
+      num.x += c.x * wgt;
+      num.y += c.y * wgt;
+      den += wgt;
+    }
+    __syncthreads();
+  }
+  if (in) {
+    const size_t i = (size_t)py * f.w + px;
+    const float2 chi = make_float2(num.x / den, num.y / den);
+    f.chi0[i] = chi;
+    if (f.y0) f.y0[i] = f.s[i] - re_mul(chi, carrier_of(f, py, px));
+  }
+}
+
+}  // namespace hvdk

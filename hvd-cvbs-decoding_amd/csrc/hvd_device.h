@@ -1,0 +1,130 @@
+// hvd_device.h -- device-side types and helpers shared by the HIP kernels.
+//
+// Data layout in HBM (one active plane per field, row-major h x w):
+//   s        float   composite (IRE)
+//   chi      float2  chroma phasor chi = V - iU, interleaved (re, im)
+//   g, d     float2  CG gradient / search direction
+//   wts      float4  IRLS diffusivities (wx, wy, wcx, wcy) per pixel
+//   carrier  never stored: c(y, x) = rowph[y] * colph[x], with the PAL
+//            V-switch folded in as c -> -conj(c) on switched lines
+//            (ntsc_geometry.cpp:75-93).  An explicit plane is accepted too.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hvdk {
+
+constexpr int kTileW = 64;  // one wavefront per tile row
+constexpr int kTileH = 4;
+constexpr int kBlock = kTileW * kTileH;  // 256 threads
+
+// One field of a batch, as every per-pixel kernel sees it.
+struct DevField {
+  int h, w;            // active lines, active width
+  int tiles_x;         // ceil(w / kTileW)
+  int tile0;           // first tile index of this field in the batch
+  const float* s;      // h*w
+  const float2* rowph; // h: e^{i theta_y}
+  const int8_t* vsw;   // h or null (NTSC)
+  const float2* colph; // w: e^{i w0 (a0 + x)}
+  const float2* carrier;  // explicit h*w carrier, or null (factored form)
+  float2* chi[2];      // ping-pong chroma (chi[0] is the caller's buffer)
+  float2* g;           // gradient
+  float2* d[2];        // ping-pong search direction
+  float4* wts;         // IRLS weights
+  float* y;            // output luma
+  // holographic-init scratch
+  float2* spec;        // (h + 2 pad) x w spectrum
+  float2* var[3];      // per-variant chroma estimates (h x w)
+  int pad, hp;         // mirror padding and padded height
+  float sample_rate;
+  int n_variants;      // 2, or 3 with the symmetry variant
+};
+
+// Per-field CG state (variational.cpp:519-549), updated by the last block of
+// each reduction kernel.
+struct FieldCG {
+  double gg, gg0, hh, gd, npix;
+  double tol2;
+  float alpha, beta;
+  float mu_h, mu_v, lam_y, eps, eps_c, k;
+  float aniso_forced;   // > 0 forces chroma_aniso
+  int is_pal;
+  int n_inner;
+  int done, it;
+  int chi_cur, d_cur;
+  unsigned arrive;
+  int iters[16];        // CG iterations per IRLS stage (diagnostics)
+  int stage;
+  int pad_;
+};
+
+__device__ inline float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline float re_mul(float2 a, float2 b) { return a.x * b.x - a.y * b.y; }
+
+__device__ inline float2 carrier_at(const DevField& f, int y, int x) {
+  if (f.carrier) return f.carrier[(size_t)y * f.w + x];
+  float2 v = cmul(f.rowph[y], f.colph[x]);
+  if (f.vsw && f.vsw[y] < 0) v.x = -v.x;
+  return v;
+}
+
+// Block index -> logical tile, grouping consecutive logical tiles on one XCD
+// (blocks are dispatched round-robin over the 8 XCDs, each with its own L2),
+// so a tile and its vertical neighbours tend to share an L2.
+__device__ inline int xcd_remap(int b, int nb) {
+  const int per = nb / 8, rem = nb % 8;
+  const int xcd = b % 8, idx = b / 8;
+  return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+}
+
+// Field owning logical tile t: tile_prefix has n+1 entries.
+__device__ inline int field_of_tile(const int* tile_prefix, int n, int t) {
+  int lo = 0, hi = n;  // invariant: tile_prefix[lo] <= t < tile_prefix[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (tile_prefix[mid] <= t) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Deterministic block-wide FP64 sum; result valid in thread 0.
+template <int NT>
+__device__ inline double block_sum(double v, double* lds) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) lds[wv] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < NT / 64; ++i) r += lds[i];
+  __syncthreads();
+  return r;
+}
+
+__device__ inline void store_partial(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline double load_partial(const double* p) {
+  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Last-block election: thread 0 of every block calls this after publishing its
+// partials; returns true (in thread 0) in exactly one block, the last to
+// arrive, which may then read every block's partials.  The counter is reset
+// by the elected block.
+__device__ inline bool arrive_last(unsigned* counter, unsigned total) {
+  __threadfence();
+  const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+  const bool last = prev + 1 == total;
+  if (last) {
+    __threadfence();
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return last;
+}
+
+}  // namespace hvdk

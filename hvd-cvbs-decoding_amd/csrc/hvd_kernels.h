@@ -1,0 +1,108 @@
+// hvd_kernels.h -- kernel declarations and the descriptor structs the host
+// side fills for them.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hvd_device.h"
+
+namespace hvdk {
+
+constexpr int kMaxLines = 640;   // lines per field the lock-in handles in LDS
+constexpr int kColGroup = 8;     // columns per block in the column FFTs
+constexpr int kBlendW = 32;      // holographic blend tile
+constexpr int kBlendH = 8;
+constexpr int kMaxRadix = 16;    // stages per 1-D FFT plan
+
+struct LockinGeom {
+  int fw, fh, x0, x1;
+  int std_phase;       // phase per sample exactly pi/2 (no custom subcarrier)
+  int vswitch;         // PAL family: joint phase + V-switch parity
+  double w0;           // phase per sample (rad)
+  double line_advance; // per-line model advance (rad)
+  float nominal_burst; // 20 NTSC / PAL-M, 21.43 PAL
+  int pad_;
+};
+
+struct PrepDesc {
+  const float* field;        // fh x fw IRE
+  const float* theta_field;  // fh
+  const int8_t* vswitch_field;  // fh or null
+  float* s;                  // lines x width
+  float2* rowph;             // lines
+  int8_t* vsw_active;        // lines or null
+  float2* colph;             // width
+  int fw, fal, a0, lines, width;
+  int std_phase, signed_by_parity;
+  float phase_offset;        // -chroma_phase_deg * pi / 180
+  double w0;
+};
+
+struct FftPlan {
+  int n, nst;
+  int radix[kMaxRadix];
+  int root_off;  // offset of this size's n roots in the shared root buffer
+  int pad_;
+};
+
+struct InitField {
+  int h, w, hp, pad;
+  const float* s;
+  const float2* rowph;
+  const int8_t* vsw;
+  const float2* colph;
+  const float2* carrier;
+  float2* spec;     // hp x w
+  float2* var[3];   // h x w each
+  float2* chi0;     // out
+  float* y0;        // out (may be null)
+  int row_plan, col_plan;
+  int nvar;
+  float sample_rate;
+};
+
+struct SadDesc {
+  const uint16_t* ref;
+  const uint16_t* cur;
+  int16_t* vy;
+  int16_t* vx;
+  int32_t* sad;
+  int h, w, bh, bw, sy, sx, nby, nbx;
+};
+
+// lockin.hip
+__global__ void k_lockin(const LockinGeom* G, const float* const* fields, float* const* theta, int8_t* const* vswitch);
+__global__ void k_burst_amp(const LockinGeom* G, const float* const* fields, float* out);
+__global__ void k_u16_to_ire(const uint16_t* raw, size_t n, float blanking, float cpi, float* out);
+__global__ void k_prepare(const PrepDesc* P);
+__global__ void k_crop(const PrepDesc* P);
+
+// holo.hip
+__global__ void k_fft_rows_fwd(const InitField* F, const FftPlan* plans, const float2* roots, const int* row_prefix,
+                               int nf);
+__global__ void k_fft_cols_fwd(const InitField* F, const FftPlan* plans, const float2* roots, const int* grp_prefix,
+                               int nf);
+__global__ void k_fft_cols_inv(const InitField* F, const FftPlan* plans, const float2* roots, const int* grp_prefix,
+                               int nf);
+__global__ void k_fft_rows_inv(const InitField* F, const FftPlan* plans, const float2* roots, const int* row_prefix,
+                               int nf);
+__global__ void k_holo_blend(const InitField* F, const int* tp, int nf);
+
+// solve.hip
+__global__ void k_refine_begin(FieldCG* st, const FieldCG* init, int nf);
+__global__ void k_aniso_samples(const DevField* F, const FieldCG* st, const int* sample_prefix, int nf, float* gx,
+                                float* gy);
+__global__ void k_aniso_select(FieldCG* st, const int* sample_prefix, const float* gx, const float* gy,
+                               float lambda_c);
+__global__ void k_irls_weights(const DevField* F, FieldCG* st, const int* tp, int nf, int nb);
+__global__ void k_grad_init(const DevField* F, FieldCG* st, const int* tp, int nf, int nb, double* partials);
+__global__ void k_cg_curv(const DevField* F, FieldCG* st, const int* tp, int nf, int nb, double* partials, int it);
+__global__ void k_cg_step(const DevField* F, FieldCG* st, const int* tp, int nf, int nb, double* partials);
+__global__ void k_stage_end(FieldCG* st, int nf, int stage);
+__global__ void k_refine_finalize(const DevField* F, const FieldCG* st, const int* tp, int nf, int nb);
+
+// motion.hip
+__global__ void k_sad_bm(const SadDesc* D, const int* blk_prefix, int nf);
+
+}  // namespace hvdk

@@ -1,0 +1,281 @@
+"""ctypes binding of the C ABI in include/hvd_gpu.h (libhvd_gpu.so).
+
+Device buffers are torch tensors on the HIP device; torch is only the
+allocator and the stream here.  Every call goes through the C ABI -- there is
+no CPU fallback: a missing or unloadable library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import pathlib
+import subprocess
+
+import numpy as np
+
+PKG = pathlib.Path(__file__).resolve().parent.parent  # hvd-cvbs-decoding_amd/
+LIB_PATH = PKG / "lib" / "libhvd_gpu.so"
+
+ABI_SYMBOLS = (
+    "hvd_gpu_create", "hvd_gpu_destroy", "hvd_gpu_last_error", "hvd_gpu_synchronize",
+    "hvd_gpu_abi_version", "hvd_gpu_sample_to_ire_u16", "hvd_gpu_burst_lockin_phase",
+    "hvd_gpu_burst_amplitude_ire", "hvd_gpu_holographic_init", "hvd_gpu_variational_refine",
+    "hvd_gpu_decode_fields_u16", "hvd_gpu_decode_fields_ire", "hvd_gpu_block_match_sad_u16",
+    "hvd_gpu_set_profiling", "hvd_gpu_cg_timing",
+)
+
+
+def build(arch: str = "gfx950") -> pathlib.Path:
+    """Compile libhvd_gpu.so in-tree with hipcc."""
+    subprocess.run(["make", "-s", "-C", str(PKG), f"ARCH={arch}"], check=True)
+    return LIB_PATH
+
+
+class Geometry(C.Structure):
+    _fields_ = [(n, C.c_int) for n in (
+        "field_width", "field_height", "active_video_start", "active_video_end",
+        "colour_burst_start", "colour_burst_end", "first_active_field_line",
+        "last_active_field_line")] + [("sample_rate", C.c_double), ("subcarrier_hz", C.c_double),
+                                      ("standard", C.c_int)]
+
+    @classmethod
+    def of(cls, g) -> "Geometry":
+        return cls(g.field_width, g.field_height, g.active_video_start, g.active_video_end,
+                   g.colour_burst_start, g.colour_burst_end, g.first_active_field_line,
+                   g.last_active_field_line, g.sample_rate, g.subcarrier_hz, g.standard)
+
+
+class SolverParams(C.Structure):
+    _fields_ = [("lambda_c", C.c_double), ("chroma_aniso", C.c_double),
+                ("charbonnier_eps", C.c_double), ("chroma_eps", C.c_double),
+                ("structure_coupling", C.c_double), ("diag_prior", C.c_double),
+                ("cg_iterations", C.c_int), ("irls_outer", C.c_int), ("cg_tol", C.c_double),
+                ("fast", C.c_int), ("is_pal", C.c_int), ("lambda_y", C.c_double)]
+
+    @classmethod
+    def make(cls, **kw) -> "SolverParams":
+        """hvd::HvdConfig defaults (engine/hvd_config.h:26-102), overridable."""
+        d = dict(lambda_c=1.0, chroma_aniso=0.0, charbonnier_eps=0.5, chroma_eps=1.0,
+                 structure_coupling=0.25, diag_prior=0.0, cg_iterations=2, irls_outer=4,
+                 cg_tol=0.0, fast=1, is_pal=0, lambda_y=1.0)
+        d.update(kw)
+        return cls(**d)
+
+
+class Plane(C.Structure):
+    _fields_ = [("lines", C.c_int), ("width", C.c_int), ("s", C.c_void_p), ("theta", C.c_void_p),
+                ("vswitch", C.c_void_p), ("carrier", C.c_void_p), ("active_video_start", C.c_int),
+                ("phase_per_sample", C.c_double), ("sample_rate", C.c_double), ("chi", C.c_void_p),
+                ("y", C.c_void_p)]
+
+
+class HvdGpuError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise HvdGpuError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        lib = C.CDLL(str(LIB_PATH))
+        vp, i, pp = C.c_void_p, C.c_int, C.POINTER(C.c_void_p)
+        lib.hvd_gpu_create.argtypes = [i, C.POINTER(vp)]
+        lib.hvd_gpu_destroy.argtypes = [vp]
+        lib.hvd_gpu_destroy.restype = None
+        lib.hvd_gpu_last_error.argtypes = [vp]
+        lib.hvd_gpu_last_error.restype = C.c_char_p
+        lib.hvd_gpu_synchronize.argtypes = [vp, vp]
+        lib.hvd_gpu_sample_to_ire_u16.argtypes = [vp, vp, C.c_size_t, C.c_float, C.c_float, vp, vp]
+        lib.hvd_gpu_burst_lockin_phase.argtypes = [vp, pp, i, C.POINTER(Geometry), pp, pp, vp]
+        lib.hvd_gpu_burst_amplitude_ire.argtypes = [vp, pp, i, C.POINTER(Geometry), vp, vp]
+        lib.hvd_gpu_holographic_init.argtypes = [vp, C.POINTER(Plane), i, i, vp]
+        lib.hvd_gpu_variational_refine.argtypes = [vp, C.POINTER(Plane), i, C.POINTER(SolverParams),
+                                                   C.POINTER(C.c_int), C.POINTER(C.c_float), vp]
+        lib.hvd_gpu_decode_fields_u16.argtypes = [vp, pp, i, C.POINTER(Geometry), C.c_float, C.c_float,
+                                                  C.POINTER(SolverParams), C.c_float, pp, pp, pp, vp]
+        lib.hvd_gpu_decode_fields_ire.argtypes = [vp, pp, i, C.POINTER(Geometry), C.POINTER(SolverParams),
+                                                  C.c_float, pp, pp, pp, vp]
+        lib.hvd_gpu_block_match_sad_u16.argtypes = [vp, pp, pp, i, i, i, i, i, i, i, pp, pp, pp, vp]
+        lib.hvd_gpu_set_profiling.argtypes = [vp, i]
+        lib.hvd_gpu_cg_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
+                                          C.POINTER(C.c_double)]
+        _lib = lib
+    return _lib
+
+
+def _ptrs(tensors) -> C.Array:
+    arr = (C.c_void_p * max(len(tensors), 1))()
+    for k, t in enumerate(tensors):
+        arr[k] = None if t is None else t.data_ptr()
+    return arr
+
+
+def _stream():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Engine:
+    """One GPU context (the analogue of an hvd::HvdEngine: one per thread)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        self.ctx = C.c_void_p()
+        rc = self.lib.hvd_gpu_create(device, C.byref(self.ctx))
+        if rc != 0:
+            raise HvdGpuError(f"hvd_gpu_create({device}) failed with status {rc}")
+        self.device = device
+
+    def close(self):
+        if self.ctx:
+            self.lib.hvd_gpu_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            msg = self.lib.hvd_gpu_last_error(self.ctx).decode()
+            raise HvdGpuError(f"{what}: status {rc}: {msg}")
+
+    def synchronize(self):
+        self._check(self.lib.hvd_gpu_synchronize(self.ctx, _stream()), "synchronize")
+
+    def set_profiling(self, on: bool):
+        self._check(self.lib.hvd_gpu_set_profiling(self.ctx, int(on)), "set_profiling")
+
+    def cg_timing(self):
+        """(CG-loop ms, CG launches, pixel-iterations) since profiling was enabled."""
+        ms, nl, pxit = C.c_double(), C.c_longlong(), C.c_double()
+        self._check(self.lib.hvd_gpu_cg_timing(self.ctx, C.byref(ms), C.byref(nl), C.byref(pxit)), "cg_timing")
+        return ms.value, nl.value, pxit.value
+
+    # ---- level conversion ----------------------------------------------
+    def sample_to_ire_u16(self, raw, blanking: float, white: float):
+        import torch
+        out = torch.empty(raw.shape, dtype=torch.float32, device=raw.device)
+        self._check(self.lib.hvd_gpu_sample_to_ire_u16(self.ctx, C.c_void_p(raw.data_ptr()), raw.numel(),
+                                                       blanking, white, C.c_void_p(out.data_ptr()), _stream()),
+                    "sample_to_ire_u16")
+        return out
+
+    # ---- lock-in -----------------------------------------------------------
+    def burst_lockin_phase(self, fields, geoms):
+        """fields: list of (fh, fw) float32 IRE tensors.  Returns (theta list,
+        vswitch list (None for NTSC))."""
+        import torch
+        n = len(fields)
+        G = (Geometry * n)(*[Geometry.of(g) for g in geoms])
+        th = [torch.empty(g.field_height, dtype=torch.float32, device=f.device) for f, g in zip(fields, geoms)]
+        vs = [torch.empty(g.field_height, dtype=torch.int8, device=f.device) if g.standard in (1, 2) else None
+              for f, g in zip(fields, geoms)]
+        self._check(self.lib.hvd_gpu_burst_lockin_phase(self.ctx, _ptrs(fields), n, G, _ptrs(th), _ptrs(vs),
+                                                        _stream()), "burst_lockin_phase")
+        return th, vs
+
+    def burst_amplitude_ire(self, fields, geoms):
+        import torch
+        n = len(fields)
+        G = (Geometry * n)(*[Geometry.of(g) for g in geoms])
+        out = torch.empty(n, dtype=torch.float32, device=fields[0].device)
+        self._check(self.lib.hvd_gpu_burst_amplitude_ire(self.ctx, _ptrs(fields), n, G, C.c_void_p(out.data_ptr()),
+                                                         _stream()), "burst_amplitude_ire")
+        return out
+
+    # ---- init / solve --------------------------------------------------------
+    @staticmethod
+    def plane(s, chi, y=None, theta=None, vswitch=None, carrier=None, a0=0, w0=0.5 * math.pi,
+              sample_rate=4.0 * 315.0e6 / 88.0) -> Plane:
+        """chi/carrier: (lines, width, 2) float32 (interleaved complex)."""
+        lines, width = s.shape
+        p = Plane(lines, width, s.data_ptr(), None if theta is None else theta.data_ptr(),
+                  None if vswitch is None else vswitch.data_ptr(),
+                  None if carrier is None else carrier.data_ptr(), a0, w0, sample_rate,
+                  chi.data_ptr(), None if y is None else y.data_ptr())
+        # the struct holds raw device pointers: keep the tensors alive with it
+        p._keep = (s, chi, y, theta, vswitch, carrier)
+        return p
+
+    def holographic_init(self, planes, symmetry_variant=False):
+        n = len(planes)
+        P = (Plane * n)(*planes)
+        self._check(self.lib.hvd_gpu_holographic_init(self.ctx, P, n, int(symmetry_variant), _stream()),
+                    "holographic_init")
+
+    def variational_refine(self, planes, params: SolverParams, want_stats=False):
+        n = len(planes)
+        P = (Plane * n)(*planes)
+        outer = max(params.irls_outer, 1)
+        iters = (C.c_int * (n * outer))() if want_stats else None
+        aniso = (C.c_float * n)() if want_stats else None
+        self._check(self.lib.hvd_gpu_variational_refine(self.ctx, P, n, C.byref(params), iters, aniso, _stream()),
+                    "variational_refine")
+        if want_stats:
+            return np.array(iters[:], dtype=np.int32).reshape(n, outer), np.array(aniso[:], dtype=np.float32)
+        return None
+
+    # ---- field driver -------------------------------------------------------
+    def decode_fields_u16(self, raws, geoms, params: SolverParams, blanking=20480.0, white=61440.0,
+                          chroma_phase_deg=0.0, want_theta=False, outputs=None):
+        """raws: list of (fh, fw) uint16 tensors (torch.uint16 or int16 views).
+        Returns (ys, chis, thetas)."""
+        import torch
+        n = len(raws)
+        G = (Geometry * n)(*[Geometry.of(g) for g in geoms])
+        if outputs is None:
+            ys = [torch.empty((g.lines, g.width), dtype=torch.float32, device=r.device) for r, g in zip(raws, geoms)]
+            chis = [torch.empty((g.lines, g.width, 2), dtype=torch.float32, device=r.device)
+                    for r, g in zip(raws, geoms)]
+        else:
+            ys, chis = outputs
+        ths = ([torch.empty(g.field_height, dtype=torch.float32, device=r.device) for r, g in zip(raws, geoms)]
+               if want_theta else None)
+        self._check(self.lib.hvd_gpu_decode_fields_u16(self.ctx, _ptrs(raws), n, G, blanking, white,
+                                                       C.byref(params), chroma_phase_deg, _ptrs(ys), _ptrs(chis),
+                                                       _ptrs(ths) if ths else None, _stream()),
+                    "decode_fields_u16")
+        return ys, chis, ths
+
+    def decode_fields_ire(self, fields, geoms, params: SolverParams, chroma_phase_deg=0.0, want_theta=False):
+        import torch
+        n = len(fields)
+        G = (Geometry * n)(*[Geometry.of(g) for g in geoms])
+        ys = [torch.empty((g.lines, g.width), dtype=torch.float32, device=f.device) for f, g in zip(fields, geoms)]
+        chis = [torch.empty((g.lines, g.width, 2), dtype=torch.float32, device=f.device)
+                for f, g in zip(fields, geoms)]
+        ths = ([torch.empty(g.field_height, dtype=torch.float32, device=f.device) for f, g in zip(fields, geoms)]
+               if want_theta else None)
+        self._check(self.lib.hvd_gpu_decode_fields_ire(self.ctx, _ptrs(fields), n, G, C.byref(params),
+                                                       chroma_phase_deg, _ptrs(ys), _ptrs(chis),
+                                                       _ptrs(ths) if ths else None, _stream()),
+                    "decode_fields_ire")
+        return ys, chis, ths
+
+    # ---- motion ---------------------------------------------------------------
+    def block_match_sad_u16(self, refs, curs, bh=8, bw=16, sy=4, sx=16):
+        import torch
+        n = len(refs)
+        h, w = refs[0].shape
+        nby, nbx = -(-h // bh), -(-w // bw)
+        dev = refs[0].device
+        vy = [torch.empty((nby, nbx), dtype=torch.int16, device=dev) for _ in range(n)]
+        vx = [torch.empty((nby, nbx), dtype=torch.int16, device=dev) for _ in range(n)]
+        sad = [torch.empty((nby, nbx), dtype=torch.int32, device=dev) for _ in range(n)]
+        self._check(self.lib.hvd_gpu_block_match_sad_u16(self.ctx, _ptrs(refs), _ptrs(curs), n, h, w, bh, bw, sy, sx,
+                                                         _ptrs(vy), _ptrs(vx), _ptrs(sad), _stream()),
+                    "block_match_sad_u16")
+        return vy, vx, sad
+
+
+def geometry_phase_per_sample(g) -> float:
+    if g.subcarrier_hz <= 0:
+        return 0.5 * math.pi
+    return 2 * math.pi * g.subcarrier_hz / g.sample_rate

@@ -1,0 +1,195 @@
+/*
+ * hvd_gpu.h -- C ABI of the MI355X (gfx950) Y/C-separation engine.
+ *
+ * The host side of hvd-core (its C++ engine API and frame_bridge) calls
+ * these entry points instead of its CPU loops; each declaration names the
+ * hvd-core routine it replaces.  Plain C: pointers, sizes, POD structs and
+ * int status codes; no C++ types or exceptions cross this boundary.
+ *
+ * Conventions
+ *  - Planes are row-major; complex planes are interleaved (re, im) float
+ *    pairs, i.e. std::complex<float> / hvd::ComplexPlane storage.
+ *  - Every data pointer is a DEVICE pointer unless the parameter says host.
+ *  - Work is ordered on `stream` (a hipStream_t; NULL = the default stream)
+ *    and is asynchronous unless stated otherwise.
+ *  - A context owns its device workspace; calls on one context must be
+ *    issued from one host thread on one stream at a time, like an
+ *    hvd::HvdEngine (engine.h:70-76, frame_bridge.h:139-141).
+ *  - Status: HVD_GPU_OK (0) or a positive error code; the message is
+ *    available from hvd_gpu_last_error().  Invalid arguments are rejected
+ *    before any work is queued.
+ */
+#ifndef HVD_GPU_H_
+#define HVD_GPU_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HVD_GPU_OK 0
+#define HVD_GPU_ERR_INVALID 1     /* bad argument / size out of range */
+#define HVD_GPU_ERR_HIP 2         /* HIP runtime error */
+#define HVD_GPU_ERR_UNSUPPORTED 3 /* a configuration this build does not run */
+#define HVD_GPU_ERR_NOMEM 4
+
+typedef struct hvd_gpu_ctx hvd_gpu_ctx;
+
+/* hvd::FieldGeometry (engine/ntsc_geometry.h:142-267). */
+typedef struct {
+  int field_width;
+  int field_height;
+  int active_video_start;
+  int active_video_end;
+  int colour_burst_start;
+  int colour_burst_end;
+  int first_active_field_line;
+  int last_active_field_line; /* 0 => field_height */
+  double sample_rate;
+  double subcarrier_hz; /* 0 => the standard's nominal fsc */
+  int standard;         /* 0 NTSC, 1 PAL, 2 PAL-M (hvd::VideoStandard) */
+} hvd_field_geometry;
+
+/* The solver knobs of hvd::HvdConfig the hot path reads
+ * (engine/hvd_config.h:26-102), plus lambda_y: the luma prior weight,
+ * implicitly 1 in hvd-core (variational.cpp:302); 1.0 reproduces it. */
+typedef struct {
+  double lambda_c;
+  double chroma_aniso; /* 0 => AUTO (variational.cpp:130-196) */
+  double charbonnier_eps;
+  double chroma_eps;
+  double structure_coupling;
+  double diag_prior; /* must be 0 in this build (HVD_GPU_ERR_UNSUPPORTED) */
+  int cg_iterations;
+  int irls_outer;
+  double cg_tol; /* 0 => auto (0.02 slow / 0.10 fast) */
+  int fast;
+  int is_pal; /* derived from the geometry by the field entry points */
+  double lambda_y;
+} hvd_solver_params;
+
+/* One field's active plane for the init and the solve.  The carrier is
+ * either an explicit interleaved plane (`carrier` non-NULL, as hvd-core's
+ * ComplexPlane) or, with `carrier` NULL, the factored form
+ *     c(y, x) = exp(i (theta[y] + phase_per_sample (active_video_start + x)))
+ * with c -> -conj(c) on lines whose vswitch < 0 (MakeCarrier /
+ * MakeCarrierPal, ntsc_geometry.cpp:29-93), which costs no memory traffic. */
+typedef struct {
+  int lines;
+  int width;
+  const float* s;        /* lines * width, IRE */
+  const float* theta;    /* lines (factored carrier) */
+  const int8_t* vswitch; /* lines, or NULL */
+  const float* carrier;  /* lines * width * 2, or NULL */
+  int active_video_start;
+  double phase_per_sample;
+  double sample_rate;
+  float* chi; /* lines * width * 2: refine in/out; init out */
+  float* y;   /* lines * width: luma out, may be NULL */
+} hvd_gpu_plane;
+
+/* ---- context ---------------------------------------------------------- */
+
+/* Creates a context on HIP device `device` (replaces the construction of
+ * hvd::HvdEngine, engine.h:70-76, and the Fft2d it owns, fft2d.h:22-58). */
+int hvd_gpu_create(int device, hvd_gpu_ctx** out);
+void hvd_gpu_destroy(hvd_gpu_ctx* ctx);
+/* Message for the last failing call on this context ("" if none). */
+const char* hvd_gpu_last_error(const hvd_gpu_ctx* ctx);
+/* Blocks until the context's queued work on `stream` is complete. */
+int hvd_gpu_synchronize(hvd_gpu_ctx* ctx, void* stream);
+/* ABI version (major * 100 + minor). */
+int hvd_gpu_abi_version(void);
+
+/* Diagnostics (no hvd-core counterpart): with profiling enabled every
+ * refine records HIP events on its stream around each IRLS stage's CG loop
+ * and keeps a copy of the per-field iteration counts.  hvd_gpu_cg_timing()
+ * synchronizes the device and returns, over all refines since profiling was
+ * enabled or last read: the summed CG-loop time (ms), the number of CG kernel
+ * launches in those loops, and sum over fields of pixels x CG iterations run;
+ * it then clears the record. */
+int hvd_gpu_set_profiling(hvd_gpu_ctx* ctx, int enable);
+int hvd_gpu_cg_timing(hvd_gpu_ctx* ctx, double* ms, long long* launches, double* pixel_iterations);
+
+/* ---- level conversion -------------------------------------------------- */
+
+/* uint16 codes -> IRE, ire = (code - blanking) / ((white - blanking) / 100)
+ * (SampleToIre, engine/ntsc_geometry.h:301-304; the split of int16 frames
+ * in SplitFrameFields, frame_bridge.cpp:61-99, is the same conversion). */
+int hvd_gpu_sample_to_ire_u16(hvd_gpu_ctx* ctx, const uint16_t* raw, size_t n, float blanking, float white,
+                              float* out, void* stream);
+
+/* ---- burst lock-in ----------------------------------------------------- */
+
+/* Per-line carrier phase for `n` fields: BurstLockinPhase (NTSC,
+ * engine/lockin.cpp:101-186) or, for PAL-family geometries, the joint phase
+ * + V-switch estimate BurstLockinPhasePal (lockin.cpp:257-349).
+ * fields[i]: field_height x field_width IRE; theta[i]: field_height floats;
+ * vswitch[i]: field_height int8 (written for PAL-family fields only; may be
+ * NULL for NTSC).  `fields`, `theta`, `vswitch` and `geom` are HOST arrays of
+ * n entries (the pointers they hold are device pointers). */
+int hvd_gpu_burst_lockin_phase(hvd_gpu_ctx* ctx, const float* const* fields, int n, const hvd_field_geometry* geom,
+                               float* const* theta, int8_t* const* vswitch, void* stream);
+
+/* Burst amplitude in IRE for ACC, BurstAmplitudeIre (lockin.cpp:188-255);
+ * amp: device array of n floats. */
+int hvd_gpu_burst_amplitude_ire(hvd_gpu_ctx* ctx, const float* const* fields, int n, const hvd_field_geometry* geom,
+                                float* amp, void* stream);
+
+/* ---- holographic init and variational solve ------------------------------ */
+
+/* HolographicInit (engine/holographic_init.cpp:200-317) for n planes of
+ * possibly different sizes: writes chi0 into planes[i].chi and Y0 into
+ * planes[i].y.  `planes` is a HOST array. */
+int hvd_gpu_holographic_init(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, int symmetry_variant,
+                             void* stream);
+
+/* VariationalRefine, 2-D path (engine/variational.cpp:198-556) for n planes
+ * in one batch: planes[i].chi holds chi0 on entry and chi on return;
+ * planes[i].y receives S - Re[chi c].  p->is_pal selects the AUTO aniso
+ * line grouping for every plane.  Optional HOST outputs: stage_iters
+ * (n * irls_outer ints, CG iterations run per stage) and aniso (n floats);
+ * asking for either makes the call synchronous. */
+int hvd_gpu_variational_refine(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hvd_solver_params* p,
+                               int* stage_iters, float* aniso, void* stream);
+
+/* ---- field driver ------------------------------------------------------ */
+
+/* The 2-D field path of DecodeFieldWindow (engine/sequence.cpp:229-281 and
+ * :1046, output luma :1102-1114) for n raw fields of possibly different
+ * standards and sizes, in one batch: uint16 codes -> IRE -> burst lock-in
+ * -> active crop + carrier -> holographic init -> IRLS/CG solve.
+ * raw[i]: field_height x field_width uint16 codes; y_out[i]: lines x width
+ * floats; chi_out[i]: lines x width x 2 floats; theta_out[i] (optional, may be
+ * NULL or hold NULLs): field_height floats.  `raw`, `geom`, `y_out`,
+ * `chi_out`, `theta_out` are HOST arrays of n entries.  blanking / white are
+ * the code levels of 0 and 100 IRE. */
+int hvd_gpu_decode_fields_u16(hvd_gpu_ctx* ctx, const uint16_t* const* raw, int n, const hvd_field_geometry* geom,
+                              float blanking, float white, const hvd_solver_params* p, float chroma_phase_deg,
+                              float* const* y_out, float* const* chi_out, float* const* theta_out, void* stream);
+
+/* Same, from fields already in IRE (FieldInput::samples, engine.h:48-51). */
+int hvd_gpu_decode_fields_ire(hvd_gpu_ctx* ctx, const float* const* field_ire, int n, const hvd_field_geometry* geom,
+                              const hvd_solver_params* p, float chroma_phase_deg, float* const* y_out,
+                              float* const* chi_out, float* const* theta_out, void* stream);
+
+/* ---- motion ------------------------------------------------------------ */
+
+/* Integer-SAD block matching over uint16 planes (no hvd-core counterpart:
+ * the integer form of BmPass, engine/motion.cpp:105-260).  For each of n
+ * (ref, cur) pairs of h x w planes: bw x bh blocks (edge blocks clipped),
+ * candidates |dy| <= sy, |dx| <= sx comparing cur(y, x) with the
+ * edge-clamped ref(y - dy, x - dx); winner = least (SAD, |dy| + |dx|,
+ * raster order).  Outputs ceil(h / bh) x ceil(w / bw) per pair.  Limits:
+ * bw <= 32, bh <= 16, sx <= 32, sy <= 16.  Pointer arrays are HOST arrays. */
+int hvd_gpu_block_match_sad_u16(hvd_gpu_ctx* ctx, const uint16_t* const* ref, const uint16_t* const* cur, int n,
+                                int h, int w, int bh, int bw, int sy, int sx, int16_t* const* vy, int16_t* const* vx,
+                                int32_t* const* sad, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HVD_GPU_H_ */

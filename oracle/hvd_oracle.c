@@ -941,7 +941,7 @@ void OR_FN(prepare_field)(const R *field, const or_geom *g, R chroma_phase_deg, 
 }
 
 /* One field through the whole 2-D path: prepare -> init -> refine.  Returns
- * the resolved aniso (0 when the refine is skipped). */
+ * the resolved aniso. */
 R OR_FN(decode_field)(const R *field, const or_geom *g, const or_cfg *cfg, R chroma_phase_deg,
                       R *y_out, R *chi_out, R *theta, int8_t *vswitch, int *stage_iters) {
   const int fh = g->field_height;
@@ -952,12 +952,11 @@ R OR_FN(decode_field)(const R *field, const or_geom *g, const or_cfg *cfg, R chr
   R *car = (R *)xmalloc(sizeof(R) * 2 * n);
   OR_FN(prepare_field)(field, g, chroma_phase_deg, s, car, theta, vswitch);
   OR_FN(holographic_init)(s, car, lines, width, g->sample_rate, 0, y_out, chi_out);
-  R aniso = 0;
-  if (cfg->cg_iterations > 0) {
-    or_cfg c2 = *cfg;
-    c2.is_pal = uses_vswitch(g);
-    aniso = OR_FN(variational_refine)(s, car, lines, width, &c2, chi_out, y_out, stage_iters);
-  }
+  /* sequence.cpp:1046 runs VariationalRefine unconditionally on the field
+   * path (with cg_iterations == 0 every stage still takes one step). */
+  or_cfg c2 = *cfg;
+  c2.is_pal = uses_vswitch(g);
+  const R aniso = OR_FN(variational_refine)(s, car, lines, width, &c2, chi_out, y_out, stage_iters);
   free(s); free(car);
   return aniso;
 }

@@ -17,6 +17,7 @@ HERE = pathlib.Path(__file__).resolve().parent
 LIB64 = HERE / "lib" / "liboracle64.so"
 LIB32 = HERE / "lib" / "liboracle32.so"
 REFLIB = HERE / "_ref" / "libhvdref.so"
+REFLIB_OMP = HERE / "_ref" / "libhvdref_omp.so"
 REF_SRC = pathlib.Path("/root/reference/hvd-core/src")
 
 
@@ -240,10 +241,11 @@ class Oracle:
 class Reference:
     """hvd-core's own engine code (FFT-free units), FP32 as it ships."""
 
-    def __init__(self):
-        if not REFLIB.exists():
-            raise FileNotFoundError(str(REFLIB))
-        self.lib = C.CDLL(str(REFLIB))
+    def __init__(self, openmp: bool = False):
+        path = REFLIB_OMP if openmp else REFLIB
+        if not path.exists():
+            raise FileNotFoundError(str(path))
+        self.lib = C.CDLL(str(path))
         self.lib.ref_burst_amplitude_ire.restype = C.c_float
         self.lib.ref_acc_gain.restype = C.c_float
 

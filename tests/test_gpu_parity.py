@@ -1,0 +1,307 @@
+"""GPU parity: every hot-path entry point of the C ABI against the FP64 oracle.
+
+Float results are held to TOL_IRE = 1e-4 of the 140 IRE code range (the
+precision bar for Y, C = S - Y and chi = V - iU); integer results (V-switch
+parity, SAD vectors) must be bit-exact.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import TOL_IRE
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from hvd_amd import gpu, synth  # noqa: E402
+import oracle as O  # noqa: E402
+
+DEV = "cuda:0"
+
+
+def dev(a, dtype=None):
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.to(DEV)
+
+
+def cplx_dev(z):
+    return dev(np.stack([z.real, z.imag], -1).astype(np.float32))
+
+
+def cplx_host(t):
+    a = t.cpu().numpy()
+    return a[..., 0].astype(np.float64) + 1j * a[..., 1]
+
+
+def field_ire(raw):
+    return synth.ire_from_raw(raw)
+
+
+def theta_err_ok(got, want):
+    """theta is stored as float32 (as hvd-core returns it); allow one float32
+    ulp of the unwrapped phase plus 2e-6 rad of arithmetic difference."""
+    err = np.abs(np.angle(np.exp(1j * (np.asarray(got, np.float64) - want))))
+    bound = np.spacing(np.abs(want).astype(np.float32)).astype(np.float64) + 2e-6
+    return bool(np.all(err <= bound)), float(err.max())
+
+
+# ---------------------------------------------------------------- lock-in
+@pytest.mark.parametrize("seed", [1, 2])
+def test_lockin_ntsc(engine, orc64, seed):
+    raw, _, _ = synth.ntsc_field(seed)
+    f = field_ire(raw)
+    th, vs = engine.burst_lockin_phase([dev(f)], [synth.NTSC])
+    got = th[0].cpu().numpy().astype(np.float64)
+    want = orc64.burst_lockin_phase(f, synth.NTSC)
+    assert vs[0] is None
+    ok, worst = theta_err_ok(got, want)
+    assert ok, worst
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_lockin_pal(engine, orc64, seed):
+    raw, _, _ = synth.pal_field(seed)
+    f = field_ire(raw)
+    th, vs = engine.burst_lockin_phase([dev(f)], [synth.PAL])
+    want_t, want_v = orc64.burst_lockin_phase_pal(f, synth.PAL)
+    assert np.array_equal(vs[0].cpu().numpy(), want_v)
+    got = th[0].cpu().numpy().astype(np.float64)
+    ok, worst = theta_err_ok(got, want_t)
+    assert ok, worst
+
+
+def test_lockin_no_burst(engine, orc64):
+    # a field with no burst anywhere keeps raw angles (NTSC) / model + alternating parity (PAL)
+    for g in (synth.NTSC, synth.PAL):
+        f = np.zeros((g.field_height, g.field_width), np.float32)
+        th, vs = engine.burst_lockin_phase([dev(f)], [g])
+        if g.uses_vswitch:
+            wt, wv = orc64.burst_lockin_phase_pal(f, g)
+            assert np.array_equal(vs[0].cpu().numpy(), wv)
+        else:
+            wt = orc64.burst_lockin_phase(f, g)
+        ok, worst = theta_err_ok(th[0].cpu().numpy(), wt)
+        assert ok, worst
+
+
+def test_burst_amplitude(engine, orc64):
+    raw_n, _, _ = synth.ntsc_field(5)
+    raw_p, _, _ = synth.pal_field(6)
+    fs = [field_ire(raw_n), field_ire(raw_p)]
+    amp = engine.burst_amplitude_ire([dev(f) for f in fs], [synth.NTSC, synth.PAL]).cpu().numpy()
+    want = [orc64.burst_amplitude_ire(fs[0], synth.NTSC), orc64.burst_amplitude_ire(fs[1], synth.PAL)]
+    assert np.allclose(amp, want, atol=1e-4)
+    assert abs(amp[0] - 20.0) < 0.3 and abs(amp[1] - 21.43) < 0.3
+
+
+# ---------------------------------------------------------------- init
+def _prepared(orc64, g, raw):
+    f = field_ire(raw)
+    s, car, th, vs = orc64.prepare_field(f, g)
+    fal = g.first_active_field_line
+    th_act = th[fal:fal + g.lines].astype(np.float32)
+    vs_act = vs[fal:fal + g.lines] if g.uses_vswitch else None
+    # the oracle's carrier from the same float theta the GPU receives
+    car = orc64.make_carrier(th_act, vs_act, g)
+    return s.astype(np.float32), car, th_act, vs_act
+
+
+def _plane(g, s, chi, y, th, vs):
+    return gpu.Engine.plane(s, chi, y, theta=th, vswitch=vs, a0=g.active_video_start,
+                            w0=gpu.geometry_phase_per_sample(g), sample_rate=g.sample_rate)
+
+
+@pytest.mark.parametrize("std", ["ntsc", "pal"])
+def test_holographic_init(engine, orc64, std):
+    g = synth.NTSC if std == "ntsc" else synth.PAL
+    raw = (synth.ntsc_field if std == "ntsc" else synth.pal_field)(11)[0]
+    s, car, th, vs = _prepared(orc64, g, raw)
+    y0_ref, chi0_ref = orc64.holographic_init(s, car, g.sample_rate)
+    ds, dchi, dy = dev(s), torch.zeros((g.lines, g.width, 2), device=DEV), torch.zeros((g.lines, g.width), device=DEV)
+    dth = dev(th)
+    dvs = None if vs is None else dev(vs)
+    engine.holographic_init([_plane(g, ds, dchi, dy, dth, dvs)])
+    chi0 = cplx_host(dchi)
+    y0 = dy.cpu().numpy()
+    assert np.abs(chi0 - chi0_ref).max() <= TOL_IRE
+    assert np.abs(y0 - y0_ref).max() <= TOL_IRE
+    # lossless split Y0 + Re[chi0 c] = S (lossless_identity_test.cpp:73-74)
+    assert np.abs(y0 + (chi0 * car).real - s).max() < 1e-3
+
+
+def test_holographic_init_symmetry_variant(engine, orc64):
+    g = synth.NTSC
+    s, car, th, vs = _prepared(orc64, g, synth.ntsc_field(12)[0])
+    y0_ref, chi0_ref = orc64.holographic_init(s, car, g.sample_rate, symmetry_variant=True)
+    dchi = torch.zeros((g.lines, g.width, 2), device=DEV)
+    dy = torch.zeros((g.lines, g.width), device=DEV)
+    engine.holographic_init([_plane(g, dev(s), dchi, dy, dev(th), None)], symmetry_variant=True)
+    assert np.abs(cplx_host(dchi) - chi0_ref).max() <= TOL_IRE
+    assert np.abs(dy.cpu().numpy() - y0_ref).max() <= TOL_IRE
+
+
+# ---------------------------------------------------------------- solve
+CFG0 = dict(cg_iterations=100, irls_outer=4, fast=0, cg_tol=1e-12)
+
+
+def _refine_case(engine, orc64, g, raw, **kw):
+    s, car, th, vs = _prepared(orc64, g, raw)
+    _, chi0 = orc64.holographic_init(s, car, g.sample_rate)
+    chi0 = chi0.astype(np.complex64).astype(np.complex128)
+    cfg = O.Cfg.make(is_pal=int(g.uses_vswitch), **kw)
+    y_ref, chi_ref, an_ref, it_ref = orc64.variational_refine(s, car, chi0, cfg)
+    dchi = cplx_dev(chi0)
+    dy = torch.zeros((g.lines, g.width), device=DEV)
+    p = gpu.SolverParams.make(is_pal=int(g.uses_vswitch), **kw)
+    iters, aniso = engine.variational_refine([_plane(g, dev(s), dchi, dy, dev(th), None if vs is None else dev(vs))],
+                                             p, want_stats=True)
+    chi = cplx_host(dchi)
+    y = dy.cpu().numpy()
+    return dict(s=s, car=car, y=y, chi=chi, y_ref=y_ref, chi_ref=chi_ref, aniso=aniso[0], aniso_ref=an_ref,
+                iters=iters[0], iters_ref=it_ref)
+
+
+@pytest.mark.parametrize("std", ["ntsc", "pal"])
+def test_refine_full_field(engine, orc64, std):
+    g = synth.NTSC if std == "ntsc" else synth.PAL
+    raw = (synth.ntsc_field if std == "ntsc" else synth.pal_field)(21)[0]
+    r = _refine_case(engine, orc64, g, raw, **CFG0)
+    assert abs(r["aniso"] - r["aniso_ref"]) < 1e-4
+    assert np.abs(r["y"] - r["y_ref"]).max() <= TOL_IRE
+    assert np.abs(r["chi"] - r["chi_ref"]).max() <= TOL_IRE
+    c, c_ref = r["s"] - r["y"], r["s"] - r["y_ref"]
+    assert np.abs(c - c_ref).max() <= TOL_IRE
+    # lossless split (variational_v2_test.cpp:112)
+    assert np.abs(r["y"] + (r["chi"] * r["car"]).real - r["s"]).max() < 1e-3
+
+
+@pytest.mark.parametrize("kw", [dict(cg_iterations=2, irls_outer=4, fast=1),          # hvd-core defaults
+                                dict(cg_iterations=60, irls_outer=4, fast=1),
+                                dict(cg_iterations=30, irls_outer=3, fast=0, chroma_aniso=0.5),
+                                dict(cg_iterations=40, irls_outer=4, fast=0, cg_tol=0.2, lambda_c=2.0)])
+def test_refine_configs(engine, orc64, kw):
+    r = _refine_case(engine, orc64, synth.NTSC, synth.ntsc_field(31)[0], **kw)
+    assert np.array_equal(r["iters"], r["iters_ref"])
+    assert np.abs(r["y"] - r["y_ref"]).max() <= TOL_IRE
+    assert np.abs(r["chi"] - r["chi_ref"]).max() <= TOL_IRE
+
+
+def _scene(h, w):
+    # variational_v2_test.cpp:64-95 style solver-only scene with a corrupted chi0
+    r, c = np.mgrid[0:h, 0:w].astype(np.float64)
+    car = np.exp(1j * (0.5 * math.pi * c))
+    y_true = 30 + 20 * np.sin(0.35 * (r + c))
+    diag = np.sin(0.4 * (c - r))
+    chi_true = 10 * diag - 6j * diag
+    s = (y_true + (chi_true * car).real).astype(np.float32)
+    e = 3 * np.sin(0.9 * c) * np.cos(0.7 * r)
+    chi0 = (chi_true + e - 1j * e).astype(np.complex64).astype(np.complex128)
+    return s, car.astype(np.complex64).astype(np.complex128), chi0
+
+
+@pytest.mark.parametrize("hw", [(48, 56), (1, 1), (1, 9), (9, 1), (2, 2), (5, 130), (67, 3)])
+def test_refine_explicit_carrier_shapes(engine, orc64, hw):
+    h, w = hw
+    s, car, chi0 = _scene(h, w)
+    cfg = dict(cg_iterations=60, irls_outer=4, fast=0, chroma_aniso=0.5)
+    y_ref, chi_ref, _, it_ref = orc64.variational_refine(s, car, chi0, O.Cfg.make(**cfg))
+    dchi = cplx_dev(chi0)
+    dy = torch.zeros((h, w), device=DEV)
+    pl = gpu.Engine.plane(dev(s), dchi, dy, carrier=cplx_dev(car))
+    iters, _ = engine.variational_refine([pl], gpu.SolverParams.make(**cfg), want_stats=True)
+    assert np.abs(dy.cpu().numpy() - y_ref).max() <= TOL_IRE
+    assert np.abs(cplx_host(dchi) - chi_ref).max() <= TOL_IRE
+
+
+def test_refine_batch_matches_single(engine, orc64):
+    # a batch of uneven planes gives the same result as one launch per plane
+    cases = [_scene(48, 56), _scene(30, 200), _scene(7, 7)]
+    cfg = gpu.SolverParams.make(cg_iterations=40, irls_outer=4, fast=0, chroma_aniso=0.7)
+    single = []
+    for s, car, chi0 in cases:
+        dchi, dy = cplx_dev(chi0), torch.zeros(s.shape, device=DEV)
+        engine.variational_refine([gpu.Engine.plane(dev(s), dchi, dy, carrier=cplx_dev(car))], cfg)
+        single.append((dy.cpu().numpy(), dchi.cpu().numpy()))
+    planes, outs = [], []
+    for s, car, chi0 in cases:
+        dchi, dy = cplx_dev(chi0), torch.zeros(s.shape, device=DEV)
+        planes.append(gpu.Engine.plane(dev(s), dchi, dy, carrier=cplx_dev(car)))
+        outs.append((dy, dchi))
+    engine.variational_refine(planes, cfg)
+    for (y1, c1), (dy, dchi) in zip(single, outs):
+        assert np.array_equal(y1, dy.cpu().numpy()) and np.array_equal(c1, dchi.cpu().numpy())
+
+
+# ---------------------------------------------------------------- field driver
+@pytest.mark.parametrize("std", ["ntsc", "pal"])
+def test_decode_field_u16(engine, orc64, std):
+    g = synth.NTSC if std == "ntsc" else synth.PAL
+    raw = (synth.ntsc_field if std == "ntsc" else synth.pal_field)(41)[0]
+    cfg = dict(cg_iterations=40, irls_outer=4, fast=0, cg_tol=1e-6)
+    want = orc64.decode_field(field_ire(raw), g, O.Cfg.make(**cfg))
+    ys, chis, ths = engine.decode_fields_u16([dev(raw.view(np.int16))], [g], gpu.SolverParams.make(**cfg),
+                                             want_theta=True)
+    ok, worst = theta_err_ok(ths[0].cpu().numpy(), want["theta"])
+    assert ok, worst
+    assert np.abs(ys[0].cpu().numpy() - want["y"]).max() <= TOL_IRE
+    assert np.abs(cplx_host(chis[0]) - want["chi"]).max() <= TOL_IRE
+
+
+def test_decode_mixed_batch(engine, orc64):
+    # 2 NTSC + 2 PAL fields of uneven size in one call equal per-field calls
+    raws = [synth.ntsc_field(51)[0], synth.pal_field(52)[0], synth.ntsc_field(53)[0], synth.pal_field(54)[0]]
+    gs = [synth.NTSC, synth.PAL, synth.NTSC, synth.PAL]
+    p = gpu.SolverParams.make(cg_iterations=40, irls_outer=4, fast=0, cg_tol=1e-6)
+    ys, chis, _ = engine.decode_fields_u16([dev(r.view(np.int16)) for r in raws], gs, p)
+    for r, g, y, c in zip(raws, gs, ys, chis):
+        y1, c1, _ = engine.decode_fields_u16([dev(r.view(np.int16))], [g], p)
+        assert np.array_equal(y.cpu().numpy(), y1[0].cpu().numpy())
+        assert np.array_equal(c.cpu().numpy(), c1[0].cpu().numpy())
+
+
+def test_decode_chroma_phase(engine, orc64):
+    g = synth.PAL
+    raw = synth.pal_field(61)[0]
+    cfg = dict(cg_iterations=20, irls_outer=2, fast=0)
+    want = orc64.decode_field(field_ire(raw), g, O.Cfg.make(**cfg), chroma_phase_deg=12.5)
+    ys, chis, _ = engine.decode_fields_u16([dev(raw.view(np.int16))], [g], gpu.SolverParams.make(**cfg),
+                                           chroma_phase_deg=12.5)
+    assert np.abs(ys[0].cpu().numpy() - want["y"]).max() <= TOL_IRE
+    assert np.abs(cplx_host(chis[0]) - want["chi"]).max() <= TOL_IRE
+
+
+# ---------------------------------------------------------------- motion
+def test_sad_block_match_sequence(engine, orc64):
+    raws, _ = synth.motion_sequence(71, n_fields=4)
+    refs, curs = raws[:-1], raws[1:]
+    vy, vx, sad = engine.block_match_sad_u16([dev(r.view(np.int16)) for r in refs],
+                                             [dev(c.view(np.int16)) for c in curs])
+    for k in range(len(refs)):
+        wy, wx, ws = orc64.block_match_sad_u16(refs[k], curs[k])
+        assert np.array_equal(vy[k].cpu().numpy(), wy)
+        assert np.array_equal(vx[k].cpu().numpy(), wx)
+        assert np.array_equal(sad[k].cpu().numpy(), ws)
+
+
+@pytest.mark.parametrize("shape,blk,srch", [((17, 23), (8, 16), (4, 16)), ((8, 16), (8, 16), (0, 0)),
+                                             ((40, 70), (16, 32), (16, 32)), ((33, 5), (4, 4), (2, 3))])
+def test_sad_block_match_edges(engine, orc64, shape, blk, srch):
+    rng = np.random.default_rng(5)
+    ref_p = rng.integers(0, 65536, size=shape, dtype=np.uint16)
+    cur_p = np.roll(ref_p, (1, -2), axis=(0, 1))
+    flat = np.full(shape, 1234, np.uint16)  # all-tie field: zero vector must win
+    for a, b in ((ref_p, cur_p), (flat, flat)):
+        vy, vx, sad = engine.block_match_sad_u16([dev(a.view(np.int16))], [dev(b.view(np.int16))],
+                                                 bh=blk[0], bw=blk[1], sy=srch[0], sx=srch[1])
+        wy, wx, ws = orc64.block_match_sad_u16(a, b, bh=blk[0], bw=blk[1], sy=srch[0], sx=srch[1])
+        assert np.array_equal(vy[0].cpu().numpy(), wy) and np.array_equal(vx[0].cpu().numpy(), wx)
+        assert np.array_equal(sad[0].cpu().numpy(), ws)
+
+
+def test_sample_to_ire(engine):
+    raw = np.arange(0, 65536, 7, dtype=np.uint16)
+    got = engine.sample_to_ire_u16(dev(raw.view(np.int16)), 20480.0, 61440.0).cpu().numpy()
+    assert np.array_equal(got, synth.ire_from_raw(raw))
