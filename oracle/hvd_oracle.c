@@ -26,6 +26,20 @@ typedef OR_REAL R;
 
 #define PI_D 3.14159265358979323846
 
+/* Scalar math in the build's precision: the FP32 build calls the float
+ * functions hvd-core's float code resolves to (cosf, atan2f, hypotf ...). */
+#if OR_FP32
+#define RCOS cosf
+#define RSIN sinf
+#define RATAN2 atan2f
+#define RHYPOT hypotf
+#else
+#define RCOS cos
+#define RSIN sin
+#define RATAN2 atan2
+#define RHYPOT hypot
+#endif
+
 typedef struct { R re, im; } C;
 typedef struct { double re, im; } CD;
 
@@ -33,7 +47,7 @@ static inline C cmul(C a, C b) { C r = {a.re * b.re - a.im * b.im, a.re * b.im +
 static inline C cmulconj(C a, C b) { C r = {a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im}; return r; } /* a*conj(b) */
 static inline R re_prod(C a, C b) { return a.re * b.re - a.im * b.im; } /* Re[a*b] */
 static inline R cabs2(C a) { return a.re * a.re + a.im * a.im; }
-static inline R cabsr(C a) { return (R)hypot((double)a.re, (double)a.im); }
+static inline R cabsr(C a) { return RHYPOT(a.re, a.im); }
 
 static void *xmalloc(size_t n) { void *p = calloc(n ? n : 1, 1); if (!p) abort(); return p; }
 
@@ -89,7 +103,7 @@ static R median_np(const R *v, int n) {
   return m;
 }
 
-static R wrap_pi(R a) { return (R)atan2(sin((double)a), cos((double)a)); }
+static R wrap_pi(R a) { return RATAN2(RSIN(a), RCOS(a)); }
 
 /* ------------------------------------------------------------------------ */
 /* lock-in: engine/lockin.cpp                                              */
@@ -111,16 +125,16 @@ void OR_FN(burst_lockin)(const R *field, int h, int w, int x0, int x1, double w0
     mean /= (R)n;
     R ar = 0, ai = 0;
     for (int x = x0; x < x1; ++x) {
-      const double ph = -(double)(w0 * (R)x);
+      const R ph = -w0 * (R)x;
       const R v = row[x] - mean;
-      ar += v * (R)cos(ph);
-      ai += v * (R)sin(ph);
+      ar += v * RCOS(ph);
+      ai += v * RSIN(ph);
     }
     ar /= (R)n;
     ai /= (R)n;
     z[2 * y] = ar;
     z[2 * y + 1] = ai;
-    amp[y] = (R)hypot((double)ar, (double)ai);
+    amp[y] = RHYPOT(ar, ai);
   }
 }
 
@@ -181,7 +195,7 @@ void OR_FN(burst_lockin_phase)(const R *field, const or_geom *g, R *theta) {
   OR_FN(burst_lockin)(field, h, w, g->colour_burst_start, g->colour_burst_end,
                       OR_FN(phase_per_sample)(g), z, amp);
   const R half_pi = (R)(0.5 * PI_D);
-  for (int y = 0; y < h; ++y) theta[y] = (R)atan2((double)z[2 * y + 1], (double)z[2 * y]) - half_pi;
+  for (int y = 0; y < h; ++y) theta[y] = RATAN2(z[2 * y + 1], z[2 * y]) - half_pi;
   R amax = 0;
   for (int y = 0; y < h; ++y) if (amp[y] > amax) amax = amp[y];
   const R thr = amax > 0 ? amax * (R)0.2 : 1;
@@ -227,7 +241,7 @@ void OR_FN(burst_lockin_phase_pal)(const R *field, const or_geom *g, R *theta, i
     return;
   }
   R *am = (R *)xmalloc(sizeof(R) * (size_t)h);
-  for (int y = 0; y < h; ++y) am[y] = (R)atan2((double)z[2 * y + 1], (double)z[2 * y]);
+  for (int y = 0; y < h; ++y) am[y] = RATAN2(z[2 * y + 1], z[2 * y]);
   R best = (R)-1e30;
   R *model = (R *)xmalloc(sizeof(R) * (size_t)h);
   R *m = (R *)xmalloc(sizeof(R) * (size_t)h);
@@ -243,7 +257,7 @@ void OR_FN(burst_lockin_phase_pal)(const R *field, const or_geom *g, R *theta, i
       s[y] = sy > 0 ? 1 : -1;
       if (good[y]) {
         const R dev = wrap_pi(am[y] - (m[y] + half_pi - sy * qpi));
-        score += (R)cos((double)dev) * amp[y];
+        score += RCOS(dev) * amp[y];
       }
     }
     if (score > best) {
@@ -277,11 +291,11 @@ R OR_FN(burst_amplitude_ire)(const R *field, const or_geom *g) {
       R ar = 0, ai = 0;
       for (int k = 0; k < 4; ++k) {
         const int x = s0 + k;
-        const double ph = -(double)(w0 * (R)x);
-        ar += row[x] * (R)cos(ph);
-        ai += row[x] * (R)sin(ph);
+        const R ph = -w0 * (R)x;
+        ar += row[x] * RCOS(ph);
+        ai += row[x] * RSIN(ph);
       }
-      env[s0 - x0] = 2 * (R)hypot((double)ar, (double)ai) * (R)0.25;
+      env[s0 - x0] = 2 * RHYPOT(ar, ai) * (R)0.25;
     }
     per[y] = median_upper(env, ne);
   }
@@ -305,12 +319,17 @@ R OR_FN(acc_gain)(R measured, R nominal) {
 void OR_FN(make_carrier)(const R *theta, const int8_t *vswitch /* may be NULL */, int lines,
                          const or_geom *g, R *carrier) {
   const int a0 = g->active_video_start, width = g->active_video_end - g->active_video_start;
-  const double w0 = OR_FN(phase_per_sample)(g);
+  const R w0 = (R)OR_FN(phase_per_sample)(g);
   for (int y = 0; y < lines; ++y) {
     const int sw = vswitch && vswitch[y] < 0;
     for (int x = 0; x < width; ++x) {
-      const double ph = (double)theta[y] + w0 * (double)(a0 + x);
-      R cr = (R)cos(ph), ci = (R)sin(ph);
+      /* the phase argument is formed in R, as hvd-core forms it in float */
+      const R ph = theta[y] + w0 * (R)(a0 + x);
+#if OR_FP32
+      R cr = cosf(ph), ci = sinf(ph);
+#else
+      R cr = cos(ph), ci = sin(ph);
+#endif
       if (sw) cr = -cr;
       carrier[2 * ((size_t)y * width + x)] = cr;
       carrier[2 * ((size_t)y * width + x) + 1] = ci;
