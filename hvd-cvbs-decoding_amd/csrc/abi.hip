@@ -1045,4 +1045,139 @@ int hvd_gpu_block_match_sad_u16(hvd_gpu_ctx* ctx, const uint16_t* const* ref, co
   });
 }
 
+namespace {
+
+// motion.cpp:286-298: 0, -1, +1, -2, +2, ... within [lo, hi].
+std::vector<int> range_from_centre(int lo, int hi) {
+  std::vector<int> v;
+  if (lo > hi) return v;
+  if (lo <= 0 && 0 <= hi) v.push_back(0);
+  for (int d = 1; d <= std::max(-lo, hi); ++d) {
+    if (-d >= lo && -d <= hi) v.push_back(-d);
+    if (d >= lo && d <= hi) v.push_back(d);
+  }
+  if (v.empty())
+    for (int i = lo; i <= hi; ++i) v.push_back(i);
+  return v;
+}
+
+inline unsigned blocks_for(size_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+int hvd_gpu_estimate_motion(hvd_gpu_ctx* ctx, const float* const* y_ref, const float* const* y_cur, int n, int h,
+                            int w, int tile, int search, float* const* dy, float* const* dx, float* const* conf,
+                            void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, n >= 0 && (n == 0 || (y_ref && y_cur && dy && dx && conf)), "null argument");
+    REQUIRE(ctx, h >= 4 && w >= 4, "planes must be at least 4 x 4 (the coarse level decimates by 4)");
+    REQUIRE(ctx, tile >= 1 && tile <= 256, "tile %d outside 1..256", tile);
+    REQUIRE(ctx, search >= 0 && search <= 64, "search %d outside 0..64", search);
+    if (n == 0) return;
+    hipStream_t st = S(stream);
+    constexpr int f = 4;
+    const int th = (h + tile - 1) / tile, tw = (w + tile - 1) / tile, nt = th * tw;
+    const int hh = h / f, ww = w / f;
+    const int ct = std::max(4, tile / f);
+    const int cth = (hh + ct - 1) / ct, ctw = (ww + ct - 1) / ct, nct = cth * ctw;
+    const int cs = std::max(1, (search + f - 1) / f);
+    const std::vector<int> rc = range_from_centre(-cs, cs), r3 = range_from_centre(-3, 3), r0 = {0};
+    const size_t np = (size_t)h * w;
+    const size_t nse = std::max((size_t)nct * rc.size() * rc.size(), (size_t)nt * r3.size() * r3.size());
+    Carve cv;
+    const size_t o_tmp = cv.take(sizeof(float) * n * 2 * np), o_P = cv.take(sizeof(float) * n * 2 * np);
+    const size_t o_Pd = cv.take(sizeof(float) * n * 2 * (size_t)hh * ww);
+    const size_t o_se = cv.take(sizeof(float) * n * nse);
+    const size_t o_c = cv.take(sizeof(float) * n * nct * 3);
+    const size_t o_v = cv.take(sizeof(float) * n * nt * 19);
+    char* ws = ensure_ws(ctx, cv.off, st);
+    float* tmp = (float*)(ws + o_tmp);
+    float* P = (float*)(ws + o_P);
+    float* Pd = (float*)(ws + o_Pd);
+    float* se = (float*)(ws + o_se);
+    float *cdy = (float*)(ws + o_c), *cdx = cdy + n * nct, *cbest = cdx + n * nct;
+    float* v = (float*)(ws + o_v);
+    auto vec = [&](int k) { return v + (size_t)k * n * nt; };
+    float *bdy = vec(0), *bdx = vec(1), *mdy = vec(2), *mdx = vec(3), *best = vec(4), *tdy = vec(5), *tdx = vec(6);
+    float *jy = vec(7), *jx = vec(8), *cm[4] = {vec(9), vec(10), vec(11), vec(12)}, *best0 = vec(13), *cost0 = vec(14);
+    float *ody = vec(15), *odx = vec(16), *scratch = vec(17);  // scratch spans vec(17..18)
+    std::vector<MotionDesc> D(n);
+    for (int i = 0; i < n; ++i) {
+      REQUIRE(ctx, y_ref[i] && y_cur[i] && dy[i] && dx[i] && conf[i], "pair %d: null pointer", i);
+      D[i] = MotionDesc{y_ref[i], y_cur[i]};
+    }
+    std::vector<float*> cptr(conf, conf + n);
+    Stage stg;
+    const size_t oD = stg.put(D), oRc = stg.put(rc), oR3 = stg.put(r3), oR0 = stg.put(r0), oC = stg.put(cptr);
+    Slot* sl;
+    char* base = upload(ctx, stg, st, &sl);
+    const int* drc = (const int*)(base + oRc);
+    const int* dr3 = (const int*)(base + oR3);
+    const int* dr0 = (const int*)(base + oR0);
+    const int nrc = (int)rc.size(), nr3 = (int)r3.size();
+    // (1) BoxBlur r = 2 of both planes (:398-399)
+    hipLaunchKernelGGL(k_me_blur_cols, dim3(blocks_for((size_t)n * 2 * w)), dim3(256), 0, st,
+                       (const MotionDesc*)(base + oD), n, h, w, 2, tmp);
+    LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_me_blur_rows, dim3(blocks_for((size_t)n * 2 * h)), dim3(256), 0, st, n, h, w, 2,
+                       (const float*)tmp, P);
+    LAUNCH_CHECK(ctx);
+    // (2) 4x-decimated exhaustive coarse search (:405-414)
+    hipLaunchKernelGGL(k_me_decimate, dim3(blocks_for((size_t)n * 2 * hh * ww)), dim3(256), 0, st, n, h, w, f,
+                       (const float*)P, Pd);
+    LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_me_ssd, dim3(blocks_for((size_t)n * nct * nrc * nrc)), dim3(256), 0, st, n, hh, ww, ct,
+                       (const float*)Pd, drc, drc, nrc, nrc, (const float*)nullptr, (const float*)nullptr, se);
+    LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_me_select, dim3(blocks_for((size_t)n * nct)), dim3(256), 0, st, n, nct, drc, drc, nrc, nrc,
+                       (const float*)se, (const float*)nullptr, (const float*)nullptr, cdy, cdx, cbest, (float*)nullptr);
+    LAUNCH_CHECK(ctx);
+    // (3) upscale onto the full-res tile grid, x 4 (:416-428)
+    hipLaunchKernelGGL(k_me_upscale, dim3(blocks_for((size_t)n * nt)), dim3(256), 0, st, n, th, tw, cth, ctw, f,
+                       (const float*)cdy, (const float*)cdx, bdy, bdx);
+    LAUNCH_CHECK(ctx);
+    // (4) +/-3 refinement around the coarse vector (:431-435)
+    hipLaunchKernelGGL(k_me_ssd, dim3(blocks_for((size_t)n * nt * nr3 * nr3)), dim3(256), 0, st, n, h, w, tile,
+                       (const float*)P, dr3, dr3, nr3, nr3, (const float*)bdy, (const float*)bdx, se);
+    LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_me_select, dim3(blocks_for((size_t)n * nt)), dim3(256), 0, st, n, nt, dr3, dr3, nr3, nr3,
+                       (const float*)se, (const float*)bdy, (const float*)bdx, mdy, mdx, best, (float*)nullptr);
+    LAUNCH_CHECK(ctx);
+    // (5) the four costs of the parabolic fit (:438-441)
+    const float offs[4][2] = {{-1.f, 0.f}, {1.f, 0.f}, {0.f, -1.f}, {0.f, 1.f}};
+    for (int k = 0; k < 4; ++k) {
+      hipLaunchKernelGGL(k_me_offset, dim3(blocks_for((size_t)n * nt)), dim3(256), 0, st, n * nt, (const float*)mdy,
+                         (const float*)mdx, offs[k][0], offs[k][1], tdy, tdx);
+      LAUNCH_CHECK(ctx);
+      hipLaunchKernelGGL(k_me_ssd, dim3(blocks_for((size_t)n * nt)), dim3(256), 0, st, n, h, w, tile,
+                         (const float*)P, dr0, dr0, 1, 1, (const float*)tdy, (const float*)tdx, se);
+      LAUNCH_CHECK(ctx);
+      hipLaunchKernelGGL(k_me_select, dim3(blocks_for((size_t)n * nt)), dim3(256), 0, st, n, nt, dr0, dr0, 1, 1,
+                         (const float*)se, (const float*)tdy, (const float*)tdx, jy, jx, cm[k], (float*)nullptr);
+      LAUNCH_CHECK(ctx);
+    }
+    // (6) explicit zero-motion evaluation (:458-460)
+    hipLaunchKernelGGL(k_me_ssd, dim3(blocks_for((size_t)n * nt)), dim3(256), 0, st, n, h, w, tile, (const float*)P,
+                       dr0, dr0, 1, 1, (const float*)nullptr, (const float*)nullptr, se);
+    LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_me_select, dim3(blocks_for((size_t)n * nt)), dim3(256), 0, st, n, nt, dr0, dr0, 1, 1,
+                       (const float*)se, (const float*)nullptr, (const float*)nullptr, jy, jx, best0, cost0);
+    LAUNCH_CHECK(ctx);
+    // (7) parabolic half-pel, prefer-zero margin, vector sum (:443-485)
+    hipLaunchKernelGGL(k_me_finish, dim3(blocks_for((size_t)n * nt)), dim3(256), 0, st, n * nt, (const float*)cm[0],
+                       (const float*)cm[1], (const float*)cm[2], (const float*)cm[3], (const float*)best0,
+                       (const float*)cost0, mdy, mdx, best, ody, odx);
+    LAUNCH_CHECK(ctx);
+    // (8) confidence (:321-387, :475)
+    hipLaunchKernelGGL(k_me_confidence, dim3(n), dim3(256), 0, st, h, w, tile, (const float*)P, (const float*)best,
+                       (float* const*)(base + oC), scratch);
+    LAUNCH_CHECK(ctx);
+    for (int i = 0; i < n; ++i) {
+      HIPCHK(ctx, hipMemcpyAsync(dy[i], ody + (size_t)i * nt, sizeof(float) * nt, hipMemcpyDeviceToDevice, st));
+      HIPCHK(ctx, hipMemcpyAsync(dx[i], odx + (size_t)i * nt, sizeof(float) * nt, hipMemcpyDeviceToDevice, st));
+    }
+    arm(ctx, sl, st);
+  });
+}
+
 }  // extern "C"

@@ -71,6 +71,11 @@ struct SadDesc {
   int h, w, bh, bw, sy, sx, nby, nbx;
 };
 
+struct MotionDesc {
+  const float* ref;
+  const float* cur;
+};
+
 // lockin.hip
 __global__ void k_lockin(const LockinGeom* G, const float* const* fields, float* const* theta, int8_t* const* vswitch);
 __global__ void k_burst_amp(const LockinGeom* G, const float* const* fields, float* out);
@@ -104,5 +109,21 @@ __global__ void k_refine_finalize(const DevField* F, const FieldCG* st, const in
 
 // motion.hip
 __global__ void k_sad_bm(const SadDesc* D, const int* blk_prefix, int nf);
+__global__ void k_me_blur_cols(const MotionDesc* D, int npairs, int h, int w, int r, float* tmp);
+__global__ void k_me_blur_rows(int npairs, int h, int w, int r, const float* tmp, float* out);
+__global__ void k_me_decimate(int npairs, int h, int w, int f, const float* in, float* out);
+__global__ void k_me_ssd(int npairs, int h, int w, int tile, const float* planes, const int* dys, const int* dxs,
+                         int ndy, int ndx, const float* base_dy, const float* base_dx, float* se);
+__global__ void k_me_select(int npairs, int nt, const int* dys, const int* dxs, int ndy, int ndx, const float* se,
+                            const float* base_dy, const float* base_dx, float* mdy, float* mdx, float* best,
+                            float* cost0);
+__global__ void k_me_upscale(int npairs, int th, int tw, int hh, int ww, int f, const float* cdy, const float* cdx,
+                             float* bdy, float* bdx);
+__global__ void k_me_offset(int n, const float* mdy, const float* mdx, float ody, float odx, float* tdy, float* tdx);
+__global__ void k_me_finish(int n, const float* cym, const float* cyp, const float* cxm, const float* cxp,
+                            const float* best0, const float* cost0, float* mdy, float* mdx, float* best, float* out_dy,
+                            float* out_dx);
+__global__ void k_me_confidence(int h, int w, int tile, const float* planes, const float* best, float* const* conf_out,
+                                float* scratch);
 
 }  // namespace hvdk

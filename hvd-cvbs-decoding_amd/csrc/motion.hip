@@ -80,3 +80,243 @@ __global__ void __launch_bounds__(256) k_sad_bm(const SadDesc* D, const int* blk
 }
 
 }  // namespace hvdk
+
+// ===========================================================================
+// hvd-core EstimateMotion (engine/motion.cpp:397-488) on the device, for a
+// batch of (ref, cur) plane pairs of one shape.  Every float is produced in
+// the reference's order of operations so the vectors and confidences are
+// bit-exact with it: box blur by FP64 prefix sums (:21-55), 4x decimation
+// summed in (dy, dx) order (:59-78), SSD per (tile, candidate) accumulated
+// sequentially in raster order without fused multiply-add (:171-231, the
+// partial-distortion early-out is exact and omitted), candidates selected in
+// centre-out order with a strict '<' (:233-252, :286-298).
+// ===========================================================================
+
+namespace hvdk {
+
+namespace {
+
+__device__ inline const float* mplane(const MotionDesc& d, int which) {
+  return which == 0 ? d.ref : d.cur;
+}
+
+}  // namespace
+
+// Box blur pass 1 (motion.cpp:21-36): along y for every column, FP64 prefix.
+// Thread per (pair, plane, column).
+__global__ void k_me_blur_cols(const MotionDesc* D, int npairs, int h, int w, int r, float* tmp) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= npairs * 2 * w) return;
+  const int x = idx % w, pl = (idx / w) % 2, pi = idx / (2 * w);
+  const float* a = mplane(D[pi], pl);
+  float* o = tmp + ((size_t)pi * 2 + pl) * h * w;
+  const float norm = 1.0f / (float)(2 * r + 1);
+  // prefix sums in FP64 over a sliding window of the column
+  double c_lo = 0.0, c_hi = 0.0;  // c[y0], c[y1]
+  int i_lo = 0, i_hi = 0;
+  for (int y = 0; y < h; ++y) {
+    const int y0 = max(y - r, 0), y1 = min(y + r + 1, h);
+    while (i_hi < y1) { c_hi += (double)a[(size_t)i_hi * w + x]; ++i_hi; }
+    while (i_lo < y0) { c_lo += (double)a[(size_t)i_lo * w + x]; ++i_lo; }
+    o[(size_t)y * w + x] = (float)((c_hi - c_lo) * (double)norm);
+  }
+}
+
+// Box blur pass 2 (motion.cpp:38-55): along x for every row.
+__global__ void k_me_blur_rows(int npairs, int h, int w, int r, const float* tmp, float* out) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= npairs * 2 * h) return;
+  const int y = idx % h, pp = idx / h;  // pp = pair * 2 + plane
+  const float* a = tmp + (size_t)pp * h * w + (size_t)y * w;
+  float* o = out + (size_t)pp * h * w + (size_t)y * w;
+  const float norm = 1.0f / (float)(2 * r + 1);
+  double c_lo = 0.0, c_hi = 0.0;
+  int i_lo = 0, i_hi = 0;
+  for (int x = 0; x < w; ++x) {
+    const int x0 = max(x - r, 0), x1 = min(x + r + 1, w);
+    while (i_hi < x1) { c_hi += (double)a[i_hi]; ++i_hi; }
+    while (i_lo < x0) { c_lo += (double)a[i_lo]; ++i_lo; }
+    o[x] = (float)((c_hi - c_lo) * (double)norm);
+  }
+}
+
+// f x f block average, remainder cropped (motion.cpp:59-78).
+__global__ void k_me_decimate(int npairs, int h, int w, int f, const float* in, float* out) {
+  const int hh = h / f, ww = w / f;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= npairs * 2 * hh * ww) return;
+  const int x = idx % ww, y = (idx / ww) % hh, pp = idx / (hh * ww);
+  const float* a = in + (size_t)pp * h * w;
+  const float norm = 1.0f / (float)(f * f);
+  float acc = 0.0f;
+  for (int dy = 0; dy < f; ++dy)
+    for (int dx = 0; dx < f; ++dx) acc = __fadd_rn(acc, a[(size_t)(y * f + dy) * w + x * f + dx]);
+  out[(size_t)pp * hh * ww + (size_t)y * ww + x] = __fmul_rn(acc, norm);
+}
+
+// SSD of one candidate for one tile (motion.cpp:171-231 without the early
+// out).  planes: [pair][2][h][w] (A = cur = plane 1, B = ref = plane 0).
+// base_dy/base_dx (per tile, may be null) make the candidate relative.
+// se: [pair][tile][cand].  Thread per (pair, tile, cand).
+__global__ void k_me_ssd(int npairs, int h, int w, int tile, const float* planes, const int* dys, const int* dxs,
+                         int ndy, int ndx, const float* base_dy, const float* base_dx, float* se) {
+  const int th = (h + tile - 1) / tile, tw = (w + tile - 1) / tile;
+  const int nc = ndy * ndx, nt = th * tw;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= npairs * nt * nc) return;
+  const int c = idx % nc, t = (idx / nc) % nt, pi = idx / (nc * nt);
+  const int ty = t / tw, tx = t % tw;
+  int sy = dys[c / ndx], sx = dxs[c % ndx];
+  if (base_dy) {
+    sy += (int)lroundf(base_dy[(size_t)pi * nt + t]);
+    sx += (int)lroundf(base_dx[(size_t)pi * nt + t]);
+  }
+  const float* B = planes + ((size_t)pi * 2 + 0) * h * w;
+  const float* A = planes + ((size_t)pi * 2 + 1) * h * w;
+  const int y_end = min((ty + 1) * tile, h), x_end = min((tx + 1) * tile, w);
+  float acc = 0.0f;
+  for (int y = ty * tile; y < y_end; ++y) {
+    const int yy = min(max(y - sy, 0), h - 1);
+    for (int x = tx * tile; x < x_end; ++x) {
+      const int xx = min(max(x - sx, 0), w - 1);
+      const float diff = __fsub_rn(A[(size_t)y * w + x], B[(size_t)yy * w + xx]);
+      acc = __fadd_rn(acc, __fmul_rn(diff, diff));
+    }
+  }
+  se[idx] = acc;
+}
+
+// In-order candidate selection (motion.cpp:233-252): cost = se * bias,
+// strict '<'.  Thread per (pair, tile).  cost0 (may be null) receives the
+// SSD of the (0, 0) candidate.
+__global__ void k_me_select(int npairs, int nt, const int* dys, const int* dxs, int ndy, int ndx, const float* se,
+                            const float* base_dy, const float* base_dx, float* mdy, float* mdx, float* best,
+                            float* cost0) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= npairs * nt) return;
+  const int nc = ndy * ndx;
+  float b = __int_as_float(0x7f800000);  // +inf
+  float vy = 0.0f, vx = 0.0f;
+  for (int c = 0; c < nc; ++c) {
+    const int dy = dys[c / ndx], dx = dxs[c % ndx];
+    const float bias = __fadd_rn(1.0f, __fmul_rn(0.02f, (float)(abs(dy) + abs(dx))));
+    const float s = se[(size_t)idx * nc + c];
+    if (cost0 && dy == 0 && dx == 0) cost0[idx] = s;
+    const float cost = __fmul_rn(s, bias);
+    if (cost < b) {
+      b = cost;
+      if (base_dy) {
+        vy = __fadd_rn(base_dy[idx], (float)dy);
+        vx = __fadd_rn(base_dx[idx], (float)dx);
+      } else {
+        vy = (float)dy;
+        vx = (float)dx;
+      }
+    }
+  }
+  mdy[idx] = vy;
+  mdx[idx] = vx;
+  best[idx] = b;
+}
+
+// Coarse tile grid -> full-res tile grid, nearest, x 4 (motion.cpp:416-428).
+__global__ void k_me_upscale(int npairs, int th, int tw, int hh, int ww, int f, const float* cdy, const float* cdx,
+                             float* bdy, float* bdx) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= npairs * th * tw) return;
+  const int t = idx % (th * tw), pi = idx / (th * tw);
+  const int ty = t / tw, tx = t % tw;
+  const int gy = min((ty * hh) / th, hh - 1), gx = min((tx * ww) / tw, ww - 1);
+  bdy[idx] = __fmul_rn(cdy[(size_t)pi * hh * ww + gy * ww + gx], (float)f);
+  bdx[idx] = __fmul_rn(cdx[(size_t)pi * hh * ww + gy * ww + gx], (float)f);
+}
+
+// base +/- 1 in y or x for the four parabolic-fit costs (motion.cpp:309-315).
+__global__ void k_me_offset(int n, const float* mdy, const float* mdx, float ody, float odx, float* tdy, float* tdx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  tdy[i] = __fadd_rn(mdy[i], ody);
+  tdx[i] = __fadd_rn(mdx[i], odx);
+}
+
+// Parabolic half-pel fit, prefer-zero margin rule, vector sum
+// (motion.cpp:441-485 without the confidence).  Thread per (pair, tile).
+__global__ void k_me_finish(int n, const float* cym, const float* cyp, const float* cxm, const float* cxp,
+                            const float* best0, const float* cost0, float* mdy, float* mdx, float* best, float* out_dy,
+                            float* out_dx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  auto parabolic = [](float cm, float c0, float cp) {
+    const float den = __fadd_rn(__fsub_rn(cm, __fmul_rn(2.0f, c0)), cp);
+    float sub = 0.0f;
+    if (den > 1e-9f) sub = __fdiv_rn(__fmul_rn(0.5f, __fsub_rn(cm, cp)), fmaxf(den, 1e-9f));
+    return fminf(fmaxf(sub, -0.5f), 0.5f);
+  };
+  float sdy = parabolic(cym[i], best[i], cyp[i]);
+  float sdx = parabolic(cxm[i], best[i], cxp[i]);
+  float vy = mdy[i], vx = mdx[i], b = best[i];
+  if (best0[i] < b) { vy = 0.0f; vx = 0.0f; b = best0[i]; }
+  if (b > __fmul_rn(0.85f, cost0[i])) { vy = 0.0f; vx = 0.0f; b = cost0[i]; sdy = 0.0f; sdx = 0.0f; }
+  best[i] = b;
+  out_dy[i] = __fadd_rn(vy, sdy);
+  out_dx[i] = __fadd_rn(vx, sdx);
+}
+
+// MotionConfidence (motion.cpp:321-387).  One block per pair; per-tile sums
+// in the reference's raster order, the numpy median of best / npix, then the
+// outlier and global-validity scalings.  scratch: 2 * nt floats per pair.
+__global__ void __launch_bounds__(256) k_me_confidence(int h, int w, int tile, const float* planes, const float* best,
+                                                       float* const* conf_out, float* scratch) {
+  const int pi = blockIdx.x;
+  const int th = (h + tile - 1) / tile, tw = (w + tile - 1) / tile, nt = th * tw;
+  const float* A = planes + ((size_t)pi * 2 + 1) * h * w;
+  const float* bst = best + (size_t)pi * nt;
+  float* ratio = scratch + (size_t)pi * nt * 2;
+  float* conf = ratio + nt;
+  __shared__ float s_lo, s_hi;
+  for (int t = threadIdx.x; t < nt; t += blockDim.x) {
+    const int ty = t / tw, tx = t % tw;
+    const int y0 = ty * tile, y1 = min((ty + 1) * tile, h);
+    const int x0 = tx * tile, x1 = min((tx + 1) * tile, w);
+    float sum = 0.0f, sum2 = 0.0f, np = 0.0f;
+    for (int y = y0; y < y1; ++y)
+      for (int x = x0; x < x1; ++x) {
+        const float v = A[(size_t)y * w + x];
+        sum = __fadd_rn(sum, v);
+        sum2 = __fadd_rn(sum2, __fmul_rn(v, v));
+        np = __fadd_rn(np, 1.0f);
+      }
+    const float npc = fmaxf(1.0f, np);
+    const float norm = __fdiv_rn(1.0f, npc);
+    const float mean = __fmul_rn(sum, norm);
+    const float var = __fsub_rn(__fmul_rn(sum2, norm), __fmul_rn(mean, mean));
+    const float energy = __fmul_rn(__fadd_rn(var, 1.0f), npc);
+    const float c = __fsub_rn(1.0f, __fdiv_rn(bst[t], energy));
+    conf[t] = fminf(fmaxf(c, 0.0f), 1.0f);
+    ratio[t] = __fdiv_rn(bst[t], npc);
+  }
+  __syncthreads();
+  // numpy median: the element of rank nt/2, averaged with rank nt/2 - 1 when nt is even
+  for (int t = threadIdx.x; t < nt; t += blockDim.x) {
+    const float vi = ratio[t];
+    int rank = 0;
+    for (int j = 0; j < nt; ++j) {
+      const float vj = ratio[j];
+      rank += (vj < vi) || (vj == vi && j < t);
+    }
+    if (rank == nt / 2) s_hi = vi;
+    if (nt % 2 == 0 && rank == nt / 2 - 1) s_lo = vi;
+  }
+  __syncthreads();
+  const float med = nt % 2 == 0 ? __fmul_rn(0.5f, __fadd_rn(s_hi, s_lo)) : s_hi;
+  float* out = conf_out[pi];
+  for (int t = threadIdx.x; t < nt; t += blockDim.x) {
+    const float excess = fmaxf(0.0f, __fsub_rn(ratio[t], med));
+    const float m1 = __fadd_rn(med, 1.0f);
+    float c = __fmul_rn(conf[t], __fdiv_rn(m1, __fadd_rn(m1, excess)));
+    c = __fmul_rn(c, __fdiv_rn(6.0f, __fadd_rn(6.0f, med)));
+    out[t] = c;
+  }
+}
+
+}  // namespace hvdk

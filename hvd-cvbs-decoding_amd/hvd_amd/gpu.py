@@ -21,7 +21,7 @@ ABI_SYMBOLS = (
     "hvd_gpu_abi_version", "hvd_gpu_sample_to_ire_u16", "hvd_gpu_burst_lockin_phase",
     "hvd_gpu_burst_amplitude_ire", "hvd_gpu_holographic_init", "hvd_gpu_variational_refine",
     "hvd_gpu_decode_fields_u16", "hvd_gpu_decode_fields_ire", "hvd_gpu_block_match_sad_u16",
-    "hvd_gpu_set_profiling", "hvd_gpu_cg_timing",
+    "hvd_gpu_set_profiling", "hvd_gpu_cg_timing", "hvd_gpu_estimate_motion",
 )
 
 
@@ -100,6 +100,7 @@ def load() -> C.CDLL:
         lib.hvd_gpu_decode_fields_ire.argtypes = [vp, pp, i, C.POINTER(Geometry), C.POINTER(SolverParams),
                                                   C.c_float, pp, pp, pp, vp]
         lib.hvd_gpu_block_match_sad_u16.argtypes = [vp, pp, pp, i, i, i, i, i, i, i, pp, pp, pp, vp]
+        lib.hvd_gpu_estimate_motion.argtypes = [vp, pp, pp, i, i, i, i, i, pp, pp, pp, vp]
         lib.hvd_gpu_set_profiling.argtypes = [vp, i]
         lib.hvd_gpu_cg_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                                           C.POINTER(C.c_double)]
@@ -260,6 +261,20 @@ class Engine:
         return ys, chis, ths
 
     # ---- motion ---------------------------------------------------------------
+    def estimate_motion(self, refs, curs, tile=32, search=8):
+        """hvd-core EstimateMotion for pairs of (h, w) float32 planes: returns
+        (dy list, dx list, confidence list), each ceil(h/tile) x ceil(w/tile)."""
+        import torch
+        n = len(refs)
+        h, w = refs[0].shape
+        th, tw = -(-h // tile), -(-w // tile)
+        dev = refs[0].device
+        mk = lambda: [torch.empty((th, tw), dtype=torch.float32, device=dev) for _ in range(n)]  # noqa: E731
+        dy, dx, cf = mk(), mk(), mk()
+        self._check(self.lib.hvd_gpu_estimate_motion(self.ctx, _ptrs(refs), _ptrs(curs), n, h, w, tile, search,
+                                                     _ptrs(dy), _ptrs(dx), _ptrs(cf), _stream()), "estimate_motion")
+        return dy, dx, cf
+
     def block_match_sad_u16(self, refs, curs, bh=8, bw=16, sy=4, sx=16):
         import torch
         n = len(refs)

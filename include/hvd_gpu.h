@@ -177,6 +177,17 @@ int hvd_gpu_decode_fields_ire(hvd_gpu_ctx* ctx, const float* const* field_ire, i
 
 /* ---- motion ------------------------------------------------------------ */
 
+/* EstimateMotion (engine/motion.cpp:397-488) for n (ref, cur) pairs of
+ * h x w float planes: box blur r = 2, 4x-decimated exhaustive coarse search
+ * over +/-ceil(search / 4), +/-3 full-resolution refinement, parabolic
+ * half-pel fit, prefer-zero margin rule, per-tile confidence.  Outputs are
+ * ceil(h / tile) x ceil(w / tile) per pair.  Requires h, w >= 4.  The
+ * vectors and confidences are produced in hvd-core's order of operations.
+ * Pointer arrays are HOST arrays. */
+int hvd_gpu_estimate_motion(hvd_gpu_ctx* ctx, const float* const* y_ref, const float* const* y_cur, int n, int h,
+                            int w, int tile, int search, float* const* dy, float* const* dx, float* const* conf,
+                            void* stream);
+
 /* Integer-SAD block matching over uint16 planes (no hvd-core counterpart:
  * the integer form of BmPass, engine/motion.cpp:105-260).  For each of n
  * (ref, cur) pairs of h x w planes: bw x bh blocks (edge blocks clipped),

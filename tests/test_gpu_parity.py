@@ -305,3 +305,63 @@ def test_sample_to_ire(engine):
     raw = np.arange(0, 65536, 7, dtype=np.uint16)
     got = engine.sample_to_ire_u16(dev(raw.view(np.int16)), 20480.0, 61440.0).cpu().numpy()
     assert np.array_equal(got, synth.ire_from_raw(raw))
+
+
+# ---------------------------------------------------------------- EstimateMotion (float SSD pyramid)
+def _motion_pairs():
+    from test_oracle_math import _motion_texture
+    ref = _motion_texture(128, 128).astype(np.float32)
+    yy, xx = np.mgrid[0:128, 0:128]
+    cur = ref[np.clip(yy - 3, 0, 127), np.clip(xx + 2, 0, 127)]
+    pairs = [(ref, cur, 32, 8)]
+    # consecutive luma planes of a moving NTSC scene at active size
+    sc = synth.make_scene(synth.NTSC, np.random.default_rng(81), canvas_pad=24)
+    Y = sc.y.astype(np.float32)
+    a = Y[24:24 + 242, 24:24 + 760]
+    b = Y[24 - 1:24 - 1 + 242, 24 + 5:24 + 5 + 760]
+    pairs.append((a, b, 32, 16))
+    r = np.random.default_rng(82)
+    pairs.append((r.normal(size=(37, 53)).astype(np.float32) * 20, r.normal(size=(37, 53)).astype(np.float32) * 20,
+                  16, 8))
+    return pairs
+
+
+@pytest.mark.parametrize("k", [0, 1, 2])
+def test_estimate_motion_bit_exact(engine, orc32, k):
+    ref_p, cur_p, tile, search = _motion_pairs()[k]
+    dy, dx, cf = engine.estimate_motion([dev(ref_p)], [dev(cur_p)], tile=tile, search=search)
+    wdy, wdx, wcf = orc32.estimate_motion(ref_p, cur_p, tile, search)
+    assert np.array_equal(dy[0].cpu().numpy(), wdy)
+    assert np.array_equal(dx[0].cpu().numpy(), wdx)
+    assert np.array_equal(cf[0].cpu().numpy(), wcf)
+
+
+def test_estimate_motion_batch(engine, orc32):
+    seq, _ = synth.motion_sequence(83, n_fields=4)
+    g = synth.NTSC
+    ys = [synth.ire_from_raw(r)[g.first_active_field_line:, g.active_video_start:g.active_video_end] for r in seq]
+    ys = [np.ascontiguousarray(y) for y in ys]
+    dy, dx, cf = engine.estimate_motion([dev(y) for y in ys[:-1]], [dev(y) for y in ys[1:]], tile=32, search=16)
+    for k in range(3):
+        wdy, wdx, wcf = orc32.estimate_motion(ys[k], ys[k + 1], 32, 16)
+        assert np.array_equal(dy[k].cpu().numpy(), wdy) and np.array_equal(dx[k].cpu().numpy(), wdx)
+        assert np.array_equal(cf[k].cpu().numpy(), wcf)
+
+
+# ---------------------------------------------------------------- error paths
+def test_abi_rejects_bad_arguments(engine):
+    g = synth.NTSC
+    raw = dev(synth.ntsc_field(1)[0].view(np.int16))
+    with pytest.raises(gpu.HvdGpuError, match="diag_prior"):
+        engine.decode_fields_u16([raw], [g], gpu.SolverParams.make(diag_prior=0.5))
+    bad = synth.Geometry(910, 700, 134, 894, 76, 112, 21, 0, g.sample_rate, 0.0, 0)
+    with pytest.raises(gpu.HvdGpuError, match="lines"):
+        engine.burst_lockin_phase([dev(np.zeros((700, 910), np.float32))], [bad])
+    bad = synth.Geometry(910, 263, 134, 2000, 76, 112, 21, 0, g.sample_rate, 0.0, 0)
+    with pytest.raises(gpu.HvdGpuError, match="active"):
+        engine.decode_fields_u16([raw], [bad], gpu.SolverParams.make())
+    with pytest.raises(gpu.HvdGpuError, match="search"):
+        engine.block_match_sad_u16([raw], [raw], sy=40)
+    # the context stays usable after a rejected call
+    ys, _, _ = engine.decode_fields_u16([raw], [g], gpu.SolverParams.make())
+    assert torch.isfinite(ys[0]).all()
