@@ -421,6 +421,8 @@ void run_refine(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_par
     LAUNCH_CHECK(ctx);
     hipLaunchKernelGGL(k_grad_init, dim3(nb), dim3(kBlock), 0, st, dF, dstate, dtp, nf, nb, partials);
     LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_reduce_init, dim3(nf), dim3(256), 0, st, dstate, dtp, (const double*)partials);
+    LAUNCH_CHECK(ctx);
     std::pair<hipEvent_t, hipEvent_t> evs{nullptr, nullptr};
     if (ctx->profiling) {
       HIPCHK(ctx, hipEventCreate(&evs.first));
@@ -430,13 +432,17 @@ void run_refine(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_par
     for (int it = 0; it < n_inner; ++it) {
       hipLaunchKernelGGL(k_cg_curv, dim3(nb), dim3(kBlock), 0, st, dF, dstate, dtp, nf, nb, partials, it);
       LAUNCH_CHECK(ctx);
+      hipLaunchKernelGGL(k_reduce_curv, dim3(nf), dim3(256), 0, st, dstate, dtp, (const double*)partials, it);
+      LAUNCH_CHECK(ctx);
       hipLaunchKernelGGL(k_cg_step, dim3(nb), dim3(kBlock), 0, st, dF, dstate, dtp, nf, nb, partials);
+      LAUNCH_CHECK(ctx);
+      hipLaunchKernelGGL(k_reduce_step, dim3(nf), dim3(256), 0, st, dstate, dtp, (const double*)partials);
       LAUNCH_CHECK(ctx);
     }
     if (ctx->profiling) {
       HIPCHK(ctx, hipEventRecord(evs.second, st));
       ctx->cg_events.push_back(evs);
-      ctx->cg_launches.push_back(2LL * n_inner);
+      ctx->cg_launches.push_back(4LL * n_inner);
     }
     hipLaunchKernelGGL(k_stage_end, dim3((nf + 63) / 64), dim3(64), 0, st, dstate, nf, o);
     LAUNCH_CHECK(ctx);
@@ -817,6 +823,182 @@ int hvd_gpu_variational_refine(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, in
 
 namespace {
 
+// Workspace layout of the field driver, carved after `cv`; `end` covers the
+// solver's carve too, so one ensure_ws(end) sizes the whole call.
+struct DecodeLayout {
+  std::vector<size_t> ire, th, vs, s, row, vsa, col, spec, var;
+  Carve solver;  // where refine_planes starts carving
+  size_t end = 0;
+};
+
+DecodeLayout decode_layout(int n, const hvd_field_geometry* geom, bool from_raw, Carve cv) {
+  DecodeLayout L;
+  for (auto* v : {&L.ire, &L.th, &L.vs, &L.s, &L.row, &L.vsa, &L.col, &L.spec, &L.var}) v->resize(n);
+  std::vector<DevField> shapes(n);
+  bool any_pal = false;
+  for (int i = 0; i < n; ++i) {
+    const hvd_field_geometry& g = geom[i];
+    const int lines = last_line(g) - g.first_active_field_line, width = g.active_video_end - g.active_video_start;
+    const int pad = lines > 1 ? std::min(16, lines - 1) : 0;
+    const size_t np = (size_t)lines * width;
+    L.ire[i] = from_raw ? cv.take(sizeof(float) * (size_t)g.field_width * g.field_height) : 0;
+    L.th[i] = cv.take(sizeof(float) * g.field_height);
+    L.vs[i] = cv.take(g.field_height);
+    L.s[i] = cv.take(sizeof(float) * np);
+    L.row[i] = cv.take(sizeof(float2) * lines);
+    L.vsa[i] = cv.take(lines);
+    L.col[i] = cv.take(sizeof(float2) * width);
+    L.spec[i] = cv.take(sizeof(float2) * (size_t)(lines + 2 * pad) * width);
+    L.var[i] = cv.take(sizeof(float2) * np * 2);
+    shapes[i].h = lines;
+    shapes[i].w = width;
+    any_pal = any_pal || uses_vswitch(g);
+  }
+  L.solver = cv;
+  // the AUTO-aniso sample count is largest with the NTSC 2-line grouping
+  L.end = refine_ws_end(cv, shapes, 0);
+  (void)any_pal;
+  return L;
+}
+
+void check_fields(hvd_gpu_ctx* ctx, int n, const hvd_field_geometry* geom) {
+  for (int i = 0; i < n; ++i) {
+    const hvd_field_geometry& g = geom[i];
+    check_geom(ctx, g, i);
+    check_active(ctx, g, i);
+    const int lines = last_line(g) - g.first_active_field_line, width = g.active_video_end - g.active_video_start;
+    REQUIRE(ctx, width <= 2048 && lines + 32 <= 1024, "field %d: active area exceeds the FFT limits", i);
+  }
+}
+
+// The field driver proper.  The workspace must already hold L.end bytes.
+// dec_out (optional) receives, per field, the pointers the frame weave reads.
+void decode_fields_run(hvd_gpu_ctx* ctx, const uint16_t* const* raw, const float* const* field_ire, int n,
+                       const hvd_field_geometry* geom, float blanking, float white, const hvd_solver_params* p,
+                       float chroma_phase_deg, float* const* y_out, float* const* chi_out, float* const* theta_out,
+                       hipStream_t st, const DecodeLayout& L, std::vector<BridgeField>* dec_out) {
+  char* ws = ctx->ws;
+  REQUIRE(ctx, ctx->ws_cap >= L.end, "internal: workspace not sized");
+  // (1) codes -> IRE
+  std::vector<const float*> fields(n);
+  for (int i = 0; i < n; ++i) {
+    if (raw) {
+      const hvd_field_geometry& g = geom[i];
+      const size_t cnt = (size_t)g.field_width * g.field_height;
+      float* dst = (float*)(ws + L.ire[i]);
+      hipLaunchKernelGGL(k_u16_to_ire, dim3((unsigned)std::min<size_t>((cnt + 255) / 256, 65536)), dim3(256), 0, st,
+                         raw[i], cnt, blanking, (white - blanking) / 100.0f, dst);
+      LAUNCH_CHECK(ctx);
+      fields[i] = dst;
+    } else {
+      fields[i] = field_ire[i];
+    }
+  }
+  // (2) lock-in
+  std::vector<LockinGeom> G(n);
+  std::vector<float*> T(n);
+  std::vector<int8_t*> V(n);
+  for (int i = 0; i < n; ++i) {
+    G[i] = lockin_geom(geom[i]);
+    T[i] = (float*)(ws + L.th[i]);
+    V[i] = (int8_t*)(ws + L.vs[i]);
+  }
+  run_lockin(ctx, G, fields, T, V, st);
+  // (3) active crop + carrier tables (PrepareFieldObs + ApplyChromaPhase)
+  std::vector<PrepDesc> prep(n);
+  for (int i = 0; i < n; ++i) {
+    const hvd_field_geometry& g = geom[i];
+    PrepDesc d{};
+    d.field = fields[i];
+    d.theta_field = T[i];
+    d.vswitch_field = uses_vswitch(g) ? V[i] : nullptr;
+    d.s = (float*)(ws + L.s[i]);
+    d.rowph = (float2*)(ws + L.row[i]);
+    d.vsw_active = uses_vswitch(g) ? (int8_t*)(ws + L.vsa[i]) : nullptr;
+    d.colph = (float2*)(ws + L.col[i]);
+    d.fw = g.field_width;
+    d.fal = g.first_active_field_line;
+    d.a0 = g.active_video_start;
+    d.lines = last_line(g) - g.first_active_field_line;
+    d.width = g.active_video_end - g.active_video_start;
+    d.std_phase = g.subcarrier_hz <= 0.0;
+    d.signed_by_parity = uses_vswitch(g);
+    d.phase_offset = chroma_phase_deg == 0.0f ? 0.0f : -chroma_phase_deg * 3.14159265358979323846f / 180.0f;
+    d.w0 = phase_per_sample(g);
+    prep[i] = d;
+  }
+  run_prepare(ctx, prep, true, st);
+  if (theta_out)
+    for (int i = 0; i < n; ++i)
+      if (theta_out[i])
+        HIPCHK(ctx, hipMemcpyAsync(theta_out[i], T[i], sizeof(float) * geom[i].field_height, hipMemcpyDeviceToDevice,
+                                   st));
+  // (4) holographic init into the caller's chi / y
+  PlanSet ps;
+  std::vector<InitField> IF(n);
+  for (int i = 0; i < n; ++i) {
+    InitField f{};
+    f.h = prep[i].lines;
+    f.w = prep[i].width;
+    f.pad = f.h > 1 ? std::min(16, f.h - 1) : 0;
+    f.hp = f.h + 2 * f.pad;
+    f.s = prep[i].s;
+    f.rowph = prep[i].rowph;
+    f.vsw = prep[i].vsw_active;
+    f.colph = prep[i].colph;
+    f.spec = (float2*)(ws + L.spec[i]);
+    f.var[0] = (float2*)(ws + L.var[i]);
+    f.var[1] = f.var[0] + (size_t)f.h * f.w;
+    f.chi0 = reinterpret_cast<float2*>(chi_out[i]);
+    f.y0 = y_out[i];
+    f.row_plan = ps.get(ctx, f.w);
+    f.col_plan = ps.get(ctx, f.hp);
+    f.nvar = 2;
+    f.sample_rate = (float)geom[i].sample_rate;
+    IF[i] = f;
+  }
+  run_init(ctx, IF, ps, st);
+  // (5) solve (sequence.cpp:1046: always, with is_pal from the geometry)
+  std::vector<DevField> F(n);
+  bool any_pal = false, any_ntsc = false;
+  for (int i = 0; i < n; ++i) {
+    DevField f{};
+    f.h = IF[i].h;
+    f.w = IF[i].w;
+    f.s = IF[i].s;
+    f.rowph = IF[i].rowph;
+    f.vsw = IF[i].vsw;
+    f.colph = IF[i].colph;
+    f.chi[0] = IF[i].chi0;
+    f.y = y_out[i];
+    F[i] = f;
+    (uses_vswitch(geom[i]) ? any_pal : any_ntsc) = true;
+  }
+  hvd_solver_params pp = *p;
+  if (any_pal && any_ntsc) {
+    // Mixed batch: the AUTO aniso line grouping differs per standard, so
+    // solve each standard as its own batch on the same stream.
+    std::vector<DevField> Fp, Fn;
+    for (int i = 0; i < n; ++i) (uses_vswitch(geom[i]) ? Fp : Fn).push_back(F[i]);
+    Carve c1 = L.solver;
+    pp.is_pal = 1;
+    refine_planes(ctx, Fp, pp, st, c1, RefineOut{});
+    Carve c2 = L.solver;
+    pp.is_pal = 0;
+    refine_planes(ctx, Fn, pp, st, c2, RefineOut{});
+  } else {
+    Carve c = L.solver;
+    pp.is_pal = any_pal ? 1 : 0;
+    refine_planes(ctx, F, pp, st, c, RefineOut{});
+  }
+  if (dec_out) {
+    dec_out->resize(n);
+    for (int i = 0; i < n; ++i)
+      (*dec_out)[i] = BridgeField{IF[i].h, IF[i].w, y_out[i], reinterpret_cast<const float2*>(chi_out[i]),
+                                  IF[i].rowph, IF[i].colph, IF[i].vsw};
+  }
+}
+
 int decode_fields(hvd_gpu_ctx* ctx, const uint16_t* const* raw, const float* const* field_ire, int n,
                   const hvd_field_geometry* geom, float blanking, float white, const hvd_solver_params* p,
                   float chroma_phase_deg, float* const* y_out, float* const* chi_out, float* const* theta_out,
@@ -826,166 +1008,147 @@ int decode_fields(hvd_gpu_ctx* ctx, const uint16_t* const* raw, const float* con
     check_params(ctx, p);
     if (raw) REQUIRE(ctx, white != blanking, "white level equals blanking");
     if (n == 0) return;
-    hipStream_t st = S(stream);
-    Carve cv;
-    std::vector<size_t> o_ire(n), o_th(n), o_vs(n), o_s(n), o_row(n), o_vsa(n), o_col(n), o_spec(n), o_var(n);
-    for (int i = 0; i < n; ++i) {
-      const hvd_field_geometry& g = geom[i];
-      check_geom(ctx, g, i);
-      check_active(ctx, g, i);
+    check_fields(ctx, n, geom);
+    for (int i = 0; i < n; ++i)
       REQUIRE(ctx, (raw ? (const void*)raw[i] : (const void*)field_ire[i]) && y_out[i] && chi_out[i],
               "field %d: null pointer", i);
-      const int lines = last_line(g) - g.first_active_field_line, width = g.active_video_end - g.active_video_start;
-      REQUIRE(ctx, width <= 2048 && lines + 32 <= 1024, "field %d: active area exceeds the FFT limits", i);
-      const int pad = lines > 1 ? std::min(16, lines - 1) : 0;
-      const size_t np = (size_t)lines * width;
-      o_ire[i] = raw ? cv.take(sizeof(float) * (size_t)g.field_width * g.field_height) : 0;
-      o_th[i] = cv.take(sizeof(float) * g.field_height);
-      o_vs[i] = cv.take(g.field_height);
-      o_s[i] = cv.take(sizeof(float) * np);
-      o_row[i] = cv.take(sizeof(float2) * lines);
-      o_vsa[i] = cv.take(lines);
-      o_col[i] = cv.take(sizeof(float2) * width);
-      o_spec[i] = cv.take(sizeof(float2) * (size_t)(lines + 2 * pad) * width);
-      o_var[i] = cv.take(sizeof(float2) * np * 2);
-    }
-    // size for the solver's part too, then carve it in refine_planes
-    {
-      Carve full = cv;
-      size_t tiles = 0, samples = 0;
-      for (int i = 0; i < n; ++i) {
-        const hvd_field_geometry& g = geom[i];
-        const int lines = last_line(g) - g.first_active_field_line, width = g.active_video_end - g.active_video_start;
-        const size_t np = (size_t)lines * width;
-        full.take(sizeof(float2) * np);
-        full.take(sizeof(float2) * np);
-        full.take(sizeof(float2) * np * 2);
-        full.take(sizeof(float4) * np);
-        tiles += (size_t)((width + kTileW - 1) / kTileW) * ((lines + kTileH - 1) / kTileH);
-        samples += aniso_samples(lines, width, uses_vswitch(g));
-      }
-      full.take(sizeof(double) * 2 * tiles);
-      full.take(sizeof(float) * (samples + 1));
-      full.take(sizeof(float) * (samples + 1));
-      full.take(sizeof(FieldCG) * n);
-      ensure_ws(ctx, full.off, st);
-    }
-    char* ws = ctx->ws;
-    // (1) codes -> IRE
-    std::vector<const float*> fields(n);
-    for (int i = 0; i < n; ++i) {
-      if (raw) {
-        const hvd_field_geometry& g = geom[i];
-        const size_t cnt = (size_t)g.field_width * g.field_height;
-        float* dst = (float*)(ws + o_ire[i]);
-        hipLaunchKernelGGL(k_u16_to_ire, dim3((unsigned)std::min<size_t>((cnt + 255) / 256, 65536)), dim3(256), 0, st,
-                           raw[i], cnt, blanking, (white - blanking) / 100.0f, dst);
-        LAUNCH_CHECK(ctx);
-        fields[i] = dst;
-      } else {
-        fields[i] = field_ire[i];
-      }
-    }
-    // (2) lock-in
-    std::vector<LockinGeom> G(n);
-    std::vector<float*> T(n);
-    std::vector<int8_t*> V(n);
-    for (int i = 0; i < n; ++i) {
-      G[i] = lockin_geom(geom[i]);
-      T[i] = (float*)(ws + o_th[i]);
-      V[i] = (int8_t*)(ws + o_vs[i]);
-    }
-    run_lockin(ctx, G, fields, T, V, st);
-    // (3) active crop + carrier tables (PrepareFieldObs + ApplyChromaPhase)
-    std::vector<PrepDesc> prep(n);
-    for (int i = 0; i < n; ++i) {
-      const hvd_field_geometry& g = geom[i];
-      PrepDesc d{};
-      d.field = fields[i];
-      d.theta_field = T[i];
-      d.vswitch_field = uses_vswitch(g) ? V[i] : nullptr;
-      d.s = (float*)(ws + o_s[i]);
-      d.rowph = (float2*)(ws + o_row[i]);
-      d.vsw_active = uses_vswitch(g) ? (int8_t*)(ws + o_vsa[i]) : nullptr;
-      d.colph = (float2*)(ws + o_col[i]);
-      d.fw = g.field_width;
-      d.fal = g.first_active_field_line;
-      d.a0 = g.active_video_start;
-      d.lines = last_line(g) - g.first_active_field_line;
-      d.width = g.active_video_end - g.active_video_start;
-      d.std_phase = g.subcarrier_hz <= 0.0;
-      d.signed_by_parity = uses_vswitch(g);
-      d.phase_offset = chroma_phase_deg == 0.0f ? 0.0f : -chroma_phase_deg * 3.14159265358979323846f / 180.0f;
-      d.w0 = phase_per_sample(g);
-      prep[i] = d;
-    }
-    run_prepare(ctx, prep, true, st);
-    if (theta_out)
-      for (int i = 0; i < n; ++i)
-        if (theta_out[i])
-          HIPCHK(ctx, hipMemcpyAsync(theta_out[i], T[i], sizeof(float) * geom[i].field_height,
-                                     hipMemcpyDeviceToDevice, st));
-    // (4) holographic init into the caller's chi / y
-    PlanSet ps;
-    std::vector<InitField> IF(n);
-    for (int i = 0; i < n; ++i) {
-      InitField f{};
-      f.h = prep[i].lines;
-      f.w = prep[i].width;
-      f.pad = f.h > 1 ? std::min(16, f.h - 1) : 0;
-      f.hp = f.h + 2 * f.pad;
-      f.s = prep[i].s;
-      f.rowph = prep[i].rowph;
-      f.vsw = prep[i].vsw_active;
-      f.colph = prep[i].colph;
-      f.spec = (float2*)(ws + o_spec[i]);
-      f.var[0] = (float2*)(ws + o_var[i]);
-      f.var[1] = f.var[0] + (size_t)f.h * f.w;
-      f.chi0 = reinterpret_cast<float2*>(chi_out[i]);
-      f.y0 = y_out[i];
-      f.row_plan = ps.get(ctx, f.w);
-      f.col_plan = ps.get(ctx, f.hp);
-      f.nvar = 2;
-      f.sample_rate = (float)geom[i].sample_rate;
-      IF[i] = f;
-    }
-    run_init(ctx, IF, ps, st);
-    // (5) solve (sequence.cpp:1046: always, with is_pal from the geometry)
-    std::vector<DevField> F(n);
-    bool any_pal = false, any_ntsc = false;
-    for (int i = 0; i < n; ++i) {
-      DevField f{};
-      f.h = IF[i].h;
-      f.w = IF[i].w;
-      f.s = IF[i].s;
-      f.rowph = IF[i].rowph;
-      f.vsw = IF[i].vsw;
-      f.colph = IF[i].colph;
-      f.chi[0] = IF[i].chi0;
-      f.y = y_out[i];
-      F[i] = f;
-      (uses_vswitch(geom[i]) ? any_pal : any_ntsc) = true;
-    }
-    hvd_solver_params pp = *p;
-    if (any_pal && any_ntsc) {
-      // Mixed batch: the AUTO aniso line grouping differs per standard, so
-      // solve each standard as its own batch on the same stream.
-      std::vector<DevField> Fp, Fn;
-      for (int i = 0; i < n; ++i) (uses_vswitch(geom[i]) ? Fp : Fn).push_back(F[i]);
-      Carve c1 = cv;
-      pp.is_pal = 1;
-      refine_planes(ctx, Fp, pp, st, c1, RefineOut{});
-      Carve c2 = cv;
-      pp.is_pal = 0;
-      refine_planes(ctx, Fn, pp, st, c2, RefineOut{});
-    } else {
-      pp.is_pal = any_pal ? 1 : 0;
-      refine_planes(ctx, F, pp, st, cv, RefineOut{});
-    }
+    hipStream_t st = S(stream);
+    const DecodeLayout L = decode_layout(n, geom, raw != nullptr, Carve{});
+    ensure_ws(ctx, L.end, st);
+    decode_fields_run(ctx, raw, field_ire, n, geom, blanking, white, p, chroma_phase_deg, y_out, chi_out, theta_out,
+                      st, L, nullptr);
   });
 }
 
 }  // namespace
+
+int hvd_gpu_decode_frame_window_s16(hvd_gpu_ctx* ctx, const int16_t* const* frames, int nframes, int core_begin,
+                                    int core_end, const hvd_frame_params* fp, const hvd_solver_params* p,
+                                    const hvd_bridge_params* b, const hvd_yc_frame* out, float* acc_gain,
+                                    void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, fp && p && b && frames && out, "null argument");
+    REQUIRE(ctx, nframes >= 1 && core_begin >= 0 && core_end <= nframes && core_begin < core_end,
+            "bad window: %d frames, core [%d, %d)", nframes, core_begin, core_end);
+    check_params(ctx, p);
+    const int fw = fp->frame_width, fh = fp->frame_height, f1 = fp->field1_lines;
+    REQUIRE(ctx, fw > 0 && fh > 0 && f1 > 0 && f1 <= fh && fh - f1 <= f1, "bad frame geometry %d x %d (field 1: %d)",
+            fw, fh, f1);
+    REQUIRE(ctx, fp->extra_sample_line_count >= 0 && fp->extra_sample_line_count <= 8, "bad extra-sample table");
+    REQUIRE(ctx, fp->white_level != fp->blanking_level, "white level equals blanking");
+    const int amb_stride = (fp->standard == 1 || fp->standard == 2) ? 4 : 2;
+    const bool three_d = p->cg_iterations > 0 &&
+                         (b->temporal_strength > 0.0f || (b->temporal_strength == 0.0f && 2 * nframes > amb_stride));
+    if (three_d) {
+      set_err(ctx, "3-D decoding (temporal strength %g) is not built; pass a negative strength for 2-D",
+              (double)b->temporal_strength);
+      throw Fail{HVD_GPU_ERR_UNSUPPORTED};
+    }
+    // FieldGeometryFromParams (frame_bridge.cpp:43-58)
+    hvd_field_geometry g{};
+    g.field_width = fw;
+    g.field_height = f1;
+    g.active_video_start = fp->active_video_start;
+    g.active_video_end = fp->active_video_end;
+    g.colour_burst_start = fp->colour_burst_start;
+    g.colour_burst_end = fp->colour_burst_end;
+    g.first_active_field_line = fp->first_active_frame_line / 2;
+    g.last_active_field_line = fp->last_active_frame_line != 0 ? fp->last_active_frame_line / 2 : 0;
+    g.sample_rate = fp->sample_rate;
+    g.subcarrier_hz = fp->subcarrier_hz;
+    g.standard = fp->standard;
+    check_geom(ctx, g, 0);
+    check_active(ctx, g, 0);
+    const int nf = 2 * nframes, ncore = core_end - core_begin;
+    for (int t = 0; t < nframes; ++t) REQUIRE(ctx, frames[t], "frame %d: null pointer", t);
+    for (int k = 0; k < ncore; ++k)
+      REQUIRE(ctx, out[k].luma && out[k].chroma && out[k].u_plane && out[k].v_plane, "output %d: null plane", k);
+    const int lines = last_line(g) - g.first_active_field_line, width = g.active_video_end - g.active_video_start;
+    std::vector<hvd_field_geometry> geoms(nf, g);
+    check_fields(ctx, nf, geoms.data());
+    hipStream_t st = S(stream);
+    Carve cv;
+    const size_t o_fields = cv.take(sizeof(float) * (size_t)nf * f1 * fw);
+    const size_t o_amps = cv.take(sizeof(float) * nf);
+    const size_t o_dec = cv.take(sizeof(int) + sizeof(float) * 4);
+    const size_t o_y = cv.take(sizeof(float) * (size_t)nf * lines * width);
+    const size_t o_chi = cv.take(sizeof(float2) * (size_t)nf * lines * width);
+    const DecodeLayout L = decode_layout(nf, geoms.data(), false, cv);
+    char* ws = ensure_ws(ctx, L.end, st);
+    float* fields = (float*)(ws + o_fields);
+    float* amps = (float*)(ws + o_amps);
+    int* f1_bottom = (int*)(ws + o_dec);
+    float* gain = (float*)(ws + o_dec + sizeof(int));
+    BridgeFrameParams bp{};
+    bp.standard = fp->standard;
+    bp.frame_width = fw;
+    bp.frame_height = fh;
+    bp.field1_lines = f1;
+    bp.active_video_start = fp->active_video_start;
+    bp.active_video_end = fp->active_video_end;
+    bp.colour_burst_start = fp->colour_burst_start;
+    bp.colour_burst_end = fp->colour_burst_end;
+    bp.first_active_frame_line = fp->first_active_frame_line;
+    bp.last_active_frame_line = fp->last_active_frame_line;
+    bp.black_level = fp->black_level;
+    bp.white_level = fp->white_level;
+    bp.blanking_level = fp->blanking_level;
+    bp.chroma_dc = fp->chroma_dc;
+    bp.sample_rate = fp->sample_rate;
+    bp.subcarrier_hz = fp->subcarrier_hz;
+    for (int i = 0; i < 8; ++i) bp.extra_lines[i] = fp->extra_sample_lines[i];
+    bp.extra_count = fp->extra_sample_line_count;
+    std::vector<const int16_t*> fr(frames, frames + nframes);
+    std::vector<const float*> fptr(nf);
+    for (int j = 0; j < nf; ++j) fptr[j] = fields + (size_t)j * f1 * fw;
+    std::vector<LockinGeom> LG(nf, lockin_geom(g));
+    std::vector<BridgeOut> bo(ncore);
+    for (int k = 0; k < ncore; ++k) bo[k] = BridgeOut{out[k].luma, out[k].chroma, out[k].u_plane, out[k].v_plane};
+    Stage stg;
+    const size_t oP = stg.put(&bp, 1), oF = stg.put(fr), oFP = stg.put(fptr), oG = stg.put(LG), oO = stg.put(bo);
+    Slot* sl;
+    char* base = upload(ctx, stg, st, &sl);
+    const BridgeFrameParams* dP = (const BridgeFrameParams*)(base + oP);
+    // (1) split every frame into its two IRE fields
+    hipLaunchKernelGGL(k_split_fields, dim3((fw + 255) / 256, f1, nframes), dim3(256), 0, st, dP,
+                       (const int16_t* const*)(base + oF), fields);
+    LAUNCH_CHECK(ctx);
+    // (2) burst amplitude per field, window ACC and field order
+    hipLaunchKernelGGL(k_burst_amp, dim3(nf), dim3(256), 0, st, (const LockinGeom*)(base + oG),
+                       (const float* const*)(base + oFP), amps);
+    LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_window_decisions, dim3(1), dim3(256), 0, st, dP, (const float*)fields, nframes,
+                       (const float*)amps, b->acc, b->chroma_gain, g.standard == 1 ? 21.43f : 20.0f, b->field_order,
+                       f1_bottom, gain);
+    LAUNCH_CHECK(ctx);
+    arm(ctx, sl, st);
+    // (3) the field driver on all fields of the window
+    std::vector<float*> ys(nf), chis(nf);
+    for (int j = 0; j < nf; ++j) {
+      ys[j] = (float*)(ws + o_y) + (size_t)j * lines * width;
+      chis[j] = (float*)(ws + o_chi) + (size_t)j * lines * width * 2;
+    }
+    std::vector<BridgeField> dec;
+    decode_fields_run(ctx, nullptr, fptr.data(), nf, geoms.data(), 0.0f, 100.0f, p, b->chroma_phase_deg, ys.data(),
+                      chis.data(), nullptr, st, L, &dec);
+    // (4) weave + requantise the core frames
+    Stage stg2;
+    const size_t oD = stg2.put(dec), oF2 = stg2.put(fr), oP2 = stg2.put(&bp, 1), oO2 = stg2.put(bo);
+    Slot* sl2;
+    char* base2 = upload(ctx, stg2, st, &sl2);
+    hipLaunchKernelGGL(k_weave_requant, dim3((fw + 255) / 256, fh, ncore), dim3(256), 0, st,
+                       (const BridgeFrameParams*)(base2 + oP2), (const int16_t* const*)(base2 + oF2), core_begin,
+                       (const BridgeField*)(base2 + oD), (const int*)f1_bottom, (const float*)gain, b->monochrome,
+                       (const BridgeOut*)(base2 + oO2));
+    LAUNCH_CHECK(ctx);
+    arm(ctx, sl2, st);
+    (void)oO;
+    if (acc_gain) {
+      HIPCHK(ctx, hipMemcpyAsync(acc_gain, gain, sizeof(float), hipMemcpyDeviceToHost, st));
+      HIPCHK(ctx, hipStreamSynchronize(st));
+    }
+  });
+}
 
 int hvd_gpu_decode_fields_u16(hvd_gpu_ctx* ctx, const uint16_t* const* raw, int n, const hvd_field_geometry* geom,
                               float blanking, float white, const hvd_solver_params* p, float chroma_phase_deg,

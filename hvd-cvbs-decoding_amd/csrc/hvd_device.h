@@ -34,16 +34,10 @@ struct DevField {
   float2* d[2];        // ping-pong search direction
   float4* wts;         // IRLS weights
   float* y;            // output luma
-  // holographic-init scratch
-  float2* spec;        // (h + 2 pad) x w spectrum
-  float2* var[3];      // per-variant chroma estimates (h x w)
-  int pad, hp;         // mirror padding and padded height
-  float sample_rate;
-  int n_variants;      // 2, or 3 with the symmetry variant
 };
 
-// Per-field CG state (variational.cpp:519-549), updated by the last block of
-// each reduction kernel.
+// Per-field CG state (variational.cpp:519-549), advanced by the one-block
+// reduction kernels between the stencil launches.
 struct FieldCG {
   double gg, gg0, hh, gd, npix;
   double tol2;
@@ -54,7 +48,6 @@ struct FieldCG {
   int n_inner;
   int done, it;
   int chi_cur, d_cur;
-  unsigned arrive;
   int iters[16];        // CG iterations per IRLS stage (diagnostics)
   int stage;
   int pad_;
@@ -103,28 +96,6 @@ __device__ inline double block_sum(double v, double* lds) {
     for (int i = 0; i < NT / 64; ++i) r += lds[i];
   __syncthreads();
   return r;
-}
-
-__device__ inline void store_partial(double* p, double v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline double load_partial(const double* p) {
-  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Last-block election: thread 0 of every block calls this after publishing its
-// partials; returns true (in thread 0) in exactly one block, the last to
-// arrive, which may then read every block's partials.  The counter is reset
-// by the elected block.
-__device__ inline bool arrive_last(unsigned* counter, unsigned total) {
-  __threadfence();
-  const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-  const bool last = prev + 1 == total;
-  if (last) {
-    __threadfence();
-    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  return last;
 }
 
 }  // namespace hvdk

@@ -76,6 +76,35 @@ struct MotionDesc {
   const float* cur;
 };
 
+// hvd::FrameParams (frame_bridge.h:40-101) as the device sees it.
+struct BridgeFrameParams {
+  int standard, frame_width, frame_height, field1_lines;
+  int active_video_start, active_video_end, colour_burst_start, colour_burst_end;
+  int first_active_frame_line, last_active_frame_line;
+  float black_level, white_level, blanking_level, chroma_dc;
+  double sample_rate, subcarrier_hz;
+  int extra_lines[8];
+  int extra_count;
+  int pad_;
+};
+
+// One decoded field as the weave reads it.
+struct BridgeField {
+  int lines, width;
+  const float* y;
+  const float2* chi;
+  const float2* rowph;
+  const float2* colph;
+  const int8_t* vsw;
+};
+
+struct BridgeOut {
+  int16_t* luma;
+  int16_t* chroma;
+  double* u;
+  double* v;
+};
+
 // lockin.hip
 __global__ void k_lockin(const LockinGeom* G, const float* const* fields, float* const* theta, int8_t* const* vswitch);
 __global__ void k_burst_amp(const LockinGeom* G, const float* const* fields, float* out);
@@ -104,8 +133,20 @@ __global__ void k_irls_weights(const DevField* F, FieldCG* st, const int* tp, in
 __global__ void k_grad_init(const DevField* F, FieldCG* st, const int* tp, int nf, int nb, double* partials);
 __global__ void k_cg_curv(const DevField* F, FieldCG* st, const int* tp, int nf, int nb, double* partials, int it);
 __global__ void k_cg_step(const DevField* F, FieldCG* st, const int* tp, int nf, int nb, double* partials);
+__global__ void k_reduce_init(FieldCG* st, const int* tp, const double* partials);
+__global__ void k_reduce_curv(FieldCG* st, const int* tp, const double* partials, int it);
+__global__ void k_reduce_step(FieldCG* st, const int* tp, const double* partials);
 __global__ void k_stage_end(FieldCG* st, int nf, int stage);
 __global__ void k_refine_finalize(const DevField* F, const FieldCG* st, const int* tp, int nf, int nb);
+
+// bridge.hip
+__global__ void k_split_fields(const BridgeFrameParams* FP, const int16_t* const* frames, float* fields);
+__global__ void k_window_decisions(const BridgeFrameParams* FP, const float* fields, int nframes, const float* amps,
+                                   int acc, float chroma_gain, float nominal_burst, int field_order,
+                                   int* out_f1_bottom, float* out_gain);
+__global__ void k_weave_requant(const BridgeFrameParams* FP, const int16_t* const* frames, int core_begin,
+                                const BridgeField* dec, const int* f1_bottom, const float* gain_p, int monochrome,
+                                const BridgeOut* out);
 
 // motion.hip
 __global__ void k_sad_bm(const SadDesc* D, const int* blk_prefix, int nf);

@@ -92,6 +92,10 @@ __global__ void __launch_bounds__(256) k_sad_bm(const SadDesc* D, const int* blk
 // centre-out order with a strict '<' (:233-252, :286-298).
 // ===========================================================================
 
+// hipcc contracts a * b + c into one FMA by default; hvd-core's float code
+// rounds the product first, so contraction is switched off from here on.
+#pragma clang fp contract(off)
+
 namespace hvdk {
 
 namespace {
@@ -150,8 +154,8 @@ __global__ void k_me_decimate(int npairs, int h, int w, int f, const float* in, 
   const float norm = 1.0f / (float)(f * f);
   float acc = 0.0f;
   for (int dy = 0; dy < f; ++dy)
-    for (int dx = 0; dx < f; ++dx) acc = __fadd_rn(acc, a[(size_t)(y * f + dy) * w + x * f + dx]);
-  out[(size_t)pp * hh * ww + (size_t)y * ww + x] = __fmul_rn(acc, norm);
+    for (int dx = 0; dx < f; ++dx) acc = (acc + a[(size_t)(y * f + dy) * w + x * f + dx]);
+  out[(size_t)pp * hh * ww + (size_t)y * ww + x] = (acc * norm);
 }
 
 // SSD of one candidate for one tile (motion.cpp:171-231 without the early
@@ -179,8 +183,8 @@ __global__ void k_me_ssd(int npairs, int h, int w, int tile, const float* planes
     const int yy = min(max(y - sy, 0), h - 1);
     for (int x = tx * tile; x < x_end; ++x) {
       const int xx = min(max(x - sx, 0), w - 1);
-      const float diff = __fsub_rn(A[(size_t)y * w + x], B[(size_t)yy * w + xx]);
-      acc = __fadd_rn(acc, __fmul_rn(diff, diff));
+      const float diff = (A[(size_t)y * w + x] - B[(size_t)yy * w + xx]);
+      acc = (acc + (diff * diff));
     }
   }
   se[idx] = acc;
@@ -199,15 +203,15 @@ __global__ void k_me_select(int npairs, int nt, const int* dys, const int* dxs, 
   float vy = 0.0f, vx = 0.0f;
   for (int c = 0; c < nc; ++c) {
     const int dy = dys[c / ndx], dx = dxs[c % ndx];
-    const float bias = __fadd_rn(1.0f, __fmul_rn(0.02f, (float)(abs(dy) + abs(dx))));
+    const float bias = (1.0f + (0.02f * (float)(abs(dy) + abs(dx))));
     const float s = se[(size_t)idx * nc + c];
     if (cost0 && dy == 0 && dx == 0) cost0[idx] = s;
-    const float cost = __fmul_rn(s, bias);
+    const float cost = (s * bias);
     if (cost < b) {
       b = cost;
       if (base_dy) {
-        vy = __fadd_rn(base_dy[idx], (float)dy);
-        vx = __fadd_rn(base_dx[idx], (float)dx);
+        vy = (base_dy[idx] + (float)dy);
+        vx = (base_dx[idx] + (float)dx);
       } else {
         vy = (float)dy;
         vx = (float)dx;
@@ -227,16 +231,16 @@ __global__ void k_me_upscale(int npairs, int th, int tw, int hh, int ww, int f, 
   const int t = idx % (th * tw), pi = idx / (th * tw);
   const int ty = t / tw, tx = t % tw;
   const int gy = min((ty * hh) / th, hh - 1), gx = min((tx * ww) / tw, ww - 1);
-  bdy[idx] = __fmul_rn(cdy[(size_t)pi * hh * ww + gy * ww + gx], (float)f);
-  bdx[idx] = __fmul_rn(cdx[(size_t)pi * hh * ww + gy * ww + gx], (float)f);
+  bdy[idx] = (cdy[(size_t)pi * hh * ww + gy * ww + gx] * (float)f);
+  bdx[idx] = (cdx[(size_t)pi * hh * ww + gy * ww + gx] * (float)f);
 }
 
 // base +/- 1 in y or x for the four parabolic-fit costs (motion.cpp:309-315).
 __global__ void k_me_offset(int n, const float* mdy, const float* mdx, float ody, float odx, float* tdy, float* tdx) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  tdy[i] = __fadd_rn(mdy[i], ody);
-  tdx[i] = __fadd_rn(mdx[i], odx);
+  tdy[i] = (mdy[i] + ody);
+  tdx[i] = (mdx[i] + odx);
 }
 
 // Parabolic half-pel fit, prefer-zero margin rule, vector sum
@@ -247,19 +251,19 @@ __global__ void k_me_finish(int n, const float* cym, const float* cyp, const flo
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   auto parabolic = [](float cm, float c0, float cp) {
-    const float den = __fadd_rn(__fsub_rn(cm, __fmul_rn(2.0f, c0)), cp);
+    const float den = ((cm - (2.0f * c0)) + cp);
     float sub = 0.0f;
-    if (den > 1e-9f) sub = __fdiv_rn(__fmul_rn(0.5f, __fsub_rn(cm, cp)), fmaxf(den, 1e-9f));
+    if (den > 1e-9f) sub = ((0.5f * (cm - cp)) / fmaxf(den, 1e-9f));
     return fminf(fmaxf(sub, -0.5f), 0.5f);
   };
   float sdy = parabolic(cym[i], best[i], cyp[i]);
   float sdx = parabolic(cxm[i], best[i], cxp[i]);
   float vy = mdy[i], vx = mdx[i], b = best[i];
   if (best0[i] < b) { vy = 0.0f; vx = 0.0f; b = best0[i]; }
-  if (b > __fmul_rn(0.85f, cost0[i])) { vy = 0.0f; vx = 0.0f; b = cost0[i]; sdy = 0.0f; sdx = 0.0f; }
+  if (b > (0.85f * cost0[i])) { vy = 0.0f; vx = 0.0f; b = cost0[i]; sdy = 0.0f; sdx = 0.0f; }
   best[i] = b;
-  out_dy[i] = __fadd_rn(vy, sdy);
-  out_dx[i] = __fadd_rn(vx, sdx);
+  out_dy[i] = (vy + sdy);
+  out_dx[i] = (vx + sdx);
 }
 
 // MotionConfidence (motion.cpp:321-387).  One block per pair; per-tile sums
@@ -282,18 +286,18 @@ __global__ void __launch_bounds__(256) k_me_confidence(int h, int w, int tile, c
     for (int y = y0; y < y1; ++y)
       for (int x = x0; x < x1; ++x) {
         const float v = A[(size_t)y * w + x];
-        sum = __fadd_rn(sum, v);
-        sum2 = __fadd_rn(sum2, __fmul_rn(v, v));
-        np = __fadd_rn(np, 1.0f);
+        sum = (sum + v);
+        sum2 = (sum2 + (v * v));
+        np = (np + 1.0f);
       }
     const float npc = fmaxf(1.0f, np);
-    const float norm = __fdiv_rn(1.0f, npc);
-    const float mean = __fmul_rn(sum, norm);
-    const float var = __fsub_rn(__fmul_rn(sum2, norm), __fmul_rn(mean, mean));
-    const float energy = __fmul_rn(__fadd_rn(var, 1.0f), npc);
-    const float c = __fsub_rn(1.0f, __fdiv_rn(bst[t], energy));
+    const float norm = (1.0f / npc);
+    const float mean = (sum * norm);
+    const float var = ((sum2 * norm) - (mean * mean));
+    const float energy = ((var + 1.0f) * npc);
+    const float c = (1.0f - (bst[t] / energy));
     conf[t] = fminf(fmaxf(c, 0.0f), 1.0f);
-    ratio[t] = __fdiv_rn(bst[t], npc);
+    ratio[t] = (bst[t] / npc);
   }
   __syncthreads();
   // numpy median: the element of rank nt/2, averaged with rank nt/2 - 1 when nt is even
@@ -308,13 +312,13 @@ __global__ void __launch_bounds__(256) k_me_confidence(int h, int w, int tile, c
     if (nt % 2 == 0 && rank == nt / 2 - 1) s_lo = vi;
   }
   __syncthreads();
-  const float med = nt % 2 == 0 ? __fmul_rn(0.5f, __fadd_rn(s_hi, s_lo)) : s_hi;
+  const float med = nt % 2 == 0 ? (0.5f * (s_hi + s_lo)) : s_hi;
   float* out = conf_out[pi];
   for (int t = threadIdx.x; t < nt; t += blockDim.x) {
-    const float excess = fmaxf(0.0f, __fsub_rn(ratio[t], med));
-    const float m1 = __fadd_rn(med, 1.0f);
-    float c = __fmul_rn(conf[t], __fdiv_rn(m1, __fadd_rn(m1, excess)));
-    c = __fmul_rn(c, __fdiv_rn(6.0f, __fadd_rn(6.0f, med)));
+    const float excess = fmaxf(0.0f, (ratio[t] - med));
+    const float m1 = (med + 1.0f);
+    float c = (conf[t] * (m1 / (m1 + excess)));
+    c = (c * (6.0f / (6.0f + med)));
     out[t] = c;
   }
 }

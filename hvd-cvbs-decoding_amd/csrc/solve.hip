@@ -7,10 +7,10 @@
 //              sums of curv(d) (:352-399) and Re<g, d> (:72-117);
 //   k_cg_step  chi <- chi + alpha d (on the fly), g <- grad(chi) (:286-347)
 //              and the FP64 partial of |g|^2;
-// the last block of each field reduces that field's partials in a fixed
-// order and advances its scalar CG state (alpha, beta, exits :530,:547-548),
-// so no host round trip happens inside the solve and converged fields drop
-// out by themselves (per-field masks).  chi and d are ping-ponged so a block
+// a one-block-per-field kernel then sums that field's block partials in a
+// fixed order and advances its scalar CG state (alpha, beta, the exits of
+// :530 and :547-548), so no host round trip happens inside the solve and
+// converged fields drop out by themselves (per-field masks).  chi and d are ping-ponged so a block
 // never reads a neighbour's already-updated value.
 #include "hvd_device.h"
 #include "hvd_kernels.h"
@@ -102,29 +102,28 @@ __device__ inline float2 grad_at(const DevField& f, const FieldCG& st, const flo
   return make_float2(-img * c0.x + cp.x, img * c0.y + cp.y);
 }
 
-// Sum this block's per-thread values v[0..NV-1], publish the partials and, in
-// the last block of the field, reduce every block's partials in block order.
-// Returns true in the elected block (all threads); `tot` then holds the sums.
+// Sum this block's per-thread values v[0..NV-1] and publish them as this
+// tile's partials.  The per-field totals are formed by the k_reduce_* kernels
+// that follow (a kernel boundary orders the two; no device-scope fence per
+// block, which on a multi-XCD part would write back the L2 each time).
 template <int NV>
-__device__ inline bool reduce_field(const double (&v)[NV], double* partials, int tile0, int lt, int nblk,
-                                    unsigned* counter, double (&tot)[NV]) {
+__device__ inline void publish_partials(const double (&v)[NV], double* partials, int tile0, int lt) {
   __shared__ double lds[kBlock / 64];
-  __shared__ int s_last;
-  double bs[NV];
-  for (int k = 0; k < NV; ++k) bs[k] = block_sum<kBlock>(v[k], lds);
-  if (threadIdx.x == 0) {
-    for (int k = 0; k < NV; ++k) store_partial(&partials[(size_t)(tile0 + lt) * NV + k], bs[k]);
-    s_last = arrive_last(counter, (unsigned)nblk) ? 1 : 0;
-  }
-  __syncthreads();
-  if (!s_last) return false;
-  __threadfence();
   for (int k = 0; k < NV; ++k) {
-    double acc = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += kBlock) acc += load_partial(&partials[(size_t)(tile0 + b) * NV + k]);
-    tot[k] = block_sum<kBlock>(acc, lds);
+    const double bs = block_sum<kBlock>(v[k], lds);
+    if (threadIdx.x == 0) partials[(size_t)(tile0 + lt) * NV + k] = bs;
   }
-  return true;
+}
+
+// Fixed-order sum of field fi's tile partials (component k of NV), valid in
+// thread 0 of a 256-thread block.
+template <int NV>
+__device__ inline double field_total(const double* partials, const int* tp, int fi, int k) {
+  __shared__ double lds[4];
+  const int t0 = tp[fi], nblk = tp[fi + 1] - t0;
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) acc += partials[(size_t)(t0 + b) * NV + k];
+  return block_sum<256>(acc, lds);
 }
 
 }  // namespace
@@ -272,10 +271,16 @@ __global__ void __launch_bounds__(kBlock) k_grad_init(const DevField* F, FieldCG
     f.d[s.d_cur][i] = make_float2(-g.x, -g.y);
     v[0] = (double)g.x * g.x + (double)g.y * g.y;
   }
-  double tot[1];
-  if (reduce_field<1>(v, partials, tp[c.f], c.lt, c.nblk, &s.arrive, tot) && threadIdx.x == 0) {
-    s.gg = tot[0];
-    s.gg0 = tot[0];
+  publish_partials<1>(v, partials, tp[c.f], c.lt);
+}
+
+__global__ void __launch_bounds__(256) k_reduce_init(FieldCG* st, const int* tp, const double* partials) {
+  const int fi = blockIdx.x;
+  FieldCG& s = st[fi];
+  const double gg = field_total<1>(partials, tp, fi, 0);
+  if (threadIdx.x == 0) {
+    s.gg = gg;
+    s.gg0 = gg;
     s.it = 0;
     s.done = s.n_inner <= 0 ? 1 : 0;
   }
@@ -327,15 +332,25 @@ __global__ void __launch_bounds__(kBlock) k_cg_curv(const DevField* F, FieldCG* 
     v[0] = h;
     v[1] = (double)g0.x * d0.x + (double)g0.y * d0.y;
   }
-  double tot[2];
-  if (reduce_field<2>(v, partials, tp[c.f], c.lt, c.nblk, &s.arrive, tot) && threadIdx.x == 0) {
-    if (upd) s.d_cur ^= 1;
-    s.hh = tot[0];
-    s.gd = tot[1];
-    if (tot[0] <= 1e-12) {
+  publish_partials<2>(v, partials, tp[c.f], c.lt);
+}
+
+// alpha = -1/2 <g, d> / curv(d), or stop when curv(d) <= 1e-12
+// (variational.cpp:528-531); commits the d ping-pong of iteration it.
+__global__ void __launch_bounds__(256) k_reduce_curv(FieldCG* st, const int* tp, const double* partials, int it) {
+  const int fi = blockIdx.x;
+  FieldCG& s = st[fi];
+  if (s.done) return;
+  const double hh = field_total<2>(partials, tp, fi, 0);
+  const double gd = field_total<2>(partials, tp, fi, 1);
+  if (threadIdx.x == 0) {
+    if (it > 0) s.d_cur ^= 1;
+    s.hh = hh;
+    s.gd = gd;
+    if (hh <= 1e-12) {
       s.done = 1;
     } else {
-      s.alpha = (float)(-0.5 * tot[1] / tot[0]);
+      s.alpha = (float)(-0.5 * gd / hh);
     }
   }
 }
@@ -360,10 +375,17 @@ __global__ void __launch_bounds__(kBlock) k_cg_step(const DevField* F, FieldCG* 
     chi_out[i] = cc;
     v[0] = (double)g.x * g.x + (double)g.y * g.y;
   }
-  double tot[1];
-  if (reduce_field<1>(v, partials, tp[c.f], c.lt, c.nblk, &s.arrive, tot) && threadIdx.x == 0) {
+  publish_partials<1>(v, partials, tp[c.f], c.lt);
+}
+
+// beta, the chi ping-pong, and the exits of variational.cpp:540-548.
+__global__ void __launch_bounds__(256) k_reduce_step(FieldCG* st, const int* tp, const double* partials) {
+  const int fi = blockIdx.x;
+  FieldCG& s = st[fi];
+  if (s.done) return;
+  const double ggn = field_total<1>(partials, tp, fi, 0);
+  if (threadIdx.x == 0) {
     s.chi_cur ^= 1;
-    const double ggn = tot[0];
     s.beta = (float)(ggn / (s.gg > 1e-30 ? s.gg : 1e-30));
     s.gg = ggn;
     s.it += 1;

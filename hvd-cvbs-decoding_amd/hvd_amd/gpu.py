@@ -22,6 +22,7 @@ ABI_SYMBOLS = (
     "hvd_gpu_burst_amplitude_ire", "hvd_gpu_holographic_init", "hvd_gpu_variational_refine",
     "hvd_gpu_decode_fields_u16", "hvd_gpu_decode_fields_ire", "hvd_gpu_block_match_sad_u16",
     "hvd_gpu_set_profiling", "hvd_gpu_cg_timing", "hvd_gpu_estimate_motion",
+    "hvd_gpu_decode_frame_window_s16",
 )
 
 
@@ -69,6 +70,31 @@ class Plane(C.Structure):
                 ("y", C.c_void_p)]
 
 
+class FrameParams(C.Structure):
+    """hvd::FrameParams (frame_bridge.h:40-101)."""
+    _fields_ = [(n, C.c_int) for n in (
+        "standard", "frame_width", "frame_height", "field1_lines", "active_video_start", "active_video_end",
+        "colour_burst_start", "colour_burst_end", "first_active_frame_line", "last_active_frame_line")] + \
+        [(n, C.c_float) for n in ("black_level", "white_level", "blanking_level", "chroma_dc")] + \
+        [("sample_rate", C.c_double), ("subcarrier_hz", C.c_double), ("extra_sample_lines", C.c_int32 * 8),
+         ("extra_sample_line_count", C.c_int)]
+
+
+class BridgeParams(C.Structure):
+    _fields_ = [("chroma_gain", C.c_float), ("acc", C.c_int), ("field_order", C.c_int), ("monochrome", C.c_int),
+                ("chroma_phase_deg", C.c_float), ("temporal_strength", C.c_float)]
+
+    @classmethod
+    def make(cls, **kw) -> "BridgeParams":
+        d = dict(chroma_gain=1.0, acc=1, field_order=0, monochrome=0, chroma_phase_deg=0.0, temporal_strength=-1.0)
+        d.update(kw)
+        return cls(**d)
+
+
+class YcFrame(C.Structure):
+    _fields_ = [("luma", C.c_void_p), ("chroma", C.c_void_p), ("u_plane", C.c_void_p), ("v_plane", C.c_void_p)]
+
+
 class HvdGpuError(RuntimeError):
     pass
 
@@ -101,6 +127,9 @@ def load() -> C.CDLL:
                                                   C.c_float, pp, pp, pp, vp]
         lib.hvd_gpu_block_match_sad_u16.argtypes = [vp, pp, pp, i, i, i, i, i, i, i, pp, pp, pp, vp]
         lib.hvd_gpu_estimate_motion.argtypes = [vp, pp, pp, i, i, i, i, i, pp, pp, pp, vp]
+        lib.hvd_gpu_decode_frame_window_s16.argtypes = [vp, pp, i, i, i, C.POINTER(FrameParams),
+                                                        C.POINTER(SolverParams), C.POINTER(BridgeParams),
+                                                        C.POINTER(YcFrame), C.POINTER(C.c_float), vp]
         lib.hvd_gpu_set_profiling.argtypes = [vp, i]
         lib.hvd_gpu_cg_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                                           C.POINTER(C.c_double)]
@@ -259,6 +288,28 @@ class Engine:
                                                        _ptrs(ths) if ths else None, _stream()),
                     "decode_fields_ire")
         return ys, chis, ths
+
+    # ---- frame bridge ---------------------------------------------------------
+    def decode_frame_window_s16(self, frames, core_begin, core_end, fp: FrameParams, params: SolverParams,
+                                bridge: BridgeParams, want_gain=False):
+        """frames: list of flat int16 device tensors (field-sequential frames).
+        Returns a list of (luma, chroma, u, v) device tensors per core frame and
+        the applied chroma gain (None unless want_gain)."""
+        import torch
+        dev = frames[0].device
+        n_px = fp.frame_width * fp.frame_height
+        outs, descs = [], []
+        for _ in range(core_end - core_begin):
+            o = (torch.empty(n_px, dtype=torch.int16, device=dev), torch.empty(n_px, dtype=torch.int16, device=dev),
+                 torch.empty(n_px, dtype=torch.float64, device=dev), torch.empty(n_px, dtype=torch.float64, device=dev))
+            outs.append(o)
+            descs.append(YcFrame(*[t.data_ptr() for t in o]))
+        arr = (YcFrame * max(len(descs), 1))(*descs)
+        gain = C.c_float(0.0)
+        self._check(self.lib.hvd_gpu_decode_frame_window_s16(
+            self.ctx, _ptrs(frames), len(frames), core_begin, core_end, C.byref(fp), C.byref(params),
+            C.byref(bridge), arr, C.byref(gain) if want_gain else None, _stream()), "decode_frame_window_s16")
+        return outs, (gain.value if want_gain else None)
 
     # ---- motion ---------------------------------------------------------------
     def estimate_motion(self, refs, curs, tile=32, search=8):

@@ -97,11 +97,12 @@ class Scene:
 
 
 def make_scene(g: Geometry, rng: np.random.Generator, canvas_pad: int = 0,
-               saturation: float = 40.0) -> Scene:
+               saturation: float = 40.0, line_factor: int = 1) -> Scene:
     """Luma = horizontal ramp + 8 Gaussian blobs clipped to 0..100 IRE;
     chroma = bilinear-upsampled 16x8 grid with |C| <= `saturation` IRE.
-    `canvas_pad` enlarges the canvas on every side (for motion crops)."""
-    h = g.lines + 2 * canvas_pad
+    `canvas_pad` enlarges the canvas on every side (for motion crops);
+    `line_factor` = 2 renders at frame resolution (both fields' lines)."""
+    h = g.lines * line_factor + 2 * canvas_pad
     w = g.width + 2 * canvas_pad
     yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
     luma = 10.0 + 70.0 * xx / max(w - 1, 1)
@@ -230,3 +231,49 @@ def motion_sequence(seed: int, n_fields: int = 8, g: Geometry = NTSC, noise_ire:
         raws.append(raw)
     truth = {"global": (gdy, gdx), "rects": rects}
     return np.stack(raws), truth
+
+
+# ---------------------------------------------------------------------------
+# Field-sequential 10-bit frames (the int16 layout of hvd::FrameParams,
+# frame_bridge.h:40-112): field 1 on flat lines [0, field1_lines), field 2
+# after it; PAL frames carry two extra samples on flat lines 312 and 624.
+
+FRAME_NTSC = dict(standard=0, frame_width=910, frame_height=525, field1_lines=263, active_video_start=134,
+                  active_video_end=894, colour_burst_start=76, colour_burst_end=112, first_active_frame_line=42,
+                  last_active_frame_line=0, black_level=282.0, white_level=800.0, blanking_level=240.0,
+                  chroma_dc=512.0, sample_rate=FS_NTSC, subcarrier_hz=0.0, extra_sample_lines=())
+FRAME_PAL = dict(standard=1, frame_width=1135, frame_height=625, field1_lines=313, active_video_start=185,
+                 active_video_end=1107, colour_burst_start=98, colour_burst_end=138, first_active_frame_line=44,
+                 last_active_frame_line=620, black_level=256.0, white_level=844.0, blanking_level=256.0,
+                 chroma_dc=512.0, sample_rate=FS_PAL, subcarrier_hz=0.0, extra_sample_lines=(312, 312, 624, 624))
+
+
+def frame_line_offset(fp: dict, line: int) -> int:
+    return line * fp["frame_width"] + sum(1 for e in fp["extra_sample_lines"] if e < line)
+
+
+def frames_s16(fp: dict, n: int, seed: int, noise_ire: float = 0.5) -> list:
+    """n field-sequential int16 frames of one static scene (field 2 samples
+    the scene half a frame line below field 1)."""
+    g = PAL if fp["standard"] == 1 else NTSC
+    rng = np.random.default_rng(seed)
+    sc = make_scene(g, rng, line_factor=2)
+    cpi = (fp["white_level"] - fp["blanking_level"]) / 100.0
+    fw, fh, f1 = fp["frame_width"], fp["frame_height"], fp["field1_lines"]
+    out = []
+    for t in range(n):
+        buf = np.full(frame_line_offset(fp, fh), int(fp["blanking_level"]), np.int16)
+        for blk in range(2):
+            field_scene = Scene(sc.y[blk::2], sc.chi[blk::2])
+            theta0 = ntsc_theta0(2 * t + blk) if fp["standard"] != 1 else 0.7 * (2 * t + blk)
+            _, ire = render_field(g, field_scene, rng, theta0=theta0, noise_ire=0.0,
+                                  vswitch_first=1 if blk == 0 else -1)
+            if noise_ire > 0:
+                ire = ire + rng.normal(0.0, noise_ire, size=ire.shape)
+            codes = np.clip(np.round(fp["blanking_level"] + cpi * ire), 0, 1023).astype(np.int16)
+            nlines = f1 if blk == 0 else fh - f1
+            for y in range(nlines):
+                o = frame_line_offset(fp, y + blk * f1)
+                buf[o:o + fw] = codes[y]
+        out.append(buf)
+    return out

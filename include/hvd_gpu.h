@@ -90,6 +90,50 @@ typedef struct {
   float* y;   /* lines * width: luma out, may be NULL */
 } hvd_gpu_plane;
 
+/* hvd::FrameParams (frame_bridge.h:40-101): a field-sequential frame's
+ * geometry and levels in the 10-bit sample domain.  extra_sample_lines lists,
+ * once per extra sample, the frame-flat lines that carry one more sample
+ * than frame_width (PAL: 312 and 624, twice each). */
+typedef struct {
+  int standard; /* 0 NTSC, 1 PAL, 2 PAL-M */
+  int frame_width;
+  int frame_height;
+  int field1_lines;
+  int active_video_start;
+  int active_video_end;
+  int colour_burst_start;
+  int colour_burst_end;
+  int first_active_frame_line;
+  int last_active_frame_line; /* 0 => from the field height */
+  float black_level;
+  float white_level;
+  float blanking_level;
+  float chroma_dc;
+  double sample_rate;
+  double subcarrier_hz;
+  int32_t extra_sample_lines[8];
+  int extra_sample_line_count;
+} hvd_frame_params;
+
+/* The HvdConfig fields the frame bridge reads besides the solver's. */
+typedef struct {
+  float chroma_gain;
+  int acc;
+  int field_order; /* 0 auto (measured), 1 field 1 top, 2 field 1 bottom */
+  int monochrome;
+  float chroma_phase_deg;
+  float temporal_strength; /* < 0: 3-D off */
+} hvd_bridge_params;
+
+/* hvd::YcFrameS16 planes (frame_bridge.h:123-131), each frame_width x
+ * frame_height, DEVICE pointers. */
+typedef struct {
+  int16_t* luma;
+  int16_t* chroma;
+  double* u_plane;
+  double* v_plane;
+} hvd_yc_frame;
+
 /* ---- context ---------------------------------------------------------- */
 
 /* Creates a context on HIP device `device` (replaces the construction of
@@ -174,6 +218,22 @@ int hvd_gpu_decode_fields_u16(hvd_gpu_ctx* ctx, const uint16_t* const* raw, int 
 int hvd_gpu_decode_fields_ire(hvd_gpu_ctx* ctx, const float* const* field_ire, int n, const hvd_field_geometry* geom,
                               const hvd_solver_params* p, float chroma_phase_deg, float* const* y_out,
                               float* const* chi_out, float* const* theta_out, void* stream);
+
+/* DecodeFrameSequenceWindow, 2-D path (frame_bridge.cpp:722-947): frames[t]
+ * (t < nframes) are field-sequential int16 10-bit frames (device); the call
+ * splits them into fields (SplitFrameFields :61-99, PAL line offsets
+ * FrameLineOffset frame_bridge.h:106-112), measures the window ACC gain
+ * (:776-795) and field order (:759-765), runs the field driver on all
+ * 2 * nframes fields in one batch, and weaves and requantises the core
+ * frames [core_begin, core_end) into out[0 .. core_end - core_begin)
+ * (:867-945).  3-D decoding (temporal strength >= 0 with more fields than
+ * the ambiguity stride, and cg_iterations > 0) is HVD_GPU_ERR_UNSUPPORTED.
+ * `frames`, `out` are HOST arrays; acc_gain (HOST, optional) receives
+ * chroma_gain x ACC and makes the call synchronous. */
+int hvd_gpu_decode_frame_window_s16(hvd_gpu_ctx* ctx, const int16_t* const* frames, int nframes, int core_begin,
+                                    int core_end, const hvd_frame_params* fp, const hvd_solver_params* p,
+                                    const hvd_bridge_params* b, const hvd_yc_frame* out, float* acc_gain,
+                                    void* stream);
 
 /* ---- motion ------------------------------------------------------------ */
 

@@ -1,0 +1,165 @@
+"""Oracle for hvd-core's frame bridge, 2-D sequence window.
+
+TEST INFRASTRUCTURE ONLY.  Restates frame_bridge.cpp (hvd-core/src):
+FrameLineOffset (frame_bridge.h:106-112), SplitFrameFields (:61-99),
+DetectFieldParity (:102-149), the window ACC and field-order vote of
+DecodeFrameSequenceWindow (:776-795, :759-765) and its parity weave with
+ClampU10 / ClampS10 requantisation (:867-945, :25-39).  The per-field decode
+is the FP64 oracle's decode_field (sequence.cpp:229-281 with the temporal
+terms off).  Integer / byte plumbing is numpy.
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+
+@dataclasses.dataclass
+class FrameParams:
+    standard: int
+    frame_width: int
+    frame_height: int
+    field1_lines: int
+    active_video_start: int
+    active_video_end: int
+    colour_burst_start: int
+    colour_burst_end: int
+    first_active_frame_line: int
+    last_active_frame_line: int
+    black_level: float
+    white_level: float
+    blanking_level: float
+    chroma_dc: float
+    sample_rate: float
+    subcarrier_hz: float = 0.0
+    extra_sample_lines: tuple = ()
+
+    def buffer_len(self) -> int:
+        return frame_line_offset(self, self.frame_height)
+
+
+def frame_line_offset(fp: FrameParams, line: int) -> int:
+    return line * fp.frame_width + sum(1 for e in fp.extra_sample_lines if e < line)
+
+
+def field_geometry(fp: FrameParams, Geometry):
+    """FieldGeometryFromParams (frame_bridge.cpp:43-58)."""
+    return Geometry(fp.frame_width, fp.field1_lines, fp.active_video_start, fp.active_video_end,
+                    fp.colour_burst_start, fp.colour_burst_end, fp.first_active_frame_line // 2,
+                    fp.last_active_frame_line // 2 if fp.last_active_frame_line else 0, fp.sample_rate,
+                    fp.subcarrier_hz, fp.standard)
+
+
+def split_frame_fields(frame: np.ndarray, fp: FrameParams):
+    fw, fh, f1 = fp.frame_width, fp.frame_height, fp.field1_lines
+    f2 = fh - f1
+    cpi = np.float32((fp.white_level - fp.blanking_level) / 100.0)
+    top = np.full((f1, fw), fp.blanking_level, np.float32)
+    bot = np.full((f1, fw), fp.blanking_level, np.float32)
+    for y in range(f1):
+        o = frame_line_offset(fp, y)
+        top[y] = frame[o:o + fw].astype(np.float32)
+        if y < f2:
+            o = frame_line_offset(fp, f1 + y)
+            bot[y] = frame[o:o + fw].astype(np.float32)
+    to_ire = lambda a: ((a - np.float32(fp.blanking_level)) / cpi).astype(np.float32)  # noqa: E731
+    return to_ire(top), to_ire(bot)
+
+
+def detect_field_parity(b1, b2, fp: FrameParams) -> int:
+    f1 = fp.field1_lines
+    fal = fp.first_active_frame_line // 2
+    lal = fp.last_active_frame_line // 2 if fp.last_active_frame_line else f1
+    a0, a1 = fp.active_video_start, fp.active_video_end
+    lines, width = lal - fal, a1 - a0
+    if lines < 8 or width < 32:
+        return 0
+    ystep = max(1, lines // 120)
+    ep = em = 0.0
+    n = 0
+    q = np.float32(0.25)
+    h = np.float32(0.5)
+    for r in range(fal + 1, lal - 1, ystep):
+        for x in range(a0, a1 - 3, 16):
+            lp = lambda p, row: q * (((p[row, x] + p[row, x + 1]) + p[row, x + 2]) + p[row, x + 3])  # noqa: E731
+            b, ac, au, ad = lp(b2, r), lp(b1, r), lp(b1, r - 1), lp(b1, r + 1)
+            ep += abs(float(np.float32(b - h * np.float32(ac + ad))))
+            em += abs(float(np.float32(b - h * np.float32(ac + au))))
+            n += 1
+    if n == 0:
+        return 0
+    ep /= n
+    em /= n
+    lo, hi = min(ep, em), max(ep, em)
+    if hi <= 1e-9 or (hi - lo) / hi < 0.02:
+        return 0
+    return 1 if ep < em else -1
+
+
+def lround(x):
+    """std::lround: half away from zero."""
+    x = np.asarray(x, np.float64)
+    return (np.sign(x) * np.floor(np.abs(x) + 0.5)).astype(np.int64)
+
+
+def decode_frame_sequence_window(orc, frames, core_begin, core_end, fp: FrameParams, cfg, Geometry, *,
+                                 chroma_gain=1.0, acc=True, field_order=0, monochrome=False, chroma_phase_deg=0.0):
+    """Returns (list of (luma, chroma, u, v) flat planes per core frame, gain)."""
+    g = field_geometry(fp, Geometry)
+    fw, fh, f1 = fp.frame_width, fp.frame_height, fp.field1_lines
+    pairs = [split_frame_fields(fr, fp) for fr in frames]
+    amps = []
+    for top, bot in pairs:
+        amps += [orc.burst_amplitude_ire(top, g), orc.burst_amplitude_ire(bot, g)]
+    if field_order == 0:
+        vote = sum(detect_field_parity(t, b, fp) for t, b in pairs)
+        f1_bottom = vote < 0
+    else:
+        f1_bottom = field_order == 2
+    gain = 1.0
+    if acc:
+        med = sorted(amps)[len(amps) // 2]
+        gain = orc.acc_gain(med, 21.43 if g.standard == 1 else 20.0)
+    gain = np.float32(np.float32(chroma_gain) * np.float32(gain))
+    dec = []
+    for top, bot in pairs:
+        for fld in (top, bot):
+            r = orc.decode_field(fld, g, cfg, chroma_phase_deg)
+            fal = g.first_active_field_line
+            vs = r["vswitch"][fal:fal + g.lines] if g.uses_vswitch else None
+            car = orc.make_carrier(r["theta"][fal:fal + g.lines].astype(np.float32), vs, g)
+            dec.append((r["y"], r["chi"], car))
+    scale = np.float32((fp.white_level - fp.blanking_level) / 100.0)
+    p0 = 1 if f1_bottom else 0
+    outs = []
+    fal = g.first_active_field_line
+    a0 = fp.active_video_start
+    for t in range(core_begin, core_end):
+        luma = np.zeros(fw * fh, np.int16)
+        for line in range(fh):
+            o = frame_line_offset(fp, line)
+            luma[line * fw:(line + 1) * fw] = frames[t][o:o + fw]
+        chroma = np.zeros(fw * fh, np.int16)
+        u = np.zeros(fw * fh)
+        v = np.zeros(fw * fh)
+        lines = dec[2 * t][0].shape[0]
+        aw = dec[2 * t][0].shape[1]
+        for r in range(2 * lines):
+            use_first = (r % 2) == p0
+            y, chi, car = dec[2 * t] if use_first else dec[2 * t + 1]
+            fl = r // 2
+            flat = fal + fl if r % 2 == 0 else f1 + fal + fl
+            if flat >= fh:
+                continue
+            idx = flat * fw + a0 + np.arange(aw)
+            yi = y[fl].astype(np.float32)
+            ch = np.zeros(aw, complex) if monochrome else chi[fl]
+            lum = np.float32(fp.blanking_level) + yi * scale
+            luma[idx] = np.clip(lround(lum), 0, 1023)
+            re = (ch * car[fl]).real.astype(np.float32) * scale
+            chroma[idx] = np.clip(lround(re), -1023, 1023)
+            v[idx] = float(gain) * ch.real * float(scale)
+            u[idx] = float(gain) * (-ch.imag) * float(scale)
+        outs.append((luma, chroma, u, v))
+    return outs, float(gain)

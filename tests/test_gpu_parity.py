@@ -365,3 +365,70 @@ def test_abi_rejects_bad_arguments(engine):
     # the context stays usable after a rejected call
     ys, _, _ = engine.decode_fields_u16([raw], [g], gpu.SolverParams.make())
     assert torch.isfinite(ys[0]).all()
+
+
+# ---------------------------------------------------------------- frame bridge
+def _bridge_case(engine, orc64, fpd, nframes, core, **bkw):
+    import bridge as OB
+    frames = synth.frames_s16(fpd, nframes, seed=91)
+    fp_o = OB.FrameParams(**fpd)
+    fp_g = gpu.FrameParams(**{k: v for k, v in fpd.items() if k != "extra_sample_lines"})
+    ex = fpd["extra_sample_lines"]
+    for i, e in enumerate(ex):
+        fp_g.extra_sample_lines[i] = e
+    fp_g.extra_sample_line_count = len(ex)
+    cfg = dict(cg_iterations=40, irls_outer=4, fast=0, cg_tol=1e-6)
+    want, wgain = OB.decode_frame_sequence_window(orc64, frames, core[0], core[1], fp_o, O.Cfg.make(**cfg),
+                                                  synth.Geometry, **bkw)
+    got, gain = engine.decode_frame_window_s16([dev(f) for f in frames], core[0], core[1], fp_g,
+                                               gpu.SolverParams.make(**cfg), gpu.BridgeParams.make(**bkw),
+                                               want_gain=True)
+    return frames, want, wgain, got, gain
+
+
+@pytest.mark.parametrize("std,nframes,core", [("ntsc", 2, (0, 2)), ("ntsc", 3, (1, 2)), ("pal", 1, (0, 1))])
+def test_frame_window_bridge(engine, orc64, std, nframes, core):
+    fpd = synth.FRAME_PAL if std == "pal" else synth.FRAME_NTSC
+    frames, want, wgain, got, gain = _bridge_case(engine, orc64, fpd, nframes, core)
+    assert abs(gain - wgain) < 1e-6
+    fw, fh = fpd["frame_width"], fpd["frame_height"]
+    scale = (fpd["white_level"] - fpd["blanking_level"]) / 100.0
+    for (wl, wc, wu, wv), (gl, gc, gu, gv), t in zip(want, got, range(core[0], core[1])):
+        gl, gc, gu, gv = (x.cpu().numpy() for x in (gl, gc, gu, gv))
+        dl = np.abs(gl.astype(np.int32) - wl)
+        dc = np.abs(gc.astype(np.int32) - wc)
+        assert dl.max() <= 1 and dc.max() <= 1           # requantised codes within one code
+        assert (dl == 0).mean() > 0.999 and (dc == 0).mean() > 0.999
+        assert np.abs(gu - wu).max() <= TOL_IRE * scale * max(gain, 1.0)
+        assert np.abs(gv - wv).max() <= TOL_IRE * scale * max(gain, 1.0)
+        # lossless split: luma + chroma reproduces the composite within one code
+        # (frame_bridge_test.cpp:90-96), checked in the active picture
+        comp = np.concatenate([frames[t][synth.frame_line_offset(fpd, L):synth.frame_line_offset(fpd, L) + fw]
+                               for L in range(fh)]).astype(np.int32)
+        active = gc != 0
+        assert np.abs(gl.astype(np.int32)[active] + gc.astype(np.int32)[active] - comp[active]).max() <= 1
+        # outside the active picture: composite passthrough, zero chroma
+        assert np.array_equal(gl[~active & (wc == 0)], wl[~active & (wc == 0)])
+
+
+def test_frame_window_options(engine, orc64):
+    # forced field order, monochrome, chroma gain and phase trim, ACC off
+    fpd = synth.FRAME_NTSC
+    _, want, wgain, got, gain = _bridge_case(engine, orc64, fpd, 1, (0, 1), field_order=2, monochrome=1,
+                                             chroma_gain=1.3, acc=0, chroma_phase_deg=20.0)
+    assert abs(gain - 1.3) < 1e-6 and abs(wgain - 1.3) < 1e-6
+    gl, gc, gu, gv = (x.cpu().numpy() for x in got[0])
+    assert not gc.any() and not gu.any() and not gv.any()
+    assert np.abs(gl.astype(np.int32) - want[0][0]).max() <= 1
+
+
+def test_frame_window_rejects_3d(engine):
+    fpd = synth.FRAME_NTSC
+    frames = synth.frames_s16(fpd, 2, seed=92)
+    fp_g = gpu.FrameParams(**{k: v for k, v in fpd.items() if k != "extra_sample_lines"})
+    with pytest.raises(gpu.HvdGpuError, match="3-D"):
+        engine.decode_frame_window_s16([dev(f) for f in frames], 0, 2, fp_g, gpu.SolverParams.make(),
+                                       gpu.BridgeParams.make(temporal_strength=0.0))
+    with pytest.raises(gpu.HvdGpuError, match="window"):
+        engine.decode_frame_window_s16([dev(f) for f in frames], 1, 1, fp_g, gpu.SolverParams.make(),
+                                       gpu.BridgeParams.make())
