@@ -169,7 +169,7 @@ def main():
     eng.set_profiling(False)
     fields_s = BATCH * args.steps / dt
     ms_step = 1000.0 * dt / args.steps
-    pairs = cg_launches / 2
+    pairs = cg_launches / 4  # per CG iteration: k_cg_curv, k_reduce_curv, k_cg_step, k_reduce_step
     avg_pair_ms = cg_ms / pairs
     px_per_pair = pxit / pairs  # pixels actually iterated per CG launch pair
     achieved = CG_BYTES_PER_PX_ITER * px_per_pair / (avg_pair_ms * 1e-3) / 1e9
@@ -184,7 +184,7 @@ def main():
         "config": {"workload": "ntsc_910x263_batched64_lockin_fftinit_irls4x25", "fields_per_step": BATCH,
                    "field": "NTSC 4fsc 910x263 uint16, active 760x242", "irls_outer": 4, "cg_inner": 25,
                    "cg_tol": 1e-12, "motion_estimation": False, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": "CG iteration (k_cg_curv + k_cg_step)",
+        "roofline": {"bound": "hbm", "kernel": "CG iteration (k_cg_curv + k_reduce_curv + k_cg_step + k_reduce_step)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "avg_us_per_iteration": round(avg_pair_ms * 1e3, 2),

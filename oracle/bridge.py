@@ -105,7 +105,8 @@ def lround(x):
 
 def decode_frame_sequence_window(orc, frames, core_begin, core_end, fp: FrameParams, cfg, Geometry, *,
                                  chroma_gain=1.0, acc=True, field_order=0, monochrome=False, chroma_phase_deg=0.0):
-    """Returns (list of (luma, chroma, u, v) flat planes per core frame, gain)."""
+    """Returns (list of (luma, chroma, u, v) flat planes per core frame, gain,
+    field 1 is bottom)."""
     g = field_geometry(fp, Geometry)
     fw, fh, f1 = fp.frame_width, fp.frame_height, fp.field1_lines
     pairs = [split_frame_fields(fr, fp) for fr in frames]
@@ -162,4 +163,4 @@ def decode_frame_sequence_window(orc, frames, core_begin, core_end, fp: FramePar
             v[idx] = float(gain) * ch.real * float(scale)
             u[idx] = float(gain) * (-ch.imag) * float(scale)
         outs.append((luma, chroma, u, v))
-    return outs, float(gain)
+    return outs, float(gain), bool(f1_bottom)

@@ -378,18 +378,36 @@ def _bridge_case(engine, orc64, fpd, nframes, core, **bkw):
         fp_g.extra_sample_lines[i] = e
     fp_g.extra_sample_line_count = len(ex)
     cfg = dict(cg_iterations=40, irls_outer=4, fast=0, cg_tol=1e-6)
-    want, wgain = OB.decode_frame_sequence_window(orc64, frames, core[0], core[1], fp_o, O.Cfg.make(**cfg),
-                                                  synth.Geometry, **bkw)
+    want, wgain, f1b = OB.decode_frame_sequence_window(orc64, frames, core[0], core[1], fp_o,
+                                                       O.Cfg.make(**cfg), synth.Geometry, **bkw)
     got, gain = engine.decode_frame_window_s16([dev(f) for f in frames], core[0], core[1], fp_g,
                                                gpu.SolverParams.make(**cfg), gpu.BridgeParams.make(**bkw),
                                                want_gain=True)
-    return frames, want, wgain, got, gain
+    return frames, want, wgain, got, gain, f1b
+
+
+def _woven_composite(frame, fpd, f1_bottom):
+    """The raw composite codes rearranged the way the parity weave places the
+    decoded fields (frame_bridge.cpp:899-918), flat frame_width x frame_height."""
+    fw, fh, f1 = fpd["frame_width"], fpd["frame_height"], fpd["field1_lines"]
+    flat = np.concatenate([frame[synth.frame_line_offset(fpd, L):synth.frame_line_offset(fpd, L) + fw]
+                           for L in range(fh)]).astype(np.int32).reshape(fh, fw)
+    if not f1_bottom:
+        return flat.reshape(-1)
+    g = synth.PAL if fpd["standard"] == 1 else synth.NTSC
+    fal, lines = fpd["first_active_frame_line"] // 2, g.lines
+    out = flat.copy()
+    for fl in range(lines):
+        top, bot = fal + fl, f1 + fal + fl
+        if bot < fh:
+            out[top], out[bot] = flat[bot], flat[top]
+    return out.reshape(-1)
 
 
 @pytest.mark.parametrize("std,nframes,core", [("ntsc", 2, (0, 2)), ("ntsc", 3, (1, 2)), ("pal", 1, (0, 1))])
 def test_frame_window_bridge(engine, orc64, std, nframes, core):
     fpd = synth.FRAME_PAL if std == "pal" else synth.FRAME_NTSC
-    frames, want, wgain, got, gain = _bridge_case(engine, orc64, fpd, nframes, core)
+    frames, want, wgain, got, gain, f1b = _bridge_case(engine, orc64, fpd, nframes, core)
     assert abs(gain - wgain) < 1e-6
     fw, fh = fpd["frame_width"], fpd["frame_height"]
     scale = (fpd["white_level"] - fpd["blanking_level"]) / 100.0
@@ -403,8 +421,7 @@ def test_frame_window_bridge(engine, orc64, std, nframes, core):
         assert np.abs(gv - wv).max() <= TOL_IRE * scale * max(gain, 1.0)
         # lossless split: luma + chroma reproduces the composite within one code
         # (frame_bridge_test.cpp:90-96), checked in the active picture
-        comp = np.concatenate([frames[t][synth.frame_line_offset(fpd, L):synth.frame_line_offset(fpd, L) + fw]
-                               for L in range(fh)]).astype(np.int32)
+        comp = _woven_composite(frames[t], fpd, f1b)
         active = gc != 0
         assert np.abs(gl.astype(np.int32)[active] + gc.astype(np.int32)[active] - comp[active]).max() <= 1
         # outside the active picture: composite passthrough, zero chroma
@@ -414,8 +431,9 @@ def test_frame_window_bridge(engine, orc64, std, nframes, core):
 def test_frame_window_options(engine, orc64):
     # forced field order, monochrome, chroma gain and phase trim, ACC off
     fpd = synth.FRAME_NTSC
-    _, want, wgain, got, gain = _bridge_case(engine, orc64, fpd, 1, (0, 1), field_order=2, monochrome=1,
-                                             chroma_gain=1.3, acc=0, chroma_phase_deg=20.0)
+    _, want, wgain, got, gain, f1b = _bridge_case(engine, orc64, fpd, 1, (0, 1), field_order=2, monochrome=1,
+                                                  chroma_gain=1.3, acc=0, chroma_phase_deg=20.0)
+    assert f1b
     assert abs(gain - 1.3) < 1e-6 and abs(wgain - 1.3) < 1e-6
     gl, gc, gu, gv = (x.cpu().numpy() for x in got[0])
     assert not gc.any() and not gu.any() and not gv.any()
