@@ -369,17 +369,41 @@ struct RefineOut {
   float* aniso = nullptr;
 };
 
+constexpr int kMinStripRows = 2;
+
+// Lines per solver strip: the tallest strips (the fewest re-read halo rows)
+// that still give a launch enough waves to cover every CU several times.
+int strip_rows(const std::vector<DevField>& F) {
+#ifdef HVD_FORCE_STRIP_ROWS
+  static_assert(HVD_FORCE_STRIP_ROWS >= 1 && HVD_FORCE_STRIP_ROWS <= kMaxStripRows, "strip rows out of range");
+  (void)F;
+  return HVD_FORCE_STRIP_ROWS;
+#else
+  constexpr long kMinWaves = 2048;  // 8 per CU
+  for (int rows : {8, 4}) {
+    long waves = 0;
+    for (const DevField& f : F) waves += solver_strips(f.h, f.w, rows);
+    if (waves >= kMinWaves) return rows;
+  }
+  return kMinStripRows;
+#endif
+}
+
 void run_refine(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_params& p, double* partials,
                 float* gx, float* gy, const std::vector<int>& sample_prefix, FieldCG* dstate, hipStream_t st,
                 RefineOut out) {
   const int nf = (int)F.size();
+  const int rows = strip_rows(F);
   std::vector<int> tp(nf + 1, 0);
   for (int i = 0; i < nf; ++i) {
-    F[i].tiles_x = (F[i].w + kTileW - 1) / kTileW;
+    F[i].tiles_x = (F[i].w + kStripCols - 1) / kStripCols;
     F[i].tile0 = tp[i];
-    tp[i + 1] = tp[i] + F[i].tiles_x * ((F[i].h + kTileH - 1) / kTileH);
+    tp[i + 1] = tp[i] + solver_strips(F[i].h, F[i].w, rows);
   }
   const int nb = tp[nf];
+  int spf = tp[1];
+  for (int i = 1; i < nf; ++i)
+    if (F[i].h != F[0].h || F[i].w != F[0].w) spf = 0;
   const int outer = std::max(1, p.irls_outer);
   const int cg_total = p.fast ? p.cg_iterations * 2 / 3 : p.cg_iterations;
   const int n_inner = std::max(1, cg_total / outer);
@@ -406,6 +430,7 @@ void run_refine(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_par
   const DevField* dF = (const DevField*)(base + oF);
   const int* dtp = (const int*)(base + oT);
   const int* dsp = (const int*)(base + oS);
+  const StripGrid sg{dtp, nf, nb, rows, spf};
   hipLaunchKernelGGL(k_refine_begin, dim3((nf + 63) / 64), dim3(64), 0, st, dstate, (const FieldCG*)(base + oI), nf);
   LAUNCH_CHECK(ctx);
   if (!(p.chroma_aniso > 0.0) && sample_prefix[nf] > 0) {
@@ -417,9 +442,9 @@ void run_refine(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_par
                      (float)p.lambda_c);
   LAUNCH_CHECK(ctx);
   for (int o = 0; o < outer; ++o) {
-    hipLaunchKernelGGL(k_irls_weights, dim3(nb), dim3(kBlock), 0, st, dF, dstate, dtp, nf, nb);
+    hipLaunchKernelGGL(k_irls_weights, dim3(nb), dim3(64), 0, st, dF, dstate, sg);
     LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_grad_init, dim3(nb), dim3(kBlock), 0, st, dF, dstate, dtp, nf, nb, partials);
+    hipLaunchKernelGGL(k_grad_init, dim3(nb), dim3(64), 0, st, dF, dstate, sg, partials);
     LAUNCH_CHECK(ctx);
     hipLaunchKernelGGL(k_reduce_init, dim3(nf), dim3(256), 0, st, dstate, dtp, (const double*)partials);
     LAUNCH_CHECK(ctx);
@@ -430,11 +455,11 @@ void run_refine(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_par
       HIPCHK(ctx, hipEventRecord(evs.first, st));
     }
     for (int it = 0; it < n_inner; ++it) {
-      hipLaunchKernelGGL(k_cg_curv, dim3(nb), dim3(kBlock), 0, st, dF, dstate, dtp, nf, nb, partials, it);
+      hipLaunchKernelGGL(k_cg_curv, dim3(nb), dim3(64), 0, st, dF, dstate, sg, partials, it);
       LAUNCH_CHECK(ctx);
       hipLaunchKernelGGL(k_reduce_curv, dim3(nf), dim3(256), 0, st, dstate, dtp, (const double*)partials, it);
       LAUNCH_CHECK(ctx);
-      hipLaunchKernelGGL(k_cg_step, dim3(nb), dim3(kBlock), 0, st, dF, dstate, dtp, nf, nb, partials);
+      hipLaunchKernelGGL(k_cg_step, dim3(nb), dim3(64), 0, st, dF, dstate, sg, partials);
       LAUNCH_CHECK(ctx);
       hipLaunchKernelGGL(k_reduce_step, dim3(nf), dim3(256), 0, st, dstate, dtp, (const double*)partials);
       LAUNCH_CHECK(ctx);
@@ -447,7 +472,8 @@ void run_refine(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_par
     hipLaunchKernelGGL(k_stage_end, dim3((nf + 63) / 64), dim3(64), 0, st, dstate, nf, o);
     LAUNCH_CHECK(ctx);
   }
-  hipLaunchKernelGGL(k_refine_finalize, dim3(nb), dim3(kBlock), 0, st, dF, (const FieldCG*)dstate, dtp, nf, nb);
+  hipLaunchKernelGGL(k_refine_finalize, dim3(nb), dim3(64), 0, st, dF, (const FieldCG*)dstate,
+                     sg);
   LAUNCH_CHECK(ctx);
   arm(ctx, sl, st);
   if (ctx->profiling) {
@@ -475,23 +501,72 @@ int aniso_samples(int h, int w, int is_pal) {
   return (w >= 2 && nt > 0) ? nt * nx : 0;
 }
 
+// Solve groups: consecutive fields whose solver working set (about 60 B per
+// pixel: chi x2, d x2, g, weights, composite) stays within the 256 MB
+// Infinity Cache, so a group's CG iterations re-read their planes from the
+// cache rather than HBM.  Groups run one after another on the stream and
+// share one workspace region.
+#ifndef HVD_SOLVE_GROUP_PX
+#define HVD_SOLVE_GROUP_PX (size_t(3) << 21)
+#endif
+constexpr size_t kSolveGroupPx = HVD_SOLVE_GROUP_PX;  // pixels per solve group
+
+std::vector<std::pair<int, int>> solve_groups(const std::vector<DevField>& F) {
+  std::vector<std::pair<int, int>> g;
+  int b = 0;
+  size_t px = 0;
+  for (int i = 0; i < (int)F.size(); ++i) {
+    const size_t np = (size_t)F[i].h * F[i].w;
+    if (i > b && px + np > kSolveGroupPx) {
+      g.emplace_back(b, i);
+      b = i;
+      px = 0;
+    }
+    px += np;
+  }
+  if (!F.empty()) g.emplace_back(b, (int)F.size());
+  return g;
+}
+
+// One solve group's workspace, carved after `cv`.  Partials are sized for the
+// shortest strips so any strip height the launch picks fits.
+struct GroupLayout {
+  std::vector<size_t> chi1, g, d, w;
+  std::vector<int> sp;  // aniso sample prefix
+  size_t part = 0, gx = 0, gy = 0, st = 0, end = 0;
+};
+
+GroupLayout group_layout(Carve cv, const DevField* F, int n, int is_pal) {
+  GroupLayout L;
+  L.chi1.resize(n);
+  L.g.resize(n);
+  L.d.resize(n);
+  L.w.resize(n);
+  L.sp.assign(n + 1, 0);
+  size_t strips = 0;
+  for (int i = 0; i < n; ++i) {
+    const size_t np = (size_t)F[i].h * F[i].w;
+    L.chi1[i] = cv.take(sizeof(float2) * np);
+    L.g[i] = cv.take(sizeof(float2) * np);
+    L.d[i] = cv.take(sizeof(float2) * np * 2);
+    L.w[i] = cv.take(sizeof(float4) * np);
+    strips += (size_t)solver_strips(F[i].h, F[i].w, kMinStripRows);
+    L.sp[i + 1] = L.sp[i] + aniso_samples(F[i].h, F[i].w, is_pal);
+  }
+  L.part = cv.take(sizeof(double) * 2 * strips);
+  L.gx = cv.take(sizeof(float) * (L.sp[n] + 1));
+  L.gy = cv.take(sizeof(float) * (L.sp[n] + 1));
+  L.st = cv.take(sizeof(FieldCG) * n);
+  L.end = cv.off;
+  return L;
+}
+
 // End offset of the solver workspace refine_planes carves after `cv`.
 size_t refine_ws_end(Carve cv, const std::vector<DevField>& F, int is_pal) {
-  size_t tiles = 0, samples = 0;
-  for (const DevField& f : F) {
-    const size_t np = (size_t)f.h * f.w;
-    cv.take(sizeof(float2) * np);
-    cv.take(sizeof(float2) * np);
-    cv.take(sizeof(float2) * np * 2);
-    cv.take(sizeof(float4) * np);
-    tiles += (size_t)((f.w + kTileW - 1) / kTileW) * ((f.h + kTileH - 1) / kTileH);
-    samples += aniso_samples(f.h, f.w, is_pal);
-  }
-  cv.take(sizeof(double) * 2 * tiles);
-  cv.take(sizeof(float) * (samples + 1));
-  cv.take(sizeof(float) * (samples + 1));
-  cv.take(sizeof(FieldCG) * F.size());
-  return cv.off;
+  size_t end = cv.off;
+  for (const auto& gr : solve_groups(F))
+    end = std::max(end, group_layout(cv, F.data() + gr.first, gr.second - gr.first, is_pal).end);
+  return end;
 }
 
 // Refine workspace for planes already described by DevField (chi[0], s, y and
@@ -739,43 +814,40 @@ int hvd_gpu_holographic_init(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int 
   });
 }
 
+}  // extern "C"
+
 namespace {
 
 // Shared by the plane-level refine and the field driver: carve the solver
 // workspace after `cv` and run.
 void refine_planes(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_params& p, hipStream_t st,
-                   Carve& cv, RefineOut out) {
-  const int nf = (int)F.size();
-  std::vector<size_t> o_chi1(nf), o_g(nf), o_d(nf), o_w(nf);
-  std::vector<int> sp(nf + 1, 0);
-  size_t tiles = 0;
-  for (int i = 0; i < nf; ++i) {
-    const size_t np = (size_t)F[i].h * F[i].w;
-    o_chi1[i] = cv.take(sizeof(float2) * np);
-    o_g[i] = cv.take(sizeof(float2) * np);
-    o_d[i] = cv.take(sizeof(float2) * np * 2);
-    o_w[i] = cv.take(sizeof(float4) * np);
-    tiles += (size_t)((F[i].w + kTileW - 1) / kTileW) * ((F[i].h + kTileH - 1) / kTileH);
-    sp[i + 1] = sp[i] + aniso_samples(F[i].h, F[i].w, p.is_pal);
+                   Carve cv, RefineOut out) {
+  // the caller sized the workspace (its tables point into it already)
+  REQUIRE(ctx, refine_ws_end(cv, F, p.is_pal) <= ctx->ws_cap, "internal: solver workspace not sized");
+  char* ws = ctx->ws;
+  const int outer = std::max(1, p.irls_outer);
+  for (const auto& gr : solve_groups(F)) {
+    const int b = gr.first, n = gr.second - gr.first;
+    const GroupLayout L = group_layout(cv, F.data() + b, n, p.is_pal);
+    std::vector<DevField> G(F.begin() + b, F.begin() + gr.second);
+    for (int i = 0; i < n; ++i) {
+      G[i].chi[1] = (float2*)(ws + L.chi1[i]);
+      G[i].g = (float2*)(ws + L.g[i]);
+      G[i].d[0] = (float2*)(ws + L.d[i]);
+      G[i].d[1] = G[i].d[0] + (size_t)G[i].h * G[i].w;
+      G[i].wts = (float4*)(ws + L.w[i]);
+    }
+    RefineOut o;
+    o.stage_iters = out.stage_iters ? out.stage_iters + (size_t)b * outer : nullptr;
+    o.aniso = out.aniso ? out.aniso + b : nullptr;
+    run_refine(ctx, G, p, (double*)(ws + L.part), (float*)(ws + L.gx), (float*)(ws + L.gy), L.sp,
+               (FieldCG*)(ws + L.st), st, o);
   }
-  const size_t o_part = cv.take(sizeof(double) * 2 * tiles);
-  const size_t o_gx = cv.take(sizeof(float) * (sp[nf] + 1));
-  const size_t o_gy = cv.take(sizeof(float) * (sp[nf] + 1));
-  const size_t o_st = cv.take(sizeof(FieldCG) * nf);
-  char* ws = ensure_ws(ctx, cv.off, st);
-  // table pointers were expressed as offsets until the workspace existed
-  for (int i = 0; i < nf; ++i) {
-    F[i].chi[1] = (float2*)(ws + o_chi1[i]);
-    F[i].g = (float2*)(ws + o_g[i]);
-    F[i].d[0] = (float2*)(ws + o_d[i]);
-    F[i].d[1] = F[i].d[0] + (size_t)F[i].h * F[i].w;
-    F[i].wts = (float4*)(ws + o_w[i]);
-  }
-  run_refine(ctx, F, p, (double*)(ws + o_part), (float*)(ws + o_gx), (float*)(ws + o_gy), sp,
-             (FieldCG*)(ws + o_st), st, out);
 }
 
 }  // namespace
+
+extern "C" {
 
 int hvd_gpu_variational_refine(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hvd_solver_params* p,
                                int* stage_iters, float* aniso, void* stream) {
@@ -821,6 +893,8 @@ int hvd_gpu_variational_refine(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, in
   });
 }
 
+}  // extern "C"
+
 namespace {
 
 // Workspace layout of the field driver, carved after `cv`; `end` covers the
@@ -834,8 +908,7 @@ struct DecodeLayout {
 DecodeLayout decode_layout(int n, const hvd_field_geometry* geom, bool from_raw, Carve cv) {
   DecodeLayout L;
   for (auto* v : {&L.ire, &L.th, &L.vs, &L.s, &L.row, &L.vsa, &L.col, &L.spec, &L.var}) v->resize(n);
-  std::vector<DevField> shapes(n);
-  bool any_pal = false;
+  std::vector<DevField> shapes(n), shapes_pal, shapes_ntsc;
   for (int i = 0; i < n; ++i) {
     const hvd_field_geometry& g = geom[i];
     const int lines = last_line(g) - g.first_active_field_line, width = g.active_video_end - g.active_video_start;
@@ -852,12 +925,11 @@ DecodeLayout decode_layout(int n, const hvd_field_geometry* geom, bool from_raw,
     L.var[i] = cv.take(sizeof(float2) * np * 2);
     shapes[i].h = lines;
     shapes[i].w = width;
-    any_pal = any_pal || uses_vswitch(g);
+    (uses_vswitch(g) ? shapes_pal : shapes_ntsc).push_back(shapes[i]);
   }
   L.solver = cv;
-  // the AUTO-aniso sample count is largest with the NTSC 2-line grouping
-  L.end = refine_ws_end(cv, shapes, 0);
-  (void)any_pal;
+  // the solve runs per standard (see decode_fields_run), each from L.solver
+  L.end = std::max(refine_ws_end(cv, shapes_pal, 1), refine_ws_end(cv, shapes_ntsc, 0));
   return L;
 }
 
@@ -1021,6 +1093,8 @@ int decode_fields(hvd_gpu_ctx* ctx, const uint16_t* const* raw, const float* con
 }
 
 }  // namespace
+
+extern "C" {
 
 int hvd_gpu_decode_frame_window_s16(hvd_gpu_ctx* ctx, const int16_t* const* frames, int nframes, int core_begin,
                                     int core_end, const hvd_frame_params* fp, const hvd_solver_params* p,
@@ -1208,6 +1282,8 @@ int hvd_gpu_block_match_sad_u16(hvd_gpu_ctx* ctx, const uint16_t* const* ref, co
   });
 }
 
+}  // extern "C"
+
 namespace {
 
 // motion.cpp:286-298: 0, -1, +1, -2, +2, ... within [lo, hi].
@@ -1227,6 +1303,8 @@ std::vector<int> range_from_centre(int lo, int hi) {
 inline unsigned blocks_for(size_t n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
+
+extern "C" {
 
 int hvd_gpu_estimate_motion(hvd_gpu_ctx* ctx, const float* const* y_ref, const float* const* y_cur, int n, int h,
                             int w, int tile, int search, float* const* dy, float* const* dx, float* const* conf,

@@ -15,15 +15,33 @@
 
 namespace hvdk {
 
-constexpr int kTileW = 64;  // one wavefront per tile row
+constexpr int kTileW = 64;  // one wavefront per tile row (init kernels)
 constexpr int kTileH = 4;
 constexpr int kBlock = kTileW * kTileH;  // 256 threads
+constexpr int kStripCols = 62;  // output columns of a solver strip (64 lanes - 2 halo)
+// Launch geometry of the solver kernels: strip prefix over the batch, and
+// the strips per field when every field has the same size (else 0).
+struct StripGrid {
+  const int* tp;
+  int nf, nb, rows, spf;
+};
+
+// Rows of carrier factors a solver strip of `rows` lines holds, one per lane
+// (the halo row above and the row below); at most 64.
+constexpr int carrier_span(int rows) { return rows + 2; }
+constexpr int kMaxStripRows = 32;
+static_assert(carrier_span(kMaxStripRows) <= 64, "strip carrier rows must fit one wave");
+
+// Solver strips of `rows` lines covering an h x w plane.
+inline int solver_strips(int h, int w, int rows) {
+  return ((w + kStripCols - 1) / kStripCols) * ((h + rows - 1) / rows);
+}
 
 // One field of a batch, as every per-pixel kernel sees it.
 struct DevField {
   int h, w;            // active lines, active width
-  int tiles_x;         // ceil(w / kTileW)
-  int tile0;           // first tile index of this field in the batch
+  int tiles_x;         // strips across: ceil(w / kStripCols)
+  int tile0;           // first strip index of this field in the batch
   const float* s;      // h*w
   const float2* rowph; // h: e^{i theta_y}
   const int8_t* vsw;   // h or null (NTSC)

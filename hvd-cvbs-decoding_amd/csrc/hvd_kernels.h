@@ -129,15 +129,15 @@ __global__ void k_aniso_samples(const DevField* F, const FieldCG* st, const int*
                                 float* gy);
 __global__ void k_aniso_select(FieldCG* st, const int* sample_prefix, const float* gx, const float* gy,
                                float lambda_c);
-__global__ void k_irls_weights(const DevField* F, FieldCG* st, const int* tp, int nf, int nb);
-__global__ void k_grad_init(const DevField* F, FieldCG* st, const int* tp, int nf, int nb, double* partials);
-__global__ void k_cg_curv(const DevField* F, FieldCG* st, const int* tp, int nf, int nb, double* partials, int it);
-__global__ void k_cg_step(const DevField* F, FieldCG* st, const int* tp, int nf, int nb, double* partials);
+__global__ void k_irls_weights(const DevField* F, const FieldCG* st, StripGrid sg);
+__global__ void k_grad_init(const DevField* F, const FieldCG* st, StripGrid sg, double* partials);
+__global__ void k_cg_curv(const DevField* F, const FieldCG* st, StripGrid sg, double* partials, int it);
+__global__ void k_cg_step(const DevField* F, const FieldCG* st, StripGrid sg, double* partials);
 __global__ void k_reduce_init(FieldCG* st, const int* tp, const double* partials);
 __global__ void k_reduce_curv(FieldCG* st, const int* tp, const double* partials, int it);
 __global__ void k_reduce_step(FieldCG* st, const int* tp, const double* partials);
 __global__ void k_stage_end(FieldCG* st, int nf, int stage);
-__global__ void k_refine_finalize(const DevField* F, const FieldCG* st, const int* tp, int nf, int nb);
+__global__ void k_refine_finalize(const DevField* F, const FieldCG* st, StripGrid sg);
 
 // bridge.hip
 __global__ void k_split_fields(const BridgeFrameParams* FP, const int16_t* const* frames, float* fields);
