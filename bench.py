@@ -130,6 +130,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only-batched", action="store_true", help="time the headline workload only (profiling)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one 512 MiB device-to-device copy and exit (reference point for PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU layout is described in "
@@ -140,6 +142,14 @@ def main():
     from hvd_amd import gpu, synth
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    if args.calibrate_copy:
+        n = 512 << 20
+        src = torch.ones(n // 4, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copied {n} bytes device to device")
+        return
     eng = gpu.Engine(0)
     g = synth.NTSC
     px_field = g.lines * g.width

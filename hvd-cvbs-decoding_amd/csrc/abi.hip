@@ -375,16 +375,21 @@ constexpr int kMinStripRows = 2;
 // that still give a launch enough waves to cover every CU several times.
 int strip_rows(const std::vector<DevField>& F) {
 #ifdef HVD_FORCE_STRIP_ROWS
-  static_assert(HVD_FORCE_STRIP_ROWS >= 1 && HVD_FORCE_STRIP_ROWS <= kMaxStripRows, "strip rows out of range");
+  static_assert(HVD_FORCE_STRIP_ROWS >= kMinStripRows && HVD_FORCE_STRIP_ROWS <= kMaxStripRows,
+                "strip rows out of range");
   (void)F;
   return HVD_FORCE_STRIP_ROWS;
 #else
-  constexpr long kMinWaves = 2048;  // 8 per CU
-  for (int rows : {8, 4}) {
-    long waves = 0;
-    for (const DevField& f : F) waves += solver_strips(f.h, f.w, rows);
-    if (waves >= kMinWaves) return rows;
-  }
+  // measured on 64 NTSC fields: 16-line strips re-read the fewest halo rows
+  // once a launch has ~16 waves per CU; below that, shorter strips win
+  auto waves = [&](int rows) {
+    long n = 0;
+    for (const DevField& f : F) n += solver_strips(f.h, f.w, rows);
+    return n;
+  };
+  if (waves(16) >= 4096) return 16;
+  for (int rows : {8, 4})
+    if (waves(rows) >= 2048) return rows;
   return kMinStripRows;
 #endif
 }

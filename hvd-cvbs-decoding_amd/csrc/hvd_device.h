@@ -71,6 +71,47 @@ struct FieldCG {
   int pad_;
 };
 
+// Plane accesses through global-address-space pointers: the pointers arrive
+// as generic addresses in descriptor tables, and a generic (flat) access is
+// counted by both the vector-memory and the LDS/scalar counters, so every wait
+// on one becomes a wait on all.  Global accesses wait on the vector counter
+// alone, in issue order.
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const float g_cf;
+typedef __attribute__((address_space(1))) const f2v g_cf2;
+typedef __attribute__((address_space(1))) const f4v g_cf4;
+typedef __attribute__((address_space(1))) const int8_t g_ci8;
+typedef __attribute__((address_space(1))) float g_f;
+typedef __attribute__((address_space(1))) f2v g_f2;
+typedef __attribute__((address_space(1))) f4v g_f4;
+
+__device__ inline float gld(const float* p, size_t i) { return ((g_cf*)p)[i]; }
+__device__ inline int8_t gld(const int8_t* p, size_t i) { return ((g_ci8*)p)[i]; }
+__device__ inline float2 gld(const float2* p, size_t i) {
+  const f2v v = ((g_cf2*)p)[i];
+  return make_float2(v.x, v.y);
+}
+__device__ inline float4 gld(const float4* p, size_t i) {
+  const f4v v = ((g_cf4*)p)[i];
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ inline void gst(float* p, size_t i, float v) { ((g_f*)p)[i] = v; }
+__device__ inline void gst(float2* p, size_t i, float2 v) {
+  f2v t;
+  t.x = v.x;
+  t.y = v.y;
+  ((g_f2*)p)[i] = t;
+}
+__device__ inline void gst(float4* p, size_t i, float4 v) {
+  f4v t;
+  t.x = v.x;
+  t.y = v.y;
+  t.z = v.z;
+  t.w = v.w;
+  ((g_f4*)p)[i] = t;
+}
+
 __device__ inline float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }

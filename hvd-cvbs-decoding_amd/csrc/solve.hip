@@ -66,7 +66,7 @@ struct Carrier {
   int x, ybase, w;
   const float2* plane;  // explicit carrier plane, or null
   __device__ inline float2 at(int y) const {
-    if (plane) return plane[(size_t)y * w + x];
+    if (plane) return gld(plane, (size_t)y * w + x);
     const int l = y - ybase;  // uniform across the wave
     const float2 rp = make_float2(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(rowl.x), l)),
                                   __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rowl.y), l)));
@@ -86,11 +86,11 @@ __device__ inline Carrier carrier_for(const DevField& f, const StripCtx& c) {
   k.rowl = make_float2(0.f, 0.f);
   k.vsl = 1;
   if (!f.carrier) {
-    if (c.valid) k.col = f.colph[c.x];
+    if (c.valid) k.col = gld(f.colph, c.x);
     const int yy = k.ybase + lane_id();
     if (yy >= 0 && yy < f.h) {
-      k.rowl = f.rowph[yy];
-      if (f.vsw) k.vsl = f.vsw[yy];
+      k.rowl = gld(f.rowph, yy);
+      if (f.vsw) k.vsl = gld(f.vsw, yy);
     }
   }
   return k;
@@ -126,7 +126,7 @@ __device__ inline float2 dir_update(float2 g, float beta, float2 d) {
 
 inline __device__ bool row_in(const DevField& f, int y, bool valid) { return valid && y >= 0 && y < f.h; }
 
-// Loads of one row of this lane's column, issued ahead of their use.
+// Loads of one row of this lane's column.
 struct RawRow {
   float2 chi, d;
   float s;
@@ -137,9 +137,9 @@ __device__ inline RawRow raw_row(const DevField& f, const Carrier& k, const floa
   RawRow r{make_float2(0.f, 0.f), make_float2(0.f, 0.f), 0.f};
   if (row_in(f, y, valid)) {
     const size_t i = (size_t)y * f.w + k.x;
-    r.chi = chi[i];
-    if (dd) r.d = dd[i];
-    r.s = f.s[i];
+    r.chi = gld(chi, i);
+    if (dd) r.d = gld(dd, i);
+    r.s = gld(f.s, i);
   }
   return r;
 }
@@ -164,7 +164,7 @@ __device__ inline Sample make_sample(const DevField& f, const Carrier& k, const 
 }
 
 __device__ inline float4 weights_row(const DevField& f, int y, int x, bool valid) {
-  return row_in(f, y, valid) ? f.wts[(size_t)y * f.w + x] : make_float4(0.f, 0.f, 0.f, 0.f);
+  return row_in(f, y, valid) ? gld(f.wts, (size_t)y * f.w + x) : make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // g = -2 lam_y [DxT(wx Dx yc) + DyT(wy Dy yc)] conj(c)
@@ -375,7 +375,7 @@ __global__ void __launch_bounds__(64) k_irls_weights(const DevField* __restrict_
       w.y = s.eps / sqrtf(gy * gy + e2);
       w.z = s.eps_c / sqrtf(mx * mx + s.k * gx * gx + ec2);
       w.w = s.eps_c / sqrtf(my * my + s.k * gy * gy + ec2);
-      f.wts[(size_t)y * f.w + c.x] = w;
+      gst(f.wts, (size_t)y * f.w + c.x, w);
     }
     cur = dn;
   }
@@ -392,8 +392,8 @@ __global__ void __launch_bounds__(64) k_grad_init(const DevField* __restrict__ F
   float2* dd = pick(f.d, s.d_cur);
   double gg = 0.0;
   grad_strip(f, s, c, rows, pick(f.chi, s.chi_cur), nullptr, 0.f, [&](size_t i, float2 g, float2) {
-    f.g[i] = g;
-    dd[i] = make_float2(-g.x, -g.y);
+    gst(f.g, i, g);
+    gst(dd, i, make_float2(-g.x, -g.y));
     gg += (double)g.x * g.x + (double)g.y * g.y;
   });
   publish(partials, F[c.f].tile0, c.lt, 1, 0, gg);
@@ -434,8 +434,8 @@ __global__ void __launch_bounds__(64) k_cg_curv(const DevField* __restrict__ F,
     Raw r{make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
     if (row_in(f, y, c.valid)) {
       const size_t i = (size_t)y * f.w + c.x;
-      r.g = f.g[i];
-      r.d = dsrc[i];
+      r.g = gld(f.g, i);
+      r.d = gld(dsrc, i);
     }
     return r;
   };
@@ -458,13 +458,13 @@ __global__ void __launch_bounds__(64) k_cg_curv(const DevField* __restrict__ F,
   for (int r = 0; r < rows; ++r) {
     const int y = c.y0 + r;
     if (y >= f.h) break;
-    const float4 w = c.out ? f.wts[(size_t)y * f.w + c.x] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 w = c.out ? gld(f.wts, (size_t)y * f.w + c.x) : make_float4(0.f, 0.f, 0.f, 0.f);
     const Row dn = row(raw(y + 1), y + 1);
     const float2 dr = shr(cur.d);
     const float dyr = shr(cur.dy);
     if (c.out) {
       const size_t i = (size_t)y * f.w + c.x;
-      if (upd) ddst[i] = cur.d;
+      if (upd) gst(ddst, i, cur.d);
       const bool hr = c.x + 1 < f.w, hd = y + 1 < f.h;
       float dxdy = 0.f, dydy = 0.f;
       float2 dxc = make_float2(0.f, 0.f), dyc = dxc;
@@ -522,8 +522,8 @@ __global__ void __launch_bounds__(64) k_cg_step(const DevField* __restrict__ F,
   double gg = 0.0;
   grad_strip(f, s, c, rows, pick(f.chi, s.chi_cur), pick(f.d, s.d_cur), s.alpha,
              [&](size_t i, float2 g, float2 cc) {
-    f.g[i] = g;
-    chi_out[i] = cc;
+    gst(f.g, i, g);
+    gst(chi_out, i, cc);
     gg += (double)g.x * g.x + (double)g.y * g.y;
   });
   publish(partials, F[c.f].tile0, c.lt, 1, 0, gg);
@@ -565,9 +565,9 @@ __global__ void __launch_bounds__(64) k_refine_finalize(const DevField* __restri
     const int y = c.y0 + r;
     if (y >= f.h) break;
     const size_t i = (size_t)y * f.w + c.x;
-    const float2 chi = pick(f.chi, cur)[i];
-    if (cur) f.chi[0][i] = chi;
-    if (f.y) f.y[i] = f.s[i] - re_mul(chi, k.at(y));
+    const float2 chi = gld(pick(f.chi, cur), i);
+    if (cur) gst(f.chi[0], i, chi);
+    if (f.y) gst(f.y, i, gld(f.s, i) - re_mul(chi, k.at(y)));
   }
 }
 
