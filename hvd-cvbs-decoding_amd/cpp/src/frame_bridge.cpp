@@ -1,0 +1,241 @@
+// frame_bridge.cpp -- hvd-core's frame-buffer bridge (frame_bridge.cpp) over
+// the GPU library.  Frames are uploaded once; split, field order, decode,
+// weave and requantisation all run on the device.
+#include "frame_bridge.h"
+
+#include <algorithm>
+
+#include "woven.h"
+
+namespace hvd {
+
+using gpu::DevBuf;
+using gpu::Session;
+
+FieldGeometry FieldGeometryFromParams(const FrameParams& fp) {
+  // frame_bridge.cpp:43-58: field geometry from the frame's, with the woven
+  // active-line convention halved.
+  FieldGeometry g;
+  g.field_width = fp.frame_width;
+  g.field_height = fp.field1_lines;
+  g.active_video_start = fp.active_video_start;
+  g.active_video_end = fp.active_video_end;
+  g.colour_burst_start = fp.colour_burst_start;
+  g.colour_burst_end = fp.colour_burst_end;
+  g.first_active_field_line = fp.first_active_frame_line / 2;
+  g.last_active_field_line = fp.last_active_frame_line != 0 ? fp.last_active_frame_line / 2 : 0;
+  g.sample_rate = fp.sample_rate;
+  g.standard = fp.standard;
+  g.subcarrier_hz = fp.subcarrier_hz;
+  return g;
+}
+
+namespace {
+
+size_t FrameSamples(const FrameParams& fp) { return static_cast<size_t>(FrameLineOffset(fp, fp.frame_height)); }
+
+void CheckFrame(const FrameParams& fp) {
+  if (fp.frame_width <= 0 || fp.frame_height <= 0 || fp.field1_lines <= 0 || fp.field1_lines > fp.frame_height)
+    throw gpu::Error("bad frame geometry");
+}
+
+struct DeviceSplit {
+  DevBuf<float> block1, block2;
+};
+
+DeviceSplit Split(Session& ss, const int16_t* dframe, const FrameParams& fp, float zero) {
+  DeviceSplit d;
+  const size_t n = (size_t)fp.field1_lines * fp.frame_width;
+  d.block1.resize(n);
+  d.block2.resize(n);
+  const hvd_frame_params a = gpu::ToAbi(fp);
+  ss.check(hvd_gpu_split_frame_fields(ss.ctx(), dframe, &a, zero, d.block1.get(), d.block2.get(), nullptr),
+           "hvd_gpu_split_frame_fields");
+  return d;
+}
+
+int Vote(Session& ss, const float* b1, const float* b2, const FrameParams& fp) {
+  const hvd_frame_params a = gpu::ToAbi(fp);
+  int v = 0;
+  ss.check(hvd_gpu_field_parity_vote(ss.ctx(), b1, b2, &a, &v, nullptr), "hvd_gpu_field_parity_vote");
+  return v;
+}
+
+YcFrameS16 Output(const FrameParams& fp, size_t n, const DevBuf<int16_t>& lum, const DevBuf<int16_t>& chr,
+                  const DevBuf<double>& u, const DevBuf<double>& v) {
+  YcFrameS16 out;
+  out.width = fp.frame_width;
+  out.height = fp.frame_height;
+  out.chroma_dc = fp.chroma_dc;
+  out.luma.resize(n);
+  out.chroma.resize(n);
+  out.u_plane.resize(n);
+  out.v_plane.resize(n);
+  lum.download(out.luma.data(), n);
+  chr.download(out.chroma.data(), n);
+  u.download(out.u_plane.data(), n);
+  v.download(out.v_plane.data(), n);
+  return out;
+}
+
+// The woven decode of one frame re-woven into the flat layout
+// (frame_bridge.cpp:161-255 and :263-377).
+YcFrameS16 WovenFrame(Session& ss, const int16_t* passthrough_host, const gpu::DeviceFields& f, const FrameParams& fp,
+                      const HvdConfig& cfg, bool chroma_only) {
+  const FieldGeometry g = FieldGeometryFromParams(fp);
+  gpu::WovenResult w = gpu::WovenDecode(ss, f, g, cfg, chroma_only);
+  const size_t nflat = (size_t)fp.frame_width * fp.frame_height;
+  DevBuf<int16_t> src = gpu::Upload(passthrough_host, FrameSamples(fp));
+  DevBuf<int16_t> lum(nflat), chr(nflat);
+  DevBuf<double> u(nflat), v(nflat);
+  const hvd_yc_frame o{lum.get(), chr.get(), u.get(), v.get()};
+  const hvd_frame_params a = gpu::ToAbi(fp);
+  ss.check(hvd_gpu_weave_frame_s16(ss.ctx(), src.get(), &a, w.rows, w.width, chroma_only ? nullptr : w.y.get(),
+                                   w.chroma.get(), w.chi.get(), cfg.chroma_gain * w.acc_gain, &o, nullptr),
+           "hvd_gpu_weave_frame_s16");
+  return Output(fp, nflat, lum, chr, u, v);
+}
+
+}  // namespace
+
+void SplitFrameFields(const int16_t* frame, const FrameParams& fp, FieldInput* top, FieldInput* bottom,
+                      bool field1_is_bottom) {
+  try {
+    CheckFrame(fp);
+    Session& ss = gpu::ThreadSession();
+    DevBuf<int16_t> df = gpu::Upload(frame, FrameSamples(fp));
+    DeviceSplit d = Split(ss, df.get(), fp, fp.blanking_level);
+    top->samples = Plane(fp.field1_lines, fp.frame_width);
+    bottom->samples = Plane(fp.field1_lines, fp.frame_width);
+    d.block1.download(top->samples.data(), top->samples.size());
+    d.block2.download(bottom->samples.data(), bottom->samples.size());
+    top->is_first_field = !field1_is_bottom;
+    bottom->is_first_field = field1_is_bottom;
+  } catch (const std::exception& e) {
+    gpu::Report("SplitFrameFields", e);
+    top->samples = Plane();
+    bottom->samples = Plane();
+  }
+}
+
+int DetectFieldParity(const FieldInput& block1, const FieldInput& block2, const FrameParams& fp) {
+  try {
+    CheckFrame(fp);
+    const size_t n = (size_t)fp.field1_lines * fp.frame_width;
+    if (block1.samples.size() != n || block2.samples.size() != n) throw gpu::Error("blocks do not match the frame");
+    Session& ss = gpu::ThreadSession();
+    DevBuf<float> a = gpu::Upload(block1.samples.data(), n), b = gpu::Upload(block2.samples.data(), n);
+    const int v = Vote(ss, a.get(), b.get(), fp);
+    return v > 0 ? 1 : (v < 0 ? -1 : 0);
+  } catch (const std::exception& e) {
+    gpu::Report("DetectFieldParity", e);
+    return 0;
+  }
+}
+
+bool ResolveField1IsBottom(const FieldInput& block1, const FieldInput& block2, const FrameParams& fp,
+                           const HvdConfig& cfg) {
+  if (cfg.field_order == 1) return false;
+  if (cfg.field_order == 2) return true;
+  return DetectFieldParity(block1, block2, fp) < 0;
+}
+
+YcFrameS16 DecodeFrameBuffer(const int16_t* frame, const FrameParams& fp, const HvdConfig& cfg, HvdEngine& engine) {
+  try {
+    CheckFrame(fp);
+    Session& ss = engine.session();
+    DevBuf<int16_t> df = gpu::Upload(frame, FrameSamples(fp));
+    DeviceSplit d = Split(ss, df.get(), fp, fp.blanking_level);
+    bool f1_bottom = cfg.field_order == 2;
+    if (cfg.field_order == 0) f1_bottom = Vote(ss, d.block1.get(), d.block2.get(), fp) < 0;
+    gpu::DeviceFields f;
+    f.first = d.block1.get();
+    f.second = d.block2.get();
+    f.first_is_first = !f1_bottom;
+    f.second_is_first = f1_bottom;
+    f.field_lines = fp.field1_lines;
+    f.field_width = fp.frame_width;
+    return WovenFrame(ss, frame, f, fp, cfg, false);
+  } catch (const std::exception& e) {
+    gpu::Report("DecodeFrameBuffer", e);
+    return {};
+  }
+}
+
+YcFrameS16 DecodeFrameBuffer(const int16_t* frame, const FrameParams& fp, const HvdConfig& cfg) {
+  HvdEngine engine;
+  return DecodeFrameBuffer(frame, fp, cfg, engine);
+}
+
+YcFrameS16 DecodeYcFrameBuffer(const int16_t* luma, const int16_t* chroma, const FrameParams& fp,
+                               const HvdConfig& cfg, HvdEngine& engine) {
+  try {
+    CheckFrame(fp);
+    Session& ss = engine.session();
+    DevBuf<int16_t> dc = gpu::Upload(chroma, FrameSamples(fp));
+    // chroma re-centred on chroma_dc, on the IRE scale (frame_bridge.cpp:276-297)
+    DeviceSplit d = Split(ss, dc.get(), fp, fp.chroma_dc);
+    gpu::DeviceFields f;
+    f.first = d.block1.get();
+    f.second = d.block2.get();
+    f.first_is_first = cfg.field_order != 2;
+    f.second_is_first = cfg.field_order == 2;
+    f.field_lines = fp.field1_lines;
+    f.field_width = fp.frame_width;
+    return WovenFrame(ss, luma, f, fp, cfg, true);
+  } catch (const std::exception& e) {
+    gpu::Report("DecodeYcFrameBuffer", e);
+    return {};
+  }
+}
+
+YcFrameS16 DecodeYcFrameBuffer(const int16_t* luma, const int16_t* chroma, const FrameParams& fp,
+                               const HvdConfig& cfg) {
+  HvdEngine engine;
+  return DecodeYcFrameBuffer(luma, chroma, fp, cfg, engine);
+}
+
+std::vector<YcFrameS16> DecodeFrameSequenceWindow(const std::vector<const int16_t*>& frames, int core_begin,
+                                                  int core_end, const FrameParams& fp, const HvdConfig& cfg,
+                                                  HvdEngine& engine, int64_t /*base_frame_id*/) {
+  const int nframes = static_cast<int>(frames.size());
+  if (nframes == 0 || core_begin < 0 || core_end > nframes || core_begin >= core_end) return {};
+  try {
+    CheckFrame(fp);
+    if (cfg.selective_3d && cfg.cg_iterations > 0)
+      throw gpu::Error("selective 3-D decoding is not run by this library");
+    Session& ss = engine.session();
+    const size_t nsrc = FrameSamples(fp), nflat = (size_t)fp.frame_width * fp.frame_height;
+    std::vector<DevBuf<int16_t>> df;
+    std::vector<const int16_t*> dptr;
+    for (const int16_t* fr : frames) {
+      df.push_back(gpu::Upload(fr, nsrc));
+      dptr.push_back(df.back().get());
+    }
+    const int ncore = core_end - core_begin;
+    std::vector<DevBuf<int16_t>> lum, chr;
+    std::vector<DevBuf<double>> u, v;
+    std::vector<hvd_yc_frame> outs(ncore);
+    for (int k = 0; k < ncore; ++k) {
+      lum.emplace_back(nflat);
+      chr.emplace_back(nflat);
+      u.emplace_back(nflat);
+      v.emplace_back(nflat);
+      outs[k] = hvd_yc_frame{lum[k].get(), chr[k].get(), u[k].get(), v[k].get()};
+    }
+    const hvd_frame_params a = gpu::ToAbi(fp);
+    const hvd_solver_params sp = gpu::ToAbi(cfg, fp.uses_vswitch());
+    const hvd_bridge_params bp = gpu::BridgeOf(cfg);
+    ss.check(hvd_gpu_decode_frame_window_s16(ss.ctx(), dptr.data(), nframes, core_begin, core_end, &a, &sp, &bp,
+                                             outs.data(), nullptr, nullptr),
+             "hvd_gpu_decode_frame_window_s16");
+    std::vector<YcFrameS16> out;
+    for (int k = 0; k < ncore; ++k) out.push_back(Output(fp, nflat, lum[k], chr[k], u[k], v[k]));
+    return out;
+  } catch (const std::exception& e) {
+    gpu::Report("DecodeFrameSequenceWindow", e);
+    return {};
+  }
+}
+
+}  // namespace hvd

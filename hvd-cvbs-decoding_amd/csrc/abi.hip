@@ -285,6 +285,57 @@ LockinGeom lockin_geom(const hvd_field_geometry& g) {
   return L;
 }
 
+// FieldGeometryFromParams (frame_bridge.cpp:43-58).
+hvd_field_geometry frame_field_geometry(const hvd_frame_params& fp) {
+  hvd_field_geometry g{};
+  g.field_width = fp.frame_width;
+  g.field_height = fp.field1_lines;
+  g.active_video_start = fp.active_video_start;
+  g.active_video_end = fp.active_video_end;
+  g.colour_burst_start = fp.colour_burst_start;
+  g.colour_burst_end = fp.colour_burst_end;
+  g.first_active_field_line = fp.first_active_frame_line / 2;
+  g.last_active_field_line = fp.last_active_frame_line != 0 ? fp.last_active_frame_line / 2 : 0;
+  g.sample_rate = fp.sample_rate;
+  g.subcarrier_hz = fp.subcarrier_hz;
+  g.standard = fp.standard;
+  return g;
+}
+
+void check_frame(hvd_gpu_ctx* ctx, const hvd_frame_params& fp) {
+  const int fw = fp.frame_width, fh = fp.frame_height, f1 = fp.field1_lines;
+  REQUIRE(ctx, fw > 0 && fh > 0 && f1 > 0 && f1 <= fh && fh - f1 <= f1, "bad frame geometry %d x %d (field 1: %d)",
+          fw, fh, f1);
+  REQUIRE(ctx, fp.extra_sample_line_count >= 0 && fp.extra_sample_line_count <= 8, "bad extra-sample table");
+  for (int i = 0; i < fp.extra_sample_line_count; ++i)
+    REQUIRE(ctx, fp.extra_sample_lines[i] >= 0 && fp.extra_sample_lines[i] < fh, "extra-sample line %d outside the frame",
+            fp.extra_sample_lines[i]);
+  REQUIRE(ctx, fp.white_level != fp.blanking_level, "white level equals blanking");
+}
+
+BridgeFrameParams bridge_params(const hvd_frame_params& fp) {
+  BridgeFrameParams bp{};
+  bp.standard = fp.standard;
+  bp.frame_width = fp.frame_width;
+  bp.frame_height = fp.frame_height;
+  bp.field1_lines = fp.field1_lines;
+  bp.active_video_start = fp.active_video_start;
+  bp.active_video_end = fp.active_video_end;
+  bp.colour_burst_start = fp.colour_burst_start;
+  bp.colour_burst_end = fp.colour_burst_end;
+  bp.first_active_frame_line = fp.first_active_frame_line;
+  bp.last_active_frame_line = fp.last_active_frame_line;
+  bp.black_level = fp.black_level;
+  bp.white_level = fp.white_level;
+  bp.blanking_level = fp.blanking_level;
+  bp.chroma_dc = fp.chroma_dc;
+  bp.sample_rate = fp.sample_rate;
+  bp.subcarrier_hz = fp.subcarrier_hz;
+  for (int i = 0; i < 8; ++i) bp.extra_lines[i] = fp.extra_sample_lines[i];
+  bp.extra_count = fp.extra_sample_line_count;
+  return bp;
+}
+
 // ---- launch sequences -------------------------------------------------------------
 
 void run_lockin(hvd_gpu_ctx* ctx, const std::vector<LockinGeom>& G, const std::vector<const float*>& fields,
@@ -625,7 +676,7 @@ void clear_profile(hvd_gpu_ctx* ctx) {
 // ============================================================================
 extern "C" {
 
-int hvd_gpu_abi_version(void) { return 100; }
+int hvd_gpu_abi_version(void) { return 101; }
 
 int hvd_gpu_create(int device, hvd_gpu_ctx** out) {
   if (!out) return HVD_GPU_ERR_INVALID;
@@ -1110,11 +1161,8 @@ int hvd_gpu_decode_frame_window_s16(hvd_gpu_ctx* ctx, const int16_t* const* fram
     REQUIRE(ctx, nframes >= 1 && core_begin >= 0 && core_end <= nframes && core_begin < core_end,
             "bad window: %d frames, core [%d, %d)", nframes, core_begin, core_end);
     check_params(ctx, p);
+    check_frame(ctx, *fp);
     const int fw = fp->frame_width, fh = fp->frame_height, f1 = fp->field1_lines;
-    REQUIRE(ctx, fw > 0 && fh > 0 && f1 > 0 && f1 <= fh && fh - f1 <= f1, "bad frame geometry %d x %d (field 1: %d)",
-            fw, fh, f1);
-    REQUIRE(ctx, fp->extra_sample_line_count >= 0 && fp->extra_sample_line_count <= 8, "bad extra-sample table");
-    REQUIRE(ctx, fp->white_level != fp->blanking_level, "white level equals blanking");
     const int amb_stride = (fp->standard == 1 || fp->standard == 2) ? 4 : 2;
     const bool three_d = p->cg_iterations > 0 &&
                          (b->temporal_strength > 0.0f || (b->temporal_strength == 0.0f && 2 * nframes > amb_stride));
@@ -1123,19 +1171,7 @@ int hvd_gpu_decode_frame_window_s16(hvd_gpu_ctx* ctx, const int16_t* const* fram
               (double)b->temporal_strength);
       throw Fail{HVD_GPU_ERR_UNSUPPORTED};
     }
-    // FieldGeometryFromParams (frame_bridge.cpp:43-58)
-    hvd_field_geometry g{};
-    g.field_width = fw;
-    g.field_height = f1;
-    g.active_video_start = fp->active_video_start;
-    g.active_video_end = fp->active_video_end;
-    g.colour_burst_start = fp->colour_burst_start;
-    g.colour_burst_end = fp->colour_burst_end;
-    g.first_active_field_line = fp->first_active_frame_line / 2;
-    g.last_active_field_line = fp->last_active_frame_line != 0 ? fp->last_active_frame_line / 2 : 0;
-    g.sample_rate = fp->sample_rate;
-    g.subcarrier_hz = fp->subcarrier_hz;
-    g.standard = fp->standard;
+    const hvd_field_geometry g = frame_field_geometry(*fp);
     check_geom(ctx, g, 0);
     check_active(ctx, g, 0);
     const int nf = 2 * nframes, ncore = core_end - core_begin;
@@ -1149,7 +1185,7 @@ int hvd_gpu_decode_frame_window_s16(hvd_gpu_ctx* ctx, const int16_t* const* fram
     Carve cv;
     const size_t o_fields = cv.take(sizeof(float) * (size_t)nf * f1 * fw);
     const size_t o_amps = cv.take(sizeof(float) * nf);
-    const size_t o_dec = cv.take(sizeof(int) + sizeof(float) * 4);
+    const size_t o_dec = cv.take(sizeof(int) * 2 + sizeof(float) * 4);
     const size_t o_y = cv.take(sizeof(float) * (size_t)nf * lines * width);
     const size_t o_chi = cv.take(sizeof(float2) * (size_t)nf * lines * width);
     const DecodeLayout L = decode_layout(nf, geoms.data(), false, cv);
@@ -1157,48 +1193,34 @@ int hvd_gpu_decode_frame_window_s16(hvd_gpu_ctx* ctx, const int16_t* const* fram
     float* fields = (float*)(ws + o_fields);
     float* amps = (float*)(ws + o_amps);
     int* f1_bottom = (int*)(ws + o_dec);
-    float* gain = (float*)(ws + o_dec + sizeof(int));
-    BridgeFrameParams bp{};
-    bp.standard = fp->standard;
-    bp.frame_width = fw;
-    bp.frame_height = fh;
-    bp.field1_lines = f1;
-    bp.active_video_start = fp->active_video_start;
-    bp.active_video_end = fp->active_video_end;
-    bp.colour_burst_start = fp->colour_burst_start;
-    bp.colour_burst_end = fp->colour_burst_end;
-    bp.first_active_frame_line = fp->first_active_frame_line;
-    bp.last_active_frame_line = fp->last_active_frame_line;
-    bp.black_level = fp->black_level;
-    bp.white_level = fp->white_level;
-    bp.blanking_level = fp->blanking_level;
-    bp.chroma_dc = fp->chroma_dc;
-    bp.sample_rate = fp->sample_rate;
-    bp.subcarrier_hz = fp->subcarrier_hz;
-    for (int i = 0; i < 8; ++i) bp.extra_lines[i] = fp->extra_sample_lines[i];
-    bp.extra_count = fp->extra_sample_line_count;
+    int* vote = f1_bottom + 1;
+    float* gain = (float*)(ws + o_dec + 2 * sizeof(int));
+    const BridgeFrameParams bp = bridge_params(*fp);
     std::vector<const int16_t*> fr(frames, frames + nframes);
     std::vector<const float*> fptr(nf);
-    for (int j = 0; j < nf; ++j) fptr[j] = fields + (size_t)j * f1 * fw;
+    std::vector<float*> fout(nf);
+    for (int j = 0; j < nf; ++j) fout[j] = fields + (size_t)j * f1 * fw;
+    for (int j = 0; j < nf; ++j) fptr[j] = fout[j];
     std::vector<LockinGeom> LG(nf, lockin_geom(g));
     std::vector<BridgeOut> bo(ncore);
     for (int k = 0; k < ncore; ++k) bo[k] = BridgeOut{out[k].luma, out[k].chroma, out[k].u_plane, out[k].v_plane};
     Stage stg;
-    const size_t oP = stg.put(&bp, 1), oF = stg.put(fr), oFP = stg.put(fptr), oG = stg.put(LG), oO = stg.put(bo);
+    const size_t oP = stg.put(&bp, 1), oF = stg.put(fr), oFP = stg.put(fptr), oG = stg.put(LG), oO = stg.put(bo),
+                 oFO = stg.put(fout);
     Slot* sl;
     char* base = upload(ctx, stg, st, &sl);
     const BridgeFrameParams* dP = (const BridgeFrameParams*)(base + oP);
     // (1) split every frame into its two IRE fields
     hipLaunchKernelGGL(k_split_fields, dim3((fw + 255) / 256, f1, nframes), dim3(256), 0, st, dP,
-                       (const int16_t* const*)(base + oF), fields);
+                       (const int16_t* const*)(base + oF), fp->blanking_level, (float* const*)(base + oFO));
     LAUNCH_CHECK(ctx);
     // (2) burst amplitude per field, window ACC and field order
     hipLaunchKernelGGL(k_burst_amp, dim3(nf), dim3(256), 0, st, (const LockinGeom*)(base + oG),
                        (const float* const*)(base + oFP), amps);
     LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_window_decisions, dim3(1), dim3(256), 0, st, dP, (const float*)fields, nframes,
+    hipLaunchKernelGGL(k_window_decisions, dim3(1), dim3(256), 0, st, dP, (const float* const*)(base + oFP), nframes,
                        (const float*)amps, b->acc, b->chroma_gain, g.standard == 1 ? 21.43f : 20.0f, b->field_order,
-                       f1_bottom, gain);
+                       f1_bottom, gain, vote);
     LAUNCH_CHECK(ctx);
     arm(ctx, sl, st);
     // (3) the field driver on all fields of the window
@@ -1422,6 +1444,146 @@ int hvd_gpu_estimate_motion(hvd_gpu_ctx* ctx, const float* const* y_ref, const f
       HIPCHK(ctx, hipMemcpyAsync(dy[i], ody + (size_t)i * nt, sizeof(float) * nt, hipMemcpyDeviceToDevice, st));
       HIPCHK(ctx, hipMemcpyAsync(dx[i], odx + (size_t)i * nt, sizeof(float) * nt, hipMemcpyDeviceToDevice, st));
     }
+    arm(ctx, sl, st);
+  });
+}
+
+// ---- frame-bridge and component entry points of the C++ host API ----------
+
+int hvd_gpu_split_frame_fields(hvd_gpu_ctx* ctx, const int16_t* frame, const hvd_frame_params* fp, float zero_code,
+                               float* field1, float* field2, void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, frame && fp && field1 && field2, "null argument");
+    check_frame(ctx, *fp);
+    hipStream_t st = S(stream);
+    const BridgeFrameParams bp = bridge_params(*fp);
+    const int16_t* fr[1] = {frame};
+    float* outs[2] = {field1, field2};
+    Stage stg;
+    const size_t oP = stg.put(&bp, 1), oF = stg.put(fr, 1), oO = stg.put(outs, 2);
+    Slot* sl;
+    char* base = upload(ctx, stg, st, &sl);
+    hipLaunchKernelGGL(k_split_fields, dim3((fp->frame_width + 255) / 256, fp->field1_lines, 1), dim3(256), 0, st,
+                       (const BridgeFrameParams*)(base + oP), (const int16_t* const*)(base + oF), zero_code,
+                       (float* const*)(base + oO));
+    LAUNCH_CHECK(ctx);
+    arm(ctx, sl, st);
+  });
+}
+
+int hvd_gpu_field_parity_vote(hvd_gpu_ctx* ctx, const float* block1, const float* block2, const hvd_frame_params* fp,
+                              int* vote, void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, block1 && block2 && fp && vote, "null argument");
+    check_frame(ctx, *fp);
+    hipStream_t st = S(stream);
+    Carve cv;
+    const size_t o = cv.take(sizeof(int) * 4);
+    char* ws = ensure_ws(ctx, cv.off, st);
+    int* dec = (int*)(ws + o);
+    const BridgeFrameParams bp = bridge_params(*fp);
+    const float* fl[2] = {block1, block2};
+    Stage stg;
+    const size_t oP = stg.put(&bp, 1), oF = stg.put(fl, 2);
+    Slot* sl;
+    char* base = upload(ctx, stg, st, &sl);
+    hipLaunchKernelGGL(k_window_decisions, dim3(1), dim3(256), 0, st, (const BridgeFrameParams*)(base + oP),
+                       (const float* const*)(base + oF), 1, (const float*)nullptr, 0, 1.0f, 20.0f, 0, dec,
+                       (float*)(dec + 2), dec + 1);
+    LAUNCH_CHECK(ctx);
+    arm(ctx, sl, st);
+    int h[2] = {0, 0};
+    HIPCHK(ctx, hipMemcpyAsync(h, dec, sizeof(h), hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    *vote = h[1];
+  });
+}
+
+int hvd_gpu_make_carrier(hvd_gpu_ctx* ctx, const float* theta, const int8_t* vswitch, int lines,
+                         const hvd_field_geometry* g, float* carrier, void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, g && (lines == 0 || (theta && carrier)), "null argument");
+    const int width = g->active_video_end - g->active_video_start;
+    REQUIRE(ctx, lines >= 0 && width >= 0, "bad carrier size %d x %d", lines, width);
+    if (lines == 0 || width == 0) return;
+    hipStream_t st = S(stream);
+    hipLaunchKernelGGL(k_make_carrier, dim3((width + 255) / 256, lines), dim3(256), 0, st, theta, vswitch, width,
+                       g->active_video_start, (float)phase_per_sample(*g), (float2*)carrier);
+    LAUNCH_CHECK(ctx);
+  });
+}
+
+int hvd_gpu_tridiag_smooth(hvd_gpu_ctx* ctx, const float* d, const float* a, int n, float lam, float* out,
+                           void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, n >= 0 && (n == 0 || (d && a && out)), "null argument");
+    if (n == 0) return;
+    hipStream_t st = S(stream);
+    Carve cv;
+    const size_t o = cv.take(sizeof(double) * 5 * (size_t)n);
+    char* ws = ensure_ws(ctx, cv.off, st);
+    hipLaunchKernelGGL(k_tridiag, dim3(1), dim3(64), 0, st, d, a, n, lam, (double*)(ws + o), out);
+    LAUNCH_CHECK(ctx);
+  });
+}
+
+int hvd_gpu_box_blur(hvd_gpu_ctx* ctx, const float* in, int h, int w, int r, float* out, void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, in && out, "null argument");
+    REQUIRE(ctx, h >= 1 && w >= 1 && r >= 0, "bad blur %d x %d, radius %d", h, w, r);
+    hipStream_t st = S(stream);
+    const size_t np = (size_t)h * w;
+    Carve cv;
+    const size_t o_tmp = cv.take(sizeof(float) * 2 * np), o_out = cv.take(sizeof(float) * 2 * np);
+    char* ws = ensure_ws(ctx, cv.off, st);
+    const MotionDesc md{in, in};
+    Stage stg;
+    const size_t oD = stg.put(&md, 1);
+    Slot* sl;
+    char* base = upload(ctx, stg, st, &sl);
+    hipLaunchKernelGGL(k_me_blur_cols, dim3((2 * w + 255) / 256), dim3(256), 0, st, (const MotionDesc*)(base + oD), 1,
+                       h, w, r, (float*)(ws + o_tmp));
+    LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_me_blur_rows, dim3((2 * h + 255) / 256), dim3(256), 0, st, 1, h, w, r,
+                       (const float*)(ws + o_tmp), (float*)(ws + o_out));
+    LAUNCH_CHECK(ctx);
+    arm(ctx, sl, st);
+    HIPCHK(ctx, hipMemcpyAsync(out, ws + o_out, sizeof(float) * np, hipMemcpyDeviceToDevice, st));
+  });
+}
+
+int hvd_gpu_chroma_plane(hvd_gpu_ctx* ctx, int mode, const float* s, const float* y, const float* chi,
+                         const float* carrier, size_t n, float* out, void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, mode == 0 || mode == 1, "bad mode %d", mode);
+    REQUIRE(ctx, out && (mode == 0 ? (s && y) : (chi && carrier)), "null argument");
+    if (n == 0) return;
+    hipStream_t st = S(stream);
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_chroma_plane, dim3(blocks), dim3(256), 0, st, mode, s, y, (const float2*)chi,
+                       (const float2*)carrier, n, out);
+    LAUNCH_CHECK(ctx);
+  });
+}
+
+int hvd_gpu_weave_frame_s16(hvd_gpu_ctx* ctx, const int16_t* src, const hvd_frame_params* fp, int rows, int width,
+                            const float* y, const float* chroma, const float* chi, float gain, const hvd_yc_frame* out,
+                            void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, src && fp && out && out->luma && out->chroma && out->u_plane && out->v_plane, "null argument");
+    REQUIRE(ctx, rows >= 0 && width >= 0 && (rows * width == 0 || (chroma && chi)), "bad woven plane");
+    check_frame(ctx, *fp);
+    hipStream_t st = S(stream);
+    const BridgeFrameParams bp = bridge_params(*fp);
+    Stage stg;
+    const size_t oP = stg.put(&bp, 1);
+    Slot* sl;
+    char* base = upload(ctx, stg, st, &sl);
+    const BridgeOut o{out->luma, out->chroma, out->u_plane, out->v_plane};
+    hipLaunchKernelGGL(k_weave_woven, dim3((fp->frame_width + 255) / 256, fp->frame_height), dim3(256), 0, st,
+                       (const BridgeFrameParams*)(base + oP), src, rows, width, y, chroma, (const float2*)chi, gain,
+                       o);
+    LAUNCH_CHECK(ctx);
     arm(ctx, sl, st);
   });
 }

@@ -26,10 +26,6 @@ __device__ inline long long line_offset(const BridgeFrameParams& fp, int line) {
   return off;
 }
 
-__device__ inline float sample_to_ire(float s, const BridgeFrameParams& fp) {
-  return (s - fp.blanking_level) / ((fp.white_level - fp.blanking_level) / 100.0f);
-}
-
 __device__ inline int16_t clamp_u10(float v) {
   const long r = lroundf(v);
   return (int16_t)(r < 0 ? 0 : (r > 1023 ? 1023 : r));
@@ -41,31 +37,34 @@ __device__ inline int16_t clamp_s10(float v) {
 
 }  // namespace
 
-// SplitFrameFields: frame t -> IRE fields 2t (block 1) and 2t + 1 (block 2),
-// each field1_lines x frame_width; rows past a block's end read blanking.
+// SplitFrameFields: frame t -> fields[2t] (block 1) and fields[2t + 1]
+// (block 2), each field1_lines x frame_width, in IRE relative to the code
+// `zero` (blanking for composite; chroma_dc for a separate chroma channel,
+// frame_bridge.cpp:276-297); rows past a block's end read `zero`.
 // Grid: (ceil(fw / 256), field1_lines, nframes).
-__global__ void k_split_fields(const BridgeFrameParams* FP, const int16_t* const* frames, float* fields) {
+__global__ void k_split_fields(const BridgeFrameParams* FP, const int16_t* const* frames, float zero,
+                               float* const* fields) {
   const BridgeFrameParams& fp = *FP;
   const int t = blockIdx.z, y = blockIdx.y;
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int fw = fp.frame_width, f1 = fp.field1_lines, f2 = fp.frame_height - f1;
   if (x >= fw) return;
   const int16_t* fr = frames[t];
-  const float st = y < f1 ? (float)fr[line_offset(fp, y) + x] : fp.blanking_level;
-  const float sb = y < f2 ? (float)fr[line_offset(fp, f1 + y) + x] : fp.blanking_level;
-  float* top = fields + (size_t)(2 * t) * f1 * fw;
-  float* bot = top + (size_t)f1 * fw;
-  top[(size_t)y * fw + x] = sample_to_ire(st, fp);
-  bot[(size_t)y * fw + x] = sample_to_ire(sb, fp);
+  const float st = y < f1 ? (float)fr[line_offset(fp, y) + x] : zero;
+  const float sb = y < f2 ? (float)fr[line_offset(fp, f1 + y) + x] : zero;
+  const float cpi = (fp.white_level - fp.blanking_level) / 100.0f;
+  fields[2 * t][(size_t)y * fw + x] = (st - zero) / cpi;
+  fields[2 * t + 1][(size_t)y * fw + x] = (sb - zero) / cpi;
 }
 
 // DetectFieldParity for every frame, then the window majority vote and the
 // window ACC gain (median burst amplitude, AccGain with the standard's
-// nominal burst).  One block; out[0] = f1_bottom, out_gain[0] = gain.
-__global__ void __launch_bounds__(256) k_window_decisions(const BridgeFrameParams* FP, const float* fields, int nframes,
-                                                          const float* amps, int acc, float chroma_gain,
+// nominal burst).  One block; out[0] = f1_bottom, out_gain[0] = gain,
+// out_vote[0] = the summed per-frame votes.  `amps` may be null when acc is 0.
+__global__ void __launch_bounds__(256) k_window_decisions(const BridgeFrameParams* FP, const float* const* fields,
+                                                          int nframes, const float* amps, int acc, float chroma_gain,
                                                           float nominal_burst, int field_order, int* out_f1_bottom,
-                                                          float* out_gain) {
+                                                          float* out_gain, int* out_vote) {
   const BridgeFrameParams& fp = *FP;
   const int fw = fp.frame_width, f1 = fp.field1_lines;
   const int fal = fp.first_active_frame_line / 2;
@@ -81,8 +80,8 @@ __global__ void __launch_bounds__(256) k_window_decisions(const BridgeFrameParam
   const int nx = (a1 - a0 >= 4) ? ((a1 - 4 - a0) / 16 + 1) : 0;  // x = a0, a0 + 16, ..., x + 4 <= a1
   const int nr = (lal - 1 - (fal + 1) > 0) ? ((lal - 2 - (fal + 1)) / ystep + 1) : 0;
   for (int t = 0; t < nframes && field_order == 0; ++t) {
-    const float* b1 = fields + (size_t)(2 * t) * f1 * fw;
-    const float* b2 = b1 + (size_t)f1 * fw;
+    const float* b1 = fields[2 * t];
+    const float* b2 = fields[2 * t + 1];
     double ep = 0.0, em = 0.0;
     if (lines >= 8 && width >= 32) {
       for (int k = threadIdx.x; k < nr * nx; k += blockDim.x) {
@@ -113,7 +112,7 @@ __global__ void __launch_bounds__(256) k_window_decisions(const BridgeFrameParam
     __syncthreads();
   }
   // ACC: the element of rank n/2 of the 2 * nframes burst amplitudes
-  const int na = 2 * nframes;
+  const int na = acc ? 2 * nframes : 0;
   for (int i = threadIdx.x; i < na; i += blockDim.x) {
     int rank = 0;
     for (int j = 0; j < na; ++j) rank += (amps[j] < amps[i]) || (amps[j] == amps[i] && j < i);
@@ -125,6 +124,7 @@ __global__ void __launch_bounds__(256) k_window_decisions(const BridgeFrameParam
     float gain = 1.0f;
     if (acc) gain = fminf(fmaxf(nominal_burst / fmaxf(s_sel, 1.0f), 0.5f), 2.0f);
     out_gain[0] = chroma_gain * gain;
+    out_vote[0] = s_vote;
   }
 }
 
@@ -178,6 +178,72 @@ __global__ void k_weave_requant(const BridgeFrameParams* FP, const int16_t* cons
   o.chroma[idx] = chr;
   o.u[idx] = u;
   o.v[idx] = v;
+}
+
+// Parity weave of a WOVEN decode (DecodeFrameBuffer :161-255 and
+// DecodeYcFrameBuffer :263-377): woven row r belongs to flat block r % 2 at
+// field line first_active + r / 2.  Every line first takes `src` (the
+// composite, or the separate luma channel) through FrameLineOffset; inside
+// the active picture luma becomes ClampU10(blank + Y scale) when `y` is given
+// (left as passed through otherwise), chroma ClampS10(C scale), and
+// v / u = gain (Re chi, -Im chi) scale in double.  Grid: (ceil(fw / 256), fh).
+__global__ void k_weave_woven(const BridgeFrameParams* FP, const int16_t* src, int rows, int width, const float* y,
+                              const float* chroma, const float2* chi, float gain, BridgeOut o) {
+  const BridgeFrameParams& fp = *FP;
+  const int line = blockIdx.y;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int fw = fp.frame_width, f1 = fp.field1_lines;
+  if (x >= fw) return;
+  const size_t idx = (size_t)line * fw + x;
+  int16_t lum = src[line_offset(fp, line) + x];
+  int16_t chr = 0;
+  double u = 0.0, v = 0.0;
+  const int fal = fp.first_active_frame_line / 2;
+  const int field = line < f1 ? 0 : 1;
+  const int r = 2 * ((field == 0 ? line : line - f1) - fal) + field;
+  const int c = x - fp.active_video_start;
+  if (r >= 0 && r < rows && c >= 0 && c < width) {
+    const size_t i = (size_t)r * width + c;
+    const float scale = (fp.white_level - fp.blanking_level) / 100.0f;
+    if (y) lum = clamp_u10((fp.blanking_level + (y[i] * scale)));
+    chr = clamp_s10((chroma[i] * scale));
+    const float2 ph = chi[i];
+    v = (double)gain * (double)ph.x * (double)scale;
+    u = (double)gain * (double)(-ph.y) * (double)scale;
+  }
+  o.luma[idx] = lum;
+  o.chroma[idx] = chr;
+  o.u[idx] = u;
+  o.v[idx] = v;
+}
+
+// MakeCarrier / MakeCarrierPal (ntsc_geometry.cpp:29-47, :75-93):
+// c(y, x) = polar(1, theta[y] + w0 (a0 + x)) in float, -conj(c) on lines
+// whose vswitch < 0.  Grid: (ceil(width / 256), lines).
+__global__ void k_make_carrier(const float* theta, const int8_t* vswitch, int width, int a0, float w0,
+                               float2* carrier) {
+  const int yy = blockIdx.y;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= width) return;
+  const float phase = theta[yy] + w0 * (float)(a0 + x);
+  float sn, cs;
+  sincosf(phase, &sn, &cs);
+  if (vswitch && vswitch[yy] < 0) cs = -cs;
+  carrier[(size_t)yy * width + x] = make_float2(cs, sn);
+}
+
+// FrameYc's chroma plane (engine.cpp:181-183, :221-224): mode 0 C = S - Y,
+// mode 1 C = Re[chi c].
+__global__ void k_chroma_plane(int mode, const float* s, const float* yl, const float2* chi, const float2* car,
+                               size_t n, float* out) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    if (mode == 0) {
+      out[i] = s[i] - yl[i];
+    } else {
+      const float2 a = chi[i], b = car[i];
+      out[i] = (a.x * b.x) - (a.y * b.y);
+    }
+  }
 }
 
 }  // namespace hvdk

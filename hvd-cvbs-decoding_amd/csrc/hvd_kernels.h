@@ -140,13 +140,21 @@ __global__ void k_stage_end(FieldCG* st, int nf, int stage);
 __global__ void k_refine_finalize(const DevField* F, const FieldCG* st, StripGrid sg);
 
 // bridge.hip
-__global__ void k_split_fields(const BridgeFrameParams* FP, const int16_t* const* frames, float* fields);
-__global__ void k_window_decisions(const BridgeFrameParams* FP, const float* fields, int nframes, const float* amps,
-                                   int acc, float chroma_gain, float nominal_burst, int field_order,
-                                   int* out_f1_bottom, float* out_gain);
+__global__ void k_split_fields(const BridgeFrameParams* FP, const int16_t* const* frames, float zero,
+                               float* const* fields);
+__global__ void k_window_decisions(const BridgeFrameParams* FP, const float* const* fields, int nframes,
+                                   const float* amps, int acc, float chroma_gain, float nominal_burst, int field_order,
+                                   int* out_f1_bottom, float* out_gain, int* out_vote);
 __global__ void k_weave_requant(const BridgeFrameParams* FP, const int16_t* const* frames, int core_begin,
                                 const BridgeField* dec, const int* f1_bottom, const float* gain_p, int monochrome,
                                 const BridgeOut* out);
+__global__ void k_weave_woven(const BridgeFrameParams* FP, const int16_t* src, int rows, int width, const float* y,
+                              const float* chroma, const float2* chi, float gain, BridgeOut o);
+__global__ void k_make_carrier(const float* theta, const int8_t* vswitch, int width, int a0, float w0,
+                               float2* carrier);
+__global__ void k_chroma_plane(int mode, const float* s, const float* yl, const float2* chi, const float2* car,
+                               size_t n, float* out);
+__global__ void k_tridiag(const float* d, const float* a, int n, float lam, double* scratch, float* out);
 
 // motion.hip
 __global__ void k_sad_bm(const SadDesc* D, const int* blk_prefix, int nf);

@@ -271,6 +271,23 @@ __global__ void __launch_bounds__(256) k_burst_amp(const LockinGeom* G, const fl
   if (threadIdx.x == 0) out[fi] = (float)r;
 }
 
+// TridiagSmooth on its own (lockin.cpp:65-99): FP64 Thomas solve in one lane,
+// rounded to float.  scratch: 5 n doubles.  One block of 64.
+__global__ void k_tridiag(const float* d, const float* a, int n, float lam, double* scratch, float* out) {
+  if (threadIdx.x != 0) return;
+  double* dd = scratch;
+  double* aa = scratch + n;
+  double* x = scratch + 2 * n;
+  double* c = scratch + 3 * n;
+  double* e = scratch + 4 * n;
+  for (int i = 0; i < n; ++i) {
+    dd[i] = d[i];
+    aa[i] = a[i];
+  }
+  tridiag_smooth(dd, aa, n, (double)lam, x, c, e);
+  for (int i = 0; i < n; ++i) out[i] = (float)x[i];
+}
+
 // uint16 codes -> IRE (ntsc_geometry.h:301-304 SampleToIre).
 __global__ void k_u16_to_ire(const uint16_t* raw, size_t n, float blanking, float cpi, float* out) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)

@@ -22,13 +22,16 @@ ABI_SYMBOLS = (
     "hvd_gpu_burst_amplitude_ire", "hvd_gpu_holographic_init", "hvd_gpu_variational_refine",
     "hvd_gpu_decode_fields_u16", "hvd_gpu_decode_fields_ire", "hvd_gpu_block_match_sad_u16",
     "hvd_gpu_set_profiling", "hvd_gpu_cg_timing", "hvd_gpu_estimate_motion",
-    "hvd_gpu_decode_frame_window_s16",
+    "hvd_gpu_decode_frame_window_s16", "hvd_gpu_split_frame_fields", "hvd_gpu_field_parity_vote",
+    "hvd_gpu_make_carrier", "hvd_gpu_tridiag_smooth", "hvd_gpu_box_blur", "hvd_gpu_chroma_plane",
+    "hvd_gpu_weave_frame_s16",
 )
 
 
 def build(arch: str = "gfx950") -> pathlib.Path:
-    """Compile libhvd_gpu.so in-tree with hipcc."""
-    subprocess.run(["make", "-s", "-C", str(PKG), f"ARCH={arch}"], check=True)
+    """Compile, from scratch, libhvd_gpu.so (hipcc, the kernels and the C ABI)
+    and libhvd_core.so (g++, the hvd-core C++ API over it), in-tree."""
+    subprocess.run(["make", "-s", "-B", "-C", str(PKG), f"ARCH={arch}"], check=True)
     return LIB_PATH
 
 
@@ -130,6 +133,14 @@ def load() -> C.CDLL:
         lib.hvd_gpu_decode_frame_window_s16.argtypes = [vp, pp, i, i, i, C.POINTER(FrameParams),
                                                         C.POINTER(SolverParams), C.POINTER(BridgeParams),
                                                         C.POINTER(YcFrame), C.POINTER(C.c_float), vp]
+        lib.hvd_gpu_split_frame_fields.argtypes = [vp, vp, C.POINTER(FrameParams), C.c_float, vp, vp, vp]
+        lib.hvd_gpu_field_parity_vote.argtypes = [vp, vp, vp, C.POINTER(FrameParams), C.POINTER(C.c_int), vp]
+        lib.hvd_gpu_make_carrier.argtypes = [vp, vp, vp, i, C.POINTER(Geometry), vp, vp]
+        lib.hvd_gpu_tridiag_smooth.argtypes = [vp, vp, vp, i, C.c_float, vp, vp]
+        lib.hvd_gpu_box_blur.argtypes = [vp, vp, i, i, i, vp, vp]
+        lib.hvd_gpu_chroma_plane.argtypes = [vp, i, vp, vp, vp, vp, C.c_size_t, vp, vp]
+        lib.hvd_gpu_weave_frame_s16.argtypes = [vp, vp, C.POINTER(FrameParams), i, i, vp, vp, vp, C.c_float,
+                                                C.POINTER(YcFrame), vp]
         lib.hvd_gpu_set_profiling.argtypes = [vp, i]
         lib.hvd_gpu_cg_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                                           C.POINTER(C.c_double)]
@@ -310,6 +321,76 @@ class Engine:
             self.ctx, _ptrs(frames), len(frames), core_begin, core_end, C.byref(fp), C.byref(params),
             C.byref(bridge), arr, C.byref(gain) if want_gain else None, _stream()), "decode_frame_window_s16")
         return outs, (gain.value if want_gain else None)
+
+    # ---- frame-bridge and component pieces ------------------------------------
+    def split_frame_fields(self, frame, fp: FrameParams, zero_code: float):
+        """Block 1 / block 2 of a field-sequential int16 frame (device tensor)
+        as (field1_lines, frame_width) float32 IRE planes."""
+        import torch
+        f1 = [torch.empty((fp.field1_lines, fp.frame_width), dtype=torch.float32, device=frame.device)
+              for _ in range(2)]
+        self._check(self.lib.hvd_gpu_split_frame_fields(self.ctx, frame.data_ptr(), C.byref(fp), zero_code,
+                                                        f1[0].data_ptr(), f1[1].data_ptr(), _stream()),
+                    "split_frame_fields")
+        return f1[0], f1[1]
+
+    def field_parity_vote(self, block1, block2, fp: FrameParams) -> int:
+        v = C.c_int(0)
+        self._check(self.lib.hvd_gpu_field_parity_vote(self.ctx, block1.data_ptr(), block2.data_ptr(), C.byref(fp),
+                                                       C.byref(v), _stream()), "field_parity_vote")
+        return v.value
+
+    def make_carrier(self, theta, vswitch, g):
+        """Explicit (lines, active_width, 2) carrier plane from per-line theta."""
+        import torch
+        lines = theta.shape[0]
+        width = g.active_video_end - g.active_video_start
+        out = torch.empty((lines, width, 2), dtype=torch.float32, device=theta.device)
+        geo = Geometry.of(g)
+        self._check(self.lib.hvd_gpu_make_carrier(self.ctx, theta.data_ptr(),
+                                                  vswitch.data_ptr() if vswitch is not None else None, lines,
+                                                  C.byref(geo), out.data_ptr(), _stream()), "make_carrier")
+        return out
+
+    def tridiag_smooth(self, d, a, lam):
+        import torch
+        out = torch.empty_like(d)
+        self._check(self.lib.hvd_gpu_tridiag_smooth(self.ctx, d.data_ptr(), a.data_ptr(), d.numel(), lam,
+                                                    out.data_ptr(), _stream()), "tridiag_smooth")
+        return out
+
+    def box_blur(self, plane, r):
+        import torch
+        out = torch.empty_like(plane)
+        h, w = plane.shape
+        self._check(self.lib.hvd_gpu_box_blur(self.ctx, plane.data_ptr(), h, w, r, out.data_ptr(), _stream()),
+                    "box_blur")
+        return out
+
+    def chroma_plane(self, mode, s=None, y=None, chi=None, carrier=None):
+        import torch
+        ref = s if mode == 0 else chi
+        n = ref.shape[0] * ref.shape[1]
+        out = torch.empty(ref.shape[:2], dtype=torch.float32, device=ref.device)
+        ptr = lambda t: t.data_ptr() if t is not None else None  # noqa: E731
+        self._check(self.lib.hvd_gpu_chroma_plane(self.ctx, mode, ptr(s), ptr(y), ptr(chi), ptr(carrier), n,
+                                                  out.data_ptr(), _stream()), "chroma_plane")
+        return out
+
+    def weave_frame_s16(self, src, fp: FrameParams, y, chroma, chi, gain):
+        """Woven (rows, width) planes -> flat (luma, chroma, u, v) frame tensors."""
+        import torch
+        dev = src.device
+        n_px = fp.frame_width * fp.frame_height
+        o = (torch.empty(n_px, dtype=torch.int16, device=dev), torch.empty(n_px, dtype=torch.int16, device=dev),
+             torch.empty(n_px, dtype=torch.float64, device=dev), torch.empty(n_px, dtype=torch.float64, device=dev))
+        desc = YcFrame(*[t.data_ptr() for t in o])
+        rows, width = chroma.shape
+        self._check(self.lib.hvd_gpu_weave_frame_s16(self.ctx, src.data_ptr(), C.byref(fp), rows, width,
+                                                     y.data_ptr() if y is not None else None, chroma.data_ptr(),
+                                                     chi.data_ptr(), gain, C.byref(desc), _stream()),
+                    "weave_frame_s16")
+        return o
 
     # ---- motion ---------------------------------------------------------------
     def estimate_motion(self, refs, curs, tile=32, search=8):

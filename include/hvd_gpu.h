@@ -259,6 +259,57 @@ int hvd_gpu_block_match_sad_u16(hvd_gpu_ctx* ctx, const uint16_t* const* ref, co
                                 int h, int w, int bh, int bw, int sy, int sx, int16_t* const* vy, int16_t* const* vx,
                                 int32_t* const* sad, void* stream);
 
+/* ---- frame bridge and component pieces for the C++ host API ------------- */
+
+/* SplitFrameFields (frame_bridge.cpp:61-99; FrameLineOffset,
+ * frame_bridge.h:106-112): the field-sequential int16 frame (device,
+ * FrameLineOffset(fp, frame_height) samples) -> block 1 (flat lines
+ * [0, field1_lines)) and block 2 (the rest) as field1_lines x frame_width IRE
+ * planes, ire = (code - zero_code) / ((white - blanking) / 100); rows past a
+ * block's end read zero_code.  zero_code = blanking for composite, chroma_dc
+ * for a separate chroma channel (DecodeYcFrameBuffer, :276-297). */
+int hvd_gpu_split_frame_fields(hvd_gpu_ctx* ctx, const int16_t* frame, const hvd_frame_params* fp, float zero_code,
+                               float* field1, float* field2, void* stream);
+
+/* DetectFieldParity (frame_bridge.cpp:102-149) on two field1_lines x
+ * frame_width IRE blocks: *vote (HOST) = +1 field 1 top, -1 field 1 bottom,
+ * 0 no confident vote.  Synchronous. */
+int hvd_gpu_field_parity_vote(hvd_gpu_ctx* ctx, const float* block1, const float* block2, const hvd_frame_params* fp,
+                              int* vote, void* stream);
+
+/* MakeCarrier / MakeCarrierPal (engine/ntsc_geometry.cpp:29-47, :75-93): an
+ * explicit lines x active_width interleaved carrier plane from per-line
+ * theta (and vswitch, NULL for NTSC). */
+int hvd_gpu_make_carrier(hvd_gpu_ctx* ctx, const float* theta, const int8_t* vswitch, int lines,
+                         const hvd_field_geometry* g, float* carrier, void* stream);
+
+/* TridiagSmooth (engine/lockin.cpp:65-99): x solving
+ * (diag(a) + lam L) x = a d with Neumann ends; n floats each. */
+int hvd_gpu_tridiag_smooth(hvd_gpu_ctx* ctx, const float* d, const float* a, int n, float lam, float* out,
+                           void* stream);
+
+/* BoxBlur (engine/motion.cpp:21-55): separable (2r + 1)-box mean with
+ * edge-truncated windows and FP64 running sums, h x w floats. */
+int hvd_gpu_box_blur(hvd_gpu_ctx* ctx, const float* in, int h, int w, int r, float* out, void* stream);
+
+/* The chroma plane of hvd::FrameYc: mode 0 C = S - Y (DecodeFrame,
+ * engine.cpp:181-183), mode 1 C = Re[chi c] with an explicit carrier
+ * (DecodeChromaOnly, :221-224); n pixels. */
+int hvd_gpu_chroma_plane(hvd_gpu_ctx* ctx, int mode, const float* s, const float* y, const float* chi,
+                         const float* carrier, size_t n, float* out, void* stream);
+
+/* The re-weave of a woven-frame decode into a field-sequential YcFrameS16
+ * (DecodeFrameBuffer, frame_bridge.cpp:186-254; DecodeYcFrameBuffer
+ * :309-377): every line starts as `src` read through FrameLineOffset (the
+ * composite, or the separate luma channel); in the active picture woven row r
+ * goes to flat block r % 2, field line first_active + r / 2, with luma
+ * ClampU10(blanking + y scale) when `y` is non-NULL (else left as passed
+ * through), chroma ClampS10(chroma scale) and v / u = gain (Re chi, -Im chi)
+ * scale.  y, chroma: rows x width; chi: rows x width x 2. */
+int hvd_gpu_weave_frame_s16(hvd_gpu_ctx* ctx, const int16_t* src, const hvd_frame_params* fp, int rows, int width,
+                            const float* y, const float* chroma, const float* chi, float gain, const hvd_yc_frame* out,
+                            void* stream);
+
 #ifdef __cplusplus
 }
 #endif

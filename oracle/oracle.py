@@ -24,11 +24,15 @@ REF_SRC = pathlib.Path("/root/reference/hvd-core/src")
 def build(with_ref: bool | None = None) -> None:
     """Compile the oracle (and the hvd-core shim when the reference checkout
     is present)."""
-    subprocess.run(["make", "-s", "-C", str(HERE), "all"], check=True)
+    subprocess.run(["make", "-s", "-B", "-C", str(HERE), "all"], check=True)
     if with_ref is None:
         with_ref = REF_SRC.is_dir()
     if with_ref:
-        subprocess.run(["make", "-s", "-C", str(HERE), "ref"], check=True)
+        subprocess.run(["make", "-s", "-B", "-C", str(HERE), "ref"], check=True)
+        # hvd-core's own test programs against the package's C++ API (needs
+        # the package's libhvd_core.so, built first by __graft_entry__.build)
+        if (HERE.parent / "hvd-cvbs-decoding_amd" / "lib" / "libhvd_core.so").exists():
+            subprocess.run(["make", "-s", "-B", "-C", str(HERE), "reftests"], check=True)
 
 
 class Geom(C.Structure):

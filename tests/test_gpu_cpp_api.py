@@ -1,0 +1,217 @@
+"""GPU parity of the hvd-core C++ API (libhvd_core.so: engine/*.h and
+frame_bridge.h over the C ABI) against the FP64 oracle.
+
+tests/cpp/api_driver (built by __graft_entry__.build()) runs one API entry
+point per call on raw files written here; the outputs are held to the same
+bars as the C ABI's own parity tests: floats within TOL_IRE of the oracle,
+requantised 10-bit codes within one code (and almost all exact)."""
+import pathlib
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import TOL_IRE
+
+pytestmark = pytest.mark.gpu
+
+from hvd_amd import synth  # noqa: E402
+import bridge as OB  # noqa: E402
+import oracle as O  # noqa: E402
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+DRIVER = ROOT / "tests" / "cpp" / "bin" / "api_driver"
+
+
+def run(args):
+    if not DRIVER.exists():
+        pytest.fail(f"{DRIVER} is not built (run __graft_entry__.build())")
+    print("api_driver", *args[:2], flush=True)
+    r = subprocess.run([str(DRIVER)] + [str(a) for a in args], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, f"rc={r.returncode}\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}"
+
+
+def frame_geom(std):
+    fpd = synth.FRAME_PAL if std == "pal" else synth.FRAME_NTSC
+    fp_o = OB.FrameParams(**fpd)
+    return fpd, fp_o, OB.field_geometry(fp_o, synth.Geometry)
+
+
+@pytest.mark.parametrize("std,nf", [("ntsc", 3), ("pal", 2)])
+def test_decode_sequence_window(tmp_path, orc64, std, nf):
+    """PrepareFieldObs + HvdEngine::DecodeSequenceWindow (2-D) per field vs
+    the oracle's decode_field (sequence.cpp:229-281)."""
+    fpd, fp_o, g = frame_geom(std)
+    make = synth.pal_field if std == "pal" else synth.ntsc_field
+    fields = [synth.ire_from_raw(make(40 + j)[0]) for j in range(nf)]
+    np.concatenate([f.reshape(-1) for f in fields]).astype(np.float32).tofile(tmp_path / "in.f32")
+    run(["window", std, nf, tmp_path / "in.f32", tmp_path / "out.bin"])
+    out = np.fromfile(tmp_path / "out.bin", np.float32)
+    n = g.lines * g.width
+    cfg = O.Cfg.make(cg_iterations=100, irls_outer=4, fast=0, cg_tol=1e-12, is_pal=int(std == "pal"))
+    for j, f in enumerate(fields):
+        blk = out[j * 3 * n:(j + 1) * 3 * n]
+        y = blk[:n].reshape(g.lines, g.width)
+        chi = blk[n:].reshape(g.lines, g.width, 2)
+        want = orc64.decode_field(f, g, cfg)
+        assert np.abs(y - want["y"]).max() <= TOL_IRE
+        assert np.abs(chi[..., 0] + 1j * chi[..., 1] - want["chi"]).max() <= TOL_IRE
+
+
+def _read_frames(path, fpd, count):
+    n = fpd["frame_width"] * fpd["frame_height"]
+    raw = open(path, "rb").read()
+    out, off = [], 0
+    for _ in range(count):
+        lum = np.frombuffer(raw, np.int16, n, off); off += 2 * n  # noqa: E702
+        chr_ = np.frombuffer(raw, np.int16, n, off); off += 2 * n  # noqa: E702
+        u = np.frombuffer(raw, np.float64, n, off); off += 8 * n  # noqa: E702
+        v = np.frombuffer(raw, np.float64, n, off); off += 8 * n  # noqa: E702
+        out.append((lum, chr_, u, v))
+    return out
+
+
+def _codes_close(got, want):
+    d = np.abs(got.astype(np.int32) - want.astype(np.int32))
+    assert d.max() <= 1
+    assert (d == 0).mean() > 0.999
+
+
+@pytest.mark.parametrize("std,nframes,core", [("ntsc", 2, (0, 2)), ("pal", 1, (0, 1))])
+def test_decode_frame_sequence_window(tmp_path, orc64, std, nframes, core):
+    fpd, fp_o, g = frame_geom(std)
+    frames = synth.frames_s16(fpd, nframes, seed=93)
+    np.concatenate(frames).astype(np.int16).tofile(tmp_path / "in.s16")
+    run(["frames", std, nframes, core[0], core[1], tmp_path / "in.s16", tmp_path / "out.bin"])
+    got = _read_frames(tmp_path / "out.bin", fpd, core[1] - core[0])
+    want, gain, _ = OB.decode_frame_sequence_window(
+        orc64, frames, core[0], core[1], fp_o, O.Cfg.make(cg_iterations=40, irls_outer=4, fast=0, cg_tol=1e-6),
+        synth.Geometry)
+    scale = (fpd["white_level"] - fpd["blanking_level"]) / 100.0
+    for (gl, gc, gu, gv), (wl, wc, wu, wv) in zip(got, want):
+        _codes_close(gl, wl)
+        _codes_close(gc, wc)
+        assert np.abs(gu - wu).max() <= TOL_IRE * scale * max(gain, 1.0)
+        assert np.abs(gv - wv).max() <= TOL_IRE * scale * max(gain, 1.0)
+
+
+def _split(frame, fpd, zero):
+    """SplitFrameFields with a chosen zero code (frame_bridge.cpp:61-99, :276-297)."""
+    fw, fh, f1 = fpd["frame_width"], fpd["frame_height"], fpd["field1_lines"]
+    cpi = np.float32((fpd["white_level"] - fpd["blanking_level"]) / 100.0)
+    blocks = []
+    for first in (0, f1):
+        b = np.full((f1, fw), np.float32(zero), np.float32)
+        for y in range(f1):
+            if first + y < fh:
+                o = synth.frame_line_offset(fpd, first + y)
+                b[y] = frame[o:o + fw].astype(np.float32)
+        blocks.append(((b - np.float32(zero)) / cpi).astype(np.float32))
+    return blocks
+
+
+def _woven_oracle(orc, std, frame, fpd, g, chroma_only):
+    """The woven 2-D decode of hvd-core's DecodeFrame / DecodeChromaOnly
+    (engine.cpp:53-236) restated on the oracle's functions."""
+    fp_o = OB.FrameParams(**fpd)
+    b1, b2 = _split(frame, fpd, fpd["chroma_dc"] if chroma_only else fpd["blanking_level"])
+    f1_bottom = False if chroma_only else OB.detect_field_parity(b1, b2, fp_o) < 0
+    top, bot = (b2, b1) if f1_bottom else (b1, b2)
+    pal = std == "pal"
+    fal, a0 = g.first_active_field_line, g.active_video_start
+    lines, width = g.lines, g.width
+    s = np.zeros((2 * lines, width), np.float32)
+    s[0::2] = top[fal:fal + lines, a0:a0 + width]
+    s[1::2] = bot[fal:fal + lines, a0:a0 + width]
+    theta = np.zeros(2 * lines)
+    vs = np.ones(2 * lines, np.int8)
+    for k, f in enumerate((top, bot)):
+        if pal:
+            th, v = orc.burst_lockin_phase_pal(f, g)
+            vs[k::2] = v[fal:fal + lines]
+        else:
+            th = orc.burst_lockin_phase(f, g)
+        theta[k::2] = np.asarray(th, np.float32)[fal:fal + lines]
+    car = orc.make_carrier(theta.astype(np.float32), vs if pal else None, g)
+    y0, chi0 = orc.holographic_init(s, car, g.sample_rate)
+    if chroma_only:
+        y, chi = None, chi0
+        chroma = (chi0 * car).real
+    else:
+        y, chi, _, _ = orc.variational_refine(s, car, chi0, O.Cfg.make(cg_iterations=40, irls_outer=4, fast=0,
+                                                                       cg_tol=1e-6, is_pal=int(pal)))
+        chroma = s - y
+    amp = 0.5 * (orc.burst_amplitude_ire(top, g) + orc.burst_amplitude_ire(bot, g))
+    gain = orc.acc_gain(amp, 21.43 if pal else 20.0)
+    return y, chroma, chi, gain
+
+
+def _reweave(src, fpd, y, chroma, chi, gain):
+    fw, fh, f1 = fpd["frame_width"], fpd["frame_height"], fpd["field1_lines"]
+    scale = np.float32((fpd["white_level"] - fpd["blanking_level"]) / 100.0)
+    lum = np.concatenate([src[synth.frame_line_offset(fpd, L):synth.frame_line_offset(fpd, L) + fw]
+                          for L in range(fh)]).astype(np.int16)
+    chr_ = np.zeros(fw * fh, np.int16)
+    u, v = np.zeros(fw * fh), np.zeros(fw * fh)
+    fal, a0 = fpd["first_active_frame_line"] // 2, fpd["active_video_start"]
+    for r in range(chroma.shape[0]):
+        line = (fal + r // 2) if r % 2 == 0 else f1 + fal + r // 2
+        if line >= fh:
+            continue
+        idx = line * fw + a0 + np.arange(chroma.shape[1])
+        if y is not None:
+            lum[idx] = np.clip(OB.lround(np.float32(fpd["blanking_level"]) + y[r] * scale), 0, 1023)
+        chr_[idx] = np.clip(OB.lround(chroma[r] * scale), -1023, 1023)
+        v[idx] = gain * chi[r].real * float(scale)
+        u[idx] = gain * (-chi[r].imag) * float(scale)
+    return lum, chr_, u, v
+
+
+@pytest.mark.parametrize("std", ["ntsc", "pal"])
+def test_decode_frame_buffer(tmp_path, orc64, std):
+    """DecodeFrameBuffer (woven 2-D, frame_bridge.cpp:161-255) vs the oracle's
+    woven decode, plus the lossless split luma + chroma == composite."""
+    fpd, fp_o, g = frame_geom(std)
+    frame = synth.frames_s16(fpd, 1, seed=94)[0]
+    frame.astype(np.int16).tofile(tmp_path / "in.s16")
+    run(["frame", std, tmp_path / "in.s16", tmp_path / "out.bin"])
+    (gl, gc, gu, gv), = _read_frames(tmp_path / "out.bin", fpd, 1)
+    y, chroma, chi, gain = _woven_oracle(orc64, std, frame, fpd, g, False)
+    wl, wc, wu, wv = _reweave(frame, fpd, y, chroma, chi, gain)
+    _codes_close(gl, wl)
+    _codes_close(gc, wc)
+    scale = (fpd["white_level"] - fpd["blanking_level"]) / 100.0
+    assert np.abs(gu - wu).max() <= TOL_IRE * scale * max(gain, 1.0)
+    assert np.abs(gv - wv).max() <= TOL_IRE * scale * max(gain, 1.0)
+    # lossless (frame_bridge_test.cpp:90-96): luma + chroma == composite within
+    # one code wherever the decode wrote; elsewhere the composite passes through
+    fw = fpd["frame_width"]
+    comp = np.concatenate([frame[synth.frame_line_offset(fpd, L):synth.frame_line_offset(fpd, L) + fw]
+                           for L in range(fpd["frame_height"])]).astype(np.int32)
+    written = (wc != 0) | (wu != 0) | (wv != 0)
+    assert np.abs(gl.astype(np.int32)[written] + gc.astype(np.int32)[written] - comp[written]).max() <= 1
+    assert np.array_equal(gl[~written].astype(np.int32), comp[~written])
+
+
+@pytest.mark.parametrize("std", ["ntsc", "pal"])
+def test_decode_yc_frame_buffer(tmp_path, orc64, std):
+    """DecodeYcFrameBuffer (frame_bridge.cpp:263-377): luma passes through,
+    chroma is the woven holographic crop of the re-centred chroma channel."""
+    fpd, fp_o, g = frame_geom(std)
+    comp = synth.frames_s16(fpd, 1, seed=95)[0]
+    rng = np.random.default_rng(96)
+    luma = rng.integers(0, 1024, size=comp.shape).astype(np.int16)
+    # a chroma channel: the composite's subcarrier content re-centred on chroma_dc
+    chroma = np.clip(comp.astype(np.int32) - int(fpd["blanking_level"]) + int(fpd["chroma_dc"]), 0, 1023)
+    chroma = chroma.astype(np.int16)
+    luma.tofile(tmp_path / "l.s16")
+    chroma.tofile(tmp_path / "c.s16")
+    run(["ycframe", std, tmp_path / "l.s16", tmp_path / "c.s16", tmp_path / "out.bin"])
+    (gl, gc, gu, gv), = _read_frames(tmp_path / "out.bin", fpd, 1)
+    _, ch, chi, gain = _woven_oracle(orc64, std, chroma, fpd, g, True)
+    wl, wc, wu, wv = _reweave(luma, fpd, None, ch, chi, gain)
+    assert np.array_equal(gl, wl)
+    _codes_close(gc, wc)
+    scale = (fpd["white_level"] - fpd["blanking_level"]) / 100.0
+    assert np.abs(gu - wu).max() <= TOL_IRE * scale * max(gain, 1.0)
+    assert np.abs(gv - wv).max() <= TOL_IRE * scale * max(gain, 1.0)
