@@ -110,9 +110,11 @@ def _split(frame, fpd, zero):
     return blocks
 
 
-def _woven_oracle(orc, std, frame, fpd, g, chroma_only):
+def _woven_oracle(orc, orc32, std, frame, fpd, g, chroma_only):
     """The woven 2-D decode of hvd-core's DecodeFrame / DecodeChromaOnly
-    (engine.cpp:53-236) restated on the oracle's functions."""
+    (engine.cpp:53-236) restated on the oracle's functions.  The explicit
+    carrier is hvd-core's float MakeCarrier (the FP32 oracle, pinned bit-exact
+    to it), as the woven path carries it; lock-in, init and solve are FP64."""
     fp_o = OB.FrameParams(**fpd)
     b1, b2 = _split(frame, fpd, fpd["chroma_dc"] if chroma_only else fpd["blanking_level"])
     f1_bottom = False if chroma_only else OB.detect_field_parity(b1, b2, fp_o) < 0
@@ -132,7 +134,7 @@ def _woven_oracle(orc, std, frame, fpd, g, chroma_only):
         else:
             th = orc.burst_lockin_phase(f, g)
         theta[k::2] = np.asarray(th, np.float32)[fal:fal + lines]
-    car = orc.make_carrier(theta.astype(np.float32), vs if pal else None, g)
+    car = orc32.make_carrier(theta.astype(np.float32), vs if pal else None, g).astype(np.complex128)
     y0, chi0 = orc.holographic_init(s, car, g.sample_rate)
     if chroma_only:
         y, chi = None, chi0
@@ -168,7 +170,7 @@ def _reweave(src, fpd, y, chroma, chi, gain):
 
 
 @pytest.mark.parametrize("std", ["ntsc", "pal"])
-def test_decode_frame_buffer(tmp_path, orc64, std):
+def test_decode_frame_buffer(tmp_path, orc64, orc32, std):
     """DecodeFrameBuffer (woven 2-D, frame_bridge.cpp:161-255) vs the oracle's
     woven decode, plus the lossless split luma + chroma == composite."""
     fpd, fp_o, g = frame_geom(std)
@@ -176,7 +178,7 @@ def test_decode_frame_buffer(tmp_path, orc64, std):
     frame.astype(np.int16).tofile(tmp_path / "in.s16")
     run(["frame", std, tmp_path / "in.s16", tmp_path / "out.bin"])
     (gl, gc, gu, gv), = _read_frames(tmp_path / "out.bin", fpd, 1)
-    y, chroma, chi, gain = _woven_oracle(orc64, std, frame, fpd, g, False)
+    y, chroma, chi, gain = _woven_oracle(orc64, orc32, std, frame, fpd, g, False)
     wl, wc, wu, wv = _reweave(frame, fpd, y, chroma, chi, gain)
     _codes_close(gl, wl)
     _codes_close(gc, wc)
@@ -194,7 +196,7 @@ def test_decode_frame_buffer(tmp_path, orc64, std):
 
 
 @pytest.mark.parametrize("std", ["ntsc", "pal"])
-def test_decode_yc_frame_buffer(tmp_path, orc64, std):
+def test_decode_yc_frame_buffer(tmp_path, orc64, orc32, std):
     """DecodeYcFrameBuffer (frame_bridge.cpp:263-377): luma passes through,
     chroma is the woven holographic crop of the re-centred chroma channel."""
     fpd, fp_o, g = frame_geom(std)
@@ -208,7 +210,7 @@ def test_decode_yc_frame_buffer(tmp_path, orc64, std):
     chroma.tofile(tmp_path / "c.s16")
     run(["ycframe", std, tmp_path / "l.s16", tmp_path / "c.s16", tmp_path / "out.bin"])
     (gl, gc, gu, gv), = _read_frames(tmp_path / "out.bin", fpd, 1)
-    _, ch, chi, gain = _woven_oracle(orc64, std, chroma, fpd, g, True)
+    _, ch, chi, gain = _woven_oracle(orc64, orc32, std, chroma, fpd, g, True)
     wl, wc, wu, wv = _reweave(luma, fpd, None, ch, chi, gain)
     assert np.array_equal(gl, wl)
     _codes_close(gc, wc)
