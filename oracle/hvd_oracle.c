@@ -862,8 +862,15 @@ static void sv_weights(solver *sv, const C *chi, R eps, R eps_c, R k) {
  * entry and chi on return; y receives S - Re[chi c].  stage_iters (optional,
  * irls_outer entries) receives the CG iterations each stage ran; the return
  * value is the resolved aniso. */
-R OR_FN(variational_refine)(const R *s, const R *carrier_r, int h, int w, const or_cfg *cfg,
-                            R *chi_r, R *y, int *stage_iters) {
+/* As variational_refine, and optionally (stages non-NULL) saves chi at the
+ * start of each IRLS stage (irls_outer planes of h*w complex): the weights
+ * of stage k are a function of stage k's starting chi and S, which is all
+ * the adjoint needs to recompute them.  hh_floor and gg_floor are the two
+ * absolute CG exits (curv <= hh_floor, |g|^2 < gg_floor N); hvd-core's are
+ * 1e-12 and 1e-10.  Tests that need each stage solved to round-off pass
+ * smaller floors. */
+R OR_FN(variational_refine_ex)(const R *s, const R *carrier_r, int h, int w, const or_cfg *cfg,
+                               R *chi_r, R *y, int *stage_iters, R *stages, double hh_floor, double gg_floor) {
   solver sv;
   memset(&sv, 0, sizeof sv);
   sv.h = h; sv.w = w; sv.n = (size_t)h * w; sv.s = s; sv.car = (const C *)carrier_r;
@@ -895,6 +902,7 @@ R OR_FN(variational_refine)(const R *s, const R *carrier_r, int h, int w, const 
   C *g = cpl[4], *gn = cpl[5], *d = cpl[6];
 
   for (int outer = 0; outer < irls_outer; ++outer) {
+    if (stages) memcpy(stages + (size_t)outer * 2 * n, chi, sizeof(C) * n);
     sv_weights(&sv, chi, eps, eps_c, k);
     sv_grad(&sv, chi, g);
     for (size_t i = 0; i < n; ++i) { d[i].re = -g[i].re; d[i].im = -g[i].im; }
@@ -903,7 +911,7 @@ R OR_FN(variational_refine)(const R *s, const R *carrier_r, int h, int w, const 
     int it = 0;
     for (; it < n_inner; ++it) {
       const double hh = sv_curv(&sv, d);
-      if (hh <= 1e-12) break;
+      if (hh <= hh_floor) break;
       const double gd = dot_real(g, d, n);
       const R alpha = (R)(-0.5 * gd / hh);
       for (size_t i = 0; i < n; ++i) { chi[i].re += alpha * d[i].re; chi[i].im += alpha * d[i].im; }
@@ -917,7 +925,7 @@ R OR_FN(variational_refine)(const R *s, const R *carrier_r, int h, int w, const 
       memcpy(g, gn, sizeof(C) * n);
       gg = ggn;
       if (gg < (double)cg_tol * cg_tol * gg0) { ++it; break; }
-      if (gg < 1e-10 * (double)n) { ++it; break; }
+      if (gg < gg_floor * (double)n) { ++it; break; }
     }
     if (stage_iters) stage_iters[outer] = it;
   }
@@ -926,6 +934,180 @@ R OR_FN(variational_refine)(const R *s, const R *carrier_r, int h, int w, const 
   for (int i = 0; i < 10; ++i) free(planes[i]);
   for (int i = 0; i < 7; ++i) free(cpl[i]);
   return aniso;
+}
+
+R OR_FN(variational_refine)(const R *s, const R *carrier_r, int h, int w, const or_cfg *cfg,
+                            R *chi_r, R *y, int *stage_iters) {
+  return OR_FN(variational_refine_ex)(s, carrier_r, h, w, cfg, chi_r, y, stage_iters, NULL, 1e-12, 1e-10);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Adjoint of the 2-D solve (no hvd-core counterpart: the differentiation   */
+/* of VariationalRefine the training path needs).                           */
+/*                                                                          */
+/* Each IRLS stage k (weights w_k = W(chi_{k-1}, S), lagged diffusivity)     */
+/* is taken as solved exactly: r_k(chi_k) = dE_k/dchi = 0.  Implicit         */
+/* differentiation then gives, from gbar = dL/dchi_k:                       */
+/*   a      = (d2E_k)^-1 gbar                  (adjoint solve, same operator) */
+/*   dL/dlam_y -= <a, g_data(chi_k)> / lam_y,  dL/dlam_c -= <a, g_prior> / lam_c */
+/*   dL/dS  += 2 lam_y L_w Re[a c]             (S in the data term)          */
+/*   wbar   = -d<a, r_k>/dw,  chained through W into dL/dS and dL/dchi_{k-1} */
+/* and gbar for stage k-1 is dL/dchi_{k-1}; after stage 1 it is dL/dchi0.    */
+/* The output luma Y = S - Re[chi c] seeds gbar = dL/dchi + dL/dY (-c_re,    */
+/* c_im) and dL/dS = dL/dY.  aniso (an order statistic of chi0) and the      */
+/* stage weights' dependence on nothing else are taken as is: aniso is      */
+/* locally constant.                                                        */
+/* ------------------------------------------------------------------------ */
+
+/* Standard CG on the SPD operator v -> d2E v (= grad with S = 0). */
+static int adj_solve(solver *sv, const C *b, C *x, int iters, double tol, C *r, C *p, C *ap) {
+  const size_t n = sv->n;
+  for (size_t i = 0; i < n; ++i) { x[i].re = 0; x[i].im = 0; r[i] = b[i]; p[i] = b[i]; }
+  double rr = sum_abs2(r, n);
+  const double rr0 = rr;
+  int it = 0;
+  for (; it < iters && rr > tol * tol * rr0 && rr > 1e-28 * (double)n * (rr0 > 0 ? 1.0 : 0.0); ++it) {
+    sv_grad(sv, p, ap);
+    const double pap = dot_real(p, ap, n);
+    if (!(pap > 0)) break;
+    const double a = rr / pap;
+    for (size_t i = 0; i < n; ++i) {
+      x[i].re += (R)(a * p[i].re); x[i].im += (R)(a * p[i].im);
+      r[i].re -= (R)(a * ap[i].re); r[i].im -= (R)(a * ap[i].im);
+    }
+    const double rn = sum_abs2(r, n);
+    const double be = rn / rr;
+    for (size_t i = 0; i < n; ++i) { p[i].re = r[i].re + (R)be * p[i].re; p[i].im = r[i].im + (R)be * p[i].im; }
+    rr = rn;
+  }
+  return it;
+}
+
+/* stages: irls_outer planes of chi at each stage's start (as saved by
+ * variational_refine_ex); chi_final: chi after the last stage; gchi, gy:
+ * dL/dchi and dL/dY of the outputs (gy may be NULL).  Writes dL/dS (gs),
+ * dL/dchi0 (gchi0) and dL/dlambda_y, dL/dlambda_c (glam[2]); returns the
+ * total adjoint CG iterations. */
+int OR_FN(refine_adjoint)(const R *s, const R *carrier_r, int h, int w, const or_cfg *cfg, const R *stages,
+                          const R *chi_final, const R *gchi, const R *gy, int adj_iters, double adj_tol, R *gs,
+                          R *gchi0, double *glam) {
+  solver sv;
+  memset(&sv, 0, sizeof sv);
+  sv.h = h; sv.w = w; sv.n = (size_t)h * w; sv.car = (const C *)carrier_r;
+  const size_t n = sv.n;
+  const C *car = sv.car;
+  const R eps = (R)cfg->charbonnier_eps, eps_c = (R)cfg->chroma_eps, kc = (R)cfg->structure_coupling;
+  const R lambda_c = (R)cfg->lambda_c;
+  const R aniso = OR_FN(resolve_chroma_aniso)(stages, h, w, cfg->is_pal, (R)cfg->chroma_aniso);
+  sv.mu_h = lambda_c * aniso; sv.mu_v = lambda_c; sv.mu_d = 0;
+  sv.lam_y = (R)(cfg->lambda_y > 0 ? cfg->lambda_y : 1.0);
+  const int K = cfg->irls_outer > 1 ? cfg->irls_outer : 1;
+  R *zero = (R *)xmalloc(sizeof(R) * n);
+  memset(zero, 0, sizeof(R) * n);
+  R *planes[16];
+  for (int i = 0; i < 16; ++i) planes[i] = (R *)xmalloc(sizeof(R) * n);
+  sv.wx = planes[0]; sv.wy = planes[1]; sv.wcx = planes[2]; sv.wcy = planes[3];
+  sv.wd1 = planes[4]; sv.wd2 = planes[5]; sv.yc = planes[6]; sv.t1 = planes[7]; sv.t2 = planes[8];
+  sv.img = planes[9];
+  R *q = planes[10], *qx = planes[11], *qy = planes[12], *gxb = planes[13], *gyb = planes[14], *ycb = planes[15];
+  C *cpl[14];
+  for (int i = 0; i < 14; ++i) cpl[i] = (C *)xmalloc(sizeof(C) * n);
+  sv.c1 = cpl[0]; sv.c2 = cpl[1]; sv.c3 = cpl[2]; sv.cp = cpl[3];
+  C *gbar = cpl[4], *a = cpl[5], *r = cpl[6], *p = cpl[7], *ap = cpl[8], *dxa = cpl[9], *dya = cpl[10];
+  C *dxc = cpl[11], *dyc = cpl[12], *tmpc = cpl[13];
+  R *sbar = gs;
+  double lam_y_bar = 0, lam_c_bar = 0;
+  int total = 0;
+  /* seed: Y = S - Re[chi c] */
+  const C *gc = (const C *)gchi;
+  for (size_t i = 0; i < n; ++i) {
+    const R yb = gy ? gy[i] : 0;
+    gbar[i].re = gc[i].re - yb * car[i].re;
+    gbar[i].im = gc[i].im + yb * car[i].im;
+    sbar[i] = yb;
+  }
+  for (int k = K; k >= 1; --k) {
+    const C *chi_prev = (const C *)(stages + (size_t)(k - 1) * 2 * n);
+    const C *chi_k = k == K ? (const C *)chi_final : (const C *)(stages + (size_t)k * 2 * n);
+    /* weights of stage k from its starting chi */
+    sv.s = s;
+    sv_weights(&sv, chi_prev, eps, eps_c, kc);
+    /* adjoint solve with the stage operator (S = 0 makes grad linear) */
+    sv.s = zero;
+    total += adj_solve(&sv, gbar, a, adj_iters, adj_tol, r, p, ap);
+    /* parameter gradients: data and prior parts of grad at chi_k */
+    sv.s = s;
+    sv_grad(&sv, chi_k, tmpc);            /* full grad; sv.img holds the data term's L_w yc factor */
+    double dd = 0, dp = 0;
+    for (size_t i = 0; i < n; ++i) {
+      const double gdr = -(double)sv.img[i] * car[i].re, gdi = (double)sv.img[i] * car[i].im;
+      dd += (double)a[i].re * gdr + (double)a[i].im * gdi;
+      dp += (double)a[i].re * ((double)tmpc[i].re - gdr) + (double)a[i].im * ((double)tmpc[i].im - gdi);
+    }
+    lam_y_bar -= dd / (double)sv.lam_y;
+    lam_c_bar -= dp / (double)lambda_c;
+    /* dL/dS through the data term: 2 lam_y L_w q, q = Re[a c] */
+    for (size_t i = 0; i < n; ++i) q[i] = re_prod(a[i], car[i]);
+    OR_FN(dx)(q, h, w, qx);
+    OR_FN(dy)(q, h, w, qy);
+    for (size_t i = 0; i < n; ++i) sv.t1[i] = sv.wx[i] * qx[i];
+    OR_FN(dxt)(sv.t1, h, w, sv.img);
+    for (size_t i = 0; i < n; ++i) sv.t2[i] = sv.wy[i] * qy[i];
+    OR_FN(dyt)(sv.t2, h, w, sv.t1);
+    for (size_t i = 0; i < n; ++i) sbar[i] += 2 * sv.lam_y * (sv.img[i] + sv.t1[i]);
+    /* weight adjoint at chi_k, then through W(chi_prev, S) */
+    for (size_t i = 0; i < n; ++i) sv.yc[i] = s[i] - re_prod(chi_k[i], car[i]);
+    OR_FN(dx)(sv.yc, h, w, sv.t1);
+    OR_FN(dy)(sv.yc, h, w, sv.t2);
+    cdx(chi_k, h, w, dxc);
+    cdy(chi_k, h, w, dyc);
+    cdx(a, h, w, dxa);
+    cdy(a, h, w, dya);
+    for (size_t i = 0; i < n; ++i) {
+      const R wxb = 2 * sv.lam_y * sv.t1[i] * qx[i];
+      const R wyb = 2 * sv.lam_y * sv.t2[i] * qy[i];
+      const R wcxb = -2 * sv.mu_h * (dxc[i].re * dxa[i].re + dxc[i].im * dxa[i].im);
+      const R wcyb = -2 * sv.mu_v * (dyc[i].re * dya[i].re + dyc[i].im * dya[i].im);
+      /* reuse: stash the weight adjoints in the weight planes */
+      sv.wx[i] = wxb; sv.wy[i] = wyb; sv.wcx[i] = wcxb; sv.wcy[i] = wcyb;
+    }
+    for (size_t i = 0; i < n; ++i) sv.yc[i] = s[i] - re_prod(chi_prev[i], car[i]);
+    OR_FN(dx)(sv.yc, h, w, sv.t1);
+    OR_FN(dy)(sv.yc, h, w, sv.t2);
+    cdx(chi_prev, h, w, dxc);
+    cdy(chi_prev, h, w, dyc);
+    const R e2 = eps * eps, ec2 = eps_c * eps_c;
+    for (size_t i = 0; i < n; ++i) {
+      const R gx = sv.t1[i], gy_ = sv.t2[i];
+      const R mx2 = (R)cabs2(dxc[i]), my2 = (R)cabs2(dyc[i]);
+      const R ux = gx * gx + e2, uy = gy_ * gy_ + e2;
+      const R vx = mx2 + kc * gx * gx + ec2, vy = my2 + kc * gy_ * gy_ + ec2;
+      const R dwx = -eps * gx / (ux * (R)sqrt((double)ux));
+      const R dwy = -eps * gy_ / (uy * (R)sqrt((double)uy));
+      const R fx = -eps_c / (vx * (R)sqrt((double)vx)), fy = -eps_c / (vy * (R)sqrt((double)vy));
+      gxb[i] = sv.wx[i] * dwx + sv.wcx[i] * fx * kc * gx;
+      gyb[i] = sv.wy[i] * dwy + sv.wcy[i] * fy * kc * gy_;
+      dxc[i].re *= sv.wcx[i] * fx; dxc[i].im *= sv.wcx[i] * fx;
+      dyc[i].re *= sv.wcy[i] * fy; dyc[i].im *= sv.wcy[i] * fy;
+    }
+    OR_FN(dxt)(gxb, h, w, ycb);
+    OR_FN(dyt)(gyb, h, w, sv.t1);
+    for (size_t i = 0; i < n; ++i) ycb[i] += sv.t1[i];
+    cdxt(dxc, h, w, dxa);
+    cdyt(dyc, h, w, dya);
+    for (size_t i = 0; i < n; ++i) {
+      sbar[i] += ycb[i];
+      gbar[i].re = -ycb[i] * car[i].re + dxa[i].re + dya[i].re;
+      gbar[i].im = ycb[i] * car[i].im + dxa[i].im + dya[i].im;
+    }
+  }
+  memcpy(gchi0, gbar, sizeof(C) * n);
+  glam[0] = lam_y_bar;
+  glam[1] = lam_c_bar;
+  free(zero);
+  for (int i = 0; i < 16; ++i) free(planes[i]);
+  for (int i = 0; i < 14; ++i) free(cpl[i]);
+  return total;
 }
 
 /* ------------------------------------------------------------------------ */

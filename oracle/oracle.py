@@ -83,7 +83,7 @@ class Oracle:
         for name, res in (("phase_per_sample", C.c_double), ("line_advance", C.c_double),
                           ("burst_amplitude_ire", self.R), ("acc_gain", self.R),
                           ("resolve_chroma_aniso", self.R), ("variational_refine", self.R),
-                          ("decode_field", self.R)):
+                          ("variational_refine_ex", self.R), ("decode_field", self.R)):
             getattr(self.lib, self.pre + name).restype = res
 
     def f(self, name):
@@ -193,6 +193,40 @@ class Oracle:
         iters = np.zeros(max(cfg.irls_outer, 1), np.int32)
         aniso = self.f("variational_refine")(_p(s), _p(cb), h, w, C.byref(cfg), _p(chi), _p(y), _p(iters))
         return y, self.uncbuf(chi), float(aniso), iters
+
+    def variational_refine_stages(self, s, carrier, chi0, cfg: Cfg, hh_floor=1e-12, gg_floor=1e-10):
+        """The solve, also returning chi at the start of every IRLS stage
+        (irls_outer complex planes), as the adjoint consumes them; the floors
+        are the absolute CG exits (hvd-core's by default)."""
+        s = self.arr(s)
+        h, w = s.shape
+        cb = self.cbuf(carrier)
+        chi = self.cbuf(chi0)
+        y = np.zeros_like(s)
+        k = max(cfg.irls_outer, 1)
+        iters = np.zeros(k, np.int32)
+        stages = np.zeros((k, h, 2 * w), self.dt)
+        aniso = self.f("variational_refine_ex")(_p(s), _p(cb), h, w, C.byref(cfg), _p(chi), _p(y), _p(iters),
+                                                _p(stages), C.c_double(hh_floor), C.c_double(gg_floor))
+        return y, self.uncbuf(chi), self.uncbuf(stages), float(aniso), iters
+
+    def refine_adjoint(self, s, carrier, cfg: Cfg, stages, chi_final, gchi, gy=None, adj_iters=2000,
+                       adj_tol=1e-13):
+        """Gradients of L through the solve from dL/dchi and dL/dY:
+        returns (dL/dS, dL/dchi0, [dL/dlambda_y, dL/dlambda_c], adjoint CG
+        iterations)."""
+        s = self.arr(s)
+        h, w = s.shape
+        st = self.cbuf(stages)
+        gs = np.zeros_like(s)
+        gchi0 = np.zeros((h, 2 * w), self.dt)
+        glam = np.zeros(2, np.float64)
+        gyb = None if gy is None else self.arr(gy)
+        it = self.f("refine_adjoint")(_p(s), _p(self.cbuf(carrier)), h, w, C.byref(cfg), _p(st),
+                                      _p(self.cbuf(chi_final)), _p(self.cbuf(gchi)),
+                                      None if gyb is None else _p(gyb), int(adj_iters), C.c_double(adj_tol),
+                                      _p(gs), _p(gchi0), _p(glam))
+        return gs, self.uncbuf(gchi0), glam, int(it)
 
     def prepare_field(self, field_ire, g, chroma_phase_deg=0.0):
         field = self.arr(field_ire)
