@@ -445,40 +445,92 @@ int strip_rows(const std::vector<DevField>& F) {
 #endif
 }
 
+// Strip decomposition of a batch for the solver kernels; fills each field's
+// tiles_x and tile0.
+struct StripSetup {
+  std::vector<int> tp;
+  int nb = 0, rows = 0, spf = 0;
+};
+
+StripSetup strip_setup(std::vector<DevField>& F) {
+  StripSetup S;
+  const int nf = (int)F.size();
+  S.rows = strip_rows(F);
+  S.tp.assign(nf + 1, 0);
+  for (int i = 0; i < nf; ++i) {
+    F[i].tiles_x = (F[i].w + kStripCols - 1) / kStripCols;
+    F[i].tile0 = S.tp[i];
+    S.tp[i + 1] = S.tp[i] + solver_strips(F[i].h, F[i].w, S.rows);
+  }
+  S.nb = S.tp[nf];
+  S.spf = S.tp[1];
+  for (int i = 1; i < nf; ++i)
+    if (F[i].h != F[0].h || F[i].w != F[0].w) S.spf = 0;
+  return S;
+}
+
+// One stage's CG from the current chi: gradient, d = -g, then n_inner
+// curvature / step launches with per-field exits (variational.cpp:519-549).
+void cg_loop(hvd_gpu_ctx* ctx, const DevField* dF, FieldCG* dstate, const StripGrid& sg, double* partials, int nf,
+             int n_inner, hipStream_t st) {
+  hipLaunchKernelGGL(k_grad_init, dim3(sg.nb), dim3(64), 0, st, dF, dstate, sg, partials);
+  LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_reduce_init, dim3(nf), dim3(256), 0, st, dstate, sg.tp, (const double*)partials);
+  LAUNCH_CHECK(ctx);
+  std::pair<hipEvent_t, hipEvent_t> evs{nullptr, nullptr};
+  if (ctx->profiling) {
+    HIPCHK(ctx, hipEventCreate(&evs.first));
+    HIPCHK(ctx, hipEventCreate(&evs.second));
+    HIPCHK(ctx, hipEventRecord(evs.first, st));
+  }
+  for (int it = 0; it < n_inner; ++it) {
+    hipLaunchKernelGGL(k_cg_curv, dim3(sg.nb), dim3(64), 0, st, dF, dstate, sg, partials, it);
+    LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_reduce_curv, dim3(nf), dim3(256), 0, st, dstate, sg.tp, (const double*)partials, it);
+    LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_cg_step, dim3(sg.nb), dim3(64), 0, st, dF, dstate, sg, partials);
+    LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_reduce_step, dim3(nf), dim3(256), 0, st, dstate, sg.tp, (const double*)partials);
+    LAUNCH_CHECK(ctx);
+  }
+  if (ctx->profiling) {
+    HIPCHK(ctx, hipEventRecord(evs.second, st));
+    ctx->cg_events.push_back(evs);
+    ctx->cg_launches.push_back(4LL * n_inner);
+  }
+}
+
+// Per-field CG state of a solve with these parameters (floors: hvd-core's).
+FieldCG field_cg_init(const DevField& f, const hvd_solver_params& p, int n_inner, double tol) {
+  FieldCG s{};
+  s.npix = (double)f.h * (double)f.w;
+  s.tol2 = tol * tol;
+  s.hh_floor = 1e-12;
+  s.gg_floor = 1e-10;
+  s.lam_y = (float)(p.lambda_y > 0.0 ? p.lambda_y : 1.0);
+  s.eps = (float)p.charbonnier_eps;
+  s.eps_c = (float)p.chroma_eps;
+  s.k = (float)p.structure_coupling;
+  s.aniso_forced = (float)p.chroma_aniso;
+  s.is_pal = p.is_pal;
+  s.n_inner = n_inner;
+  s.done = 1;
+  return s;
+}
+
 void run_refine(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_params& p, double* partials,
                 float* gx, float* gy, const std::vector<int>& sample_prefix, FieldCG* dstate, hipStream_t st,
                 RefineOut out) {
   const int nf = (int)F.size();
-  const int rows = strip_rows(F);
-  std::vector<int> tp(nf + 1, 0);
-  for (int i = 0; i < nf; ++i) {
-    F[i].tiles_x = (F[i].w + kStripCols - 1) / kStripCols;
-    F[i].tile0 = tp[i];
-    tp[i + 1] = tp[i] + solver_strips(F[i].h, F[i].w, rows);
-  }
-  const int nb = tp[nf];
-  int spf = tp[1];
-  for (int i = 1; i < nf; ++i)
-    if (F[i].h != F[0].h || F[i].w != F[0].w) spf = 0;
+  const StripSetup SS = strip_setup(F);
+  const std::vector<int>& tp = SS.tp;
+  const int nb = SS.nb, rows = SS.rows, spf = SS.spf;
   const int outer = std::max(1, p.irls_outer);
   const int cg_total = p.fast ? p.cg_iterations * 2 / 3 : p.cg_iterations;
   const int n_inner = std::max(1, cg_total / outer);
   const float tolf = p.cg_tol > 0.0 ? (float)p.cg_tol : (p.fast ? 0.10f : 0.02f);
   std::vector<FieldCG> init(nf);
-  for (int i = 0; i < nf; ++i) {
-    FieldCG s{};
-    s.npix = (double)F[i].h * (double)F[i].w;
-    s.tol2 = (double)tolf * (double)tolf;
-    s.lam_y = (float)(p.lambda_y > 0.0 ? p.lambda_y : 1.0);
-    s.eps = (float)p.charbonnier_eps;
-    s.eps_c = (float)p.chroma_eps;
-    s.k = (float)p.structure_coupling;
-    s.aniso_forced = (float)p.chroma_aniso;
-    s.is_pal = p.is_pal;
-    s.n_inner = n_inner;
-    s.done = 1;
-    init[i] = s;
-  }
+  for (int i = 0; i < nf; ++i) init[i] = field_cg_init(F[i], p, n_inner, (double)tolf);
   Stage stg;
   const size_t oF = stg.put(F), oT = stg.put(tp), oI = stg.put(init), oS = stg.put(sample_prefix);
   Slot* sl;
@@ -498,33 +550,9 @@ void run_refine(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_par
                      (float)p.lambda_c);
   LAUNCH_CHECK(ctx);
   for (int o = 0; o < outer; ++o) {
-    hipLaunchKernelGGL(k_irls_weights, dim3(nb), dim3(64), 0, st, dF, dstate, sg);
+    hipLaunchKernelGGL(k_irls_weights, dim3(nb), dim3(64), 0, st, dF, dstate, sg, o);
     LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_grad_init, dim3(nb), dim3(64), 0, st, dF, dstate, sg, partials);
-    LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_reduce_init, dim3(nf), dim3(256), 0, st, dstate, dtp, (const double*)partials);
-    LAUNCH_CHECK(ctx);
-    std::pair<hipEvent_t, hipEvent_t> evs{nullptr, nullptr};
-    if (ctx->profiling) {
-      HIPCHK(ctx, hipEventCreate(&evs.first));
-      HIPCHK(ctx, hipEventCreate(&evs.second));
-      HIPCHK(ctx, hipEventRecord(evs.first, st));
-    }
-    for (int it = 0; it < n_inner; ++it) {
-      hipLaunchKernelGGL(k_cg_curv, dim3(nb), dim3(64), 0, st, dF, dstate, sg, partials, it);
-      LAUNCH_CHECK(ctx);
-      hipLaunchKernelGGL(k_reduce_curv, dim3(nf), dim3(256), 0, st, dstate, dtp, (const double*)partials, it);
-      LAUNCH_CHECK(ctx);
-      hipLaunchKernelGGL(k_cg_step, dim3(nb), dim3(64), 0, st, dF, dstate, sg, partials);
-      LAUNCH_CHECK(ctx);
-      hipLaunchKernelGGL(k_reduce_step, dim3(nf), dim3(256), 0, st, dstate, dtp, (const double*)partials);
-      LAUNCH_CHECK(ctx);
-    }
-    if (ctx->profiling) {
-      HIPCHK(ctx, hipEventRecord(evs.second, st));
-      ctx->cg_events.push_back(evs);
-      ctx->cg_launches.push_back(4LL * n_inner);
-    }
+    cg_loop(ctx, dF, dstate, sg, partials, nf, n_inner, st);
     hipLaunchKernelGGL(k_stage_end, dim3((nf + 63) / 64), dim3(64), 0, st, dstate, nf, o);
     LAUNCH_CHECK(ctx);
   }
@@ -905,9 +933,186 @@ void refine_planes(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_
 
 extern "C" {
 
-int hvd_gpu_variational_refine(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hvd_solver_params* p,
-                               int* stage_iters, float* aniso, void* stream) {
-  return guarded(ctx, [&] {
+}  // extern "C"
+
+namespace {
+
+// The solve's adjoint (adjoint.hip): stages K..1, each an adjoint CG with the
+// stage's weights (recomputed from its starting chi), then the local and
+// transposed per-pixel passes.  All fields advance together.
+void run_adjoint(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hvd_solver_params& p,
+                 const float* const* stages, const float* const* gchi, const float* const* gy, int adj_iters,
+                 double adj_tol, float* const* gs, float* const* gchi0, double* glam_host, hipStream_t st) {
+  const int K = std::max(1, p.irls_outer);
+  Carve cv;
+  std::vector<size_t> o_row(n), o_col(n), o_vsw(n), o_chi1(n), o_g(n), o_d(n), o_w(n), o_a(n), o_sc(n);
+  std::vector<int> sp(n + 1, 0);
+  size_t strips = 0, maxpx = 0;
+  for (int i = 0; i < n; ++i) {
+    check_plane(ctx, planes[i], i);
+    REQUIRE(ctx, stages[i] && gchi[i] && gs[i] && gchi0[i], "plane %d: null buffer", i);
+    const size_t np = (size_t)planes[i].lines * planes[i].width;
+    maxpx = std::max(maxpx, np);
+    o_row[i] = cv.take(sizeof(float2) * planes[i].lines);
+    o_vsw[i] = cv.take(planes[i].lines);
+    o_col[i] = cv.take(sizeof(float2) * planes[i].width);
+    o_a[i] = cv.take(sizeof(float2) * np);
+    o_chi1[i] = cv.take(sizeof(float2) * np);
+    o_g[i] = cv.take(sizeof(float2) * np);
+    o_d[i] = cv.take(sizeof(float2) * np * 2);
+    o_w[i] = cv.take(sizeof(float4) * np);
+    o_sc[i] = cv.take(sizeof(float) * 6 * np);
+    strips += (size_t)solver_strips(planes[i].lines, planes[i].width, kMinStripRows);
+    sp[i + 1] = sp[i] + aniso_samples(planes[i].lines, planes[i].width, p.is_pal);
+  }
+  const int nbx = (int)((maxpx + 255) / 256);
+  const size_t o_part = cv.take(sizeof(double) * 2 * strips);
+  const size_t o_apart = cv.take(sizeof(double) * 2 * (size_t)nbx * n);
+  const size_t o_glam = cv.take(sizeof(double) * 2 * n);
+  const size_t o_gx = cv.take(sizeof(float) * (sp[n] + 1));
+  const size_t o_gy = cv.take(sizeof(float) * (sp[n] + 1));
+  const size_t o_st = cv.take(sizeof(FieldCG) * n);
+  char* ws = ensure_ws(ctx, cv.off, st);
+  // carrier tables (factored planes) as for the forward
+  std::vector<PrepDesc> prep;
+  std::vector<DevField> Fw(n), Fa(n);
+  std::vector<AdjField> A(n);
+  for (int i = 0; i < n; ++i) {
+    const hvd_gpu_plane& pl = planes[i];
+    const size_t np = (size_t)pl.lines * pl.width;
+    DevField f{};
+    f.h = pl.lines;
+    f.w = pl.width;
+    f.s = pl.s;
+    f.carrier = reinterpret_cast<const float2*>(pl.carrier);
+    if (!pl.carrier) {
+      prep.push_back(plane_tables(pl, (float2*)(ws + o_row[i]), (int8_t*)(ws + o_vsw[i]), (float2*)(ws + o_col[i])));
+      f.rowph = (const float2*)(ws + o_row[i]);
+      f.colph = (const float2*)(ws + o_col[i]);
+      f.vsw = pl.vswitch ? (const int8_t*)(ws + o_vsw[i]) : nullptr;
+    }
+    f.wts = (float4*)(ws + o_w[i]);
+    f.g = (float2*)(ws + o_g[i]);
+    f.d[0] = (float2*)(ws + o_d[i]);
+    f.d[1] = f.d[0] + np;
+    Fw[i] = f;  // weights pass: chi[0] set per stage
+    DevField a = f;
+    a.s = nullptr;  // S = 0: the gradient is the Hessian applied to chi
+    a.chi[0] = (float2*)(ws + o_a[i]);
+    a.chi[1] = (float2*)(ws + o_chi1[i]);
+    a.rhs = reinterpret_cast<const float2*>(gchi0[i]);  // gbar lives in the gchi0 output
+    a.y = nullptr;
+    Fa[i] = a;
+    AdjField& q = A[i];
+    q = AdjField{};
+    q.h = f.h;
+    q.w = f.w;
+    q.s = pl.s;
+    q.rowph = f.rowph;
+    q.colph = f.colph;
+    q.vsw = f.vsw;
+    q.carrier = f.carrier;
+    q.abar = a.chi[0];
+    q.wts = f.wts;
+    q.gbar = reinterpret_cast<float2*>(gchi0[i]);
+    q.sbar = gs[i];
+    q.scratch = (float*)(ws + o_sc[i]);
+    q.gchi = reinterpret_cast<const float2*>(gchi[i]);
+    q.gy = gy ? gy[i] : nullptr;
+  }
+  run_prepare(ctx, prep, false, st);
+  const StripSetup SS = strip_setup(Fw);
+  for (int i = 0; i < n; ++i) {
+    Fa[i].tiles_x = Fw[i].tiles_x;
+    Fa[i].tile0 = Fw[i].tile0;
+  }
+  std::vector<FieldCG> init(n);
+  for (int i = 0; i < n; ++i) {
+    init[i] = field_cg_init(Fw[i], p, adj_iters, adj_tol);
+    init[i].hh_floor = 0.0;  // the adjoint's right-hand side may be tiny: relative exit only
+    init[i].gg_floor = 0.0;
+  }
+  FieldCG* dstate = (FieldCG*)(ws + o_st);
+  double* partials = (double*)(ws + o_part);
+  double* apart = (double*)(ws + o_apart);
+  double* glam = (double*)(ws + o_glam);
+  // per stage, the weights table reads that stage's starting chi
+  std::vector<std::vector<DevField>> FwK(K, Fw);
+  std::vector<std::vector<AdjField>> AK(K, A);
+  for (int k = 0; k < K; ++k)
+    for (int i = 0; i < n; ++i) {
+      const size_t np = (size_t)planes[i].lines * planes[i].width;
+      const float2* chip = reinterpret_cast<const float2*>(stages[i]) + (size_t)k * np;
+      // both ping-pong slots: the weights kernel reads chi[chi_cur], and the
+      // adjoint CG leaves chi_cur at either value
+      FwK[k][i].chi[0] = const_cast<float2*>(chip);
+      FwK[k][i].chi[1] = const_cast<float2*>(chip);
+      AK[k][i].chip = chip;
+      AK[k][i].chik = k + 1 < K ? reinterpret_cast<const float2*>(stages[i]) + (size_t)(k + 1) * np
+                                : reinterpret_cast<const float2*>(planes[i].chi);
+    }
+  Stage stg;
+  const size_t oT = stg.put(SS.tp), oI = stg.put(init), oS = stg.put(sp), oFa = stg.put(Fa);
+  std::vector<size_t> oFw(K), oA(K);
+  for (int k = 0; k < K; ++k) {
+    oFw[k] = stg.put(FwK[k]);
+    oA[k] = stg.put(AK[k]);
+  }
+  Slot* sl;
+  char* base = upload(ctx, stg, st, &sl);
+  const int* dtp = (const int*)(base + oT);
+  const StripGrid sg{dtp, n, SS.nb, SS.rows, SS.spf};
+  const DevField* dFa = (const DevField*)(base + oFa);
+  hipLaunchKernelGGL(k_refine_begin, dim3((n + 63) / 64), dim3(64), 0, st, dstate, (const FieldCG*)(base + oI), n);
+  LAUNCH_CHECK(ctx);
+  // the forward's aniso, from chi0 (stage 1's start)
+  if (!(p.chroma_aniso > 0.0) && sp[n] > 0) {
+    const int blocks = std::min(4096, (sp[n] + 255) / 256);
+    hipLaunchKernelGGL(k_aniso_samples, dim3(blocks), dim3(256), 0, st, (const DevField*)(base + oFw[0]),
+                       (const FieldCG*)dstate, (const int*)(base + oS), n, (float*)(ws + o_gx), (float*)(ws + o_gy));
+    LAUNCH_CHECK(ctx);
+  }
+  hipLaunchKernelGGL(k_aniso_select, dim3(n), dim3(256), 0, st, dstate, (const int*)(base + oS),
+                     (const float*)(ws + o_gx), (const float*)(ws + o_gy), (float)p.lambda_c);
+  LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_zero_d, dim3((2 * n + 255) / 256), dim3(256), 0, st, glam, 2 * n);
+  LAUNCH_CHECK(ctx);
+  const dim3 pgrid(nbx, n);
+  hipLaunchKernelGGL(k_adj_seed, pgrid, dim3(256), 0, st, (const AdjField*)(base + oA[K - 1]));
+  LAUNCH_CHECK(ctx);
+  for (int k = K - 1; k >= 0; --k) {
+    hipLaunchKernelGGL(k_irls_weights, dim3(SS.nb), dim3(64), 0, st, (const DevField*)(base + oFw[k]), dstate, sg, 0);
+    LAUNCH_CHECK(ctx);
+    for (int i = 0; i < n; ++i) {
+      const size_t np = (size_t)planes[i].lines * planes[i].width;
+      hipLaunchKernelGGL(k_zero2, dim3((unsigned)std::min<size_t>((np + 255) / 256, 4096)), dim3(256), 0, st,
+                         Fa[i].chi[0], np);
+      LAUNCH_CHECK(ctx);
+      hipLaunchKernelGGL(k_zero2, dim3((unsigned)std::min<size_t>((np + 255) / 256, 4096)), dim3(256), 0, st,
+                         Fa[i].chi[1], np);
+      LAUNCH_CHECK(ctx);
+    }
+    cg_loop(ctx, dFa, dstate, sg, partials, n, adj_iters, st);
+    hipLaunchKernelGGL(k_refine_finalize, dim3(SS.nb), dim3(64), 0, st, dFa, (const FieldCG*)dstate, sg);
+    LAUNCH_CHECK(ctx);
+    const AdjField* dA = (const AdjField*)(base + oA[k]);
+    hipLaunchKernelGGL(k_adj_local, pgrid, dim3(256), 0, st, dA, (const FieldCG*)dstate, nbx, apart);
+    LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_adj_reduce, dim3(n), dim3(256), 0, st, (const FieldCG*)dstate, nbx, (const double*)apart,
+                       glam);
+    LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_adj_transpose, pgrid, dim3(256), 0, st, dA);
+    LAUNCH_CHECK(ctx);
+  }
+  arm(ctx, sl, st);
+  HIPCHK(ctx, hipMemcpyAsync(glam_host, glam, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipStreamSynchronize(st));
+}
+
+// The refine entry points: optional per-stage chi saving for the adjoint.
+void refine_entry(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hvd_solver_params* p,
+                  int* stage_iters, float* aniso, float* const* stages, void* stream) {
+  {
     REQUIRE(ctx, n >= 0 && (n == 0 || planes), "null argument");
     check_params(ctx, p);
     if (n == 0) return;
@@ -929,6 +1134,10 @@ int hvd_gpu_variational_refine(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, in
       f.carrier = reinterpret_cast<const float2*>(planes[i].carrier);
       f.chi[0] = reinterpret_cast<float2*>(planes[i].chi);
       f.y = planes[i].y;
+      if (stages) {
+        REQUIRE(ctx, stages[i], "plane %d: null stage buffer", i);
+        f.save = reinterpret_cast<float2*>(stages[i]);
+      }
       F[i] = f;
     }
     // size the whole call (tables + solver) before any pointer is taken
@@ -946,6 +1155,37 @@ int hvd_gpu_variational_refine(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, in
     }
     run_prepare(ctx, prep, false, st);
     refine_planes(ctx, F, *p, st, cv, RefineOut{stage_iters, aniso});
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int hvd_gpu_variational_refine(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hvd_solver_params* p,
+                               int* stage_iters, float* aniso, void* stream) {
+  return guarded(ctx, [&] { refine_entry(ctx, planes, n, p, stage_iters, aniso, nullptr, stream); });
+}
+
+int hvd_gpu_variational_refine_stages(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n,
+                                      const hvd_solver_params* p, float* const* stages, void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, n == 0 || stages, "null stage array");
+    refine_entry(ctx, planes, n, p, nullptr, nullptr, stages, stream);
+  });
+}
+
+int hvd_gpu_refine_adjoint(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hvd_solver_params* p,
+                           const float* const* stages, const float* const* gchi, const float* const* gy,
+                           int adj_iterations, double adj_tol, float* const* gs, float* const* gchi0, double* glam,
+                           void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, n >= 0 && (n == 0 || (planes && stages && gchi && gs && gchi0 && glam)), "null argument");
+    check_params(ctx, p);
+    REQUIRE(ctx, adj_iterations >= 1, "adj_iterations must be positive");
+    REQUIRE(ctx, adj_tol >= 0.0, "negative adj_tol");
+    if (n == 0) return;
+    run_adjoint(ctx, planes, n, *p, stages, gchi, gy, adj_iterations, adj_tol, gs, gchi0, glam, S(stream));
   });
 }
 

@@ -52,6 +52,11 @@ struct DevField {
   float2* d[2];        // ping-pong search direction
   float4* wts;         // IRLS weights
   float* y;            // output luma
+  // adjoint solves: s == null means the data term's S is 0 (the gradient is
+  // then the Hessian applied to chi) and rhs, when set, is subtracted from
+  // the gradient, so the CG solves Hessian . chi = rhs
+  const float2* rhs;
+  float2* save;        // when set: chi at the start of each IRLS stage (stage-major planes)
 };
 
 // Per-field CG state (variational.cpp:519-549), advanced by the one-block
@@ -59,6 +64,8 @@ struct DevField {
 struct FieldCG {
   double gg, gg0, hh, gd, npix;
   double tol2;
+  double hh_floor;  // stop when curv(d) <= hh_floor (hvd-core: 1e-12)
+  double gg_floor;  // stop when |g|^2 < gg_floor * npix (hvd-core: 1e-10)
   float alpha, beta;
   float mu_h, mu_v, lam_y, eps, eps_c, k;
   float aniso_forced;   // > 0 forces chroma_aniso

@@ -129,7 +129,7 @@ __global__ void k_aniso_samples(const DevField* F, const FieldCG* st, const int*
                                 float* gy);
 __global__ void k_aniso_select(FieldCG* st, const int* sample_prefix, const float* gx, const float* gy,
                                float lambda_c);
-__global__ void k_irls_weights(const DevField* F, const FieldCG* st, StripGrid sg);
+__global__ void k_irls_weights(const DevField* F, const FieldCG* st, StripGrid sg, int stage);
 __global__ void k_grad_init(const DevField* F, const FieldCG* st, StripGrid sg, double* partials);
 __global__ void k_cg_curv(const DevField* F, const FieldCG* st, StripGrid sg, double* partials, int it);
 __global__ void k_cg_step(const DevField* F, const FieldCG* st, StripGrid sg, double* partials);
@@ -155,6 +155,32 @@ __global__ void k_make_carrier(const float* theta, const int8_t* vswitch, int wi
 __global__ void k_chroma_plane(int mode, const float* s, const float* yl, const float2* chi, const float2* car,
                                size_t n, float* out);
 __global__ void k_tridiag(const float* d, const float* a, int n, float lam, double* scratch, float* out);
+
+// One field of the solve's adjoint (adjoint.hip).
+struct AdjField {
+  int h, w;
+  const float* s;
+  const float2* rowph;
+  const float2* colph;
+  const int8_t* vsw;
+  const float2* carrier;  // explicit carrier, or null (factored)
+  const float2* abar;     // adjoint solution of the current stage
+  const float2* chik;     // chi at the end of the stage
+  const float2* chip;     // chi at its start (the weights' argument)
+  const float4* wts;      // the stage's weights
+  float2* gbar;           // dL/dchi: the stage's right-hand side, then dL/dchi_{k-1}
+  float* sbar;            // dL/dS accumulator
+  float* scratch;         // 6 floats per pixel
+  const float2* gchi;     // seeds
+  const float* gy;
+};
+
+__global__ void k_adj_seed(const AdjField* A);
+__global__ void k_adj_local(const AdjField* A, const FieldCG* st, int nbx, double* partials);
+__global__ void k_adj_reduce(const FieldCG* st, int nbx, const double* partials, double* glam);
+__global__ void k_adj_transpose(const AdjField* A);
+__global__ void k_zero2(float2* a, size_t n);
+__global__ void k_zero_d(double* a, int n);
 
 // motion.hip
 __global__ void k_sad_bm(const SadDesc* D, const int* blk_prefix, int nf);

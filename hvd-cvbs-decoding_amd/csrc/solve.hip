@@ -139,7 +139,7 @@ __device__ inline RawRow raw_row(const DevField& f, const Carrier& k, const floa
     const size_t i = (size_t)y * f.w + k.x;
     r.chi = gld(chi, i);
     if (dd) r.d = gld(dd, i);
-    r.s = gld(f.s, i);
+    r.s = f.s ? gld(f.s, i) : 0.f;
   }
   return r;
 }
@@ -218,8 +218,13 @@ __device__ inline void grad_strip(const DevField& f, const FieldCG& st, const St
     pr.chi = shr(cur.chi);
     const float4 wl = shl(wc);
     if (c.out) {
-      const float2 g = grad_from(st, cur, pl, pr, up, dn, wc, wl, wu, c.x > 0, c.x + 1 < f.w, y > 0, y + 1 < f.h);
-      emit((size_t)y * f.w + c.x, g, cur.chi);
+      float2 g = grad_from(st, cur, pl, pr, up, dn, wc, wl, wu, c.x > 0, c.x + 1 < f.w, y > 0, y + 1 < f.h);
+      const size_t i = (size_t)y * f.w + c.x;
+      if (f.rhs) {
+        const float2 b = gld(f.rhs, i);
+        g = make_float2(g.x - b.x, g.y - b.y);
+      }
+      emit(i, g, cur.chi);
     }
     up = cur;
     cur = dn;
@@ -348,7 +353,7 @@ __global__ void k_aniso_select(FieldCG* st, const int* sample_prefix, const floa
 
 // IRLS lagged-diffusivity weights at the current chi (variational.cpp:406-433).
 __global__ void __launch_bounds__(64) k_irls_weights(const DevField* __restrict__ F,
-    const FieldCG* __restrict__ st, StripGrid sg) {
+    const FieldCG* __restrict__ st, StripGrid sg, int stage) {
   const StripCtx c = strip_ctx(F, sg);
   const int rows = sg.rows;
   const DevField f = F[c.f];  // a private copy: stores cannot alias it
@@ -376,6 +381,7 @@ __global__ void __launch_bounds__(64) k_irls_weights(const DevField* __restrict_
       w.z = s.eps_c / sqrtf(mx * mx + s.k * gx * gx + ec2);
       w.w = s.eps_c / sqrtf(my * my + s.k * gy * gy + ec2);
       gst(f.wts, (size_t)y * f.w + c.x, w);
+      if (f.save) gst(f.save + (size_t)stage * f.h * f.w, (size_t)y * f.w + c.x, cur.chi);
     }
     cur = dn;
   }
@@ -500,7 +506,7 @@ __global__ void __launch_bounds__(256) k_reduce_curv(FieldCG* st, const int* tp,
     if (it > 0) s.d_cur ^= 1;
     s.hh = hh;
     s.gd = gd;
-    if (hh <= 1e-12) {
+    if (hh <= s.hh_floor) {
       s.done = 1;
     } else {
       s.alpha = (float)(-0.5 * gd / hh);
@@ -540,7 +546,7 @@ __global__ void __launch_bounds__(256) k_reduce_step(FieldCG* st, const int* tp,
     s.beta = (float)(ggn / (s.gg > 1e-30 ? s.gg : 1e-30));
     s.gg = ggn;
     s.it += 1;
-    if (ggn < s.tol2 * s.gg0 || ggn < 1e-10 * s.npix || s.it >= s.n_inner) s.done = 1;
+    if (ggn < s.tol2 * s.gg0 || ggn < s.gg_floor * s.npix || s.it >= s.n_inner) s.done = 1;
   }
 }
 

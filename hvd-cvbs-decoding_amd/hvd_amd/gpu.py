@@ -24,7 +24,7 @@ ABI_SYMBOLS = (
     "hvd_gpu_set_profiling", "hvd_gpu_cg_timing", "hvd_gpu_estimate_motion",
     "hvd_gpu_decode_frame_window_s16", "hvd_gpu_split_frame_fields", "hvd_gpu_field_parity_vote",
     "hvd_gpu_make_carrier", "hvd_gpu_tridiag_smooth", "hvd_gpu_box_blur", "hvd_gpu_chroma_plane",
-    "hvd_gpu_weave_frame_s16",
+    "hvd_gpu_weave_frame_s16", "hvd_gpu_variational_refine_stages", "hvd_gpu_refine_adjoint",
 )
 
 
@@ -141,6 +141,9 @@ def load() -> C.CDLL:
         lib.hvd_gpu_chroma_plane.argtypes = [vp, i, vp, vp, vp, vp, C.c_size_t, vp, vp]
         lib.hvd_gpu_weave_frame_s16.argtypes = [vp, vp, C.POINTER(FrameParams), i, i, vp, vp, vp, C.c_float,
                                                 C.POINTER(YcFrame), vp]
+        lib.hvd_gpu_variational_refine_stages.argtypes = [vp, C.POINTER(Plane), i, C.POINTER(SolverParams), pp, vp]
+        lib.hvd_gpu_refine_adjoint.argtypes = [vp, C.POINTER(Plane), i, C.POINTER(SolverParams), pp, pp, pp, i,
+                                               C.c_double, pp, pp, C.POINTER(C.c_double), vp]
         lib.hvd_gpu_set_profiling.argtypes = [vp, i]
         lib.hvd_gpu_cg_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                                           C.POINTER(C.c_double)]
@@ -262,6 +265,37 @@ class Engine:
         if want_stats:
             return np.array(iters[:], dtype=np.int32).reshape(n, outer), np.array(aniso[:], dtype=np.float32)
         return None
+
+    def variational_refine_stages(self, planes, params: SolverParams):
+        """The solve, returning chi at the start of every IRLS stage: a list of
+        (irls_outer, lines, width, 2) float32 tensors (the adjoint's input)."""
+        import torch
+        n = len(planes)
+        P = (Plane * n)(*planes)
+        outer = max(params.irls_outer, 1)
+        dev = torch.device("cuda", torch.cuda.current_device())
+        stages = [torch.empty((outer, p.lines, p.width, 2), dtype=torch.float32, device=dev) for p in planes]
+        self._check(self.lib.hvd_gpu_variational_refine_stages(self.ctx, P, n, C.byref(params), _ptrs(stages),
+                                                               _stream()), "variational_refine_stages")
+        return stages
+
+    def refine_adjoint(self, planes, params: SolverParams, stages, gchi, gy=None, adj_iterations=200,
+                       adj_tol=1e-6):
+        """Gradients through the solve (planes[i].chi holds the final chi):
+        returns (dL/dS list, dL/dchi0 list, (n, 2) float64 array of
+        dL/dlambda_y, dL/dlambda_c)."""
+        import torch
+        n = len(planes)
+        P = (Plane * n)(*planes)
+        dev = gchi[0].device
+        gs = [torch.empty((p.lines, p.width), dtype=torch.float32, device=dev) for p in planes]
+        gchi0 = [torch.empty((p.lines, p.width, 2), dtype=torch.float32, device=dev) for p in planes]
+        glam = (C.c_double * (2 * n))()
+        self._check(self.lib.hvd_gpu_refine_adjoint(self.ctx, P, n, C.byref(params), _ptrs(stages), _ptrs(gchi),
+                                                    _ptrs(gy) if gy is not None else None, int(adj_iterations),
+                                                    float(adj_tol), _ptrs(gs), _ptrs(gchi0), glam, _stream()),
+                    "refine_adjoint")
+        return gs, gchi0, np.array(glam[:], dtype=np.float64).reshape(n, 2)
 
     # ---- field driver -------------------------------------------------------
     def decode_fields_u16(self, raws, geoms, params: SolverParams, blanking=20480.0, white=61440.0,

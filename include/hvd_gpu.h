@@ -199,6 +199,32 @@ int hvd_gpu_holographic_init(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int 
 int hvd_gpu_variational_refine(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hvd_solver_params* p,
                                int* stage_iters, float* aniso, void* stream);
 
+/* ---- differentiation of the solve (no hvd-core counterpart) ------------ */
+
+/* hvd_gpu_variational_refine that also saves chi at the start of each IRLS
+ * stage: stages[i] holds irls_outer planes of lines * width * 2 floats, stage
+ * major.  These, S and the carrier are all the adjoint needs (the stage
+ * weights are recomputed from them rather than stored). */
+int hvd_gpu_variational_refine_stages(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n,
+                                      const hvd_solver_params* p, float* const* stages, void* stream);
+
+/* The adjoint of VariationalRefine's 2-D solve for a loss L of its outputs.
+ * Each IRLS stage is taken as solved exactly (implicit differentiation):
+ * per stage an adjoint CG with the stage's own operator (adj_iterations
+ * iterations at most, relative exit adj_tol), then the lambda, S and weight
+ * terms; the weights' dependence on the stage's starting chi and on S is
+ * followed back stage by stage.  planes[i]: the forward's S, carrier and, in
+ * .chi, the final chi; p: the forward's parameters (chroma_aniso forced or
+ * AUTO: the AUTO value is recomputed from chi0 and held constant); stages:
+ * as saved by hvd_gpu_variational_refine_stages; gchi[i] = dL/dchi, gy[i] =
+ * dL/dY (gy, or its entries, may be NULL).  Outputs: gs[i] = dL/dS (lines *
+ * width), gchi0[i] = dL/dchi0 (lines * width * 2), and HOST glam[2 i],
+ * glam[2 i + 1] = dL/dlambda_y, dL/dlambda_c.  Synchronous. */
+int hvd_gpu_refine_adjoint(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hvd_solver_params* p,
+                           const float* const* stages, const float* const* gchi, const float* const* gy,
+                           int adj_iterations, double adj_tol, float* const* gs, float* const* gchi0, double* glam,
+                           void* stream);
+
 /* ---- field driver ------------------------------------------------------ */
 
 /* The 2-D field path of DecodeFieldWindow (engine/sequence.cpp:229-281 and
