@@ -6,9 +6,11 @@ burst lock-in -> active crop + carrier -> FFT holographic init -> IRLS 4 x
 CG 25 (cg_tol 1e-12, so every iteration runs) -> Y and chi.  `value` is
 fields/s of that batched path.  Also reported: the plain path (one field per
 call), the mixed 32 NTSC + 32 PAL batch (stop at relative residual 1e-6 or
-40 CG iterations per stage), integer-SAD block matching, a live CG roofline
-from HIP events recorded on the kernels' own stream, and the hvd-core CPU
-baseline (its own lock-in + VariationalRefine compiled from source) timed on
+40 CG iterations per stage), integer-SAD block matching, 512 mixed fields streamed in 8-field motion
+windows (block matching + batched decode per window), 200 Adam steps on the
+regularisation weights of 16 fields through the solve's adjoint, a live CG
+roofline from HIP events recorded on the kernels' own stream, and the
+hvd-core CPU baseline (its own lock-in + VariationalRefine compiled from source) timed on
 this host, which doubles as a check that the GPU solve matches hvd-core on
 identical inputs.
 
@@ -121,6 +123,110 @@ def cpu_baseline(torch, gpu, synth, eng, n_fields=6):
             "sample": f"{n_fields} NTSC 910x263 fields: hvd-core BurstLockinPhase + VariationalRefine "
                       f"(IRLS 4 x CG 25), OpenMP {cores} threads, one field at a time; init excluded (FFTW absent)",
             "gpu_vs_reference_max_abs_err_ire": float(max(errs))}
+
+
+def training_bench(torch, gpu, synth, eng, steps=200, n_fields=16):
+    """Regularisation weights by gradient on synthetic ground truth: n_fields
+    NTSC fields in FP32 (lock-in, carrier and holographic init on the GPU),
+    then `steps` Adam steps on (log lambda_Y, log lambda_C), each a forward
+    IRLS 4 x CG 25 solve of the whole batch and its adjoint (IRLS weights
+    recomputed in the backward), loss = MSE of Y, U, V against the truth."""
+    from hvd_amd import autograd as HA
+    g = synth.NTSC
+    s_l, car_l, chi_l, yt_l, ct_l = [], [], [], [], []
+    for k in range(n_fields):
+        raw, sc, _ = synth.ntsc_field(7000 + k)
+        field = eng.sample_to_ire_u16(torch.from_numpy(raw.view(np.int16)).to("cuda:0"), 20480.0, 61440.0)
+        th, _ = eng.burst_lockin_phase([field], [g])
+        fal, a0 = g.first_active_field_line, g.active_video_start
+        s = field[fal:fal + g.lines, a0:a0 + g.width].contiguous()
+        car = eng.make_carrier(th[0][fal:fal + g.lines].contiguous(), None, g)
+        chi0 = torch.empty((g.lines, g.width, 2), dtype=torch.float32, device="cuda:0")
+        y0 = torch.empty_like(s)
+        eng.holographic_init([gpu.Engine.plane(s, chi0, y=y0, carrier=car)])
+        s_l.append(s), car_l.append(car), chi_l.append(chi0)
+        yt_l.append(torch.from_numpy(sc.y.astype(np.float32)))
+        ct_l.append(torch.from_numpy(np.stack([sc.chi.real, sc.chi.imag], -1).astype(np.float32)))
+    S, CAR, CHI0 = torch.stack(s_l), torch.stack(car_l), torch.stack(chi_l)
+    YT, CT = torch.stack(yt_l).to("cuda:0"), torch.stack(ct_l).to("cuda:0")
+    ly = torch.tensor(0.0, device="cuda:0", requires_grad=True)
+    lc = torch.tensor(0.0, device="cuda:0", requires_grad=True)
+    opt = torch.optim.Adam([ly, lc], lr=0.02)
+    npx = g.lines * g.width
+
+    def step():
+        opt.zero_grad()
+        y, chi = HA.refine(S, CAR, CHI0, ly, lc, eng, irls_outer=4, cg_iterations=100, fast=0, cg_tol=1e-12,
+                           adj_iterations=25, adj_tol=1e-6)
+        loss = (((y - YT) ** 2).sum() + ((chi - CT) ** 2).sum()) / (npx * n_fields)
+        loss.backward()
+        opt.step()
+        return float(loss.detach())
+
+    first = step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    last = first
+    for _ in range(steps - 1):
+        last = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / max(steps - 1, 1)
+    return {"fields": n_fields, "adam_steps": steps, "ms_per_step": round(dt * 1e3, 2),
+            "fields_per_s_fwd_bwd": round(n_fields / dt, 1), "loss_first": round(first, 5),
+            "loss_last": round(last, 5), "lambda_y": round(float(torch.exp(ly.detach())), 4),
+            "lambda_c": round(float(torch.exp(lc.detach())), 4)}
+
+
+def streaming_bench(torch, gpu, synth, eng, n_windows=64, window=8, unique=16):
+    """512 mixed fields (3 NTSC : 1 PAL) in 8-field motion windows: per
+    window, integer-SAD block matching of its 7 consecutive pairs and the
+    batched field decode (lock-in, FFT init, IRLS 4 x CG <= 40 to relative
+    residual 1e-6).  Noise 0.2 / 1 / 3 IRE, chroma up to 60 IRE, one window
+    in five with fast motion (<= 12 px/field).  `unique` distinct windows are
+    generated and streamed round-robin from HBM."""
+    rng = np.random.default_rng(11)
+    wins = []
+    for k in range(unique):
+        pal = k % 4 == 3
+        gg = synth.PAL if pal else synth.NTSC
+        fast = k % 5 == 4
+        raws, _ = synth.motion_sequence(9000 + k, n_fields=window, g=gg, noise_ire=float([0.2, 1.0, 3.0][k % 3]),
+                                        max_rect_v=12 if fast else 8, saturation=float(rng.uniform(30.0, 60.0)))
+        d = [torch.from_numpy(r.view(np.int16)).to("cuda:0") for r in raws]
+        ys = [torch.empty((gg.lines, gg.width), device="cuda:0") for _ in d]
+        chis = [torch.empty((gg.lines, gg.width, 2), device="cuda:0") for _ in d]
+        wins.append((d, [gg] * window, ys, chis))
+    p = gpu.SolverParams.make(**CFG2)
+
+    def run(w, plain=False):
+        d, geoms, ys, chis = w
+        eng.block_match_sad_u16(d[:-1], d[1:])
+        if plain:
+            for k in range(len(d)):
+                eng.decode_fields_u16([d[k]], [geoms[k]], p, outputs=([ys[k]], [chis[k]]))
+        else:
+            eng.decode_fields_u16(d, geoms, p, outputs=(ys, chis))
+
+    for w in wins[:2]:
+        run(w)
+    torch.cuda.synchronize()
+    eng.set_profiling(True)
+    t0 = time.perf_counter()
+    for i in range(n_windows):
+        run(wins[i % unique])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    cg_ms, _, pxit = eng.cg_timing()
+    eng.set_profiling(False)
+    t0 = time.perf_counter()
+    nplain = 8
+    for i in range(nplain):
+        run(wins[i % unique], plain=True)
+    torch.cuda.synchronize()
+    dtp = time.perf_counter() - t0
+    return {"fields": n_windows * window, "windows": n_windows, "fields_per_s": round(n_windows * window / dt, 1),
+            "plain_fields_per_s": round(nplain * window / dtp, 1),
+            "cg_algorithmic_gbs": round(CG_BYTES_PER_PX_ITER * pxit / (cg_ms * 1e-3) / 1e9, 1)}
 
 
 def main():
@@ -240,6 +346,13 @@ def main():
         result["sad_bm_field_pairs_per_s"] = round(sad_pairs, 1)
         result["sad_bm_gops"] = round(sad_pairs * 910 * 263 * 33 * 9 * 3 / 1e9, 1)  # sub, abs, add per candidate px
         log(f"[bench] SAD BM: {sad_pairs:.0f} field pairs/s")
+    if not args.only_batched:
+        log("[bench] streaming: 512 mixed fields in 8-field windows")
+        result["streaming_512_mixed_windows8"] = streaming_bench(torch, gpu, synth, eng)
+        log(f"[bench] streaming: {result['streaming_512_mixed_windows8']}")
+        log("[bench] training: 16 NTSC fields, 200 Adam steps on log lambda")
+        result["training_16ntsc_adam200"] = training_bench(torch, gpu, synth, eng)
+        log(f"[bench] training: {result['training_16ntsc_adam200']}")
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (hvd-core reference build)")
         cb = cpu_baseline(torch, gpu, synth, eng)

@@ -193,14 +193,15 @@ def ntsc_theta0(field_index: int) -> float:
 
 
 def motion_sequence(seed: int, n_fields: int = 8, g: Geometry = NTSC, noise_ire: float = 0.5,
-                    max_global=(1, 3), max_rect_v: int = 8):
+                    max_global=(1, 3), max_rect_v: int = 8, saturation: float = 40.0):
     """Consecutive fields of one scene under seeded integer motion: a global
     translation (|dy| <= max_global[0], |dx| <= max_global[1] per field) plus
     four 32x16 rectangles with their own integer velocities (<= max_rect_v
-    px/field).  Returns (raws [n, fh, fw] uint16, truth dict)."""
+    px/field); chroma up to `saturation` IRE.  Returns (raws [n, fh, fw]
+    uint16, truth dict)."""
     rng = np.random.default_rng(seed)
     pad = 64
-    sc = make_scene(g, rng, canvas_pad=pad)
+    sc = make_scene(g, rng, canvas_pad=pad, saturation=saturation)
     gdy = int(rng.integers(-max_global[0], max_global[0] + 1))
     gdx = int(rng.integers(-max_global[1], max_global[1] + 1))
     rects = []
