@@ -52,6 +52,18 @@ def make_fields(torch, synth, n, pal_every=0, seed0=1000):
     return raws, geoms
 
 
+def autotune(eng, geom, n):
+    """Run the library's autotuner for n fields of this geometry's active shape
+    (records the fastest strip height and solve-group size in the context)."""
+    t0 = time.perf_counter()
+    rows, gpx, ns = eng.autotune_solver(geom.lines, geom.width, geom.uses_vswitch, n)
+    dt = time.perf_counter() - t0
+    log(f"[bench] autotune {geom.lines}x{geom.width} x{n}: strip rows {rows}, group {gpx} px, "
+        f"{ns:.4f} ns/px-iteration ({dt:.1f} s)")
+    return {"fields": n, "strip_rows": rows, "group_pixels": gpx, "ns_per_pixel_iteration": round(ns, 5),
+            "tuning_s": round(dt, 2)}
+
+
 def traffic_from_profile():
     """HBM bytes per CG iteration (both launches) measured with rocprofv3 --pmc
     FETCH_SIZE / WRITE_SIZE, as recorded in profiles/; None if absent."""
@@ -236,6 +248,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only-batched", action="store_true", help="time the headline workload only (profiling)")
+    ap.add_argument("--no-autotune", action="store_true", help="use the built-in launch shapes (no autotuner)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 512 MiB device-to-device copy and exit (reference point for PMC byte counters)")
     args = ap.parse_args()
@@ -269,6 +282,14 @@ def main():
     def step():
         eng.decode_fields_u16(raws, geoms, p0, outputs=(ys, chis))
 
+    tuned = []
+    if not args.no_autotune:  # before any timing: the tuner's own runs are not part of a step
+        tuned.append(autotune(eng, g, BATCH))
+        if not args.only_batched:
+            # the other lines' batches: mixed (32 + 32), training (16), streaming windows (8), plain (1)
+            for geom, n in ((g, 32), (g, 16), (g, 8), (g, 1), (synth.PAL, 32), (synth.PAL, 8), (synth.PAL, 1)):
+                tuned.append(autotune(eng, geom, n))
+
     log(f"[bench] warmup {args.warmup}")
     for _ in range(args.warmup):
         step()
@@ -281,15 +302,14 @@ def main():
         step()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    cg_ms, cg_launches, pxit = eng.cg_timing()
+    cg_ms, cg_iters, pxit = eng.cg_timing()
     eng.set_profiling(False)
     fields_s = BATCH * args.steps / dt
     ms_step = 1000.0 * dt / args.steps
-    pairs = cg_launches / 4  # per CG iteration: k_cg_curv, k_reduce_curv, k_cg_step, k_reduce_step
-    avg_pair_ms = cg_ms / pairs
-    px_per_pair = pxit / pairs  # pixels actually iterated per CG launch pair
+    avg_pair_ms = cg_ms / cg_iters  # per CG iteration of one solve group (k_cg_iter + k_reduce_iter)
+    px_per_pair = pxit / cg_iters  # pixels actually iterated per group-iteration
     achieved = CG_BYTES_PER_PX_ITER * px_per_pair / (avg_pair_ms * 1e-3) / 1e9
-    log(f"[bench] batched: {fields_s:.1f} fields/s, {ms_step:.2f} ms/step; CG pair {avg_pair_ms*1e3:.1f} us, "
+    log(f"[bench] batched: {fields_s:.1f} fields/s, {ms_step:.2f} ms/step; CG iteration {avg_pair_ms*1e3:.1f} us, "
         f"{achieved:.0f} GB/s algorithmic")
     traffic = traffic_from_profile()
     result = {
@@ -300,11 +320,12 @@ def main():
         "config": {"workload": "ntsc_910x263_batched64_lockin_fftinit_irls4x25", "fields_per_step": BATCH,
                    "field": "NTSC 4fsc 910x263 uint16, active 760x242", "irls_outer": 4, "cg_inner": 25,
                    "cg_tol": 1e-12, "motion_estimation": False, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": "CG iteration (k_cg_curv + k_reduce_curv + k_cg_step + k_reduce_step)",
+        "roofline": {"bound": "hbm", "kernel": "CG iteration (k_cg_iter + k_reduce_iter)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "avg_us_per_iteration": round(avg_pair_ms * 1e3, 2),
                      "algorithmic_bytes_per_iteration": int(CG_BYTES_PER_PX_ITER * px_per_pair)},
+        "solver_tuning": tuned if tuned else "built-in policy",
     }
     if not args.only_batched:
         # plain path: one field per call

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -36,6 +37,13 @@ struct Slot {
 
 }  // namespace
 
+// Launch shape of the solver: lines per strip and pixels per solve group
+// (0: the built-in policy / default).
+struct SolverTuning {
+  int rows = 0;
+  size_t group_px = 0;
+};
+
 struct hvd_gpu_ctx {
   int device = 0;
   std::string err;
@@ -49,8 +57,12 @@ struct hvd_gpu_ctx {
   // copies of the per-field solver state, read by hvd_gpu_cg_timing().
   bool profiling = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> cg_events;
-  std::vector<long long> cg_launches;
+  std::vector<long long> cg_iters;  // CG iterations launched per loop
   std::vector<std::pair<void*, int>> cg_states;  // pinned FieldCG copies, count
+  // solver launch shape: fixed by hvd_gpu_set_solver_tuning, else measured per
+  // field shape by hvd_gpu_autotune_solver, else the built-in policy
+  SolverTuning fixed;
+  std::map<std::array<int, 3>, std::map<int, SolverTuning>> tuned;  // (lines, width, pal) -> fields -> tuning
 };
 
 namespace {
@@ -423,26 +435,20 @@ struct RefineOut {
 constexpr int kMinStripRows = 2;
 
 // Lines per solver strip: the tallest strips (the fewest re-read halo rows)
-// that still give a launch enough waves to cover every CU several times.
-int strip_rows(const std::vector<DevField>& F) {
-#ifdef HVD_FORCE_STRIP_ROWS
-  static_assert(HVD_FORCE_STRIP_ROWS >= kMinStripRows && HVD_FORCE_STRIP_ROWS <= kMaxStripRows,
-                "strip rows out of range");
-  (void)F;
-  return HVD_FORCE_STRIP_ROWS;
-#else
-  // measured on 64 NTSC fields: 16-line strips re-read the fewest halo rows
-  // once a launch has ~16 waves per CU; below that, shorter strips win
+// that still give a launch enough waves to keep every CU busy; `forced` > 0
+// (a tuning) wins.
+int strip_rows(const std::vector<DevField>& F, int forced) {
+  if (forced > 0) return std::min(std::max(forced, kMinStripRows), kMaxStripRows);
+  // measured on NTSC fields in 16-field solve groups: 16-line strips are
+  // fastest once a launch has ~2 waves per SIMD; below that, shorter strips
   auto waves = [&](int rows) {
     long n = 0;
     for (const DevField& f : F) n += solver_strips(f.h, f.w, rows);
     return n;
   };
-  if (waves(16) >= 4096) return 16;
-  for (int rows : {8, 4})
+  for (int rows : {16, 8, 4})
     if (waves(rows) >= 2048) return rows;
   return kMinStripRows;
-#endif
 }
 
 // Strip decomposition of a batch for the solver kernels; fills each field's
@@ -452,10 +458,10 @@ struct StripSetup {
   int nb = 0, rows = 0, spf = 0;
 };
 
-StripSetup strip_setup(std::vector<DevField>& F) {
+StripSetup strip_setup(std::vector<DevField>& F, int rows) {
   StripSetup S;
   const int nf = (int)F.size();
-  S.rows = strip_rows(F);
+  S.rows = strip_rows(F, rows);
   S.tp.assign(nf + 1, 0);
   for (int i = 0; i < nf; ++i) {
     F[i].tiles_x = (F[i].w + kStripCols - 1) / kStripCols;
@@ -469,13 +475,15 @@ StripSetup strip_setup(std::vector<DevField>& F) {
   return S;
 }
 
-// One stage's CG from the current chi: gradient, d = -g, then n_inner
-// curvature / step launches with per-field exits (variational.cpp:519-549).
+// One stage's CG from the current chi: the stage-start gradient, then n_inner
+// fused iterations, each a stencil launch and a per-field reduction, with
+// per-field exits (variational.cpp:519-549).  adjoint: the fields describe an
+// adjoint solve (no composite, a right-hand side).
 void cg_loop(hvd_gpu_ctx* ctx, const DevField* dF, FieldCG* dstate, const StripGrid& sg, double* partials, int nf,
-             int n_inner, hipStream_t st) {
-  hipLaunchKernelGGL(k_grad_init, dim3(sg.nb), dim3(64), 0, st, dF, dstate, sg, partials);
+             int n_inner, bool adjoint, hipStream_t st) {
+  hipLaunchKernelGGL(k_cg_start, dim3(sg.nb), dim3(64), 0, st, dF, dstate, sg, partials, (int)adjoint);
   LAUNCH_CHECK(ctx);
-  hipLaunchKernelGGL(k_reduce_init, dim3(nf), dim3(256), 0, st, dstate, sg.tp, (const double*)partials);
+  hipLaunchKernelGGL(k_reduce_start, dim3(nf), dim3(256), 0, st, dstate, sg.tp, (const double*)partials);
   LAUNCH_CHECK(ctx);
   std::pair<hipEvent_t, hipEvent_t> evs{nullptr, nullptr};
   if (ctx->profiling) {
@@ -484,19 +492,18 @@ void cg_loop(hvd_gpu_ctx* ctx, const DevField* dF, FieldCG* dstate, const StripG
     HIPCHK(ctx, hipEventRecord(evs.first, st));
   }
   for (int it = 0; it < n_inner; ++it) {
-    hipLaunchKernelGGL(k_cg_curv, dim3(sg.nb), dim3(64), 0, st, dF, dstate, sg, partials, it);
+    if (adjoint)
+      hipLaunchKernelGGL(k_cg_iter_adj, dim3(sg.nb), dim3(64), 0, st, dF, dstate, sg, partials, it);
+    else
+      hipLaunchKernelGGL(k_cg_iter, dim3(sg.nb), dim3(64), 0, st, dF, dstate, sg, partials, it);
     LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_reduce_curv, dim3(nf), dim3(256), 0, st, dstate, sg.tp, (const double*)partials, it);
-    LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_cg_step, dim3(sg.nb), dim3(64), 0, st, dF, dstate, sg, partials);
-    LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_reduce_step, dim3(nf), dim3(256), 0, st, dstate, sg.tp, (const double*)partials);
+    hipLaunchKernelGGL(k_reduce_iter, dim3(nf), dim3(256), 0, st, dstate, sg.tp, (const double*)partials);
     LAUNCH_CHECK(ctx);
   }
   if (ctx->profiling) {
     HIPCHK(ctx, hipEventRecord(evs.second, st));
     ctx->cg_events.push_back(evs);
-    ctx->cg_launches.push_back(4LL * n_inner);
+    ctx->cg_iters.push_back(n_inner);
   }
 }
 
@@ -520,9 +527,10 @@ FieldCG field_cg_init(const DevField& f, const hvd_solver_params& p, int n_inner
 
 void run_refine(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_params& p, double* partials,
                 float* gx, float* gy, const std::vector<int>& sample_prefix, FieldCG* dstate, hipStream_t st,
-                RefineOut out) {
+                RefineOut out, int rows_tuned) {
   const int nf = (int)F.size();
-  const StripSetup SS = strip_setup(F);
+  for (int i = 0; i < nf; ++i) REQUIRE(ctx, F[i].s != nullptr, "field %d: no composite plane", i);
+  const StripSetup SS = strip_setup(F, rows_tuned);
   const std::vector<int>& tp = SS.tp;
   const int nb = SS.nb, rows = SS.rows, spf = SS.spf;
   const int outer = std::max(1, p.irls_outer);
@@ -552,7 +560,7 @@ void run_refine(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_par
   for (int o = 0; o < outer; ++o) {
     hipLaunchKernelGGL(k_irls_weights, dim3(nb), dim3(64), 0, st, dF, dstate, sg, o);
     LAUNCH_CHECK(ctx);
-    cg_loop(ctx, dF, dstate, sg, partials, nf, n_inner, st);
+    cg_loop(ctx, dF, dstate, sg, partials, nf, n_inner, false, st);
     hipLaunchKernelGGL(k_stage_end, dim3((nf + 63) / 64), dim3(64), 0, st, dstate, nf, o);
     LAUNCH_CHECK(ctx);
   }
@@ -585,31 +593,66 @@ int aniso_samples(int h, int w, int is_pal) {
   return (w >= 2 && nt > 0) ? nt * nx : 0;
 }
 
-// Solve groups: consecutive fields whose solver working set (about 60 B per
-// pixel: chi x2, d x2, g, weights, composite) stays within the 256 MB
+// Solve groups: consecutive fields whose solver working set (about 68 B per
+// pixel: chi x2, g x2, d x2, weights, composite) mostly stays in the 256 MB
 // Infinity Cache, so a group's CG iterations re-read their planes from the
-// cache rather than HBM.  Groups run one after another on the stream and
-// share one workspace region.
-#ifndef HVD_SOLVE_GROUP_PX
-#define HVD_SOLVE_GROUP_PX (size_t(3) << 21)
-#endif
-constexpr size_t kSolveGroupPx = HVD_SOLVE_GROUP_PX;  // pixels per solve group
+// cache rather than HBM.  A batch is cut into the fewest groups of at most
+// about `max_px` pixels, balanced in size; groups run one after another on
+// the stream and share one workspace region.
+constexpr size_t kSolveGroupPx = 3000000;  // default; 16 NTSC fields measured best
 
-std::vector<std::pair<int, int>> solve_groups(const std::vector<DevField>& F) {
+std::vector<std::pair<int, int>> solve_groups(const std::vector<DevField>& F, size_t max_px) {
   std::vector<std::pair<int, int>> g;
+  const int n = (int)F.size();
+  if (n == 0) return g;
+  if (max_px == 0) max_px = kSolveGroupPx;
+  double total = 0.0;
+  for (const DevField& f : F) total += (double)f.h * f.w;
+  const int ng = std::max(1, std::min(n, (int)std::ceil(total / (double)max_px)));
+  const double target = total / ng;
   int b = 0;
-  size_t px = 0;
-  for (int i = 0; i < (int)F.size(); ++i) {
-    const size_t np = (size_t)F[i].h * F[i].w;
-    if (i > b && px + np > kSolveGroupPx) {
+  double acc = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const double np = (double)F[i].h * F[i].w;
+    // close the group before field i when it holds enough and i would
+    // overshoot the target by more than half of itself
+    if (i > b && (int)g.size() < ng - 1 && acc + 0.5 * np > target) {
       g.emplace_back(b, i);
       b = i;
-      px = 0;
+      acc = 0.0;
     }
-    px += np;
+    acc += np;
   }
-  if (!F.empty()) g.emplace_back(b, (int)F.size());
+  g.emplace_back(b, n);
   return g;
+}
+
+// The launch shape for a batch: the fixed tuning, else the autotuned entry for
+// its field shape (when every field shares one): all of it for the same field
+// count, else the strip height of the entry with the nearest count (its group
+// size belongs to that count; the default applies); else the built-in policy.
+SolverTuning tuning_for(const hvd_gpu_ctx* ctx, const std::vector<DevField>& F, int is_pal) {
+  SolverTuning t = ctx->fixed;
+  if ((t.rows > 0 && t.group_px > 0) || F.empty()) return t;
+  for (const DevField& f : F)
+    if (f.h != F[0].h || f.w != F[0].w) return t;
+  const auto it = ctx->tuned.find({F[0].h, F[0].w, is_pal ? 1 : 0});
+  if (it == ctx->tuned.end() || it->second.empty()) return t;
+  const int n = (int)F.size();
+  const auto exact = it->second.find(n);
+  const SolverTuning* best = exact != it->second.end() ? &exact->second : nullptr;
+  double bd = 1e300;
+  if (!best)
+    for (const auto& kv : it->second) {
+      const double d = std::fabs(std::log((double)kv.first / n));
+      if (d < bd) {
+        bd = d;
+        best = &kv.second;
+      }
+    }
+  if (t.rows <= 0) t.rows = best->rows;
+  if (t.group_px == 0 && exact != it->second.end()) t.group_px = best->group_px;
+  return t;
 }
 
 // One solve group's workspace, carved after `cv`.  Partials are sized for the
@@ -631,13 +674,13 @@ GroupLayout group_layout(Carve cv, const DevField* F, int n, int is_pal) {
   for (int i = 0; i < n; ++i) {
     const size_t np = (size_t)F[i].h * F[i].w;
     L.chi1[i] = cv.take(sizeof(float2) * np);
-    L.g[i] = cv.take(sizeof(float2) * np);
+    L.g[i] = cv.take(sizeof(float2) * np * 2);
     L.d[i] = cv.take(sizeof(float2) * np * 2);
     L.w[i] = cv.take(sizeof(float4) * np);
     strips += (size_t)solver_strips(F[i].h, F[i].w, kMinStripRows);
     L.sp[i + 1] = L.sp[i] + aniso_samples(F[i].h, F[i].w, is_pal);
   }
-  L.part = cv.take(sizeof(double) * 2 * strips);
+  L.part = cv.take(sizeof(double) * kCgPartials * strips);
   L.gx = cv.take(sizeof(float) * (L.sp[n] + 1));
   L.gy = cv.take(sizeof(float) * (L.sp[n] + 1));
   L.st = cv.take(sizeof(FieldCG) * n);
@@ -646,9 +689,9 @@ GroupLayout group_layout(Carve cv, const DevField* F, int n, int is_pal) {
 }
 
 // End offset of the solver workspace refine_planes carves after `cv`.
-size_t refine_ws_end(Carve cv, const std::vector<DevField>& F, int is_pal) {
+size_t refine_ws_end(const hvd_gpu_ctx* ctx, Carve cv, const std::vector<DevField>& F, int is_pal) {
   size_t end = cv.off;
-  for (const auto& gr : solve_groups(F))
+  for (const auto& gr : solve_groups(F, tuning_for(ctx, F, is_pal).group_px))
     end = std::max(end, group_layout(cv, F.data() + gr.first, gr.second - gr.first, is_pal).end);
   return end;
 }
@@ -696,7 +739,7 @@ void clear_profile(hvd_gpu_ctx* ctx) {
   }
   for (auto& p : ctx->cg_states) (void)hipHostFree(p.first);
   ctx->cg_events.clear();
-  ctx->cg_launches.clear();
+  ctx->cg_iters.clear();
   ctx->cg_states.clear();
 }
 }  // namespace
@@ -704,7 +747,7 @@ void clear_profile(hvd_gpu_ctx* ctx) {
 // ============================================================================
 extern "C" {
 
-int hvd_gpu_abi_version(void) { return 101; }
+int hvd_gpu_abi_version(void) { return 102; }
 
 int hvd_gpu_create(int device, hvd_gpu_ctx** out) {
   if (!out) return HVD_GPU_ERR_INVALID;
@@ -758,9 +801,9 @@ int hvd_gpu_set_profiling(hvd_gpu_ctx* ctx, int enable) {
   });
 }
 
-int hvd_gpu_cg_timing(hvd_gpu_ctx* ctx, double* ms, long long* launches, double* pixel_iterations) {
+int hvd_gpu_cg_timing(hvd_gpu_ctx* ctx, double* ms, long long* iterations, double* pixel_iterations) {
   return guarded(ctx, [&] {
-    REQUIRE(ctx, ms && launches && pixel_iterations, "null output");
+    REQUIRE(ctx, ms && iterations && pixel_iterations, "null output");
     HIPCHK(ctx, hipDeviceSynchronize());
     double tot = 0.0, pxit = 0.0;
     long long nl = 0;
@@ -768,7 +811,7 @@ int hvd_gpu_cg_timing(hvd_gpu_ctx* ctx, double* ms, long long* launches, double*
       float t = 0.f;
       HIPCHK(ctx, hipEventElapsedTime(&t, ctx->cg_events[i].first, ctx->cg_events[i].second));
       tot += t;
-      nl += ctx->cg_launches[i];
+      nl += ctx->cg_iters[i];
     }
     for (auto& p : ctx->cg_states) {
       const FieldCG* st = static_cast<const FieldCG*>(p.first);
@@ -779,7 +822,7 @@ int hvd_gpu_cg_timing(hvd_gpu_ctx* ctx, double* ms, long long* launches, double*
       }
     }
     *ms = tot;
-    *launches = nl;
+    *iterations = nl;
     *pixel_iterations = pxit;
     clear_profile(ctx);
   });
@@ -907,16 +950,18 @@ namespace {
 void refine_planes(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_params& p, hipStream_t st,
                    Carve cv, RefineOut out) {
   // the caller sized the workspace (its tables point into it already)
-  REQUIRE(ctx, refine_ws_end(cv, F, p.is_pal) <= ctx->ws_cap, "internal: solver workspace not sized");
+  REQUIRE(ctx, refine_ws_end(ctx, cv, F, p.is_pal) <= ctx->ws_cap, "internal: solver workspace not sized");
   char* ws = ctx->ws;
   const int outer = std::max(1, p.irls_outer);
-  for (const auto& gr : solve_groups(F)) {
+  const SolverTuning tune = tuning_for(ctx, F, p.is_pal);
+  for (const auto& gr : solve_groups(F, tune.group_px)) {
     const int b = gr.first, n = gr.second - gr.first;
     const GroupLayout L = group_layout(cv, F.data() + b, n, p.is_pal);
     std::vector<DevField> G(F.begin() + b, F.begin() + gr.second);
     for (int i = 0; i < n; ++i) {
       G[i].chi[1] = (float2*)(ws + L.chi1[i]);
-      G[i].g = (float2*)(ws + L.g[i]);
+      G[i].g[0] = (float2*)(ws + L.g[i]);
+      G[i].g[1] = G[i].g[0] + (size_t)G[i].h * G[i].w;
       G[i].d[0] = (float2*)(ws + L.d[i]);
       G[i].d[1] = G[i].d[0] + (size_t)G[i].h * G[i].w;
       G[i].wts = (float4*)(ws + L.w[i]);
@@ -925,7 +970,7 @@ void refine_planes(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_
     o.stage_iters = out.stage_iters ? out.stage_iters + (size_t)b * outer : nullptr;
     o.aniso = out.aniso ? out.aniso + b : nullptr;
     run_refine(ctx, G, p, (double*)(ws + L.part), (float*)(ws + L.gx), (float*)(ws + L.gy), L.sp,
-               (FieldCG*)(ws + L.st), st, o);
+               (FieldCG*)(ws + L.st), st, o, tune.rows);
   }
 }
 
@@ -958,7 +1003,7 @@ void run_adjoint(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hvd
     o_col[i] = cv.take(sizeof(float2) * planes[i].width);
     o_a[i] = cv.take(sizeof(float2) * np);
     o_chi1[i] = cv.take(sizeof(float2) * np);
-    o_g[i] = cv.take(sizeof(float2) * np);
+    o_g[i] = cv.take(sizeof(float2) * np * 2);
     o_d[i] = cv.take(sizeof(float2) * np * 2);
     o_w[i] = cv.take(sizeof(float4) * np);
     o_sc[i] = cv.take(sizeof(float) * 6 * np);
@@ -966,7 +1011,7 @@ void run_adjoint(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hvd
     sp[i + 1] = sp[i] + aniso_samples(planes[i].lines, planes[i].width, p.is_pal);
   }
   const int nbx = (int)((maxpx + 255) / 256);
-  const size_t o_part = cv.take(sizeof(double) * 2 * strips);
+  const size_t o_part = cv.take(sizeof(double) * kCgPartials * strips);
   const size_t o_apart = cv.take(sizeof(double) * 2 * (size_t)nbx * n);
   const size_t o_glam = cv.take(sizeof(double) * 2 * n);
   const size_t o_gx = cv.take(sizeof(float) * (sp[n] + 1));
@@ -992,7 +1037,8 @@ void run_adjoint(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hvd
       f.vsw = pl.vswitch ? (const int8_t*)(ws + o_vsw[i]) : nullptr;
     }
     f.wts = (float4*)(ws + o_w[i]);
-    f.g = (float2*)(ws + o_g[i]);
+    f.g[0] = (float2*)(ws + o_g[i]);
+    f.g[1] = f.g[0] + np;
     f.d[0] = (float2*)(ws + o_d[i]);
     f.d[1] = f.d[0] + np;
     Fw[i] = f;  // weights pass: chi[0] set per stage
@@ -1021,7 +1067,7 @@ void run_adjoint(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hvd
     q.gy = gy ? gy[i] : nullptr;
   }
   run_prepare(ctx, prep, false, st);
-  const StripSetup SS = strip_setup(Fw);
+  const StripSetup SS = strip_setup(Fw, tuning_for(ctx, Fw, p.is_pal).rows);
   for (int i = 0; i < n; ++i) {
     Fa[i].tiles_x = Fw[i].tiles_x;
     Fa[i].tile0 = Fw[i].tile0;
@@ -1043,8 +1089,8 @@ void run_adjoint(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hvd
     for (int i = 0; i < n; ++i) {
       const size_t np = (size_t)planes[i].lines * planes[i].width;
       const float2* chip = reinterpret_cast<const float2*>(stages[i]) + (size_t)k * np;
-      // both ping-pong slots: the weights kernel reads chi[chi_cur], and the
-      // adjoint CG leaves chi_cur at either value
+      // both ping-pong slots: the weights kernel reads chi[cur], and the
+      // adjoint CG leaves cur at either value
       FwK[k][i].chi[0] = const_cast<float2*>(chip);
       FwK[k][i].chi[1] = const_cast<float2*>(chip);
       AK[k][i].chip = chip;
@@ -1092,7 +1138,7 @@ void run_adjoint(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hvd
                          Fa[i].chi[1], np);
       LAUNCH_CHECK(ctx);
     }
-    cg_loop(ctx, dFa, dstate, sg, partials, n, adj_iters, st);
+    cg_loop(ctx, dFa, dstate, sg, partials, n, adj_iters, true, st);
     hipLaunchKernelGGL(k_refine_finalize, dim3(SS.nb), dim3(64), 0, st, dFa, (const FieldCG*)dstate, sg);
     LAUNCH_CHECK(ctx);
     const AdjField* dA = (const AdjField*)(base + oA[k]);
@@ -1141,7 +1187,7 @@ void refine_entry(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hv
       F[i] = f;
     }
     // size the whole call (tables + solver) before any pointer is taken
-    ensure_ws(ctx, refine_ws_end(cv, F, p->is_pal), st);
+    ensure_ws(ctx, refine_ws_end(ctx, cv, F, p->is_pal), st);
     char* ws = ctx->ws;
     std::vector<PrepDesc> prep;
     for (int i = 0; i < n; ++i) {
@@ -1156,6 +1202,148 @@ void refine_entry(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hv
     run_prepare(ctx, prep, false, st);
     refine_planes(ctx, F, *p, st, cv, RefineOut{stage_iters, aniso});
   }
+}
+
+}  // namespace
+
+namespace {
+
+// The autotuner: solves `n` synthetic fields of one shape with every
+// candidate launch shape (strip lines x fields per solve group) and keeps the
+// one with the least CG-loop time per pixel-iteration, timed with the HIP
+// events of the profiling record.  The fields' data are smooth patterns with
+// a modulated chroma term; the solve's cost does not depend on them because
+// cg_tol 0 runs every iteration.
+struct TuneResult {
+  SolverTuning t;
+  double ns = 0.0;
+};
+
+TuneResult autotune(hvd_gpu_ctx* ctx, int h, int w, bool pal, int n, int iterations) {
+  const size_t np = (size_t)h * w;
+  std::vector<float> hs(np), hchi(2 * np), hth(h);
+  std::vector<int8_t> hvs(h);
+  for (int y = 0; y < h; ++y) {
+    hth[y] = (float)(0.5 * kPi * y);
+    hvs[y] = (y & 1) ? -1 : 1;
+    for (int x = 0; x < w; ++x) {
+      const size_t i = (size_t)y * w + x;
+      hs[i] = (float)(30.0 + 20.0 * std::sin(0.21 * x + 0.37 * y) + 8.0 * std::sin(0.5 * kPi * x + hth[y]));
+      hchi[2 * i] = (float)(4.0 * std::sin(0.013 * x));
+      hchi[2 * i + 1] = (float)(3.0 * std::cos(0.017 * y));
+    }
+  }
+  // device copies: one composite / theta / V-switch table shared by every
+  // field (read only), a pristine chi0 and each field's own chi
+  const size_t b_s = sizeof(float) * np, b_c = sizeof(float2) * np;
+  char* buf = nullptr;
+  HIPCHK(ctx, hipMalloc(&buf, b_s + sizeof(float) * h + h + b_c * (n + 1) + 256));
+  struct Free {
+    char* p;
+    ~Free() { (void)hipFree(p); }
+  } guard{buf};
+  float* ds = (float*)buf;
+  float* dth = (float*)(buf + b_s);
+  int8_t* dvs = (int8_t*)(buf + b_s + sizeof(float) * h);
+  float* dchi0 = (float*)(buf + ((b_s + sizeof(float) * h + h + 255) / 256) * 256);
+  HIPCHK(ctx, hipMemcpy(ds, hs.data(), b_s, hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemcpy(dth, hth.data(), sizeof(float) * h, hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemcpy(dvs, hvs.data(), h, hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemcpy(dchi0, hchi.data(), b_c, hipMemcpyHostToDevice));
+  std::vector<hvd_gpu_plane> planes(n);
+  for (int i = 0; i < n; ++i) {
+    hvd_gpu_plane& pl = planes[i];
+    pl = hvd_gpu_plane{};
+    pl.lines = h;
+    pl.width = w;
+    pl.s = ds;
+    pl.theta = dth;
+    pl.vswitch = pal ? dvs : nullptr;
+    pl.active_video_start = 0;
+    pl.phase_per_sample = 0.5 * kPi;
+    pl.sample_rate = 4.0 * 3579545.0;
+    pl.chi = dchi0 + 2 * np * (size_t)(i + 1);
+    pl.y = nullptr;
+  }
+  hvd_solver_params p{};
+  p.lambda_c = 1.0;
+  p.chroma_aniso = 0.7;
+  p.charbonnier_eps = 0.5;
+  p.chroma_eps = 1.0;
+  p.structure_coupling = 0.25;
+  p.cg_iterations = iterations;
+  p.irls_outer = 1;
+  p.cg_tol = 1e-30;
+  p.fast = 0;
+  p.is_pal = pal ? 1 : 0;
+  p.lambda_y = 1.0;
+  // candidates: fields per group from ceil(n / G), strips of 2..32 lines
+  std::vector<int> per;
+  for (int G = 1; G <= n; ++G) {
+    const int k = (n + G - 1) / G;
+    const double px = (double)k * np;
+    if ((px >= 2.5e5 && px <= 1.2e7) || k == n)
+      if (per.empty() || per.back() != k) per.push_back(k);
+  }
+  const int rows_c[] = {2, 4, 6, 8, 10, 12, 14, 16, 20, 24, 32};
+  // the user's fixed tuning and profiling state are restored on every exit
+  struct Restore {
+    hvd_gpu_ctx* c;
+    SolverTuning fixed;
+    bool prof;
+    ~Restore() {
+      c->fixed = fixed;
+      clear_profile(c);
+      c->profiling = prof;
+    }
+  } restore{ctx, ctx->fixed, ctx->profiling};
+  HIPCHK(ctx, hipDeviceSynchronize());
+  clear_profile(ctx);
+  ctx->profiling = true;
+  auto time_one = [&](const SolverTuning& t) {
+    ctx->fixed = t;
+    for (int i = 0; i < n; ++i) HIPCHK(ctx, hipMemcpyAsync(planes[i].chi, dchi0, b_c, hipMemcpyDeviceToDevice, nullptr));
+    refine_entry(ctx, planes.data(), n, &p, nullptr, nullptr, nullptr, nullptr);
+    HIPCHK(ctx, hipDeviceSynchronize());
+    double ms = 0.0;
+    for (auto& e : ctx->cg_events) {
+      float t_ms = 0.f;
+      HIPCHK(ctx, hipEventElapsedTime(&t_ms, e.first, e.second));
+      ms += t_ms;
+    }
+    clear_profile(ctx);
+    return 1e6 * ms / ((double)np * n * iterations);  // ns per pixel-iteration
+  };
+  std::vector<std::pair<double, SolverTuning>> res;
+  auto by_time = [](const auto& a, const auto& b) { return a.first < b.first; };
+  time_one(SolverTuning{16, (size_t)n * np});  // warm the workspace and code
+  // coarse pass: every group size at four strip heights
+  for (int k : per)
+    for (int r : {4, 8, 16, 32}) {
+      SolverTuning t{r, (size_t)k * np};
+      res.emplace_back(time_one(t), t);
+    }
+  std::sort(res.begin(), res.end(), by_time);
+  // fine pass: every strip height at the best group size and its neighbours
+  const int kbest = (int)(res[0].second.group_px / np);
+  const size_t at = std::find(per.begin(), per.end(), kbest) - per.begin();
+  for (size_t q = at > 0 ? at - 1 : 0; q <= std::min(at + 1, per.size() - 1); ++q)
+    for (int r : rows_c) {
+      if (r == 4 || r == 8 || r == 16 || r == 32) continue;
+      SolverTuning t{r, (size_t)per[q] * np};
+      res.emplace_back(time_one(t), t);
+    }
+  std::sort(res.begin(), res.end(), by_time);
+  // re-time the leaders and keep the best of three
+  const size_t lead = std::min<size_t>(res.size(), 4);
+  for (size_t i = 0; i < lead; ++i)
+    for (int rep = 0; rep < 2; ++rep) res[i].first = std::min(res[i].first, time_one(res[i].second));
+  std::sort(res.begin(), res.begin() + lead, by_time);
+  TuneResult out;
+  out.t = res[0].second;
+  out.ns = res[0].first;
+  ctx->tuned[{h, w, pal ? 1 : 0}][n] = out.t;
+  return out;
 }
 
 }  // namespace
@@ -1201,7 +1389,7 @@ struct DecodeLayout {
   size_t end = 0;
 };
 
-DecodeLayout decode_layout(int n, const hvd_field_geometry* geom, bool from_raw, Carve cv) {
+DecodeLayout decode_layout(const hvd_gpu_ctx* ctx, int n, const hvd_field_geometry* geom, bool from_raw, Carve cv) {
   DecodeLayout L;
   for (auto* v : {&L.ire, &L.th, &L.vs, &L.s, &L.row, &L.vsa, &L.col, &L.spec, &L.var}) v->resize(n);
   std::vector<DevField> shapes(n), shapes_pal, shapes_ntsc;
@@ -1225,7 +1413,7 @@ DecodeLayout decode_layout(int n, const hvd_field_geometry* geom, bool from_raw,
   }
   L.solver = cv;
   // the solve runs per standard (see decode_fields_run), each from L.solver
-  L.end = std::max(refine_ws_end(cv, shapes_pal, 1), refine_ws_end(cv, shapes_ntsc, 0));
+  L.end = std::max(refine_ws_end(ctx, cv, shapes_pal, 1), refine_ws_end(ctx, cv, shapes_ntsc, 0));
   return L;
 }
 
@@ -1381,7 +1569,7 @@ int decode_fields(hvd_gpu_ctx* ctx, const uint16_t* const* raw, const float* con
       REQUIRE(ctx, (raw ? (const void*)raw[i] : (const void*)field_ire[i]) && y_out[i] && chi_out[i],
               "field %d: null pointer", i);
     hipStream_t st = S(stream);
-    const DecodeLayout L = decode_layout(n, geom, raw != nullptr, Carve{});
+    const DecodeLayout L = decode_layout(ctx, n, geom, raw != nullptr, Carve{});
     ensure_ws(ctx, L.end, st);
     decode_fields_run(ctx, raw, field_ire, n, geom, blanking, white, p, chroma_phase_deg, y_out, chi_out, theta_out,
                       st, L, nullptr);
@@ -1428,7 +1616,7 @@ int hvd_gpu_decode_frame_window_s16(hvd_gpu_ctx* ctx, const int16_t* const* fram
     const size_t o_dec = cv.take(sizeof(int) * 2 + sizeof(float) * 4);
     const size_t o_y = cv.take(sizeof(float) * (size_t)nf * lines * width);
     const size_t o_chi = cv.take(sizeof(float2) * (size_t)nf * lines * width);
-    const DecodeLayout L = decode_layout(nf, geoms.data(), false, cv);
+    const DecodeLayout L = decode_layout(ctx, nf, geoms.data(), false, cv);
     char* ws = ensure_ws(ctx, L.end, st);
     float* fields = (float*)(ws + o_fields);
     float* amps = (float*)(ws + o_amps);
@@ -1825,6 +2013,32 @@ int hvd_gpu_weave_frame_s16(hvd_gpu_ctx* ctx, const int16_t* src, const hvd_fram
                        o);
     LAUNCH_CHECK(ctx);
     arm(ctx, sl, st);
+  });
+}
+
+int hvd_gpu_set_solver_tuning(hvd_gpu_ctx* ctx, const hvd_gpu_solver_tuning* t) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, t != nullptr, "null tuning");
+    REQUIRE(ctx, t->strip_rows == 0 || (t->strip_rows >= kMinStripRows && t->strip_rows <= kMaxStripRows),
+            "strip_rows %d outside %d..%d", t->strip_rows, kMinStripRows, kMaxStripRows);
+    REQUIRE(ctx, t->group_pixels >= 0, "negative group_pixels");
+    ctx->fixed.rows = t->strip_rows;
+    ctx->fixed.group_px = (size_t)t->group_pixels;
+  });
+}
+
+int hvd_gpu_autotune_solver(hvd_gpu_ctx* ctx, int lines, int width, int is_pal, int nfields, int iterations,
+                            hvd_gpu_solver_tuning* best, double* ns_per_pixel_iteration) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, best != nullptr, "null output");
+    REQUIRE(ctx, lines >= 2 && width >= 2 && lines + 32 <= 1024 && width <= 2048, "bad field shape %d x %d", lines,
+            width);
+    REQUIRE(ctx, nfields >= 1 && nfields <= 4096, "nfields %d outside 1..4096", nfields);
+    REQUIRE(ctx, (double)lines * width * nfields <= 2.0e8, "autotune batch too large");
+    const TuneResult r = autotune(ctx, lines, width, is_pal != 0, nfields, std::max(2, std::min(iterations, 64)));
+    best->strip_rows = r.t.rows;
+    best->group_pixels = (long long)r.t.group_px;
+    if (ns_per_pixel_iteration) *ns_per_pixel_iteration = r.ns;
   });
 }
 

@@ -3,7 +3,7 @@
 // Data layout in HBM (one active plane per field, row-major h x w):
 //   s        float   composite (IRE)
 //   chi      float2  chroma phasor chi = V - iU, interleaved (re, im)
-//   g, d     float2  CG gradient / search direction
+//   g, d     float2  CG gradient / search direction (each ping-ponged)
 //   wts      float4  IRLS diffusivities (wx, wy, wcx, wcy) per pixel
 //   carrier  never stored: c(y, x) = rowph[y] * colph[x], with the PAL
 //            V-switch folded in as c -> -conj(c) on switched lines
@@ -18,7 +18,10 @@ namespace hvdk {
 constexpr int kTileW = 64;  // one wavefront per tile row (init kernels)
 constexpr int kTileH = 4;
 constexpr int kBlock = kTileW * kTileH;  // 256 threads
-constexpr int kStripCols = 62;  // output columns of a solver strip (64 lanes - 2 halo)
+// Output columns of a solver strip: 64 lanes less a halo lane on the left and
+// two on the right (the fused CG iteration needs the new gradient one column
+// past its last output, and that column's own right neighbour).
+constexpr int kStripCols = 61;
 // Launch geometry of the solver kernels: strip prefix over the batch, and
 // the strips per field when every field has the same size (else 0).
 struct StripGrid {
@@ -27,9 +30,12 @@ struct StripGrid {
 };
 
 // Rows of carrier factors a solver strip of `rows` lines holds, one per lane
-// (the halo row above and the row below); at most 64.
-constexpr int carrier_span(int rows) { return rows + 2; }
+// (the halo row above and the two below); at most 64.
+constexpr int carrier_span(int rows) { return rows + 3; }
 constexpr int kMaxStripRows = 32;
+// FP64 partial sums per solver strip and CG launch (|g|^2, curv(g),
+// curv(g, d), Re<g, d>).
+constexpr int kCgPartials = 4;
 static_assert(carrier_span(kMaxStripRows) <= 64, "strip carrier rows must fit one wave");
 
 // Solver strips of `rows` lines covering an h x w plane.
@@ -48,7 +54,7 @@ struct DevField {
   const float2* colph; // w: e^{i w0 (a0 + x)}
   const float2* carrier;  // explicit h*w carrier, or null (factored form)
   float2* chi[2];      // ping-pong chroma (chi[0] is the caller's buffer)
-  float2* g;           // gradient
+  float2* g[2];        // ping-pong gradient
   float2* d[2];        // ping-pong search direction
   float4* wts;         // IRLS weights
   float* y;            // output luma
@@ -60,7 +66,8 @@ struct DevField {
 };
 
 // Per-field CG state (variational.cpp:519-549), advanced by the one-block
-// reduction kernels between the stencil launches.
+// reduction kernels between the stencil launches.  hh is curv(d) of the
+// direction the next iteration moves along, gd = Re<g, d> of it.
 struct FieldCG {
   double gg, gg0, hh, gd, npix;
   double tol2;
@@ -72,7 +79,8 @@ struct FieldCG {
   int is_pal;
   int n_inner;
   int done, it;
-  int chi_cur, d_cur;
+  int cur;              // ping-pong slot of chi, g and d(previous) read next
+  int pad0_;
   int iters[16];        // CG iterations per IRLS stage (diagnostics)
   int stage;
   int pad_;
@@ -117,6 +125,28 @@ __device__ inline void gst(float4* p, size_t i, float4 v) {
   t.z = v.z;
   t.w = v.w;
   ((g_f4*)p)[i] = t;
+}
+
+// The same accesses at a 32-bit element offset from a wave-uniform base: the
+// base stays in scalar registers and the byte offset in one VGPR (the
+// global "saddr" form), instead of a 64-bit address pair per load.  Every
+// plane here is far below 4 GB.
+typedef __attribute__((address_space(1))) const char g_cc;
+typedef __attribute__((address_space(1))) char g_c;
+__device__ inline float gld32(const float* p, unsigned i) { return *(g_cf*)((g_cc*)p + (i << 2)); }
+__device__ inline float2 gld32(const float2* p, unsigned i) {
+  const f2v v = *(g_cf2*)((g_cc*)p + (i << 3));
+  return make_float2(v.x, v.y);
+}
+__device__ inline float4 gld32(const float4* p, unsigned i) {
+  const f4v v = *(g_cf4*)((g_cc*)p + (i << 4));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ inline void gst32(float2* p, unsigned i, float2 v) {
+  f2v t;
+  t.x = v.x;
+  t.y = v.y;
+  *(g_f2*)((g_c*)p + (i << 3)) = t;
 }
 
 __device__ inline float2 cmul(float2 a, float2 b) {

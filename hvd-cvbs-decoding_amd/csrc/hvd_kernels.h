@@ -130,12 +130,11 @@ __global__ void k_aniso_samples(const DevField* F, const FieldCG* st, const int*
 __global__ void k_aniso_select(FieldCG* st, const int* sample_prefix, const float* gx, const float* gy,
                                float lambda_c);
 __global__ void k_irls_weights(const DevField* F, const FieldCG* st, StripGrid sg, int stage);
-__global__ void k_grad_init(const DevField* F, const FieldCG* st, StripGrid sg, double* partials);
-__global__ void k_cg_curv(const DevField* F, const FieldCG* st, StripGrid sg, double* partials, int it);
-__global__ void k_cg_step(const DevField* F, const FieldCG* st, StripGrid sg, double* partials);
-__global__ void k_reduce_init(FieldCG* st, const int* tp, const double* partials);
-__global__ void k_reduce_curv(FieldCG* st, const int* tp, const double* partials, int it);
-__global__ void k_reduce_step(FieldCG* st, const int* tp, const double* partials);
+__global__ void k_cg_start(const DevField* F, const FieldCG* st, StripGrid sg, double* partials, int adjoint);
+__global__ void k_cg_iter(const DevField* F, const FieldCG* st, StripGrid sg, double* partials, int it);
+__global__ void k_cg_iter_adj(const DevField* F, const FieldCG* st, StripGrid sg, double* partials, int it);
+__global__ void k_reduce_start(FieldCG* st, const int* tp, const double* partials);
+__global__ void k_reduce_iter(FieldCG* st, const int* tp, const double* partials);
 __global__ void k_stage_end(FieldCG* st, int nf, int stage);
 __global__ void k_refine_finalize(const DevField* F, const FieldCG* st, StripGrid sg);
 

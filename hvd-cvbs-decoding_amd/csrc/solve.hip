@@ -2,18 +2,20 @@
 //
 // Restates hvd-core's VariationalRefine 2-D path (engine/variational.cpp:
 // 198-556) as matrix-free stencil kernels over a batch of fields of uneven
-// size.  Per CG iteration two kernels run:
-//   k_cg_curv  d <- -g + beta d (fused, on the fly), then the FP64 partial
-//              sums of curv(d) (:352-399) and Re<g, d> (:72-117);
-//   k_cg_step  chi <- chi + alpha d (on the fly), g <- grad(chi) (:286-347)
-//              and the FP64 partial of |g|^2;
-// a one-block-per-field kernel then sums that field's strip partials in a
-// fixed order and advances its scalar CG state (alpha, beta, the exits of
-// :530 and :547-548), so no host round trip happens inside the solve and
-// converged fields drop out by themselves (per-field masks).  chi and d are ping-ponged so a block
-// never reads a neighbour's already-updated value.
+// size.  Per CG iteration one stencil kernel runs, k_cg_iter: it forms the
+// direction d <- -g + beta d and chi <- chi + alpha d on the fly, writes chi, d
+// and the recomputed gradient g <- grad(chi) (:286-347), and leaves the FP64
+// partial sums from which the next direction's curvature curv(d) (:352-399)
+// and Re<g, d> (:72-117) follow by expansion in beta, so no second pass over
+// the planes is needed.  A one-block-per-field kernel then sums that field's
+// strip partials in a fixed order and advances its scalar CG state (alpha,
+// beta, the exits of :530 and :547-548), so no host round trip happens inside
+// the solve and converged fields drop out by themselves (per-field masks).
+// chi, g and d are ping-ponged so a block never reads a neighbour's
+// already-updated value.
 #include "hvd_device.h"
 #include "hvd_kernels.h"
+
 
 namespace hvdk {
 
@@ -63,10 +65,12 @@ struct Carrier {
   float2 col;
   float2 rowl;          // row phasor of row ybase + lane
   int vsl;              // its V-switch sign (< 0: switched line)
-  int x, ybase, w;
+  int x, ybase, w, h;
   const float2* plane;  // explicit carrier plane, or null
+  // Valid for rows and columns inside the plane; elsewhere (halo rows and
+  // lanes) the value is unspecified but the access stays in bounds.
   __device__ inline float2 at(int y) const {
-    if (plane) return gld(plane, (size_t)y * w + x);
+    if (plane) return gld(plane, (size_t)min(max(y, 0), h - 1) * w + min(max(x, 0), w - 1));
     const int l = y - ybase;  // uniform across the wave
     const float2 rp = make_float2(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(rowl.x), l)),
                                   __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rowl.y), l)));
@@ -80,6 +84,7 @@ __device__ inline Carrier carrier_for(const DevField& f, const StripCtx& c) {
   Carrier k;
   k.x = c.x;
   k.w = f.w;
+  k.h = f.h;
   k.ybase = c.y0 - 1;
   k.plane = f.carrier;
   k.col = make_float2(0.f, 0.f);
@@ -106,10 +111,16 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-// Neighbour exchange: value of lane - 1 (left) / lane + 1 (right).  Lanes 0
-// and 63 get their own value back and never use it.
-__device__ inline float shl(float v) { return __shfl_up(v, 1, 64); }
-__device__ inline float shr(float v) { return __shfl_down(v, 1, 64); }
+// Neighbour exchange across the whole wave with DPP lane shifts (wave_shr:1,
+// wave_shl:1; no LDS round trip): shl gives lane l the value of lane l - 1,
+// shr the value of lane l + 1.  Lanes 0 (shl) and 63 (shr) get 0 and never
+// use it.  Call with every lane active.
+__device__ inline float shl(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+__device__ inline float shr(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
 __device__ inline float2 shl(float2 v) { return make_float2(shl(v.x), shl(v.y)); }
 __device__ inline float2 shr(float2 v) { return make_float2(shr(v.x), shr(v.y)); }
 __device__ inline float4 shl(float4 v) { return make_float4(shl(v.x), shl(v.y), shl(v.z), shl(v.w)); }
@@ -163,74 +174,29 @@ __device__ inline Sample make_sample(const DevField& f, const Carrier& k, const 
   return s;
 }
 
-__device__ inline float4 weights_row(const DevField& f, int y, int x, bool valid) {
-  return row_in(f, y, valid) ? gld(f.wts, (size_t)y * f.w + x) : make_float4(0.f, 0.f, 0.f, 0.f);
-}
-
 // g = -2 lam_y [DxT(wx Dx yc) + DyT(wy Dy yc)] conj(c)
 //     + 2 [mu_h DxT(wcx Dx chi) + mu_v DyT(wcy Dy chi)]      (:286-335)
-// from the 5-point neighbourhood; the has-neighbour flags apply the Neumann
-// boundaries of gradients.h (a missing neighbour contributes a zero difference).
-__device__ inline float2 grad_from(const FieldCG& st, const Sample& p0, const Sample& pl, const Sample& pr,
-                                   const Sample& pu, const Sample& pd, float4 w0, float4 wl, float4 wu, bool hl,
-                                   bool hr, bool hu, bool hd) {
-  if (!hl) wl = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (!hu) wu = make_float4(0.f, 0.f, 0.f, 0.f);
+// from the 5-point neighbourhood.  The Neumann boundaries of gradients.h (a
+// missing neighbour contributes a zero difference) come from the weights: the
+// stored weight of an edge that leaves the plane is zero (k_irls_weights), and
+// the weights of rows and columns outside the plane load as zero, so every
+// neighbour value only has to be finite.
+__device__ inline float2 grad_at(const FieldCG& st, const Sample& p0, const Sample& pl, const Sample& pr,
+                                 const Sample& pu, const Sample& pd, float4 w0, float4 wl, float4 wu) {
   // luma data term
-  const float gx0 = hr ? pr.yc - p0.yc : 0.f;
-  const float gxl = hl ? p0.yc - pl.yc : 0.f;
-  const float gy0 = hd ? pd.yc - p0.yc : 0.f;
-  const float gyu = hu ? p0.yc - pu.yc : 0.f;
+  const float gx0 = pr.yc - p0.yc, gxl = p0.yc - pl.yc;
+  const float gy0 = pd.yc - p0.yc, gyu = p0.yc - pu.yc;
   const float img = 2.f * st.lam_y * ((wl.x * gxl - w0.x * gx0) + (wu.y * gyu - w0.y * gy0));
   // chroma prior
-  const float2 cx0 = hr ? make_float2(pr.chi.x - p0.chi.x, pr.chi.y - p0.chi.y) : make_float2(0.f, 0.f);
-  const float2 cxl = hl ? make_float2(p0.chi.x - pl.chi.x, p0.chi.y - pl.chi.y) : make_float2(0.f, 0.f);
-  const float2 cy0 = hd ? make_float2(pd.chi.x - p0.chi.x, pd.chi.y - p0.chi.y) : make_float2(0.f, 0.f);
-  const float2 cyu = hu ? make_float2(p0.chi.x - pu.chi.x, p0.chi.y - pu.chi.y) : make_float2(0.f, 0.f);
+  const float2 cx0 = make_float2(pr.chi.x - p0.chi.x, pr.chi.y - p0.chi.y);
+  const float2 cxl = make_float2(p0.chi.x - pl.chi.x, p0.chi.y - pl.chi.y);
+  const float2 cy0 = make_float2(pd.chi.x - p0.chi.x, pd.chi.y - p0.chi.y);
+  const float2 cyu = make_float2(p0.chi.x - pu.chi.x, p0.chi.y - pu.chi.y);
   const float2 dxt = make_float2(wl.z * cxl.x - w0.z * cx0.x, wl.z * cxl.y - w0.z * cx0.y);
   const float2 dyt = make_float2(wu.w * cyu.x - w0.w * cy0.x, wu.w * cyu.y - w0.w * cy0.y);
   const float2 cp = make_float2(2.f * (st.mu_h * dxt.x + st.mu_v * dyt.x), 2.f * (st.mu_h * dxt.y + st.mu_v * dyt.y));
   const float2 c0 = p0.car;
   return make_float2(-img * c0.x + cp.x, img * c0.y + cp.y);
-}
-
-// Walks a strip computing g at every output pixel from chi (+ alpha d); calls
-// emit(i, g, chi_centre) for each.  Shared by the stage-start gradient and the
-// CG step.
-template <typename Emit>
-__device__ inline void grad_strip(const DevField& f, const FieldCG& st, const StripCtx& c, int rows,
-                                  const float2* chi, const float2* dd, float alpha, Emit emit) {
-  const Carrier k = carrier_for(f, c);
-  const bool wd = dd != nullptr;
-  Sample up = make_sample(f, k, raw_row(f, k, chi, dd, c.y0 - 1, c.valid), wd, alpha, c.y0 - 1, c.valid);
-  Sample cur = make_sample(f, k, raw_row(f, k, chi, dd, c.y0, c.valid), wd, alpha, c.y0, c.valid);
-  float4 wu = weights_row(f, c.y0 - 1, c.x, c.valid);
-  float4 wc = weights_row(f, c.y0, c.x, c.valid);
-  for (int r = 0; r < rows; ++r) {
-    const int y = c.y0 + r;
-    if (y >= f.h) break;  // uniform across the wave
-    const float4 wn = weights_row(f, y + 1, c.x, c.valid);
-    const Sample dn = make_sample(f, k, raw_row(f, k, chi, dd, y + 1, c.valid), wd, alpha, y + 1, c.valid);
-    Sample pl, pr;
-    pl.yc = shl(cur.yc);
-    pr.yc = shr(cur.yc);
-    pl.chi = shl(cur.chi);
-    pr.chi = shr(cur.chi);
-    const float4 wl = shl(wc);
-    if (c.out) {
-      float2 g = grad_from(st, cur, pl, pr, up, dn, wc, wl, wu, c.x > 0, c.x + 1 < f.w, y > 0, y + 1 < f.h);
-      const size_t i = (size_t)y * f.w + c.x;
-      if (f.rhs) {
-        const float2 b = gld(f.rhs, i);
-        g = make_float2(g.x - b.x, g.y - b.y);
-      }
-      emit(i, g, cur.chi);
-    }
-    up = cur;
-    cur = dn;
-    wu = wc;
-    wc = wn;
-  }
 }
 
 // Wave-level FP64 partial of this strip, published to its slot.  The
@@ -251,6 +217,183 @@ __device__ inline double field_total(const double* partials, const int* tp, int 
   double acc = 0.0;
   for (int b = threadIdx.x; b < nblk; b += blockDim.x) acc += partials[(size_t)(t0 + b) * NV + k];
   return block_sum<256>(acc, lds);
+}
+
+// One row of this lane's column as the fused CG iteration forms it: the
+// direction d(it) = -g + beta d(it-1) (or -g on the first iteration), the moved
+// chroma chi' = chi + alpha d(it), its luma S - Re[chi' c] and the carrier.
+// In the stage-start pass (Start) chi' = chi and there is no direction.
+struct IterRow {
+  Sample p;
+  float2 d;
+  float dY;  // -Re[d c]: the luma the direction moves
+};
+
+// Loads of one row of this lane's column for the fused iteration, issued
+// without branches at clamped addresses, so rows and columns outside the
+// plane read finite values from its edge (the zero weights there cancel
+// them).  Upd: the previous direction is read; Adj: the adjoint solve (S = 0,
+// no composite plane).
+struct IterRaw {
+  float2 chi, g, dp;
+  float s;
+};
+
+template <bool Start, bool Upd, bool Adj>
+__device__ inline IterRaw iter_fetch(const DevField& f, int x, const float2* chi, const float2* g, const float2* dprev,
+                                     int y) {
+  IterRaw r;
+  const int yy = min(max(y, 0), f.h - 1), xx = min(max(x, 0), f.w - 1);
+  const unsigned i = (unsigned)(yy * f.w + xx);
+  r.chi = gld32(chi, i);
+  r.s = Adj ? 0.f : gld32(f.s, i);
+  r.g = Start ? make_float2(0.f, 0.f) : gld32(g, i);
+  r.dp = Upd ? gld32(dprev, i) : make_float2(0.f, 0.f);
+  return r;
+}
+
+template <bool Start, bool Upd>
+__device__ inline IterRow iter_make(const Carrier& k, const IterRaw& q, float alpha, float beta, int y) {
+  IterRow r;
+  r.p.car = k.at(y);
+  float2 c = q.chi;
+  r.d = make_float2(0.f, 0.f);
+  r.dY = 0.f;
+  if (!Start) {
+    r.d = Upd ? dir_update(q.g, beta, q.dp) : make_float2(-q.g.x, -q.g.y);
+    r.dY = -re_mul(r.d, r.p.car);
+    c = axpy(c, alpha, r.d);
+  }
+  r.p.chi = c;
+  r.p.yc = q.s - re_mul(c, r.p.car);
+  return r;
+}
+
+__device__ inline float4 weights_fetch(const DevField& f, int y, int x, bool valid) {
+  const int yy = min(max(y, 0), f.h - 1), xx = min(max(x, 0), f.w - 1);
+  const float4 v = gld32(f.wts, (unsigned)(yy * f.w + xx));
+  return row_in(f, y, valid) ? v : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// One pixel's share of curv(a) and of the bilinear form curv(a, b)
+// (variational.cpp:352-399) from forward differences to the right and down
+// neighbours, with the pixel's own weights (zero on edges that leave the
+// plane); aY = -Re[a c].  The pixel's terms are formed in float and summed in
+// FP64; `on` false adds nothing.
+__device__ inline void curv_terms(const FieldCG& s, float4 w, bool on, float2 a0, float2 ar, float2 ad, float aY0,
+                                  float aYr, float aYd, float2 b0, float2 br, float2 bd, float bY0, float bYr,
+                                  float bYd, bool with_b, double& A, double& B) {
+  const float axY = aYr - aY0, ayY = aYd - aY0;
+  const float2 ax = make_float2(ar.x - a0.x, ar.y - a0.y);
+  const float2 ay = make_float2(ad.x - a0.x, ad.y - a0.y);
+  const float ta = s.lam_y * (w.x * axY * axY + w.y * ayY * ayY) + s.mu_h * w.z * (ax.x * ax.x + ax.y * ax.y) +
+                   s.mu_v * w.w * (ay.x * ay.x + ay.y * ay.y);
+  A += (double)(on ? ta : 0.f);
+  if (with_b) {
+    const float bxY = bYr - bY0, byY = bYd - bY0;
+    const float2 bx = make_float2(br.x - b0.x, br.y - b0.y);
+    const float2 by = make_float2(bd.x - b0.x, bd.y - b0.y);
+    const float tb = s.lam_y * (w.x * axY * bxY + w.y * ayY * byY) + s.mu_h * w.z * (ax.x * bx.x + ax.y * bx.y) +
+                     s.mu_v * w.w * (ay.x * by.x + ay.y * by.y);
+    B += (double)(on ? tb : 0.f);
+  }
+}
+
+// The fused CG iteration over one strip (variational.cpp:527-548, one
+// iteration per launch).  With d = d(it) formed on the fly and
+// chi' = chi + alpha d, it writes chi', d and g' = grad(chi') (:286-347, less
+// rhs in the adjoint solve) and the strip's FP64 partials of
+//   |g'|^2,  curv(g'),  curv(g', d),  Re<g', d>,
+// from which the next reduction forms beta = |g'|^2 / |g|^2 and, without
+// another pass over the planes, the next direction's
+//   curv(-g' + beta d) = curv(g') - 2 beta curv(g', d) + beta^2 curv(d)
+//   Re<g', -g' + beta d> = -|g'|^2 + beta Re<g', d>.
+// The stage-start pass (Start) computes g = grad(chi) and the partials of |g|^2
+// and curv(g) (the first direction is -g).  Lanes 1..kStripCols + 1 form g';
+// lanes 1..kStripCols own the outputs and the partials (a pixel's curvature
+// terms need g' one column right and one row down, so the walk forms g' for
+// rows + 1 lines).  The loop body is branch-free apart from the stores: loads
+// are clamped and results selected.
+template <bool Start, bool Upd, bool Adj>
+__device__ inline void cg_strip(const DevField& f, const FieldCG& s, const StripCtx& c, int rows, double* partials) {
+  const Carrier k = carrier_for(f, c);
+  const float2* chi = pick(f.chi, s.cur);
+  const float2* gsrc = pick(f.g, s.cur);
+  const float2* dprev = pick(f.d, s.cur);
+  float2* chi_out = pick(f.chi, s.cur ^ 1);
+  float2* g_out = pick(f.g, Start ? s.cur : s.cur ^ 1);
+  float2* d_out = pick(f.d, s.cur ^ 1);
+  const float alpha = s.alpha, beta = s.beta;
+  const bool calc = c.valid && lane_id() >= 1 && lane_id() <= kStripCols + 1;
+  auto fetch = [&](int y) { return iter_fetch<Start, Upd, Adj>(f, c.x, chi, gsrc, dprev, y); };
+  auto make = [&](const IterRaw& q, int y) { return iter_make<Start, Upd>(k, q, alpha, beta, y); };
+  float2 gp = make_float2(0.f, 0.f);  // g' of the row above
+  float gYp = 0.f;
+  double gg = 0.0, A = 0.0, B = 0.0, D = 0.0;
+  // One line of the walk (row y = y0 + j).  The rows above / at / below y and
+  // their weights live in three slots each that rotate between calls, so no
+  // registers are copied from line to line.  Returns false when the walk ends.
+  auto step = [&](int j, const IterRow& up, const IterRow& cur, IterRow& dn, const float4& wu, const float4& wc,
+                  float4& wn) -> bool {
+    if (j > rows) return false;
+    const int y = c.y0 + j;
+    if (y >= f.h) {  // uniform: the plane ends inside the walk
+      const float2 gpr = shr(gp), dpr = shr(up.d);
+      const float gYpr = shr(gYp), dYpr = shr(up.dY);
+      curv_terms(s, wu, j >= 1 && c.out, gp, gpr, gp, gYp, gYpr, gYp, up.d, dpr, up.d, up.dY, dYpr, up.dY, !Start, A,
+                 B);
+      return false;
+    }
+    wn = weights_fetch(f, y + 1, c.x, c.valid);
+    dn = make(fetch(y + 1), y + 1);
+    Sample pl, pr;
+    pl.yc = shl(cur.p.yc);
+    pr.yc = shr(cur.p.yc);
+    pl.chi = shl(cur.p.chi);
+    pr.chi = shr(cur.p.chi);
+    const float4 wl = shl(wc);
+    float2 gv = grad_at(s, cur.p, pl, pr, up.p, dn.p, wc, wl, wu);
+    if (Adj) {
+      const int xx = min(max(c.x, 0), f.w - 1);
+      const float2 b = gld32(f.rhs, (unsigned)(y * f.w + xx));
+      gv = make_float2(gv.x - b.x, gv.y - b.y);
+    }
+    gv = calc ? gv : make_float2(0.f, 0.f);
+    const float gY = -re_mul(gv, cur.p.car);
+    // curvature terms of the row above, whose down neighbour is this row
+    const float2 gpr = shr(gp), dpr = shr(up.d);
+    const float gYpr = shr(gYp), dYpr = shr(up.dY);
+    curv_terms(s, wu, j >= 1 && c.out, gp, gpr, gv, gYp, gYpr, gY, up.d, dpr, cur.d, up.dY, dYpr, cur.dY, !Start, A,
+               B);
+    const bool own = j < rows && c.out;
+    gg += own ? (double)gv.x * gv.x + (double)gv.y * gv.y : 0.0;
+    if (!Start) D += own ? (double)gv.x * cur.d.x + (double)gv.y * cur.d.y : 0.0;
+    if (own) {
+      const unsigned i = (unsigned)(y * f.w + c.x);
+      gst32(g_out, i, gv);
+      if (!Start) {
+        gst32(chi_out, i, cur.p.chi);
+        gst32(d_out, i, cur.d);
+      }
+    }
+    gp = gv;
+    gYp = gY;
+    return true;
+  };
+  IterRow r0 = make(fetch(c.y0 - 1), c.y0 - 1), r1 = make(fetch(c.y0), c.y0), r2;
+  float4 w0 = weights_fetch(f, c.y0 - 1, c.x, c.valid), w1 = weights_fetch(f, c.y0, c.x, c.valid), w2;
+  for (int j = 0;; j += 3) {
+    if (!step(j, r0, r1, r2, w0, w1, w2)) break;
+    if (!step(j + 1, r1, r2, r0, w1, w2, w0)) break;
+    if (!step(j + 2, r2, r0, r1, w2, w0, w1)) break;
+  }
+  const int slot = f.tile0 + c.lt;
+  publish(partials, slot, 0, kCgPartials, 0, gg);
+  publish(partials, slot, 0, kCgPartials, 1, A);
+  if (!Start) {
+    publish(partials, slot, 0, kCgPartials, 2, B);
+    publish(partials, slot, 0, kCgPartials, 3, D);
+  }
 }
 
 }  // namespace
@@ -359,7 +502,7 @@ __global__ void __launch_bounds__(64) k_irls_weights(const DevField* __restrict_
   const DevField f = F[c.f];  // a private copy: stores cannot alias it
   const FieldCG s = st[c.f];
   const Carrier k = carrier_for(f, c);
-  const float2* chi = pick(f.chi, s.chi_cur);
+  const float2* chi = pick(f.chi, s.cur);
   const float e2 = s.eps * s.eps, ec2 = s.eps_c * s.eps_c;
   Sample cur = make_sample(f, k, raw_row(f, k, chi, nullptr, c.y0, c.valid), false, 0.f, c.y0, c.valid);
   for (int r = 0; r < rows; ++r) {
@@ -375,11 +518,13 @@ __global__ void __launch_bounds__(64) k_irls_weights(const DevField* __restrict_
       const float2 cx = hr ? make_float2(cr.x - cur.chi.x, cr.y - cur.chi.y) : make_float2(0.f, 0.f);
       const float2 cy = hd ? make_float2(dn.chi.x - cur.chi.x, dn.chi.y - cur.chi.y) : make_float2(0.f, 0.f);
       const float mx = sqrtf(cx.x * cx.x + cx.y * cx.y), my = sqrtf(cy.x * cy.x + cy.y * cy.y);
+      // an edge that leaves the plane stores weight 0 (its difference is 0 in
+      // hvd-core, so its term vanishes either way)
       float4 w;
-      w.x = s.eps / sqrtf(gx * gx + e2);
-      w.y = s.eps / sqrtf(gy * gy + e2);
-      w.z = s.eps_c / sqrtf(mx * mx + s.k * gx * gx + ec2);
-      w.w = s.eps_c / sqrtf(my * my + s.k * gy * gy + ec2);
+      w.x = hr ? s.eps / sqrtf(gx * gx + e2) : 0.f;
+      w.y = hd ? s.eps / sqrtf(gy * gy + e2) : 0.f;
+      w.z = hr ? s.eps_c / sqrtf(mx * mx + s.k * gx * gx + ec2) : 0.f;
+      w.w = hd ? s.eps_c / sqrtf(my * my + s.k * gy * gy + ec2) : 0.f;
       gst(f.wts, (size_t)y * f.w + c.x, w);
       if (f.save) gst(f.save + (size_t)stage * f.h * f.w, (size_t)y * f.w + c.x, cur.chi);
     }
@@ -387,166 +532,90 @@ __global__ void __launch_bounds__(64) k_irls_weights(const DevField* __restrict_
   }
 }
 
-// Stage start (variational.cpp:519-525): g = grad(chi), d = -g, partial |g|^2.
-__global__ void __launch_bounds__(64) k_grad_init(const DevField* __restrict__ F,
-    const FieldCG* __restrict__ st, StripGrid sg,
-                                                                 double* partials) {
+// Stage start (variational.cpp:519-525): g = grad(chi) and the partials of
+// |g|^2 and curv(g) (the first direction is d = -g).
+__global__ void __launch_bounds__(64) k_cg_start(const DevField* __restrict__ F, const FieldCG* __restrict__ st,
+                                                 StripGrid sg, double* partials, int adjoint) {
   const StripCtx c = strip_ctx(F, sg);
-  const int rows = sg.rows;
   const DevField f = F[c.f];  // a private copy: stores cannot alias it
   const FieldCG s = st[c.f];
-  float2* dd = pick(f.d, s.d_cur);
-  double gg = 0.0;
-  grad_strip(f, s, c, rows, pick(f.chi, s.chi_cur), nullptr, 0.f, [&](size_t i, float2 g, float2) {
-    gst(f.g, i, g);
-    gst(dd, i, make_float2(-g.x, -g.y));
-    gg += (double)g.x * g.x + (double)g.y * g.y;
-  });
-  publish(partials, F[c.f].tile0, c.lt, 1, 0, gg);
+  if (adjoint) cg_strip<true, false, true>(f, s, c, sg.rows, partials);
+  else cg_strip<true, false, false>(f, s, c, sg.rows, partials);
 }
 
-// gg0 = |g|^2 and the inner-loop reset (variational.cpp:524-525).
-__global__ void __launch_bounds__(256) k_reduce_init(FieldCG* st, const int* tp, const double* partials) {
+// gg0 = |g|^2, the inner-loop reset, and the first iteration's curv(d) =
+// curv(g), Re<g, d> = -|g|^2 and alpha (variational.cpp:524-531).
+__global__ void __launch_bounds__(256) k_reduce_start(FieldCG* st, const int* tp, const double* partials) {
   const int fi = blockIdx.x;
   FieldCG& s = st[fi];
-  const double gg = field_total<1>(partials, tp, fi, 0);
+  const double gg = field_total<kCgPartials>(partials, tp, fi, 0);
+  const double hh = field_total<kCgPartials>(partials, tp, fi, 1);
   if (threadIdx.x == 0) {
     s.gg = gg;
     s.gg0 = gg;
     s.it = 0;
-    s.done = s.n_inner <= 0 ? 1 : 0;
-  }
-}
-
-// One CG curvature pass (variational.cpp:527-531): for it > 0 first applies
-// the previous iteration's d <- -g + beta d (:541-543) on the fly.
-__global__ void __launch_bounds__(64) k_cg_curv(const DevField* __restrict__ F,
-    const FieldCG* __restrict__ st, StripGrid sg,
-                                                               double* partials, int it) {
-  const StripCtx c = strip_ctx(F, sg);
-  const int rows = sg.rows;
-  const DevField f = F[c.f];  // a private copy: stores cannot alias it
-  const FieldCG s = st[c.f];
-  if (s.done) return;
-  const Carrier k = carrier_for(f, c);
-  const float2* dsrc = pick(f.d, s.d_cur);
-  float2* ddst = pick(f.d, s.d_cur ^ 1);
-  const bool upd = it > 0;
-  const float beta = s.beta;
-  struct Raw {
-    float2 g, d;
-  };
-  auto raw = [&](int y) -> Raw {
-    Raw r{make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
-    if (row_in(f, y, c.valid)) {
-      const size_t i = (size_t)y * f.w + c.x;
-      r.g = gld(f.g, i);
-      r.d = gld(dsrc, i);
-    }
-    return r;
-  };
-  // direction, gradient and -Re[d c] of row y in this lane's column
-  struct Row {
-    float2 d, g;
-    float dy;
-  };
-  auto row = [&](const Raw& q, int y) -> Row {
-    Row r{make_float2(0.f, 0.f), make_float2(0.f, 0.f), 0.f};
-    if (row_in(f, y, c.valid)) {
-      r.g = q.g;
-      r.d = upd ? dir_update(q.g, beta, q.d) : q.d;
-      r.dy = -re_mul(r.d, k.at(y));
-    }
-    return r;
-  };
-  double hh = 0.0, gd = 0.0;
-  Row cur = row(raw(c.y0), c.y0);
-  for (int r = 0; r < rows; ++r) {
-    const int y = c.y0 + r;
-    if (y >= f.h) break;
-    const float4 w = c.out ? gld(f.wts, (size_t)y * f.w + c.x) : make_float4(0.f, 0.f, 0.f, 0.f);
-    const Row dn = row(raw(y + 1), y + 1);
-    const float2 dr = shr(cur.d);
-    const float dyr = shr(cur.dy);
-    if (c.out) {
-      const size_t i = (size_t)y * f.w + c.x;
-      if (upd) gst(ddst, i, cur.d);
-      const bool hr = c.x + 1 < f.w, hd = y + 1 < f.h;
-      float dxdy = 0.f, dydy = 0.f;
-      float2 dxc = make_float2(0.f, 0.f), dyc = dxc;
-      if (hr) {
-        dxdy = dyr - cur.dy;
-        dxc = make_float2(dr.x - cur.d.x, dr.y - cur.d.y);
-      }
-      if (hd) {
-        dydy = dn.dy - cur.dy;
-        dyc = make_float2(dn.d.x - cur.d.x, dn.d.y - cur.d.y);
-      }
-      double h = (double)s.lam_y * ((double)w.x * dxdy * dxdy + (double)w.y * dydy * dydy);
-      h += (double)s.mu_h * w.z * (double)(dxc.x * dxc.x + dxc.y * dxc.y);
-      h += (double)s.mu_v * w.w * (double)(dyc.x * dyc.x + dyc.y * dyc.y);
-      hh += h;
-      gd += (double)cur.g.x * cur.d.x + (double)cur.g.y * cur.d.y;
-    }
-    cur = dn;
-  }
-  publish(partials, F[c.f].tile0, c.lt, 2, 0, hh);
-  publish(partials, F[c.f].tile0, c.lt, 2, 1, gd);
-}
-
-// alpha = -1/2 <g, d> / curv(d), or stop when curv(d) <= 1e-12
-// (variational.cpp:528-531); commits the d ping-pong of iteration it.
-__global__ void __launch_bounds__(256) k_reduce_curv(FieldCG* st, const int* tp, const double* partials, int it) {
-  const int fi = blockIdx.x;
-  FieldCG& s = st[fi];
-  if (s.done) return;
-  const double hh = field_total<2>(partials, tp, fi, 0);
-  const double gd = field_total<2>(partials, tp, fi, 1);
-  if (threadIdx.x == 0) {
-    if (it > 0) s.d_cur ^= 1;
+    s.beta = 0.f;
     s.hh = hh;
-    s.gd = gd;
-    if (hh <= s.hh_floor) {
-      s.done = 1;
-    } else {
-      s.alpha = (float)(-0.5 * gd / hh);
-    }
+    s.gd = -gg;
+    s.done = s.n_inner <= 0 || hh <= s.hh_floor ? 1 : 0;
+    if (!s.done) s.alpha = (float)(-0.5 * s.gd / hh);
   }
 }
 
-// One CG step (variational.cpp:532-548): chi <- chi + alpha d, g <- grad(chi),
-// partial |g|^2.
-__global__ void __launch_bounds__(64) k_cg_step(const DevField* __restrict__ F,
-    const FieldCG* __restrict__ st, StripGrid sg,
-                                                               double* partials) {
+// CG iteration `it` (variational.cpp:527-548): d <- -g + beta d, chi <- chi +
+// alpha d, g <- grad(chi), and the partials the next reduction needs.
+__global__ void __launch_bounds__(64) k_cg_iter(const DevField* __restrict__ F, const FieldCG* __restrict__ st,
+                                                StripGrid sg, double* partials, int it) {
   const StripCtx c = strip_ctx(F, sg);
-  const int rows = sg.rows;
   const DevField f = F[c.f];  // a private copy: stores cannot alias it
   const FieldCG s = st[c.f];
   if (s.done) return;
-  float2* chi_out = pick(f.chi, s.chi_cur ^ 1);
-  double gg = 0.0;
-  grad_strip(f, s, c, rows, pick(f.chi, s.chi_cur), pick(f.d, s.d_cur), s.alpha,
-             [&](size_t i, float2 g, float2 cc) {
-    gst(f.g, i, g);
-    gst(chi_out, i, cc);
-    gg += (double)g.x * g.x + (double)g.y * g.y;
-  });
-  publish(partials, F[c.f].tile0, c.lt, 1, 0, gg);
+  if (it > 0) cg_strip<false, true, false>(f, s, c, sg.rows, partials);
+  else cg_strip<false, false, false>(f, s, c, sg.rows, partials);
 }
 
-// beta, the chi ping-pong, and the exits of variational.cpp:540-548.
-__global__ void __launch_bounds__(256) k_reduce_step(FieldCG* st, const int* tp, const double* partials) {
+// The same iteration for an adjoint solve (no composite; the right-hand side
+// is subtracted from the gradient).
+__global__ void __launch_bounds__(64) k_cg_iter_adj(const DevField* __restrict__ F, const FieldCG* __restrict__ st,
+                                                    StripGrid sg, double* partials, int it) {
+  const StripCtx c = strip_ctx(F, sg);
+  const DevField f = F[c.f];  // a private copy: stores cannot alias it
+  const FieldCG s = st[c.f];
+  if (s.done) return;
+  if (it > 0) cg_strip<false, true, true>(f, s, c, sg.rows, partials);
+  else cg_strip<false, false, true>(f, s, c, sg.rows, partials);
+}
+
+// End of an iteration (variational.cpp:536-548, then :528-531 of the next):
+// beta, the exits on |g|^2 and the iteration budget, the next direction's
+// curvature (exit when <= hh_floor) and alpha; flips the ping-pong slot.
+__global__ void __launch_bounds__(256) k_reduce_iter(FieldCG* st, const int* tp, const double* partials) {
   const int fi = blockIdx.x;
   FieldCG& s = st[fi];
   if (s.done) return;
-  const double ggn = field_total<1>(partials, tp, fi, 0);
+  const double ggn = field_total<kCgPartials>(partials, tp, fi, 0);
+  const double cg = field_total<kCgPartials>(partials, tp, fi, 1);
+  const double cgd = field_total<kCgPartials>(partials, tp, fi, 2);
+  const double gdp = field_total<kCgPartials>(partials, tp, fi, 3);
   if (threadIdx.x == 0) {
-    s.chi_cur ^= 1;
-    s.beta = (float)(ggn / (s.gg > 1e-30 ? s.gg : 1e-30));
+    s.cur ^= 1;
+    const float beta = (float)(ggn / (s.gg > 1e-30 ? s.gg : 1e-30));
     s.gg = ggn;
     s.it += 1;
-    if (ggn < s.tol2 * s.gg0 || ggn < s.gg_floor * s.npix || s.it >= s.n_inner) s.done = 1;
+    if (ggn < s.tol2 * s.gg0 || ggn < s.gg_floor * s.npix || s.it >= s.n_inner) {
+      s.done = 1;
+      return;
+    }
+    const double b = (double)beta;
+    const double hh = cg - 2.0 * b * cgd + b * b * s.hh;
+    s.hh = hh;
+    if (hh <= s.hh_floor) {
+      s.done = 1;
+      return;
+    }
+    s.beta = beta;
+    s.gd = -ggn + b * gdp;
+    s.alpha = (float)(-0.5 * s.gd / hh);
   }
 }
 
@@ -566,7 +635,7 @@ __global__ void __launch_bounds__(64) k_refine_finalize(const DevField* __restri
   const DevField f = F[c.f];  // a private copy: stores cannot alias it
   if (!c.out) return;
   const Carrier k = carrier_for(f, c);
-  const int cur = st[c.f].chi_cur;
+  const int cur = st[c.f].cur;
   for (int r = 0; r < rows; ++r) {
     const int y = c.y0 + r;
     if (y >= f.h) break;

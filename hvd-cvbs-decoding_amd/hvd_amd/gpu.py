@@ -25,6 +25,7 @@ ABI_SYMBOLS = (
     "hvd_gpu_decode_frame_window_s16", "hvd_gpu_split_frame_fields", "hvd_gpu_field_parity_vote",
     "hvd_gpu_make_carrier", "hvd_gpu_tridiag_smooth", "hvd_gpu_box_blur", "hvd_gpu_chroma_plane",
     "hvd_gpu_weave_frame_s16", "hvd_gpu_variational_refine_stages", "hvd_gpu_refine_adjoint",
+    "hvd_gpu_set_solver_tuning", "hvd_gpu_autotune_solver",
 )
 
 
@@ -64,6 +65,11 @@ class SolverParams(C.Structure):
                  cg_tol=0.0, fast=1, is_pal=0, lambda_y=1.0)
         d.update(kw)
         return cls(**d)
+
+
+class SolverTuning(C.Structure):
+    """Solver launch shape: lines per strip, most pixels per solve group (0: built-in)."""
+    _fields_ = [("strip_rows", C.c_int), ("group_pixels", C.c_longlong)]
 
 
 class Plane(C.Structure):
@@ -147,6 +153,8 @@ def load() -> C.CDLL:
         lib.hvd_gpu_set_profiling.argtypes = [vp, i]
         lib.hvd_gpu_cg_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                                           C.POINTER(C.c_double)]
+        lib.hvd_gpu_set_solver_tuning.argtypes = [vp, C.POINTER(SolverTuning)]
+        lib.hvd_gpu_autotune_solver.argtypes = [vp, i, i, i, i, i, C.POINTER(SolverTuning), C.POINTER(C.c_double)]
         _lib = lib
     return _lib
 
@@ -197,10 +205,25 @@ class Engine:
         self._check(self.lib.hvd_gpu_set_profiling(self.ctx, int(on)), "set_profiling")
 
     def cg_timing(self):
-        """(CG-loop ms, CG launches, pixel-iterations) since profiling was enabled."""
+        """(CG-loop ms, CG iterations launched per solve group, pixel-iterations) since profiling was enabled."""
         ms, nl, pxit = C.c_double(), C.c_longlong(), C.c_double()
         self._check(self.lib.hvd_gpu_cg_timing(self.ctx, C.byref(ms), C.byref(nl), C.byref(pxit)), "cg_timing")
         return ms.value, nl.value, pxit.value
+
+    # ---- solver launch shape ------------------------------------------
+    def set_solver_tuning(self, strip_rows: int = 0, group_pixels: int = 0):
+        """Fix the solve's strip height and solve-group size (zeros: autotuned / built-in)."""
+        t = SolverTuning(strip_rows, group_pixels)
+        self._check(self.lib.hvd_gpu_set_solver_tuning(self.ctx, C.byref(t)), "set_solver_tuning")
+
+    def autotune_solver(self, lines: int, width: int, is_pal: bool, nfields: int, iterations: int = 8):
+        """Time every candidate launch shape on synthetic fields of this shape and record the fastest.
+
+        Returns (strip_rows, group_pixels, ns per pixel-iteration)."""
+        t, ns = SolverTuning(), C.c_double()
+        self._check(self.lib.hvd_gpu_autotune_solver(self.ctx, lines, width, int(is_pal), nfields, iterations,
+                                                     C.byref(t), C.byref(ns)), "autotune_solver")
+        return t.strip_rows, t.group_pixels, ns.value
 
     # ---- level conversion ----------------------------------------------
     def sample_to_ire_u16(self, raw, blanking: float, white: float):

@@ -151,11 +151,42 @@ int hvd_gpu_abi_version(void);
  * refine records HIP events on its stream around each IRLS stage's CG loop
  * and keeps a copy of the per-field iteration counts.  hvd_gpu_cg_timing()
  * synchronizes the device and returns, over all refines since profiling was
- * enabled or last read: the summed CG-loop time (ms), the number of CG kernel
- * launches in those loops, and sum over fields of pixels x CG iterations run;
+ * enabled or last read: the summed CG-loop time (ms), the number of CG
+ * iterations launched in those loops (one stencil launch and one reduction
+ * each, per solve group), and sum over fields of pixels x CG iterations run;
  * it then clears the record. */
 int hvd_gpu_set_profiling(hvd_gpu_ctx* ctx, int enable);
-int hvd_gpu_cg_timing(hvd_gpu_ctx* ctx, double* ms, long long* launches, double* pixel_iterations);
+int hvd_gpu_cg_timing(hvd_gpu_ctx* ctx, double* ms, long long* iterations, double* pixel_iterations);
+
+/* ---- solver launch shape (no hvd-core counterpart) ----------------------- */
+
+/* How the IRLS/CG solve is launched: lines per solver strip (2..32; each
+ * 64-lane wave walks a strip of 61 columns) and the most pixels per solve
+ * group (consecutive fields solved together, sized so their planes stay in
+ * the Infinity Cache).  0 selects the built-in choice for that member. */
+typedef struct {
+  int strip_rows;
+  long long group_pixels;
+} hvd_gpu_solver_tuning;
+
+/* Fixes the launch shape for every later solve on this context; zeros hand
+ * the choice back to the autotuned entries and the built-in policy.  The
+ * results of a solve depend on the strip height only through the order of
+ * its FP64 sums. */
+int hvd_gpu_set_solver_tuning(hvd_gpu_ctx* ctx, const hvd_gpu_solver_tuning* t);
+
+/* The autotuner: times the CG loop of `nfields` synthetic fields of
+ * lines x width (PAL: with a V-switched carrier) for every candidate strip
+ * height and solve-group size, `iterations` CG iterations each, and records
+ * the fastest for that field shape.  Later solves whose fields all share the
+ * shape use that entry when their field count matches; otherwise they take
+ * the strip height of the entry with the nearest count and the default group
+ * size.  A fixed tuning wins over both.
+ * Returns the choice and its CG time per pixel-iteration in ns.  Allocates
+ * and frees its own synthetic planes, synchronizes the device and clears
+ * the profiling record. */
+int hvd_gpu_autotune_solver(hvd_gpu_ctx* ctx, int lines, int width, int is_pal, int nfields, int iterations,
+                            hvd_gpu_solver_tuning* best, double* ns_per_pixel_iteration);
 
 /* ---- level conversion -------------------------------------------------- */
 
