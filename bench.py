@@ -249,6 +249,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only-batched", action="store_true", help="time the headline workload only (profiling)")
     ap.add_argument("--no-autotune", action="store_true", help="use the built-in launch shapes (no autotuner)")
+    ap.add_argument("--solver-tuning", default="", metavar="ROWS,GROUP_PX",
+                    help="fix the solver's strip height and solve-group pixels instead of autotuning")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 512 MiB device-to-device copy and exit (reference point for PMC byte counters)")
     args = ap.parse_args()
@@ -283,7 +285,11 @@ def main():
         eng.decode_fields_u16(raws, geoms, p0, outputs=(ys, chis))
 
     tuned = []
-    if not args.no_autotune:  # before any timing: the tuner's own runs are not part of a step
+    if args.solver_tuning:
+        rows, gpx = (int(v) for v in args.solver_tuning.split(","))
+        eng.set_solver_tuning(rows, gpx)
+        tuned.append({"fixed": True, "strip_rows": rows, "group_pixels": gpx})
+    elif not args.no_autotune:  # before any timing: the tuner's own runs are not part of a step
         tuned.append(autotune(eng, g, BATCH))
         if not args.only_batched:
             # the other lines' batches: mixed (32 + 32), training (16), streaming windows (8), plain (1)
