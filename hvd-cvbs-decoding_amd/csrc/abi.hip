@@ -476,14 +476,14 @@ StripSetup strip_setup(std::vector<DevField>& F, int rows) {
 }
 
 // One stage's CG from the current chi: the stage-start gradient, then n_inner
-// fused iterations, each a stencil launch and a per-field reduction, with
-// per-field exits (variational.cpp:519-549).  adjoint: the fields describe an
-// adjoint solve (no composite, a right-hand side).
+// fused iterations (one launch each; a launch first completes the previous
+// one's per-field reductions), then the last update; per-field exits
+// (variational.cpp:519-549).  adjoint: the fields describe an adjoint solve
+// (no composite, a right-hand side).  stage: index for the iteration record.
 void cg_loop(hvd_gpu_ctx* ctx, const DevField* dF, FieldCG* dstate, const StripGrid& sg, double* partials, int nf,
-             int n_inner, bool adjoint, hipStream_t st) {
-  hipLaunchKernelGGL(k_cg_start, dim3(sg.nb), dim3(64), 0, st, dF, dstate, sg, partials, (int)adjoint);
-  LAUNCH_CHECK(ctx);
-  hipLaunchKernelGGL(k_reduce_start, dim3(nf), dim3(256), 0, st, dstate, sg.tp, (const double*)partials);
+             int n_inner, bool adjoint, int stage, hipStream_t st) {
+  hipLaunchKernelGGL(k_cg_start, dim3(sg.nb), dim3(64), 0, st, dF, (const FieldCG*)dstate, sg, partials,
+                     (int)adjoint);
   LAUNCH_CHECK(ctx);
   std::pair<hipEvent_t, hipEvent_t> evs{nullptr, nullptr};
   if (ctx->profiling) {
@@ -497,9 +497,9 @@ void cg_loop(hvd_gpu_ctx* ctx, const DevField* dF, FieldCG* dstate, const StripG
     else
       hipLaunchKernelGGL(k_cg_iter, dim3(sg.nb), dim3(64), 0, st, dF, dstate, sg, partials, it);
     LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_reduce_iter, dim3(nf), dim3(256), 0, st, dstate, sg.tp, (const double*)partials);
-    LAUNCH_CHECK(ctx);
   }
+  hipLaunchKernelGGL(k_cg_finish, dim3(nf), dim3(64), 0, st, dstate, sg, (const double*)partials, n_inner, stage);
+  LAUNCH_CHECK(ctx);
   if (ctx->profiling) {
     HIPCHK(ctx, hipEventRecord(evs.second, st));
     ctx->cg_events.push_back(evs);
@@ -521,7 +521,6 @@ FieldCG field_cg_init(const DevField& f, const hvd_solver_params& p, int n_inner
   s.aniso_forced = (float)p.chroma_aniso;
   s.is_pal = p.is_pal;
   s.n_inner = n_inner;
-  s.done = 1;
   return s;
 }
 
@@ -560,9 +559,7 @@ void run_refine(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_par
   for (int o = 0; o < outer; ++o) {
     hipLaunchKernelGGL(k_irls_weights, dim3(nb), dim3(64), 0, st, dF, dstate, sg, o);
     LAUNCH_CHECK(ctx);
-    cg_loop(ctx, dF, dstate, sg, partials, nf, n_inner, false, st);
-    hipLaunchKernelGGL(k_stage_end, dim3((nf + 63) / 64), dim3(64), 0, st, dstate, nf, o);
-    LAUNCH_CHECK(ctx);
+    cg_loop(ctx, dF, dstate, sg, partials, nf, n_inner, false, o, st);
   }
   hipLaunchKernelGGL(k_refine_finalize, dim3(nb), dim3(64), 0, st, dF, (const FieldCG*)dstate,
                      sg);
@@ -680,7 +677,7 @@ GroupLayout group_layout(Carve cv, const DevField* F, int n, int is_pal) {
     strips += (size_t)solver_strips(F[i].h, F[i].w, kMinStripRows);
     L.sp[i + 1] = L.sp[i] + aniso_samples(F[i].h, F[i].w, is_pal);
   }
-  L.part = cv.take(sizeof(double) * kCgPartials * strips);
+  L.part = cv.take(sizeof(double) * 2 * kCgPartials * strips);  // two launch slots
   L.gx = cv.take(sizeof(float) * (L.sp[n] + 1));
   L.gy = cv.take(sizeof(float) * (L.sp[n] + 1));
   L.st = cv.take(sizeof(FieldCG) * n);
@@ -1011,7 +1008,7 @@ void run_adjoint(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hvd
     sp[i + 1] = sp[i] + aniso_samples(planes[i].lines, planes[i].width, p.is_pal);
   }
   const int nbx = (int)((maxpx + 255) / 256);
-  const size_t o_part = cv.take(sizeof(double) * kCgPartials * strips);
+  const size_t o_part = cv.take(sizeof(double) * 2 * kCgPartials * strips);  // two launch slots
   const size_t o_apart = cv.take(sizeof(double) * 2 * (size_t)nbx * n);
   const size_t o_glam = cv.take(sizeof(double) * 2 * n);
   const size_t o_gx = cv.take(sizeof(float) * (sp[n] + 1));
@@ -1138,7 +1135,7 @@ void run_adjoint(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hvd
                          Fa[i].chi[1], np);
       LAUNCH_CHECK(ctx);
     }
-    cg_loop(ctx, dFa, dstate, sg, partials, n, adj_iters, true, st);
+    cg_loop(ctx, dFa, dstate, sg, partials, n, adj_iters, true, -1, st);
     hipLaunchKernelGGL(k_refine_finalize, dim3(SS.nb), dim3(64), 0, st, dFa, (const FieldCG*)dstate, sg);
     LAUNCH_CHECK(ctx);
     const AdjField* dA = (const AdjField*)(base + oA[k]);

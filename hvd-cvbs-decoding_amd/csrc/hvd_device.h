@@ -65,22 +65,32 @@ struct DevField {
   float2* save;        // when set: chi at the start of each IRLS stage (stage-major planes)
 };
 
-// Per-field CG state (variational.cpp:519-549), advanced by the one-block
-// reduction kernels between the stencil launches.  hh is curv(d) of the
-// direction the next iteration moves along, gd = Re<g, d> of it.
+// The scalar state of one field's CG loop after a launch (variational.cpp:
+// 519-549): hh is curv(d) of the direction the next iteration moves along,
+// gd = Re<g, d> of it, cur the ping-pong slot of chi, g and d(previous) that
+// it reads.
+struct CgDyn {
+  double gg, gg0, hh, gd;
+  float alpha, beta;
+  int it, done, cur, pad_;
+};
+
+// Per-field solver state.  Each CG iteration launch first sums the previous
+// launch's strip partials of its field (every wave forms the same sums in the
+// same order) and advances the previous snapshot; the field's first strip
+// publishes the new snapshot in slot it & 1, so a launch reads one slot while
+// the next is written.
 struct FieldCG {
-  double gg, gg0, hh, gd, npix;
+  double npix;
   double tol2;
   double hh_floor;  // stop when curv(d) <= hh_floor (hvd-core: 1e-12)
   double gg_floor;  // stop when |g|^2 < gg_floor * npix (hvd-core: 1e-10)
-  float alpha, beta;
+  CgDyn dyn[2];
   float mu_h, mu_v, lam_y, eps, eps_c, k;
   float aniso_forced;   // > 0 forces chroma_aniso
   int is_pal;
   int n_inner;
-  int done, it;
-  int cur;              // ping-pong slot of chi, g and d(previous) read next
-  int pad0_;
+  int cur;              // slot of chi (and g) at the start of the next stage / at the end
   int iters[16];        // CG iterations per IRLS stage (diagnostics)
   int stage;
   int pad_;

@@ -131,11 +131,9 @@ __global__ void k_aniso_select(FieldCG* st, const int* sample_prefix, const floa
                                float lambda_c);
 __global__ void k_irls_weights(const DevField* F, const FieldCG* st, StripGrid sg, int stage);
 __global__ void k_cg_start(const DevField* F, const FieldCG* st, StripGrid sg, double* partials, int adjoint);
-__global__ void k_cg_iter(const DevField* F, const FieldCG* st, StripGrid sg, double* partials, int it);
-__global__ void k_cg_iter_adj(const DevField* F, const FieldCG* st, StripGrid sg, double* partials, int it);
-__global__ void k_reduce_start(FieldCG* st, const int* tp, const double* partials);
-__global__ void k_reduce_iter(FieldCG* st, const int* tp, const double* partials);
-__global__ void k_stage_end(FieldCG* st, int nf, int stage);
+__global__ void k_cg_iter(const DevField* F, FieldCG* st, StripGrid sg, double* partials, int it);
+__global__ void k_cg_iter_adj(const DevField* F, FieldCG* st, StripGrid sg, double* partials, int it);
+__global__ void k_cg_finish(FieldCG* st, StripGrid sg, const double* partials, int n_inner, int stage);
 __global__ void k_refine_finalize(const DevField* F, const FieldCG* st, StripGrid sg);
 
 // bridge.hip

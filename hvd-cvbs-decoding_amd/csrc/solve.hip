@@ -200,23 +200,77 @@ __device__ inline float2 grad_at(const FieldCG& st, const Sample& p0, const Samp
 }
 
 // Wave-level FP64 partial of this strip, published to its slot.  The
-// per-field totals are formed by the k_reduce_* kernels that follow (a kernel
-// boundary orders the two; no device-scope fence per wave, which on a
-// multi-XCD part would write back the L2 each time).
+// per-field totals are formed by the next launch (a kernel boundary orders
+// the two; no device-scope fence per wave, which on a multi-XCD part would
+// write back the L2 each time).
 __device__ inline void publish(double* partials, int tile0, int lt, int nv, int k, double v) {
   v = wave_sum(v);
   if (lane_id() == 0) partials[(size_t)(tile0 + lt) * nv + k] = v;
 }
 
-// Fixed-order sum of field fi's strip partials (component k of NV), valid in
-// thread 0 of a 256-thread block.
-template <int NV>
-__device__ inline double field_total(const double* partials, const int* tp, int fi, int k) {
-  __shared__ double lds[4];
-  const int t0 = tp[fi], nblk = tp[fi + 1] - t0;
-  double acc = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += blockDim.x) acc += partials[(size_t)(t0 + b) * NV + k];
-  return block_sum<256>(acc, lds);
+// The four per-field sums of a launch's strip partials (strips t0 .. t0+n-1),
+// in one fixed order: lane l adds strips l, l + 64, ..., then a shuffle tree.
+// Every wave that forms them gets the same bits, in every lane.
+struct CgSums {
+  double v[kCgPartials];
+};
+
+__device__ inline CgSums field_sums(const double* partials, int t0, int n) {
+  CgSums r;
+#pragma unroll
+  for (int k = 0; k < kCgPartials; ++k) r.v[k] = 0.0;
+  for (int j = lane_id(); j < n; j += 64) {
+    const double* p = partials + (size_t)(t0 + j) * kCgPartials;
+#pragma unroll
+    for (int k = 0; k < kCgPartials; ++k) r.v[k] += p[k];
+  }
+#pragma unroll
+  for (int k = 0; k < kCgPartials; ++k) r.v[k] = wave_sum(r.v[k]);
+  return r;
+}
+
+// Scalars of the first iteration from the stage-start sums (variational.cpp:
+// 524-531): gg0 = |g|^2, d = -g so curv(d) = curv(g) and Re<g, d> = -|g|^2.
+__device__ inline CgDyn cg_first(const FieldCG& s, int cur, const CgSums& q) {
+  CgDyn n{};
+  n.gg = q.v[0];
+  n.gg0 = q.v[0];
+  n.hh = q.v[1];
+  n.gd = -q.v[0];
+  n.cur = cur;
+  n.done = s.n_inner <= 0 || n.hh <= s.hh_floor ? 1 : 0;
+  if (!n.done) n.alpha = (float)(-0.5 * n.gd / n.hh);
+  return n;
+}
+
+// End of an iteration (variational.cpp:536-548, then :528-531 of the next)
+// from the snapshot before it and the sums of its launch: beta, the exits on
+// |g|^2 and the iteration budget, the next direction's
+//   curv(-g' + beta d) = curv(g') - 2 beta curv(g', d) + beta^2 curv(d)
+// (exit when <= hh_floor), Re<g', d'> = -|g'|^2 + beta Re<g', d> and alpha;
+// flips the ping-pong slot.
+__device__ inline CgDyn cg_advance(const FieldCG& s, const CgDyn& prev, const CgSums& q) {
+  CgDyn n = prev;
+  if (prev.done) return n;
+  const double ggn = q.v[0];
+  n.cur ^= 1;
+  const float beta = (float)(ggn / (prev.gg > 1e-30 ? prev.gg : 1e-30));
+  n.gg = ggn;
+  n.it += 1;
+  if (ggn < s.tol2 * prev.gg0 || ggn < s.gg_floor * s.npix || n.it >= s.n_inner) {
+    n.done = 1;
+    return n;
+  }
+  const double b = (double)beta;
+  n.hh = q.v[1] - 2.0 * b * q.v[2] + b * b * prev.hh;
+  if (n.hh <= s.hh_floor) {
+    n.done = 1;
+    return n;
+  }
+  n.beta = beta;
+  n.gd = -ggn + b * q.v[3];
+  n.alpha = (float)(-0.5 * n.gd / n.hh);
+  return n;
 }
 
 // One row of this lane's column as the fused CG iteration forms it: the
@@ -315,15 +369,16 @@ __device__ inline void curv_terms(const FieldCG& s, float4 w, bool on, float2 a0
 // rows + 1 lines).  The loop body is branch-free apart from the stores: loads
 // are clamped and results selected.
 template <bool Start, bool Upd, bool Adj>
-__device__ inline void cg_strip(const DevField& f, const FieldCG& s, const StripCtx& c, int rows, double* partials) {
+__device__ inline void cg_strip(const DevField& f, const FieldCG& s, const CgDyn& q, const StripCtx& c, int rows,
+                                double* partials) {
   const Carrier k = carrier_for(f, c);
-  const float2* chi = pick(f.chi, s.cur);
-  const float2* gsrc = pick(f.g, s.cur);
-  const float2* dprev = pick(f.d, s.cur);
-  float2* chi_out = pick(f.chi, s.cur ^ 1);
-  float2* g_out = pick(f.g, Start ? s.cur : s.cur ^ 1);
-  float2* d_out = pick(f.d, s.cur ^ 1);
-  const float alpha = s.alpha, beta = s.beta;
+  const float2* chi = pick(f.chi, q.cur);
+  const float2* gsrc = pick(f.g, q.cur);
+  const float2* dprev = pick(f.d, q.cur);
+  float2* chi_out = pick(f.chi, q.cur ^ 1);
+  float2* g_out = pick(f.g, Start ? q.cur : q.cur ^ 1);
+  float2* d_out = pick(f.d, q.cur ^ 1);
+  const float alpha = q.alpha, beta = q.beta;
   const bool calc = c.valid && lane_id() >= 1 && lane_id() <= kStripCols + 1;
   auto fetch = [&](int y) { return iter_fetch<Start, Upd, Adj>(f, c.x, chi, gsrc, dprev, y); };
   auto make = [&](const IterRaw& q, int y) { return iter_make<Start, Upd>(k, q, alpha, beta, y); };
@@ -533,96 +588,85 @@ __global__ void __launch_bounds__(64) k_irls_weights(const DevField* __restrict_
 }
 
 // Stage start (variational.cpp:519-525): g = grad(chi) and the partials of
-// |g|^2 and curv(g) (the first direction is d = -g).
+// |g|^2 and curv(g) (the first direction is d = -g), into partial slot 1.
 __global__ void __launch_bounds__(64) k_cg_start(const DevField* __restrict__ F, const FieldCG* __restrict__ st,
                                                  StripGrid sg, double* partials, int adjoint) {
   const StripCtx c = strip_ctx(F, sg);
   const DevField f = F[c.f];  // a private copy: stores cannot alias it
   const FieldCG s = st[c.f];
-  if (adjoint) cg_strip<true, false, true>(f, s, c, sg.rows, partials);
-  else cg_strip<true, false, false>(f, s, c, sg.rows, partials);
+  CgDyn q{};
+  q.cur = s.cur;
+  double* out = partials + (size_t)sg.nb * kCgPartials;
+  if (adjoint) cg_strip<true, false, true>(f, s, q, c, sg.rows, out);
+  else cg_strip<true, false, false>(f, s, q, c, sg.rows, out);
 }
 
-// gg0 = |g|^2, the inner-loop reset, and the first iteration's curv(d) =
-// curv(g), Re<g, d> = -|g|^2 and alpha (variational.cpp:524-531).
-__global__ void __launch_bounds__(256) k_reduce_start(FieldCG* st, const int* tp, const double* partials) {
-  const int fi = blockIdx.x;
-  FieldCG& s = st[fi];
-  const double gg = field_total<kCgPartials>(partials, tp, fi, 0);
-  const double hh = field_total<kCgPartials>(partials, tp, fi, 1);
-  if (threadIdx.x == 0) {
-    s.gg = gg;
-    s.gg0 = gg;
-    s.it = 0;
-    s.beta = 0.f;
-    s.hh = hh;
-    s.gd = -gg;
-    s.done = s.n_inner <= 0 || hh <= s.hh_floor ? 1 : 0;
-    if (!s.done) s.alpha = (float)(-0.5 * s.gd / hh);
-  }
-}
+namespace {
 
-// CG iteration `it` (variational.cpp:527-548): d <- -g + beta d, chi <- chi +
-// alpha d, g <- grad(chi), and the partials the next reduction needs.
-__global__ void __launch_bounds__(64) k_cg_iter(const DevField* __restrict__ F, const FieldCG* __restrict__ st,
-                                                StripGrid sg, double* partials, int it) {
+// CG iteration `it` (variational.cpp:527-548): the wave first sums its field's
+// partials of the previous launch (slot (it + 1) & 1) and advances the
+// field's scalars; the field's first strip publishes them in snapshot it & 1.
+// Then d <- -g + beta d, chi <- chi + alpha d, g <- grad(chi), and this
+// launch's partials into slot it & 1.
+template <bool Adj>
+__device__ inline void cg_iter(const DevField* __restrict__ F, FieldCG* __restrict__ st, const StripGrid& sg,
+                               double* partials, int it) {
   const StripCtx c = strip_ctx(F, sg);
   const DevField f = F[c.f];  // a private copy: stores cannot alias it
   const FieldCG s = st[c.f];
-  if (s.done) return;
-  if (it > 0) cg_strip<false, true, false>(f, s, c, sg.rows, partials);
-  else cg_strip<false, false, false>(f, s, c, sg.rows, partials);
+  const size_t slot = (size_t)sg.nb * kCgPartials;
+  const int ns = sg.spf > 0 ? sg.spf : sg.tp[c.f + 1] - sg.tp[c.f];
+  const CgSums sums = field_sums(partials + ((it + 1) & 1) * slot, f.tile0, ns);
+  // the snapshot is read from memory with a uniform index (a dynamic index
+  // into the register copy would put it on the stack)
+  CgDyn q = it == 0 ? cg_first(s, s.cur, sums) : cg_advance(s, st[c.f].dyn[(it - 1) & 1], sums);
+  if (c.lt == 0 && lane_id() == 0) st[c.f].dyn[it & 1] = q;
+  // every lane holds the same values; make that visible to the compiler
+  // (scalar registers, a uniform branch)
+  q.done = __builtin_amdgcn_readfirstlane(q.done);
+  q.cur = __builtin_amdgcn_readfirstlane(q.cur);
+  q.alpha = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(q.alpha)));
+  q.beta = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(q.beta)));
+  if (q.done) return;
+  double* out = partials + (it & 1) * slot;
+  if (it > 0) cg_strip<false, true, Adj>(f, s, q, c, sg.rows, out);
+  else cg_strip<false, false, Adj>(f, s, q, c, sg.rows, out);
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(64) k_cg_iter(const DevField* __restrict__ F, FieldCG* __restrict__ st,
+                                                StripGrid sg, double* partials, int it) {
+  cg_iter<false>(F, st, sg, partials, it);
 }
 
 // The same iteration for an adjoint solve (no composite; the right-hand side
 // is subtracted from the gradient).
-__global__ void __launch_bounds__(64) k_cg_iter_adj(const DevField* __restrict__ F, const FieldCG* __restrict__ st,
+__global__ void __launch_bounds__(64) k_cg_iter_adj(const DevField* __restrict__ F, FieldCG* __restrict__ st,
                                                     StripGrid sg, double* partials, int it) {
-  const StripCtx c = strip_ctx(F, sg);
-  const DevField f = F[c.f];  // a private copy: stores cannot alias it
-  const FieldCG s = st[c.f];
-  if (s.done) return;
-  if (it > 0) cg_strip<false, true, true>(f, s, c, sg.rows, partials);
-  else cg_strip<false, false, true>(f, s, c, sg.rows, partials);
+  cg_iter<true>(F, st, sg, partials, it);
 }
 
-// End of an iteration (variational.cpp:536-548, then :528-531 of the next):
-// beta, the exits on |g|^2 and the iteration budget, the next direction's
-// curvature (exit when <= hh_floor) and alpha; flips the ping-pong slot.
-__global__ void __launch_bounds__(256) k_reduce_iter(FieldCG* st, const int* tp, const double* partials) {
+// End of a stage's CG loop of n_inner launches: the last launch's update,
+// the final chi slot, and the iteration count of stage `stage` (diagnostics,
+// when 0 <= stage < 16).  One 64-thread block per field.
+__global__ void __launch_bounds__(64) k_cg_finish(FieldCG* st, StripGrid sg, const double* partials, int n_inner,
+                                                  int stage) {
   const int fi = blockIdx.x;
-  FieldCG& s = st[fi];
-  if (s.done) return;
-  const double ggn = field_total<kCgPartials>(partials, tp, fi, 0);
-  const double cg = field_total<kCgPartials>(partials, tp, fi, 1);
-  const double cgd = field_total<kCgPartials>(partials, tp, fi, 2);
-  const double gdp = field_total<kCgPartials>(partials, tp, fi, 3);
-  if (threadIdx.x == 0) {
-    s.cur ^= 1;
-    const float beta = (float)(ggn / (s.gg > 1e-30 ? s.gg : 1e-30));
-    s.gg = ggn;
-    s.it += 1;
-    if (ggn < s.tol2 * s.gg0 || ggn < s.gg_floor * s.npix || s.it >= s.n_inner) {
-      s.done = 1;
-      return;
-    }
-    const double b = (double)beta;
-    const double hh = cg - 2.0 * b * cgd + b * b * s.hh;
-    s.hh = hh;
-    if (hh <= s.hh_floor) {
-      s.done = 1;
-      return;
-    }
-    s.beta = beta;
-    s.gd = -ggn + b * gdp;
-    s.alpha = (float)(-0.5 * s.gd / hh);
+  const FieldCG s = st[fi];
+  int cur = s.cur, it = 0;
+  if (n_inner > 0) {
+    const size_t slot = (size_t)sg.nb * kCgPartials;
+    const int t0 = sg.tp[fi], ns = sg.tp[fi + 1] - t0;
+    const CgSums sums = field_sums(partials + ((n_inner - 1) & 1) * slot, t0, ns);
+    const CgDyn q = cg_advance(s, st[fi].dyn[(n_inner - 1) & 1], sums);
+    cur = q.cur;
+    it = q.it;
   }
-}
-
-// Record the stage's iteration count (diagnostics).
-__global__ void k_stage_end(FieldCG* st, int nf, int stage) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f < nf && stage < 16) st[f].iters[stage] = st[f].it;
+  if (threadIdx.x == 0) {
+    st[fi].cur = cur;
+    if (stage >= 0 && stage < 16) st[fi].iters[stage] = it;
+  }
 }
 
 // Output (variational.cpp:552-555): Y = S - Re[chi c]; chi lands in the
