@@ -249,6 +249,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only-batched", action="store_true", help="time the headline workload only (profiling)")
     ap.add_argument("--no-autotune", action="store_true", help="use the built-in launch shapes (no autotuner)")
+    ap.add_argument("--no-graphs", action="store_true", help="launch the CG iterations one by one (no HIP graphs)")
     ap.add_argument("--solver-tuning", default="", metavar="ROWS,GROUP_PX",
                     help="fix the solver's strip height and solve-group pixels instead of autotuning")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -284,6 +285,7 @@ def main():
     def step():
         eng.decode_fields_u16(raws, geoms, p0, outputs=(ys, chis))
 
+    eng.set_graphs(not args.no_graphs)
     tuned = []
     if args.solver_tuning:
         rows, gpx = (int(v) for v in args.solver_tuning.split(","))
@@ -332,6 +334,7 @@ def main():
                      "traffic": traffic, "avg_us_per_iteration": round(avg_pair_ms * 1e3, 2),
                      "algorithmic_bytes_per_iteration": int(CG_BYTES_PER_PX_ITER * px_per_pair)},
         "solver_tuning": tuned if tuned else "built-in policy",
+        "hip_graphs": not args.no_graphs,
     }
     if not args.only_batched:
         # plain path: one field per call

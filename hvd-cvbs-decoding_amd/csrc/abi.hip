@@ -11,6 +11,7 @@
 #include <array>
 #include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -63,6 +64,11 @@ struct hvd_gpu_ctx {
   // field shape by hvd_gpu_autotune_solver, else the built-in policy
   SolverTuning fixed;
   std::map<std::array<int, 3>, std::map<int, SolverTuning>> tuned;  // (lines, width, pal) -> fields -> tuning
+  // HIP graphs of the forward solve's CG iterations (hvd_gpu_set_graphs),
+  // keyed by the launch arguments they bake in; dropped when the workspace
+  // they point into is reallocated
+  bool graphs = true;
+  std::map<std::array<uintptr_t, 10>, hipGraphExec_t> cg_graphs;
 };
 
 namespace {
@@ -135,10 +141,17 @@ struct Carve {
   }
 };
 
+// Destroys the cached CG graphs (their arguments point into the workspace).
+void drop_graphs(hvd_gpu_ctx* ctx) {
+  for (auto& kv : ctx->cg_graphs) (void)hipGraphExecDestroy(kv.second);
+  ctx->cg_graphs.clear();
+}
+
 char* ensure_ws(hvd_gpu_ctx* ctx, size_t bytes, hipStream_t st) {
   if (bytes <= ctx->ws_cap) return ctx->ws;
   HIPCHK(ctx, hipStreamSynchronize(st));
   HIPCHK(ctx, hipDeviceSynchronize());
+  drop_graphs(ctx);
   if (ctx->ws) HIPCHK(ctx, hipFree(ctx->ws));
   ctx->ws = nullptr;
   ctx->ws_cap = 0;
@@ -189,6 +202,29 @@ char* upload(hvd_gpu_ctx* ctx, const Stage& stg, hipStream_t st, Slot** used) {
   HIPCHK(ctx, hipMemcpyAsync(sl.dev, sl.host, stg.buf.size(), hipMemcpyHostToDevice, st));
   *used = &sl;
   return static_cast<char*>(sl.dev);
+}
+
+// As upload, but into a caller-chosen device region (a fixed place in the
+// workspace, so launches that read the descriptors keep the same arguments
+// from call to call); the slot only lends its pinned host buffer.
+char* upload_into(hvd_gpu_ctx* ctx, const Stage& stg, char* dst, hipStream_t st, Slot** used) {
+  Slot& sl = ctx->slots[ctx->next_slot];
+  ctx->next_slot = (ctx->next_slot + 1) % kSlots;
+  if (sl.armed) HIPCHK(ctx, hipEventSynchronize(sl.ev));
+  const size_t need = std::max<size_t>(stg.buf.size(), 1);
+  if (need > sl.cap) {
+    if (sl.host) HIPCHK(ctx, hipHostFree(sl.host));
+    if (sl.dev) HIPCHK(ctx, hipFree(sl.dev));
+    sl.host = sl.dev = nullptr;
+    const size_t cap = std::max(need * 2, size_t(1) << 16);
+    HIPCHK(ctx, hipHostMalloc(&sl.host, cap, hipHostMallocDefault));
+    HIPCHK(ctx, hipMalloc(&sl.dev, cap));
+    sl.cap = cap;
+  }
+  std::memcpy(sl.host, stg.buf.data(), stg.buf.size());
+  HIPCHK(ctx, hipMemcpyAsync(dst, sl.host, stg.buf.size(), hipMemcpyHostToDevice, st));
+  *used = &sl;
+  return dst;
 }
 
 void arm(hvd_gpu_ctx* ctx, Slot* sl, hipStream_t st) {
@@ -475,13 +511,63 @@ StripSetup strip_setup(std::vector<DevField>& F, int rows) {
   return S;
 }
 
+// The CG iterations of one forward stage as a HIP graph: n_inner k_cg_iter
+// launches and k_cg_finish, built once per argument set (every argument
+// points into the workspace or is a size) and then replayed.
+hipGraphExec_t cg_graph(hvd_gpu_ctx* ctx, const DevField* dF, FieldCG* dstate, const StripGrid& sg, double* partials,
+                        int nf, int n_inner, int stage) {
+  const std::array<uintptr_t, 10> key{(uintptr_t)dF,      (uintptr_t)dstate,  (uintptr_t)sg.tp,   (uintptr_t)partials,
+                                      (uintptr_t)nf,      (uintptr_t)sg.nb,   (uintptr_t)sg.rows, (uintptr_t)sg.spf,
+                                      (uintptr_t)n_inner, (uintptr_t)(stage + 1)};
+  const auto found = ctx->cg_graphs.find(key);
+  if (found != ctx->cg_graphs.end()) return found->second;
+  if (ctx->cg_graphs.size() >= 256) drop_graphs(ctx);  // bounded cache
+  hipGraph_t g = nullptr;
+  HIPCHK(ctx, hipGraphCreate(&g, 0));
+  struct Free {
+    hipGraph_t g;
+    ~Free() { (void)hipGraphDestroy(g); }
+  } guard{g};
+  StripGrid sgv = sg;
+  const double* cpart = partials;
+  int nin = n_inner, stg = stage;
+  hipGraphNode_t prev = nullptr;
+  for (int i = 0; i <= n_inner; ++i) {
+    int itv = i;
+    void* a_iter[] = {(void*)&dF, (void*)&dstate, (void*)&sgv, (void*)&partials, (void*)&itv};
+    void* a_fin[] = {(void*)&dstate, (void*)&sgv, (void*)&cpart, (void*)&nin, (void*)&stg};
+    hipKernelNodeParams kp{};
+    if (i < n_inner) {
+      kp.func = (void*)k_cg_iter;
+      kp.gridDim = dim3(sg.nb);
+      kp.kernelParams = a_iter;
+    } else {
+      kp.func = (void*)k_cg_finish;
+      kp.gridDim = dim3(nf);
+      kp.kernelParams = a_fin;
+    }
+    kp.blockDim = dim3(64);
+    kp.sharedMemBytes = 0;
+    kp.extra = nullptr;
+    hipGraphNode_t node = nullptr;
+    HIPCHK(ctx, hipGraphAddKernelNode(&node, g, prev ? &prev : nullptr, prev ? 1 : 0, &kp));
+    prev = node;
+  }
+  hipGraphExec_t ex = nullptr;
+  HIPCHK(ctx, hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+  ctx->cg_graphs[key] = ex;
+  return ex;
+}
+
 // One stage's CG from the current chi: the stage-start gradient, then n_inner
 // fused iterations (one launch each; a launch first completes the previous
 // one's per-field reductions), then the last update; per-field exits
 // (variational.cpp:519-549).  adjoint: the fields describe an adjoint solve
 // (no composite, a right-hand side).  stage: index for the iteration record.
+// graph: replay the iterations as a cached HIP graph (forward solves whose
+// descriptors sit at a fixed place in the workspace).
 void cg_loop(hvd_gpu_ctx* ctx, const DevField* dF, FieldCG* dstate, const StripGrid& sg, double* partials, int nf,
-             int n_inner, bool adjoint, int stage, hipStream_t st) {
+             int n_inner, bool adjoint, int stage, bool graph, hipStream_t st) {
   hipLaunchKernelGGL(k_cg_start, dim3(sg.nb), dim3(64), 0, st, dF, (const FieldCG*)dstate, sg, partials,
                      (int)adjoint);
   LAUNCH_CHECK(ctx);
@@ -491,15 +577,19 @@ void cg_loop(hvd_gpu_ctx* ctx, const DevField* dF, FieldCG* dstate, const StripG
     HIPCHK(ctx, hipEventCreate(&evs.second));
     HIPCHK(ctx, hipEventRecord(evs.first, st));
   }
-  for (int it = 0; it < n_inner; ++it) {
-    if (adjoint)
-      hipLaunchKernelGGL(k_cg_iter_adj, dim3(sg.nb), dim3(64), 0, st, dF, dstate, sg, partials, it);
-    else
-      hipLaunchKernelGGL(k_cg_iter, dim3(sg.nb), dim3(64), 0, st, dF, dstate, sg, partials, it);
+  if (graph && !adjoint && n_inner > 0) {
+    HIPCHK(ctx, hipGraphLaunch(cg_graph(ctx, dF, dstate, sg, partials, nf, n_inner, stage), st));
+  } else {
+    for (int it = 0; it < n_inner; ++it) {
+      if (adjoint)
+        hipLaunchKernelGGL(k_cg_iter_adj, dim3(sg.nb), dim3(64), 0, st, dF, dstate, sg, partials, it);
+      else
+        hipLaunchKernelGGL(k_cg_iter, dim3(sg.nb), dim3(64), 0, st, dF, dstate, sg, partials, it);
+      LAUNCH_CHECK(ctx);
+    }
+    hipLaunchKernelGGL(k_cg_finish, dim3(nf), dim3(64), 0, st, dstate, sg, (const double*)partials, n_inner, stage);
     LAUNCH_CHECK(ctx);
   }
-  hipLaunchKernelGGL(k_cg_finish, dim3(nf), dim3(64), 0, st, dstate, sg, (const double*)partials, n_inner, stage);
-  LAUNCH_CHECK(ctx);
   if (ctx->profiling) {
     HIPCHK(ctx, hipEventRecord(evs.second, st));
     ctx->cg_events.push_back(evs);
@@ -524,9 +614,18 @@ FieldCG field_cg_init(const DevField& f, const hvd_solver_params& p, int n_inner
   return s;
 }
 
+// Bytes run_refine stages for n fields (its descriptor region).
+size_t refine_desc_bytes(int n) {
+  return align256(sizeof(DevField) * n) + align256(sizeof(int) * (n + 1)) + align256(sizeof(FieldCG) * n) +
+         align256(sizeof(int) * (n + 1));
+}
+
+// One solve group.  desc: a fixed workspace region of refine_desc_bytes(n)
+// for the group's descriptors (the CG iterations then replay as a graph when
+// the context allows it), or null (descriptors in an upload slot, eager).
 void run_refine(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_params& p, double* partials,
                 float* gx, float* gy, const std::vector<int>& sample_prefix, FieldCG* dstate, hipStream_t st,
-                RefineOut out, int rows_tuned) {
+                RefineOut out, int rows_tuned, char* desc) {
   const int nf = (int)F.size();
   for (int i = 0; i < nf; ++i) REQUIRE(ctx, F[i].s != nullptr, "field %d: no composite plane", i);
   const StripSetup SS = strip_setup(F, rows_tuned);
@@ -541,7 +640,8 @@ void run_refine(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_par
   Stage stg;
   const size_t oF = stg.put(F), oT = stg.put(tp), oI = stg.put(init), oS = stg.put(sample_prefix);
   Slot* sl;
-  char* base = upload(ctx, stg, st, &sl);
+  REQUIRE(ctx, !desc || stg.buf.size() <= refine_desc_bytes(nf), "internal: descriptor region too small");
+  char* base = desc ? upload_into(ctx, stg, desc, st, &sl) : upload(ctx, stg, st, &sl);
   const DevField* dF = (const DevField*)(base + oF);
   const int* dtp = (const int*)(base + oT);
   const int* dsp = (const int*)(base + oS);
@@ -559,7 +659,7 @@ void run_refine(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_par
   for (int o = 0; o < outer; ++o) {
     hipLaunchKernelGGL(k_irls_weights, dim3(nb), dim3(64), 0, st, dF, dstate, sg, o);
     LAUNCH_CHECK(ctx);
-    cg_loop(ctx, dF, dstate, sg, partials, nf, n_inner, false, o, st);
+    cg_loop(ctx, dF, dstate, sg, partials, nf, n_inner, false, o, desc != nullptr && ctx->graphs, st);
   }
   hipLaunchKernelGGL(k_refine_finalize, dim3(nb), dim3(64), 0, st, dF, (const FieldCG*)dstate,
                      sg);
@@ -657,7 +757,7 @@ SolverTuning tuning_for(const hvd_gpu_ctx* ctx, const std::vector<DevField>& F, 
 struct GroupLayout {
   std::vector<size_t> chi1, g, d, w;
   std::vector<int> sp;  // aniso sample prefix
-  size_t part = 0, gx = 0, gy = 0, st = 0, end = 0;
+  size_t part = 0, gx = 0, gy = 0, st = 0, desc = 0, end = 0;
 };
 
 GroupLayout group_layout(Carve cv, const DevField* F, int n, int is_pal) {
@@ -681,6 +781,7 @@ GroupLayout group_layout(Carve cv, const DevField* F, int n, int is_pal) {
   L.gx = cv.take(sizeof(float) * (L.sp[n] + 1));
   L.gy = cv.take(sizeof(float) * (L.sp[n] + 1));
   L.st = cv.take(sizeof(FieldCG) * n);
+  L.desc = cv.take(refine_desc_bytes(n));
   L.end = cv.off;
   return L;
 }
@@ -779,6 +880,7 @@ void hvd_gpu_destroy(hvd_gpu_ctx* ctx) {
     if (s.host) (void)hipHostFree(s.host);
     if (s.dev) (void)hipFree(s.dev);
   }
+  drop_graphs(ctx);
   if (ctx->ws) (void)hipFree(ctx->ws);
   clear_profile(ctx);
   delete ctx;
@@ -967,7 +1069,7 @@ void refine_planes(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_
     o.stage_iters = out.stage_iters ? out.stage_iters + (size_t)b * outer : nullptr;
     o.aniso = out.aniso ? out.aniso + b : nullptr;
     run_refine(ctx, G, p, (double*)(ws + L.part), (float*)(ws + L.gx), (float*)(ws + L.gy), L.sp,
-               (FieldCG*)(ws + L.st), st, o, tune.rows);
+               (FieldCG*)(ws + L.st), st, o, tune.rows, ws + L.desc);
   }
 }
 
@@ -1135,7 +1237,7 @@ void run_adjoint(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hvd
                          Fa[i].chi[1], np);
       LAUNCH_CHECK(ctx);
     }
-    cg_loop(ctx, dFa, dstate, sg, partials, n, adj_iters, true, -1, st);
+    cg_loop(ctx, dFa, dstate, sg, partials, n, adj_iters, true, -1, false, st);
     hipLaunchKernelGGL(k_refine_finalize, dim3(SS.nb), dim3(64), 0, st, dFa, (const FieldCG*)dstate, sg);
     LAUNCH_CHECK(ctx);
     const AdjField* dA = (const AdjField*)(base + oA[k]);
@@ -2010,6 +2112,14 @@ int hvd_gpu_weave_frame_s16(hvd_gpu_ctx* ctx, const int16_t* src, const hvd_fram
                        o);
     LAUNCH_CHECK(ctx);
     arm(ctx, sl, st);
+  });
+}
+
+int hvd_gpu_set_graphs(hvd_gpu_ctx* ctx, int enable) {
+  return guarded(ctx, [&] {
+    HIPCHK(ctx, hipDeviceSynchronize());
+    if (!enable) drop_graphs(ctx);
+    ctx->graphs = enable != 0;
   });
 }
 

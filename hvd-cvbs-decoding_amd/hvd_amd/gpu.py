@@ -25,7 +25,7 @@ ABI_SYMBOLS = (
     "hvd_gpu_decode_frame_window_s16", "hvd_gpu_split_frame_fields", "hvd_gpu_field_parity_vote",
     "hvd_gpu_make_carrier", "hvd_gpu_tridiag_smooth", "hvd_gpu_box_blur", "hvd_gpu_chroma_plane",
     "hvd_gpu_weave_frame_s16", "hvd_gpu_variational_refine_stages", "hvd_gpu_refine_adjoint",
-    "hvd_gpu_set_solver_tuning", "hvd_gpu_autotune_solver",
+    "hvd_gpu_set_solver_tuning", "hvd_gpu_autotune_solver", "hvd_gpu_set_graphs",
 )
 
 
@@ -154,6 +154,7 @@ def load() -> C.CDLL:
         lib.hvd_gpu_cg_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                                           C.POINTER(C.c_double)]
         lib.hvd_gpu_set_solver_tuning.argtypes = [vp, C.POINTER(SolverTuning)]
+        lib.hvd_gpu_set_graphs.argtypes = [vp, i]
         lib.hvd_gpu_autotune_solver.argtypes = [vp, i, i, i, i, i, C.POINTER(SolverTuning), C.POINTER(C.c_double)]
         _lib = lib
     return _lib
@@ -211,6 +212,10 @@ class Engine:
         return ms.value, nl.value, pxit.value
 
     # ---- solver launch shape ------------------------------------------
+    def set_graphs(self, on: bool):
+        """Replay each forward IRLS stage's CG iterations from a cached HIP graph (default on)."""
+        self._check(self.lib.hvd_gpu_set_graphs(self.ctx, int(on)), "set_graphs")
+
     def set_solver_tuning(self, strip_rows: int = 0, group_pixels: int = 0):
         """Fix the solve's strip height and solve-group size (zeros: autotuned / built-in)."""
         t = SolverTuning(strip_rows, group_pixels)
