@@ -223,6 +223,8 @@ std::vector<YcFrameS16> DecodeFrameSequenceWindow(const std::vector<const int16_
       v.emplace_back(nflat);
       outs[k] = hvd_yc_frame{lum[k].get(), chr[k].get(), u[k].get(), v[k].get()};
     }
+    const FieldGeometry fg = FieldGeometryFromParams(fp);
+    ss.autotune(fg.active_lines(), fg.active_width(), fp.uses_vswitch(), 2 * nframes);
     const hvd_frame_params a = gpu::ToAbi(fp);
     const hvd_solver_params sp = gpu::ToAbi(cfg, fp.uses_vswitch());
     const hvd_bridge_params bp = gpu::BridgeOf(cfg);

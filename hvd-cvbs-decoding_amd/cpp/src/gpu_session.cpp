@@ -2,6 +2,7 @@
 #include "gpu_session.h"
 
 #include <cstdio>
+#include <cstdlib>
 
 #include "engine/gpu_status.h"
 #include "frame_bridge.h"
@@ -40,6 +41,20 @@ void Session::check(int status, const char* what) {
 }
 
 void Session::sync() { check(hvd_gpu_synchronize(ctx(), nullptr), "hvd_gpu_synchronize"); }
+
+void Session::autotune(int lines, int width, bool pal, int nfields) {
+  static const bool on = [] {
+    const char* e = std::getenv("HVD_AUTOTUNE");
+    return e && *e && std::atoi(e) != 0;
+  }();
+  if (!on || nfields <= 0) return;
+  const std::array<int, 4> key{lines, width, pal ? 1 : 0, nfields};
+  if (!tuned_.insert(key).second) return;
+  hvd_gpu_solver_tuning t{};
+  const int st = hvd_gpu_autotune_solver(ctx(), lines, width, pal ? 1 : 0, nfields, 8, &t, nullptr);
+  if (st != HVD_GPU_OK)
+    std::fprintf(stderr, "hvd-gpu: autotune %dx%d x%d: %s\n", lines, width, nfields, hvd_gpu_last_error(ctx_));
+}
 
 Session& ThreadSession() {
   thread_local Session s;

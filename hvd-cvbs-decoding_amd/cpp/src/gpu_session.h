@@ -6,7 +6,9 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <array>
 #include <cstddef>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -37,9 +39,15 @@ class Session {
   // Throws Error carrying the context's message when status != HVD_GPU_OK.
   void check(int status, const char* what);
   void sync();
+  // With HVD_AUTOTUNE set to a non-zero value in the environment: the first
+  // time this session solves `nfields` fields of lines x width, it runs the
+  // library's autotuner for that batch first (hvd_gpu_autotune_solver).
+  // Failures of the tuner are reported and otherwise ignored.
+  void autotune(int lines, int width, bool pal, int nfields);
 
  private:
   hvd_gpu_ctx* ctx_ = nullptr;
+  std::set<std::array<int, 4>> tuned_;
 };
 
 // The session free functions (BurstLockinPhase, HolographicInit, ...) use.

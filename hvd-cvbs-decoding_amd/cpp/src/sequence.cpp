@@ -79,6 +79,7 @@ std::vector<DecodedField> WindowOn(Session& ss, const std::vector<FieldObs>& fie
     p.chi = chi[j].get();
     p.y = y[j].get();
   }
+  ss.autotune(planes[0].lines, planes[0].width, g.uses_vswitch(), nf);
   // sequence.cpp:423-431 inits, then the decoupled per-field solve with the
   // standard flag from the geometry (:446) and data-fidelity luma (:1102-1114)
   ss.check(hvd_gpu_holographic_init(ss.ctx(), planes.data(), nf, cfg.symmetry_variant ? 1 : 0, nullptr),

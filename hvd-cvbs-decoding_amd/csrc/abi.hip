@@ -614,6 +614,23 @@ FieldCG field_cg_init(const DevField& f, const hvd_solver_params& p, int n_inner
   return s;
 }
 
+// Static LDS of k_aniso_select (per-wave histograms, the scan, two scalars).
+constexpr int kSelStaticLds = (16 * 256 + 256 + 2) * 4;
+
+// ResolveChromaAniso's p98 selection for the fields of sample_prefix (host
+// copy; one block per field).  A field's samples are staged in LDS when the
+// largest field's set fits.
+void aniso_select(hvd_gpu_ctx* ctx, FieldCG* dstate, const int* dsp, const float* gx, const float* gy, float lambda_c,
+                  const std::vector<int>& sample_prefix, hipStream_t st) {
+  const int nf = (int)sample_prefix.size() - 1;
+  int max_n = 0;
+  for (int i = 0; i < nf; ++i) max_n = std::max(max_n, sample_prefix[i + 1] - sample_prefix[i]);
+  const int lds_floats = (size_t)max_n * 4 + kSelStaticLds <= (size_t)ctx->max_dyn_lds ? max_n : 0;
+  hipLaunchKernelGGL(k_aniso_select, dim3(nf), dim3(kSelThreads), sizeof(float) * lds_floats, st, dstate, dsp, gx, gy,
+                     lambda_c, lds_floats);
+  LAUNCH_CHECK(ctx);
+}
+
 // Bytes run_refine stages for n fields (its descriptor region).
 size_t refine_desc_bytes(int n) {
   return align256(sizeof(DevField) * n) + align256(sizeof(int) * (n + 1)) + align256(sizeof(FieldCG) * n) +
@@ -653,9 +670,7 @@ void run_refine(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_par
     hipLaunchKernelGGL(k_aniso_samples, dim3(blocks), dim3(256), 0, st, dF, (const FieldCG*)dstate, dsp, nf, gx, gy);
     LAUNCH_CHECK(ctx);
   }
-  hipLaunchKernelGGL(k_aniso_select, dim3(nf), dim3(256), 0, st, dstate, dsp, (const float*)gx, (const float*)gy,
-                     (float)p.lambda_c);
-  LAUNCH_CHECK(ctx);
+  aniso_select(ctx, dstate, dsp, gx, gy, (float)p.lambda_c, sample_prefix, st);
   for (int o = 0; o < outer; ++o) {
     hipLaunchKernelGGL(k_irls_weights, dim3(nb), dim3(64), 0, st, dF, dstate, sg, o);
     LAUNCH_CHECK(ctx);
@@ -725,15 +740,21 @@ std::vector<std::pair<int, int>> solve_groups(const std::vector<DevField>& F, si
 }
 
 // The launch shape for a batch: the fixed tuning, else the autotuned entry for
-// its field shape (when every field shares one): all of it for the same field
-// count, else the strip height of the entry with the nearest count (its group
-// size belongs to that count; the default applies); else the built-in policy.
+// its field shape (when every field has that width and within two lines of
+// that height, as the two fields of a frame do): all of it for the same
+// field count, else the strip height of the entry with the nearest count (its
+// group size belongs to that count; the default applies); else the built-in
+// policy.
 SolverTuning tuning_for(const hvd_gpu_ctx* ctx, const std::vector<DevField>& F, int is_pal) {
   SolverTuning t = ctx->fixed;
-  if ((t.rows > 0 && t.group_px > 0) || F.empty()) return t;
+  if ((t.rows > 0 && t.group_px > 0) || F.empty() || ctx->tuned.empty()) return t;
   for (const DevField& f : F)
-    if (f.h != F[0].h || f.w != F[0].w) return t;
-  const auto it = ctx->tuned.find({F[0].h, F[0].w, is_pal ? 1 : 0});
+    if (f.w != F[0].w || std::abs(f.h - F[0].h) > 2) return t;
+  auto it = ctx->tuned.end();
+  for (int dh : {0, -1, 1, -2, 2}) {
+    it = ctx->tuned.find({F[0].h + dh, F[0].w, is_pal ? 1 : 0});
+    if (it != ctx->tuned.end()) break;
+  }
   if (it == ctx->tuned.end() || it->second.empty()) return t;
   const int n = (int)F.size();
   const auto exact = it->second.find(n);
@@ -862,6 +883,8 @@ int hvd_gpu_create(int device, hvd_gpu_ctx** out) {
     for (const void* k : {(const void*)k_fft_rows_fwd, (const void*)k_fft_cols_fwd, (const void*)k_fft_cols_inv,
                           (const void*)k_fft_rows_inv})
       HIPCHK(ctx, hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+    HIPCHK(ctx, hipFuncSetAttribute((const void*)k_aniso_select, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    std::max(0, lds - kSelStaticLds)));
   });
   if (rc != HVD_GPU_OK) {
     hvd_gpu_destroy(ctx);
@@ -1217,9 +1240,8 @@ void run_adjoint(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hvd
                        (const FieldCG*)dstate, (const int*)(base + oS), n, (float*)(ws + o_gx), (float*)(ws + o_gy));
     LAUNCH_CHECK(ctx);
   }
-  hipLaunchKernelGGL(k_aniso_select, dim3(n), dim3(256), 0, st, dstate, (const int*)(base + oS),
-                     (const float*)(ws + o_gx), (const float*)(ws + o_gy), (float)p.lambda_c);
-  LAUNCH_CHECK(ctx);
+  aniso_select(ctx, dstate, (const int*)(base + oS), (float*)(ws + o_gx), (float*)(ws + o_gy), (float)p.lambda_c, sp,
+               st);
   hipLaunchKernelGGL(k_zero_d, dim3((2 * n + 255) / 256), dim3(256), 0, st, glam, 2 * n);
   LAUNCH_CHECK(ctx);
   const dim3 pgrid(nbx, n);

@@ -127,8 +127,9 @@ __global__ void k_holo_blend(const InitField* F, const int* tp, int nf);
 __global__ void k_refine_begin(FieldCG* st, const FieldCG* init, int nf);
 __global__ void k_aniso_samples(const DevField* F, const FieldCG* st, const int* sample_prefix, int nf, float* gx,
                                 float* gy);
+constexpr int kSelThreads = 1024;  // threads of the aniso p98 selection block
 __global__ void k_aniso_select(FieldCG* st, const int* sample_prefix, const float* gx, const float* gy,
-                               float lambda_c);
+                               float lambda_c, int lds_floats);
 __global__ void k_irls_weights(const DevField* F, const FieldCG* st, StripGrid sg, int stage);
 __global__ void k_cg_start(const DevField* F, const FieldCG* st, StripGrid sg, double* partials, int adjoint);
 __global__ void k_cg_iter(const DevField* F, FieldCG* st, StripGrid sg, double* partials, int it);

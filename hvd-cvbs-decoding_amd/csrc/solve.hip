@@ -493,31 +493,47 @@ __global__ void k_aniso_samples(const DevField* F, const FieldCG* st, const int*
 }
 
 // Exact k-th smallest of n non-negative floats by 4 x 8-bit radix selection
-// (the value std::nth_element returns at k).  One block, 256 threads.
-__device__ float radix_select(const float* v, int n, int k) {
-  __shared__ unsigned hist[256];
+// (their bit patterns order as unsigned integers; the value
+// std::nth_element returns at k).  One block of kSelThreads threads; each
+// wave counts into its own 256-bin histogram (hist: 16 x 256 in LDS), the
+// bin totals are scanned in parallel.  v may be in LDS or global memory.
+__device__ float radix_select(const float* v, int n, int k, unsigned* hist) {
+  __shared__ unsigned tot[256];
   __shared__ unsigned s_prefix, s_k;
-  if (threadIdx.x == 0) { s_prefix = 0u; s_k = (unsigned)k; }
-  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (threadIdx.x == 0) {
+    s_prefix = 0u;
+    s_k = (unsigned)k;
+  }
   for (int pass = 3; pass >= 0; --pass) {
-    hist[threadIdx.x] = 0u;
+    for (int i = threadIdx.x; i < 16 * 256; i += kSelThreads) hist[i] = 0u;
     __syncthreads();
-    const unsigned prefix = s_prefix;
+    const unsigned prefix = s_prefix, kk = s_k;  // both read before this pass may update them
     const int hi_shift = (pass + 1) * 8;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    unsigned* h = hist + wv * 256;
+    for (int i = threadIdx.x; i < n; i += kSelThreads) {
       const unsigned u = __float_as_uint(v[i]);
-      const bool match = hi_shift >= 32 ? true : ((u >> hi_shift) == prefix);
-      if (match) atomicAdd(&hist[(u >> (pass * 8)) & 255u], 1u);
+      if (hi_shift >= 32 || (u >> hi_shift) == prefix) atomicAdd(&h[(u >> (pass * 8)) & 255u], 1u);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-      unsigned kk = s_k, b = 0;
-      for (; b < 256; ++b) {
-        if (kk < hist[b]) break;
-        kk -= hist[b];
+    unsigned cnt = 0u, incl = 0u;
+    if (threadIdx.x < 256) {
+      for (int w = 0; w < 16; ++w) cnt += hist[w * 256 + threadIdx.x];
+      incl = cnt;
+      for (int o = 1; o < 64; o <<= 1) {
+        const unsigned y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
       }
-      s_k = kk;
-      s_prefix = (prefix << 8) | b;
+      tot[threadIdx.x] = incl;  // inclusive scan within the wave
+    }
+    __syncthreads();
+    if (threadIdx.x < 256) {
+      for (int w = 0; w < wv; ++w) incl += tot[w * 64 + 63];
+      const unsigned excl = incl - cnt;
+      if (excl <= kk && kk < incl) {  // exactly one bin holds the k-th value
+        s_k = kk - excl;
+        s_prefix = (prefix << 8) | (unsigned)threadIdx.x;
+      }
     }
     __syncthreads();
   }
@@ -525,9 +541,12 @@ __device__ float radix_select(const float* v, int n, int k) {
 }
 
 // p98 of both sample sets, the ratio and the NTSC / PAL map into aniso
-// (variational.cpp:182-195); sets mu_h, mu_v.  One block per field.
-__global__ void k_aniso_select(FieldCG* st, const int* sample_prefix, const float* gx, const float* gy,
-                               float lambda_c) {
+// (variational.cpp:182-195); sets mu_h, mu_v.  One block per field; the
+// field's samples are staged in dynamic LDS when they fit (lds_floats).
+__global__ void __launch_bounds__(kSelThreads) k_aniso_select(FieldCG* st, const int* sample_prefix, const float* gx,
+                                                              const float* gy, float lambda_c, int lds_floats) {
+  extern __shared__ float staged[];
+  __shared__ unsigned hist[16 * 256];
   const int fi = blockIdx.x;
   FieldCG& s = st[fi];
   float aniso = s.aniso_forced;
@@ -536,8 +555,18 @@ __global__ void k_aniso_select(FieldCG* st, const int* sample_prefix, const floa
     float px = 0.f, py = 0.f;
     if (n > 0) {
       const int k = n - 1 - n / 50;
-      px = radix_select(gx + off, n, k);
-      py = radix_select(gy + off, n, k);
+      const bool fit = n <= lds_floats;
+      for (int comp = 0; comp < 2; ++comp) {
+        const float* src = (comp ? gy : gx) + off;
+        if (fit) {
+          for (int i = threadIdx.x; i < n; i += kSelThreads) staged[i] = src[i];
+          __syncthreads();
+          src = staged;
+        }
+        const float r = radix_select(src, n, k, hist);
+        if (comp) py = r; else px = r;
+        __syncthreads();  // the staging buffer is reused
+      }
     }
     const float r = py / fmaxf(px, 1e-6f);
     aniso = s.is_pal ? fminf(fmaxf(0.7f + 0.9f * (r - 1.3f), 0.55f), 1.f)

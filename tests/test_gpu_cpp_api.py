@@ -23,11 +23,16 @@ ROOT = pathlib.Path(__file__).resolve().parents[1]
 DRIVER = ROOT / "tests" / "cpp" / "bin" / "api_driver"
 
 
-def run(args):
+def run(args, env=None):
     if not DRIVER.exists():
         pytest.fail(f"{DRIVER} is not built (run __graft_entry__.build())")
-    print("api_driver", *args[:2], flush=True)
-    r = subprocess.run([str(DRIVER)] + [str(a) for a in args], capture_output=True, text=True, timeout=300)
+    print("api_driver", *args[:2], "(env " + str(env) + ")" if env else "", flush=True)
+    full_env = None
+    if env:
+        import os
+        full_env = dict(os.environ, **env)
+    r = subprocess.run([str(DRIVER)] + [str(a) for a in args], capture_output=True, text=True, timeout=300,
+                       env=full_env)
     assert r.returncode == 0, f"rc={r.returncode}\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}"
 
 
@@ -37,15 +42,16 @@ def frame_geom(std):
     return fpd, fp_o, OB.field_geometry(fp_o, synth.Geometry)
 
 
-@pytest.mark.parametrize("std,nf", [("ntsc", 3), ("pal", 2)])
-def test_decode_sequence_window(tmp_path, orc64, std, nf):
+@pytest.mark.parametrize("std,nf,tune", [("ntsc", 3, False), ("pal", 2, False), ("ntsc", 4, True)])
+def test_decode_sequence_window(tmp_path, orc64, std, nf, tune):
     """PrepareFieldObs + HvdEngine::DecodeSequenceWindow (2-D) per field vs
-    the oracle's decode_field (sequence.cpp:229-281)."""
+    the oracle's decode_field (sequence.cpp:229-281); `tune`: with
+    HVD_AUTOTUNE=1 the session runs the solver autotuner for the batch first."""
     fpd, fp_o, g = frame_geom(std)
     make = synth.pal_field if std == "pal" else synth.ntsc_field
     fields = [synth.ire_from_raw(make(40 + j)[0]) for j in range(nf)]
     np.concatenate([f.reshape(-1) for f in fields]).astype(np.float32).tofile(tmp_path / "in.f32")
-    run(["window", std, nf, tmp_path / "in.f32", tmp_path / "out.bin"])
+    run(["window", std, nf, tmp_path / "in.f32", tmp_path / "out.bin"], env={"HVD_AUTOTUNE": "1"} if tune else None)
     out = np.fromfile(tmp_path / "out.bin", np.float32)
     n = g.lines * g.width
     cfg = O.Cfg.make(cg_iterations=100, irls_outer=4, fast=0, cg_tol=1e-12, is_pal=int(std == "pal"))
