@@ -43,38 +43,96 @@ __device__ inline int mirror_row(int y, int h) {
 // In-LDS transform of `ncols` interleaved sequences of length n:
 // element (j, c) lives at buf[j * ncols + c].  Returns the buffer holding the
 // result (a or b).  sign < 0 forward (e^{-2 pi i/n}), > 0 inverse, unscaled.
+//
+// A stage of odd radix R >= 11 pairs the terms r and R - r of each output's
+// R-term sum: with w = W^{rE} and rho = W^{R E} (which depends only on the
+// output group), x_r w + x_{R-r} W^{(R-r)E} = u_r Re(w) + i v_r Im(w) where
+// u_r = x_r + rho x_{R-r}, v_r = x_r - rho x_{R-r}.  A first pass forms
+// [x_0, u_1, v_1, ..., u_h, v_h] per group (h = (R - 1) / 2) into the spare
+// buffer, a second the R outputs of the group from it: half the
+// multiply-adds of the direct sum.
 __device__ float2* fft_lds(float2* a, float2* b, const float2* root, const FftPlan& P, int sign, int ncols) {
   const int n = P.n;
   float2* src = a;
   float2* dst = b;
   int p = 1;
+  auto twiddle = [&](int ex) {
+    const float2 w = root[ex];
+    return make_float2(w.x, sign < 0 ? w.y : -w.y);
+  };
   for (int s = 0; s < P.nst; ++s) {
     const int R = P.radix[s];
     const int M = n / R;
     const int step1 = n / (p * R);
-    for (int e = threadIdx.x; e < n * ncols; e += blockDim.x) {
-      const int c = e % ncols, o = e / ncols;
-      const int k = o % p, t = o / p;
-      const int q = t % R, ib = (t / R) * p + k;
-      int E = k * step1 + q * M;
-      if (E >= n) E -= n;
-      float ar = 0.f, ai = 0.f;
-      int ex = 0;
-      for (int r = 0; r < R; ++r) {
+    if (R >= 11 && (R & 1)) {
+      const int h = (R - 1) / 2;
+      // pass 1: group ib (0 <= ib < M) -> dst[ib R ..] = x_0, u_1, v_1, ...
+      for (int e = threadIdx.x; e < M * (h + 1) * ncols; e += blockDim.x) {
+        const int c = e % ncols, g = e / ncols;
+        const int ib = g / (h + 1), r = g % (h + 1);
         const float2 x = src[(ib + r * M) * ncols + c];
-        const float2 w = root[ex];
-        const float wi = sign < 0 ? w.y : -w.y;
-        ar = __fmaf_rn(x.x, w.x, __fmaf_rn(-x.y, wi, ar));
-        ai = __fmaf_rn(x.x, wi, __fmaf_rn(x.y, w.x, ai));
-        ex += E;
-        if (ex >= n) ex -= n;
+        if (r == 0) {
+          dst[(ib * R) * ncols + c] = x;
+          continue;
+        }
+        float2 y = src[(ib + (R - r) * M) * ncols + c];
+        const int k = ib % p;
+        if (k != 0) {
+          int er = R * k * step1;
+          er %= n;
+          const float2 rho = twiddle(er);
+          y = make_float2(y.x * rho.x - y.y * rho.y, y.x * rho.y + y.y * rho.x);
+        }
+        dst[(ib * R + 2 * r - 1) * ncols + c] = make_float2(x.x + y.x, x.y + y.y);
+        dst[(ib * R + 2 * r) * ncols + c] = make_float2(x.x - y.x, x.y - y.y);
       }
-      dst[o * ncols + c] = make_float2(ar, ai);
+      __syncthreads();
+      // pass 2: the group's outputs into src
+      for (int e = threadIdx.x; e < n * ncols; e += blockDim.x) {
+        const int c = e % ncols, o = e / ncols;
+        const int k = o % p, t = o / p;
+        const int q = t % R, ib = (t / R) * p + k;
+        int E = k * step1 + q * M;
+        if (E >= n) E -= n;
+        const float2* g = dst + (ib * R) * ncols + c;
+        const float2 x0 = g[0];
+        float ar = x0.x, ai = x0.y;
+        int ex = E;
+        for (int r = 1; r <= h; ++r) {
+          const float2 u = g[(2 * r - 1) * ncols], v = g[(2 * r) * ncols];
+          const float2 w = twiddle(ex);
+          ar = __fmaf_rn(u.x, w.x, __fmaf_rn(-v.y, w.y, ar));
+          ai = __fmaf_rn(u.y, w.x, __fmaf_rn(v.x, w.y, ai));
+          ex += E;
+          if (ex >= n) ex -= n;
+        }
+        src[o * ncols + c] = make_float2(ar, ai);
+      }
+      __syncthreads();
+    } else {
+      for (int e = threadIdx.x; e < n * ncols; e += blockDim.x) {
+        const int c = e % ncols, o = e / ncols;
+        const int k = o % p, t = o / p;
+        const int q = t % R, ib = (t / R) * p + k;
+        int E = k * step1 + q * M;
+        if (E >= n) E -= n;
+        float ar = 0.f, ai = 0.f;
+        int ex = 0;
+        for (int r = 0; r < R; ++r) {
+          const float2 x = src[(ib + r * M) * ncols + c];
+          const float2 w = twiddle(ex);
+          ar = __fmaf_rn(x.x, w.x, __fmaf_rn(-x.y, w.y, ar));
+          ai = __fmaf_rn(x.x, w.y, __fmaf_rn(x.y, w.x, ai));
+          ex += E;
+          if (ex >= n) ex -= n;
+        }
+        dst[o * ncols + c] = make_float2(ar, ai);
+      }
+      __syncthreads();
+      float2* t2 = src;
+      src = dst;
+      dst = t2;
     }
-    __syncthreads();
-    float2* t2 = src;
-    src = dst;
-    dst = t2;
     p *= R;
   }
   return src;
