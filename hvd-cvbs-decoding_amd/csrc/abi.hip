@@ -634,7 +634,35 @@ void aniso_select(hvd_gpu_ctx* ctx, FieldCG* dstate, const int* dsp, const float
 // Bytes run_refine stages for n fields (its descriptor region).
 size_t refine_desc_bytes(int n) {
   return align256(sizeof(DevField) * n) + align256(sizeof(int) * (n + 1)) + align256(sizeof(FieldCG) * n) +
-         align256(sizeof(int) * (n + 1));
+         align256(sizeof(int) * (n + 1)) + align256(sizeof(int) * (n + 1));
+}
+
+// Strip layout of the per-stage weights pass and the output pass: strips of
+// at most 8 lines (no taller than the CG's), shorter still while a launch has
+// fewer than 4096 waves.
+struct AuxStrips {
+  std::vector<int> tp;
+  int nb = 0, rows = 0, spf = 0;
+};
+
+AuxStrips aux_strips(const std::vector<DevField>& F, int cg_rows) {
+  AuxStrips A;
+  const int nf = (int)F.size();
+  auto strips = [&](int rows) {
+    long n = 0;
+    for (const DevField& f : F) n += solver_strips(f.h, f.w, rows);
+    return n;
+  };
+  A.rows = std::min(cg_rows, 8);
+  while (A.rows > kMinStripRows && strips(A.rows) < 4096) A.rows /= 2;
+  A.rows = std::max(A.rows, kMinStripRows);
+  A.tp.assign(nf + 1, 0);
+  for (int i = 0; i < nf; ++i) A.tp[i + 1] = A.tp[i] + solver_strips(F[i].h, F[i].w, A.rows);
+  A.nb = A.tp[nf];
+  A.spf = A.tp[1];
+  for (int i = 1; i < nf; ++i)
+    if (F[i].h != F[0].h || F[i].w != F[0].w) A.spf = 0;
+  return A;
 }
 
 // One solve group.  desc: a fixed workspace region of refine_desc_bytes(n)
@@ -654,8 +682,12 @@ void run_refine(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_par
   const float tolf = p.cg_tol > 0.0 ? (float)p.cg_tol : (p.fast ? 0.10f : 0.02f);
   std::vector<FieldCG> init(nf);
   for (int i = 0; i < nf; ++i) init[i] = field_cg_init(F[i], p, n_inner, (double)tolf);
+  // the once-per-stage weights pass and the output pass walk shorter strips
+  // (more waves; they publish no partials, so their strip layout is free)
+  const AuxStrips AX = aux_strips(F, rows);
   Stage stg;
-  const size_t oF = stg.put(F), oT = stg.put(tp), oI = stg.put(init), oS = stg.put(sample_prefix);
+  const size_t oF = stg.put(F), oT = stg.put(tp), oI = stg.put(init), oS = stg.put(sample_prefix),
+               oA = stg.put(AX.tp);
   Slot* sl;
   REQUIRE(ctx, !desc || stg.buf.size() <= refine_desc_bytes(nf), "internal: descriptor region too small");
   char* base = desc ? upload_into(ctx, stg, desc, st, &sl) : upload(ctx, stg, st, &sl);
@@ -663,6 +695,7 @@ void run_refine(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_par
   const int* dtp = (const int*)(base + oT);
   const int* dsp = (const int*)(base + oS);
   const StripGrid sg{dtp, nf, nb, rows, spf};
+  const StripGrid sga{(const int*)(base + oA), nf, AX.nb, AX.rows, AX.spf};
   hipLaunchKernelGGL(k_refine_begin, dim3((nf + 63) / 64), dim3(64), 0, st, dstate, (const FieldCG*)(base + oI), nf);
   LAUNCH_CHECK(ctx);
   if (!(p.chroma_aniso > 0.0) && sample_prefix[nf] > 0) {
@@ -672,12 +705,11 @@ void run_refine(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_par
   }
   aniso_select(ctx, dstate, dsp, gx, gy, (float)p.lambda_c, sample_prefix, st);
   for (int o = 0; o < outer; ++o) {
-    hipLaunchKernelGGL(k_irls_weights, dim3(nb), dim3(64), 0, st, dF, dstate, sg, o);
+    hipLaunchKernelGGL(k_irls_weights, dim3(sga.nb), dim3(64), 0, st, dF, dstate, sga, o);
     LAUNCH_CHECK(ctx);
     cg_loop(ctx, dF, dstate, sg, partials, nf, n_inner, false, o, desc != nullptr && ctx->graphs, st);
   }
-  hipLaunchKernelGGL(k_refine_finalize, dim3(nb), dim3(64), 0, st, dF, (const FieldCG*)dstate,
-                     sg);
+  hipLaunchKernelGGL(k_refine_finalize, dim3(sga.nb), dim3(64), 0, st, dF, (const FieldCG*)dstate, sga);
   LAUNCH_CHECK(ctx);
   arm(ctx, sl, st);
   if (ctx->profiling) {
@@ -825,6 +857,7 @@ void check_params(hvd_gpu_ctx* ctx, const hvd_solver_params* p) {
 }
 
 void check_plane(hvd_gpu_ctx* ctx, const hvd_gpu_plane& pl, int i) {
+  REQUIRE(ctx, i < 65535, "more than 65535 planes in one call");
   REQUIRE(ctx, pl.lines >= 1 && pl.width >= 1, "plane %d: empty", i);
   REQUIRE(ctx, pl.s && pl.chi, "plane %d: null s or chi", i);
   REQUIRE(ctx, pl.carrier || pl.theta, "plane %d: needs a carrier or theta", i);
@@ -1539,6 +1572,7 @@ DecodeLayout decode_layout(const hvd_gpu_ctx* ctx, int n, const hvd_field_geomet
 }
 
 void check_fields(hvd_gpu_ctx* ctx, int n, const hvd_field_geometry* geom) {
+  REQUIRE(ctx, n <= 65535, "%d fields in one call (at most 65535)", n);
   for (int i = 0; i < n; ++i) {
     const hvd_field_geometry& g = geom[i];
     check_geom(ctx, g, i);
@@ -1556,20 +1590,27 @@ void decode_fields_run(hvd_gpu_ctx* ctx, const uint16_t* const* raw, const float
                        hipStream_t st, const DecodeLayout& L, std::vector<BridgeField>* dec_out) {
   char* ws = ctx->ws;
   REQUIRE(ctx, ctx->ws_cap >= L.end, "internal: workspace not sized");
-  // (1) codes -> IRE
+  // (1) codes -> IRE, every field in one launch
   std::vector<const float*> fields(n);
-  for (int i = 0; i < n; ++i) {
-    if (raw) {
+  if (raw) {
+    std::vector<IreDesc> D(n);
+    size_t most = 0;
+    for (int i = 0; i < n; ++i) {
       const hvd_field_geometry& g = geom[i];
-      const size_t cnt = (size_t)g.field_width * g.field_height;
-      float* dst = (float*)(ws + L.ire[i]);
-      hipLaunchKernelGGL(k_u16_to_ire, dim3((unsigned)std::min<size_t>((cnt + 255) / 256, 65536)), dim3(256), 0, st,
-                         raw[i], cnt, blanking, (white - blanking) / 100.0f, dst);
-      LAUNCH_CHECK(ctx);
-      fields[i] = dst;
-    } else {
-      fields[i] = field_ire[i];
+      D[i] = IreDesc{raw[i], (float*)(ws + L.ire[i]), (size_t)g.field_width * g.field_height};
+      most = std::max(most, D[i].n);
+      fields[i] = D[i].out;
     }
+    Stage stg;
+    const size_t oD = stg.put(D);
+    Slot* sl;
+    char* base = upload(ctx, stg, st, &sl);
+    hipLaunchKernelGGL(k_u16_to_ire_batch, dim3((unsigned)std::min<size_t>((most + 255) / 256, 4096), (unsigned)n),
+                       dim3(256), 0, st, (const IreDesc*)(base + oD), blanking, (white - blanking) / 100.0f);
+    LAUNCH_CHECK(ctx);
+    arm(ctx, sl, st);
+  } else {
+    for (int i = 0; i < n; ++i) fields[i] = field_ire[i];
   }
   // (2) lock-in
   std::vector<LockinGeom> G(n);

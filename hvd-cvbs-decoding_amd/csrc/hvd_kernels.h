@@ -109,6 +109,12 @@ struct BridgeOut {
 __global__ void k_lockin(const LockinGeom* G, const float* const* fields, float* const* theta, int8_t* const* vswitch);
 __global__ void k_burst_amp(const LockinGeom* G, const float* const* fields, float* out);
 __global__ void k_u16_to_ire(const uint16_t* raw, size_t n, float blanking, float cpi, float* out);
+struct IreDesc {
+  const uint16_t* raw;
+  float* out;
+  size_t n;
+};
+__global__ void k_u16_to_ire_batch(const IreDesc* D, float blanking, float cpi);
 __global__ void k_prepare(const PrepDesc* P);
 __global__ void k_crop(const PrepDesc* P);
 

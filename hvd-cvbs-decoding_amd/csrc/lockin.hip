@@ -294,6 +294,13 @@ __global__ void k_u16_to_ire(const uint16_t* raw, size_t n, float blanking, floa
     out[i] = ((float)raw[i] - blanking) / cpi;
 }
 
+// The same for a batch of fields in one launch: grid (x blocks, fields).
+__global__ void k_u16_to_ire_batch(const IreDesc* D, float blanking, float cpi) {
+  const IreDesc d = D[blockIdx.y];
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < d.n; i += (size_t)gridDim.x * blockDim.x)
+    d.out[i] = ((float)d.raw[i] - blanking) / cpi;
+}
+
 // Active crop + factored carrier tables for a batch (sequence.cpp:229-268):
 // s = field[fal.., a0..]; rowph[y] = e^{i theta[fal + y]} with the chroma
 // phase trim signed by the V-switch (ntsc_geometry.h:265-276); colph[x] =
