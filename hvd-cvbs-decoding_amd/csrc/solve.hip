@@ -17,6 +17,7 @@
 #include "hvd_kernels.h"
 
 
+
 namespace hvdk {
 
 namespace {
@@ -123,7 +124,6 @@ __device__ inline float shr(float v) {
 }
 __device__ inline float2 shl(float2 v) { return make_float2(shl(v.x), shl(v.y)); }
 __device__ inline float2 shr(float2 v) { return make_float2(shr(v.x), shr(v.y)); }
-__device__ inline float4 shl(float4 v) { return make_float4(shl(v.x), shl(v.y), shl(v.z), shl(v.w)); }
 
 // chi + alpha d, written once so the centre pixel and its neighbours evaluate
 // the identical expression (the value stored must equal the value used).
@@ -176,26 +176,34 @@ __device__ inline Sample make_sample(const DevField& f, const Carrier& k, const 
 
 // g = -2 lam_y [DxT(wx Dx yc) + DyT(wy Dy yc)] conj(c)
 //     + 2 [mu_h DxT(wcx Dx chi) + mu_v DyT(wcy Dy chi)]      (:286-335)
-// from the 5-point neighbourhood.  The Neumann boundaries of gradients.h (a
-// missing neighbour contributes a zero difference) come from the weights: the
-// stored weight of an edge that leaves the plane is zero (k_irls_weights), and
-// the weights of rows and columns outside the plane load as zero, so every
-// neighbour value only has to be finite.
-__device__ inline float2 grad_at(const FieldCG& st, const Sample& p0, const Sample& pl, const Sample& pr,
-                                 const Sample& pu, const Sample& pd, float4 w0, float4 wl, float4 wu) {
-  // luma data term
-  const float gx0 = pr.yc - p0.yc, gxl = p0.yc - pl.yc;
-  const float gy0 = pd.yc - p0.yc, gyu = p0.yc - pu.yc;
-  const float img = 2.f * st.lam_y * ((wl.x * gxl - w0.x * gx0) + (wu.y * gyu - w0.y * gy0));
-  // chroma prior
-  const float2 cx0 = make_float2(pr.chi.x - p0.chi.x, pr.chi.y - p0.chi.y);
-  const float2 cxl = make_float2(p0.chi.x - pl.chi.x, p0.chi.y - pl.chi.y);
-  const float2 cy0 = make_float2(pd.chi.x - p0.chi.x, pd.chi.y - p0.chi.y);
-  const float2 cyu = make_float2(p0.chi.x - pu.chi.x, p0.chi.y - pu.chi.y);
-  const float2 dxt = make_float2(wl.z * cxl.x - w0.z * cx0.x, wl.z * cxl.y - w0.z * cx0.y);
-  const float2 dyt = make_float2(wu.w * cyu.x - w0.w * cy0.x, wu.w * cyu.y - w0.w * cy0.y);
+// from the weighted differences ("fluxes") across the pixel's four edges.
+// Each edge's flux is formed once, by the pixel on its left / upper side,
+// and reaches the other pixel by a lane shift or from the previous line.
+// The Neumann boundaries of gradients.h (a missing neighbour contributes a
+// zero difference) come from the weights: the stored weight of an edge that
+// leaves the plane is zero (k_irls_weights), and the weights of rows and
+// columns outside the plane load as zero, so every neighbour value only has
+// to be finite.
+struct Flux {
+  float y;      // luma: w (yc_next - yc)
+  float2 c;     // chroma: w (chi_next - chi)
+};
+
+__device__ inline Flux flux(float w_luma, float w_chroma, const Sample& a, const Sample& b) {
+  Flux f;
+  f.y = w_luma * (b.yc - a.yc);
+  f.c = make_float2(w_chroma * (b.chi.x - a.chi.x), w_chroma * (b.chi.y - a.chi.y));
+  return f;
+}
+
+// The gradient at a pixel from the fluxes through its left (xl), right (x0),
+// upper (yu) and lower (y0) edges; c0 is the pixel's carrier.
+__device__ inline float2 grad_from_flux(const FieldCG& st, float2 c0, const Flux& xl, const Flux& x0, const Flux& yu,
+                                       const Flux& y0) {
+  const float img = 2.f * st.lam_y * ((xl.y - x0.y) + (yu.y - y0.y));
+  const float2 dxt = make_float2(xl.c.x - x0.c.x, xl.c.y - x0.c.y);
+  const float2 dyt = make_float2(yu.c.x - y0.c.x, yu.c.y - y0.c.y);
   const float2 cp = make_float2(2.f * (st.mu_h * dxt.x + st.mu_v * dyt.x), 2.f * (st.mu_h * dxt.y + st.mu_v * dyt.y));
-  const float2 c0 = p0.car;
   return make_float2(-img * c0.x + cp.x, img * c0.y + cp.y);
 }
 
@@ -384,6 +392,7 @@ __device__ inline void cg_strip(const DevField& f, const FieldCG& s, const CgDyn
   auto make = [&](const IterRaw& q, int y) { return iter_make<Start, Upd>(k, q, alpha, beta, y); };
   float2 gp = make_float2(0.f, 0.f);  // g' of the row above
   float gYp = 0.f;
+  Flux fyu;  // flux through the upper edge of the current line
   double gg = 0.0, A = 0.0, B = 0.0, D = 0.0;
   // One line of the walk (row y = y0 + j).  The rows above / at / below y and
   // their weights live in three slots each that rotate between calls, so no
@@ -401,13 +410,16 @@ __device__ inline void cg_strip(const DevField& f, const FieldCG& s, const CgDyn
     }
     wn = weights_fetch(f, y + 1, c.x, c.valid);
     dn = make(fetch(y + 1), y + 1);
-    Sample pl, pr;
-    pl.yc = shl(cur.p.yc);
+    Sample pr;
     pr.yc = shr(cur.p.yc);
-    pl.chi = shl(cur.p.chi);
     pr.chi = shr(cur.p.chi);
-    const float4 wl = shl(wc);
-    float2 gv = grad_at(s, cur.p, pl, pr, up.p, dn.p, wc, wl, wu);
+    const Flux fx = flux(wc.x, wc.z, cur.p, pr);  // through the right edge
+    const Flux fy = flux(wc.y, wc.w, cur.p, dn.p);  // through the lower edge
+    Flux fxl;
+    fxl.y = shl(fx.y);
+    fxl.c = shl(fx.c);
+    float2 gv = grad_from_flux(s, cur.p.car, fxl, fx, fyu, fy);
+    fyu = fy;
     if (Adj) {
       const int xx = min(max(c.x, 0), f.w - 1);
       const float2 b = gld32(f.rhs, (unsigned)(y * f.w + xx));
@@ -420,9 +432,11 @@ __device__ inline void cg_strip(const DevField& f, const FieldCG& s, const CgDyn
     const float gYpr = shr(gYp), dYpr = shr(up.dY);
     curv_terms(s, wu, j >= 1 && c.out, gp, gpr, gv, gYp, gYpr, gY, up.d, dpr, cur.d, up.dY, dYpr, cur.dY, !Start, A,
                B);
+    // |g|^2 and Re<g, d> of the pixel, formed in float and summed in FP64
+    // (as the curvature terms)
     const bool own = j < rows && c.out;
-    gg += own ? (double)gv.x * gv.x + (double)gv.y * gv.y : 0.0;
-    if (!Start) D += own ? (double)gv.x * cur.d.x + (double)gv.y * cur.d.y : 0.0;
+    gg += (double)(own ? __fmaf_rn(gv.x, gv.x, gv.y * gv.y) : 0.f);
+    if (!Start) D += (double)(own ? __fmaf_rn(gv.x, cur.d.x, gv.y * cur.d.y) : 0.f);
     if (own) {
       const unsigned i = (unsigned)(y * f.w + c.x);
       gst32(g_out, i, gv);
@@ -437,6 +451,7 @@ __device__ inline void cg_strip(const DevField& f, const FieldCG& s, const CgDyn
   };
   IterRow r0 = make(fetch(c.y0 - 1), c.y0 - 1), r1 = make(fetch(c.y0), c.y0), r2;
   float4 w0 = weights_fetch(f, c.y0 - 1, c.x, c.valid), w1 = weights_fetch(f, c.y0, c.x, c.valid), w2;
+  fyu = flux(w0.y, w0.w, r0.p, r1.p);
   for (int j = 0;; j += 3) {
     if (!step(j, r0, r1, r2, w0, w1, w2)) break;
     if (!step(j + 1, r1, r2, r0, w1, w2, w0)) break;
