@@ -61,7 +61,9 @@ __device__ inline StripCtx strip_ctx(const DevField* F, const StripGrid& sg) {
 // The carrier down this lane's column (ntsc_geometry.cpp:75-93).  The row
 // phasors (and V-switch signs) of the strip's rows y0 - 1 ... y0 + rows + 1
 // are loaded once, one per lane, and broadcast per row with a lane read; the
-// column phasor is loaded once per lane.
+// column phasor is loaded once per lane.  Every lane of the wave must have
+// run carrier_for before at() is used (the lane read does not look at
+// whether the lane it reads is still active).
 struct Carrier {
   float2 col;
   float2 rowl;          // row phasor of row ybase + lane
@@ -721,8 +723,10 @@ __global__ void __launch_bounds__(64) k_refine_finalize(const DevField* __restri
   const StripCtx c = strip_ctx(F, sg);
   const int rows = sg.rows;
   const DevField f = F[c.f];  // a private copy: stores cannot alias it
-  if (!c.out) return;
+  // every lane loads its row factor before any lane leaves: at(y) reads the
+  // factor from lane y - ybase, which may be a lane that produces no output
   const Carrier k = carrier_for(f, c);
+  if (!c.out) return;
   const int cur = st[c.f].cur;
   for (int r = 0; r < rows; ++r) {
     const int y = c.y0 + r;
