@@ -569,7 +569,7 @@ hipGraphExec_t cg_graph(hvd_gpu_ctx* ctx, const DevField* dF, FieldCG* dstate, c
 void cg_loop(hvd_gpu_ctx* ctx, const DevField* dF, FieldCG* dstate, const StripGrid& sg, double* partials, int nf,
              int n_inner, bool adjoint, int stage, bool graph, hipStream_t st) {
   hipLaunchKernelGGL(k_cg_start, dim3(sg.nb), dim3(64), 0, st, dF, (const FieldCG*)dstate, sg, partials,
-                     (int)adjoint);
+                     (int)adjoint, stage);
   LAUNCH_CHECK(ctx);
   std::pair<hipEvent_t, hipEvent_t> evs{nullptr, nullptr};
   if (ctx->profiling) {
@@ -704,11 +704,9 @@ void run_refine(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_par
     LAUNCH_CHECK(ctx);
   }
   aniso_select(ctx, dstate, dsp, gx, gy, (float)p.lambda_c, sample_prefix, st);
-  for (int o = 0; o < outer; ++o) {
-    hipLaunchKernelGGL(k_irls_weights, dim3(sga.nb), dim3(64), 0, st, dF, dstate, sga, o);
-    LAUNCH_CHECK(ctx);
+  // each stage: weights + stage-start gradient (one launch), then the CG
+  for (int o = 0; o < outer; ++o)
     cg_loop(ctx, dF, dstate, sg, partials, nf, n_inner, false, o, desc != nullptr && ctx->graphs, st);
-  }
   hipLaunchKernelGGL(k_refine_finalize, dim3(sga.nb), dim3(64), 0, st, dF, (const FieldCG*)dstate, sga);
   LAUNCH_CHECK(ctx);
   arm(ctx, sl, st);
@@ -1431,7 +1429,7 @@ TuneResult autotune(hvd_gpu_ctx* ctx, int h, int w, bool pal, int n, int iterati
   p.fast = 0;
   p.is_pal = pal ? 1 : 0;
   p.lambda_y = 1.0;
-  // candidates: fields per group from ceil(n / G), strips of 2..32 lines
+  // candidates: fields per group from ceil(n / G)
   std::vector<int> per;
   for (int G = 1; G <= n; ++G) {
     const int k = (n + G - 1) / G;
@@ -1439,7 +1437,10 @@ TuneResult autotune(hvd_gpu_ctx* ctx, int h, int w, bool pal, int n, int iterati
     if ((px >= 2.5e5 && px <= 1.2e7) || k == n)
       if (per.empty() || per.back() != k) per.push_back(k);
   }
-  const int rows_c[] = {2, 4, 6, 8, 10, 12, 14, 16, 20, 24, 32};
+  // fine-pass strip heights: every one from 2 to 32 (how evenly a height
+  // divides the field and how many waves it leaves both matter)
+  std::vector<int> rows_c;
+  for (int r = kMinStripRows; r <= kMaxStripRows; ++r) rows_c.push_back(r);
   // the user's fixed tuning and profiling state are restored on every exit
   struct Restore {
     hvd_gpu_ctx* c;

@@ -137,7 +137,8 @@ constexpr int kSelThreads = 1024;  // threads of the aniso p98 selection block
 __global__ void k_aniso_select(FieldCG* st, const int* sample_prefix, const float* gx, const float* gy,
                                float lambda_c, int lds_floats);
 __global__ void k_irls_weights(const DevField* F, const FieldCG* st, StripGrid sg, int stage);
-__global__ void k_cg_start(const DevField* F, const FieldCG* st, StripGrid sg, double* partials, int adjoint);
+__global__ void k_cg_start(const DevField* F, const FieldCG* st, StripGrid sg, double* partials, int adjoint,
+                           int stage);
 __global__ void k_cg_iter(const DevField* F, FieldCG* st, StripGrid sg, double* partials, int it);
 __global__ void k_cg_iter_adj(const DevField* F, FieldCG* st, StripGrid sg, double* partials, int it);
 __global__ void k_cg_finish(FieldCG* st, StripGrid sg, const double* partials, int n_inner, int stage);

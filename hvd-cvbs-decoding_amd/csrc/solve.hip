@@ -363,6 +363,26 @@ __device__ inline void curv_terms(const FieldCG& s, float4 w, bool on, float2 a0
   }
 }
 
+// IRLS lagged-diffusivity weights of one pixel (variational.cpp:406-433) from
+// its own sample p0, its right neighbour's luma yr and chroma cr and its lower
+// neighbour pd.  An edge that leaves the plane (hr / hd false) gets weight 0
+// (its difference is 0 in hvd-core, so its term vanishes either way).
+__device__ inline float4 irls_weight(const FieldCG& s, const Sample& p0, float yr, float2 cr, const Sample& pd, bool hr,
+                                     bool hd) {
+  const float e2 = s.eps * s.eps, ec2 = s.eps_c * s.eps_c;
+  const float gx = hr ? yr - p0.yc : 0.f;
+  const float gy = hd ? pd.yc - p0.yc : 0.f;
+  const float2 cx = hr ? make_float2(cr.x - p0.chi.x, cr.y - p0.chi.y) : make_float2(0.f, 0.f);
+  const float2 cy = hd ? make_float2(pd.chi.x - p0.chi.x, pd.chi.y - p0.chi.y) : make_float2(0.f, 0.f);
+  const float mx = sqrtf(cx.x * cx.x + cx.y * cx.y), my = sqrtf(cy.x * cy.x + cy.y * cy.y);
+  float4 w;
+  w.x = hr ? s.eps / sqrtf(gx * gx + e2) : 0.f;
+  w.y = hd ? s.eps / sqrtf(gy * gy + e2) : 0.f;
+  w.z = hr ? s.eps_c / sqrtf(mx * mx + s.k * gx * gx + ec2) : 0.f;
+  w.w = hd ? s.eps_c / sqrtf(my * my + s.k * gy * gy + ec2) : 0.f;
+  return w;
+}
+
 // The fused CG iteration over one strip (variational.cpp:527-548, one
 // iteration per launch).  With d = d(it) formed on the fly and
 // chi' = chi + alpha d, it writes chi', d and g' = grad(chi') (:286-347, less
@@ -378,9 +398,14 @@ __device__ inline void curv_terms(const FieldCG& s, float4 w, bool on, float2 a0
 // terms need g' one column right and one row down, so the walk forms g' for
 // rows + 1 lines).  The loop body is branch-free apart from the stores: loads
 // are clamped and results selected.
-template <bool Start, bool Upd, bool Adj>
+//
+// FuseW (the forward's stage start): the IRLS weights of the stage are formed
+// here from chi (variational.cpp:406-433) instead of loaded, and stored for
+// the stage's iterations (with chi into f.save + stage when set).
+template <bool Start, bool Upd, bool Adj, bool FuseW = false>
 __device__ inline void cg_strip(const DevField& f, const FieldCG& s, const CgDyn& q, const StripCtx& c, int rows,
-                                double* partials) {
+                                double* partials, int stage = 0) {
+  static_assert(!FuseW || Start, "weights are formed by the stage-start pass only");
   const Carrier k = carrier_for(f, c);
   const float2* chi = pick(f.chi, q.cur);
   const float2* gsrc = pick(f.g, q.cur);
@@ -399,7 +424,7 @@ __device__ inline void cg_strip(const DevField& f, const FieldCG& s, const CgDyn
   // One line of the walk (row y = y0 + j).  The rows above / at / below y and
   // their weights live in three slots each that rotate between calls, so no
   // registers are copied from line to line.  Returns false when the walk ends.
-  auto step = [&](int j, const IterRow& up, const IterRow& cur, IterRow& dn, const float4& wu, const float4& wc,
+  auto step = [&](int j, const IterRow& up, const IterRow& cur, IterRow& dn, const float4& wu, float4& wc,
                   float4& wn) -> bool {
     if (j > rows) return false;
     const int y = c.y0 + j;
@@ -410,11 +435,20 @@ __device__ inline void cg_strip(const DevField& f, const FieldCG& s, const CgDyn
                  B);
       return false;
     }
-    wn = weights_fetch(f, y + 1, c.x, c.valid);
+    if (!FuseW) wn = weights_fetch(f, y + 1, c.x, c.valid);
     dn = make(fetch(y + 1), y + 1);
     Sample pr;
     pr.yc = shr(cur.p.yc);
     pr.chi = shr(cur.p.chi);
+    if (FuseW) {  // this line's weights, from this line and the next
+      wc = c.valid ? irls_weight(s, cur.p, pr.yc, pr.chi, dn.p, c.x + 1 < f.w, y + 1 < f.h)
+                   : make_float4(0.f, 0.f, 0.f, 0.f);
+      if (j < rows && c.out) {
+        const unsigned i = (unsigned)(y * f.w + c.x);
+        gst(f.wts, (size_t)i, wc);
+        if (f.save) gst(f.save + (size_t)stage * f.h * f.w, (size_t)i, cur.p.chi);
+      }
+    }
     const Flux fx = flux(wc.x, wc.z, cur.p, pr);  // through the right edge
     const Flux fy = flux(wc.y, wc.w, cur.p, dn.p);  // through the lower edge
     Flux fxl;
@@ -452,7 +486,16 @@ __device__ inline void cg_strip(const DevField& f, const FieldCG& s, const CgDyn
     return true;
   };
   IterRow r0 = make(fetch(c.y0 - 1), c.y0 - 1), r1 = make(fetch(c.y0), c.y0), r2;
-  float4 w0 = weights_fetch(f, c.y0 - 1, c.x, c.valid), w1 = weights_fetch(f, c.y0, c.x, c.valid), w2;
+  float4 w0, w1, w2;
+  if (FuseW) {  // the weights of the line above the strip (rows outside the plane: 0)
+    const float yr = shr(r0.p.yc);
+    const float2 cr = shr(r0.p.chi);
+    w0 = c.valid && c.y0 >= 1 ? irls_weight(s, r0.p, yr, cr, r1.p, c.x + 1 < f.w, c.y0 < f.h)
+                              : make_float4(0.f, 0.f, 0.f, 0.f);
+  } else {
+    w0 = weights_fetch(f, c.y0 - 1, c.x, c.valid);
+    w1 = weights_fetch(f, c.y0, c.x, c.valid);
+  }
   fyu = flux(w0.y, w0.w, r0.p, r1.p);
   for (int j = 0;; j += 3) {
     if (!step(j, r0, r1, r2, w0, w1, w2)) break;
@@ -604,7 +647,6 @@ __global__ void __launch_bounds__(64) k_irls_weights(const DevField* __restrict_
   const FieldCG s = st[c.f];
   const Carrier k = carrier_for(f, c);
   const float2* chi = pick(f.chi, s.cur);
-  const float e2 = s.eps * s.eps, ec2 = s.eps_c * s.eps_c;
   Sample cur = make_sample(f, k, raw_row(f, k, chi, nullptr, c.y0, c.valid), false, 0.f, c.y0, c.valid);
   for (int r = 0; r < rows; ++r) {
     const int y = c.y0 + r;
@@ -613,19 +655,7 @@ __global__ void __launch_bounds__(64) k_irls_weights(const DevField* __restrict_
     const float yr = shr(cur.yc);
     const float2 cr = shr(cur.chi);
     if (c.out) {
-      const bool hr = c.x + 1 < f.w, hd = y + 1 < f.h;
-      const float gx = hr ? yr - cur.yc : 0.f;
-      const float gy = hd ? dn.yc - cur.yc : 0.f;
-      const float2 cx = hr ? make_float2(cr.x - cur.chi.x, cr.y - cur.chi.y) : make_float2(0.f, 0.f);
-      const float2 cy = hd ? make_float2(dn.chi.x - cur.chi.x, dn.chi.y - cur.chi.y) : make_float2(0.f, 0.f);
-      const float mx = sqrtf(cx.x * cx.x + cx.y * cx.y), my = sqrtf(cy.x * cy.x + cy.y * cy.y);
-      // an edge that leaves the plane stores weight 0 (its difference is 0 in
-      // hvd-core, so its term vanishes either way)
-      float4 w;
-      w.x = hr ? s.eps / sqrtf(gx * gx + e2) : 0.f;
-      w.y = hd ? s.eps / sqrtf(gy * gy + e2) : 0.f;
-      w.z = hr ? s.eps_c / sqrtf(mx * mx + s.k * gx * gx + ec2) : 0.f;
-      w.w = hd ? s.eps_c / sqrtf(my * my + s.k * gy * gy + ec2) : 0.f;
+      const float4 w = irls_weight(s, cur, yr, cr, dn, c.x + 1 < f.w, y + 1 < f.h);
       gst(f.wts, (size_t)y * f.w + c.x, w);
       if (f.save) gst(f.save + (size_t)stage * f.h * f.w, (size_t)y * f.w + c.x, cur.chi);
     }
@@ -633,18 +663,22 @@ __global__ void __launch_bounds__(64) k_irls_weights(const DevField* __restrict_
   }
 }
 
-// Stage start (variational.cpp:519-525): g = grad(chi) and the partials of
-// |g|^2 and curv(g) (the first direction is d = -g), into partial slot 1.
+// Stage start (variational.cpp:406-433, 519-525): for a forward solve the
+// stage's IRLS weights from chi (stored for the iterations), then g = grad(chi)
+// and the partials of |g|^2 and curv(g) (the first direction is d = -g), into
+// partial slot 1.
 __global__ void __launch_bounds__(64) k_cg_start(const DevField* __restrict__ F, const FieldCG* __restrict__ st,
-                                                 StripGrid sg, double* partials, int adjoint) {
+                                                 StripGrid sg, double* partials, int adjoint, int stage) {
   const StripCtx c = strip_ctx(F, sg);
   const DevField f = F[c.f];  // a private copy: stores cannot alias it
   const FieldCG s = st[c.f];
   CgDyn q{};
   q.cur = s.cur;
   double* out = partials + (size_t)sg.nb * kCgPartials;
+  // the adjoint's stage weights come from k_irls_weights (they belong to the
+  // forward's composite, which the adjoint fields do not carry)
   if (adjoint) cg_strip<true, false, true>(f, s, q, c, sg.rows, out);
-  else cg_strip<true, false, false>(f, s, q, c, sg.rows, out);
+  else cg_strip<true, false, false, true>(f, s, q, c, sg.rows, out, stage);
 }
 
 namespace {
