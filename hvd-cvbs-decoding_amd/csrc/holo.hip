@@ -44,13 +44,15 @@ __device__ inline int mirror_row(int y, int h) {
 // element (j, c) lives at buf[j * ncols + c].  Returns the buffer holding the
 // result (a or b).  sign < 0 forward (e^{-2 pi i/n}), > 0 inverse, unscaled.
 //
-// A stage of odd radix R >= 11 pairs the terms r and R - r of each output's
-// R-term sum: with w = W^{rE} and rho = W^{R E} (which depends only on the
-// output group), x_r w + x_{R-r} W^{(R-r)E} = u_r Re(w) + i v_r Im(w) where
-// u_r = x_r + rho x_{R-r}, v_r = x_r - rho x_{R-r}.  A first pass forms
-// [x_0, u_1, v_1, ..., u_h, v_h] per group (h = (R - 1) / 2) into the spare
-// buffer, a second the R outputs of the group from it: half the
-// multiply-adds of the direct sum.
+// A stage of odd radix R >= 11 runs as a length-R DFT of pre-twiddled inputs
+// a_r = x_r W^{r k step1} (k the output group's phase), using both of its
+// symmetries: the terms r and R - r pair as u_r = a_r + a_{R-r},
+// v_r = a_r - a_{R-r}, and the outputs q and R - q share
+// U = sum_r u_r cos(2 pi r q / R), V = sum_r v_r sin(2 pi r q / R):
+//     y_q = a_0 + U + i V,   y_{R-q} = a_0 + U - i V.
+// A first pass forms [a_0, u_1, v_1, ..., u_h, v_h] per group (h = (R - 1) / 2)
+// into the spare buffer, a second forms two outputs per work item from it: a
+// quarter of the multiply-adds of the direct R-term sums.
 __device__ float2* fft_lds(float2* a, float2* b, const float2* root, const FftPlan& P, int sign, int ncols) {
   const int n = P.n;
   float2* src = a;
@@ -66,47 +68,48 @@ __device__ float2* fft_lds(float2* a, float2* b, const float2* root, const FftPl
     const int step1 = n / (p * R);
     if (R >= 11 && (R & 1)) {
       const int h = (R - 1) / 2;
-      // pass 1: group ib (0 <= ib < M) -> dst[ib R ..] = x_0, u_1, v_1, ...
+      // pass 1: group ib (0 <= ib < M) -> dst[ib R ..] = a_0, u_1, v_1, ...
       for (int e = threadIdx.x; e < M * (h + 1) * ncols; e += blockDim.x) {
         const int c = e % ncols, g = e / ncols;
         const int ib = g / (h + 1), r = g % (h + 1);
-        const float2 x = src[(ib + r * M) * ncols + c];
+        const int k = ib % p;
+        auto pre = [&](int rr) {  // a_rr = x_rr W^{rr k step1}
+          const float2 x = src[(ib + rr * M) * ncols + c];
+          if (k == 0) return x;
+          const float2 w = twiddle((int)(((long long)rr * k * step1) % n));
+          return make_float2(x.x * w.x - x.y * w.y, x.x * w.y + x.y * w.x);
+        };
         if (r == 0) {
-          dst[(ib * R) * ncols + c] = x;
+          dst[(ib * R) * ncols + c] = pre(0);
           continue;
         }
-        float2 y = src[(ib + (R - r) * M) * ncols + c];
-        const int k = ib % p;
-        if (k != 0) {
-          int er = R * k * step1;
-          er %= n;
-          const float2 rho = twiddle(er);
-          y = make_float2(y.x * rho.x - y.y * rho.y, y.x * rho.y + y.y * rho.x);
-        }
+        const float2 x = pre(r), y = pre(R - r);
         dst[(ib * R + 2 * r - 1) * ncols + c] = make_float2(x.x + y.x, x.y + y.y);
         dst[(ib * R + 2 * r) * ncols + c] = make_float2(x.x - y.x, x.y - y.y);
       }
       __syncthreads();
-      // pass 2: the group's outputs into src
-      for (int e = threadIdx.x; e < n * ncols; e += blockDim.x) {
-        const int c = e % ncols, o = e / ncols;
-        const int k = o % p, t = o / p;
-        const int q = t % R, ib = (t / R) * p + k;
-        int E = k * step1 + q * M;
-        if (E >= n) E -= n;
-        const float2* g = dst + (ib * R) * ncols + c;
-        const float2 x0 = g[0];
-        float ar = x0.x, ai = x0.y;
+      // pass 2: outputs q and R - q of group ib into src
+      for (int e = threadIdx.x; e < M * (h + 1) * ncols; e += blockDim.x) {
+        const int c = e % ncols, g = e / ncols;
+        const int ib = g / (h + 1), q = g % (h + 1);
+        const int k = ib % p, j = ib / p;
+        const float2* gp = dst + (ib * R) * ncols + c;
+        const float2 a0 = gp[0];
+        float ux = 0.f, uy = 0.f, vx = 0.f, vy = 0.f;
+        const int E = q * M;  // < n
         int ex = E;
         for (int r = 1; r <= h; ++r) {
-          const float2 u = g[(2 * r - 1) * ncols], v = g[(2 * r) * ncols];
+          const float2 u = gp[(2 * r - 1) * ncols], v = gp[(2 * r) * ncols];
           const float2 w = twiddle(ex);
-          ar = __fmaf_rn(u.x, w.x, __fmaf_rn(-v.y, w.y, ar));
-          ai = __fmaf_rn(u.y, w.x, __fmaf_rn(v.x, w.y, ai));
+          ux = __fmaf_rn(u.x, w.x, ux);
+          uy = __fmaf_rn(u.y, w.x, uy);
+          vx = __fmaf_rn(v.x, w.y, vx);
+          vy = __fmaf_rn(v.y, w.y, vy);
           ex += E;
           if (ex >= n) ex -= n;
         }
-        src[o * ncols + c] = make_float2(ar, ai);
+        src[((j * R + q) * p + k) * ncols + c] = make_float2(a0.x + ux - vy, a0.y + uy + vx);
+        if (q > 0) src[((j * R + R - q) * p + k) * ncols + c] = make_float2(a0.x + ux + vy, a0.y + uy - vx);
       }
       __syncthreads();
     } else {
