@@ -53,7 +53,97 @@ __device__ inline int mirror_row(int y, int h) {
 // A first pass forms [a_0, u_1, v_1, ..., u_h, v_h] per group (h = (R - 1) / 2)
 // into the spare buffer, a second forms two outputs per work item from it: a
 // quarter of the multiply-adds of the direct R-term sums.
-__device__ float2* fft_lds(float2* a, float2* b, const float2* root, const FftPlan& P, int sign, int ncols) {
+__device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// x * (si i), si = +-1
+__device__ inline float2 mul_si(float2 x, float si) { return make_float2(-si * x.y, si * x.x); }
+
+// y = DFT_4(a) with W_4 = si i (si = -1 forward, +1 inverse).
+__device__ inline void dft4(float2 a0, float2 a1, float2 a2, float2 a3, float si, float2& y0, float2& y1, float2& y2,
+                            float2& y3) {
+  const float2 t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = mul_si(csub(a1, a3), si);
+  y0 = cadd(t0, t2);
+  y2 = csub(t0, t2);
+  y1 = cadd(t1, t3);
+  y3 = csub(t1, t3);
+}
+
+// One Stockham stage of small radix R (2, 3, 4, 5, 7, 8) with one work item
+// per output group: the group's R inputs are pre-twiddled by W^{r k step1}
+// and transformed by an R-point DFT written out (butterflies for 2, 4, 8; the
+// symmetric pair form for 3, 5, 7, whose constants come from the root table).
+template <int R, typename Tw>
+__device__ inline void small_stage(const float2* src, float2* dst, int n, int p, int M, int step1, float si, int ncols,
+                                   Tw twiddle) {
+  for (int e = threadIdx.x; e < M * ncols; e += blockDim.x) {
+    const int c = e % ncols, ib = e / ncols;
+    const int k = ib % p, j = ib / p;
+    float2 a[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      float2 x = src[(ib + r * M) * ncols + c];
+      if (r > 0 && k != 0) {
+        const float2 w = twiddle(r * k * step1);  // < R (p / p) (n / R) = n
+        x = make_float2(x.x * w.x - x.y * w.y, x.x * w.y + x.y * w.x);
+      }
+      a[r] = x;
+    }
+    float2 y[R];
+    if constexpr (R == 2) {
+      y[0] = cadd(a[0], a[1]);
+      y[1] = csub(a[0], a[1]);
+    } else if constexpr (R == 4) {
+      dft4(a[0], a[1], a[2], a[3], si, y[0], y[1], y[2], y[3]);
+    } else if constexpr (R == 8) {
+      float2 e0, e1, e2, e3, o0, o1, o2, o3;
+      dft4(a[0], a[2], a[4], a[6], si, e0, e1, e2, e3);
+      dft4(a[1], a[3], a[5], a[7], si, o0, o1, o2, o3);
+      const float h2 = 0.70710678118654752f;
+      // W_8^1 = (h2, si h2), W_8^2 = si i, W_8^3 = (-h2, si h2)
+      const float2 w1 = make_float2(h2 * (o1.x - si * o1.y), h2 * (o1.y + si * o1.x));
+      const float2 w2 = mul_si(o2, si);
+      const float2 w3 = make_float2(h2 * (-o3.x - si * o3.y), h2 * (-o3.y + si * o3.x));
+      y[0] = cadd(e0, o0);
+      y[4] = csub(e0, o0);
+      y[1] = cadd(e1, w1);
+      y[5] = csub(e1, w1);
+      y[2] = cadd(e2, w2);
+      y[6] = csub(e2, w2);
+      y[3] = cadd(e3, w3);
+      y[7] = csub(e3, w3);
+    } else {  // odd R: pairs r / R - r and outputs q / R - q
+      constexpr int h = (R - 1) / 2;
+      float2 u[h + 1], v[h + 1];
+      float2 y0 = a[0];
+#pragma unroll
+      for (int r = 1; r <= h; ++r) {
+        u[r] = cadd(a[r], a[R - r]);
+        v[r] = csub(a[r], a[R - r]);
+        y0 = cadd(y0, u[r]);
+      }
+      y[0] = y0;
+#pragma unroll
+      for (int q = 1; q <= h; ++q) {
+        float ux = 0.f, uy = 0.f, vx = 0.f, vy = 0.f;
+#pragma unroll
+        for (int r = 1; r <= h; ++r) {
+          const float2 w = twiddle((r * q) % R * M);
+          ux = __fmaf_rn(u[r].x, w.x, ux);
+          uy = __fmaf_rn(u[r].y, w.x, uy);
+          vx = __fmaf_rn(v[r].x, w.y, vx);
+          vy = __fmaf_rn(v[r].y, w.y, vy);
+        }
+        y[q] = make_float2(a[0].x + ux - vy, a[0].y + uy + vx);
+        y[R - q] = make_float2(a[0].x + ux + vy, a[0].y + uy - vx);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) dst[((j * R + q) * p + k) * ncols + c] = y[q];
+  }
+}
+
+__device__ __attribute__((always_inline)) float2* fft_lds(float2* a, float2* b, const float2* root, const FftPlan& P,
+                                                          int sign, int ncols) {
   const int n = P.n;
   float2* src = a;
   float2* dst = b;
@@ -76,7 +166,7 @@ __device__ float2* fft_lds(float2* a, float2* b, const float2* root, const FftPl
         auto pre = [&](int rr) {  // a_rr = x_rr W^{rr k step1}
           const float2 x = src[(ib + rr * M) * ncols + c];
           if (k == 0) return x;
-          const float2 w = twiddle((int)(((long long)rr * k * step1) % n));
+          const float2 w = twiddle(rr * k * step1);  // k step1 < n / R
           return make_float2(x.x * w.x - x.y * w.y, x.x * w.y + x.y * w.x);
         };
         if (r == 0) {
@@ -112,6 +202,20 @@ __device__ float2* fft_lds(float2* a, float2* b, const float2* root, const FftPl
         if (q > 0) src[((j * R + R - q) * p + k) * ncols + c] = make_float2(a0.x + ux + vy, a0.y + uy - vx);
       }
       __syncthreads();
+    } else if (R <= 8 && R != 6) {
+      const float si = sign < 0 ? -1.f : 1.f;
+      switch (R) {
+        case 2: small_stage<2>(src, dst, n, p, M, step1, si, ncols, twiddle); break;
+        case 3: small_stage<3>(src, dst, n, p, M, step1, si, ncols, twiddle); break;
+        case 4: small_stage<4>(src, dst, n, p, M, step1, si, ncols, twiddle); break;
+        case 5: small_stage<5>(src, dst, n, p, M, step1, si, ncols, twiddle); break;
+        case 7: small_stage<7>(src, dst, n, p, M, step1, si, ncols, twiddle); break;
+        default: small_stage<8>(src, dst, n, p, M, step1, si, ncols, twiddle); break;
+      }
+      __syncthreads();
+      float2* t2 = src;
+      src = dst;
+      dst = t2;
     } else {
       for (int e = threadIdx.x; e < n * ncols; e += blockDim.x) {
         const int c = e % ncols, o = e / ncols;

@@ -375,11 +375,21 @@ __device__ inline float4 irls_weight(const FieldCG& s, const Sample& p0, float y
   const float2 cx = hr ? make_float2(cr.x - p0.chi.x, cr.y - p0.chi.y) : make_float2(0.f, 0.f);
   const float2 cy = hd ? make_float2(pd.chi.x - p0.chi.x, pd.chi.y - p0.chi.y) : make_float2(0.f, 0.f);
   const float mx = sqrtf(cx.x * cx.x + cx.y * cx.y), my = sqrtf(cy.x * cy.x + cy.y * cy.y);
+  // eps / sqrt(.) as eps * rsq(.) (v_rsq_f32, specified to 1 ulp, like the
+  // correctly rounded quotient of a correctly rounded root) when every
+  // argument is a normal float (>= eps^2); otherwise the IEEE forms
   float4 w;
-  w.x = hr ? s.eps / sqrtf(gx * gx + e2) : 0.f;
-  w.y = hd ? s.eps / sqrtf(gy * gy + e2) : 0.f;
-  w.z = hr ? s.eps_c / sqrtf(mx * mx + s.k * gx * gx + ec2) : 0.f;
-  w.w = hd ? s.eps_c / sqrtf(my * my + s.k * gy * gy + ec2) : 0.f;
+  if (e2 >= 1.17549435e-38f && ec2 >= 1.17549435e-38f) {
+    w.x = hr ? s.eps * __builtin_amdgcn_rsqf(gx * gx + e2) : 0.f;
+    w.y = hd ? s.eps * __builtin_amdgcn_rsqf(gy * gy + e2) : 0.f;
+    w.z = hr ? s.eps_c * __builtin_amdgcn_rsqf(mx * mx + s.k * gx * gx + ec2) : 0.f;
+    w.w = hd ? s.eps_c * __builtin_amdgcn_rsqf(my * my + s.k * gy * gy + ec2) : 0.f;
+  } else {
+    w.x = hr ? s.eps / sqrtf(gx * gx + e2) : 0.f;
+    w.y = hd ? s.eps / sqrtf(gy * gy + e2) : 0.f;
+    w.z = hr ? s.eps_c / sqrtf(mx * mx + s.k * gx * gx + ec2) : 0.f;
+    w.w = hd ? s.eps_c / sqrtf(my * my + s.k * gy * gy + ec2) : 0.f;
+  }
   return w;
 }
 
