@@ -735,13 +735,13 @@ int aniso_samples(int h, int w, int is_pal) {
   return (w >= 2 && nt > 0) ? nt * nx : 0;
 }
 
-// Solve groups: consecutive fields whose solver working set (about 68 B per
-// pixel: chi x2, g x2, d x2, weights, composite) mostly stays in the 256 MB
+// Solve groups: consecutive fields whose solver working set (about 52 B per
+// pixel: chi x2, d x2, weights, composite) mostly stays in the 256 MB
 // Infinity Cache, so a group's CG iterations re-read their planes from the
 // cache rather than HBM.  A batch is cut into the fewest groups of at most
 // about `max_px` pixels, balanced in size; groups run one after another on
 // the stream and share one workspace region.
-constexpr size_t kSolveGroupPx = 3000000;  // default; 16 NTSC fields measured best
+constexpr size_t kSolveGroupPx = 4000000;  // default; 21-22 NTSC fields measured best
 
 std::vector<std::pair<int, int>> solve_groups(const std::vector<DevField>& F, size_t max_px) {
   std::vector<std::pair<int, int>> g;
@@ -806,7 +806,7 @@ SolverTuning tuning_for(const hvd_gpu_ctx* ctx, const std::vector<DevField>& F, 
 // One solve group's workspace, carved after `cv`.  Partials are sized for the
 // shortest strips so any strip height the launch picks fits.
 struct GroupLayout {
-  std::vector<size_t> chi1, g, d, w;
+  std::vector<size_t> chi1, d, w;
   std::vector<int> sp;  // aniso sample prefix
   size_t part = 0, gx = 0, gy = 0, st = 0, desc = 0, end = 0;
 };
@@ -814,7 +814,6 @@ struct GroupLayout {
 GroupLayout group_layout(Carve cv, const DevField* F, int n, int is_pal) {
   GroupLayout L;
   L.chi1.resize(n);
-  L.g.resize(n);
   L.d.resize(n);
   L.w.resize(n);
   L.sp.assign(n + 1, 0);
@@ -822,7 +821,6 @@ GroupLayout group_layout(Carve cv, const DevField* F, int n, int is_pal) {
   for (int i = 0; i < n; ++i) {
     const size_t np = (size_t)F[i].h * F[i].w;
     L.chi1[i] = cv.take(sizeof(float2) * np);
-    L.g[i] = cv.take(sizeof(float2) * np * 2);
     L.d[i] = cv.take(sizeof(float2) * np * 2);
     L.w[i] = cv.take(sizeof(float4) * np);
     strips += (size_t)solver_strips(F[i].h, F[i].w, kMinStripRows);
@@ -1113,8 +1111,6 @@ void refine_planes(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_
     std::vector<DevField> G(F.begin() + b, F.begin() + gr.second);
     for (int i = 0; i < n; ++i) {
       G[i].chi[1] = (float2*)(ws + L.chi1[i]);
-      G[i].g[0] = (float2*)(ws + L.g[i]);
-      G[i].g[1] = G[i].g[0] + (size_t)G[i].h * G[i].w;
       G[i].d[0] = (float2*)(ws + L.d[i]);
       G[i].d[1] = G[i].d[0] + (size_t)G[i].h * G[i].w;
       G[i].wts = (float4*)(ws + L.w[i]);
@@ -1143,7 +1139,7 @@ void run_adjoint(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hvd
                  double adj_tol, float* const* gs, float* const* gchi0, double* glam_host, hipStream_t st) {
   const int K = std::max(1, p.irls_outer);
   Carve cv;
-  std::vector<size_t> o_row(n), o_col(n), o_vsw(n), o_chi1(n), o_g(n), o_d(n), o_w(n), o_a(n), o_sc(n);
+  std::vector<size_t> o_row(n), o_col(n), o_vsw(n), o_chi1(n), o_d(n), o_w(n), o_a(n), o_sc(n);
   std::vector<int> sp(n + 1, 0);
   size_t strips = 0, maxpx = 0;
   for (int i = 0; i < n; ++i) {
@@ -1156,7 +1152,6 @@ void run_adjoint(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hvd
     o_col[i] = cv.take(sizeof(float2) * planes[i].width);
     o_a[i] = cv.take(sizeof(float2) * np);
     o_chi1[i] = cv.take(sizeof(float2) * np);
-    o_g[i] = cv.take(sizeof(float2) * np * 2);
     o_d[i] = cv.take(sizeof(float2) * np * 2);
     o_w[i] = cv.take(sizeof(float4) * np);
     o_sc[i] = cv.take(sizeof(float) * 6 * np);
@@ -1190,8 +1185,6 @@ void run_adjoint(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hvd
       f.vsw = pl.vswitch ? (const int8_t*)(ws + o_vsw[i]) : nullptr;
     }
     f.wts = (float4*)(ws + o_w[i]);
-    f.g[0] = (float2*)(ws + o_g[i]);
-    f.g[1] = f.g[0] + np;
     f.d[0] = (float2*)(ws + o_d[i]);
     f.d[1] = f.d[0] + np;
     Fw[i] = f;  // weights pass: chi[0] set per stage

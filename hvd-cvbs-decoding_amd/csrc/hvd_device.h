@@ -3,7 +3,8 @@
 // Data layout in HBM (one active plane per field, row-major h x w):
 //   s        float   composite (IRE)
 //   chi      float2  chroma phasor chi = V - iU, interleaved (re, im)
-//   g, d     float2  CG gradient / search direction (each ping-ponged)
+//   d        float2  CG search direction (ping-ponged; the gradient is never
+//                    stored: each pass recomputes it from chi)
 //   wts      float4  IRLS diffusivities (wx, wy, wcx, wcy) per pixel
 //   carrier  never stored: c(y, x) = rowph[y] * colph[x], with the PAL
 //            V-switch folded in as c -> -conj(c) on switched lines
@@ -18,10 +19,13 @@ namespace hvdk {
 constexpr int kTileW = 64;  // one wavefront per tile row (init kernels)
 constexpr int kTileH = 4;
 constexpr int kBlock = kTileW * kTileH;  // 256 threads
-// Output columns of a solver strip: 64 lanes less a halo lane on the left and
-// two on the right (the fused CG iteration needs the new gradient one column
-// past its last output, and that column's own right neighbour).
-constexpr int kStripCols = 61;
+// Output columns of a solver strip: 64 lanes less two halo lanes on the left
+// and three on the right.  The fused CG iteration recomputes the gradient of
+// chi (lanes 1 .. 62, each needs both neighbours), moves chi there, and forms
+// the new gradient on lanes 2 .. 61: the outputs' curvature terms need it one
+// column past the last output.
+constexpr int kStripCols = 59;
+constexpr int kStripHalo = 2;  // lane of the strip's first output column
 // Launch geometry of the solver kernels: strip prefix over the batch, and
 // the strips per field when every field has the same size (else 0).
 struct StripGrid {
@@ -30,8 +34,8 @@ struct StripGrid {
 };
 
 // Rows of carrier factors a solver strip of `rows` lines holds, one per lane
-// (the halo row above and the two below); at most 64.
-constexpr int carrier_span(int rows) { return rows + 3; }
+// (two halo rows above and three below); at most 64.
+constexpr int carrier_span(int rows) { return rows + 5; }
 constexpr int kMaxStripRows = 32;
 // FP64 partial sums per solver strip and CG launch (|g|^2, curv(g),
 // curv(g, d), Re<g, d>).
@@ -54,7 +58,6 @@ struct DevField {
   const float2* colph; // w: e^{i w0 (a0 + x)}
   const float2* carrier;  // explicit h*w carrier, or null (factored form)
   float2* chi[2];      // ping-pong chroma (chi[0] is the caller's buffer)
-  float2* g[2];        // ping-pong gradient
   float2* d[2];        // ping-pong search direction
   float4* wts;         // IRLS weights
   float* y;            // output luma

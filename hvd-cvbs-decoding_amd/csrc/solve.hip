@@ -23,12 +23,15 @@ namespace hvdk {
 namespace {
 
 // A wave (one 64-thread block) owns a strip of kStripCols output columns and
-// `rows` output lines of one field.  Lane l sits on column x = x0 + l - 1, so
-// lanes 0 and 63 are a one-column halo whose values reach their neighbours
-// through lane shuffles; each lane walks its rows top to bottom with the rows
-// above and below held in registers.  Every per-pixel value is formed by the
-// same expression as at the neighbouring pixel that also needs it, so a value
-// shuffled in from a neighbour lane is bit-identical to one recomputed there.
+// `rows` output lines of one field.  Lane l sits on column
+// x = x0 + l - kStripHalo, so lanes 0, 1 and 61 .. 63 are halo columns whose
+// values reach their neighbours through lane shifts; each lane walks its rows
+// top to bottom with the rows above and below held in registers.  Every
+// per-pixel value is formed by the same expression wherever it is needed (by
+// the neighbouring pixel, by the next strip, by the next pass), so a value
+// shifted in from a neighbour lane is bit-identical to one recomputed there;
+// the helpers that form them do not let the compiler fuse a multiply into an
+// add, which it might do differently at different call sites.
 // The field descriptor and CG state are copied into registers up front (they
 // are restrict-qualified, so the copies are scalar loads that no store in the
 // loop can force to be reloaded).
@@ -51,15 +54,15 @@ __device__ inline StripCtx strip_ctx(const DevField* F, const StripGrid& sg) {
   const DevField f = F[c.f];  // a private copy: stores cannot alias it
   c.lt = sg.spf > 0 ? t - c.f * sg.spf : t - sg.tp[c.f];
   const int lane = lane_id();
-  c.x = (c.lt % f.tiles_x) * kStripCols + lane - 1;
+  c.x = (c.lt % f.tiles_x) * kStripCols + lane - kStripHalo;
   c.y0 = (c.lt / f.tiles_x) * sg.rows;
   c.valid = c.x >= 0 && c.x < f.w;
-  c.out = lane >= 1 && lane <= kStripCols && c.x < f.w;
+  c.out = lane >= kStripHalo && lane < kStripHalo + kStripCols && c.x < f.w;
   return c;
 }
 
 // The carrier down this lane's column (ntsc_geometry.cpp:75-93).  The row
-// phasors (and V-switch signs) of the strip's rows y0 - 1 ... y0 + rows + 1
+// phasors (and V-switch signs) of the strip's rows y0 - 2 ... y0 + rows + 2
 // are loaded once, one per lane, and broadcast per row with a lane read; the
 // column phasor is loaded once per lane.  Every lane of the wave must have
 // run carrier_for before at() is used (the lane read does not look at
@@ -72,12 +75,17 @@ struct Carrier {
   const float2* plane;  // explicit carrier plane, or null
   // Valid for rows and columns inside the plane; elsewhere (halo rows and
   // lanes) the value is unspecified but the access stays in bounds.
-  __device__ inline float2 at(int y) const {
-    if (plane) return gld(plane, (size_t)min(max(y, 0), h - 1) * w + min(max(x, 0), w - 1));
+  __device__ inline float2 at(int y) const { return plane ? load(y) : factored(y); }
+  __device__ inline float2 load(int y) const {
+    return gld(plane, (size_t)min(max(y, 0), h - 1) * w + min(max(x, 0), w - 1));
+  }
+  __device__ inline float2 factored(int y) const {
+#pragma clang fp contract(off)
     const int l = y - ybase;  // uniform across the wave
     const float2 rp = make_float2(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(rowl.x), l)),
                                   __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rowl.y), l)));
-    float2 v = cmul(rp, col);
+    // rp * col written out here (not fused into adds at the call site)
+    float2 v = make_float2(rp.x * col.x - rp.y * col.y, rp.x * col.y + rp.y * col.x);
     if (__builtin_amdgcn_readlane(vsl, l) < 0) v.x = -v.x;
     return v;
   }
@@ -88,7 +96,7 @@ __device__ inline Carrier carrier_for(const DevField& f, const StripCtx& c) {
   k.x = c.x;
   k.w = f.w;
   k.h = f.h;
-  k.ybase = c.y0 - 1;
+  k.ybase = c.y0 - kStripHalo;
   k.plane = f.carrier;
   k.col = make_float2(0.f, 0.f);
   k.rowl = make_float2(0.f, 0.f);
@@ -157,6 +165,14 @@ __device__ inline RawRow raw_row(const DevField& f, const Carrier& k, const floa
   return r;
 }
 
+// S - Re[chi c] in one fixed operation order, so that every place forming a
+// row's luma from the same chi (the pass that moves chi, the next pass,
+// which recomputes its gradient, and the weights passes) gets the same bits.
+__device__ inline float luma_res(float s, float2 chi, float2 car) {
+#pragma clang fp contract(off)
+  return s - __fmaf_rn(chi.x, car.x, -(chi.y * car.y));
+}
+
 struct Sample {
   float yc;    // S - Re[chi c]
   float2 chi;  // chi (+ alpha d)
@@ -171,7 +187,7 @@ __device__ inline Sample make_sample(const DevField& f, const Carrier& k, const 
     const float2 c = with_d ? axpy(r.chi, alpha, r.d) : r.chi;
     s.chi = c;
     s.car = k.at(y);
-    s.yc = r.s - re_mul(c, s.car);
+    s.yc = luma_res(r.s, c, s.car);
   }
   return s;
 }
@@ -192,6 +208,7 @@ struct Flux {
 };
 
 __device__ inline Flux flux(float w_luma, float w_chroma, const Sample& a, const Sample& b) {
+#pragma clang fp contract(off)
   Flux f;
   f.y = w_luma * (b.yc - a.yc);
   f.c = make_float2(w_chroma * (b.chi.x - a.chi.x), w_chroma * (b.chi.y - a.chi.y));
@@ -202,6 +219,7 @@ __device__ inline Flux flux(float w_luma, float w_chroma, const Sample& a, const
 // upper (yu) and lower (y0) edges; c0 is the pixel's carrier.
 __device__ inline float2 grad_from_flux(const FieldCG& st, float2 c0, const Flux& xl, const Flux& x0, const Flux& yu,
                                        const Flux& y0) {
+#pragma clang fp contract(off)
   const float img = 2.f * st.lam_y * ((xl.y - x0.y) + (yu.y - y0.y));
   const float2 dxt = make_float2(xl.c.x - x0.c.x, xl.c.y - x0.c.y);
   const float2 dyt = make_float2(yu.c.x - y0.c.x, yu.c.y - y0.c.y);
@@ -290,53 +308,108 @@ __device__ inline CgDyn cg_advance(const FieldCG& s, const CgDyn& prev, const Cg
 struct IterRow {
   Sample p;
   float2 d;
-  float dY;  // -Re[d c]: the luma the direction moves
+  float dY;     // -Re[d c]: the luma the direction moves
+  float2 rhs;   // adjoint solve: the right-hand side at this pixel
 };
 
-// Loads of one row of this lane's column for the fused iteration, issued
+// Loads of one row of this lane's column, issued a line ahead of their use
 // without branches at clamped addresses, so rows and columns outside the
 // plane read finite values from its edge (the zero weights there cancel
-// them).  Upd: the previous direction is read; Adj: the adjoint solve (S = 0,
-// no composite plane).
+// them): chi, the composite (none in the adjoint solve, whose S is 0), the
+// previous direction (Upd), the weights unmasked, an explicit carrier plane's
+// value and the adjoint's right-hand side.
 struct IterRaw {
-  float2 chi, g, dp;
+  float2 chi, dp;
+  float2 car;
+  float2 rhs;
+  float4 w;
   float s;
 };
 
-template <bool Start, bool Upd, bool Adj>
-__device__ inline IterRaw iter_fetch(const DevField& f, int x, const float2* chi, const float2* g, const float2* dprev,
-                                     int y) {
+template <bool Start, bool Upd, bool Adj, bool FuseW>
+__device__ inline IterRaw iter_fetch(const DevField& f, const Carrier& k, int x, const float2* chi,
+                                     const float2* dprev, int y) {
   IterRaw r;
   const int yy = min(max(y, 0), f.h - 1), xx = min(max(x, 0), f.w - 1);
   const unsigned i = (unsigned)(yy * f.w + xx);
   r.chi = gld32(chi, i);
   r.s = Adj ? 0.f : gld32(f.s, i);
-  r.g = Start ? make_float2(0.f, 0.f) : gld32(g, i);
   r.dp = Upd ? gld32(dprev, i) : make_float2(0.f, 0.f);
+  r.car = k.plane ? gld32(k.plane, i) : make_float2(0.f, 0.f);
+  r.rhs = Adj ? gld32(f.rhs, i) : make_float2(0.f, 0.f);
+  r.w = FuseW ? make_float4(0.f, 0.f, 0.f, 0.f) : gld32(f.wts, i);
   return r;
 }
 
-template <bool Start, bool Upd>
-__device__ inline IterRow iter_make(const Carrier& k, const IterRaw& q, float alpha, float beta, int y) {
+__device__ inline float4 weights_mask(const DevField& f, float4 v, int y, bool valid) {
+  return row_in(f, y, valid) ? v : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// A row before this iteration's move: its sample (the input of the gradient
+// the previous pass formed), masked weights and what the move needs.
+struct OldRow {
+  Sample p;
+  float4 w;
+  float2 dp;
+  float2 rhs;
+  float s;
+};
+
+__device__ inline OldRow old_row(const DevField& f, const Carrier& k, const IterRaw& q, int y, bool valid) {
+  OldRow o;
+  o.p.car = k.plane ? q.car : k.factored(y);
+  o.p.chi = q.chi;
+  o.p.yc = luma_res(q.s, q.chi, o.p.car);
+  o.w = weights_mask(f, q.w, y, valid);
+  o.dp = q.dp;
+  o.rhs = q.rhs;
+  o.s = q.s;
+  return o;
+}
+
+// The row of the stage-start pass: chi' = chi, no direction.
+__device__ inline IterRow start_row(const Carrier& k, const IterRaw& q, int y) {
   IterRow r;
-  r.p.car = k.at(y);
-  float2 c = q.chi;
+  r.p.car = k.plane ? q.car : k.factored(y);
+  r.p.chi = q.chi;
+  r.p.yc = luma_res(q.s, q.chi, r.p.car);
   r.d = make_float2(0.f, 0.f);
   r.dY = 0.f;
-  if (!Start) {
-    r.d = Upd ? dir_update(q.g, beta, q.dp) : make_float2(-q.g.x, -q.g.y);
-    r.dY = -re_mul(r.d, r.p.car);
-    c = axpy(c, alpha, r.d);
-  }
-  r.p.chi = c;
-  r.p.yc = q.s - re_mul(c, r.p.car);
+  r.rhs = q.rhs;
   return r;
 }
 
-__device__ inline float4 weights_fetch(const DevField& f, int y, int x, bool valid) {
-  const int yy = min(max(y, 0), f.h - 1), xx = min(max(x, 0), f.w - 1);
-  const float4 v = gld32(f.wts, (unsigned)(yy * f.w + xx));
-  return row_in(f, y, valid) ? v : make_float4(0.f, 0.f, 0.f, 0.f);
+// The moved row from the row before the move and its gradient g.
+template <bool Upd>
+__device__ inline IterRow moved_row(const OldRow& o, float2 g, float alpha, float beta) {
+  IterRow r;
+  r.p.car = o.p.car;
+  r.d = Upd ? dir_update(g, beta, o.dp) : make_float2(-g.x, -g.y);
+  r.dY = -re_mul(r.d, r.p.car);
+  r.p.chi = axpy(o.p.chi, alpha, r.d);
+  r.p.yc = luma_res(o.s, r.p.chi, r.p.car);
+  r.rhs = o.rhs;
+  return r;
+}
+
+// The gradient at a row from its sample a (right neighbour by a lane shift),
+// the row below b, the flux through its upper edge and its weights; returns
+// the flux through its lower edge in fdn.  The same operations as the pass
+// that formed this gradient from the same samples (so the same bits).
+template <bool Adj>
+__device__ inline float2 row_grad(const FieldCG& s, const Sample& a, const float4& w, const Sample& b, float2 rhs,
+                                  const Flux& fup, Flux& fdn, bool calc) {
+  Sample ar;
+  ar.yc = shr(a.yc);
+  ar.chi = shr(a.chi);
+  const Flux fx = flux(w.x, w.z, a, ar);  // through the right edge
+  fdn = flux(w.y, w.w, a, b);              // through the lower edge
+  Flux fxl;
+  fxl.y = shl(fx.y);
+  fxl.c = shl(fx.c);
+  float2 g = grad_from_flux(s, a.car, fxl, fx, fup, fdn);
+  if (Adj) g = make_float2(g.x - rhs.x, g.y - rhs.y);
+  return calc ? g : make_float2(0.f, 0.f);
 }
 
 // One pixel's share of curv(a) and of the bilinear form curv(a, b)
@@ -369,6 +442,7 @@ __device__ inline void curv_terms(const FieldCG& s, float4 w, bool on, float2 a0
 // (its difference is 0 in hvd-core, so its term vanishes either way).
 __device__ inline float4 irls_weight(const FieldCG& s, const Sample& p0, float yr, float2 cr, const Sample& pd, bool hr,
                                      bool hd) {
+#pragma clang fp contract(off)
   const float e2 = s.eps * s.eps, ec2 = s.eps_c * s.eps_c;
   const float gx = hr ? yr - p0.yc : 0.f;
   const float gy = hd ? pd.yc - p0.yc : 0.f;
@@ -394,20 +468,26 @@ __device__ inline float4 irls_weight(const FieldCG& s, const Sample& p0, float y
 }
 
 // The fused CG iteration over one strip (variational.cpp:527-548, one
-// iteration per launch).  With d = d(it) formed on the fly and
-// chi' = chi + alpha d, it writes chi', d and g' = grad(chi') (:286-347, less
-// rhs in the adjoint solve) and the strip's FP64 partials of
+// iteration per launch).  It forms the gradient g of chi (the same bits the
+// previous pass formed and summed: no gradient plane is stored), the direction
+// d = d(it) = -g + beta d(it-1) and chi' = chi + alpha d, writes chi' and d,
+// and forms g' = grad(chi') (:286-347, less rhs in the adjoint solve) and the
+// strip's FP64 partials of
 //   |g'|^2,  curv(g'),  curv(g', d),  Re<g', d>,
 // from which the next reduction forms beta = |g'|^2 / |g|^2 and, without
 // another pass over the planes, the next direction's
 //   curv(-g' + beta d) = curv(g') - 2 beta curv(g', d) + beta^2 curv(d)
 //   Re<g', -g' + beta d> = -|g'|^2 + beta Re<g', d>.
-// The stage-start pass (Start) computes g = grad(chi) and the partials of |g|^2
-// and curv(g) (the first direction is -g).  Lanes 1..kStripCols + 1 form g';
-// lanes 1..kStripCols own the outputs and the partials (a pixel's curvature
-// terms need g' one column right and one row down, so the walk forms g' for
-// rows + 1 lines).  The loop body is branch-free apart from the stores: loads
-// are clamped and results selected.
+// The stage-start pass (Start) forms g = grad(chi) and the partials of |g|^2
+// and curv(g) (the first direction is -g).
+//
+// Lanes: g of chi on lanes 1 .. 62 (each needs both neighbours), so d and
+// chi' there; g' on lanes kStripHalo .. kStripHalo + kStripCols; the
+// kStripCols lanes from kStripHalo own the outputs and the partials (a
+// pixel's curvature terms need g' one column right).  Rows: the walk forms g'
+// for rows + 1 lines (the curvature terms need g' one row down), chi' from
+// the row above the strip and g of chi from two rows above.  Loads are
+// clamped and results selected; the walk has no exit between lines.
 //
 // FuseW (the forward's stage start): the IRLS weights of the stage are formed
 // here from chi (variational.cpp:406-433) instead of loaded, and stored for
@@ -416,37 +496,46 @@ template <bool Start, bool Upd, bool Adj, bool FuseW = false>
 __device__ inline void cg_strip(const DevField& f, const FieldCG& s, const CgDyn& q, const StripCtx& c, int rows,
                                 double* partials, int stage = 0) {
   static_assert(!FuseW || Start, "weights are formed by the stage-start pass only");
+  static_assert(!Start || !Upd, "the stage start has no previous direction");
   const Carrier k = carrier_for(f, c);
   const float2* chi = pick(f.chi, q.cur);
-  const float2* gsrc = pick(f.g, q.cur);
   const float2* dprev = pick(f.d, q.cur);
   float2* chi_out = pick(f.chi, q.cur ^ 1);
-  float2* g_out = pick(f.g, Start ? q.cur : q.cur ^ 1);
   float2* d_out = pick(f.d, q.cur ^ 1);
   const float alpha = q.alpha, beta = q.beta;
-  const bool calc = c.valid && lane_id() >= 1 && lane_id() <= kStripCols + 1;
-  auto fetch = [&](int y) { return iter_fetch<Start, Upd, Adj>(f, c.x, chi, gsrc, dprev, y); };
-  auto make = [&](const IterRaw& q, int y) { return iter_make<Start, Upd>(k, q, alpha, beta, y); };
+  const int lane = lane_id();
+  const bool calc = c.valid && lane >= kStripHalo && lane <= kStripHalo + kStripCols;  // g'
+  const bool calc0 = c.valid && lane >= 1 && lane <= 62;                               // g of chi
+  auto fetch = [&](int y) { return iter_fetch<Start, Upd, Adj, FuseW>(f, k, c.x, chi, dprev, y); };
+  auto old = [&](const IterRaw& r, int y) { return old_row(f, k, r, y, c.valid); };
   float2 gp = make_float2(0.f, 0.f);  // g' of the row above
   float gYp = 0.f;
-  Flux fyu;  // flux through the upper edge of the current line
+  Flux fyu;  // flux of chi' through the upper edge of the current line
+  Flux fdo;  // flux of chi through the lower edge of the row below the line
   double gg = 0.0, A = 0.0, B = 0.0, D = 0.0;
-  // One line of the walk (row y = y0 + j).  The rows above / at / below y and
-  // their weights live in three slots each that rotate between calls, so no
-  // registers are copied from line to line.  Returns false when the walk ends.
+  // One line of the walk (row y = y0 + j, inside the plane).  The rows above
+  // / at / below y and their weights live in three slots each that rotate
+  // between calls, so no registers are copied from line to line; so do the
+  // rows before the move (o1: row y + 1, o2: row y + 2, formed here) and the
+  // loads (pin: row y + 2 in the iteration, y + 1 in the stage start, issued
+  // by the line before; pout issued here for the next line): a line waits
+  // only for loads issued a whole line earlier.
   auto step = [&](int j, const IterRow& up, const IterRow& cur, IterRow& dn, const float4& wu, float4& wc,
-                  float4& wn) -> bool {
-    if (j > rows) return false;
+                  float4& wn, const IterRaw& pin, IterRaw& pout, const OldRow& o1, OldRow& o2) {
     const int y = c.y0 + j;
-    if (y >= f.h) {  // uniform: the plane ends inside the walk
-      const float2 gpr = shr(gp), dpr = shr(up.d);
-      const float gYpr = shr(gYp), dYpr = shr(up.dY);
-      curv_terms(s, wu, j >= 1 && c.out, gp, gpr, gp, gYp, gYpr, gYp, up.d, dpr, up.d, up.dY, dYpr, up.dY, !Start, A,
-                 B);
-      return false;
+    if (Start) {
+      pout = fetch(y + 2);
+      if (!FuseW) wn = weights_mask(f, pin.w, y + 1, c.valid);
+      dn = start_row(k, pin, y + 1);
+    } else {
+      pout = fetch(y + 3);
+      o2 = old(pin, y + 2);
+      Flux fd1;
+      const float2 g1 = row_grad<Adj>(s, o1.p, o1.w, o2.p, o1.rhs, fdo, fd1, calc0);
+      fdo = fd1;
+      wn = o1.w;
+      dn = moved_row<Upd>(o1, g1, alpha, beta);
     }
-    if (!FuseW) wn = weights_fetch(f, y + 1, c.x, c.valid);
-    dn = make(fetch(y + 1), y + 1);
     Sample pr;
     pr.yc = shr(cur.p.yc);
     pr.chi = shr(cur.p.chi);
@@ -466,11 +555,7 @@ __device__ inline void cg_strip(const DevField& f, const FieldCG& s, const CgDyn
     fxl.c = shl(fx.c);
     float2 gv = grad_from_flux(s, cur.p.car, fxl, fx, fyu, fy);
     fyu = fy;
-    if (Adj) {
-      const int xx = min(max(c.x, 0), f.w - 1);
-      const float2 b = gld32(f.rhs, (unsigned)(y * f.w + xx));
-      gv = make_float2(gv.x - b.x, gv.y - b.y);
-    }
+    if (Adj) gv = make_float2(gv.x - cur.rhs.x, gv.y - cur.rhs.y);
     gv = calc ? gv : make_float2(0.f, 0.f);
     const float gY = -re_mul(gv, cur.p.car);
     // curvature terms of the row above, whose down neighbour is this row
@@ -483,34 +568,71 @@ __device__ inline void cg_strip(const DevField& f, const FieldCG& s, const CgDyn
     const bool own = j < rows && c.out;
     gg += (double)(own ? __fmaf_rn(gv.x, gv.x, gv.y * gv.y) : 0.f);
     if (!Start) D += (double)(own ? __fmaf_rn(gv.x, cur.d.x, gv.y * cur.d.y) : 0.f);
-    if (own) {
+    if (!Start && own) {
       const unsigned i = (unsigned)(y * f.w + c.x);
-      gst32(g_out, i, gv);
-      if (!Start) {
-        gst32(chi_out, i, cur.p.chi);
-        gst32(d_out, i, cur.d);
-      }
+      gst32(chi_out, i, cur.p.chi);
+      gst32(d_out, i, cur.d);
     }
     gp = gv;
     gYp = gY;
-    return true;
   };
-  IterRow r0 = make(fetch(c.y0 - 1), c.y0 - 1), r1 = make(fetch(c.y0), c.y0), r2;
+  // Line j at the plane's last row + 1 (when the plane ends inside the walk):
+  // the curvature terms of the row above, whose down neighbour is outside.
+  auto plane_end = [&](int j, const IterRow& up, const float4& wu) {
+    const float2 gpr = shr(gp), dpr = shr(up.d);
+    const float gYpr = shr(gYp), dYpr = shr(up.dY);
+    curv_terms(s, wu, j >= 1 && c.out, gp, gpr, gp, gYp, gYpr, gYp, up.d, dpr, up.d, up.dY, dYpr, up.dY, !Start, A, B);
+  };
+  IterRow r0, r1, r2;
   float4 w0, w1, w2;
-  if (FuseW) {  // the weights of the line above the strip (rows outside the plane: 0)
-    const float yr = shr(r0.p.yc);
-    const float2 cr = shr(r0.p.chi);
-    w0 = c.valid && c.y0 >= 1 ? irls_weight(s, r0.p, yr, cr, r1.p, c.x + 1 < f.w, c.y0 < f.h)
-                              : make_float4(0.f, 0.f, 0.f, 0.f);
+  IterRaw p0, p1, p2;
+  OldRow oa, ob, oc;
+  if (Start) {
+    const IterRaw a = fetch(c.y0 - 1), b = fetch(c.y0);
+    p0 = fetch(c.y0 + 1);
+    r0 = start_row(k, a, c.y0 - 1);
+    r1 = start_row(k, b, c.y0);
+    if (FuseW) {  // the weights of the line above the strip (rows outside the plane: 0)
+      const float yr = shr(r0.p.yc);
+      const float2 cr = shr(r0.p.chi);
+      w0 = c.valid && c.y0 >= 1 ? irls_weight(s, r0.p, yr, cr, r1.p, c.x + 1 < f.w, c.y0 < f.h)
+                                : make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+      w0 = weights_mask(f, a.w, c.y0 - 1, c.valid);
+      w1 = weights_mask(f, b.w, c.y0, c.valid);
+    }
   } else {
-    w0 = weights_fetch(f, c.y0 - 1, c.x, c.valid);
-    w1 = weights_fetch(f, c.y0, c.x, c.valid);
+    // rows y0 - 2 .. y0 + 1 before the move; g of chi at y0 - 1 and y0
+    const IterRaw am2 = fetch(c.y0 - 2), am1 = fetch(c.y0 - 1), a0 = fetch(c.y0), a1 = fetch(c.y0 + 1);
+    p0 = fetch(c.y0 + 2);
+    const OldRow om2 = old(am2, c.y0 - 2), om1 = old(am1, c.y0 - 1), o0 = old(a0, c.y0);
+    oa = old(a1, c.y0 + 1);
+    fdo = flux(om2.w.y, om2.w.w, om2.p, om1.p);
+    Flux fd;
+    const float2 gm1 = row_grad<Adj>(s, om1.p, om1.w, o0.p, om1.rhs, fdo, fd, calc0);
+    const float2 g0 = row_grad<Adj>(s, o0.p, o0.w, oa.p, o0.rhs, fd, fdo, calc0);
+    r0 = moved_row<Upd>(om1, gm1, alpha, beta);
+    r1 = moved_row<Upd>(o0, g0, alpha, beta);
+    w0 = om1.w;
+    w1 = o0.w;
   }
   fyu = flux(w0.y, w0.w, r0.p, r1.p);
-  for (int j = 0;; j += 3) {
-    if (!step(j, r0, r1, r2, w0, w1, w2)) break;
-    if (!step(j + 1, r1, r2, r0, w1, w2, w0)) break;
-    if (!step(j + 2, r2, r0, r1, w2, w0, w1)) break;
+  // lines 0 .. jl lie inside the plane (jl <= rows: the walk forms g' one
+  // line past the strip's last output row)
+  const int jl = min(rows, f.h - 1 - c.y0);
+  int j = 0;
+  for (; j + 2 <= jl; j += 3) {
+    step(j, r0, r1, r2, w0, w1, w2, p0, p1, oa, ob);
+    step(j + 1, r1, r2, r0, w1, w2, w0, p1, p2, ob, oc);
+    step(j + 2, r2, r0, r1, w2, w0, w1, p2, p0, oc, oa);
+  }
+  const int rem = jl + 1 - j;  // 0, 1 or 2 lines left
+  if (rem >= 1) step(j, r0, r1, r2, w0, w1, w2, p0, p1, oa, ob);
+  if (rem >= 2) step(j + 1, r1, r2, r0, w1, w2, w0, p1, p2, ob, oc);
+  if (jl < rows) {
+    if (rem == 0) plane_end(j, r0, w0);
+    else if (rem == 1) plane_end(j + 1, r1, w1);
+    else plane_end(j + 2, r2, w2);
   }
   const int slot = f.tile0 + c.lt;
   publish(partials, slot, 0, kCgPartials, 0, gg);

@@ -161,7 +161,7 @@ int hvd_gpu_cg_timing(hvd_gpu_ctx* ctx, double* ms, long long* iterations, doubl
 /* ---- solver launch shape (no hvd-core counterpart) ----------------------- */
 
 /* How the IRLS/CG solve is launched: lines per solver strip (2..32; each
- * 64-lane wave walks a strip of 61 columns) and the most pixels per solve
+ * 64-lane wave walks a strip of 59 columns) and the most pixels per solve
  * group (consecutive fields solved together, sized so their planes stay in
  * the Infinity Cache).  0 selects the built-in choice for that member. */
 typedef struct {

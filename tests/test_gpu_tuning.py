@@ -1,8 +1,11 @@
 """GPU: the solver's launch shape (strip height, solve-group size) and its autotuner.
 
 The launch shape changes how the CG's FP64 sums are ordered and nothing
-else: results with the same strip height are bit-identical whatever the
-grouping, and any strip height stays within TOL_IRE of the FP64 oracle.
+else: every per-pixel value is formed by the same operations whichever strip
+and line forms it.  Results are bit-identical whatever the grouping, and
+across strip heights for these inputs (the sums' order reaches the float
+step sizes only when a sum lies within its last bits of a float rounding
+boundary); any strip height stays within TOL_IRE of the FP64 oracle.
 """
 import math
 
@@ -89,6 +92,20 @@ def test_strip_heights_agree(eng, orc64, rows):
         y_ref, chi_ref, _, _ = orc64.variational_refine(s, car, chi0, O.Cfg.make(**CFG))
         assert np.abs(y - y_ref).max() <= TOL_IRE
         assert np.abs((chi[..., 0] + 1j * chi[..., 1]) - chi_ref).max() <= TOL_IRE
+
+
+def test_strip_heights_bit_identical(eng):
+    # strips that start on different lines of the same plane: a line formed
+    # at the top of a strip or inside its walk gets the same bits
+    scenes = [scene(40, 150, 0)]
+    res = {}
+    for rows in (2, 3, 4, 8, 13):
+        eng.set_solver_tuning(rows, 0)
+        res[rows] = refine(eng, scenes)[0]
+    for rows in (3, 4, 8, 13):
+        same = np.array_equal(res[rows][0], res[2][0]) and np.array_equal(res[rows][1], res[2][1])
+        print(f"strip rows {rows} vs 2: {'same bits' if same else 'DIFF'}", flush=True)
+        assert same
 
 
 def test_grouping_is_bit_neutral(eng):
