@@ -178,28 +178,45 @@ __device__ __attribute__((always_inline)) float2* fft_lds(float2* a, float2* b, 
         dst[(ib * R + 2 * r) * ncols + c] = make_float2(x.x - y.x, x.y - y.y);
       }
       __syncthreads();
-      // pass 2: outputs q and R - q of group ib into src
-      for (int e = threadIdx.x; e < M * (h + 1) * ncols; e += blockDim.x) {
+      // pass 2: outputs q and R - q of group ib into src, kQ values of q per
+      // work item so that each u_r, v_r read from LDS feeds 4 kQ products
+      constexpr int kQ = 4;
+      const int nq = (h + kQ) / kQ;  // q = 0 .. h in blocks of kQ
+      for (int e = threadIdx.x; e < M * nq * ncols; e += blockDim.x) {
         const int c = e % ncols, g = e / ncols;
-        const int ib = g / (h + 1), q = g % (h + 1);
+        const int ib = g / nq, q0 = (g % nq) * kQ;
         const int k = ib % p, j = ib / p;
         const float2* gp = dst + (ib * R) * ncols + c;
         const float2 a0 = gp[0];
-        float ux = 0.f, uy = 0.f, vx = 0.f, vy = 0.f;
-        const int E = q * M;  // < n
-        int ex = E;
+        float ux[kQ], uy[kQ], vx[kQ], vy[kQ];
+        int E[kQ], ex[kQ];
+#pragma unroll
+        for (int t = 0; t < kQ; ++t) {
+          ux[t] = uy[t] = vx[t] = vy[t] = 0.f;
+          E[t] = min(q0 + t, h) * M;  // < n (q beyond h repeats h, not stored)
+          ex[t] = E[t];
+        }
         for (int r = 1; r <= h; ++r) {
           const float2 u = gp[(2 * r - 1) * ncols], v = gp[(2 * r) * ncols];
-          const float2 w = twiddle(ex);
-          ux = __fmaf_rn(u.x, w.x, ux);
-          uy = __fmaf_rn(u.y, w.x, uy);
-          vx = __fmaf_rn(v.x, w.y, vx);
-          vy = __fmaf_rn(v.y, w.y, vy);
-          ex += E;
-          if (ex >= n) ex -= n;
+#pragma unroll
+          for (int t = 0; t < kQ; ++t) {
+            const float2 w = twiddle(ex[t]);
+            ux[t] = __fmaf_rn(u.x, w.x, ux[t]);
+            uy[t] = __fmaf_rn(u.y, w.x, uy[t]);
+            vx[t] = __fmaf_rn(v.x, w.y, vx[t]);
+            vy[t] = __fmaf_rn(v.y, w.y, vy[t]);
+            ex[t] += E[t];
+            if (ex[t] >= n) ex[t] -= n;
+          }
         }
-        src[((j * R + q) * p + k) * ncols + c] = make_float2(a0.x + ux - vy, a0.y + uy + vx);
-        if (q > 0) src[((j * R + R - q) * p + k) * ncols + c] = make_float2(a0.x + ux + vy, a0.y + uy - vx);
+#pragma unroll
+        for (int t = 0; t < kQ; ++t) {
+          const int q = q0 + t;
+          if (q > h) break;
+          src[((j * R + q) * p + k) * ncols + c] = make_float2(a0.x + ux[t] - vy[t], a0.y + uy[t] + vx[t]);
+          if (q > 0)
+            src[((j * R + R - q) * p + k) * ncols + c] = make_float2(a0.x + ux[t] + vy[t], a0.y + uy[t] - vx[t]);
+        }
       }
       __syncthreads();
     } else if (R <= 8 && R != 6) {
