@@ -58,17 +58,28 @@ struct hvd_gpu_ctx {
   // copies of the per-field solver state, read by hvd_gpu_cg_timing().
   bool profiling = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> cg_events;
+  std::vector<bool> cg_events_owned;  // false: the events belong to a cached graph
   std::vector<long long> cg_iters;  // CG iterations launched per loop
   std::vector<std::pair<void*, int>> cg_states;  // pinned FieldCG copies, count
   // solver launch shape: fixed by hvd_gpu_set_solver_tuning, else measured per
   // field shape by hvd_gpu_autotune_solver, else the built-in policy
   SolverTuning fixed;
   std::map<std::array<int, 3>, std::map<int, SolverTuning>> tuned;  // (lines, width, pal) -> fields -> tuning
-  // HIP graphs of the forward solve's CG iterations (hvd_gpu_set_graphs),
-  // keyed by the launch arguments they bake in; dropped when the workspace
-  // they point into is reallocated
+  // HIP graphs of a forward solve group's launches (hvd_gpu_set_graphs),
+  // keyed by the launches they bake in (kernels, launch shapes, argument
+  // bytes); dropped when the workspace they point into is reallocated.  A
+  // graph built for profiling also holds event-record nodes around each
+  // stage's CG iterations, with its own events.
+  struct CachedGraph {
+    hipGraphExec_t exec = nullptr;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    std::vector<long long> iters;
+  };
   bool graphs = true;
-  std::map<std::array<uintptr_t, 10>, hipGraphExec_t> cg_graphs;
+  std::map<std::string, CachedGraph> cg_graphs;
+  // events of dropped graphs, destroyed with the profiling record that may
+  // still refer to them
+  std::vector<hipEvent_t> retired;
 };
 
 namespace {
@@ -141,9 +152,17 @@ struct Carve {
   }
 };
 
-// Destroys the cached CG graphs (their arguments point into the workspace).
+// Destroys the cached graphs (their arguments point into the workspace); the
+// caller has synchronised the device.  Their profiling events live on until
+// the profiling record is cleared.
 void drop_graphs(hvd_gpu_ctx* ctx) {
-  for (auto& kv : ctx->cg_graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto& kv : ctx->cg_graphs) {
+    (void)hipGraphExecDestroy(kv.second.exec);
+    for (auto& e : kv.second.events) {
+      ctx->retired.push_back(e.first);
+      ctx->retired.push_back(e.second);
+    }
+  }
   ctx->cg_graphs.clear();
 }
 
@@ -511,90 +530,165 @@ StripSetup strip_setup(std::vector<DevField>& F, int rows) {
   return S;
 }
 
-// The CG iterations of one forward stage as a HIP graph: n_inner k_cg_iter
-// launches and k_cg_finish, built once per argument set (every argument
-// points into the workspace or is a size) and then replayed.
-hipGraphExec_t cg_graph(hvd_gpu_ctx* ctx, const DevField* dF, FieldCG* dstate, const StripGrid& sg, double* partials,
-                        int nf, int n_inner, int stage) {
-  const std::array<uintptr_t, 10> key{(uintptr_t)dF,      (uintptr_t)dstate,  (uintptr_t)sg.tp,   (uintptr_t)partials,
-                                      (uintptr_t)nf,      (uintptr_t)sg.nb,   (uintptr_t)sg.rows, (uintptr_t)sg.spf,
-                                      (uintptr_t)n_inner, (uintptr_t)(stage + 1)};
-  const auto found = ctx->cg_graphs.find(key);
-  if (found != ctx->cg_graphs.end()) return found->second;
-  if (ctx->cg_graphs.size() >= 256) drop_graphs(ctx);  // bounded cache
-  hipGraph_t g = nullptr;
-  HIPCHK(ctx, hipGraphCreate(&g, 0));
-  struct Free {
-    hipGraph_t g;
-    ~Free() { (void)hipGraphDestroy(g); }
-  } guard{g};
-  StripGrid sgv = sg;
-  const double* cpart = partials;
-  int nin = n_inner, stg = stage;
-  hipGraphNode_t prev = nullptr;
-  for (int i = 0; i <= n_inner; ++i) {
-    int itv = i;
-    void* a_iter[] = {(void*)&dF, (void*)&dstate, (void*)&sgv, (void*)&partials, (void*)&itv};
-    void* a_fin[] = {(void*)&dstate, (void*)&sgv, (void*)&cpart, (void*)&nin, (void*)&stg};
-    hipKernelNodeParams kp{};
-    if (i < n_inner) {
-      kp.func = (void*)k_cg_iter;
-      kp.gridDim = dim3(sg.nb);
-      kp.kernelParams = a_iter;
-    } else {
-      kp.func = (void*)k_cg_finish;
-      kp.gridDim = dim3(nf);
-      kp.kernelParams = a_fin;
-    }
-    kp.blockDim = dim3(64);
-    kp.sharedMemBytes = 0;
-    kp.extra = nullptr;
-    hipGraphNode_t node = nullptr;
-    HIPCHK(ctx, hipGraphAddKernelNode(&node, g, prev ? &prev : nullptr, prev ? 1 : 0, &kp));
-    prev = node;
+// A recorded sequence of kernel launches (and, when profiling, event marks
+// around each CG loop): launched eagerly, or built into a HIP graph of
+// kernel and event-record nodes (no memset nodes) that is cached under the
+// bytes of the whole sequence -- kernels, launch shapes and argument values --
+// and replayed.  Every argument of the solver's launches points into the
+// workspace or is a value, so an equal sequence is an equal graph.
+struct LaunchList {
+  struct Item {
+    const void* func;  // null: an event mark
+    dim3 grid, block;
+    unsigned shmem;
+    std::vector<size_t> args;  // offsets of the argument copies in `bytes`
+    int mark;                  // event marks: 2 * loop (start) or 2 * loop + 1 (end)
+  };
+  std::vector<Item> items;
+  std::vector<char> bytes;
+  std::vector<long long> loop_iters;  // CG iterations inside each marked loop
+  std::string key;                    // the sequence as bytes
+
+  template <typename... A>
+  void kernel(const void* f, dim3 grid, dim3 block, unsigned shmem, A... a) {
+    Item it{f, grid, block, shmem, {}, -1};
+    append_key(&f, sizeof f);
+    const unsigned dims[7] = {grid.x, grid.y, grid.z, block.x, block.y, block.z, shmem};
+    append_key(dims, sizeof dims);
+    (put(it, a), ...);
+    items.push_back(std::move(it));
   }
-  hipGraphExec_t ex = nullptr;
-  HIPCHK(ctx, hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
-  ctx->cg_graphs[key] = ex;
-  return ex;
+  // marks the start (end = false) or end of CG loop `loop`
+  void mark(int loop, bool end) {
+    items.push_back(Item{nullptr, dim3(), dim3(), 0, {}, 2 * loop + (end ? 1 : 0)});
+    const int m = items.back().mark;
+    append_key(&m, sizeof m);
+  }
+  template <typename T>
+  void put(Item& it, const T& v) {
+    const size_t off = (bytes.size() + 15) & ~size_t(15);
+    bytes.resize(off + sizeof(T));
+    std::memcpy(bytes.data() + off, &v, sizeof(T));
+    it.args.push_back(off);
+    append_key(&v, sizeof(T));
+  }
+  void append_key(const void* p, size_t n) { key.append(static_cast<const char*>(p), n); }
+  std::vector<void*> argv(const Item& it) {
+    std::vector<void*> v;
+    for (size_t off : it.args) v.push_back(bytes.data() + off);
+    return v;
+  }
+};
+
+// Records one forward or adjoint CG loop (variational.cpp:519-549): the
+// stage-start pass, n_inner fused iterations and the stage's close, the
+// iterations and close marked as profiling loop `loop`.
+void record_cg_loop(LaunchList& L, const DevField* dF, FieldCG* dstate, const StripGrid& sg, double* partials,
+                    int nf, int n_inner, bool adjoint, int stage, int loop) {
+  L.kernel((const void*)k_cg_start, dim3(sg.nb), dim3(64), 0, dF, (const FieldCG*)dstate, sg, partials, (int)adjoint,
+           stage);
+  L.mark(loop, false);
+  for (int it = 0; it < n_inner; ++it)
+    L.kernel(adjoint ? (const void*)k_cg_iter_adj : (const void*)k_cg_iter, dim3(sg.nb), dim3(64), 0, dF, dstate, sg,
+             partials, it);
+  L.kernel((const void*)k_cg_finish, dim3(nf), dim3(64), 0, dstate, sg, (const double*)partials, n_inner, stage);
+  L.mark(loop, true);
+  L.loop_iters.push_back(n_inner);
 }
 
-// One stage's CG from the current chi: the stage-start gradient, then n_inner
-// fused iterations (one launch each; a launch first completes the previous
-// one's per-field reductions), then the last update; per-field exits
-// (variational.cpp:519-549).  adjoint: the fields describe an adjoint solve
-// (no composite, a right-hand side).  stage: index for the iteration record.
-// graph: replay the iterations as a cached HIP graph (forward solves whose
-// descriptors sit at a fixed place in the workspace).
-void cg_loop(hvd_gpu_ctx* ctx, const DevField* dF, FieldCG* dstate, const StripGrid& sg, double* partials, int nf,
-             int n_inner, bool adjoint, int stage, bool graph, hipStream_t st) {
-  hipLaunchKernelGGL(k_cg_start, dim3(sg.nb), dim3(64), 0, st, dF, (const FieldCG*)dstate, sg, partials,
-                     (int)adjoint, stage);
-  LAUNCH_CHECK(ctx);
-  std::pair<hipEvent_t, hipEvent_t> evs{nullptr, nullptr};
-  if (ctx->profiling) {
-    HIPCHK(ctx, hipEventCreate(&evs.first));
-    HIPCHK(ctx, hipEventCreate(&evs.second));
-    HIPCHK(ctx, hipEventRecord(evs.first, st));
-  }
-  if (graph && !adjoint && n_inner > 0) {
-    HIPCHK(ctx, hipGraphLaunch(cg_graph(ctx, dF, dstate, sg, partials, nf, n_inner, stage), st));
-  } else {
-    for (int it = 0; it < n_inner; ++it) {
-      if (adjoint)
-        hipLaunchKernelGGL(k_cg_iter_adj, dim3(sg.nb), dim3(64), 0, st, dF, dstate, sg, partials, it);
-      else
-        hipLaunchKernelGGL(k_cg_iter, dim3(sg.nb), dim3(64), 0, st, dF, dstate, sg, partials, it);
-      LAUNCH_CHECK(ctx);
+// Runs a recorded sequence on `st`: as a cached graph when `graph`, else
+// launch by launch.  When profiling, the marked loops' event pairs and
+// iteration counts go to the profiling record.
+void run_launches(hvd_gpu_ctx* ctx, LaunchList& L, bool graph, hipStream_t st) {
+  const bool prof = ctx->profiling;
+  const size_t nloops = L.loop_iters.size();
+  if (graph) {
+    const std::string key = (prof ? "P" : "N") + L.key;
+    auto found = ctx->cg_graphs.find(key);
+    if (found == ctx->cg_graphs.end()) {
+      if (ctx->cg_graphs.size() >= 64) {  // bounded cache
+        HIPCHK(ctx, hipDeviceSynchronize());
+        drop_graphs(ctx);
+      }
+      hvd_gpu_ctx::CachedGraph cg;
+      hipGraph_t g = nullptr;
+      HIPCHK(ctx, hipGraphCreate(&g, 0));
+      struct Free {
+        hipGraph_t g;
+        ~Free() { (void)hipGraphDestroy(g); }
+      } guard{g};
+      struct FreeEvents {
+        std::vector<std::pair<hipEvent_t, hipEvent_t>>* evs;
+        bool keep = false;
+        ~FreeEvents() {
+          if (keep) return;
+          for (auto& e : *evs) {
+            if (e.first) (void)hipEventDestroy(e.first);
+            if (e.second) (void)hipEventDestroy(e.second);
+          }
+        }
+      } evguard{&cg.events};
+      if (prof)
+        for (size_t i = 0; i < nloops; ++i) {
+          cg.events.emplace_back(nullptr, nullptr);
+          HIPCHK(ctx, hipEventCreate(&cg.events.back().first));
+          HIPCHK(ctx, hipEventCreate(&cg.events.back().second));
+        }
+      cg.iters = L.loop_iters;
+      hipGraphNode_t prev = nullptr;
+      for (const LaunchList::Item& it : L.items) {
+        hipGraphNode_t node = nullptr;
+        if (it.func) {
+          std::vector<void*> av = L.argv(it);
+          hipKernelNodeParams kp{};
+          kp.func = const_cast<void*>(it.func);
+          kp.gridDim = it.grid;
+          kp.blockDim = it.block;
+          kp.sharedMemBytes = it.shmem;
+          kp.kernelParams = av.data();
+          kp.extra = nullptr;
+          HIPCHK(ctx, hipGraphAddKernelNode(&node, g, prev ? &prev : nullptr, prev ? 1 : 0, &kp));
+        } else if (prof) {
+          const auto& e = cg.events[it.mark / 2];
+          HIPCHK(ctx, hipGraphAddEventRecordNode(&node, g, prev ? &prev : nullptr, prev ? 1 : 0,
+                                                 (it.mark & 1) ? e.second : e.first));
+        } else {
+          continue;
+        }
+        prev = node;
+      }
+      HIPCHK(ctx, hipGraphInstantiate(&cg.exec, g, nullptr, nullptr, 0));
+      evguard.keep = true;
+      found = ctx->cg_graphs.emplace(key, std::move(cg)).first;
     }
-    hipLaunchKernelGGL(k_cg_finish, dim3(nf), dim3(64), 0, st, dstate, sg, (const double*)partials, n_inner, stage);
-    LAUNCH_CHECK(ctx);
+    HIPCHK(ctx, hipGraphLaunch(found->second.exec, st));
+    if (prof)
+      for (size_t i = 0; i < nloops; ++i) {
+        // a cached graph's events hold its latest replay; every replay of
+        // one launch sequence does the same work
+        ctx->cg_events.push_back(found->second.events[i]);
+        ctx->cg_events_owned.push_back(false);
+        ctx->cg_iters.push_back(found->second.iters[i]);
+      }
+    return;
   }
-  if (ctx->profiling) {
-    HIPCHK(ctx, hipEventRecord(evs.second, st));
-    ctx->cg_events.push_back(evs);
-    ctx->cg_iters.push_back(n_inner);
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> evs(nloops, {nullptr, nullptr});
+  for (const LaunchList::Item& it : L.items) {
+    if (it.func) {
+      std::vector<void*> av = L.argv(it);
+      HIPCHK(ctx, hipLaunchKernel(it.func, it.grid, it.block, av.data(), it.shmem, st));
+    } else if (prof) {
+      hipEvent_t& e = (it.mark & 1) ? evs[it.mark / 2].second : evs[it.mark / 2].first;
+      HIPCHK(ctx, hipEventCreate(&e));
+      HIPCHK(ctx, hipEventRecord(e, st));
+    }
   }
+  if (prof)
+    for (size_t i = 0; i < nloops; ++i) {
+      ctx->cg_events.push_back(evs[i]);
+      ctx->cg_events_owned.push_back(true);
+      ctx->cg_iters.push_back(L.loop_iters[i]);
+    }
 }
 
 // Per-field CG state of a solve with these parameters (floors: hvd-core's).
@@ -620,15 +714,14 @@ constexpr int kSelStaticLds = (16 * 256 + 256 + 2) * 4;
 // ResolveChromaAniso's p98 selection for the fields of sample_prefix (host
 // copy; one block per field).  A field's samples are staged in LDS when the
 // largest field's set fits.
-void aniso_select(hvd_gpu_ctx* ctx, FieldCG* dstate, const int* dsp, const float* gx, const float* gy, float lambda_c,
-                  const std::vector<int>& sample_prefix, hipStream_t st) {
+void record_aniso_select(const hvd_gpu_ctx* ctx, LaunchList& L, FieldCG* dstate, const int* dsp, const float* gx,
+                         const float* gy, float lambda_c, const std::vector<int>& sample_prefix) {
   const int nf = (int)sample_prefix.size() - 1;
   int max_n = 0;
   for (int i = 0; i < nf; ++i) max_n = std::max(max_n, sample_prefix[i + 1] - sample_prefix[i]);
   const int lds_floats = (size_t)max_n * 4 + kSelStaticLds <= (size_t)ctx->max_dyn_lds ? max_n : 0;
-  hipLaunchKernelGGL(k_aniso_select, dim3(nf), dim3(kSelThreads), sizeof(float) * lds_floats, st, dstate, dsp, gx, gy,
-                     lambda_c, lds_floats);
-  LAUNCH_CHECK(ctx);
+  L.kernel((const void*)k_aniso_select, dim3(nf), dim3(kSelThreads), (unsigned)(sizeof(float) * lds_floats), dstate,
+           dsp, gx, gy, lambda_c, lds_floats);
 }
 
 // Bytes run_refine stages for n fields (its descriptor region).
@@ -696,19 +789,19 @@ void run_refine(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_par
   const int* dsp = (const int*)(base + oS);
   const StripGrid sg{dtp, nf, nb, rows, spf};
   const StripGrid sga{(const int*)(base + oA), nf, AX.nb, AX.rows, AX.spf};
-  hipLaunchKernelGGL(k_refine_begin, dim3((nf + 63) / 64), dim3(64), 0, st, dstate, (const FieldCG*)(base + oI), nf);
-  LAUNCH_CHECK(ctx);
+  // the group's launches, replayed as one graph when the descriptors sit in
+  // the fixed workspace region
+  LaunchList L;
+  L.kernel((const void*)k_refine_begin, dim3((nf + 63) / 64), dim3(64), 0, dstate, (const FieldCG*)(base + oI), nf);
   if (!(p.chroma_aniso > 0.0) && sample_prefix[nf] > 0) {
     const int blocks = std::min(4096, (sample_prefix[nf] + 255) / 256);
-    hipLaunchKernelGGL(k_aniso_samples, dim3(blocks), dim3(256), 0, st, dF, (const FieldCG*)dstate, dsp, nf, gx, gy);
-    LAUNCH_CHECK(ctx);
+    L.kernel((const void*)k_aniso_samples, dim3(blocks), dim3(256), 0, dF, (const FieldCG*)dstate, dsp, nf, gx, gy);
   }
-  aniso_select(ctx, dstate, dsp, gx, gy, (float)p.lambda_c, sample_prefix, st);
+  record_aniso_select(ctx, L, dstate, dsp, gx, gy, (float)p.lambda_c, sample_prefix);
   // each stage: weights + stage-start gradient (one launch), then the CG
-  for (int o = 0; o < outer; ++o)
-    cg_loop(ctx, dF, dstate, sg, partials, nf, n_inner, false, o, desc != nullptr && ctx->graphs, st);
-  hipLaunchKernelGGL(k_refine_finalize, dim3(sga.nb), dim3(64), 0, st, dF, (const FieldCG*)dstate, sga);
-  LAUNCH_CHECK(ctx);
+  for (int o = 0; o < outer; ++o) record_cg_loop(L, dF, dstate, sg, partials, nf, n_inner, false, o, o);
+  L.kernel((const void*)k_refine_finalize, dim3(sga.nb), dim3(64), 0, dF, (const FieldCG*)dstate, sga);
+  run_launches(ctx, L, desc != nullptr && ctx->graphs, st);
   arm(ctx, sl, st);
   if (ctx->profiling) {
     void* h = nullptr;
@@ -881,12 +974,16 @@ PrepDesc plane_tables(const hvd_gpu_plane& pl, float2* rowph, int8_t* vsw, float
 
 namespace {
 void clear_profile(hvd_gpu_ctx* ctx) {
-  for (auto& e : ctx->cg_events) {
-    (void)hipEventDestroy(e.first);
-    (void)hipEventDestroy(e.second);
-  }
+  for (size_t i = 0; i < ctx->cg_events.size(); ++i)
+    if (ctx->cg_events_owned[i]) {
+      (void)hipEventDestroy(ctx->cg_events[i].first);
+      (void)hipEventDestroy(ctx->cg_events[i].second);
+    }
   for (auto& p : ctx->cg_states) (void)hipHostFree(p.first);
   ctx->cg_events.clear();
+  ctx->cg_events_owned.clear();
+  for (hipEvent_t e : ctx->retired) (void)hipEventDestroy(e);
+  ctx->retired.clear();
   ctx->cg_iters.clear();
   ctx->cg_states.clear();
 }
@@ -1264,8 +1361,12 @@ void run_adjoint(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hvd
                        (const FieldCG*)dstate, (const int*)(base + oS), n, (float*)(ws + o_gx), (float*)(ws + o_gy));
     LAUNCH_CHECK(ctx);
   }
-  aniso_select(ctx, dstate, (const int*)(base + oS), (float*)(ws + o_gx), (float*)(ws + o_gy), (float)p.lambda_c, sp,
-               st);
+  {
+    LaunchList L;
+    record_aniso_select(ctx, L, dstate, (const int*)(base + oS), (float*)(ws + o_gx), (float*)(ws + o_gy),
+                        (float)p.lambda_c, sp);
+    run_launches(ctx, L, false, st);
+  }
   hipLaunchKernelGGL(k_zero_d, dim3((2 * n + 255) / 256), dim3(256), 0, st, glam, 2 * n);
   LAUNCH_CHECK(ctx);
   const dim3 pgrid(nbx, n);
@@ -1283,7 +1384,11 @@ void run_adjoint(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hvd
                          Fa[i].chi[1], np);
       LAUNCH_CHECK(ctx);
     }
-    cg_loop(ctx, dFa, dstate, sg, partials, n, adj_iters, true, -1, false, st);
+    {
+      LaunchList L;
+      record_cg_loop(L, dFa, dstate, sg, partials, n, adj_iters, true, -1, 0);
+      run_launches(ctx, L, false, st);
+    }
     hipLaunchKernelGGL(k_refine_finalize, dim3(SS.nb), dim3(64), 0, st, dFa, (const FieldCG*)dstate, sg);
     LAUNCH_CHECK(ctx);
     const AdjField* dA = (const AdjField*)(base + oA[k]);

@@ -188,12 +188,12 @@ int hvd_gpu_set_solver_tuning(hvd_gpu_ctx* ctx, const hvd_gpu_solver_tuning* t);
 int hvd_gpu_autotune_solver(hvd_gpu_ctx* ctx, int lines, int width, int is_pal, int nfields, int iterations,
                             hvd_gpu_solver_tuning* best, double* ns_per_pixel_iteration);
 
-/* HIP graphs (no hvd-core counterpart): with `enable` (the default) the CG
- * iterations of each forward IRLS stage -- one k_cg_iter launch per
- * iteration and the stage's closing launch -- are replayed from a HIP graph
- * built at first use for that launch shape and workspace, instead of being
- * launched one by one.  Results are identical either way.  Disabling drops
- * the cached graphs. */
+/* HIP graphs (no hvd-core counterpart): with `enable` (the default) the
+ * launches of each forward solve group -- every IRLS stage's start pass, one
+ * k_cg_iter launch per CG iteration and the stage's closing launch, and the
+ * output pass -- are replayed from one HIP graph built at first use for that
+ * launch sequence and workspace, instead of being launched one by one.
+ * Results are identical either way.  Disabling drops the cached graphs. */
 int hvd_gpu_set_graphs(hvd_gpu_ctx* ctx, int enable);
 
 /* ---- level conversion -------------------------------------------------- */
