@@ -393,23 +393,29 @@ __device__ inline IterRow moved_row(const OldRow& o, float2 g, float alpha, floa
 }
 
 // The gradient at a row from its sample a (right neighbour by a lane shift),
-// the row below b, the flux through its upper edge and its weights; returns
-// the flux through its lower edge in fdn.  The same operations as the pass
-// that formed this gradient from the same samples (so the same bits).
+// its weights and the fluxes through its upper and lower edges.  The same
+// operations as the pass that formed this gradient from the same samples (so
+// the same bits).
 template <bool Adj>
-__device__ inline float2 row_grad(const FieldCG& s, const Sample& a, const float4& w, const Sample& b, float2 rhs,
-                                  const Flux& fup, Flux& fdn, bool calc) {
+__device__ inline float2 row_grad_fluxes(const FieldCG& s, const Sample& a, const float4& w, float2 rhs,
+                                         const Flux& fup, const Flux& fdn, bool calc) {
   Sample ar;
   ar.yc = shr(a.yc);
   ar.chi = shr(a.chi);
   const Flux fx = flux(w.x, w.z, a, ar);  // through the right edge
-  fdn = flux(w.y, w.w, a, b);              // through the lower edge
   Flux fxl;
   fxl.y = shl(fx.y);
   fxl.c = shl(fx.c);
   float2 g = grad_from_flux(s, a.car, fxl, fx, fup, fdn);
   if (Adj) g = make_float2(g.x - rhs.x, g.y - rhs.y);
   return calc ? g : make_float2(0.f, 0.f);
+}
+// ... given the row below b: returns the flux through the lower edge in fdn
+template <bool Adj>
+__device__ inline float2 row_grad(const FieldCG& s, const Sample& a, const float4& w, const Sample& b, float2 rhs,
+                                  const Flux& fup, Flux& fdn, bool calc) {
+  fdn = flux(w.y, w.w, a, b);  // through the lower edge
+  return row_grad_fluxes<Adj>(s, a, w, rhs, fup, fdn, calc);
 }
 
 // One pixel's share of curv(a) and of the bilinear form curv(a, b)
@@ -643,6 +649,122 @@ __device__ inline void cg_strip(const DevField& f, const FieldCG& s, const CgDyn
   }
 }
 
+// The fused CG iteration of cg_strip walked bottom-up, for strips on odd
+// strip rows: two vertically adjacent strips then read the rows around
+// their shared edge at the same time (both at the start or both at the end
+// of their walks), so the second read is served by the L2.  Every per-pixel
+// value is formed by the same expressions as in the top-down walk (a vertical
+// flux always from the upper row's weights, upper sample first; a pixel's
+// curvature terms from its right and lower neighbours), so the results are
+// the same bits; only the order of the FP64 partial sums differs.
+//
+// The walk starts at T = min(y0 + rows, h) (g' there feeds the curvature of
+// the line above) and ends at y0; line y forms chi'(y - 1) from the gradient
+// of chi at y - 1, then g'(y), the curvature terms of line y (its lower
+// neighbour's g' came one line earlier) and line y's outputs.
+template <bool Upd, bool Adj>
+__device__ inline void cg_strip_up(const DevField& f, const FieldCG& s, const CgDyn& q, const StripCtx& c, int rows,
+                                   double* partials) {
+  const Carrier k = carrier_for(f, c);
+  const float2* chi = pick(f.chi, q.cur);
+  const float2* dprev = pick(f.d, q.cur);
+  float2* chi_out = pick(f.chi, q.cur ^ 1);
+  float2* d_out = pick(f.d, q.cur ^ 1);
+  const float alpha = q.alpha, beta = q.beta;
+  const int lane = lane_id();
+  const bool calc = c.valid && lane >= kStripHalo && lane <= kStripHalo + kStripCols;  // g'
+  const bool calc0 = c.valid && lane >= 1 && lane <= 62;                               // g of chi
+  auto fetch = [&](int y) { return iter_fetch<false, Upd, Adj, false>(f, k, c.x, chi, dprev, y); };
+  auto old = [&](const IterRaw& r, int y) { return old_row(f, k, r, y, c.valid); };
+  float2 gp = make_float2(0.f, 0.f);  // g' of the line below
+  float gYp = 0.f;
+  Flux fdn;   // flux of chi' through the lower edge of the current line
+  Flux fdo;   // flux of chi through the lower edge of the row above the line
+  double gg = 0.0, A = 0.0, B = 0.0, D = 0.0;
+  const int T = min(c.y0 + rows, f.h);
+  // Line j of the walk (row y = T - j).  Slots as in cg_strip: dn / up / cur
+  // rotate over the rows below / at / above the line (`above` is filled
+  // here), the weights with them, the rows before the move (o1: row y - 1,
+  // o2: row y - 2, formed here) and the loads (pin: row y - 2, pout: y - 3).
+  auto step = [&](int j, const IterRow& dn, const IterRow& cur, IterRow& above, const float4& wd, const float4& wc,
+                  float4& wa, const IterRaw& pin, IterRaw& pout, const OldRow& o1, OldRow& o2) {
+    const int y = T - j;
+    pout = fetch(y - 3);
+    o2 = old(pin, y - 2);
+    // gradient of chi at y - 1: its upper edge's flux from row y - 2
+    const Flux fup1 = flux(o2.w.y, o2.w.w, o2.p, o1.p);
+    const float2 g1 = row_grad_fluxes<Adj>(s, o1.p, o1.w, o1.rhs, fup1, fdo, calc0);
+    fdo = fup1;
+    wa = o1.w;
+    above = moved_row<Upd>(o1, g1, alpha, beta);
+    // g' at y: upper edge from chi'(y - 1) and the weights of row y - 1
+    Sample pr;
+    pr.yc = shr(cur.p.yc);
+    pr.chi = shr(cur.p.chi);
+    const Flux fx = flux(wc.x, wc.z, cur.p, pr);  // through the right edge
+    const Flux fu = flux(wa.y, wa.w, above.p, cur.p);  // through the upper edge
+    Flux fxl;
+    fxl.y = shl(fx.y);
+    fxl.c = shl(fx.c);
+    float2 gv = grad_from_flux(s, cur.p.car, fxl, fx, fu, fdn);
+    fdn = fu;
+    if (Adj) gv = make_float2(gv.x - cur.rhs.x, gv.y - cur.rhs.y);
+    gv = calc ? gv : make_float2(0.f, 0.f);
+    const float gY = -re_mul(gv, cur.p.car);
+    // curvature terms of line y, whose lower neighbour is the previous line
+    const bool own = j >= 1 && c.out;
+    const float2 gvr = shr(gv), dr = shr(cur.d);
+    const float gYr = shr(gY), dYr = shr(cur.dY);
+    curv_terms(s, wc, own, gv, gvr, gp, gY, gYr, gYp, cur.d, dr, dn.d, cur.dY, dYr, dn.dY, true, A, B);
+    gg += (double)(own ? __fmaf_rn(gv.x, gv.x, gv.y * gv.y) : 0.f);
+    D += (double)(own ? __fmaf_rn(gv.x, cur.d.x, gv.y * cur.d.y) : 0.f);
+    if (own) {
+      const unsigned i = (unsigned)(y * f.w + c.x);
+      gst32(chi_out, i, cur.p.chi);
+      gst32(d_out, i, cur.d);
+    }
+    gp = gv;
+    gYp = gY;
+  };
+  // rows T + 2 .. T - 1 before the move; g of chi at T + 1 and T
+  IterRow r0, r1, r2;
+  float4 w0, w1, w2;
+  IterRaw p0, p1, p2;
+  OldRow oa, ob, oc;
+  {
+    const IterRaw b2 = fetch(T + 2), b1 = fetch(T + 1), b0 = fetch(T), bm1 = fetch(T - 1);
+    p0 = fetch(T - 2);
+    const OldRow o2 = old(b2, T + 2), o1 = old(b1, T + 1), o0 = old(b0, T);
+    oa = old(bm1, T - 1);
+    const Flux fd1 = flux(o1.w.y, o1.w.w, o1.p, o2.p);  // lower edge of T + 1
+    const Flux fu1 = flux(o0.w.y, o0.w.w, o0.p, o1.p);  // upper edge of T + 1
+    const float2 g1 = row_grad_fluxes<Adj>(s, o1.p, o1.w, o1.rhs, fu1, fd1, calc0);
+    fdo = flux(oa.w.y, oa.w.w, oa.p, o0.p);  // upper edge of T
+    const float2 g0 = row_grad_fluxes<Adj>(s, o0.p, o0.w, o0.rhs, fdo, fu1, calc0);
+    r0 = moved_row<Upd>(o1, g1, alpha, beta);  // chi'(T + 1)
+    r1 = moved_row<Upd>(o0, g0, alpha, beta);  // chi'(T)
+    w0 = o1.w;
+    w1 = o0.w;
+  }
+  fdn = flux(w1.y, w1.w, r1.p, r0.p);  // chi' through the lower edge of T
+  // lines T .. y0
+  const int jl = T - c.y0;
+  int j = 0;
+  for (; j + 2 <= jl; j += 3) {
+    step(j, r0, r1, r2, w0, w1, w2, p0, p1, oa, ob);
+    step(j + 1, r1, r2, r0, w1, w2, w0, p1, p2, ob, oc);
+    step(j + 2, r2, r0, r1, w2, w0, w1, p2, p0, oc, oa);
+  }
+  const int rem = jl + 1 - j;  // 0, 1 or 2 lines left
+  if (rem >= 1) step(j, r0, r1, r2, w0, w1, w2, p0, p1, oa, ob);
+  if (rem >= 2) step(j + 1, r1, r2, r0, w1, w2, w0, p1, p2, ob, oc);
+  const int slot = f.tile0 + c.lt;
+  publish(partials, slot, 0, kCgPartials, 0, gg);
+  publish(partials, slot, 0, kCgPartials, 1, A);
+  publish(partials, slot, 0, kCgPartials, 2, B);
+  publish(partials, slot, 0, kCgPartials, 3, D);
+}
+
 }  // namespace
 
 // Reset per-field solver state before a refine (replaces any memset so the
@@ -841,8 +963,14 @@ __device__ inline void cg_iter(const DevField* __restrict__ F, FieldCG* __restri
   q.beta = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(q.beta)));
   if (q.done) return;
   double* out = partials + (it & 1) * slot;
-  if (it > 0) cg_strip<false, true, Adj>(f, s, q, c, sg.rows, out);
-  else cg_strip<false, false, Adj>(f, s, q, c, sg.rows, out);
+  // strips on odd strip rows walk bottom-up (uniform per wave)
+  if ((c.lt / f.tiles_x) & 1) {
+    if (it > 0) cg_strip_up<true, Adj>(f, s, q, c, sg.rows, out);
+    else cg_strip_up<false, Adj>(f, s, q, c, sg.rows, out);
+  } else {
+    if (it > 0) cg_strip<false, true, Adj>(f, s, q, c, sg.rows, out);
+    else cg_strip<false, false, Adj>(f, s, q, c, sg.rows, out);
+  }
 }
 
 }  // namespace
