@@ -203,11 +203,14 @@ struct Stage {
 };
 
 // Uploads the staged bytes through the next slot; returns the device base.
-char* upload(hvd_gpu_ctx* ctx, const Stage& stg, hipStream_t st, Slot** used) {
+// `scratch` more device bytes follow the staged ones, from
+// align256(stg.buf.size()), for the launches that read the slot (the slot is
+// reused only after they finish).
+char* upload(hvd_gpu_ctx* ctx, const Stage& stg, hipStream_t st, Slot** used, size_t scratch = 0) {
   Slot& sl = ctx->slots[ctx->next_slot];
   ctx->next_slot = (ctx->next_slot + 1) % kSlots;
   if (sl.armed) HIPCHK(ctx, hipEventSynchronize(sl.ev));
-  const size_t need = std::max<size_t>(stg.buf.size(), 1);
+  const size_t need = std::max<size_t>(align256(stg.buf.size()) + scratch, 1);
   if (need > sl.cap) {
     if (sl.host) HIPCHK(ctx, hipHostFree(sl.host));
     if (sl.dev) HIPCHK(ctx, hipFree(sl.dev));
@@ -408,12 +411,22 @@ BridgeFrameParams bridge_params(const hvd_frame_params& fp) {
 void run_lockin(hvd_gpu_ctx* ctx, const std::vector<LockinGeom>& G, const std::vector<const float*>& fields,
                 const std::vector<float*>& theta, const std::vector<int8_t*>& vsw, hipStream_t st) {
   const int n = (int)G.size();
+  std::vector<int> lp(n + 1, 0);  // line prefix over the batch
+  for (int i = 0; i < n; ++i) lp[i + 1] = lp[i] + G[i].fh;
   Stage stg;
-  const size_t oG = stg.put(G), oF = stg.put(fields), oT = stg.put(theta), oV = stg.put(vsw);
+  const size_t oG = stg.put(G), oF = stg.put(fields), oT = stg.put(theta), oV = stg.put(vsw), oL = stg.put(lp);
   Slot* sl;
-  char* base = upload(ctx, stg, st, &sl);
-  hipLaunchKernelGGL(k_lockin, dim3(n), dim3(256), 0, st, (const LockinGeom*)(base + oG),
-                     (const float* const*)(base + oF), (float* const*)(base + oT), (int8_t* const*)(base + oV));
+  // the per-line demodulation (zr, zi, |z|) in the slot's scratch tail
+  char* base = upload(ctx, stg, st, &sl, sizeof(double) * 3 * (size_t)std::max(lp[n], 1));
+  double* z = (double*)(base + align256(stg.buf.size()));
+  const int* dlp = (const int*)(base + oL);
+  if (lp[n] > 0) {
+    hipLaunchKernelGGL(k_lockin_demod, dim3((lp[n] + 3) / 4), dim3(256), 0, st, (const LockinGeom*)(base + oG),
+                       (const float* const*)(base + oF), dlp, n, z);
+    LAUNCH_CHECK(ctx);
+  }
+  hipLaunchKernelGGL(k_lockin, dim3(n), dim3(256), 0, st, (const LockinGeom*)(base + oG), (const double*)z, dlp,
+                     (float* const*)(base + oT), (int8_t* const*)(base + oV));
   LAUNCH_CHECK(ctx);
   arm(ctx, sl, st);
 }

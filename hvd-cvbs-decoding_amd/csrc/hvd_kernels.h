@@ -106,7 +106,10 @@ struct BridgeOut {
 };
 
 // lockin.hip
-__global__ void k_lockin(const LockinGeom* G, const float* const* fields, float* const* theta, int8_t* const* vswitch);
+__global__ void k_lockin_demod(const LockinGeom* G, const float* const* fields, const int* line_prefix, int nf,
+                               double* z);
+__global__ void k_lockin(const LockinGeom* G, const double* z, const int* line_prefix, float* const* theta,
+                         int8_t* const* vswitch);
 __global__ void k_burst_amp(const LockinGeom* G, const float* const* fields, float* out);
 __global__ void k_u16_to_ire(const uint16_t* raw, size_t n, float blanking, float cpi, float* out);
 struct IreDesc {

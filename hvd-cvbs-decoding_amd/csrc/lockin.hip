@@ -78,10 +78,44 @@ __device__ void tridiag_smooth(const double* d, const double* a, int n, double l
 
 }  // namespace
 
-// Burst lock-in and phase trajectory for a batch of fields.  PAL-family
-// fields (g.vswitch) also resolve the per-line V-switch parity.
-__global__ void __launch_bounds__(256) k_lockin(const LockinGeom* G, const float* const* fields, float* const* theta,
-                                                int8_t* const* vswitch) {
+// (1) of the lock-in for every line of the batch at once, one wavefront per
+// line: z[y] = mean_x[(s - mean s) e^{-i w0 x}] (lockin.cpp:35-62) and |z|,
+// into z (zr, zi, amp per line; lines numbered by line_prefix).
+__global__ void __launch_bounds__(256) k_lockin_demod(const LockinGeom* G, const float* const* fields,
+                                                      const int* line_prefix, int nf, double* z) {
+  const int lane = threadIdx.x & 63;
+  const int line = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  if (line >= line_prefix[nf]) return;  // uniform per wave
+  const int fi = field_of_tile(line_prefix, nf, line);
+  const LockinGeom& g = G[fi];
+  const int y = line - line_prefix[fi];
+  const int x0 = g.x0, n = g.x1 - g.x0 > 0 ? g.x1 - g.x0 : 0;
+  const float* row = fields[fi] + (size_t)y * g.fw;
+  double m = 0.0;
+  for (int k = lane; k < n; k += 64) m += (double)row[x0 + k];
+  m = n ? wave_sum(m) / (double)n : 0.0;
+  double ar = 0.0, ai = 0.0;
+  for (int k = lane; k < n; k += 64) {
+    const double v = (double)row[x0 + k] - m;
+    const double2 lo = lo_at(g, x0 + k);
+    ar += v * lo.x;
+    ai += v * lo.y;
+  }
+  ar = wave_sum(ar);
+  ai = wave_sum(ai);
+  if (lane == 0) {
+    const double zr = n ? ar / (double)n : 0.0, zi = n ? ai / (double)n : 0.0;
+    z[3 * (size_t)line] = zr;
+    z[3 * (size_t)line + 1] = zi;
+    z[3 * (size_t)line + 2] = hypot(zr, zi);
+  }
+}
+
+// Burst lock-in and phase trajectory for a batch of fields, from the
+// per-line demodulation of k_lockin_demod.  PAL-family fields (g.vswitch)
+// also resolve the per-line V-switch parity.
+__global__ void __launch_bounds__(256) k_lockin(const LockinGeom* G, const double* z, const int* line_prefix,
+                                                float* const* theta, int8_t* const* vswitch) {
   __shared__ double zr[kMaxLines], zi[kMaxLines], amp[kMaxLines];
   __shared__ double th[kMaxLines], model[kMaxLines], dv[kMaxLines], av[kMaxLines], xs[kMaxLines];
   __shared__ double t1[kMaxLines], t2[kMaxLines];
@@ -91,31 +125,14 @@ __global__ void __launch_bounds__(256) k_lockin(const LockinGeom* G, const float
   __shared__ int s_first, s_ng, s_hyp;
   const int fi = blockIdx.x;
   const LockinGeom& g = G[fi];
-  const float* field = fields[fi];
-  const int h = g.fh, x0 = g.x0, x1 = g.x1;
-  const int n = x1 - x0 > 0 ? x1 - x0 : 0;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int h = g.fh;
 
-  // (1) z[y] = mean_x[(s - mean s) e^{-i w0 x}] (lockin.cpp:35-62)
-  for (int y = wv; y < h; y += blockDim.x / 64) {
-    const float* row = field + (size_t)y * g.fw;
-    double m = 0.0;
-    for (int k = lane; k < n; k += 64) m += (double)row[x0 + k];
-    m = n ? wave_sum(m) / (double)n : 0.0;
-    double ar = 0.0, ai = 0.0;
-    for (int k = lane; k < n; k += 64) {
-      const double v = (double)row[x0 + k] - m;
-      const double2 lo = lo_at(g, x0 + k);
-      ar += v * lo.x;
-      ai += v * lo.y;
-    }
-    ar = wave_sum(ar);
-    ai = wave_sum(ai);
-    if (lane == 0) {
-      zr[y] = n ? ar / (double)n : 0.0;
-      zi[y] = n ? ai / (double)n : 0.0;
-      amp[y] = hypot(zr[y], zi[y]);
-    }
+  // (1) z[y] and |z[y]| from k_lockin_demod
+  const double* zf = z + 3 * (size_t)line_prefix[fi];
+  for (int y = threadIdx.x; y < h; y += blockDim.x) {
+    zr[y] = zf[3 * y];
+    zi[y] = zf[3 * y + 1];
+    amp[y] = zf[3 * y + 2];
   }
   __syncthreads();
   // (2) good lines: amp > 0.2 max (lockin.cpp:121-130)
