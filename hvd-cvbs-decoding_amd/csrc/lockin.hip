@@ -58,22 +58,33 @@ __device__ double select_kth(const double* v, int n, int k, double* out) {
 }
 
 // Thomas solve of (diag(a) + lam L) x = a d with Neumann ends (lockin.cpp:65-99),
-// FP64, one lane.
-__device__ void tridiag_smooth(const double* d, const double* a, int n, double lam, double* x, double* c,
-                               double* dd) {
+// FP64, one lane.  The recurrences carry their previous values in registers
+// (not re-read from LDS), so each row waits only on its own arithmetic.
+__device__ void tridiag_smooth(const double* __restrict__ d, const double* __restrict__ a, int n, double lam,
+                               double* __restrict__ x, double* __restrict__ c, double* __restrict__ dd) {
   if (n == 0) return;
   if (n == 1) { x[0] = a[0] > 0.0 ? d[0] : 0.0; return; }
   const double off = -lam;
   auto diag = [&](int i) { return a[i] + 2.0 * lam - ((i == 0 || i == n - 1) ? lam : 0.0); };
-  c[0] = off / diag(0);
-  dd[0] = a[0] * d[0] / diag(0);
+  double cp = off / diag(0);
+  double dp = a[0] * d[0] / diag(0);
+  c[0] = cp;
+  dd[0] = dp;
   for (int i = 1; i < n; ++i) {
-    const double m = diag(i) - off * c[i - 1];
-    if (i < n - 1) c[i] = off / m;
-    dd[i] = (a[i] * d[i] - off * dd[i - 1]) / m;
+    const double m = diag(i) - off * cp;
+    dp = (a[i] * d[i] - off * dp) / m;
+    dd[i] = dp;
+    if (i < n - 1) {
+      cp = off / m;
+      c[i] = cp;
+    }
   }
-  x[n - 1] = dd[n - 1];
-  for (int i = n - 2; i >= 0; --i) x[i] = dd[i] - c[i] * x[i + 1];
+  double xn = dp;
+  x[n - 1] = xn;
+  for (int i = n - 2; i >= 0; --i) {
+    xn = dd[i] - c[i] * xn;
+    x[i] = xn;
+  }
 }
 
 }  // namespace
