@@ -213,7 +213,7 @@ class Engine:
 
     # ---- solver launch shape ------------------------------------------
     def set_graphs(self, on: bool):
-        """Replay each forward IRLS stage's CG iterations from a cached HIP graph (default on)."""
+        """Replay each forward solve group's launches from one cached HIP graph (default on)."""
         self._check(self.lib.hvd_gpu_set_graphs(self.ctx, int(on)), "set_graphs")
 
     def set_solver_tuning(self, strip_rows: int = 0, group_pixels: int = 0):

@@ -1,9 +1,10 @@
-"""GPU: the forward solve's CG iterations replayed from cached HIP graphs.
+"""GPU: each forward solve group's launches replayed from one cached HIP graph.
 
 A graph holds the same kernels with the same arguments as the launch-by-launch
 path, so every replay must give the eager result bit for bit -- checked on
 several replays in a row, with other eager work and synchronisations between
-them, and with other batch shapes (other graphs) interleaved.
+them, with other batch shapes (other graphs) interleaved, and with the
+profiling event nodes in the graph.
 """
 import numpy as np
 import pytest
@@ -66,3 +67,22 @@ def test_graph_with_early_exits(eng):
     eng.set_graphs(True)
     for _ in range(2):
         assert same(decode(eng, raws, [synth.NTSC] * 4, p), want)
+
+
+def test_graph_with_profiling_events(eng):
+    # the profiling graph adds event-record nodes around each stage's CG loop:
+    # same results, and the timing record counts every loop of every replay
+    p = gpu.SolverParams.make(cg_iterations=40, irls_outer=4, fast=0, cg_tol=1e-12)
+    raws = [synth.ntsc_field(95 + k)[0] for k in range(3)]
+    eng.set_graphs(False)
+    want = decode(eng, raws, [synth.NTSC] * 3, p)
+    eng.set_graphs(True)
+    eng.set_solver_tuning(8, 0)  # one solve group
+    eng.set_profiling(True)
+    for _ in range(3):
+        assert same(decode(eng, raws, [synth.NTSC] * 3, p), want)
+    ms, iters, pxit = eng.cg_timing()
+    eng.set_profiling(False)
+    print(f"profiled replays: {ms:.3f} ms over {iters} group-iterations", flush=True)
+    assert iters == 3 * 4 * 10  # calls x stages x iterations per stage
+    assert ms > 0.0 and 0 < pxit <= 3 * 3 * 4 * 10 * 242 * 760  # fields may stop early
