@@ -686,8 +686,9 @@ __device__ inline void cg_strip_up(const DevField& f, const FieldCG& s, const Cg
   // rotate over the rows below / at / above the line (`above` is filled
   // here), the weights with them, the rows before the move (o1: row y - 1,
   // o2: row y - 2, formed here) and the loads (pin: row y - 2, pout: y - 3).
-  auto step = [&](int j, const IterRow& dn, const IterRow& cur, IterRow& above, const float4& wd, const float4& wc,
-                  float4& wa, const IterRaw& pin, IterRaw& pout, const OldRow& o1, OldRow& o2) {
+  // (the weights of the line below ride along in the rotation unused)
+  auto step = [&](int j, const IterRow& dn, const IterRow& cur, IterRow& above, const float4& /*wd*/,
+                  const float4& wc, float4& wa, const IterRaw& pin, IterRaw& pout, const OldRow& o1, OldRow& o2) {
     const int y = T - j;
     pout = fetch(y - 3);
     o2 = old(pin, y - 2);
