@@ -741,12 +741,27 @@ typedef struct {
   const C *car;
   R *wx, *wy, *wcx, *wcy, *wd1, *wd2;
   R mu_h, mu_v, mu_d, lam_y;
+  /* temporal neighbours (3-D; nn = 0: the 2-D solve): composite S_w, the
+   * fixed delta dc = carrier - c_w, confidence and the stage's robust
+   * weight wt of each (variational.cpp:255-281, :452-461) */
+  int nn;
+  const R *const *sw;
+  C **dc;
+  const R *const *conf;
+  R **wt;
+  R nu, et2;
+  R *rt;
   /* scratch */
   R *yc, *t1, *t2, *img;
   C *c1, *c2, *c3, *cp;
 } solver;
 
-/* :286-347 grad (2-D terms only; no temporal neighbours). */
+/* :270-281 temporal residual (S_w - S) + Re[c dc] of neighbour j into sv->rt. */
+static void sv_temporal_residual(solver *sv, const C *c, int j) {
+  for (size_t i = 0; i < sv->n; ++i) sv->rt[i] = (sv->sw[j][i] - sv->s[i]) + re_prod(c[i], sv->dc[j][i]);
+}
+
+/* :286-347 grad: the 2-D terms and, per neighbour, 2 nu wt rt conj(dc). */
 static void sv_grad(solver *sv, const C *c, C *out) {
   const int h = sv->h, w = sv->w;
   const size_t n = sv->n;
@@ -786,6 +801,14 @@ static void sv_grad(solver *sv, const C *c, C *out) {
     out[i].re = -sv->img[i] * sv->car[i].re + sv->cp[i].re;
     out[i].im = sv->img[i] * sv->car[i].im + sv->cp[i].im;
   }
+  for (int j = 0; j < sv->nn; ++j) {
+    sv_temporal_residual(sv, c, j);
+    for (size_t i = 0; i < n; ++i) {
+      const R k = 2 * sv->nu * sv->wt[j][i] * sv->rt[i];
+      out[i].re += k * sv->dc[j][i].re;
+      out[i].im -= k * sv->dc[j][i].im;
+    }
+  }
 }
 
 /* :352-399 curv: the quadratic form of the fixed-weight objective along d. */
@@ -813,6 +836,14 @@ static double sv_curv(solver *sv, const C *d) {
       ad += (double)sv->wd2[i] * cabs2(sv->c2[i]);
     }
     acc += sv->mu_d * ad;
+  }
+  for (int j = 0; j < sv->nn; ++j) {  /* :385-397 nu sum wt Re[d dc]^2 */
+    double at = 0;
+    for (size_t i = 0; i < n; ++i) {
+      const R re = re_prod(d[i], sv->dc[j][i]);
+      at += (double)sv->wt[j][i] * re * re;
+    }
+    acc += (double)sv->nu * at;
   }
   return acc;
 }
@@ -856,6 +887,30 @@ static void sv_weights(solver *sv, const C *chi, R eps, R eps_c, R k) {
       sv->wd2[i] = (R)1 / (R)sqrt((double)(1 + a2 * a2));
     }
   }
+  if (sv->nn > 0 && sv->nu > 0) {
+    /* :452-461 robust temporal weights wt = conf et2 / (rt^2 + et2) */
+    for (int j = 0; j < sv->nn; ++j) {
+      sv_temporal_residual(sv, chi, j);
+      for (size_t i = 0; i < n; ++i)
+        sv->wt[j][i] = sv->conf[j][i] * sv->et2 / (sv->rt[i] * sv->rt[i] + sv->et2);
+    }
+    /* :463-516 prior renormalisation: the 2-D diffusivities times
+     * 1 + nu sum_j wt_j |dc_j|^2 */
+    for (size_t i = 0; i < n; ++i) sv->yc[i] = 1;
+    for (int j = 0; j < sv->nn; ++j)
+      for (size_t i = 0; i < n; ++i) sv->yc[i] += sv->nu * sv->wt[j][i] * cabs2(sv->dc[j][i]);
+    for (size_t i = 0; i < n; ++i) {
+      const R ki = sv->yc[i];
+      sv->wx[i] *= ki;
+      sv->wy[i] *= ki;
+      sv->wcx[i] *= ki;
+      sv->wcy[i] *= ki;
+      if (sv->mu_d > 0) {
+        sv->wd1[i] *= ki;
+        sv->wd2[i] *= ki;
+      }
+    }
+  }
 }
 
 /* :198-556 VariationalRefine, 2-D path (neighbors empty).  chi_r is chi0 on
@@ -869,12 +924,36 @@ static void sv_weights(solver *sv, const C *chi, R eps, R eps_c, R k) {
  * absolute CG exits (curv <= hh_floor, |g|^2 < gg_floor N); hvd-core's are
  * 1e-12 and 1e-10.  Tests that need each stage solved to round-off pass
  * smaller floors. */
-R OR_FN(variational_refine_ex)(const R *s, const R *carrier_r, int h, int w, const or_cfg *cfg,
-                               R *chi_r, R *y, int *stage_iters, R *stages, double hh_floor, double gg_floor) {
+static R refine_impl(const R *s, const R *carrier_r, int h, int w, const or_cfg *cfg, R *chi_r, R *y,
+                     int *stage_iters, R *stages, double hh_floor, double gg_floor, int nn, const R *sw,
+                     const R *cw, const R *conf, R nu, R eps_t) {
   solver sv;
   memset(&sv, 0, sizeof sv);
   sv.h = h; sv.w = w; sv.n = (size_t)h * w; sv.s = s; sv.car = (const C *)carrier_r;
   const size_t n = sv.n;
+  /* neighbours: dc = carrier - c_w, fixed for the solve (:255-268); nu is
+   * the temporal strength when neighbours are given (:224) */
+  const R *swp[16];
+  const R *confp[16];
+  C *dcp[16];
+  R *wtp[16];
+  if (nn > 16) nn = 16;
+  sv.nn = nn;
+  sv.nu = nn > 0 ? nu : 0;
+  sv.et2 = eps_t * eps_t;
+  for (int j = 0; j < nn; ++j) {
+    swp[j] = sw + (size_t)j * n;
+    confp[j] = conf + (size_t)j * n;
+    dcp[j] = (C *)xmalloc(sizeof(C) * n);
+    wtp[j] = (R *)xmalloc(sizeof(R) * n); /* zero until the first weights pass (nu > 0) */
+    const C *cwj = (const C *)cw + (size_t)j * n;
+    for (size_t i = 0; i < n; ++i) {
+      dcp[j][i].re = sv.car[i].re - cwj[i].re;
+      dcp[j][i].im = sv.car[i].im - cwj[i].im;
+    }
+  }
+  sv.sw = swp; sv.conf = confp; sv.dc = dcp; sv.wt = wtp;
+  sv.rt = (R *)xmalloc(sizeof(R) * n);
   C *chi = (C *)chi_r;
   const R eps = (R)cfg->charbonnier_eps, eps_c = (R)cfg->chroma_eps, k = (R)cfg->structure_coupling;
   const R lambda_c = (R)cfg->lambda_c;
@@ -933,12 +1012,31 @@ R OR_FN(variational_refine_ex)(const R *s, const R *carrier_r, int h, int w, con
     for (size_t i = 0; i < n; ++i) y[i] = s[i] - re_prod(chi[i], sv.car[i]);
   for (int i = 0; i < 10; ++i) free(planes[i]);
   for (int i = 0; i < 7; ++i) free(cpl[i]);
+  for (int j = 0; j < nn; ++j) { free(dcp[j]); free(wtp[j]); }
+  free(sv.rt);
   return aniso;
+}
+
+R OR_FN(variational_refine_ex)(const R *s, const R *carrier_r, int h, int w, const or_cfg *cfg,
+                               R *chi_r, R *y, int *stage_iters, R *stages, double hh_floor, double gg_floor) {
+  return refine_impl(s, carrier_r, h, w, cfg, chi_r, y, stage_iters, stages, hh_floor, gg_floor, 0, NULL, NULL,
+                     NULL, 0, 0);
 }
 
 R OR_FN(variational_refine)(const R *s, const R *carrier_r, int h, int w, const or_cfg *cfg,
                             R *chi_r, R *y, int *stage_iters) {
   return OR_FN(variational_refine_ex)(s, carrier_r, h, w, cfg, chi_r, y, stage_iters, NULL, 1e-12, 1e-10);
+}
+
+/* :198-556 VariationalRefine with nn (<= 16) motion-compensated temporal
+ * neighbours (NeighborTerm, variational.h:54-58): sw, conf nn planes of
+ * h*w, cw nn complex planes; nu = temporal_strength, eps_t = temporal_eps
+ * (> 0).  The 2-D solve's other arguments as variational_refine. */
+R OR_FN(variational_refine_nbr)(const R *s, const R *carrier_r, int h, int w, const or_cfg *cfg, int nn,
+                                const R *sw, const R *cw, const R *conf, R nu, R eps_t, R *chi_r, R *y,
+                                int *stage_iters) {
+  return refine_impl(s, carrier_r, h, w, cfg, chi_r, y, stage_iters, NULL, 1e-12, 1e-10, nn, sw, cw, conf, nu,
+                     eps_t);
 }
 
 /* ------------------------------------------------------------------------ */

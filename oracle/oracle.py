@@ -83,7 +83,8 @@ class Oracle:
         for name, res in (("phase_per_sample", C.c_double), ("line_advance", C.c_double),
                           ("burst_amplitude_ire", self.R), ("acc_gain", self.R),
                           ("resolve_chroma_aniso", self.R), ("variational_refine", self.R),
-                          ("variational_refine_ex", self.R), ("decode_field", self.R)):
+                          ("variational_refine_ex", self.R), ("variational_refine_nbr", self.R),
+                          ("decode_field", self.R)):
             getattr(self.lib, self.pre + name).restype = res
 
     def f(self, name):
@@ -192,6 +193,27 @@ class Oracle:
         y = np.zeros_like(s)
         iters = np.zeros(max(cfg.irls_outer, 1), np.int32)
         aniso = self.f("variational_refine")(_p(s), _p(cb), h, w, C.byref(cfg), _p(chi), _p(y), _p(iters))
+        return y, self.uncbuf(chi), float(aniso), iters
+
+    def variational_refine_nbr(self, s, carrier, chi0, cfg: Cfg, neighbours, nu, eps_t):
+        """The solve with temporal neighbours: a list of (composite S_w,
+        carrier c_w, confidence) planes (hvd-core's NeighborTerm), at
+        temporal strength nu and temporal_eps eps_t."""
+        s = self.arr(s)
+        h, w = s.shape
+        cb = self.cbuf(carrier)
+        chi = self.cbuf(chi0)
+        y = np.zeros_like(s)
+        nn = len(neighbours)
+        sw = np.ascontiguousarray(np.stack([self.arr(n[0]) for n in neighbours]) if nn else np.zeros((1, h, w)),
+                                  self.dt)
+        cw = np.ascontiguousarray(np.stack([self.cbuf(n[1]) for n in neighbours]) if nn else np.zeros((1, h, 2 * w)),
+                                  self.dt)
+        cf = np.ascontiguousarray(np.stack([self.arr(n[2]) for n in neighbours]) if nn else np.zeros((1, h, w)),
+                                  self.dt)
+        iters = np.zeros(max(cfg.irls_outer, 1), np.int32)
+        aniso = self.f("variational_refine_nbr")(_p(s), _p(cb), h, w, C.byref(cfg), nn, _p(sw), _p(cw), _p(cf),
+                                                 self.R(nu), self.R(eps_t), _p(chi), _p(y), _p(iters))
         return y, self.uncbuf(chi), float(aniso), iters
 
     def variational_refine_stages(self, s, carrier, chi0, cfg: Cfg, hh_floor=1e-12, gg_floor=1e-10):
@@ -350,6 +372,24 @@ class Reference:
         chi[:, 0::2], chi[:, 1::2] = chi0.real, chi0.imag
         y = np.zeros_like(s)
         self.lib.ref_variational_refine(_p(s), _p(cb), h, w, C.byref(cfg), _p(chi), _p(y))
+        return y, chi[:, 0::2] + 1j * chi[:, 1::2]
+
+    def variational_refine_nbr(self, s, carrier, chi0, cfg: Cfg, neighbours, nu, eps_t):
+        s = self.f32(s)
+        h, w = s.shape
+
+        def cbuf(z):
+            b = np.zeros((h, 2 * w), np.float32)
+            b[:, 0::2], b[:, 1::2] = z.real, z.imag
+            return b
+        cb, chi = cbuf(carrier), cbuf(chi0)
+        nn = len(neighbours)
+        sw = np.ascontiguousarray(np.stack([self.f32(n[0]) for n in neighbours]), np.float32)
+        cw = np.ascontiguousarray(np.stack([cbuf(n[1]) for n in neighbours]), np.float32)
+        cf = np.ascontiguousarray(np.stack([self.f32(n[2]) for n in neighbours]), np.float32)
+        y = np.zeros_like(s)
+        self.lib.ref_variational_refine_nbr(_p(s), _p(cb), h, w, C.byref(cfg), nn, _p(sw), _p(cw), _p(cf),
+                                            C.c_float(nu), C.c_float(eps_t), _p(chi), _p(y))
         return y, chi[:, 0::2] + 1j * chi[:, 1::2]
 
     def box_blur(self, a, r):

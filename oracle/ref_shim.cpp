@@ -153,6 +153,39 @@ void ref_variational_refine(const float* s, const float* carrier, int h, int w, 
   std::memcpy(y, r.luma.data(), sizeof(float) * r.luma.size());
 }
 
+// hvd-core's VariationalRefine with nn temporal neighbours (NeighborTerm:
+// composite sw, carrier cw, confidence conf; nn planes each) at
+// temporal_strength nu and temporal_eps eps_t.
+void ref_variational_refine_nbr(const float* s, const float* carrier, int h, int w, const ShimCfg* sc, int nn,
+                                const float* sw, const float* cw, const float* conf, float nu, float eps_t,
+                                float* chi, float* y) {
+  hvd::HvdConfig cfg;
+  cfg.lambda_c = static_cast<float>(sc->lambda_c);
+  cfg.chroma_aniso = static_cast<float>(sc->chroma_aniso);
+  cfg.charbonnier_eps = static_cast<float>(sc->charbonnier_eps);
+  cfg.chroma_eps = static_cast<float>(sc->chroma_eps);
+  cfg.structure_coupling = static_cast<float>(sc->structure_coupling);
+  cfg.diag_prior = static_cast<float>(sc->diag_prior);
+  cfg.cg_iterations = sc->cg_iterations;
+  cfg.irls_outer = sc->irls_outer;
+  cfg.cg_tol = static_cast<float>(sc->cg_tol);
+  cfg.fast = sc->fast != 0;
+  cfg.is_pal = sc->is_pal != 0;
+  cfg.temporal_strength = nu;
+  cfg.temporal_eps = eps_t;
+  std::vector<hvd::NeighborTerm> nb(nn);
+  const size_t np = static_cast<size_t>(h) * w;
+  for (int j = 0; j < nn; ++j) {
+    nb[j].composite = ToPlane(sw + j * np, h, w);
+    nb[j].carrier = ToCPlane(cw + j * 2 * np, h, w);
+    nb[j].confidence = ToPlane(conf + j * np, h, w);
+  }
+  const hvd::RefineResult r =
+      hvd::VariationalRefine(ToPlane(s, h, w), ToCPlane(carrier, h, w), ToCPlane(chi, h, w), cfg, nb);
+  std::memcpy(chi, r.chroma.data(), sizeof(float) * 2 * r.chroma.size());
+  std::memcpy(y, r.luma.data(), sizeof(float) * r.luma.size());
+}
+
 void ref_box_blur(float* a, int h, int w, int r) {
   hvd::Plane p = hvd::BoxBlur(ToPlane(a, h, w), r);
   std::memcpy(a, p.data(), sizeof(float) * p.size());
