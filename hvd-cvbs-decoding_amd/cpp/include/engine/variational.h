@@ -22,9 +22,11 @@ struct NeighborTerm {
   Plane confidence;
 };
 
-// The 2-D solve from chi0 (variational.cpp:198-556).  Temporal neighbour
-// terms are 3-D decoding, which this library does not run: a non-empty
-// `neighbors` returns an empty result.
+// The solve from chi0 (variational.cpp:198-556): the 2-D objective and, with
+// `neighbors` non-empty and cfg.temporal_strength > 0, the motion-compensated
+// temporal data terms at strength cfg.temporal_strength and robust gate
+// cfg.temporal_eps (which must then be positive: hvd-core's sequence
+// resolves it before the solve).  Errors return an empty result.
 RefineResult VariationalRefine(const Plane& s, const ComplexPlane& carrier, const ComplexPlane& chi0,
                                const HvdConfig& cfg, const std::vector<NeighborTerm>& neighbors = {});
 

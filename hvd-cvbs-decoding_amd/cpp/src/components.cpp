@@ -204,10 +204,14 @@ HoloInit HolographicInit(const Plane& s, const ComplexPlane& carrier, const Fiel
 RefineResult VariationalRefine(const Plane& s, const ComplexPlane& carrier, const ComplexPlane& chi0,
                                const HvdConfig& cfg, const std::vector<NeighborTerm>& neighbors) {
   try {
-    if (!neighbors.empty()) throw gpu::Error("temporal neighbour terms (3-D decoding) are not run by this library");
     if (carrier.height() != s.height() || carrier.width() != s.width() || chi0.height() != s.height() ||
         chi0.width() != s.width())
       throw gpu::Error("carrier / chi0 size differs from S");
+    for (const NeighborTerm& nb : neighbors)
+      if (nb.composite.height() != s.height() || nb.composite.width() != s.width() ||
+          nb.carrier.height() != s.height() || nb.carrier.width() != s.width() ||
+          nb.confidence.height() != s.height() || nb.confidence.width() != s.width())
+        throw gpu::Error("neighbour plane size differs from S");
     RefineResult out;
     out.luma = Plane(s.height(), s.width());
     out.chroma = ComplexPlane(s.height(), s.width());
@@ -220,8 +224,27 @@ RefineResult VariationalRefine(const Plane& s, const ComplexPlane& carrier, cons
     FieldGeometry g;
     const hvd_gpu_plane p = PlaneOf(s, ds, dc, g, chi.get(), y.get());
     const hvd_solver_params sp = gpu::ToAbi(cfg, cfg.is_pal);
-    ss.check(hvd_gpu_variational_refine(ss.ctx(), &p, 1, &sp, nullptr, nullptr, nullptr),
-             "hvd_gpu_variational_refine");
+    if (neighbors.empty()) {
+      ss.check(hvd_gpu_variational_refine(ss.ctx(), &p, 1, &sp, nullptr, nullptr, nullptr),
+               "hvd_gpu_variational_refine");
+    } else {
+      // the 3-D solve: nu = temporal_strength (variational.cpp:221)
+      std::vector<DevBuf<float>> bufs;
+      std::vector<hvd_gpu_neighbor> nb(neighbors.size());
+      bufs.reserve(3 * neighbors.size());
+      for (size_t j = 0; j < neighbors.size(); ++j) {
+        const NeighborTerm& t = neighbors[j];
+        bufs.push_back(gpu::Upload(t.composite.data(), t.composite.size()));
+        nb[j].composite = bufs.back().get();
+        bufs.push_back(gpu::Upload(reinterpret_cast<const float*>(t.carrier.data()), t.carrier.size() * 2));
+        nb[j].carrier = bufs.back().get();
+        bufs.push_back(gpu::Upload(t.confidence.data(), t.confidence.size()));
+        nb[j].confidence = bufs.back().get();
+      }
+      ss.check(hvd_gpu_variational_refine_temporal(ss.ctx(), &p, nb.data(), (int)nb.size(), &sp,
+                                                   cfg.temporal_strength, cfg.temporal_eps, nullptr, nullptr),
+               "hvd_gpu_variational_refine_temporal");
+    }
     y.download(out.luma.data(), s.size());
     chi.download(reinterpret_cast<float*>(out.chroma.data()), s.size() * 2);
     return out;

@@ -1467,6 +1467,115 @@ void refine_entry(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hv
   }
 }
 
+// VariationalRefine with temporal neighbours for one field (variational.cpp:
+// 198-556, NeighborTerm of variational.h:54-58): the aniso resolution and the
+// parameters of the 2-D solve, then per IRLS stage the weights (with the
+// neighbours' robust weights and the prior scale) and the CG, one launch per
+// step, the scalars on the device so nothing waits on the host.
+void refine_temporal(hvd_gpu_ctx* ctx, const hvd_gpu_plane& pl, const hvd_gpu_neighbor* nbrs, int nn,
+                     const hvd_solver_params& p, float nu, float eps_t, int* stage_iters, hipStream_t st) {
+  const size_t np = (size_t)pl.lines * pl.width;
+  const int nblk = (int)((np + kTemporalThreads - 1) / kTemporalThreads);
+  const int ns = aniso_samples(pl.lines, pl.width, p.is_pal);
+  Carve cv;
+  const size_t o_row = cv.take(sizeof(float2) * pl.lines), o_vsw = cv.take(pl.lines),
+               o_col = cv.take(sizeof(float2) * pl.width);
+  const size_t o_st = cv.take(sizeof(FieldCG)), o_cg = cv.take(sizeof(TemporalCG));
+  const size_t o_gx = cv.take(sizeof(float) * (ns + 1)), o_gy = cv.take(sizeof(float) * (ns + 1));
+  const size_t o_g = cv.take(sizeof(float2) * np), o_d = cv.take(sizeof(float2) * np),
+               o_w = cv.take(sizeof(float4) * np);
+  std::vector<size_t> o_nb(nn);
+  for (int j = 0; j < nn; ++j) o_nb[j] = cv.take(sizeof(float4) * np);
+  const size_t o_part = cv.take(sizeof(double) * 2 * nblk);
+  char* ws = ensure_ws(ctx, cv.off, st);
+  DevField f{};
+  f.h = pl.lines;
+  f.w = pl.width;
+  f.s = pl.s;
+  f.carrier = reinterpret_cast<const float2*>(pl.carrier);
+  f.chi[0] = f.chi[1] = reinterpret_cast<float2*>(pl.chi);
+  f.y = pl.y;
+  if (!pl.carrier) {
+    run_prepare(ctx, {plane_tables(pl, (float2*)(ws + o_row), (int8_t*)(ws + o_vsw), (float2*)(ws + o_col))}, false,
+                st);
+    f.rowph = (const float2*)(ws + o_row);
+    f.colph = (const float2*)(ws + o_col);
+    f.vsw = pl.vswitch ? (const int8_t*)(ws + o_vsw) : nullptr;
+  }
+  const int outer = std::max(1, p.irls_outer);
+  const int cg_total = p.fast ? p.cg_iterations * 2 / 3 : p.cg_iterations;
+  const int n_inner = std::max(1, cg_total / outer);
+  const float tolf = p.cg_tol > 0.0 ? (float)p.cg_tol : (p.fast ? 0.10f : 0.02f);
+  const std::vector<FieldCG> init{field_cg_init(f, p, n_inner, (double)tolf)};
+  const std::vector<int> sp{0, ns};
+  Stage stg;
+  const size_t oF = stg.put(std::vector<DevField>{f}), oI = stg.put(init), oS = stg.put(sp);
+  Slot* sl;
+  char* base = upload(ctx, stg, st, &sl);
+  FieldCG* dstate = (FieldCG*)(ws + o_st);
+  float* gx = (float*)(ws + o_gx);
+  float* gy = (float*)(ws + o_gy);
+  hipLaunchKernelGGL(k_refine_begin, dim3(1), dim3(64), 0, st, dstate, (const FieldCG*)(base + oI), 1);
+  LAUNCH_CHECK(ctx);
+  if (!(p.chroma_aniso > 0.0) && ns > 0) {
+    hipLaunchKernelGGL(k_aniso_samples, dim3(std::min(4096, (ns + 255) / 256)), dim3(256), 0, st,
+                       (const DevField*)(base + oF), (const FieldCG*)dstate, (const int*)(base + oS), 1, gx, gy);
+    LAUNCH_CHECK(ctx);
+  }
+  {
+    LaunchList L;
+    record_aniso_select(ctx, L, dstate, (const int*)(base + oS), gx, gy, (float)p.lambda_c, sp);
+    run_launches(ctx, L, false, st);
+  }
+  TemporalField T{};
+  T.f = f;
+  T.g = (float2*)(ws + o_g);
+  T.d = (float2*)(ws + o_d);
+  T.wts = (float4*)(ws + o_w);
+  TemporalInputs in{};
+  for (int j = 0; j < nn; ++j) {
+    T.nb[j] = (float4*)(ws + o_nb[j]);
+    T.conf[j] = nbrs[j].confidence;
+    in.sw[j] = nbrs[j].composite;
+    in.cw[j] = reinterpret_cast<const float2*>(nbrs[j].carrier);
+  }
+  T.nn = nn;
+  T.nu = nu;
+  T.et2 = eps_t * eps_t;
+  T.part = (double*)(ws + o_part);
+  T.cg = (TemporalCG*)(ws + o_cg);
+  T.st = dstate;
+  const dim3 grid(nblk), blk(kTemporalThreads);
+  auto launch = [&](const void* fn, dim3 g, std::vector<void*> args) {
+    HIPCHK(ctx, hipLaunchKernel(fn, g, blk, args.data(), 0, st));
+  };
+  int start = 1, cont = 0;
+  launch((const void*)k_t_prep, grid, {&T, &in});
+  for (int o = 0; o < outer; ++o) {
+    launch((const void*)k_t_stage_begin, dim3(1), {&T});
+    launch((const void*)k_t_weights, grid, {&T});
+    launch((const void*)k_t_grad, grid, {&T, &start});
+    launch((const void*)k_t_sum_grad, dim3(1), {&T, (void*)&nblk, &start});
+    for (int it = 0; it < n_inner; ++it) {
+      launch((const void*)k_t_curv, grid, {&T});
+      launch((const void*)k_t_alpha, dim3(1), {&T, (void*)&nblk});
+      launch((const void*)k_t_move, grid, {&T});
+      launch((const void*)k_t_grad, grid, {&T, &cont});
+      launch((const void*)k_t_sum_grad, dim3(1), {&T, (void*)&nblk, &cont});
+      launch((const void*)k_t_dir, grid, {&T});
+    }
+    launch((const void*)k_t_stage_end, dim3(1), {&T, &o});
+  }
+  launch((const void*)k_t_finalize, grid, {&T});
+  arm(ctx, sl, st);
+  if (stage_iters) {
+    TemporalCG h{};
+    HIPCHK(ctx, hipMemcpyAsync(&h, T.cg, sizeof(h), hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    for (int o = 0; o < outer; ++o) stage_iters[o] = o < 16 ? h.iters[o] : -1;
+  }
+}
+
 }  // namespace
 
 namespace {
@@ -1619,6 +1728,28 @@ extern "C" {
 int hvd_gpu_variational_refine(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hvd_solver_params* p,
                                int* stage_iters, float* aniso, void* stream) {
   return guarded(ctx, [&] { refine_entry(ctx, planes, n, p, stage_iters, aniso, nullptr, stream); });
+}
+
+int hvd_gpu_variational_refine_temporal(hvd_gpu_ctx* ctx, const hvd_gpu_plane* plane, const hvd_gpu_neighbor* nbrs,
+                                        int nn, const hvd_solver_params* p, float temporal_strength,
+                                        float temporal_eps, int* stage_iters, void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, plane != nullptr, "null plane");
+    REQUIRE(ctx, nn >= 0 && nn <= kMaxNeighbors, "neighbour count %d outside 0..%d", nn, kMaxNeighbors);
+    REQUIRE(ctx, nn == 0 || nbrs, "null neighbour array");
+    // no neighbours or no strength: the weights stay zero and every temporal
+    // term vanishes (variational.cpp:221, :452): the 2-D solve
+    if (nn == 0 || !(temporal_strength > 0.0f)) {
+      refine_entry(ctx, plane, 1, p, stage_iters, nullptr, nullptr, stream);
+      return;
+    }
+    check_params(ctx, p);
+    check_plane(ctx, *plane, 0);
+    REQUIRE(ctx, temporal_eps > 0.0f, "temporal_eps must be positive (resolve it first, as sequence.cpp:445-446 does)");
+    for (int j = 0; j < nn; ++j)
+      REQUIRE(ctx, nbrs[j].composite && nbrs[j].carrier && nbrs[j].confidence, "neighbour %d: null plane", j);
+    refine_temporal(ctx, *plane, nbrs, nn, *p, temporal_strength, temporal_eps, stage_iters, S(stream));
+  });
 }
 
 int hvd_gpu_variational_refine_stages(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n,

@@ -147,6 +147,51 @@ __global__ void k_cg_iter_adj(const DevField* F, FieldCG* st, StripGrid sg, doub
 __global__ void k_cg_finish(FieldCG* st, StripGrid sg, const double* partials, int n_inner, int stage);
 __global__ void k_refine_finalize(const DevField* F, const FieldCG* st, StripGrid sg);
 
+// The solve with motion-compensated temporal neighbours (variational.cpp:
+// 198-556 with NeighborTerm, variational.h:54-58), one field per call.
+constexpr int kMaxNeighbors = 16;
+constexpr int kTemporalThreads = 256;
+
+// Scalar CG state of the temporal solve (variational.cpp:519-549).
+struct TemporalCG {
+  double gg, gg0;
+  float alpha, beta;
+  int it, done;
+  int iters[16];
+};
+
+struct TemporalField {
+  DevField f;               // chi[0]: chi in place; s, carrier (plane or tables), y
+  float2* g;                // gradient at chi
+  float2* d;                // search direction
+  float4* wts;              // wx, wy, wcx, wcy times the prior scale
+  float4* nb[kMaxNeighbors];  // per neighbour: S_w - S, dc = c - c_w, robust weight wt
+  const float* conf[kMaxNeighbors];
+  int nn;
+  float nu, et2;
+  int pad_;
+  double* part;             // two per block
+  TemporalCG* cg;
+  const FieldCG* st;        // mu_h, mu_v, lam_y, eps, eps_c, k, exits
+};
+
+struct TemporalInputs {
+  const float* sw[kMaxNeighbors];   // neighbour composite
+  const float2* cw[kMaxNeighbors];  // neighbour carrier
+};
+
+__global__ void k_t_prep(TemporalField T, TemporalInputs in);
+__global__ void k_t_stage_begin(TemporalField T);
+__global__ void k_t_weights(TemporalField T);
+__global__ void k_t_grad(TemporalField T, int start);
+__global__ void k_t_sum_grad(TemporalField T, int nblocks, int start);
+__global__ void k_t_curv(TemporalField T);
+__global__ void k_t_alpha(TemporalField T, int nblocks);
+__global__ void k_t_move(TemporalField T);
+__global__ void k_t_dir(TemporalField T);
+__global__ void k_t_stage_end(TemporalField T, int stage);
+__global__ void k_t_finalize(TemporalField T);
+
 // bridge.hip
 __global__ void k_split_fields(const BridgeFrameParams* FP, const int16_t* const* frames, float zero,
                                float* const* fields);

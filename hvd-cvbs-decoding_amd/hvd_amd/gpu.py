@@ -26,6 +26,7 @@ ABI_SYMBOLS = (
     "hvd_gpu_make_carrier", "hvd_gpu_tridiag_smooth", "hvd_gpu_box_blur", "hvd_gpu_chroma_plane",
     "hvd_gpu_weave_frame_s16", "hvd_gpu_variational_refine_stages", "hvd_gpu_refine_adjoint",
     "hvd_gpu_set_solver_tuning", "hvd_gpu_autotune_solver", "hvd_gpu_set_graphs",
+    "hvd_gpu_variational_refine_temporal",
 )
 
 
@@ -77,6 +78,11 @@ class Plane(C.Structure):
                 ("vswitch", C.c_void_p), ("carrier", C.c_void_p), ("active_video_start", C.c_int),
                 ("phase_per_sample", C.c_double), ("sample_rate", C.c_double), ("chi", C.c_void_p),
                 ("y", C.c_void_p)]
+
+
+class Neighbor(C.Structure):
+    """hvd::NeighborTerm (variational.h:54-58) as device pointers."""
+    _fields_ = [("composite", C.c_void_p), ("carrier", C.c_void_p), ("confidence", C.c_void_p)]
 
 
 class FrameParams(C.Structure):
@@ -148,6 +154,8 @@ def load() -> C.CDLL:
         lib.hvd_gpu_weave_frame_s16.argtypes = [vp, vp, C.POINTER(FrameParams), i, i, vp, vp, vp, C.c_float,
                                                 C.POINTER(YcFrame), vp]
         lib.hvd_gpu_variational_refine_stages.argtypes = [vp, C.POINTER(Plane), i, C.POINTER(SolverParams), pp, vp]
+        lib.hvd_gpu_variational_refine_temporal.argtypes = [vp, C.POINTER(Plane), C.POINTER(Neighbor), i,
+                                                            C.POINTER(SolverParams), C.c_float, C.c_float, vp, vp]
         lib.hvd_gpu_refine_adjoint.argtypes = [vp, C.POINTER(Plane), i, C.POINTER(SolverParams), pp, pp, pp, i,
                                                C.c_double, pp, pp, C.POINTER(C.c_double), vp]
         lib.hvd_gpu_set_profiling.argtypes = [vp, i]
@@ -292,6 +300,25 @@ class Engine:
                     "variational_refine")
         if want_stats:
             return np.array(iters[:], dtype=np.int32).reshape(n, outer), np.array(aniso[:], dtype=np.float32)
+        return None
+
+    def variational_refine_temporal(self, plane, neighbours, params: SolverParams, temporal_strength,
+                                    temporal_eps, want_stats=False):
+        """VariationalRefine of one plane with temporal neighbours
+        (variational.cpp:198-556): neighbours is a list of (composite,
+        carrier (lines, width, 2), confidence) float32 device tensors on the
+        plane's grid.  Returns the per-stage CG iterations when want_stats."""
+        nn = len(neighbours)
+        keep = [t for nb in neighbours for t in nb]
+        N = (Neighbor * max(nn, 1))(*[Neighbor(a.data_ptr(), b.data_ptr(), c.data_ptr()) for a, b, c in neighbours])
+        outer = max(params.irls_outer, 1)
+        iters = (C.c_int * outer)() if want_stats else None
+        self._check(self.lib.hvd_gpu_variational_refine_temporal(
+            self.ctx, C.byref(plane), N, nn, C.byref(params), float(temporal_strength), float(temporal_eps), iters,
+            _stream()), "variational_refine_temporal")
+        del keep
+        if want_stats:
+            return np.array(iters[:], dtype=np.int32)
         return None
 
     def variational_refine_stages(self, planes, params: SolverParams):

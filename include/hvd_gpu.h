@@ -238,6 +238,30 @@ int hvd_gpu_holographic_init(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int 
 int hvd_gpu_variational_refine(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hvd_solver_params* p,
                                int* stage_iters, float* aniso, void* stream);
 
+/* One motion-compensated temporal neighbour of a field (hvd::NeighborTerm,
+ * variational.h:54-58), on the field's lines x width grid: its composite S_w,
+ * its carrier c_w (interleaved complex) and a per-pixel confidence. */
+typedef struct {
+  const float* composite;  /* lines * width */
+  const float* carrier;    /* lines * width * 2 */
+  const float* confidence; /* lines * width */
+} hvd_gpu_neighbor;
+
+/* VariationalRefine with temporal neighbours (variational.cpp:198-556 with
+ * `neighbors` non-empty; engine.h:82-88): one field, the 2-D objective plus
+ * per neighbour the robust data term temporal_strength * wt * rt^2 with
+ * rt = (S_w - S) + Re[chi (c - c_w)] and wt = conf eps^2 / (rt^2 + eps^2)
+ * lagged per IRLS stage, and the smoothness weights scaled by
+ * 1 + temporal_strength * sum wt |c - c_w|^2.  nn <= 16.  With nn == 0 or
+ * temporal_strength <= 0 every temporal term vanishes and this is
+ * hvd_gpu_variational_refine of the one plane.  temporal_eps must be
+ * positive (hvd-core's sequence resolves it before the solve,
+ * sequence.cpp:445-446).  stage_iters: optional HOST output, irls_outer
+ * ints (makes the call synchronous). */
+int hvd_gpu_variational_refine_temporal(hvd_gpu_ctx* ctx, const hvd_gpu_plane* plane, const hvd_gpu_neighbor* nbrs,
+                                        int nn, const hvd_solver_params* p, float temporal_strength,
+                                        float temporal_eps, int* stage_iters, void* stream);
+
 /* ---- differentiation of the solve (no hvd-core counterpart) ------------ */
 
 /* hvd_gpu_variational_refine that also saves chi at the start of each IRLS

@@ -11,6 +11,11 @@
 //       u, v (f64).
 //   api_driver frame <ntsc|pal> <in.s16> <out>          DecodeFrameBuffer
 //   api_driver ycframe <ntsc|pal> <luma.s16> <chroma.s16> <out>  DecodeYcFrameBuffer
+//   api_driver refine3d <ntsc|pal> <h> <w> <nn> <nu> <eps_t> <dir>
+//       VariationalRefine with nn temporal neighbours (cg_iterations 40,
+//       irls_outer 4, slow mode, chroma_aniso 0.7) on dir/s, dir/car,
+//       dir/chi0, dir/sw, dir/cw, dir/conf (f32, complex interleaved, the
+//       neighbours' planes one after another); out dir/out: luma then chi.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -20,6 +25,7 @@
 #include "engine/engine.h"
 #include "engine/gpu_status.h"
 #include "engine/sequence.h"
+#include "engine/variational.h"
 #include "frame_bridge.h"
 
 namespace {
@@ -179,6 +185,46 @@ int main(int argc, char** argv) {
     if (y.luma.empty()) return Fail();
     Out o(argv[5]);
     PutFrame(o, y);
+    return 0;
+  }
+  if (mode == "refine3d" && argc == 9) {
+    const int h = std::atoi(argv[3]), w = std::atoi(argv[4]), nn = std::atoi(argv[5]);
+    const size_t n = (size_t)h * w;
+    const std::string dir = argv[8];
+    auto plane = [&](const std::vector<float>& v, size_t off) {
+      hvd::Plane p(h, w);
+      std::memcpy(p.data(), v.data() + off, sizeof(float) * n);
+      return p;
+    };
+    auto cplane = [&](const std::vector<float>& v, size_t off) {
+      hvd::ComplexPlane p(h, w);
+      std::memcpy(reinterpret_cast<float*>(p.data()), v.data() + off, sizeof(float) * 2 * n);
+      return p;
+    };
+    const std::vector<float> s = ReadAll<float>((dir + "/s").c_str(), n);
+    const std::vector<float> car = ReadAll<float>((dir + "/car").c_str(), 2 * n);
+    const std::vector<float> chi0 = ReadAll<float>((dir + "/chi0").c_str(), 2 * n);
+    const std::vector<float> sw = ReadAll<float>((dir + "/sw").c_str(), nn * n);
+    const std::vector<float> cw = ReadAll<float>((dir + "/cw").c_str(), 2 * nn * n);
+    const std::vector<float> conf = ReadAll<float>((dir + "/conf").c_str(), nn * n);
+    std::vector<hvd::NeighborTerm> nbrs(nn);
+    for (int j = 0; j < nn; ++j) {
+      nbrs[j].composite = plane(sw, j * n);
+      nbrs[j].carrier = cplane(cw, 2 * j * n);
+      nbrs[j].confidence = plane(conf, j * n);
+    }
+    hvd::HvdConfig cfg;
+    cfg.cg_iterations = 40;
+    cfg.irls_outer = 4;
+    cfg.fast = false;
+    cfg.chroma_aniso = 0.7F;
+    cfg.temporal_strength = (float)std::atof(argv[6]);
+    cfg.temporal_eps = (float)std::atof(argv[7]);
+    const hvd::RefineResult r = hvd::VariationalRefine(plane(s, 0), cplane(car, 0), cplane(chi0, 0), cfg, nbrs);
+    if (r.luma.empty()) return Fail();
+    Out o((dir + "/out").c_str());
+    o.put(r.luma.data(), n);
+    o.put(reinterpret_cast<const float*>(r.chroma.data()), 2 * n);
     return 0;
   }
   std::fprintf(stderr, "usage: see the header of api_driver.cpp\n");

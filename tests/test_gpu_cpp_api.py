@@ -223,3 +223,30 @@ def test_decode_yc_frame_buffer(tmp_path, orc64, orc32, std):
     scale = (fpd["white_level"] - fpd["blanking_level"]) / 100.0
     assert np.abs(gu - wu).max() <= TOL_IRE * scale * max(gain, 1.0)
     assert np.abs(gv - wv).max() <= TOL_IRE * scale * max(gain, 1.0)
+
+
+def test_variational_refine_neighbours(tmp_path, orc64):
+    """hvd::VariationalRefine with temporal neighbours (variational.cpp:
+    198-556, NeighborTerm) through the C++ API vs the oracle's 3-D solve."""
+    from temporal_cases import temporal_scene
+    h, w, nu, eps_t = 40, 96, 0.8, 1.5
+    s, car, chi0, nbrs = temporal_scene(h, w, 21, n_nbr=2)
+
+    def cf(z):
+        return np.stack([z.real, z.imag], -1).astype(np.float32)
+
+    s.astype(np.float32).tofile(tmp_path / "s")
+    cf(car).tofile(tmp_path / "car")
+    cf(chi0).tofile(tmp_path / "chi0")
+    np.stack([a for a, _, _ in nbrs]).astype(np.float32).tofile(tmp_path / "sw")
+    np.stack([cf(b) for _, b, _ in nbrs]).tofile(tmp_path / "cw")
+    np.stack([c for _, _, c in nbrs]).astype(np.float32).tofile(tmp_path / "conf")
+    run(["refine3d", "ntsc", h, w, len(nbrs), nu, eps_t, tmp_path])
+    out = np.fromfile(tmp_path / "out", np.float32)
+    y = out[:h * w].reshape(h, w)
+    chi = out[h * w:].reshape(h, w, 2)
+    chi = chi[..., 0].astype(np.float64) + 1j * chi[..., 1]
+    cfg = O.Cfg.make(cg_iterations=40, irls_outer=4, fast=0, chroma_aniso=0.7)
+    y_ref, c_ref, _, _ = orc64.variational_refine_nbr(s, car, chi0, cfg, nbrs, nu, eps_t)
+    assert np.abs(y - y_ref).max() <= TOL_IRE
+    assert np.abs(chi - c_ref).max() <= TOL_IRE
