@@ -1507,3 +1507,171 @@ void OR_FN(block_match_sad_u16)(const uint16_t *ref, const uint16_t *cur, int h,
       sad[by * nbx + bx] = best;
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* temporal neighbours' warps: engine/temporal.cpp                          */
+/* ------------------------------------------------------------------------ */
+
+/* :19-40 3x3 component-wise median, edge-clamped; grids under 3x3 copied. */
+static void median3(const R *v, int th, int tw, R *out) {
+  if (th < 3 || tw < 3) { memcpy(out, v, sizeof(R) * (size_t)th * tw); return; }
+  for (int y = 0; y < th; ++y)
+    for (int x = 0; x < tw; ++x) {
+      R a[9];
+      int k = 0;
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int yy = y + dy < 0 ? 0 : (y + dy > th - 1 ? th - 1 : y + dy);
+          const int xx = x + dx < 0 ? 0 : (x + dx > tw - 1 ? tw - 1 : x + dx);
+          a[k++] = v[(size_t)yy * tw + xx];
+        }
+      for (int i = 1; i < 9; ++i) /* insertion sort: the 5th smallest */
+        for (int j = i; j > 0 && a[j - 1] > a[j]; --j) { const R t = a[j]; a[j] = a[j - 1]; a[j - 1] = t; }
+      out[(size_t)y * tw + x] = a[4];
+    }
+}
+
+/* :73-87 the tile-centre bracket of a pixel coordinate. */
+static void tile_bracket(int coord, int n_tiles, int tile, int *lo, R *frac) {
+  if (n_tiles > 1) {
+    const float rel = (float)coord / (float)tile - 0.5f;
+    int l = (int)floorf(rel);
+    l = l < 0 ? 0 : (l > n_tiles - 2 ? n_tiles - 2 : l);
+    const float centre = ((float)l + 0.5f) * (float)tile;
+    float f = ((float)coord - centre) / (float)tile;
+    f = f < 0.f ? 0.f : (f > 1.f ? 1.f : f);
+    *lo = l;
+    *frac = (R)f;
+  } else {
+    *lo = 0;
+    *frac = 0;
+  }
+}
+
+/* :44-113 VectorsPerPixel: 3x3-median outlier snap (|dy-my| + |dx-mx| > 3
+ * takes the median), then bilinear interpolation between tile centres. */
+void OR_FN(vectors_per_pixel)(const R *mdy_in, const R *mdx_in, int th, int tw, int tile, int out_h, int out_w,
+                              R *dy, R *dx) {
+  const size_t nt = (size_t)th * tw;
+  R *my = (R *)xmalloc(sizeof(R) * nt), *mx = (R *)xmalloc(sizeof(R) * nt);
+  R *vy = (R *)xmalloc(sizeof(R) * nt), *vx = (R *)xmalloc(sizeof(R) * nt);
+  median3(mdy_in, th, tw, my);
+  median3(mdx_in, th, tw, mx);
+  for (size_t i = 0; i < nt; ++i) {
+    const R dis = (R)fabs((double)(mdy_in[i] - my[i])) + (R)fabs((double)(mdx_in[i] - mx[i]));
+    vy[i] = dis > 3 ? my[i] : mdy_in[i];
+    vx[i] = dis > 3 ? mx[i] : mdx_in[i];
+  }
+  for (int y = 0; y < out_h; ++y) {
+    int iy;
+    R fy;
+    tile_bracket(y, th, tile, &iy, &fy);
+    const int iy1 = iy + 1 < th - 1 ? iy + 1 : th - 1;
+    for (int x = 0; x < out_w; ++x) {
+      int ix;
+      R fx;
+      tile_bracket(x, tw, tile, &ix, &fx);
+      const int ix1 = ix + 1 < tw - 1 ? ix + 1 : tw - 1;
+      for (int c = 0; c < 2; ++c) {
+        const R *v = c ? vx : vy;
+        const R v00 = v[(size_t)iy * tw + ix], v10 = v[(size_t)iy1 * tw + ix];
+        const R v01 = v[(size_t)iy * tw + ix1], v11 = v[(size_t)iy1 * tw + ix1];
+        (c ? dx : dy)[(size_t)y * out_w + x] =
+            v00 * (1 - fy) * (1 - fx) + v10 * fy * (1 - fx) + v01 * (1 - fy) * fx + v11 * fy * fx;
+      }
+    }
+  }
+  free(my); free(mx); free(vy); free(vx);
+}
+
+/* :117-132, :198-215 WarpByTiles / WarpByTilesComplex: out(y, x) =
+ * a(clamp(y - lround(dy)), clamp(x - lround(dx))), `ch` values per pixel. */
+void OR_FN(warp_by_tiles)(const R *a, int h, int w, int ch, const R *dy, const R *dx, R *out) {
+  for (int y = 0; y < h; ++y)
+    for (int x = 0; x < w; ++x) {
+      const size_t i = (size_t)y * w + x;
+      int sy = y - (int)lround((double)dy[i]), sx = x - (int)lround((double)dx[i]);
+      sy = sy < 0 ? 0 : (sy > h - 1 ? h - 1 : sy);
+      sx = sx < 0 ? 0 : (sx > w - 1 ? w - 1 : sx);
+      for (int c = 0; c < ch; ++c) out[i * ch + c] = a[((size_t)sy * w + sx) * ch + c];
+    }
+}
+
+/* :140-166 WarpBilinearTiles with per-pixel vectors on the output grid. */
+void OR_FN(warp_bilinear)(const R *a, int hs, int ws, const R *dy, const R *dx, R row_offset, int out_h, int out_w,
+                          R *out) {
+  for (int y = 0; y < out_h; ++y)
+    for (int x = 0; x < out_w; ++x) {
+      const size_t i = (size_t)y * out_w + x;
+      R sy = (R)y - dy[i] + row_offset, sx = (R)x - dx[i];
+      const R my = (R)hs - (R)1.001f, mx = (R)ws - (R)1.001f;
+      sy = sy < 0 ? 0 : (sy > my ? my : sy);
+      sx = sx < 0 ? 0 : (sx > mx ? mx : sx);
+      const int y0 = (int)sy, x0 = (int)sx;
+      const R fy = sy - (R)y0, fx = sx - (R)x0;
+      const int y1 = y0 + 1 < hs - 1 ? y0 + 1 : hs - 1, x1 = x0 + 1 < ws - 1 ? x0 + 1 : ws - 1;
+      out[i] = a[(size_t)y0 * ws + x0] * (1 - fy) * (1 - fx) + a[(size_t)y1 * ws + x0] * fy * (1 - fx) +
+               a[(size_t)y0 * ws + x1] * (1 - fy) * fx + a[(size_t)y1 * ws + x1] * fy * fx;
+    }
+}
+
+/* :217-249 UpsampleConfidence (nearest tile, squared, BoxBlur radius 8) and
+ * the fast form (squared tile map through VectorsPerPixel). */
+void OR_FN(upsample_confidence)(const R *conf, int th, int tw, int tile, int out_h, int out_w, int fast, R *out) {
+  if (fast) {
+    const size_t nt = (size_t)th * tw;
+    R *c2 = (R *)xmalloc(sizeof(R) * nt), *tmp = (R *)xmalloc(sizeof(R) * (size_t)out_h * out_w);
+    for (size_t i = 0; i < nt; ++i) c2[i] = conf[i] * conf[i];
+    OR_FN(vectors_per_pixel)(c2, c2, th, tw, tile, out_h, out_w, out, tmp);
+    free(c2); free(tmp);
+    return;
+  }
+  for (int y = 0; y < out_h; ++y) {
+    const int ty = y / tile < th - 1 ? y / tile : th - 1;
+    for (int x = 0; x < out_w; ++x) {
+      const int tx = x / tile < tw - 1 ? x / tile : tw - 1;
+      const R c = conf[(size_t)ty * tw + tx];
+      out[(size_t)y * out_w + x] = c * c;
+    }
+  }
+  OR_FN(box_blur_prefix)(out, out_h, out_w, 8);
+}
+
+/* :169-196 ComplexCoherence: |<z1 conj z2>| / (sqrt(<|z1|^2><|z2|^2>) + 1e-6),
+ * box-blurred averages of radius r, clamped to [0, 1]. */
+void OR_FN(complex_coherence)(const R *z1, const R *z2, int h, int w, int r, R *out) {
+  const size_t n = (size_t)h * w;
+  R *p = (R *)xmalloc(sizeof(R) * n * 4);
+  R *cre = p, *cim = p + n, *m1 = p + 2 * n, *m2 = p + 3 * n;
+  for (size_t i = 0; i < n; ++i) {
+    const R ar = z1[2 * i], ai = z1[2 * i + 1], br = z2[2 * i], bi = z2[2 * i + 1];
+    cre[i] = ar * br + ai * bi;
+    cim[i] = ai * br - ar * bi;
+    m1[i] = ar * ar + ai * ai;
+    m2[i] = br * br + bi * bi;
+  }
+  for (int k = 0; k < 4; ++k) OR_FN(box_blur_prefix)(p + k * n, h, w, r);
+  for (size_t i = 0; i < n; ++i) {
+    const R num = (R)sqrt((double)(cre[i] * cre[i] + cim[i] * cim[i]));
+    const R pr = m1[i] * m2[i];
+    const R den = (R)sqrt((double)(pr > 0 ? pr : 0)) + (R)1e-6f;
+    const R g = num / den;
+    out[i] = g < 0 ? 0 : (g > 1 ? 1 : g);
+  }
+  free(p);
+}
+
+/* :253-287 MotionCompensatePrev with a given motion field: one per-pixel
+ * vector interpolation shared by the composite and carrier warps, and the
+ * confidence upsample.  car / out_car are interleaved complex. */
+void OR_FN(motion_compensate_prev)(const R *comp, const R *car, int h, int w, const R *mdy, const R *mdx,
+                                   const R *mconf, int th, int tw, int tile, int fast, R *out_comp, R *out_car,
+                                   R *out_conf) {
+  const size_t n = (size_t)h * w;
+  R *dy = (R *)xmalloc(sizeof(R) * n), *dx = (R *)xmalloc(sizeof(R) * n);
+  OR_FN(vectors_per_pixel)(mdy, mdx, th, tw, tile, h, w, dy, dx);
+  OR_FN(warp_by_tiles)(comp, h, w, 1, dy, dx, out_comp);
+  OR_FN(warp_by_tiles)(car, h, w, 2, dy, dx, out_car);
+  OR_FN(upsample_confidence)(mconf, th, tw, tile, h, w, fast, out_conf);
+  free(dy); free(dx);
+}

@@ -278,6 +278,51 @@ class Oracle:
         self.f("box_blur_prefix")(_p(a), h, w, r)
         return a
 
+    # -- temporal neighbours' warps (temporal.cpp) ---------------------------
+    def vectors_per_pixel(self, mdy, mdx, tile, h, w):
+        mdy, mdx = self.arr(mdy), self.arr(mdx)
+        dy, dx = np.zeros((h, w), self.dt), np.zeros((h, w), self.dt)
+        self.f("vectors_per_pixel")(_p(mdy), _p(mdx), mdy.shape[0], mdy.shape[1], tile, h, w, _p(dy), _p(dx))
+        return dy, dx
+
+    def warp_by_tiles(self, a, dy, dx):
+        """a: (h, w) real or complex; dy, dx: per-pixel vectors."""
+        cplx = np.iscomplexobj(a)
+        h, w = a.shape
+        src = self.cbuf(a) if cplx else self.arr(a)
+        out = np.zeros_like(src)
+        self.f("warp_by_tiles")(_p(src), h, w, 2 if cplx else 1, _p(self.arr(dy)), _p(self.arr(dx)), _p(out))
+        return self.uncbuf(out) if cplx else out
+
+    def warp_bilinear(self, a, dy, dx, row_offset):
+        a = self.arr(a)
+        out_h, out_w = dy.shape
+        out = np.zeros((out_h, out_w), self.dt)
+        self.f("warp_bilinear")(_p(a), a.shape[0], a.shape[1], _p(self.arr(dy)), _p(self.arr(dx)),
+                                self.R(row_offset), out_h, out_w, _p(out))
+        return out
+
+    def upsample_confidence(self, conf, tile, h, w, fast=False):
+        conf = self.arr(conf)
+        out = np.zeros((h, w), self.dt)
+        self.f("upsample_confidence")(_p(conf), conf.shape[0], conf.shape[1], tile, h, w, int(fast), _p(out))
+        return out
+
+    def complex_coherence(self, z1, z2, r):
+        h, w = z1.shape
+        out = np.zeros((h, w), self.dt)
+        self.f("complex_coherence")(_p(self.cbuf(z1)), _p(self.cbuf(z2)), h, w, r, _p(out))
+        return out
+
+    def motion_compensate_prev(self, comp, car, mdy, mdx, mconf, tile, fast=False):
+        comp = self.arr(comp)
+        h, w = comp.shape
+        mdy, mdx, mconf = self.arr(mdy), self.arr(mdx), self.arr(mconf)
+        oc, ocar, ocf = np.zeros((h, w), self.dt), np.zeros((h, 2 * w), self.dt), np.zeros((h, w), self.dt)
+        self.f("motion_compensate_prev")(_p(comp), _p(self.cbuf(car)), h, w, _p(mdy), _p(mdx), _p(mconf),
+                                         mdy.shape[0], mdy.shape[1], tile, int(fast), _p(oc), _p(ocar), _p(ocf))
+        return oc, self.uncbuf(ocar), ocf
+
     def estimate_motion(self, y_ref, y_cur, tile=32, search=8):
         y_ref, y_cur = self.arr(y_ref), self.arr(y_cur)
         h, w = y_ref.shape
@@ -391,6 +436,57 @@ class Reference:
         self.lib.ref_variational_refine_nbr(_p(s), _p(cb), h, w, C.byref(cfg), nn, _p(sw), _p(cw), _p(cf),
                                             C.c_float(nu), C.c_float(eps_t), _p(chi), _p(y))
         return y, chi[:, 0::2] + 1j * chi[:, 1::2]
+
+    @staticmethod
+    def cbuf32(a):
+        a = np.asarray(a)
+        b = np.zeros((a.shape[0], 2 * a.shape[1]), np.float32)
+        b[:, 0::2], b[:, 1::2] = a.real, a.imag
+        return b
+
+    def vectors_per_pixel(self, mdy, mdx, tile, h, w):
+        mdy, mdx = self.f32(mdy), self.f32(mdx)
+        dy, dx = np.zeros((h, w), np.float32), np.zeros((h, w), np.float32)
+        self.lib.ref_vectors_per_pixel(_p(mdy), _p(mdx), mdy.shape[0], mdy.shape[1], tile, h, w, _p(dy), _p(dx))
+        return dy, dx
+
+    def warp_by_tiles(self, a, mdy, mdx, tile):
+        """hvd-core's WarpByTiles[Complex] from the TILE motion field."""
+        cplx = np.iscomplexobj(a)
+        h, w = a.shape
+        src = self.cbuf32(a) if cplx else self.f32(a)
+        out = np.zeros_like(src)
+        mdy, mdx = self.f32(mdy), self.f32(mdx)
+        self.lib.ref_warp_by_tiles(_p(src), h, w, 2 if cplx else 1, _p(mdy), _p(mdx), mdy.shape[0], mdy.shape[1],
+                                   tile, _p(out))
+        return out[:, 0::2] + 1j * out[:, 1::2] if cplx else out
+
+    def warp_bilinear(self, a, mdy, mdx, tile, row_offset, out_h, out_w):
+        a, mdy, mdx = self.f32(a), self.f32(mdy), self.f32(mdx)
+        out = np.zeros((out_h, out_w), np.float32)
+        self.lib.ref_warp_bilinear(_p(a), a.shape[0], a.shape[1], _p(mdy), _p(mdx), mdy.shape[0], mdy.shape[1], tile,
+                                   C.c_float(row_offset), out_h, out_w, _p(out))
+        return out
+
+    def upsample_confidence(self, conf, tile, h, w, fast=False):
+        conf = self.f32(conf)
+        out = np.zeros((h, w), np.float32)
+        self.lib.ref_upsample_confidence(_p(conf), conf.shape[0], conf.shape[1], tile, h, w, int(fast), _p(out))
+        return out
+
+    def complex_coherence(self, z1, z2, r):
+        h, w = z1.shape
+        out = np.zeros((h, w), np.float32)
+        self.lib.ref_complex_coherence(_p(self.cbuf32(z1)), _p(self.cbuf32(z2)), h, w, r, _p(out))
+        return out
+
+    def motion_compensate_prev(self, comp, car, mdy, mdx, mconf, tile, fast=False):
+        comp, mdy, mdx, mconf = self.f32(comp), self.f32(mdy), self.f32(mdx), self.f32(mconf)
+        h, w = comp.shape
+        oc, ocar, ocf = np.zeros((h, w), np.float32), np.zeros((h, 2 * w), np.float32), np.zeros((h, w), np.float32)
+        self.lib.ref_motion_compensate_prev(_p(comp), _p(self.cbuf32(car)), h, w, _p(mdy), _p(mdx), _p(mconf), mdy.shape[0],
+                                            mdy.shape[1], tile, int(fast), _p(oc), _p(ocar), _p(ocf))
+        return oc, ocar[:, 0::2] + 1j * ocar[:, 1::2], ocf
 
     def box_blur(self, a, r):
         a = self.f32(a).copy()

@@ -20,6 +20,7 @@
 #include "engine/gradients.h"
 #include "engine/lockin.h"
 #include "engine/motion.h"
+#include "engine/temporal.h"
 #include "engine/ntsc_geometry.h"
 #include "engine/plane.h"
 #include "engine/variational.h"
@@ -197,6 +198,65 @@ void ref_estimate_motion(const float* y_ref, const float* y_cur, int h, int w, i
   std::memcpy(dy, m.dy.data(), sizeof(float) * m.dy.size());
   std::memcpy(dx, m.dx.data(), sizeof(float) * m.dx.size());
   std::memcpy(conf, m.confidence.data(), sizeof(float) * m.confidence.size());
+}
+
+static hvd::PerPixelMotion ref_vpp(const float* mdy, const float* mdx, int th, int tw, int tile, int h, int w) {
+  return hvd::VectorsPerPixel(ToPlane(mdy, th, tw), ToPlane(mdx, th, tw), tile, h, w);
+}
+
+void ref_vectors_per_pixel(const float* mdy, const float* mdx, int th, int tw, int tile, int h, int w, float* dy,
+                           float* dx) {
+  const hvd::PerPixelMotion v = ref_vpp(mdy, mdx, th, tw, tile, h, w);
+  std::memcpy(dy, v.dy.data(), sizeof(float) * v.dy.size());
+  std::memcpy(dx, v.dx.data(), sizeof(float) * v.dx.size());
+}
+
+void ref_warp_by_tiles(const float* a, int h, int w, int ch, const float* mdy, const float* mdx, int th, int tw,
+                       int tile, float* out) {
+  if (ch == 2) {
+    const hvd::ComplexPlane o = hvd::WarpByTilesComplex(ToCPlane(a, h, w), ToPlane(mdy, th, tw),
+                                                        ToPlane(mdx, th, tw), tile);
+    std::memcpy(out, o.data(), sizeof(float) * 2 * o.size());
+  } else {
+    const hvd::Plane o = hvd::WarpByTiles(ToPlane(a, h, w), ToPlane(mdy, th, tw), ToPlane(mdx, th, tw), tile);
+    std::memcpy(out, o.data(), sizeof(float) * o.size());
+  }
+}
+
+void ref_warp_bilinear(const float* a, int hs, int ws, const float* mdy, const float* mdx, int th, int tw, int tile,
+                       float row_offset, int out_h, int out_w, float* out) {
+  const hvd::Plane o = hvd::WarpBilinearTiles(ToPlane(a, hs, ws), ToPlane(mdy, th, tw), ToPlane(mdx, th, tw), tile,
+                                              row_offset, out_h, out_w);
+  std::memcpy(out, o.data(), sizeof(float) * o.size());
+}
+
+void ref_upsample_confidence(const float* conf, int th, int tw, int tile, int h, int w, int fast, float* out) {
+  const hvd::Plane o = fast ? hvd::UpsampleConfidenceFast(ToPlane(conf, th, tw), tile, h, w)
+                            : hvd::UpsampleConfidence(ToPlane(conf, th, tw), tile, h, w);
+  std::memcpy(out, o.data(), sizeof(float) * o.size());
+}
+
+void ref_complex_coherence(const float* z1, const float* z2, int h, int w, int r, float* out) {
+  const hvd::Plane o = hvd::ComplexCoherence(ToCPlane(z1, h, w), ToCPlane(z2, h, w), r);
+  std::memcpy(out, o.data(), sizeof(float) * o.size());
+}
+
+void ref_motion_compensate_prev(const float* comp, const float* car, int h, int w, const float* mdy,
+                                const float* mdx, const float* mconf, int th, int tw, int tile, int fast,
+                                float* out_comp, float* out_car, float* out_conf) {
+  hvd::NeighborRawState prev;
+  prev.luma = hvd::Plane(h, w);
+  prev.composite = ToPlane(comp, h, w);
+  prev.carrier = ToCPlane(car, h, w);
+  hvd::MotionField m;
+  m.dy = ToPlane(mdy, th, tw);
+  m.dx = ToPlane(mdx, th, tw);
+  m.confidence = ToPlane(mconf, th, tw);
+  const hvd::MotionCompensatedResult r =
+      hvd::MotionCompensatePrev(prev, hvd::Plane(h, w), tile, 8, &m, fast != 0, nullptr);
+  std::memcpy(out_comp, r.composite.data(), sizeof(float) * r.composite.size());
+  std::memcpy(out_car, r.carrier.data(), sizeof(float) * 2 * r.carrier.size());
+  std::memcpy(out_conf, r.confidence.data(), sizeof(float) * r.confidence.size());
 }
 
 }  // extern "C"
