@@ -241,6 +241,49 @@ def streaming_bench(torch, gpu, synth, eng, n_windows=64, window=8, unique=16):
             "cg_algorithmic_gbs": round(CG_BYTES_PER_PX_ITER * pxit / (cg_ms * 1e-3) / 1e9, 1)}
 
 
+def temporal_bench(torch, gpu, eng, reps=10):
+    """VariationalRefine with two temporal neighbours (the 3-D solve) on an
+    NTSC field's active grid, 263 x 758, the headline's solver settings
+    (4 IRLS stages x 25 CG iterations, every iteration run).  Fields/s and
+    the time of one CG iteration (six launches)."""
+    h, w, nn = 263, 758, 2
+    r, c = np.mgrid[0:h, 0:w].astype(np.float64)
+    car = np.exp(1j * (0.5 * np.pi * c + 0.3 * r))
+    y = 30 + 20 * np.sin(0.35 * (r + c))
+    chi = 10 * np.sin(0.4 * (c - r)) - 6j * np.cos(0.3 * (c + r))
+    s = y + (chi * car).real
+    chi0 = chi + 3 * np.sin(0.9 * c) * (1 - 1j)
+
+    def d(a):
+        return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to("cuda:0")
+
+    def cd(z):
+        return d(np.stack([z.real, z.imag], -1))
+
+    nbrs = []
+    for j in range(nn):
+        cw = car * np.exp(0.5j * np.pi * (j + 1))
+        nbrs.append((d(y + (chi * cw).real + 0.3 * np.sin(1.7 * r + j)), cd(cw), d(np.full((h, w), 0.8))))
+    ds, dcar, dchi0 = d(s), cd(car), cd(chi0)
+    dchi, dy = torch.empty_like(dchi0), torch.empty_like(ds)
+    p = gpu.SolverParams.make(**CFG0)
+    pl = gpu.Engine.plane(ds, dchi, dy, carrier=dcar)
+
+    def one(stats=False):
+        dchi.copy_(dchi0)
+        return eng.variational_refine_temporal(pl, nbrs, p, 0.5, 2.0, want_stats=stats)
+
+    iters = int(one(True).sum())  # iterations the solve actually runs (it is deterministic)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        one()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    return {"fields_per_s": round(1.0 / dt, 1), "ms_per_field": round(1e3 * dt, 3),
+            "us_per_cg_iteration": round(1e6 * dt / max(iters, 1), 2), "neighbours": nn, "cg_iterations": iters}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -383,6 +426,10 @@ def main():
         log("[bench] training: 16 NTSC fields, 200 Adam steps on log lambda")
         result["training_16ntsc_adam200"] = training_bench(torch, gpu, synth, eng)
         log(f"[bench] training: {result['training_16ntsc_adam200']}")
+    if not args.only_batched:
+        log("[bench] temporal: NTSC-sized field with 2 temporal neighbours")
+        result["temporal_2nbr_ntsc758x263"] = temporal_bench(torch, gpu, eng)
+        log(f"[bench] temporal: {result['temporal_2nbr_ntsc758x263']}")
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (hvd-core reference build)")
         cb = cpu_baseline(torch, gpu, synth, eng)

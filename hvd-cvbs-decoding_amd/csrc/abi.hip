@@ -2385,6 +2385,177 @@ int hvd_gpu_box_blur(hvd_gpu_ctx* ctx, const float* in, int h, int w, int r, flo
   });
 }
 
+}  // extern "C"
+
+namespace {
+
+void check_tiles(hvd_gpu_ctx* ctx, const void* mdy, const void* mdx, int th, int tw, int tile) {
+  REQUIRE(ctx, mdy && mdx, "null motion field");
+  REQUIRE(ctx, th >= 1 && tw >= 1 && tile >= 1, "bad motion grid %d x %d, tile %d", th, tw, tile);
+}
+
+// VectorsPerPixel (temporal.cpp:44-113) into dy, dx (dx may be null), the
+// snapped tile vectors in workspace bytes [o_tiles, o_tiles + 8 th tw).
+void vectors_per_pixel(hvd_gpu_ctx* ctx, const float* mdy, const float* mdx, int th, int tw, int tile, int out_h,
+                       int out_w, float* dy, float* dx, float* sy, float* sx, hipStream_t st) {
+  hipLaunchKernelGGL(k_vpp_snap, dim3(blocks_for((size_t)th * tw)), dim3(256), 0, st, th, tw, mdy, mdx, sy, sx);
+  LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_vpp_interp, dim3(blocks_for((size_t)out_h * out_w)), dim3(256), 0, st, th, tw, tile, out_h,
+                     out_w, (const float*)sy, (const float*)sx, dy, dx);
+  LAUNCH_CHECK(ctx);
+}
+
+// Box blur of npairs * 2 planes laid out plane after plane at `in` (radius
+// r), result at `out`; `tmp` holds as many planes.
+void blur_planes(hvd_gpu_ctx* ctx, const std::vector<MotionDesc>& pairs, int h, int w, int r, float* tmp, float* out,
+                 hipStream_t st) {
+  Stage stg;
+  const size_t oD = stg.put(pairs);
+  Slot* sl;
+  char* base = upload(ctx, stg, st, &sl);
+  const int np = (int)pairs.size();
+  hipLaunchKernelGGL(k_me_blur_cols, dim3(blocks_for((size_t)np * 2 * w)), dim3(256), 0, st,
+                     (const MotionDesc*)(base + oD), np, h, w, r, tmp);
+  LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_me_blur_rows, dim3(blocks_for((size_t)np * 2 * h)), dim3(256), 0, st, np, h, w, r,
+                     (const float*)tmp, out);
+  LAUNCH_CHECK(ctx);
+  arm(ctx, sl, st);
+}
+
+// UpsampleConfidence[Fast] (temporal.cpp:217-249) into out; scratch from ws
+// at `o` (4 planes of out_h x out_w and 2 tile planes).
+void upsample_confidence(hvd_gpu_ctx* ctx, const float* conf, int th, int tw, int tile, int out_h, int out_w,
+                         bool fast, float* out, char* ws, Carve& cv, hipStream_t st) {
+  const size_t np = (size_t)out_h * out_w, nt = (size_t)th * tw;
+  if (fast) {
+    float* c2 = (float*)(ws + cv.take(sizeof(float) * nt));
+    float* sy = (float*)(ws + cv.take(sizeof(float) * nt));
+    float* sx = (float*)(ws + cv.take(sizeof(float) * nt));
+    hipLaunchKernelGGL(k_square, dim3(blocks_for(nt)), dim3(256), 0, st, (int)nt, conf, c2);
+    LAUNCH_CHECK(ctx);
+    vectors_per_pixel(ctx, c2, c2, th, tw, tile, out_h, out_w, out, nullptr, sy, sx, st);
+    return;
+  }
+  float* px = (float*)(ws + cv.take(sizeof(float) * 2 * np));  // the plane and a zero partner
+  float* tmp = (float*)(ws + cv.take(sizeof(float) * 2 * np));
+  float* bl = (float*)(ws + cv.take(sizeof(float) * 2 * np));
+  hipLaunchKernelGGL(k_conf_nearest, dim3(blocks_for(np)), dim3(256), 0, st, th, tw, tile, out_h, out_w, conf, px);
+  LAUNCH_CHECK(ctx);
+  blur_planes(ctx, {MotionDesc{px, px}}, out_h, out_w, 8, tmp, bl, st);
+  HIPCHK(ctx, hipMemcpyAsync(out, bl, sizeof(float) * np, hipMemcpyDeviceToDevice, st));
+}
+
+}  // namespace
+
+extern "C" {
+
+int hvd_gpu_vectors_per_pixel(hvd_gpu_ctx* ctx, const float* mdy, const float* mdx, int tiles_h, int tiles_w, int tile,
+                              int out_h, int out_w, float* dy, float* dx, void* stream) {
+  return guarded(ctx, [&] {
+    check_tiles(ctx, mdy, mdx, tiles_h, tiles_w, tile);
+    REQUIRE(ctx, dy && dx && out_h >= 1 && out_w >= 1, "bad output");
+    hipStream_t st = S(stream);
+    Carve cv;
+    const size_t nt = (size_t)tiles_h * tiles_w;
+    const size_t o_y = cv.take(sizeof(float) * nt), o_x = cv.take(sizeof(float) * nt);
+    char* ws = ensure_ws(ctx, cv.off, st);
+    vectors_per_pixel(ctx, mdy, mdx, tiles_h, tiles_w, tile, out_h, out_w, dy, dx, (float*)(ws + o_y),
+                      (float*)(ws + o_x), st);
+  });
+}
+
+int hvd_gpu_warp_by_tiles(hvd_gpu_ctx* ctx, const float* a, int h, int w, int channels, const float* dy,
+                          const float* dx, float* out, void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, a && dy && dx && out, "null argument");
+    REQUIRE(ctx, h >= 1 && w >= 1 && (channels == 1 || channels == 2), "bad warp %d x %d x %d", h, w, channels);
+    REQUIRE(ctx, out != a, "the warp cannot run in place");
+    const size_t n = (size_t)h * w;
+    hipLaunchKernelGGL(k_warp_tiles, dim3(blocks_for(n)), dim3(256), 0, S(stream), h, w, channels, a, dy, dx, out);
+    LAUNCH_CHECK(ctx);
+  });
+}
+
+int hvd_gpu_warp_bilinear(hvd_gpu_ctx* ctx, const float* a, int src_h, int src_w, const float* dy, const float* dx,
+                          float row_offset, int out_h, int out_w, float* out, void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, a && dy && dx && out && out != a, "null or aliased argument");
+    REQUIRE(ctx, src_h >= 1 && src_w >= 1 && out_h >= 1 && out_w >= 1, "empty plane");
+    const size_t n = (size_t)out_h * out_w;
+    hipLaunchKernelGGL(k_warp_bilinear, dim3(blocks_for(n)), dim3(256), 0, S(stream), src_h, src_w, a, dy, dx,
+                       row_offset, out_h, out_w, out);
+    LAUNCH_CHECK(ctx);
+  });
+}
+
+int hvd_gpu_upsample_confidence(hvd_gpu_ctx* ctx, const float* conf, int tiles_h, int tiles_w, int tile, int out_h,
+                                int out_w, int fast, float* out, void* stream) {
+  return guarded(ctx, [&] {
+    check_tiles(ctx, conf, conf, tiles_h, tiles_w, tile);
+    REQUIRE(ctx, out && out_h >= 1 && out_w >= 1, "bad output");
+    hipStream_t st = S(stream);
+    const size_t np = (size_t)out_h * out_w, nt = (size_t)tiles_h * tiles_w;
+    char* ws = ensure_ws(ctx, 6 * align256(sizeof(float) * 2 * np) + 3 * align256(sizeof(float) * nt), st);
+    Carve cv;
+    upsample_confidence(ctx, conf, tiles_h, tiles_w, tile, out_h, out_w, fast != 0, out, ws, cv, st);
+  });
+}
+
+int hvd_gpu_complex_coherence(hvd_gpu_ctx* ctx, const float* z1, const float* z2, int h, int w, int r, float* out,
+                              void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, z1 && z2 && out, "null argument");
+    REQUIRE(ctx, h >= 1 && w >= 1 && r >= 0, "bad coherence %d x %d, radius %d", h, w, r);
+    hipStream_t st = S(stream);
+    const size_t n = (size_t)h * w;
+    Carve cv;
+    const size_t o_in = cv.take(sizeof(float) * 4 * n), o_tmp = cv.take(sizeof(float) * 4 * n),
+                 o_bl = cv.take(sizeof(float) * 4 * n);
+    char* ws = ensure_ws(ctx, cv.off, st);
+    float* in = (float*)(ws + o_in);
+    float* bl = (float*)(ws + o_bl);
+    hipLaunchKernelGGL(k_coh_terms, dim3(blocks_for(n)), dim3(256), 0, st, n, (const float2*)z1, (const float2*)z2,
+                       in, in + n, in + 2 * n, in + 3 * n);
+    LAUNCH_CHECK(ctx);
+    blur_planes(ctx, {MotionDesc{in, in + n}, MotionDesc{in + 2 * n, in + 3 * n}}, h, w, r, (float*)(ws + o_tmp), bl,
+                st);
+    hipLaunchKernelGGL(k_coh_out, dim3(blocks_for(n)), dim3(256), 0, st, n, (const float*)bl,
+                       (const float*)(bl + n), (const float*)(bl + 2 * n), (const float*)(bl + 3 * n), out);
+    LAUNCH_CHECK(ctx);
+  });
+}
+
+int hvd_gpu_motion_compensate_prev(hvd_gpu_ctx* ctx, const float* composite, const float* carrier, int h, int w,
+                                   const float* mdy, const float* mdx, const float* mconf, int tiles_h, int tiles_w,
+                                   int tile, int fast, float* out_composite, float* out_carrier, float* out_conf,
+                                   void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, composite && carrier && mconf && out_composite && out_carrier && out_conf, "null argument");
+    REQUIRE(ctx, out_composite != composite && out_carrier != carrier, "the warp cannot run in place");
+    REQUIRE(ctx, h >= 1 && w >= 1, "empty plane");
+    check_tiles(ctx, mdy, mdx, tiles_h, tiles_w, tile);
+    hipStream_t st = S(stream);
+    const size_t n = (size_t)h * w, nt = (size_t)tiles_h * tiles_w;
+    Carve cv;
+    const size_t o_dy = cv.take(sizeof(float) * n), o_dx = cv.take(sizeof(float) * n),
+                 o_sy = cv.take(sizeof(float) * nt), o_sx = cv.take(sizeof(float) * nt);
+    char* ws = ensure_ws(ctx, cv.off + 6 * align256(sizeof(float) * 2 * n) + 3 * align256(sizeof(float) * nt), st);
+    float* dy = (float*)(ws + o_dy);
+    float* dx = (float*)(ws + o_dx);
+    // one per-pixel interpolation shared by both warps (temporal.cpp:266-275)
+    vectors_per_pixel(ctx, mdy, mdx, tiles_h, tiles_w, tile, h, w, dy, dx, (float*)(ws + o_sy), (float*)(ws + o_sx),
+                      st);
+    hipLaunchKernelGGL(k_warp_tiles, dim3(blocks_for(n)), dim3(256), 0, st, h, w, 1, composite, (const float*)dy,
+                       (const float*)dx, out_composite);
+    LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_warp_tiles, dim3(blocks_for(n)), dim3(256), 0, st, h, w, 2, carrier, (const float*)dy,
+                       (const float*)dx, out_carrier);
+    LAUNCH_CHECK(ctx);
+    upsample_confidence(ctx, mconf, tiles_h, tiles_w, tile, h, w, fast != 0, out_conf, ws, cv, st);
+  });
+}
+
 int hvd_gpu_chroma_plane(hvd_gpu_ctx* ctx, int mode, const float* s, const float* y, const float* chi,
                          const float* carrier, size_t n, float* out, void* stream) {
   return guarded(ctx, [&] {

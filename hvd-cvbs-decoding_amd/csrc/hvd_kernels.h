@@ -192,6 +192,19 @@ __global__ void k_t_dir(TemporalField T);
 __global__ void k_t_stage_end(TemporalField T, int stage);
 __global__ void k_t_finalize(TemporalField T);
 
+// warp.hip
+__global__ void k_vpp_snap(int th, int tw, const float* mdy, const float* mdx, float* sy, float* sx);
+__global__ void k_vpp_interp(int th, int tw, int tile, int out_h, int out_w, const float* sy, const float* sx,
+                             float* dy, float* dx);
+__global__ void k_warp_tiles(int h, int w, int ch, const float* a, const float* dy, const float* dx, float* out);
+__global__ void k_warp_bilinear(int hs, int ws, const float* a, const float* dy, const float* dx, float row_offset,
+                                int out_h, int out_w, float* out);
+__global__ void k_conf_nearest(int th, int tw, int tile, int out_h, int out_w, const float* conf, float* out);
+__global__ void k_square(int n, const float* a, float* out);
+__global__ void k_coh_terms(size_t n, const float2* z1, const float2* z2, float* cre, float* cim, float* m1,
+                            float* m2);
+__global__ void k_coh_out(size_t n, const float* bre, const float* bim, const float* b1, const float* b2, float* out);
+
 // bridge.hip
 __global__ void k_split_fields(const BridgeFrameParams* FP, const int16_t* const* frames, float zero,
                                float* const* fields);

@@ -26,7 +26,9 @@ ABI_SYMBOLS = (
     "hvd_gpu_make_carrier", "hvd_gpu_tridiag_smooth", "hvd_gpu_box_blur", "hvd_gpu_chroma_plane",
     "hvd_gpu_weave_frame_s16", "hvd_gpu_variational_refine_stages", "hvd_gpu_refine_adjoint",
     "hvd_gpu_set_solver_tuning", "hvd_gpu_autotune_solver", "hvd_gpu_set_graphs",
-    "hvd_gpu_variational_refine_temporal",
+    "hvd_gpu_variational_refine_temporal", "hvd_gpu_vectors_per_pixel", "hvd_gpu_warp_by_tiles",
+    "hvd_gpu_warp_bilinear", "hvd_gpu_upsample_confidence", "hvd_gpu_complex_coherence",
+    "hvd_gpu_motion_compensate_prev",
 )
 
 
@@ -154,6 +156,12 @@ def load() -> C.CDLL:
         lib.hvd_gpu_weave_frame_s16.argtypes = [vp, vp, C.POINTER(FrameParams), i, i, vp, vp, vp, C.c_float,
                                                 C.POINTER(YcFrame), vp]
         lib.hvd_gpu_variational_refine_stages.argtypes = [vp, C.POINTER(Plane), i, C.POINTER(SolverParams), pp, vp]
+        lib.hvd_gpu_vectors_per_pixel.argtypes = [vp, vp, vp, i, i, i, i, i, vp, vp, vp]
+        lib.hvd_gpu_warp_by_tiles.argtypes = [vp, vp, i, i, i, vp, vp, vp, vp]
+        lib.hvd_gpu_warp_bilinear.argtypes = [vp, vp, i, i, vp, vp, C.c_float, i, i, vp, vp]
+        lib.hvd_gpu_upsample_confidence.argtypes = [vp, vp, i, i, i, i, i, i, vp, vp]
+        lib.hvd_gpu_complex_coherence.argtypes = [vp, vp, vp, i, i, i, vp, vp]
+        lib.hvd_gpu_motion_compensate_prev.argtypes = [vp, vp, vp, i, i, vp, vp, vp, i, i, i, i, vp, vp, vp, vp]
         lib.hvd_gpu_variational_refine_temporal.argtypes = [vp, C.POINTER(Plane), C.POINTER(Neighbor), i,
                                                             C.POINTER(SolverParams), C.c_float, C.c_float, vp, vp]
         lib.hvd_gpu_refine_adjoint.argtypes = [vp, C.POINTER(Plane), i, C.POINTER(SolverParams), pp, pp, pp, i,
@@ -320,6 +328,60 @@ class Engine:
         if want_stats:
             return np.array(iters[:], dtype=np.int32)
         return None
+
+    # ---- temporal neighbours' warps (temporal.cpp) ----------------------------
+    def vectors_per_pixel(self, mdy, mdx, tile, h, w):
+        import torch
+        dy = torch.empty((h, w), dtype=torch.float32, device=mdy.device)
+        dx = torch.empty_like(dy)
+        self._check(self.lib.hvd_gpu_vectors_per_pixel(self.ctx, mdy.data_ptr(), mdx.data_ptr(), mdy.shape[0],
+                                                       mdy.shape[1], tile, h, w, dy.data_ptr(), dx.data_ptr(),
+                                                       _stream()), "vectors_per_pixel")
+        return dy, dx
+
+    def warp_by_tiles(self, a, dy, dx):
+        """a: (h, w) real or (h, w, 2) interleaved complex float32."""
+        import torch
+        out = torch.empty_like(a)
+        ch = 2 if a.dim() == 3 else 1
+        self._check(self.lib.hvd_gpu_warp_by_tiles(self.ctx, a.data_ptr(), a.shape[0], a.shape[1], ch, dy.data_ptr(),
+                                                   dx.data_ptr(), out.data_ptr(), _stream()), "warp_by_tiles")
+        return out
+
+    def warp_bilinear(self, a, dy, dx, row_offset):
+        import torch
+        out = torch.empty(dy.shape, dtype=torch.float32, device=a.device)
+        self._check(self.lib.hvd_gpu_warp_bilinear(self.ctx, a.data_ptr(), a.shape[0], a.shape[1], dy.data_ptr(),
+                                                   dx.data_ptr(), float(row_offset), dy.shape[0], dy.shape[1],
+                                                   out.data_ptr(), _stream()), "warp_bilinear")
+        return out
+
+    def upsample_confidence(self, conf, tile, h, w, fast=False):
+        import torch
+        out = torch.empty((h, w), dtype=torch.float32, device=conf.device)
+        self._check(self.lib.hvd_gpu_upsample_confidence(self.ctx, conf.data_ptr(), conf.shape[0], conf.shape[1],
+                                                         tile, h, w, int(fast), out.data_ptr(), _stream()),
+                    "upsample_confidence")
+        return out
+
+    def complex_coherence(self, z1, z2, r):
+        import torch
+        out = torch.empty(z1.shape[:2], dtype=torch.float32, device=z1.device)
+        self._check(self.lib.hvd_gpu_complex_coherence(self.ctx, z1.data_ptr(), z2.data_ptr(), z1.shape[0],
+                                                       z1.shape[1], r, out.data_ptr(), _stream()),
+                    "complex_coherence")
+        return out
+
+    def motion_compensate_prev(self, comp, car, mdy, mdx, mconf, tile, fast=False):
+        """A neighbour's (composite, carrier (h, w, 2), confidence) warped by a
+        tile motion field (MotionCompensatePrev)."""
+        import torch
+        oc, ocar, ocf = torch.empty_like(comp), torch.empty_like(car), torch.empty_like(comp)
+        self._check(self.lib.hvd_gpu_motion_compensate_prev(
+            self.ctx, comp.data_ptr(), car.data_ptr(), comp.shape[0], comp.shape[1], mdy.data_ptr(), mdx.data_ptr(),
+            mconf.data_ptr(), mdy.shape[0], mdy.shape[1], tile, int(fast), oc.data_ptr(), ocar.data_ptr(),
+            ocf.data_ptr(), _stream()), "motion_compensate_prev")
+        return oc, ocar, ocf
 
     def variational_refine_stages(self, planes, params: SolverParams):
         """The solve, returning chi at the start of every IRLS stage: a list of

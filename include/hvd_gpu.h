@@ -262,6 +262,53 @@ int hvd_gpu_variational_refine_temporal(hvd_gpu_ctx* ctx, const hvd_gpu_plane* p
                                         int nn, const hvd_solver_params* p, float temporal_strength,
                                         float temporal_eps, int* stage_iters, void* stream);
 
+/* ---- the temporal neighbours' warps (temporal.h / temporal.cpp) -------- */
+/* Motion fields are hvd-core's per-tile MotionField (dy, dx, confidence of
+ * tiles_h x tiles_w tiles of `tile` pixels, as hvd_gpu_estimate_motion
+ * returns them); per-pixel vector planes are out_h x out_w.  All planes are
+ * device memory; the calls are asynchronous on `stream`. */
+
+/* VectorsPerPixel (temporal.h:35-36, temporal.cpp:44-113): 3x3-median
+ * outlier snap of the tile vectors, then bilinear interpolation between tile
+ * centres. */
+int hvd_gpu_vectors_per_pixel(hvd_gpu_ctx* ctx, const float* mdy, const float* mdx, int tiles_h, int tiles_w, int tile,
+                              int out_h, int out_w, float* dy, float* dx, void* stream);
+
+/* WarpByTiles / WarpByTilesComplex with precomputed per-pixel vectors
+ * (temporal.h:45-50, :83; temporal.cpp:117-132, :205-215): out(y, x) =
+ * a(clamp(y - round(dy)), clamp(x - round(dx))); channels 1 (real plane) or
+ * 2 (interleaved complex).  out must not alias a. */
+int hvd_gpu_warp_by_tiles(hvd_gpu_ctx* ctx, const float* a, int h, int w, int channels, const float* dy,
+                          const float* dx, float* out, void* stream);
+
+/* WarpBilinearTiles with precomputed vectors (temporal.h:59-61,
+ * temporal.cpp:140-166): sub-pixel warp of a baseband src_h x src_w plane onto
+ * out_h x out_w, with a constant extra vertical shift row_offset. */
+int hvd_gpu_warp_bilinear(hvd_gpu_ctx* ctx, const float* a, int src_h, int src_w, const float* dy, const float* dx,
+                          float row_offset, int out_h, int out_w, float* out, void* stream);
+
+/* UpsampleConfidence (fast == 0: nearest tile, squared, box blur radius 8)
+ * or UpsampleConfidenceFast (squared tile map through VectorsPerPixel)
+ * (temporal.h:90-101, temporal.cpp:217-249). */
+int hvd_gpu_upsample_confidence(hvd_gpu_ctx* ctx, const float* conf, int tiles_h, int tiles_w, int tile, int out_h,
+                                int out_w, int fast, float* out, void* stream);
+
+/* ComplexCoherence (temporal.h:68, temporal.cpp:169-196): |<z1 conj z2>| /
+ * (sqrt(<|z1|^2><|z2|^2>) + 1e-6) with box averages of radius r, in [0, 1];
+ * z1, z2 interleaved complex h x w. */
+int hvd_gpu_complex_coherence(hvd_gpu_ctx* ctx, const float* z1, const float* z2, int h, int w, int r, float* out,
+                              void* stream);
+
+/* MotionCompensatePrev with a given motion field (temporal.h:139-142,
+ * temporal.cpp:253-287): one per-pixel interpolation shared by the
+ * integer-pel warps of the neighbour's composite and carrier (interleaved
+ * complex), and the confidence upsample.  The results are a
+ * hvd_gpu_neighbor's three planes. */
+int hvd_gpu_motion_compensate_prev(hvd_gpu_ctx* ctx, const float* composite, const float* carrier, int h, int w,
+                                   const float* mdy, const float* mdx, const float* mconf, int tiles_h, int tiles_w,
+                                   int tile, int fast, float* out_composite, float* out_carrier, float* out_conf,
+                                   void* stream);
+
 /* ---- differentiation of the solve (no hvd-core counterpart) ------------ */
 
 /* hvd_gpu_variational_refine that also saves chi at the start of each IRLS
