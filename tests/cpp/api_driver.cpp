@@ -16,6 +16,10 @@
 //       irls_outer 4, slow mode, chroma_aniso 0.7) on dir/s, dir/car,
 //       dir/chi0, dir/sw, dir/cw, dir/conf (f32, complex interleaved, the
 //       neighbours' planes one after another); out dir/out: luma then chi.
+//   api_driver mcprev <ntsc|pal> <h> <w> <th> <tw> <tile> <fast> <dir>
+//       MotionCompensatePrev with the given tile motion field (dir/comp,
+//       dir/car, dir/mdy, dir/mdx, dir/mconf) and ComplexCoherence(r = 3) of
+//       dir/z1, dir/z2; out dir/out: composite, carrier, confidence, gamma.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -25,6 +29,7 @@
 #include "engine/engine.h"
 #include "engine/gpu_status.h"
 #include "engine/sequence.h"
+#include "engine/temporal.h"
 #include "engine/variational.h"
 #include "frame_bridge.h"
 
@@ -225,6 +230,47 @@ int main(int argc, char** argv) {
     Out o((dir + "/out").c_str());
     o.put(r.luma.data(), n);
     o.put(reinterpret_cast<const float*>(r.chroma.data()), 2 * n);
+    return 0;
+  }
+  if (mode == "mcprev" && argc == 10) {
+    const int h = std::atoi(argv[3]), w = std::atoi(argv[4]), th = std::atoi(argv[5]), tw = std::atoi(argv[6]);
+    const int tile = std::atoi(argv[7]);
+    const bool fast = std::atoi(argv[8]) != 0;
+    const std::string dir = argv[9];
+    const size_t n = (size_t)h * w, nt = (size_t)th * tw;
+    auto load = [&](const char* name, int ph, int pw, int ch) {
+      const std::vector<float> v = ReadAll<float>((dir + "/" + name).c_str(), (size_t)ph * pw * ch);
+      return v;
+    };
+    auto plane = [](const std::vector<float>& v, int ph, int pw) {
+      hvd::Plane p(ph, pw);
+      std::memcpy(p.data(), v.data(), sizeof(float) * p.size());
+      return p;
+    };
+    auto cplane = [](const std::vector<float>& v, int ph, int pw) {
+      hvd::ComplexPlane p(ph, pw);
+      std::memcpy(reinterpret_cast<float*>(p.data()), v.data(), sizeof(float) * 2 * p.size());
+      return p;
+    };
+    hvd::NeighborRawState prev;
+    prev.luma = hvd::Plane(h, w);
+    prev.composite = plane(load("comp", h, w, 1), h, w);
+    prev.carrier = cplane(load("car", h, w, 2), h, w);
+    hvd::MotionField m;
+    m.tile = tile;
+    m.dy = plane(load("mdy", th, tw, 1), th, tw);
+    m.dx = plane(load("mdx", th, tw, 1), th, tw);
+    m.confidence = plane(load("mconf", th, tw, 1), th, tw);
+    const hvd::MotionCompensatedResult r = hvd::MotionCompensatePrev(prev, hvd::Plane(h, w), tile, 8, &m, fast);
+    const hvd::Plane g =
+        hvd::ComplexCoherence(cplane(load("z1", h, w, 2), h, w), cplane(load("z2", h, w, 2), h, w), 3);
+    if (r.composite.empty() || g.empty()) return Fail();
+    (void)nt;
+    Out o((dir + "/out").c_str());
+    o.put(r.composite.data(), n);
+    o.put(reinterpret_cast<const float*>(r.carrier.data()), 2 * n);
+    o.put(r.confidence.data(), n);
+    o.put(g.data(), n);
     return 0;
   }
   std::fprintf(stderr, "usage: see the header of api_driver.cpp\n");

@@ -250,3 +250,38 @@ def test_variational_refine_neighbours(tmp_path, orc64):
     y_ref, c_ref, _, _ = orc64.variational_refine_nbr(s, car, chi0, cfg, nbrs, nu, eps_t)
     assert np.abs(y - y_ref).max() <= TOL_IRE
     assert np.abs(chi - c_ref).max() <= TOL_IRE
+
+
+@pytest.mark.parametrize("fast", [False, True])
+def test_motion_compensate_prev_and_coherence(tmp_path, orc32, fast):
+    """hvd::MotionCompensatePrev with a given motion field and
+    hvd::ComplexCoherence (temporal.cpp:169-196, :253-287) through the C++
+    API: hvd-core's FP32 bits (the FP32 oracle is bit-exact with them)."""
+    from warp_cases import motion_field, planes
+    h, w, tile = 263, 758, 32
+    th, tw = -(-h // tile), -(-w // tile)
+    mdy, mdx, conf = motion_field(th, tw, 31)
+    comp, car, z1 = planes(h, w, 32)
+    _, _, z2 = planes(h, w, 33)
+
+    def cf(z):
+        return np.stack([z.real, z.imag], -1).astype(np.float32)
+
+    comp.tofile(tmp_path / "comp")
+    cf(car).tofile(tmp_path / "car")
+    for name, a in (("mdy", mdy), ("mdx", mdx), ("mconf", conf)):
+        a.astype(np.float32).tofile(tmp_path / name)
+    cf(z1).tofile(tmp_path / "z1")
+    cf(z2).tofile(tmp_path / "z2")
+    run(["mcprev", "ntsc", h, w, th, tw, tile, int(fast), tmp_path])
+    out = np.fromfile(tmp_path / "out", np.float32)
+    n = h * w
+    oc = out[:n].reshape(h, w)
+    ocar = out[n:3 * n].reshape(h, w, 2)
+    ocf = out[3 * n:4 * n].reshape(h, w)
+    g = out[4 * n:].reshape(h, w)
+    rc, rcar, rcf = orc32.motion_compensate_prev(comp, car, mdy, mdx, conf, tile, fast)
+    np.testing.assert_array_equal(oc, rc.astype(np.float32))
+    np.testing.assert_array_equal(ocar[..., 0] + 1j * ocar[..., 1], rcar.astype(np.complex64))
+    np.testing.assert_array_equal(ocf, rcf.astype(np.float32))
+    np.testing.assert_array_equal(g, orc32.complex_coherence(z1, z2, 3).astype(np.float32))
