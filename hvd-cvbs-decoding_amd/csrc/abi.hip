@@ -1483,10 +1483,11 @@ void refine_temporal(hvd_gpu_ctx* ctx, const hvd_gpu_plane& pl, const hvd_gpu_ne
   const size_t o_st = cv.take(sizeof(FieldCG)), o_cg = cv.take(sizeof(TemporalCG));
   const size_t o_gx = cv.take(sizeof(float) * (ns + 1)), o_gy = cv.take(sizeof(float) * (ns + 1));
   const size_t o_g = cv.take(sizeof(float2) * np), o_d = cv.take(sizeof(float2) * np),
+               o_d2 = cv.take(sizeof(float2) * np), o_chi2 = cv.take(sizeof(float2) * np),
                o_w = cv.take(sizeof(float4) * np);
   std::vector<size_t> o_nb(nn);
   for (int j = 0; j < nn; ++j) o_nb[j] = cv.take(sizeof(float4) * np);
-  const size_t o_part = cv.take(sizeof(double) * 2 * nblk);
+  const size_t o_part = cv.take(sizeof(double) * 2 * nblk), o_part2 = cv.take(sizeof(double) * 2 * nblk);
   char* ws = ensure_ws(ctx, cv.off, st);
   DevField f{};
   f.h = pl.lines;
@@ -1543,29 +1544,36 @@ void refine_temporal(hvd_gpu_ctx* ctx, const hvd_gpu_plane& pl, const hvd_gpu_ne
   T.nu = nu;
   T.et2 = eps_t * eps_t;
   T.part = (double*)(ws + o_part);
+  T.part2 = (double*)(ws + o_part2);
   T.cg = (TemporalCG*)(ws + o_cg);
   T.st = dstate;
   const dim3 grid(nblk), blk(kTemporalThreads);
   auto launch = [&](const void* fn, dim3 g, std::vector<void*> args) {
     HIPCHK(ctx, hipLaunchKernel(fn, g, blk, args.data(), 0, st));
   };
-  int start = 1, cont = 0;
+  // chi and d ping-pong between the caller's / a workspace plane: two
+  // launches per CG iteration, each reading one and writing the other
+  float2* C[2] = {f.chi[0], (float2*)(ws + o_chi2)};
+  float2* D[2] = {(float2*)(ws + o_d), (float2*)(ws + o_d2)};
+  int k = 0, dc = 0, start = 1;
   launch((const void*)k_t_prep, grid, {&T, &in});
   for (int o = 0; o < outer; ++o) {
-    launch((const void*)k_t_stage_begin, dim3(1), {&T});
-    launch((const void*)k_t_weights, grid, {&T});
-    launch((const void*)k_t_grad, grid, {&T, &start});
-    launch((const void*)k_t_sum_grad, dim3(1), {&T, (void*)&nblk, &start});
+    TemporalField Ts = T;  // the stage's planes
+    Ts.f.chi[0] = Ts.f.chi[1] = C[k];
+    Ts.d = D[dc];
+    launch((const void*)k_t_stage_begin, dim3(1), {&Ts});
+    launch((const void*)k_t_weights, grid, {&Ts});
+    launch((const void*)k_t_grad, grid, {&Ts, &start});
+    launch((const void*)k_t_sum_start, dim3(1), {&Ts, (void*)&nblk});
     for (int it = 0; it < n_inner; ++it) {
-      launch((const void*)k_t_curv, grid, {&T});
-      launch((const void*)k_t_alpha, dim3(1), {&T, (void*)&nblk});
-      launch((const void*)k_t_move, grid, {&T});
-      launch((const void*)k_t_grad, grid, {&T, &cont});
-      launch((const void*)k_t_sum_grad, dim3(1), {&T, (void*)&nblk, &cont});
-      launch((const void*)k_t_dir, grid, {&T});
+      launch((const void*)k_t_dir_curv, grid, {&Ts, (void*)&nblk, &it, &D[dc], &D[dc ^ 1]});
+      dc ^= 1;
+      launch((const void*)k_t_step, grid, {&Ts, (void*)&nblk, &C[k], &C[k ^ 1], &D[dc]});
+      k ^= 1;
     }
-    launch((const void*)k_t_stage_end, dim3(1), {&T, &o});
+    launch((const void*)k_t_stage_end, dim3(1), {&Ts, &o});
   }
+  if (k) HIPCHK(ctx, hipMemcpyAsync(C[0], C[1], sizeof(float2) * np, hipMemcpyDeviceToDevice, st));
   launch((const void*)k_t_finalize, grid, {&T});
   arm(ctx, sl, st);
   if (stage_iters) {

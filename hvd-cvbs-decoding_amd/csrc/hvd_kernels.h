@@ -154,7 +154,8 @@ constexpr int kTemporalThreads = 256;
 
 // Scalar CG state of the temporal solve (variational.cpp:519-549).
 struct TemporalCG {
-  double gg, gg0;
+  double gg0;
+  double ggs[2];            // |g|^2 by iteration parity (a launch reads one slot, writes the other)
   float alpha, beta;
   int it, done;
   int iters[16];
@@ -170,7 +171,8 @@ struct TemporalField {
   int nn;
   float nu, et2;
   int pad_;
-  double* part;             // two per block
+  double* part;             // two per block: curvature and Re<g, d> (or |g|^2 at a stage start)
+  double* part2;            // |g'|^2 of a step, at stride two per block
   TemporalCG* cg;
   const FieldCG* st;        // mu_h, mu_v, lam_y, eps, eps_c, k, exits
 };
@@ -184,11 +186,9 @@ __global__ void k_t_prep(TemporalField T, TemporalInputs in);
 __global__ void k_t_stage_begin(TemporalField T);
 __global__ void k_t_weights(TemporalField T);
 __global__ void k_t_grad(TemporalField T, int start);
-__global__ void k_t_sum_grad(TemporalField T, int nblocks, int start);
-__global__ void k_t_curv(TemporalField T);
-__global__ void k_t_alpha(TemporalField T, int nblocks);
-__global__ void k_t_move(TemporalField T);
-__global__ void k_t_dir(TemporalField T);
+__global__ void k_t_sum_start(TemporalField T, int nblocks);
+__global__ void k_t_dir_curv(TemporalField T, int nblocks, int it, const float2* d_in, float2* d_out);
+__global__ void k_t_step(TemporalField T, int nblocks, const float2* chi_in, float2* chi_out, const float2* d);
 __global__ void k_t_stage_end(TemporalField T, int stage);
 __global__ void k_t_finalize(TemporalField T);
 

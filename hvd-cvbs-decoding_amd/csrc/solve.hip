@@ -1048,7 +1048,8 @@ __global__ void __launch_bounds__(64) k_refine_finalize(const DevField* __restri
 namespace {
 
 // S - Re[chi c] and chi of pixel (y, x); zeros outside the plane.
-__device__ inline Sample t_sample(const DevField& f, const float2* chi, int y, int x) {
+__device__ inline Sample t_sample(const DevField& f, const float2* chi, int y, int x, const float2* d = nullptr,
+                                  float a = 0.f) {
 #pragma clang fp contract(off)
   Sample p;
   if (y < 0 || y >= f.h || x < 0 || x >= f.w) {
@@ -1060,6 +1061,11 @@ __device__ inline Sample t_sample(const DevField& f, const float2* chi, int y, i
   const size_t i = (size_t)y * f.w + x;
   p.car = carrier_at(f, y, x);
   p.chi = chi[i];
+  if (d) {  // chi + alpha d, as the step stores it (variational.cpp:533-534)
+    const float2 di = d[i];
+    p.chi.x += a * di.x;
+    p.chi.y += a * di.y;
+  }
   p.yc = f.s[i] - (p.chi.x * p.car.x - p.chi.y * p.car.y);
   return p;
 }
@@ -1072,12 +1078,14 @@ __device__ inline float4 t_wts(const TemporalField& T, int y, int x) {
 // grad at chi (:286-347): the 2-D terms from the fluxes across the pixel's
 // four edges, then 2 nu wt rt conj(dc) per neighbour.  Weights of edges that
 // leave the plane are 0, as are hvd-core's differences there.
-__device__ inline float2 t_grad_at(const TemporalField& T, const FieldCG& s, int y, int x) {
+__device__ inline float2 t_grad_at(const TemporalField& T, const FieldCG& s, const float2* chi, const float2* d,
+                                   float a, int y, int x, float2* chi_out = nullptr) {
 #pragma clang fp contract(off)
   const DevField& f = T.f;
-  const float2* chi = f.chi[0];
-  const Sample p0 = t_sample(f, chi, y, x), pl = t_sample(f, chi, y, x - 1), pr = t_sample(f, chi, y, x + 1),
-               pu = t_sample(f, chi, y - 1, x), pd = t_sample(f, chi, y + 1, x);
+  const Sample p0 = t_sample(f, chi, y, x, d, a), pl = t_sample(f, chi, y, x - 1, d, a),
+               pr = t_sample(f, chi, y, x + 1, d, a), pu = t_sample(f, chi, y - 1, x, d, a),
+               pd = t_sample(f, chi, y + 1, x, d, a);
+  if (chi_out) *chi_out = p0.chi;
   const float4 w0 = t_wts(T, y, x), wl = t_wts(T, y, x - 1), wu = t_wts(T, y - 1, x);
   const float fx0 = w0.x * (pr.yc - p0.yc), fxl = wl.x * (p0.yc - pl.yc);
   const float fy0 = w0.y * (pd.yc - p0.yc), fyu = wu.y * (p0.yc - pu.yc);
@@ -1106,6 +1114,16 @@ __device__ inline double t_sum(const double* part, int nblocks, int k, double* l
   double v = 0.0;
   for (int b = threadIdx.x; b < nblocks; b += kTemporalThreads) v += part[2 * b + k];
   return block_sum<kTemporalThreads>(v, lds);
+}
+
+// The same sum in every thread of the block.
+__device__ inline double t_sum_all(const double* part, int nblocks, int k, double* lds, double* bcast) {
+  const double r = t_sum(part, nblocks, k, lds);
+  if (threadIdx.x == 0) *bcast = r;
+  __syncthreads();
+  const double v = *bcast;
+  __syncthreads();
+  return v;
 }
 
 }  // namespace
@@ -1168,7 +1186,7 @@ __global__ void __launch_bounds__(kTemporalThreads) k_t_grad(TemporalField T, in
   double gg = 0.0;
   if (i < n) {
     const FieldCG s = *T.st;
-    const float2 g = t_grad_at(T, s, (int)(i / T.f.w), (int)(i % T.f.w));
+    const float2 g = t_grad_at(T, s, T.f.chi[0], nullptr, 0.f, (int)(i / T.f.w), (int)(i % T.f.w));
     T.g[i] = g;
     if (start) T.d[i] = make_float2(-g.x, -g.y);
     gg = (double)g.x * g.x + (double)g.y * g.y;
@@ -1177,42 +1195,61 @@ __global__ void __launch_bounds__(kTemporalThreads) k_t_grad(TemporalField T, in
   if (threadIdx.x == 0) T.part[2 * blockIdx.x] = b;
 }
 
-// |g|^2 of the grad pass: at a stage start gg = gg0; after a step
-// beta = |g'|^2 / |g|^2 and the two exits (:538-547).
-__global__ void __launch_bounds__(kTemporalThreads) k_t_sum_grad(TemporalField T, int nblocks, int start) {
+// |g|^2 of the stage-start gradient: gg0, and the slot iteration 1 reads.
+__global__ void __launch_bounds__(kTemporalThreads) k_t_sum_start(TemporalField T, int nblocks) {
   __shared__ double lds[kTemporalThreads / 64];
-  TemporalCG& cg = *T.cg;
-  if (!start && cg.done) return;
   const double gg = t_sum(T.part, nblocks, 0, lds);
-  if (threadIdx.x != 0) return;
-  if (start) {
-    cg.gg = gg;
-    cg.gg0 = gg;
-    return;
+  if (threadIdx.x == 0) {
+    T.cg->gg0 = gg;
+    T.cg->ggs[0] = gg;
   }
-  const FieldCG& s = *T.st;
-  cg.beta = (float)(gg / (cg.gg > 1e-30 ? cg.gg : 1e-30));
-  cg.gg = gg;
-  cg.it += 1;
-  if (gg < s.tol2 * cg.gg0 || gg < s.gg_floor * s.npix) cg.done = 1;
 }
 
-// curv(d) (:352-399) and Re<g, d>, block partials.
-__global__ void __launch_bounds__(kTemporalThreads) k_t_curv(TemporalField T) {
+// First launch of CG iteration `it` (variational.cpp:539-547 of the previous
+// iteration, then :527-532): every block sums the previous gradient pass's
+// |g'|^2 partials in the same order, so all take the same beta and exits;
+// then d = -g' + beta d_prev (d = -g as stored at it = 0) into d_out, and the
+// block partials of curv(d) and Re<g', d>.  Block 0 publishes beta, gg (in
+// slot it & 1: this launch reads the other) and the iteration count.
+__global__ void __launch_bounds__(kTemporalThreads) k_t_dir_curv(TemporalField T, int nblocks, int it,
+                                                                 const float2* __restrict__ d_in,
+                                                                 float2* __restrict__ d_out) {
 #pragma clang fp contract(off)
   __shared__ double lds[kTemporalThreads / 64];
-  if (T.cg->done) return;
+  __shared__ double bc;
+  TemporalCG& cg = *T.cg;
+  if (cg.done) return;
+  const FieldCG s = *T.st;
+  float beta = 0.f;
+  if (it > 0) {
+    const double gg = t_sum_all(T.part2, nblocks, 0, lds, &bc);
+    const double prev = cg.ggs[(it - 1) & 1];
+    beta = (float)(gg / (prev > 1e-30 ? prev : 1e-30));
+    const bool stop = gg < s.tol2 * cg.gg0 || gg < s.gg_floor * s.npix;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      cg.ggs[it & 1] = gg;
+      cg.it += 1;
+      if (stop) cg.done = 1;
+    }
+    if (stop) return;
+  }
   const size_t n = (size_t)T.f.h * T.f.w;
   const size_t i = (size_t)blockIdx.x * kTemporalThreads + threadIdx.x;
   double hh = 0.0, gd = 0.0;
   if (i < n) {
-    const FieldCG s = *T.st;
     const DevField& f = T.f;
     const int y = (int)(i / f.w), x = (int)(i % f.w);
     const bool hr = x + 1 < f.w, hd = y + 1 < f.h;
-    const float2 d0 = T.d[i];
-    const float2 dr = hr ? T.d[i + 1] : make_float2(0.f, 0.f);
-    const float2 dd = hd ? T.d[i + f.w] : make_float2(0.f, 0.f);
+    auto dir = [&](size_t j) {
+      const float2 dp = d_in[j];
+      if (it == 0) return dp;
+      const float2 g = T.g[j];
+      return make_float2(-g.x + beta * dp.x, -g.y + beta * dp.y);
+    };
+    const float2 d0 = dir(i);
+    d_out[i] = d0;
+    const float2 dr = hr ? dir(i + 1) : make_float2(0.f, 0.f);
+    const float2 dd = hd ? dir(i + f.w) : make_float2(0.f, 0.f);
     const float2 c0 = carrier_at(f, y, x);
     const float2 cr = hr ? carrier_at(f, y, x + 1) : make_float2(0.f, 0.f);
     const float2 cd = hd ? carrier_at(f, y + 1, x) : make_float2(0.f, 0.f);
@@ -1242,50 +1279,56 @@ __global__ void __launch_bounds__(kTemporalThreads) k_t_curv(TemporalField T) {
   }
 }
 
-// alpha = -gd / (2 hh), or the curvature exit (:528-532).
-__global__ void __launch_bounds__(kTemporalThreads) k_t_alpha(TemporalField T, int nblocks) {
+// Second launch of a CG iteration (:528-538): every block sums the curvature
+// partials to the same alpha (or the curvature exit), writes chi' = chi +
+// alpha d into chi_out and the gradient at chi' with its |g'|^2 partials.
+// Once the solve has stopped it copies chi to chi_out, so chi is in the
+// buffer the host expects after any number of launches.
+__global__ void __launch_bounds__(kTemporalThreads) k_t_step(TemporalField T, int nblocks,
+                                                             const float2* __restrict__ chi_in,
+                                                             float2* __restrict__ chi_out,
+                                                             const float2* __restrict__ d) {
+#pragma clang fp contract(off)
   __shared__ double lds[kTemporalThreads / 64];
+  __shared__ double bc;
   TemporalCG& cg = *T.cg;
-  if (cg.done) return;
-  const double hh = t_sum(T.part, nblocks, 0, lds);
-  const double gd = t_sum(T.part, nblocks, 1, lds);
-  if (threadIdx.x != 0) return;
-  if (hh <= T.st->hh_floor) cg.done = 1;
-  else cg.alpha = (float)(-0.5 * gd / hh);
-}
-
-// chi += alpha d (:533-534).
-__global__ void __launch_bounds__(kTemporalThreads) k_t_move(TemporalField T) {
-#pragma clang fp contract(off)
-  if (T.cg->done) return;
   const size_t n = (size_t)T.f.h * T.f.w;
   const size_t i = (size_t)blockIdx.x * kTemporalThreads + threadIdx.x;
-  if (i >= n) return;
-  const float a = T.cg->alpha;
-  float2 c = T.f.chi[0][i];
-  const float2 d = T.d[i];
-  c.x += a * d.x;
-  c.y += a * d.y;
-  T.f.chi[0][i] = c;
+  bool stopped = cg.done != 0;
+  double hh = 0.0, gd = 0.0;
+  if (!stopped) {
+    hh = t_sum_all(T.part, nblocks, 0, lds, &bc);
+    gd = t_sum_all(T.part, nblocks, 1, lds, &bc);
+    if (hh <= T.st->hh_floor) {
+      stopped = true;
+      if (blockIdx.x == 0 && threadIdx.x == 0) cg.done = 1;
+    }
+  }
+  if (stopped) {
+    if (i < n) chi_out[i] = chi_in[i];
+    return;
+  }
+  const float alpha = (float)(-0.5 * gd / hh);
+  if (blockIdx.x == 0 && threadIdx.x == 0) cg.alpha = alpha;
+  double gg = 0.0;
+  if (i < n) {
+    const FieldCG s = *T.st;
+    float2 c;
+    const float2 g = t_grad_at(T, s, chi_in, d, alpha, (int)(i / T.f.w), (int)(i % T.f.w), &c);
+    chi_out[i] = c;
+    T.g[i] = g;
+    gg = (double)g.x * g.x + (double)g.y * g.y;
+  }
+  const double b = block_sum<kTemporalThreads>(gg, lds);
+  if (threadIdx.x == 0) T.part2[2 * blockIdx.x] = b;  // the stride of t_sum
 }
 
-// d = -g' + beta d (:539-543).
-__global__ void __launch_bounds__(kTemporalThreads) k_t_dir(TemporalField T) {
-#pragma clang fp contract(off)
-  if (T.cg->done) return;
-  const size_t n = (size_t)T.f.h * T.f.w;
-  const size_t i = (size_t)blockIdx.x * kTemporalThreads + threadIdx.x;
-  if (i >= n) return;
-  const float b = T.cg->beta;
-  const float2 g = T.g[i];
-  float2 d = T.d[i];
-  d.x = -g.x + b * d.x;
-  d.y = -g.y + b * d.y;
-  T.d[i] = d;
-}
-
+// The stage's close: the last iteration completes unless the solve stopped.
 __global__ void k_t_stage_end(TemporalField T, int stage) {
-  if (threadIdx.x == 0 && stage < 16) T.cg->iters[stage] = T.cg->it;
+  if (threadIdx.x != 0) return;
+  TemporalCG& cg = *T.cg;
+  if (!cg.done) cg.it += 1;
+  if (stage < 16) cg.iters[stage] = cg.it;
 }
 
 // Y = S - Re[chi c] (:552-555).
