@@ -559,12 +559,40 @@ void OR_FN(cplx_op)(int op, const R *a, int h, int w, R *d) {
 /* holographic init: engine/holographic_init.cpp                            */
 /* ------------------------------------------------------------------------ */
 
-/* :126-157 MirrorRow (reflect101). */
+/* Mutation hooks for tests/test_oracle_init_pins.py: each macro, when given
+ * on the compiler line, plants one plausible mistake in the init so the test
+ * can show that its closed-form checks catch it.  Never set in lib/. */
+#ifndef OR_MUT_BOX_R
+#define OR_MUT_BOX_R 3 /* :195 EdgeEnergy's blur radius */
+#endif
+#ifndef OR_MUT_WEIGHT_BIAS
+#define OR_MUT_WEIGHT_BIAS 0.5 /* :259,302 weight = 1/(E + 0.5) */
+#endif
+#ifndef OR_MUT_PAD_CAP
+#define OR_MUT_PAD_CAP 16 /* :220 pad = min(16, h-1) */
+#endif
+
+/* :126-131 MirrorRow (reflect101: the edge row is not repeated). */
 static int mirror_row(int y, int h) {
   if (h <= 1) return 0;
+#ifdef OR_MUT_SYMMETRIC_MIRROR
+  const int period = 2 * h; /* edge row repeated */
+  y = ((y % period) + period) % period;
+  return y < h ? y : period - 1 - y;
+#else
   const int period = 2 * h - 2;
   y = ((y % period) + period) % period;
   return y < h ? y : period - y;
+#endif
+}
+
+/* :141-150 PadRowsMirror: `pad` reflected rows above and below a complex
+ * plane (interleaved re, im).  out is (h + 2 pad) x w. */
+void OR_FN(pad_rows_mirror)(const R *a, int h, int w, int pad, R *out) {
+  for (int y = 0; y < h + 2 * pad; ++y) {
+    const int sy = mirror_row(y - pad, h);
+    memcpy(out + (size_t)y * w * 2, a + (size_t)sy * w * 2, sizeof(R) * 2 * (size_t)w);
+  }
 }
 
 /* :16-25 FftFreq (numpy fftfreq). */
@@ -574,9 +602,9 @@ static double fftfreq(int i, int n, double d) {
   return (double)k / ((double)n * d);
 }
 
-/* :61-113 BoxBlur: zero-padded 'same', constant 1/(2r+1); rows pass (along y)
+/* :58-113 BoxBlur: zero-padded 'same', constant 1/(2r+1); rows pass (along y)
  * then columns pass (along x). */
-static void box_blur_naive(R *a, int h, int w, int r) {
+void OR_FN(box_blur_zero)(R *a, int h, int w, int r) {
   const R norm = (R)1 / (R)(2 * r + 1);
   R *col = (R *)xmalloc(sizeof(R) * (size_t)(h > w ? h : w));
   for (int x = 0; x < w; ++x) {
@@ -598,7 +626,7 @@ static void box_blur_naive(R *a, int h, int w, int r) {
   free(col);
 }
 
-/* :171-196 ResidualLuma + EdgeEnergy, then the :365-369 blend weight. */
+/* :170-196 ResidualLuma + EdgeEnergy, then the :258-262 blend weight. */
 static void blend_variant(const R *s, const C *carrier, const C *chi_v, int h, int w, C *chi_num, R *w_sum) {
   const size_t n = (size_t)h * w;
   R *yv = (R *)xmalloc(sizeof(R) * n);
@@ -608,9 +636,9 @@ static void blend_variant(const R *s, const C *carrier, const C *chi_v, int h, i
   OR_FN(dx)(yv, h, w, gx);
   OR_FN(dy)(yv, h, w, gy);
   for (size_t i = 0; i < n; ++i) yv[i] = (R)fabs((double)gx[i]) + (R)fabs((double)gy[i]);
-  box_blur_naive(yv, h, w, 3);
+  OR_FN(box_blur_zero)(yv, h, w, OR_MUT_BOX_R);
   for (size_t i = 0; i < n; ++i) {
-    const R wt = (R)1 / (yv[i] + (R)0.5);
+    const R wt = (R)1 / (yv[i] + (R)OR_MUT_WEIGHT_BIAS);
     chi_num[i].re += chi_v[i].re * wt;
     chi_num[i].im += chi_v[i].im * wt;
     w_sum[i] += wt;
@@ -618,29 +646,51 @@ static void blend_variant(const R *s, const C *carrier, const C *chi_v, int h, i
   free(yv); free(gx); free(gy);
 }
 
-/* :200-317 HolographicInit.  s (h x w), carrier (h x w complex) -> y0, chi0. */
-void OR_FN(holographic_init)(const R *s, const R *carrier_r, int h, int w, double sample_rate,
-                             int symmetry_variant, R *y0, R *chi0_r) {
+/* :224-306, :308-315 the blend: chi0 = sum_v w_v chi_v / sum_v w_v with
+ * w_v = 1 / (E_v + 0.5), E_v the edge energy of S - Re[chi_v c]; then
+ * Y0 = S - Re[chi0 c].  chi_v holds nvar complex planes back to back. */
+void OR_FN(init_blend)(const R *s, const R *carrier_r, const R *chi_v_r, int nvar, int h, int w,
+                       R *y0, R *chi0_r) {
   const C *carrier = (const C *)carrier_r;
+  const C *chi_v = (const C *)chi_v_r;
   const size_t n = (size_t)h * w;
-  const int pad = h > 1 ? (16 < h - 1 ? 16 : h - 1) : 0;
-  const int hp = h + 2 * pad;
-  const size_t np = (size_t)hp * w;
-  CD *spec = (CD *)xmalloc(sizeof(CD) * np);
-  for (int y = 0; y < hp; ++y) {
-    const int sy = mirror_row(y - pad, h);
-    for (int x = 0; x < w; ++x) {
-      const size_t i = (size_t)sy * w + x;
-      /* demod = s * conj(carrier) */
-      spec[(size_t)y * w + x].re = (double)(s[i] * carrier[i].re);
-      spec[(size_t)y * w + x].im = (double)(-s[i] * carrier[i].im);
-    }
-  }
-  fft2d_cd(spec, hp, w, -1);
-
   C *chi_num = (C *)xmalloc(sizeof(C) * n);
   R *w_sum = (R *)xmalloc(sizeof(R) * n);
-  C *chi_v = (C *)xmalloc(sizeof(C) * n);
+  for (int v = 0; v < nvar; ++v) blend_variant(s, carrier, chi_v + (size_t)v * n, h, w, chi_num, w_sum);
+  C *chi0 = (C *)chi0_r;
+  for (size_t i = 0; i < n; ++i) {
+    chi0[i].re = chi_num[i].re / w_sum[i];
+    chi0[i].im = chi_num[i].im / w_sum[i];
+    y0[i] = s[i] - re_prod(chi0[i], carrier[i]);
+  }
+  free(chi_num); free(w_sum);
+}
+
+/* :207-252, :265-295 the crop variants: demod = S conj(c), reflect101-padded
+ * by min(16, h-1) rows, one forward 2-D DFT; per variant the spectrum times
+ * the separable Gaussian (the vertical cutoff against the unpadded h), the
+ * inverse DFT, the rows cropped back, x2.  The optional third variant also
+ * scales each bin by min(|D[k]|, |D[-k]|) / (|D[k]| + 1e-6).  chi_v receives
+ * 2 (or 3) complex h x w planes back to back. */
+int OR_FN(init_variants)(const R *s, const R *carrier_r, int h, int w, double sample_rate,
+                         int symmetry_variant, R *chi_v_r) {
+  const C *carrier = (const C *)carrier_r;
+  C *chi_v = (C *)chi_v_r;
+  const size_t n = (size_t)h * w;
+  const int pad = h > 1 ? (OR_MUT_PAD_CAP < h - 1 ? OR_MUT_PAD_CAP : h - 1) : 0;
+  const int hp = h + 2 * pad;
+  const size_t np = (size_t)hp * w;
+  C *demod = (C *)xmalloc(sizeof(C) * n);
+  C *padded = (C *)xmalloc(sizeof(C) * np);
+  for (size_t i = 0; i < n; ++i) {
+    demod[i].re = s[i] * carrier[i].re;
+    demod[i].im = -s[i] * carrier[i].im;
+  }
+  OR_FN(pad_rows_mirror)((const R *)demod, h, w, pad, (R *)padded);
+  CD *spec = (CD *)xmalloc(sizeof(CD) * np);
+  for (size_t i = 0; i < np; ++i) { spec[i].re = (double)padded[i].re; spec[i].im = (double)padded[i].im; }
+  fft2d_cd(spec, hp, w, -1);
+
   CD *work = (CD *)xmalloc(sizeof(CD) * np);
   const double variants[3][2] = {{0.8, 120.0}, {1.8, 30.0}, {1.3, 60.0}};
   const int nvar = symmetry_variant ? 3 : 2;
@@ -649,9 +699,14 @@ void OR_FN(holographic_init)(const R *s, const R *carrier_r, int h, int w, doubl
     mag = (double *)xmalloc(sizeof(double) * np);
     for (size_t i = 0; i < np; ++i) mag[i] = hypot(spec[i].re, spec[i].im);
   }
+#ifdef OR_MUT_HREF_PADDED
+  const int h_ref = hp;
+#else
+  const int h_ref = h;
+#endif
   for (int v = 0; v < nvar; ++v) {
     const double cx = variants[v][0] * 1.0e6;
-    const double cy = variants[v][1] / (2.0 * (double)h);
+    const double cy = variants[v][1] / (2.0 * (double)h_ref);
     for (int y = 0; y < hp; ++y) {
       const double ry = fftfreq(y, hp, 1.0) / cy;
       const double gy = exp(-0.5 * ry * ry);
@@ -660,31 +715,45 @@ void OR_FN(holographic_init)(const R *s, const R *carrier_r, int h, int w, doubl
         double k = gy * exp(-0.5 * rx * rx);
         const size_t i = (size_t)y * w + x;
         if (v == 2) {
-          /* :378-397 spectral-symmetry certification */
+          /* :281-289 spectral-symmetry certification; bin k pairs with -k */
+#ifdef OR_MUT_SYM_NOROLL
+          const int ry_ = hp - 1 - y, rx_ = w - 1 - x;
+#else
           const int ry_ = (hp - y) % hp, rx_ = (w - x) % w;
+#endif
           const double m = mag[i], mr = mag[(size_t)ry_ * w + rx_];
+#ifdef OR_MUT_SYM_NOMIN
+          k *= mr / (m + 1e-6);
+#else
           k *= (m < mr ? m : mr) / (m + 1e-6);
+#endif
         }
         work[i].re = spec[i].re * k;
         work[i].im = spec[i].im * k;
       }
     }
     fft2d_cd(work, hp, w, +1);
+    C *out = chi_v + (size_t)v * n;
     for (int y = 0; y < h; ++y)
       for (int x = 0; x < w; ++x) {
         const CD c = work[(size_t)(y + pad) * w + x];
-        chi_v[(size_t)y * w + x].re = (R)(2.0 * c.re);
-        chi_v[(size_t)y * w + x].im = (R)(2.0 * c.im);
+        out[(size_t)y * w + x].re = (R)(2.0 * c.re);
+        out[(size_t)y * w + x].im = (R)(2.0 * c.im);
       }
-    blend_variant(s, carrier, chi_v, h, w, chi_num, w_sum);
   }
-  C *chi0 = (C *)chi0_r;
-  for (size_t i = 0; i < n; ++i) {
-    chi0[i].re = chi_num[i].re / w_sum[i];
-    chi0[i].im = chi_num[i].im / w_sum[i];
-    y0[i] = s[i] - re_prod(chi0[i], carrier[i]);
-  }
-  free(spec); free(chi_num); free(w_sum); free(chi_v); free(work); free(mag);
+  free(demod); free(padded); free(spec); free(work); free(mag);
+  return nvar;
+}
+
+/* :200-317 HolographicInit = the crop variants, then their blend.
+ * s (h x w), carrier (h x w complex) -> y0, chi0. */
+void OR_FN(holographic_init)(const R *s, const R *carrier_r, int h, int w, double sample_rate,
+                             int symmetry_variant, R *y0, R *chi0_r) {
+  const size_t n = (size_t)h * w;
+  R *chi_v = (R *)xmalloc(sizeof(C) * n * 3);
+  const int nvar = OR_FN(init_variants)(s, carrier_r, h, w, sample_rate, symmetry_variant, chi_v);
+  OR_FN(init_blend)(s, carrier_r, chi_v, nvar, h, w, y0, chi0_r);
+  free(chi_v);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -72,10 +72,13 @@ def _p(a):
 class Oracle:
     """The restatement, FP64 (`fp32=False`) or FP32 with FP64 reductions."""
 
-    def __init__(self, fp32: bool = False):
-        path = LIB32 if fp32 else LIB64
-        if not path.exists():
-            build(with_ref=False)
+    def __init__(self, fp32: bool = False, path: str | os.PathLike | None = None):
+        """`path` loads another build of the same source (the init-pin tests
+        load builds with a planted mistake)."""
+        if path is None:
+            path = LIB32 if fp32 else LIB64
+            if not path.exists():
+                build(with_ref=False)
         self.lib = C.CDLL(str(path))
         self.dt = np.float32 if fp32 else np.float64
         self.pre = "or32_" if fp32 else "or64_"
@@ -179,6 +182,40 @@ class Oracle:
         self.f("holographic_init")(_p(s), _p(cb), h, w, C.c_double(sample_rate), int(symmetry_variant),
                                    _p(y0), _p(chi))
         return y0, self.uncbuf(chi)
+
+    def init_variants(self, s, carrier, sample_rate, symmetry_variant=False):
+        """The init's crop variants before the blend: [nvar, h, w] complex."""
+        s = self.arr(s)
+        h, w = s.shape
+        cb = self.cbuf(carrier)
+        out = np.zeros((3, h, 2 * w), self.dt)
+        nvar = self.f("init_variants")(_p(s), _p(cb), h, w, C.c_double(sample_rate), int(symmetry_variant),
+                                       _p(out))
+        return self.uncbuf(out[:nvar])
+
+    def init_blend(self, s, carrier, chi_v):
+        """The init's edge-energy blend of given variants -> (y0, chi0)."""
+        s = self.arr(s)
+        h, w = s.shape
+        cb = self.cbuf(carrier)
+        vb = self.cbuf(np.asarray(chi_v))
+        y0 = np.zeros_like(s)
+        chi = np.zeros((h, 2 * w), self.dt)
+        self.f("init_blend")(_p(s), _p(cb), _p(vb), len(chi_v), h, w, _p(y0), _p(chi))
+        return y0, self.uncbuf(chi)
+
+    def pad_rows_mirror(self, a, pad):
+        a = self.cbuf(a)
+        h, w2 = a.shape
+        out = np.zeros((h + 2 * pad, w2), self.dt)
+        self.f("pad_rows_mirror")(_p(a), h, w2 // 2, pad, _p(out))
+        return self.uncbuf(out)
+
+    def box_blur_zero(self, a, r):
+        a = self.arr(a).copy()
+        h, w = a.shape
+        self.f("box_blur_zero")(_p(a), h, w, r)
+        return a
 
     def resolve_chroma_aniso(self, chi0, is_pal, forced=0.0):
         chi0 = np.asarray(chi0)
