@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -23,10 +24,39 @@
 
 using namespace hvdk;
 
+// Scratch of hvd_gpu_variational_refine_noalloc: a fixed device array that
+// every GPU the library runs on holds until the process ends.  In host code
+// `scratch` names a host-side stand-in, not device memory: host code only
+// ever uses the address hipGetSymbolAddress returns for the current device
+// (kept per device by scratch_address(), looked up once at context creation).
+static __device__ __align__(256) char scratch[HVD_GPU_NOALLOC_SCRATCH_BYTES];
+
 namespace {
 
 constexpr int kSlots = 8;
 constexpr double kPi = 3.14159265358979323846;
+// The tail of the no-allocation scratch that holds a call's uploaded
+// descriptors; the rest is that call's workspace.
+constexpr size_t kNoAllocDescBytes = size_t(1) << 20;
+constexpr int kNoAllocMaxPlanes = 256;
+
+// Device address of `scratch` on each device, set once per device.
+std::mutex g_scratch_mu;
+std::map<int, char*> g_scratch_addr;
+
+// With `device` current: the address of this device's copy of `scratch`.
+hipError_t scratch_address(int device, char** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto it = g_scratch_addr.find(device);
+  if (it == g_scratch_addr.end()) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return e != hipSuccess ? e : hipErrorInvalidSymbol;
+    it = g_scratch_addr.emplace(device, static_cast<char*>(p)).first;
+  }
+  *out = it->second;
+  return hipSuccess;
+}
 
 struct Slot {
   void* host = nullptr;
@@ -50,6 +80,12 @@ struct hvd_gpu_ctx {
   std::string err;
   char* ws = nullptr;
   size_t ws_cap = 0;
+  // this device's `scratch` (hipGetSymbolAddress at creation), and whether a
+  // no-allocation call is running: then ws/ws_cap point into the scratch and
+  // nothing may allocate device or pinned memory
+  char* scratch = nullptr;
+  bool no_alloc = false;
+  Slot na_slot;  // descriptor region of a no-allocation call (no event)
   Slot slots[kSlots];
   int next_slot = 0;
   std::map<int, std::vector<float2>> roots;
@@ -168,6 +204,11 @@ void drop_graphs(hvd_gpu_ctx* ctx) {
 
 char* ensure_ws(hvd_gpu_ctx* ctx, size_t bytes, hipStream_t st) {
   if (bytes <= ctx->ws_cap) return ctx->ws;
+  if (ctx->no_alloc) {
+    set_err(ctx, "the call needs %zu bytes of device scratch; the no-allocation entry point holds at most %zu", bytes,
+            ctx->ws_cap);
+    throw Fail{HVD_GPU_ERR_INVALID};
+  }
   HIPCHK(ctx, hipStreamSynchronize(st));
   HIPCHK(ctx, hipDeviceSynchronize());
   drop_graphs(ctx);
@@ -203,14 +244,24 @@ struct Stage {
 };
 
 // Uploads the staged bytes through the next slot; returns the device base.
-// `scratch` more device bytes follow the staged ones, from
+// `extra` more device bytes follow the staged ones, from
 // align256(stg.buf.size()), for the launches that read the slot (the slot is
 // reused only after they finish).
-char* upload(hvd_gpu_ctx* ctx, const Stage& stg, hipStream_t st, Slot** used, size_t scratch = 0) {
+char* upload(hvd_gpu_ctx* ctx, const Stage& stg, hipStream_t st, Slot** used, size_t extra = 0) {
+  if (ctx->no_alloc) {
+    // the descriptor region at the scratch's tail; calls on one stream run in
+    // order, so a later upload never overwrites what an earlier launch reads
+    REQUIRE(ctx, align256(stg.buf.size()) + extra <= ctx->na_slot.cap,
+            "descriptors exceed the no-allocation region (%zu bytes)", ctx->na_slot.cap);
+    if (!stg.buf.empty())
+      HIPCHK(ctx, hipMemcpyAsync(ctx->na_slot.dev, stg.buf.data(), stg.buf.size(), hipMemcpyHostToDevice, st));
+    *used = &ctx->na_slot;
+    return static_cast<char*>(ctx->na_slot.dev);
+  }
   Slot& sl = ctx->slots[ctx->next_slot];
   ctx->next_slot = (ctx->next_slot + 1) % kSlots;
   if (sl.armed) HIPCHK(ctx, hipEventSynchronize(sl.ev));
-  const size_t need = std::max<size_t>(align256(stg.buf.size()) + scratch, 1);
+  const size_t need = std::max<size_t>(align256(stg.buf.size()) + extra, 1);
   if (need > sl.cap) {
     if (sl.host) HIPCHK(ctx, hipHostFree(sl.host));
     if (sl.dev) HIPCHK(ctx, hipFree(sl.dev));
@@ -230,6 +281,13 @@ char* upload(hvd_gpu_ctx* ctx, const Stage& stg, hipStream_t st, Slot** used, si
 // workspace, so launches that read the descriptors keep the same arguments
 // from call to call); the slot only lends its pinned host buffer.
 char* upload_into(hvd_gpu_ctx* ctx, const Stage& stg, char* dst, hipStream_t st, Slot** used) {
+  if (ctx->no_alloc) {
+    // straight from the (pageable) staging buffer: the copy is taken from it
+    // before hipMemcpyAsync returns
+    if (!stg.buf.empty()) HIPCHK(ctx, hipMemcpyAsync(dst, stg.buf.data(), stg.buf.size(), hipMemcpyHostToDevice, st));
+    *used = &ctx->na_slot;
+    return dst;
+  }
   Slot& sl = ctx->slots[ctx->next_slot];
   ctx->next_slot = (ctx->next_slot + 1) % kSlots;
   if (sl.armed) HIPCHK(ctx, hipEventSynchronize(sl.ev));
@@ -250,6 +308,7 @@ char* upload_into(hvd_gpu_ctx* ctx, const Stage& stg, char* dst, hipStream_t st,
 }
 
 void arm(hvd_gpu_ctx* ctx, Slot* sl, hipStream_t st) {
+  if (!sl->ev) return;  // the no-allocation region is guarded by stream order
   HIPCHK(ctx, hipEventRecord(sl->ev, st));
   sl->armed = true;
 }
@@ -814,7 +873,8 @@ void run_refine(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_par
   // each stage: weights + stage-start gradient (one launch), then the CG
   for (int o = 0; o < outer; ++o) record_cg_loop(L, dF, dstate, sg, partials, nf, n_inner, false, o, o);
   L.kernel((const void*)k_refine_finalize, dim3(sga.nb), dim3(64), 0, dF, (const FieldCG*)dstate, sga);
-  run_launches(ctx, L, desc != nullptr && ctx->graphs, st);
+  // no graphs in a no-allocation call: instantiating one allocates
+  run_launches(ctx, L, desc != nullptr && ctx->graphs && !ctx->no_alloc, st);
   arm(ctx, sl, st);
   if (ctx->profiling) {
     void* h = nullptr;
@@ -1024,6 +1084,7 @@ int hvd_gpu_create(int device, hvd_gpu_ctx** out) {
       HIPCHK(ctx, hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
     HIPCHK(ctx, hipFuncSetAttribute((const void*)k_aniso_select, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     std::max(0, lds - kSelStaticLds)));
+    HIPCHK(ctx, scratch_address(device, &ctx->scratch));
   });
   if (rc != HVD_GPU_OK) {
     hvd_gpu_destroy(ctx);
@@ -1736,6 +1797,33 @@ extern "C" {
 int hvd_gpu_variational_refine(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hvd_solver_params* p,
                                int* stage_iters, float* aniso, void* stream) {
   return guarded(ctx, [&] { refine_entry(ctx, planes, n, p, stage_iters, aniso, nullptr, stream); });
+}
+
+int hvd_gpu_variational_refine_noalloc(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n,
+                                       const hvd_solver_params* p, int* stage_iters, float* aniso, void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, ctx->scratch != nullptr, "no scratch on device %d", ctx->device);
+    REQUIRE(ctx, n <= kNoAllocMaxPlanes, "%d planes: the no-allocation entry point takes at most %d", n,
+            kNoAllocMaxPlanes);
+    REQUIRE(ctx, !ctx->profiling, "profiling allocates events: switch it off for the no-allocation entry point");
+    // the workspace and the upload region become the scratch for this call
+    struct Swap {
+      hvd_gpu_ctx* c;
+      char* ws;
+      size_t cap;
+      ~Swap() {
+        c->ws = ws;
+        c->ws_cap = cap;
+        c->no_alloc = false;
+      }
+    } swap{ctx, ctx->ws, ctx->ws_cap};
+    ctx->ws = ctx->scratch;
+    ctx->ws_cap = HVD_GPU_NOALLOC_SCRATCH_BYTES - kNoAllocDescBytes;
+    ctx->na_slot.dev = ctx->scratch + ctx->ws_cap;
+    ctx->na_slot.cap = kNoAllocDescBytes;
+    ctx->no_alloc = true;
+    refine_entry(ctx, planes, n, p, stage_iters, aniso, nullptr, stream);
+  });
 }
 
 int hvd_gpu_variational_refine_temporal(hvd_gpu_ctx* ctx, const hvd_gpu_plane* plane, const hvd_gpu_neighbor* nbrs,

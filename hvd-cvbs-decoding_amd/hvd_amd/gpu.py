@@ -28,7 +28,7 @@ ABI_SYMBOLS = (
     "hvd_gpu_set_solver_tuning", "hvd_gpu_autotune_solver", "hvd_gpu_set_graphs",
     "hvd_gpu_variational_refine_temporal", "hvd_gpu_vectors_per_pixel", "hvd_gpu_warp_by_tiles",
     "hvd_gpu_warp_bilinear", "hvd_gpu_upsample_confidence", "hvd_gpu_complex_coherence",
-    "hvd_gpu_motion_compensate_prev",
+    "hvd_gpu_motion_compensate_prev", "hvd_gpu_variational_refine_noalloc",
 )
 
 
@@ -138,6 +138,8 @@ def load() -> C.CDLL:
         lib.hvd_gpu_holographic_init.argtypes = [vp, C.POINTER(Plane), i, i, vp]
         lib.hvd_gpu_variational_refine.argtypes = [vp, C.POINTER(Plane), i, C.POINTER(SolverParams),
                                                    C.POINTER(C.c_int), C.POINTER(C.c_float), vp]
+        lib.hvd_gpu_variational_refine_noalloc.argtypes = [vp, C.POINTER(Plane), i, C.POINTER(SolverParams),
+                                                           C.POINTER(C.c_int), C.POINTER(C.c_float), vp]
         lib.hvd_gpu_decode_fields_u16.argtypes = [vp, pp, i, C.POINTER(Geometry), C.c_float, C.c_float,
                                                   C.POINTER(SolverParams), C.c_float, pp, pp, pp, vp]
         lib.hvd_gpu_decode_fields_ire.argtypes = [vp, pp, i, C.POINTER(Geometry), C.POINTER(SolverParams),
@@ -298,14 +300,17 @@ class Engine:
         self._check(self.lib.hvd_gpu_holographic_init(self.ctx, P, n, int(symmetry_variant), _stream()),
                     "holographic_init")
 
-    def variational_refine(self, planes, params: SolverParams, want_stats=False):
+    def variational_refine(self, planes, params: SolverParams, want_stats=False, no_alloc=False):
+        """no_alloc: through hvd_gpu_variational_refine_noalloc (the per-GPU
+        fixed scratch instead of the context's workspace)."""
         n = len(planes)
         P = (Plane * n)(*planes)
         outer = max(params.irls_outer, 1)
         iters = (C.c_int * (n * outer))() if want_stats else None
         aniso = (C.c_float * n)() if want_stats else None
-        self._check(self.lib.hvd_gpu_variational_refine(self.ctx, P, n, C.byref(params), iters, aniso, _stream()),
-                    "variational_refine")
+        fn = self.lib.hvd_gpu_variational_refine_noalloc if no_alloc else self.lib.hvd_gpu_variational_refine
+        self._check(fn(self.ctx, P, n, C.byref(params), iters, aniso, _stream()),
+                    "variational_refine_noalloc" if no_alloc else "variational_refine")
         if want_stats:
             return np.array(iters[:], dtype=np.int32).reshape(n, outer), np.array(aniso[:], dtype=np.float32)
         return None

@@ -238,6 +238,25 @@ int hvd_gpu_holographic_init(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int 
 int hvd_gpu_variational_refine(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hvd_solver_params* p,
                                int* stage_iters, float* aniso, void* stream);
 
+/* Device scratch of hvd_gpu_variational_refine_noalloc: 64 MiB per GPU, held
+ * from the library's first context on that GPU until the process ends. */
+#define HVD_GPU_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+
+/* No counterpart in hvd-core.  hvd_gpu_variational_refine for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): the same kernels and the same
+ * results, with the solver's workspace and descriptors in a fixed per-GPU
+ * scratch array instead of the context's workspace.  It calls no
+ * hipMalloc/hipFree/hipHostMalloc and builds no HIP graph (the launches are
+ * issued one by one).  Limits: at most 256 planes and 63 MiB of solver
+ * workspace per call (about 40 B per pixel: six NTSC or PAL fields);
+ * larger calls fail with HVD_GPU_ERR_INVALID before any work is queued, as
+ * does a context with profiling on.  Every call on one GPU shares that GPU's
+ * scratch, from every context: calls that overlap on one GPU (other streams,
+ * other threads) race and must be serialised by the caller. */
+int hvd_gpu_variational_refine_noalloc(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n,
+                                       const hvd_solver_params* p, int* stage_iters, float* aniso, void* stream);
+
 /* One motion-compensated temporal neighbour of a field (hvd::NeighborTerm,
  * variational.h:54-58), on the field's lines x width grid: its composite S_w,
  * its carrier c_w (interleaved complex) and a per-pixel confidence. */
