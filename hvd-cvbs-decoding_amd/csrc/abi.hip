@@ -775,6 +775,7 @@ FieldCG field_cg_init(const DevField& f, const hvd_solver_params& p, int n_inner
   s.eps_c = (float)p.chroma_eps;
   s.k = (float)p.structure_coupling;
   s.aniso_forced = (float)p.chroma_aniso;
+  s.diag = (float)p.diag_prior;
   s.is_pal = p.is_pal;
   s.n_inner = n_inner;
   return s;
@@ -889,7 +890,7 @@ void run_refine(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_par
     for (int i = 0; i < nf; ++i) {
       if (out.stage_iters)
         for (int o = 0; o < outer; ++o) out.stage_iters[i * outer + o] = o < 16 ? hs[i].iters[o] : -1;
-      if (out.aniso) out.aniso[i] = (float)(hs[i].mu_h / (p.lambda_c != 0.0 ? p.lambda_c : 1.0));
+      if (out.aniso) out.aniso[i] = hs[i].aniso;
     }
   }
 }
@@ -1001,8 +1002,55 @@ GroupLayout group_layout(Carve cv, const DevField* F, int n, int is_pal) {
   return L;
 }
 
+// Workspace of the per-pixel solve (pixsolve.hip) for fields F, field i with
+// nn[i] neighbours, carved after `cv`.
+struct PxLayout {
+  std::vector<size_t> chi1, d, g, w, wd, part;
+  std::vector<std::vector<size_t>> nb;
+  std::vector<int> cp;  // chunk prefix
+  std::vector<int> sp;  // aniso sample prefix
+  size_t px = 0, st = 0, gx = 0, gy = 0, desc = 0, end = 0;
+};
+
+size_t px_desc_bytes(int n) {
+  return align256(sizeof(PxField) * n) + align256(sizeof(DevField) * n) + align256(sizeof(int) * (n + 1)) +
+         align256(sizeof(FieldCG) * n) + align256(sizeof(int) * (n + 1));
+}
+
+PxLayout px_layout(Carve cv, const std::vector<DevField>& F, const std::vector<int>& nn, bool diag, int is_pal) {
+  const int n = (int)F.size();
+  PxLayout L;
+  for (auto* v : {&L.chi1, &L.d, &L.g, &L.w, &L.wd, &L.part}) v->resize(n);
+  L.nb.resize(n);
+  L.cp.assign(n + 1, 0);
+  L.sp.assign(n + 1, 0);
+  for (int i = 0; i < n; ++i) {
+    const size_t np = (size_t)F[i].h * F[i].w;
+    const int nc = (int)((np + kPxChunk - 1) / kPxChunk);
+    L.cp[i + 1] = L.cp[i] + nc;
+    L.sp[i + 1] = L.sp[i] + aniso_samples(F[i].h, F[i].w, is_pal);
+    L.chi1[i] = cv.take(sizeof(float2) * np);
+    L.d[i] = cv.take(sizeof(float2) * np * 2);
+    L.g[i] = cv.take(sizeof(float2) * np);
+    L.w[i] = cv.take(sizeof(float4) * np);
+    L.wd[i] = diag ? cv.take(sizeof(float2) * np) : 0;
+    for (int j = 0; j < (nn.empty() ? 0 : nn[i]); ++j) L.nb[i].push_back(cv.take(sizeof(float4) * np));
+    L.part[i] = cv.take(sizeof(double) * 4 * nc);
+  }
+  L.px = cv.take(sizeof(PxState) * n);
+  L.st = cv.take(sizeof(FieldCG) * n);
+  L.gx = cv.take(sizeof(float) * (L.sp[n] + 1));
+  L.gy = cv.take(sizeof(float) * (L.sp[n] + 1));
+  L.desc = cv.take(px_desc_bytes(n));
+  L.end = cv.off;
+  return L;
+}
+
 // End offset of the solver workspace refine_planes carves after `cv`.
-size_t refine_ws_end(const hvd_gpu_ctx* ctx, Carve cv, const std::vector<DevField>& F, int is_pal) {
+size_t refine_ws_end(const hvd_gpu_ctx* ctx, Carve cv, const std::vector<DevField>& F, int is_pal,
+                     bool diag = false) {
+  // the oriented prior runs the per-pixel solve
+  if (diag) return std::max(cv.off, px_layout(cv, F, {}, true, is_pal).end);
   size_t end = cv.off;
   for (const auto& gr : solve_groups(F, tuning_for(ctx, F, is_pal).group_px))
     end = std::max(end, group_layout(cv, F.data() + gr.first, gr.second - gr.first, is_pal).end);
@@ -1013,7 +1061,7 @@ size_t refine_ws_end(const hvd_gpu_ctx* ctx, Carve cv, const std::vector<DevFiel
 // the carrier set); fills the remaining pointers from the carve.
 void check_params(hvd_gpu_ctx* ctx, const hvd_solver_params* p) {
   REQUIRE(ctx, p != nullptr, "null solver parameters");
-  REQUIRE(ctx, p->diag_prior == 0.0, "diag_prior > 0 is not supported by this build");
+  REQUIRE(ctx, p->diag_prior >= 0.0 && std::isfinite(p->diag_prior), "diag_prior must be finite and >= 0");
   REQUIRE(ctx, p->charbonnier_eps > 0.0 && p->chroma_eps > 0.0, "Charbonnier scales must be positive");
   REQUIRE(ctx, p->cg_iterations >= 0, "negative cg_iterations");
 }
@@ -1269,10 +1317,19 @@ namespace {
 
 // Shared by the plane-level refine and the field driver: carve the solver
 // workspace after `cv` and run.
+void refine_pixel(hvd_gpu_ctx* ctx, const std::vector<DevField>& F,
+                  const std::vector<std::vector<hvd_gpu_neighbor>>& N, const hvd_solver_params& p, float nu,
+                  float eps_t, hipStream_t st, Carve cv, RefineOut out);
+
 void refine_planes(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_params& p, hipStream_t st,
                    Carve cv, RefineOut out) {
   // the caller sized the workspace (its tables point into it already)
-  REQUIRE(ctx, refine_ws_end(ctx, cv, F, p.is_pal) <= ctx->ws_cap, "internal: solver workspace not sized");
+  REQUIRE(ctx, refine_ws_end(ctx, cv, F, p.is_pal, p.diag_prior > 0.0) <= ctx->ws_cap,
+          "internal: solver workspace not sized");
+  if (p.diag_prior > 0.0) {
+    refine_pixel(ctx, F, {}, p, 0.f, 0.f, st, cv, out);
+    return;
+  }
   char* ws = ctx->ws;
   const int outer = std::max(1, p.irls_outer);
   const SolverTuning tune = tuning_for(ctx, F, p.is_pal);
@@ -1511,7 +1568,7 @@ void refine_entry(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hv
       F[i] = f;
     }
     // size the whole call (tables + solver) before any pointer is taken
-    ensure_ws(ctx, refine_ws_end(ctx, cv, F, p->is_pal), st);
+    ensure_ws(ctx, refine_ws_end(ctx, cv, F, p->is_pal, p->diag_prior > 0.0), st);
     char* ws = ctx->ws;
     std::vector<PrepDesc> prep;
     for (int i = 0; i < n; ++i) {
@@ -1528,121 +1585,137 @@ void refine_entry(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hv
   }
 }
 
-// VariationalRefine with temporal neighbours for one field (variational.cpp:
-// 198-556, NeighborTerm of variational.h:54-58): the aniso resolution and the
-// parameters of the 2-D solve, then per IRLS stage the weights (with the
-// neighbours' robust weights and the prior scale) and the CG, one launch per
-// step, the scalars on the device so nothing waits on the host.
+// The per-pixel IRLS + CG solve of fields F (chi[0], s, y and the carrier
+// set; variational.cpp:198-556) with the oriented prior when diag_prior > 0
+// and field i's neighbours N[i] (N empty: none) at strength nu, robust scale
+// eps_t.  All fields advance together, one wave per chunk of kPxChunk pixels.
+void refine_pixel(hvd_gpu_ctx* ctx, const std::vector<DevField>& F,
+                  const std::vector<std::vector<hvd_gpu_neighbor>>& N, const hvd_solver_params& p, float nu,
+                  float eps_t, hipStream_t st, Carve cv, RefineOut out) {
+  const int nf = (int)F.size();
+  if (nf == 0) return;
+  std::vector<int> nn(nf, 0);
+  if (!N.empty())
+    for (int i = 0; i < nf; ++i) nn[i] = (int)N[i].size();
+  const bool diag = p.diag_prior > 0.0;
+  const PxLayout L = px_layout(cv, F, nn, diag, p.is_pal);
+  REQUIRE(ctx, L.end <= ctx->ws_cap, "internal: pixel-solve workspace not sized");
+  char* ws = ctx->ws;
+  const int outer = std::max(1, p.irls_outer);
+  const int cg_total = p.fast ? p.cg_iterations * 2 / 3 : p.cg_iterations;
+  const int n_inner = std::max(1, cg_total / outer);
+  const float tolf = p.cg_tol > 0.0 ? (float)p.cg_tol : (p.fast ? 0.10f : 0.02f);
+  FieldCG* dstate = (FieldCG*)(ws + L.st);
+  PxState* dpx = (PxState*)(ws + L.px);
+  std::vector<PxField> P(nf);
+  std::vector<DevField> D(F);
+  std::vector<FieldCG> init(nf);
+  for (int i = 0; i < nf; ++i) {
+    const size_t np = (size_t)F[i].h * F[i].w;
+    DevField& f = D[i];
+    f.chi[1] = (float2*)(ws + L.chi1[i]);
+    f.d[0] = (float2*)(ws + L.d[i]);
+    f.d[1] = f.d[0] + np;
+    f.wts = (float4*)(ws + L.w[i]);
+    PxField& q = P[i];
+    q.f = f;
+    q.g = (float2*)(ws + L.g[i]);
+    q.wts = f.wts;
+    q.wd = diag ? (float2*)(ws + L.wd[i]) : nullptr;
+    q.nn = nn[i];
+    for (int j = 0; j < nn[i]; ++j) {
+      q.nb[j] = (float4*)(ws + L.nb[i][j]);
+      q.conf[j] = N[i][j].confidence;
+      q.sw[j] = N[i][j].composite;
+      q.cw[j] = reinterpret_cast<const float2*>(N[i][j].carrier);
+    }
+    q.nu = nn[i] > 0 ? nu : 0.f;
+    q.et2 = eps_t * eps_t;
+    q.nchunks = L.cp[i + 1] - L.cp[i];
+    q.part = (double*)(ws + L.part[i]);
+    q.px = dpx + i;
+    q.st = dstate + i;
+    init[i] = field_cg_init(f, p, n_inner, (double)tolf);
+  }
+  Stage stg;
+  const size_t oP = stg.put(P), oF = stg.put(D), oC = stg.put(L.cp), oI = stg.put(init), oS = stg.put(L.sp);
+  REQUIRE(ctx, stg.buf.size() <= px_desc_bytes(nf), "internal: descriptor region too small");
+  Slot* sl;
+  char* base = upload_into(ctx, stg, ws + L.desc, st, &sl);
+  const PxField* dP = (const PxField*)(base + oP);
+  const int* dcp = (const int*)(base + oC);
+  const int* dsp = (const int*)(base + oS);
+  float* gx = (float*)(ws + L.gx);
+  float* gy = (float*)(ws + L.gy);
+  const dim3 grid(L.cp[nf]), blk(64);
+  LaunchList Lq;
+  Lq.kernel((const void*)k_refine_begin, dim3((nf + 63) / 64), dim3(64), 0, dstate, (const FieldCG*)(base + oI), nf);
+  if (!(p.chroma_aniso > 0.0) && L.sp[nf] > 0) {
+    const int blocks = std::min(4096, (L.sp[nf] + 255) / 256);
+    Lq.kernel((const void*)k_aniso_samples, dim3(blocks), dim3(256), 0, (const DevField*)(base + oF),
+              (const FieldCG*)dstate, dsp, nf, gx, gy);
+  }
+  record_aniso_select(ctx, Lq, dstate, dsp, gx, gy, (float)p.lambda_c, L.sp);
+  bool any_nb = false;
+  for (int i = 0; i < nf; ++i) any_nb = any_nb || nn[i] > 0;
+  if (any_nb) Lq.kernel((const void*)k_px_prep, grid, blk, 0, dP, dcp, nf);
+  int cur = 0;
+  for (int o = 0; o < outer; ++o) {
+    Lq.kernel((const void*)k_px_weights, grid, blk, 0, dP, dcp, nf, cur, o);
+    Lq.kernel((const void*)k_px_grad0, grid, blk, 0, dP, dcp, nf, cur);
+    Lq.mark(o, false);
+    for (int it = 0; it < n_inner; ++it) {
+      Lq.kernel((const void*)k_px_dir, grid, blk, 0, dP, dcp, nf, it, o);
+      Lq.kernel((const void*)k_px_step, grid, blk, 0, dP, dcp, nf, it, cur, o);
+    }
+    Lq.mark(o, true);
+    Lq.loop_iters.push_back(n_inner);
+    cur ^= n_inner & 1;
+  }
+  Lq.kernel((const void*)k_px_finalize, grid, blk, 0, dP, dcp, nf, cur);
+  run_launches(ctx, Lq, ctx->graphs && !ctx->no_alloc, st);
+  arm(ctx, sl, st);
+  if (out.stage_iters || out.aniso) {
+    std::vector<PxState> hp(nf);
+    std::vector<FieldCG> hs(nf);
+    HIPCHK(ctx, hipMemcpyAsync(hp.data(), dpx, sizeof(PxState) * nf, hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx, hipMemcpyAsync(hs.data(), dstate, sizeof(FieldCG) * nf, hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    for (int i = 0; i < nf; ++i) {
+      if (out.stage_iters)
+        for (int o = 0; o < outer; ++o) out.stage_iters[i * outer + o] = o < 16 ? hp[i].iters[o] : -1;
+      if (out.aniso) out.aniso[i] = hs[i].aniso;
+    }
+  }
+}
+
+// VariationalRefine with temporal neighbours for one field (the per-pixel
+// solve of a batch of one).
 void refine_temporal(hvd_gpu_ctx* ctx, const hvd_gpu_plane& pl, const hvd_gpu_neighbor* nbrs, int nn,
                      const hvd_solver_params& p, float nu, float eps_t, int* stage_iters, hipStream_t st) {
-  const size_t np = (size_t)pl.lines * pl.width;
-  const int nblk = (int)((np + kTemporalThreads - 1) / kTemporalThreads);
-  const int ns = aniso_samples(pl.lines, pl.width, p.is_pal);
   Carve cv;
   const size_t o_row = cv.take(sizeof(float2) * pl.lines), o_vsw = cv.take(pl.lines),
                o_col = cv.take(sizeof(float2) * pl.width);
-  const size_t o_st = cv.take(sizeof(FieldCG)), o_cg = cv.take(sizeof(TemporalCG));
-  const size_t o_gx = cv.take(sizeof(float) * (ns + 1)), o_gy = cv.take(sizeof(float) * (ns + 1));
-  const size_t o_g = cv.take(sizeof(float2) * np), o_d = cv.take(sizeof(float2) * np),
-               o_d2 = cv.take(sizeof(float2) * np), o_chi2 = cv.take(sizeof(float2) * np),
-               o_w = cv.take(sizeof(float4) * np);
-  std::vector<size_t> o_nb(nn);
-  for (int j = 0; j < nn; ++j) o_nb[j] = cv.take(sizeof(float4) * np);
-  const size_t o_part = cv.take(sizeof(double) * 2 * nblk), o_part2 = cv.take(sizeof(double) * 2 * nblk);
-  char* ws = ensure_ws(ctx, cv.off, st);
   DevField f{};
   f.h = pl.lines;
   f.w = pl.width;
   f.s = pl.s;
   f.carrier = reinterpret_cast<const float2*>(pl.carrier);
-  f.chi[0] = f.chi[1] = reinterpret_cast<float2*>(pl.chi);
+  f.chi[0] = reinterpret_cast<float2*>(pl.chi);
   f.y = pl.y;
+  const std::vector<DevField> F{f};
+  const std::vector<std::vector<hvd_gpu_neighbor>> N{std::vector<hvd_gpu_neighbor>(nbrs, nbrs + nn)};
+  ensure_ws(ctx, px_layout(cv, F, {nn}, p.diag_prior > 0.0, p.is_pal).end, st);
+  char* ws = ctx->ws;
+  std::vector<DevField> G(F);
   if (!pl.carrier) {
     run_prepare(ctx, {plane_tables(pl, (float2*)(ws + o_row), (int8_t*)(ws + o_vsw), (float2*)(ws + o_col))}, false,
                 st);
-    f.rowph = (const float2*)(ws + o_row);
-    f.colph = (const float2*)(ws + o_col);
-    f.vsw = pl.vswitch ? (const int8_t*)(ws + o_vsw) : nullptr;
+    G[0].rowph = (const float2*)(ws + o_row);
+    G[0].colph = (const float2*)(ws + o_col);
+    G[0].vsw = pl.vswitch ? (const int8_t*)(ws + o_vsw) : nullptr;
   }
-  const int outer = std::max(1, p.irls_outer);
-  const int cg_total = p.fast ? p.cg_iterations * 2 / 3 : p.cg_iterations;
-  const int n_inner = std::max(1, cg_total / outer);
-  const float tolf = p.cg_tol > 0.0 ? (float)p.cg_tol : (p.fast ? 0.10f : 0.02f);
-  const std::vector<FieldCG> init{field_cg_init(f, p, n_inner, (double)tolf)};
-  const std::vector<int> sp{0, ns};
-  Stage stg;
-  const size_t oF = stg.put(std::vector<DevField>{f}), oI = stg.put(init), oS = stg.put(sp);
-  Slot* sl;
-  char* base = upload(ctx, stg, st, &sl);
-  FieldCG* dstate = (FieldCG*)(ws + o_st);
-  float* gx = (float*)(ws + o_gx);
-  float* gy = (float*)(ws + o_gy);
-  hipLaunchKernelGGL(k_refine_begin, dim3(1), dim3(64), 0, st, dstate, (const FieldCG*)(base + oI), 1);
-  LAUNCH_CHECK(ctx);
-  if (!(p.chroma_aniso > 0.0) && ns > 0) {
-    hipLaunchKernelGGL(k_aniso_samples, dim3(std::min(4096, (ns + 255) / 256)), dim3(256), 0, st,
-                       (const DevField*)(base + oF), (const FieldCG*)dstate, (const int*)(base + oS), 1, gx, gy);
-    LAUNCH_CHECK(ctx);
-  }
-  {
-    LaunchList L;
-    record_aniso_select(ctx, L, dstate, (const int*)(base + oS), gx, gy, (float)p.lambda_c, sp);
-    run_launches(ctx, L, false, st);
-  }
-  TemporalField T{};
-  T.f = f;
-  T.g = (float2*)(ws + o_g);
-  T.d = (float2*)(ws + o_d);
-  T.wts = (float4*)(ws + o_w);
-  TemporalInputs in{};
-  for (int j = 0; j < nn; ++j) {
-    T.nb[j] = (float4*)(ws + o_nb[j]);
-    T.conf[j] = nbrs[j].confidence;
-    in.sw[j] = nbrs[j].composite;
-    in.cw[j] = reinterpret_cast<const float2*>(nbrs[j].carrier);
-  }
-  T.nn = nn;
-  T.nu = nu;
-  T.et2 = eps_t * eps_t;
-  T.part = (double*)(ws + o_part);
-  T.part2 = (double*)(ws + o_part2);
-  T.cg = (TemporalCG*)(ws + o_cg);
-  T.st = dstate;
-  const dim3 grid(nblk), blk(kTemporalThreads);
-  auto launch = [&](const void* fn, dim3 g, std::vector<void*> args) {
-    HIPCHK(ctx, hipLaunchKernel(fn, g, blk, args.data(), 0, st));
-  };
-  // chi and d ping-pong between the caller's / a workspace plane: two
-  // launches per CG iteration, each reading one and writing the other
-  float2* C[2] = {f.chi[0], (float2*)(ws + o_chi2)};
-  float2* D[2] = {(float2*)(ws + o_d), (float2*)(ws + o_d2)};
-  int k = 0, dc = 0, start = 1;
-  launch((const void*)k_t_prep, grid, {&T, &in});
-  for (int o = 0; o < outer; ++o) {
-    TemporalField Ts = T;  // the stage's planes
-    Ts.f.chi[0] = Ts.f.chi[1] = C[k];
-    Ts.d = D[dc];
-    launch((const void*)k_t_stage_begin, dim3(1), {&Ts});
-    launch((const void*)k_t_weights, grid, {&Ts});
-    launch((const void*)k_t_grad, grid, {&Ts, &start});
-    launch((const void*)k_t_sum_start, dim3(1), {&Ts, (void*)&nblk});
-    for (int it = 0; it < n_inner; ++it) {
-      launch((const void*)k_t_dir_curv, grid, {&Ts, (void*)&nblk, &it, &D[dc], &D[dc ^ 1]});
-      dc ^= 1;
-      launch((const void*)k_t_step, grid, {&Ts, (void*)&nblk, &C[k], &C[k ^ 1], &D[dc]});
-      k ^= 1;
-    }
-    launch((const void*)k_t_stage_end, dim3(1), {&Ts, &o});
-  }
-  if (k) HIPCHK(ctx, hipMemcpyAsync(C[0], C[1], sizeof(float2) * np, hipMemcpyDeviceToDevice, st));
-  launch((const void*)k_t_finalize, grid, {&T});
-  arm(ctx, sl, st);
-  if (stage_iters) {
-    TemporalCG h{};
-    HIPCHK(ctx, hipMemcpyAsync(&h, T.cg, sizeof(h), hipMemcpyDeviceToHost, st));
-    HIPCHK(ctx, hipStreamSynchronize(st));
-    for (int o = 0; o < outer; ++o) stage_iters[o] = o < 16 ? h.iters[o] : -1;
-  }
+  refine_pixel(ctx, G, N, p, nu, eps_t, st, cv, RefineOut{stage_iters, nullptr});
 }
 
 }  // namespace
@@ -1852,6 +1925,7 @@ int hvd_gpu_variational_refine_stages(hvd_gpu_ctx* ctx, const hvd_gpu_plane* pla
                                       const hvd_solver_params* p, float* const* stages, void* stream) {
   return guarded(ctx, [&] {
     REQUIRE(ctx, n == 0 || stages, "null stage array");
+    REQUIRE(ctx, !p || p->diag_prior == 0.0, "the stage-saving solve (the adjoint's forward) takes diag_prior 0");
     refine_entry(ctx, planes, n, p, nullptr, nullptr, stages, stream);
   });
 }
@@ -1863,6 +1937,7 @@ int hvd_gpu_refine_adjoint(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n,
   return guarded(ctx, [&] {
     REQUIRE(ctx, n >= 0 && (n == 0 || (planes && stages && gchi && gs && gchi0 && glam)), "null argument");
     check_params(ctx, p);
+    REQUIRE(ctx, p->diag_prior == 0.0, "the adjoint takes diag_prior 0");
     REQUIRE(ctx, adj_iterations >= 1, "adj_iterations must be positive");
     REQUIRE(ctx, adj_tol >= 0.0, "negative adj_tol");
     if (n == 0) return;
@@ -1882,7 +1957,8 @@ struct DecodeLayout {
   size_t end = 0;
 };
 
-DecodeLayout decode_layout(const hvd_gpu_ctx* ctx, int n, const hvd_field_geometry* geom, bool from_raw, Carve cv) {
+DecodeLayout decode_layout(const hvd_gpu_ctx* ctx, int n, const hvd_field_geometry* geom, bool from_raw, Carve cv,
+                           bool diag) {
   DecodeLayout L;
   for (auto* v : {&L.ire, &L.th, &L.vs, &L.s, &L.row, &L.vsa, &L.col, &L.spec, &L.var}) v->resize(n);
   std::vector<DevField> shapes(n), shapes_pal, shapes_ntsc;
@@ -1906,7 +1982,7 @@ DecodeLayout decode_layout(const hvd_gpu_ctx* ctx, int n, const hvd_field_geomet
   }
   L.solver = cv;
   // the solve runs per standard (see decode_fields_run), each from L.solver
-  L.end = std::max(refine_ws_end(ctx, cv, shapes_pal, 1), refine_ws_end(ctx, cv, shapes_ntsc, 0));
+  L.end = std::max(refine_ws_end(ctx, cv, shapes_pal, 1, diag), refine_ws_end(ctx, cv, shapes_ntsc, 0, diag));
   return L;
 }
 
@@ -2070,7 +2146,7 @@ int decode_fields(hvd_gpu_ctx* ctx, const uint16_t* const* raw, const float* con
       REQUIRE(ctx, (raw ? (const void*)raw[i] : (const void*)field_ire[i]) && y_out[i] && chi_out[i],
               "field %d: null pointer", i);
     hipStream_t st = S(stream);
-    const DecodeLayout L = decode_layout(ctx, n, geom, raw != nullptr, Carve{});
+    const DecodeLayout L = decode_layout(ctx, n, geom, raw != nullptr, Carve{}, p->diag_prior > 0.0);
     ensure_ws(ctx, L.end, st);
     decode_fields_run(ctx, raw, field_ire, n, geom, blanking, white, p, chroma_phase_deg, y_out, chi_out, theta_out,
                       st, L, nullptr);
@@ -2117,7 +2193,7 @@ int hvd_gpu_decode_frame_window_s16(hvd_gpu_ctx* ctx, const int16_t* const* fram
     const size_t o_dec = cv.take(sizeof(int) * 2 + sizeof(float) * 4);
     const size_t o_y = cv.take(sizeof(float) * (size_t)nf * lines * width);
     const size_t o_chi = cv.take(sizeof(float2) * (size_t)nf * lines * width);
-    const DecodeLayout L = decode_layout(ctx, nf, geoms.data(), false, cv);
+    const DecodeLayout L = decode_layout(ctx, nf, geoms.data(), false, cv, p->diag_prior > 0.0);
     char* ws = ensure_ws(ctx, L.end, st);
     float* fields = (float*)(ws + o_fields);
     float* amps = (float*)(ws + o_amps);

@@ -91,12 +91,14 @@ struct FieldCG {
   CgDyn dyn[2];
   float mu_h, mu_v, lam_y, eps, eps_c, k;
   float aniso_forced;   // > 0 forces chroma_aniso
+  float diag;           // diag_prior: the oriented (+-45 deg) prior's share
+  float mu_d;           // its weight after the renormalisation (variational.cpp:205-219)
+  float aniso;          // the resolved chroma_aniso
   int is_pal;
   int n_inner;
   int cur;              // slot of chi (and g) at the start of the next stage / at the end
   int iters[16];        // CG iterations per IRLS stage (diagnostics)
   int stage;
-  int pad_;
 };
 
 // Plane accesses through global-address-space pointers: the pointers arrive

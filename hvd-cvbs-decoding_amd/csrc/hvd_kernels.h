@@ -147,50 +147,44 @@ __global__ void k_cg_iter_adj(const DevField* F, FieldCG* st, StripGrid sg, doub
 __global__ void k_cg_finish(FieldCG* st, StripGrid sg, const double* partials, int n_inner, int stage);
 __global__ void k_refine_finalize(const DevField* F, const FieldCG* st, StripGrid sg);
 
-// The solve with motion-compensated temporal neighbours (variational.cpp:
-// 198-556 with NeighborTerm, variational.h:54-58), one field per call.
+// The per-pixel solve (pixsolve.hip): the oriented chroma prior and
+// motion-compensated temporal neighbours (variational.cpp:198-556 with
+// diag_prior > 0 or NeighborTerm, variational.h:54-58), fields of a batch
+// each with their own neighbours.
 constexpr int kMaxNeighbors = 16;
-constexpr int kTemporalThreads = 256;
+constexpr int kPxPerLane = 8;
+constexpr int kPxChunk = 64 * kPxPerLane;  // pixels per wave
 
-// Scalar CG state of the temporal solve (variational.cpp:519-549).
-struct TemporalCG {
+// Per-field CG state of the per-pixel solve beside its FieldCG.
+struct PxState {
   double gg0;
-  double ggs[2];            // |g|^2 by iteration parity (a launch reads one slot, writes the other)
-  float alpha, beta;
-  int it, done;
-  int iters[16];
+  int done[2];    // stop flag: k_px_dir reads [0] and writes [1], k_px_step the reverse
+  int iters[16];  // CG iterations per IRLS stage
 };
 
-struct TemporalField {
-  DevField f;               // chi[0]: chi in place; s, carrier (plane or tables), y
+struct PxField {
+  DevField f;               // s, carrier (plane or tables), chi[0..1], d[0..1], y
   float2* g;                // gradient at chi
-  float2* d;                // search direction
   float4* wts;              // wx, wy, wcx, wcy times the prior scale
+  float2* wd;               // wd1, wd2 (oriented prior), or null
   float4* nb[kMaxNeighbors];  // per neighbour: S_w - S, dc = c - c_w, robust weight wt
   const float* conf[kMaxNeighbors];
-  int nn;
-  float nu, et2;
-  int pad_;
-  double* part;             // two per block: curvature and Re<g, d> (or |g|^2 at a stage start)
-  double* part2;            // |g'|^2 of a step, at stride two per block
-  TemporalCG* cg;
-  const FieldCG* st;        // mu_h, mu_v, lam_y, eps, eps_c, k, exits
-};
-
-struct TemporalInputs {
   const float* sw[kMaxNeighbors];   // neighbour composite
   const float2* cw[kMaxNeighbors];  // neighbour carrier
+  int nn;
+  float nu, et2;
+  int nchunks;
+  double* part;             // 4 * nchunks: |g|^2 by parity, curv(d), Re<g, d>
+  PxState* px;
+  const FieldCG* st;        // mu_h, mu_v, mu_d, lam_y, eps, eps_c, k, exits, n_inner
 };
 
-__global__ void k_t_prep(TemporalField T, TemporalInputs in);
-__global__ void k_t_stage_begin(TemporalField T);
-__global__ void k_t_weights(TemporalField T);
-__global__ void k_t_grad(TemporalField T, int start);
-__global__ void k_t_sum_start(TemporalField T, int nblocks);
-__global__ void k_t_dir_curv(TemporalField T, int nblocks, int it, const float2* d_in, float2* d_out);
-__global__ void k_t_step(TemporalField T, int nblocks, const float2* chi_in, float2* chi_out, const float2* d);
-__global__ void k_t_stage_end(TemporalField T, int stage);
-__global__ void k_t_finalize(TemporalField T);
+__global__ void k_px_prep(const PxField* F, const int* cp, int nf);
+__global__ void k_px_weights(const PxField* F, const int* cp, int nf, int cur, int stage);
+__global__ void k_px_grad0(const PxField* F, const int* cp, int nf, int cur);
+__global__ void k_px_dir(const PxField* F, const int* cp, int nf, int it, int stage);
+__global__ void k_px_step(const PxField* F, const int* cp, int nf, int it, int cur, int stage);
+__global__ void k_px_finalize(const PxField* F, const int* cp, int nf, int cur);
 
 // warp.hip
 __global__ void k_vpp_snap(int th, int tw, const float* mdy, const float* mdx, float* sy, float* sx);

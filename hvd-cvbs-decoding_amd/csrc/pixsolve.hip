@@ -1,0 +1,427 @@
+// pixsolve.hip -- the per-pixel IRLS + CG solve for the terms the strip
+// solver does not carry: the oriented (+-45 deg) chroma prior (diag_prior > 0)
+// and motion-compensated temporal neighbours (hvd-core engine/variational.cpp:
+// 198-556: :205-219 mu renormalisation, :255-281 neighbour deltas, :286-347
+// grad with the D1/D2 terms of gradients.h:195-277, :352-399 curv, :406-516
+// weights, prior scale), for a batch of fields of uneven size, each with its
+// own neighbours.
+//
+// Work split: one 64-lane wave per chunk of kPxChunk consecutive pixels of one
+// field (lane l takes pixels base + l + 64 k, so every load of a step is
+// coalesced); the stencil's neighbours are gathered from the L2.  Two
+// launches per CG iteration:
+//   k_px_dir  : beta and the tolerance exits from the previous step's |g|^2
+//               partials; d = -g + beta d_prev; partials of curv(d), Re<g, d>;
+//   k_px_step : alpha (or the curvature exit) from those partials; chi' =
+//               chi + alpha d formed on the fly at the 3x3 stencil; chi' and
+//               g = grad(chi') written; partials of |g|^2.
+// Every wave of a field sums the field's chunk partials of the previous
+// launch in one fixed order (lane l adds chunks l, l + 64, ..., then a
+// butterfly, which leaves the same bits in every lane), so all waves take the
+// same alpha, beta and exits.  A launch never reads state that the same
+// launch writes: the stop flag is double-buffered (k_px_dir reads done[0] and
+// writes done[1], k_px_step the reverse), gg0 is written by the first
+// iteration's k_px_dir and read from the second on, and the |g|^2 partials
+// alternate between two slots by iteration parity.
+#include "hvd_device.h"
+#include "hvd_kernels.h"
+
+namespace hvdk {
+
+namespace {
+
+__device__ inline int px_lane() { return threadIdx.x & 63; }
+
+// The wave's field and chunk.
+struct PxCtx {
+  int fi, chunk;  // field, chunk within it
+};
+
+__device__ inline PxCtx px_ctx(const int* cp, int nf) {
+  PxCtx c;
+  const int t = blockIdx.x;
+  c.fi = field_of_tile(cp, nf, t);
+  c.chunk = t - cp[c.fi];
+  return c;
+}
+
+__device__ inline double px_wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Sum of part[0 .. n) in the fixed order every wave of the field uses.
+__device__ inline double px_sum(const double* part, int n) {
+  double v = 0.0;
+  for (int i = px_lane(); i < n; i += 64) v += part[i];
+  return px_wave_sum(v);
+}
+
+__device__ inline float2 px_carrier(const DevField& f, int y, int x) {
+#pragma clang fp contract(off)
+  if (f.carrier) return f.carrier[(size_t)y * f.w + x];
+  const float2 r = f.rowph[y], c = f.colph[x];
+  float2 v = make_float2(r.x * c.x - r.y * c.y, r.x * c.y + r.y * c.x);
+  if (f.vsw && f.vsw[y] < 0) v.x = -v.x;
+  return v;
+}
+
+// chi (+ alpha d) and S - Re[chi c] of one pixel; the caller keeps (y, x)
+// inside the plane.
+struct PxSample {
+  float yc;
+  float2 chi;
+  float2 car;
+};
+
+__device__ inline PxSample px_sample(const DevField& f, const float2* chi, const float2* d, float alpha, int y, int x) {
+#pragma clang fp contract(off)
+  const size_t i = (size_t)y * f.w + x;
+  PxSample p;
+  p.chi = chi[i];
+  if (d) {
+    const float2 di = d[i];
+    p.chi.x += alpha * di.x;
+    p.chi.y += alpha * di.y;
+  }
+  p.car = px_carrier(f, y, x);
+  p.yc = f.s[i] - (p.chi.x * p.car.x - p.chi.y * p.car.y);
+  return p;
+}
+
+__device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ inline float2 cscale(float w, float2 a) { return make_float2(w * a.x, w * a.y); }
+__device__ inline float cnorm(float2 a) { return a.x * a.x + a.y * a.y; }
+
+// grad at pixel (y, x) of the fixed-weight objective (:286-347): the luma and
+// axis chroma fluxes across the four edges, the diagonal fluxes across the
+// four corners (D1T, D2T of gradients.h:214-277), then per neighbour
+// 2 nu wt rt conj(dc).  Weights of differences that leave the plane are 0
+// (k_px_weights), so those terms vanish as hvd-core's Neumann zeros do;
+// samples outside the plane are clamped reads that such a weight multiplies.
+__device__ inline float2 px_grad(const PxField& P, const FieldCG& s, const float2* chi, const float2* d, float a,
+                                 int y, int x, float2* chi_out) {
+#pragma clang fp contract(off)
+  const DevField& f = P.f;
+  const int xl = max(x - 1, 0), xr = min(x + 1, f.w - 1), yu = max(y - 1, 0), yd = min(y + 1, f.h - 1);
+  const PxSample p0 = px_sample(f, chi, d, a, y, x), pl = px_sample(f, chi, d, a, y, xl),
+                 pr = px_sample(f, chi, d, a, y, xr), pu = px_sample(f, chi, d, a, yu, x),
+                 pd = px_sample(f, chi, d, a, yd, x);
+  if (chi_out) *chi_out = p0.chi;
+  const size_t i = (size_t)y * f.w + x;
+  const float4 w0 = P.wts[i];
+  const float4 wl = x > 0 ? P.wts[i - 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const float4 wu = y > 0 ? P.wts[i - f.w] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const float fx0 = w0.x * (pr.yc - p0.yc), fxl = wl.x * (p0.yc - pl.yc);
+  const float fy0 = w0.y * (pd.yc - p0.yc), fyu = wu.y * (p0.yc - pu.yc);
+  const float img = 2.f * s.lam_y * ((fxl - fx0) + (fyu - fy0));
+  const float2 cx0 = cscale(w0.z, csub(pr.chi, p0.chi)), cxl = cscale(wl.z, csub(p0.chi, pl.chi));
+  const float2 cy0 = cscale(w0.w, csub(pd.chi, p0.chi)), cyu = cscale(wu.w, csub(p0.chi, pu.chi));
+  float2 cp = make_float2(2.f * (s.mu_h * (cxl.x - cx0.x) + s.mu_v * (cyu.x - cy0.x)),
+                          2.f * (s.mu_h * (cxl.y - cx0.y) + s.mu_v * (cyu.y - cy0.y)));
+  if (P.wd) {
+    // a1 = wd1 (chi(y+1, x+1) - chi(y, x)), a2 = wd2 (chi(y+1, x-1) - chi(y, x));
+    // D1T a1 = -a1(y, x) + a1(y-1, x-1), D2T a2 = -a2(y, x) + a2(y-1, x+1)
+    const float2 wd0 = P.wd[i];
+    const float2 pdr = px_sample(f, chi, d, a, yd, xr).chi, pdl = px_sample(f, chi, d, a, yd, xl).chi;
+    const float2 pul = px_sample(f, chi, d, a, yu, xl).chi, pur = px_sample(f, chi, d, a, yu, xr).chi;
+    const float wd1_ul = (y > 0 && x > 0) ? P.wd[i - f.w - 1].x : 0.f;
+    const float wd2_ur = (y > 0 && x + 1 < f.w) ? P.wd[i - f.w + 1].y : 0.f;
+    const float2 a1 = cscale(wd0.x, csub(pdr, p0.chi)), a1u = cscale(wd1_ul, csub(p0.chi, pul));
+    const float2 a2 = cscale(wd0.y, csub(pdl, p0.chi)), a2u = cscale(wd2_ur, csub(p0.chi, pur));
+    const float2 t1 = make_float2(-a1.x + a1u.x, -a1.y + a1u.y);
+    const float2 t2 = make_float2(-a2.x + a2u.x, -a2.y + a2u.y);
+    cp.x += 2.f * s.mu_d * (t1.x + t2.x);
+    cp.y += 2.f * s.mu_d * (t1.y + t2.y);
+  }
+  float2 g;
+  g.x = -img * p0.car.x + cp.x;
+  g.y = img * p0.car.y + cp.y;
+  for (int j = 0; j < P.nn; ++j) {
+    const float4 nb = P.nb[j][i];
+    const float rt = nb.x + (p0.chi.x * nb.y - p0.chi.y * nb.z);
+    const float k = 2.f * P.nu * nb.w * rt;
+    g.x += k * nb.y;
+    g.y -= k * nb.z;
+  }
+  return g;
+}
+
+}  // namespace
+
+// dc = c - c_w and S_w - S per neighbour (:255-268), fixed for the solve.
+__global__ void __launch_bounds__(64) k_px_prep(const PxField* __restrict__ F, const int* __restrict__ cp, int nf) {
+#pragma clang fp contract(off)
+  const PxCtx c = px_ctx(cp, nf);
+  const PxField& P = F[c.fi];
+  if (P.nn == 0) return;
+  const size_t n = (size_t)P.f.h * P.f.w;
+  for (int k = 0; k < kPxPerLane; ++k) {
+    const size_t i = (size_t)c.chunk * kPxChunk + (size_t)k * 64 + px_lane();
+    if (i >= n) break;
+    const int y = (int)(i / P.f.w), x = (int)(i % P.f.w);
+    const float2 car = px_carrier(P.f, y, x);
+    const float s = P.f.s[i];
+    for (int j = 0; j < P.nn; ++j) {
+      const float2 cw = P.cw[j][i];
+      P.nb[j][i] = make_float4(P.sw[j][i] - s, car.x - cw.x, car.y - cw.y, 0.f);
+    }
+  }
+}
+
+// IRLS weights at chi (:406-516): the 2-D diffusivities (0 on differences that
+// leave the plane), the diagonal ones when the oriented prior is on, the
+// neighbours' robust weights and the prior scale 1 + nu sum_j wt_j |dc_j|^2.
+// The field's first wave also opens the stage's CG state.
+__global__ void __launch_bounds__(64) k_px_weights(const PxField* __restrict__ F, const int* __restrict__ cp, int nf,
+                                                   int cur, int stage) {
+#pragma clang fp contract(off)
+  const PxCtx c = px_ctx(cp, nf);
+  const PxField& P = F[c.fi];
+  const FieldCG s = *P.st;
+  if (c.chunk == 0 && px_lane() == 0) {
+    P.px->done[0] = 0;
+    P.px->done[1] = 0;
+    if (stage < 16) P.px->iters[stage] = s.n_inner;
+  }
+  const DevField& f = P.f;
+  const float2* chi = f.chi[cur];
+  const size_t n = (size_t)f.h * f.w;
+  const float e2 = s.eps * s.eps, ec2 = s.eps_c * s.eps_c;
+  for (int k = 0; k < kPxPerLane; ++k) {
+    const size_t i = (size_t)c.chunk * kPxChunk + (size_t)k * 64 + px_lane();
+    if (i >= n) break;
+    const int y = (int)(i / f.w), x = (int)(i % f.w);
+    const bool hr = x + 1 < f.w, hd = y + 1 < f.h, hl = x > 0;
+    const PxSample p0 = px_sample(f, chi, nullptr, 0.f, y, x);
+    const PxSample pr = px_sample(f, chi, nullptr, 0.f, y, hr ? x + 1 : x);
+    const PxSample pd = px_sample(f, chi, nullptr, 0.f, hd ? y + 1 : y, x);
+    const float gx = hr ? pr.yc - p0.yc : 0.f;
+    const float gy = hd ? pd.yc - p0.yc : 0.f;
+    const float2 dcx = hr ? csub(pr.chi, p0.chi) : make_float2(0.f, 0.f);
+    const float2 dcy = hd ? csub(pd.chi, p0.chi) : make_float2(0.f, 0.f);
+    const float mx = sqrtf(cnorm(dcx)), my = sqrtf(cnorm(dcy));
+    float4 w;
+    w.x = hr ? s.eps / sqrtf(gx * gx + e2) : 0.f;
+    w.y = hd ? s.eps / sqrtf(gy * gy + e2) : 0.f;
+    w.z = hr ? s.eps_c / sqrtf(mx * mx + s.k * gx * gx + ec2) : 0.f;
+    w.w = hd ? s.eps_c / sqrtf(my * my + s.k * gy * gy + ec2) : 0.f;
+    float2 wd = make_float2(0.f, 0.f);
+    if (P.wd) {
+      // 1 / sqrt(1 + (|D chi| / eps_c)^2) (:434-451); 0 where D1 / D2 is 0
+      if (hd && hr) {
+        const float a1 = sqrtf(cnorm(csub(px_sample(f, chi, nullptr, 0.f, y + 1, x + 1).chi, p0.chi))) / s.eps_c;
+        wd.x = 1.f / sqrtf(1.f + a1 * a1);
+      }
+      if (hd && hl) {
+        const float a2 = sqrtf(cnorm(csub(px_sample(f, chi, nullptr, 0.f, y + 1, x - 1).chi, p0.chi))) / s.eps_c;
+        wd.y = 1.f / sqrtf(1.f + a2 * a2);
+      }
+    }
+    if (P.nn > 0 && P.nu > 0.f) {
+      float ks = 1.f;
+      for (int j = 0; j < P.nn; ++j) {
+        float4 nb = P.nb[j][i];
+        const float rt = nb.x + (p0.chi.x * nb.y - p0.chi.y * nb.z);
+        nb.w = P.conf[j][i] * P.et2 / (rt * rt + P.et2);
+        P.nb[j][i] = nb;
+        ks += P.nu * nb.w * (nb.y * nb.y + nb.z * nb.z);
+      }
+      w.x *= ks;
+      w.y *= ks;
+      w.z *= ks;
+      w.w *= ks;
+      wd.x *= ks;
+      wd.y *= ks;
+    }
+    P.wts[i] = w;
+    if (P.wd) P.wd[i] = wd;
+  }
+}
+
+// The stage-start gradient g = grad(chi) and its |g|^2 partials (slot 0).
+__global__ void __launch_bounds__(64) k_px_grad0(const PxField* __restrict__ F, const int* __restrict__ cp, int nf,
+                                                 int cur) {
+  const PxCtx c = px_ctx(cp, nf);
+  const PxField& P = F[c.fi];
+  const FieldCG s = *P.st;
+  const size_t n = (size_t)P.f.h * P.f.w;
+  double gg = 0.0;
+  for (int k = 0; k < kPxPerLane; ++k) {
+    const size_t i = (size_t)c.chunk * kPxChunk + (size_t)k * 64 + px_lane();
+    if (i >= n) break;
+    const float2 g = px_grad(P, s, P.f.chi[cur], nullptr, 0.f, (int)(i / P.f.w), (int)(i % P.f.w), nullptr);
+    P.g[i] = g;
+    gg += (double)g.x * g.x + (double)g.y * g.y;
+  }
+  gg = px_wave_sum(gg);
+  if (px_lane() == 0) P.part[c.chunk] = gg;
+}
+
+// First launch of CG iteration `it` (:539-547 of iteration it - 1, then
+// :527-531): every wave sums the |g|^2 partials of the gradient at the current
+// chi (slot it & 1) -- and, from it = 1 on, of the previous one (slot
+// (it - 1) & 1) -- into beta and the tolerance exits; then d = -g + beta d_prev
+// (d = -g at it = 0) into d[it & 1] and the partials of curv(d) and Re<g, d>.
+__global__ void __launch_bounds__(64) k_px_dir(const PxField* __restrict__ F, const int* __restrict__ cp, int nf,
+                                               int it, int stage) {
+#pragma clang fp contract(off)
+  const PxCtx c = px_ctx(cp, nf);
+  const PxField& P = F[c.fi];
+  const FieldCG s = *P.st;
+  const int nc = P.nchunks;
+  const bool done_in = P.px->done[0] != 0;
+  bool stop = false;
+  float beta = 0.f;
+  if (!done_in) {
+    const double gg = px_sum(P.part + (size_t)(it & 1) * nc, nc);
+    if (it == 0) {
+      if (c.chunk == 0 && px_lane() == 0) P.px->gg0 = gg;
+    } else {
+      const double prev = px_sum(P.part + (size_t)((it - 1) & 1) * nc, nc);
+      beta = (float)(gg / (prev > 1e-30 ? prev : 1e-30));
+      const double gg0 = P.px->gg0;
+      stop = gg < s.tol2 * gg0 || gg < s.gg_floor * s.npix;
+    }
+  }
+  if (c.chunk == 0 && px_lane() == 0) {
+    P.px->done[1] = (done_in || stop) ? 1 : 0;
+    if (!done_in && stop && stage < 16) P.px->iters[stage] = it;
+  }
+  if (done_in || stop) return;
+  const DevField& f = P.f;
+  const size_t n = (size_t)f.h * f.w;
+  const float2* dp = f.d[(it + 1) & 1];
+  float2* dn = f.d[it & 1];
+  double hh = 0.0, hd2 = 0.0, gd = 0.0;
+  auto dir = [&](size_t j) {
+    const float2 g = P.g[j];
+    if (it == 0) return make_float2(-g.x, -g.y);
+    const float2 q = dp[j];
+    return make_float2(-g.x + beta * q.x, -g.y + beta * q.y);
+  };
+  for (int k = 0; k < kPxPerLane; ++k) {
+    const size_t i = (size_t)c.chunk * kPxChunk + (size_t)k * 64 + px_lane();
+    if (i >= n) break;
+    const int y = (int)(i / f.w), x = (int)(i % f.w);
+    const bool hr = x + 1 < f.w, hdn = y + 1 < f.h;
+    const float2 d0 = dir(i);
+    dn[i] = d0;
+    const float2 c0 = px_carrier(f, y, x);
+    const float a0 = -(d0.x * c0.x - d0.y * c0.y);
+    float t1 = 0.f, t2 = 0.f;
+    float2 cx = make_float2(0.f, 0.f), cy = make_float2(0.f, 0.f);
+    if (hr) {
+      const float2 dr = dir(i + 1), cr = px_carrier(f, y, x + 1);
+      t1 = -(dr.x * cr.x - dr.y * cr.y) - a0;
+      cx = csub(dr, d0);
+    }
+    if (hdn) {
+      const float2 dd = dir(i + f.w), cd = px_carrier(f, y + 1, x);
+      t2 = -(dd.x * cd.x - dd.y * cd.y) - a0;
+      cy = csub(dd, d0);
+    }
+    const float4 w = P.wts[i];
+    hh += (double)(s.lam_y * w.x) * t1 * t1;
+    hh += (double)(s.lam_y * w.y) * t2 * t2;
+    hh += (double)s.mu_h * w.z * cnorm(cx);
+    hh += (double)s.mu_v * w.w * cnorm(cy);
+    if (P.wd && hdn) {
+      const float2 wd = P.wd[i];
+      double a = 0.0;
+      if (hr) a += (double)wd.x * cnorm(csub(dir(i + f.w + 1), d0));
+      if (x > 0) a += (double)wd.y * cnorm(csub(dir(i + f.w - 1), d0));
+      hd2 += a;
+    }
+    double at = 0.0;
+    for (int j = 0; j < P.nn; ++j) {
+      const float4 nb = P.nb[j][i];
+      const float re = d0.x * nb.y - d0.y * nb.z;
+      at += (double)nb.w * re * re;
+    }
+    hh += (double)P.nu * at;
+    const float2 g = P.g[i];
+    gd += (double)g.x * d0.x + (double)g.y * d0.y;
+  }
+  hh = px_wave_sum(hh + (double)s.mu_d * hd2);
+  gd = px_wave_sum(gd);
+  if (px_lane() == 0) {
+    P.part[(size_t)2 * nc + c.chunk] = hh;
+    P.part[(size_t)3 * nc + c.chunk] = gd;
+  }
+}
+
+// Second launch of CG iteration `it` (:528-538): alpha = -gd / (2 hh) from the
+// partials of k_px_dir (or the curvature exit), chi' = chi + alpha d into
+// chi[cur ^ ((it + 1) & 1)], g = grad(chi') and its |g|^2 partials into slot
+// (it + 1) & 1.  A stopped field copies chi forward instead, so after the
+// stage chi sits in chi[cur ^ (n_inner & 1)] whatever happened.
+__global__ void __launch_bounds__(64) k_px_step(const PxField* __restrict__ F, const int* __restrict__ cp, int nf,
+                                                int it, int cur, int stage) {
+#pragma clang fp contract(off)
+  const PxCtx c = px_ctx(cp, nf);
+  const PxField& P = F[c.fi];
+  const FieldCG s = *P.st;
+  const int nc = P.nchunks;
+  const DevField& f = P.f;
+  const size_t n = (size_t)f.h * f.w;
+  const float2* chi_in = f.chi[cur ^ (it & 1)];
+  float2* chi_out = f.chi[cur ^ ((it + 1) & 1)];
+  bool stopped = P.px->done[1] != 0;
+  double hh = 0.0, gd = 0.0;
+  if (!stopped) {
+    hh = px_sum(P.part + (size_t)2 * nc, nc);
+    gd = px_sum(P.part + (size_t)3 * nc, nc);
+    if (hh <= s.hh_floor) {
+      stopped = true;
+      if (c.chunk == 0 && px_lane() == 0 && stage < 16) P.px->iters[stage] = it;
+    }
+  }
+  if (c.chunk == 0 && px_lane() == 0) P.px->done[0] = stopped ? 1 : 0;
+  if (stopped) {
+    for (int k = 0; k < kPxPerLane; ++k) {
+      const size_t i = (size_t)c.chunk * kPxChunk + (size_t)k * 64 + px_lane();
+      if (i >= n) break;
+      chi_out[i] = chi_in[i];
+    }
+    return;
+  }
+  const float alpha = (float)(-0.5 * gd / hh);
+  const float2* d = f.d[it & 1];
+  double gg = 0.0;
+  for (int k = 0; k < kPxPerLane; ++k) {
+    const size_t i = (size_t)c.chunk * kPxChunk + (size_t)k * 64 + px_lane();
+    if (i >= n) break;
+    float2 cn;
+    const float2 g = px_grad(P, s, chi_in, d, alpha, (int)(i / f.w), (int)(i % f.w), &cn);
+    chi_out[i] = cn;
+    P.g[i] = g;
+    gg += (double)g.x * g.x + (double)g.y * g.y;
+  }
+  gg = px_wave_sum(gg);
+  if (px_lane() == 0) P.part[(size_t)((it + 1) & 1) * nc + c.chunk] = gg;
+}
+
+// Y = S - Re[chi c] (:552-555) from chi[cur], and chi copied into the
+// caller's buffer chi[0] when it ended in the workspace copy.
+__global__ void __launch_bounds__(64) k_px_finalize(const PxField* __restrict__ F, const int* __restrict__ cp, int nf,
+                                                    int cur) {
+#pragma clang fp contract(off)
+  const PxCtx c = px_ctx(cp, nf);
+  const PxField& P = F[c.fi];
+  const DevField& f = P.f;
+  const size_t n = (size_t)f.h * f.w;
+  for (int k = 0; k < kPxPerLane; ++k) {
+    const size_t i = (size_t)c.chunk * kPxChunk + (size_t)k * 64 + px_lane();
+    if (i >= n) break;
+    const float2 chi = f.chi[cur][i];
+    if (cur) f.chi[0][i] = chi;
+    if (f.y) {
+      const int y = (int)(i / f.w), x = (int)(i % f.w);
+      const float2 car = px_carrier(f, y, x);
+      f.y[i] = f.s[i] - (chi.x * car.x - chi.y * car.y);
+    }
+  }
+}
+
+}  // namespace hvdk

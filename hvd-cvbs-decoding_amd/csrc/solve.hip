@@ -888,8 +888,16 @@ __global__ void __launch_bounds__(kSelThreads) k_aniso_select(FieldCG* st, const
                      : fminf(fmaxf(0.5f + 1.1f * (r - 1.3f), 0.5f), 1.f);
   }
   if (threadIdx.x == 0) {
-    s.mu_h = lambda_c * aniso;
-    s.mu_v = lambda_c;
+#pragma clang fp contract(off)
+    // the oriented prior's weight, and the renormalisation that keeps the
+    // total chroma prior mass of the axis terms (variational.cpp:205-219)
+    float mu_h = lambda_c * aniso, mu_v = lambda_c, mu_d = lambda_c * s.diag * 0.5f;
+    const float tot0 = lambda_c * (1.f + aniso);
+    const float sc = mu_d > 0.f ? tot0 / (tot0 + 2.f * mu_d) : 1.f;
+    s.mu_h = mu_h * sc;
+    s.mu_v = mu_v * sc;
+    s.mu_d = mu_d * sc;
+    s.aniso = aniso;
   }
 }
 
@@ -1033,313 +1041,5 @@ __global__ void __launch_bounds__(64) k_refine_finalize(const DevField* __restri
   }
 }
 
-
-// ---------------------------------------------------------------------------
-// The solve with temporal neighbours (variational.cpp:198-556 with
-// NeighborTerm): the 2-D objective plus, per neighbour j with composite S_w,
-// carrier c_w and confidence conf, the robust data term
-//   nu * sum wt_j * rt_j^2,  rt_j = (S_w - S) + Re[chi (c - c_w)]
-// with wt_j = conf et^2 / (rt_j^2 + et^2) lagged per IRLS stage (:452-461)
-// and the 2-D diffusivities scaled by 1 + nu sum_j wt_j |c - c_w|^2
-// (:463-516).  One thread per pixel and one launch per CG step: the path is
-// a single field per call, so its planes stay in the L2 / Infinity Cache and
-// the launches, not the bytes, set its time.
-// ---------------------------------------------------------------------------
-namespace {
-
-// S - Re[chi c] and chi of pixel (y, x); zeros outside the plane.
-__device__ inline Sample t_sample(const DevField& f, const float2* chi, int y, int x, const float2* d = nullptr,
-                                  float a = 0.f) {
-#pragma clang fp contract(off)
-  Sample p;
-  if (y < 0 || y >= f.h || x < 0 || x >= f.w) {
-    p.yc = 0.f;
-    p.chi = make_float2(0.f, 0.f);
-    p.car = p.chi;
-    return p;
-  }
-  const size_t i = (size_t)y * f.w + x;
-  p.car = carrier_at(f, y, x);
-  p.chi = chi[i];
-  if (d) {  // chi + alpha d, as the step stores it (variational.cpp:533-534)
-    const float2 di = d[i];
-    p.chi.x += a * di.x;
-    p.chi.y += a * di.y;
-  }
-  p.yc = f.s[i] - (p.chi.x * p.car.x - p.chi.y * p.car.y);
-  return p;
-}
-
-__device__ inline float4 t_wts(const TemporalField& T, int y, int x) {
-  if (y < 0 || x < 0) return make_float4(0.f, 0.f, 0.f, 0.f);
-  return T.wts[(size_t)y * T.f.w + x];
-}
-
-// grad at chi (:286-347): the 2-D terms from the fluxes across the pixel's
-// four edges, then 2 nu wt rt conj(dc) per neighbour.  Weights of edges that
-// leave the plane are 0, as are hvd-core's differences there.
-__device__ inline float2 t_grad_at(const TemporalField& T, const FieldCG& s, const float2* chi, const float2* d,
-                                   float a, int y, int x, float2* chi_out = nullptr) {
-#pragma clang fp contract(off)
-  const DevField& f = T.f;
-  const Sample p0 = t_sample(f, chi, y, x, d, a), pl = t_sample(f, chi, y, x - 1, d, a),
-               pr = t_sample(f, chi, y, x + 1, d, a), pu = t_sample(f, chi, y - 1, x, d, a),
-               pd = t_sample(f, chi, y + 1, x, d, a);
-  if (chi_out) *chi_out = p0.chi;
-  const float4 w0 = t_wts(T, y, x), wl = t_wts(T, y, x - 1), wu = t_wts(T, y - 1, x);
-  const float fx0 = w0.x * (pr.yc - p0.yc), fxl = wl.x * (p0.yc - pl.yc);
-  const float fy0 = w0.y * (pd.yc - p0.yc), fyu = wu.y * (p0.yc - pu.yc);
-  const float img = 2.f * s.lam_y * ((fxl - fx0) + (fyu - fy0));
-  const float2 cx0 = make_float2(w0.z * (pr.chi.x - p0.chi.x), w0.z * (pr.chi.y - p0.chi.y));
-  const float2 cxl = make_float2(wl.z * (p0.chi.x - pl.chi.x), wl.z * (p0.chi.y - pl.chi.y));
-  const float2 cy0 = make_float2(w0.w * (pd.chi.x - p0.chi.x), w0.w * (pd.chi.y - p0.chi.y));
-  const float2 cyu = make_float2(wu.w * (p0.chi.x - pu.chi.x), wu.w * (p0.chi.y - pu.chi.y));
-  float2 g;
-  g.x = -img * p0.car.x + 2.f * (s.mu_h * (cxl.x - cx0.x) + s.mu_v * (cyu.x - cy0.x));
-  g.y = img * p0.car.y + 2.f * (s.mu_h * (cxl.y - cx0.y) + s.mu_v * (cyu.y - cy0.y));
-  const size_t i = (size_t)y * f.w + x;
-  for (int j = 0; j < T.nn; ++j) {
-    const float4 nb = T.nb[j][i];
-    const float rt = nb.x + (p0.chi.x * nb.y - p0.chi.y * nb.z);
-    const float k = 2.f * T.nu * nb.w * rt;
-    g.x += k * nb.y;
-    g.y -= k * nb.z;
-  }
-  return g;
-}
-
-// Deterministic sum of part[2 b + k] over b < nblocks (one block of
-// kTemporalThreads); valid in thread 0.
-__device__ inline double t_sum(const double* part, int nblocks, int k, double* lds) {
-  double v = 0.0;
-  for (int b = threadIdx.x; b < nblocks; b += kTemporalThreads) v += part[2 * b + k];
-  return block_sum<kTemporalThreads>(v, lds);
-}
-
-// The same sum in every thread of the block.
-__device__ inline double t_sum_all(const double* part, int nblocks, int k, double* lds, double* bcast) {
-  const double r = t_sum(part, nblocks, k, lds);
-  if (threadIdx.x == 0) *bcast = r;
-  __syncthreads();
-  const double v = *bcast;
-  __syncthreads();
-  return v;
-}
-
-}  // namespace
-
-// dc = c - c_w and S_w - S per neighbour (:255-268), fixed for the solve.
-__global__ void __launch_bounds__(kTemporalThreads) k_t_prep(TemporalField T, TemporalInputs in) {
-  const size_t n = (size_t)T.f.h * T.f.w;
-  const size_t i = (size_t)blockIdx.x * kTemporalThreads + threadIdx.x;
-  if (i >= n) return;
-  const int y = (int)(i / T.f.w), x = (int)(i % T.f.w);
-  const float2 c = carrier_at(T.f, y, x);
-  const float s = T.f.s[i];
-  for (int j = 0; j < T.nn; ++j) {
-    const float2 cw = in.cw[j][i];
-    T.nb[j][i] = make_float4(in.sw[j][i] - s, c.x - cw.x, c.y - cw.y, 0.f);
-  }
-}
-
-__global__ void k_t_stage_begin(TemporalField T) {
-  if (threadIdx.x == 0) {
-    T.cg->done = 0;
-    T.cg->it = 0;
-  }
-}
-
-// IRLS weights at chi (:406-461): the 2-D diffusivities, the neighbours'
-// robust weights and the prior scale 1 + nu sum_j wt_j |dc_j|^2.
-__global__ void __launch_bounds__(kTemporalThreads) k_t_weights(TemporalField T) {
-#pragma clang fp contract(off)
-  const size_t n = (size_t)T.f.h * T.f.w;
-  const size_t i = (size_t)blockIdx.x * kTemporalThreads + threadIdx.x;
-  if (i >= n) return;
-  const FieldCG s = *T.st;
-  const int y = (int)(i / T.f.w), x = (int)(i % T.f.w);
-  const float2* chi = T.f.chi[0];
-  const Sample p0 = t_sample(T.f, chi, y, x), pr = t_sample(T.f, chi, y, x + 1), pd = t_sample(T.f, chi, y + 1, x);
-  float4 w = irls_weight(s, p0, pr.yc, pr.chi, pd, x + 1 < T.f.w, y + 1 < T.f.h);
-  float k = 1.f;
-  for (int j = 0; j < T.nn; ++j) {
-    float4 nb = T.nb[j][i];
-    const float rt = nb.x + (p0.chi.x * nb.y - p0.chi.y * nb.z);
-    nb.w = T.conf[j][i] * T.et2 / (rt * rt + T.et2);
-    T.nb[j][i] = nb;
-    k += T.nu * nb.w * (nb.y * nb.y + nb.z * nb.z);
-  }
-  w.x *= k;
-  w.y *= k;
-  w.z *= k;
-  w.w *= k;
-  T.wts[i] = w;
-}
-
-// g = grad(chi) and its block partial of |g|^2; at a stage start (start = 1)
-// also d = -g.
-__global__ void __launch_bounds__(kTemporalThreads) k_t_grad(TemporalField T, int start) {
-  __shared__ double lds[kTemporalThreads / 64];
-  if (!start && T.cg->done) return;
-  const size_t n = (size_t)T.f.h * T.f.w;
-  const size_t i = (size_t)blockIdx.x * kTemporalThreads + threadIdx.x;
-  double gg = 0.0;
-  if (i < n) {
-    const FieldCG s = *T.st;
-    const float2 g = t_grad_at(T, s, T.f.chi[0], nullptr, 0.f, (int)(i / T.f.w), (int)(i % T.f.w));
-    T.g[i] = g;
-    if (start) T.d[i] = make_float2(-g.x, -g.y);
-    gg = (double)g.x * g.x + (double)g.y * g.y;
-  }
-  const double b = block_sum<kTemporalThreads>(gg, lds);
-  if (threadIdx.x == 0) T.part[2 * blockIdx.x] = b;
-}
-
-// |g|^2 of the stage-start gradient: gg0, and the slot iteration 1 reads.
-__global__ void __launch_bounds__(kTemporalThreads) k_t_sum_start(TemporalField T, int nblocks) {
-  __shared__ double lds[kTemporalThreads / 64];
-  const double gg = t_sum(T.part, nblocks, 0, lds);
-  if (threadIdx.x == 0) {
-    T.cg->gg0 = gg;
-    T.cg->ggs[0] = gg;
-  }
-}
-
-// First launch of CG iteration `it` (variational.cpp:539-547 of the previous
-// iteration, then :527-532): every block sums the previous gradient pass's
-// |g'|^2 partials in the same order, so all take the same beta and exits;
-// then d = -g' + beta d_prev (d = -g as stored at it = 0) into d_out, and the
-// block partials of curv(d) and Re<g', d>.  Block 0 publishes beta, gg (in
-// slot it & 1: this launch reads the other) and the iteration count.
-__global__ void __launch_bounds__(kTemporalThreads) k_t_dir_curv(TemporalField T, int nblocks, int it,
-                                                                 const float2* __restrict__ d_in,
-                                                                 float2* __restrict__ d_out) {
-#pragma clang fp contract(off)
-  __shared__ double lds[kTemporalThreads / 64];
-  __shared__ double bc;
-  TemporalCG& cg = *T.cg;
-  if (cg.done) return;
-  const FieldCG s = *T.st;
-  float beta = 0.f;
-  if (it > 0) {
-    const double gg = t_sum_all(T.part2, nblocks, 0, lds, &bc);
-    const double prev = cg.ggs[(it - 1) & 1];
-    beta = (float)(gg / (prev > 1e-30 ? prev : 1e-30));
-    const bool stop = gg < s.tol2 * cg.gg0 || gg < s.gg_floor * s.npix;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      cg.ggs[it & 1] = gg;
-      cg.it += 1;
-      if (stop) cg.done = 1;
-    }
-    if (stop) return;
-  }
-  const size_t n = (size_t)T.f.h * T.f.w;
-  const size_t i = (size_t)blockIdx.x * kTemporalThreads + threadIdx.x;
-  double hh = 0.0, gd = 0.0;
-  if (i < n) {
-    const DevField& f = T.f;
-    const int y = (int)(i / f.w), x = (int)(i % f.w);
-    const bool hr = x + 1 < f.w, hd = y + 1 < f.h;
-    auto dir = [&](size_t j) {
-      const float2 dp = d_in[j];
-      if (it == 0) return dp;
-      const float2 g = T.g[j];
-      return make_float2(-g.x + beta * dp.x, -g.y + beta * dp.y);
-    };
-    const float2 d0 = dir(i);
-    d_out[i] = d0;
-    const float2 dr = hr ? dir(i + 1) : make_float2(0.f, 0.f);
-    const float2 dd = hd ? dir(i + f.w) : make_float2(0.f, 0.f);
-    const float2 c0 = carrier_at(f, y, x);
-    const float2 cr = hr ? carrier_at(f, y, x + 1) : make_float2(0.f, 0.f);
-    const float2 cd = hd ? carrier_at(f, y + 1, x) : make_float2(0.f, 0.f);
-    const float a0 = -(d0.x * c0.x - d0.y * c0.y);
-    const float t1 = hr ? -(dr.x * cr.x - dr.y * cr.y) - a0 : 0.f;
-    const float t2 = hd ? -(dd.x * cd.x - dd.y * cd.y) - a0 : 0.f;
-    const float2 cx = hr ? make_float2(dr.x - d0.x, dr.y - d0.y) : make_float2(0.f, 0.f);
-    const float2 cy = hd ? make_float2(dd.x - d0.x, dd.y - d0.y) : make_float2(0.f, 0.f);
-    const float4 w = T.wts[i];
-    hh = (double)(s.lam_y * w.x) * t1 * t1 + (double)(s.lam_y * w.y) * t2 * t2 +
-         (double)s.mu_h * w.z * (cx.x * cx.x + cx.y * cx.y) + (double)s.mu_v * w.w * (cy.x * cy.x + cy.y * cy.y);
-    double at = 0.0;
-    for (int j = 0; j < T.nn; ++j) {
-      const float4 nb = T.nb[j][i];
-      const float re = d0.x * nb.y - d0.y * nb.z;
-      at += (double)nb.w * re * re;
-    }
-    hh += (double)T.nu * at;
-    const float2 g = T.g[i];
-    gd = (double)g.x * d0.x + (double)g.y * d0.y;
-  }
-  const double bh = block_sum<kTemporalThreads>(hh, lds);
-  const double bg = block_sum<kTemporalThreads>(gd, lds);
-  if (threadIdx.x == 0) {
-    T.part[2 * blockIdx.x] = bh;
-    T.part[2 * blockIdx.x + 1] = bg;
-  }
-}
-
-// Second launch of a CG iteration (:528-538): every block sums the curvature
-// partials to the same alpha (or the curvature exit), writes chi' = chi +
-// alpha d into chi_out and the gradient at chi' with its |g'|^2 partials.
-// Once the solve has stopped it copies chi to chi_out, so chi is in the
-// buffer the host expects after any number of launches.
-__global__ void __launch_bounds__(kTemporalThreads) k_t_step(TemporalField T, int nblocks,
-                                                             const float2* __restrict__ chi_in,
-                                                             float2* __restrict__ chi_out,
-                                                             const float2* __restrict__ d) {
-#pragma clang fp contract(off)
-  __shared__ double lds[kTemporalThreads / 64];
-  __shared__ double bc;
-  TemporalCG& cg = *T.cg;
-  const size_t n = (size_t)T.f.h * T.f.w;
-  const size_t i = (size_t)blockIdx.x * kTemporalThreads + threadIdx.x;
-  bool stopped = cg.done != 0;
-  double hh = 0.0, gd = 0.0;
-  if (!stopped) {
-    hh = t_sum_all(T.part, nblocks, 0, lds, &bc);
-    gd = t_sum_all(T.part, nblocks, 1, lds, &bc);
-    if (hh <= T.st->hh_floor) {
-      stopped = true;
-      if (blockIdx.x == 0 && threadIdx.x == 0) cg.done = 1;
-    }
-  }
-  if (stopped) {
-    if (i < n) chi_out[i] = chi_in[i];
-    return;
-  }
-  const float alpha = (float)(-0.5 * gd / hh);
-  if (blockIdx.x == 0 && threadIdx.x == 0) cg.alpha = alpha;
-  double gg = 0.0;
-  if (i < n) {
-    const FieldCG s = *T.st;
-    float2 c;
-    const float2 g = t_grad_at(T, s, chi_in, d, alpha, (int)(i / T.f.w), (int)(i % T.f.w), &c);
-    chi_out[i] = c;
-    T.g[i] = g;
-    gg = (double)g.x * g.x + (double)g.y * g.y;
-  }
-  const double b = block_sum<kTemporalThreads>(gg, lds);
-  if (threadIdx.x == 0) T.part2[2 * blockIdx.x] = b;  // the stride of t_sum
-}
-
-// The stage's close: the last iteration completes unless the solve stopped.
-__global__ void k_t_stage_end(TemporalField T, int stage) {
-  if (threadIdx.x != 0) return;
-  TemporalCG& cg = *T.cg;
-  if (!cg.done) cg.it += 1;
-  if (stage < 16) cg.iters[stage] = cg.it;
-}
-
-// Y = S - Re[chi c] (:552-555).
-__global__ void __launch_bounds__(kTemporalThreads) k_t_finalize(TemporalField T) {
-#pragma clang fp contract(off)
-  const size_t n = (size_t)T.f.h * T.f.w;
-  const size_t i = (size_t)blockIdx.x * kTemporalThreads + threadIdx.x;
-  if (i >= n || !T.f.y) return;
-  const int y = (int)(i / T.f.w), x = (int)(i % T.f.w);
-  const float2 c = carrier_at(T.f, y, x), chi = T.f.chi[0][i];
-  T.f.y[i] = T.f.s[i] - (chi.x * c.x - chi.y * c.y);
-}
 
 }  // namespace hvdk

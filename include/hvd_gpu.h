@@ -61,7 +61,8 @@ typedef struct {
   double charbonnier_eps;
   double chroma_eps;
   double structure_coupling;
-  double diag_prior; /* must be 0 in this build (HVD_GPU_ERR_UNSUPPORTED) */
+  double diag_prior; /* >= 0; > 0 adds the oriented +-45 deg prior (variational.cpp:205-219),
+                        solved by the per-pixel kernels; the adjoint entry points take 0 */
   int cg_iterations;
   int irls_outer;
   double cg_tol; /* 0 => auto (0.02 slow / 0.10 fast) */

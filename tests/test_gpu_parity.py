@@ -353,7 +353,7 @@ def test_abi_rejects_bad_arguments(engine):
     g = synth.NTSC
     raw = dev(synth.ntsc_field(1)[0].view(np.int16))
     with pytest.raises(gpu.HvdGpuError, match="diag_prior"):
-        engine.decode_fields_u16([raw], [g], gpu.SolverParams.make(diag_prior=0.5))
+        engine.decode_fields_u16([raw], [g], gpu.SolverParams.make(diag_prior=-0.5))
     bad = synth.Geometry(910, 700, 134, 894, 76, 112, 21, 0, g.sample_rate, 0.0, 0)
     with pytest.raises(gpu.HvdGpuError, match="lines"):
         engine.burst_lockin_phase([dev(np.zeros((700, 910), np.float32))], [bad])
