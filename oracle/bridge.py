@@ -164,3 +164,51 @@ def decode_frame_sequence_window(orc, frames, core_begin, core_end, fp: FramePar
             u[idx] = float(gain) * (-ch.imag) * float(scale)
         outs.append((luma, chroma, u, v))
     return outs, float(gain), bool(f1_bottom)
+
+
+# ---- the stage's woven picture and RGB24 export ---------------------------
+# orc-plugin_hvd/src/hvd_chroma_decoder_stage.cpp: ReorderToWoven (:316-413),
+# YuvToRgb8 (:79-88), the luma-excursion normalisation of
+# WovenToPreviewImage / WriteWovenAsRgb24 (:500-638).
+
+def reorder_to_woven(luma, u, v, fp: FrameParams, full_raster=False, field=-1):
+    """Field-sequential YcFrameS16 planes (frame_height x frame_width) ->
+    woven (y, u, v) float64 planes, raster order or one field's lines."""
+    fw, fh, f1 = fp.frame_width, fp.frame_height, fp.field1_lines
+    a0 = 0 if full_raster else max(0, fp.active_video_start)
+    a1 = fw if full_raster else (fp.active_video_end if fp.active_video_end > a0 else fw)
+    y0 = 0 if full_raster else max(0, fp.first_active_frame_line)
+    y1 = fh if full_raster else (fp.last_active_frame_line if fp.last_active_frame_line > y0 else fh)
+    width = max(0, a1 - a0)
+    planes = [np.asarray(p).reshape(fh, fw) for p in (luma, u, v)]
+    if field in (0, 1):
+        nfl = f1 if field == 0 else fh - f1
+        rows = [(fl if field == 0 else f1 + fl) for fl in range(nfl) if y0 <= 2 * fl + field < y1]
+    else:
+        rows = []
+        for frame_line in range(y0, y1):
+            rows.append(frame_line // 2 if frame_line % 2 == 0 else f1 + frame_line // 2)
+    height = len(rows)
+    if width == 0 or height == 0:
+        return (np.zeros((0, 0)),) * 3
+    cols = np.arange(a0, a0 + width)
+    ok_c = (cols >= 0) & (cols < fw)
+    out = [np.zeros((height, width)) for _ in range(3)]
+    for row, fl in enumerate(rows):
+        if fl < 0 or fl >= fh:  # a raster row past the frame stays zero
+            continue
+        for o, p in zip(out, planes):
+            o[row, ok_c] = p[fl, cols[ok_c]].astype(np.float64)
+    return tuple(out)
+
+
+def yuv_to_rgb8(ny, nu, nv):
+    """YuvToRgb8 on normalised Y, U, V (float64 arrays) -> uint8 [..., 3]."""
+    chans = (ny + 1.13983 * nv, ny - 0.39465 * nu - 0.58060 * nv, ny + 2.03211 * nu)
+    return np.stack([(np.clip(c, 0.0, 1.0) * 255.0 + 0.5).astype(np.uint8) for c in chans], -1)
+
+
+def woven_rgb24(oy, ou, ov, fp: FrameParams):
+    rng = float(np.float32(fp.white_level) - np.float32(fp.black_level))  # float difference
+    black = float(np.float32(fp.black_level))
+    return yuv_to_rgb8((oy - black) / rng, ou / rng, ov / rng)

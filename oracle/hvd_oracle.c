@@ -1744,3 +1744,335 @@ void OR_FN(motion_compensate_prev)(const R *comp, const R *car, int h, int w, co
   OR_FN(upsample_confidence)(mconf, th, tw, tile, h, w, fast, out_conf);
   free(dy); free(dx);
 }
+
+/* ------------------------------------------------------------------------ */
+/* 3-D sequence pieces: engine/ntsc_geometry.cpp, motion.cpp, variational.cpp */
+/* ------------------------------------------------------------------------ */
+
+/* ntsc_geometry.cpp:49-71 EstimateNoiseIre: the second difference at the
+ * 4-sample subcarrier period, d = S[x] - 2 S[x+4] + S[x+8]; the robust
+ * spread P25(|d - median d|) / 0.3186 / sqrt(6).  Median and P25 are the
+ * elements n/2 and n/4 of the sorted set (std::nth_element). */
+R OR_FN(estimate_noise_ire)(const R *s, int h, int w) {
+  if (w <= 8 || h <= 0) return 0;
+  const int m = w - 8, n = h * m;
+  R *d = (R *)xmalloc(sizeof(R) * (size_t)n);
+  for (int y = 0; y < h; ++y)
+    for (int x = 0; x < m; ++x) {
+      const R *r = s + (size_t)y * w + x;
+      d[y * m + x] = r[0] - (R)2 * r[4] + r[8];
+    }
+  const R med = median_upper(d, n);
+  for (int i = 0; i < n; ++i) d[i] = (R)fabs((double)(d[i] - med));
+  const R q = kth_smallest(d, n, n / 4);
+  free(d);
+  return q / (R)0.3186 / (R)sqrt((double)(R)6);
+}
+
+/* motion.cpp:490-525 VerifyMotion: the rounded trajectory prediction and the
+ * zero vector, one SSD each per tile on the r = 2 blurred planes; the better
+ * wins (zero on ties), the prediction keeping its full float value; the
+ * motion confidence of the winner.  pdy, pdx, out_* are th x tw tiles. */
+void OR_FN(verify_motion)(const R *y_ref, const R *y_cur, int h, int w, const R *pdy, const R *pdx, int tile,
+                          R *out_dy, R *out_dx, R *out_conf) {
+  const size_t n = (size_t)h * w;
+  const int th = (h + tile - 1) / tile, tw = (w + tile - 1) / tile, nt = th * tw;
+  R *A = (R *)xmalloc(sizeof(R) * n), *B = (R *)xmalloc(sizeof(R) * n);
+  memcpy(A, y_cur, sizeof(R) * n);
+  memcpy(B, y_ref, sizeof(R) * n);
+  OR_FN(box_blur_prefix)(A, h, w, 2);
+  OR_FN(box_blur_prefix)(B, h, w, 2);
+  R *pyi = (R *)xmalloc(sizeof(R) * (size_t)nt), *pxi = (R *)xmalloc(sizeof(R) * (size_t)nt);
+  for (int i = 0; i < nt; ++i) {
+    pyi[i] = (R)lround((double)pdy[i]);
+    pxi[i] = (R)lround((double)pdx[i]);
+  }
+  R *jy = (R *)xmalloc(sizeof(R) * (size_t)nt), *jx = (R *)xmalloc(sizeof(R) * (size_t)nt);
+  R *bp = (R *)xmalloc(sizeof(R) * (size_t)nt), *bz = (R *)xmalloc(sizeof(R) * (size_t)nt);
+  R *best = (R *)xmalloc(sizeof(R) * (size_t)nt);
+  int zero = 0;
+  bm_pass(A, B, h, w, tile, &zero, 1, &zero, 1, pyi, pxi, jy, jx, bp, NULL);
+  bm_pass(A, B, h, w, tile, &zero, 1, &zero, 1, NULL, NULL, jy, jx, bz, NULL);
+  for (int i = 0; i < nt; ++i) {
+    if (bz[i] <= bp[i]) {
+      out_dy[i] = 0; out_dx[i] = 0; best[i] = bz[i];
+    } else {
+      out_dy[i] = pdy[i]; out_dx[i] = pdx[i]; best[i] = bp[i];
+    }
+  }
+  motion_confidence(A, h, w, best, tile, th, tw, out_conf);
+  free(A); free(B); free(pyi); free(pxi); free(jy); free(jx); free(bp); free(bz); free(best);
+}
+
+/* motion.cpp:527-562 FitTrajectory: per tile the numpy median over the
+ * measurements (offset != 0, confidence > 0.15) of (dy_k - h_k) / k and
+ * dx_k / k; 0 where none qualifies.  dy, dx, conf: m planes of nt tiles. */
+void OR_FN(fit_trajectory)(int m, const int *offsets, const R *pshift, const R *dy, const R *dx, const R *conf,
+                           int nt, R *vy, R *vx) {
+  R *sy = (R *)xmalloc(sizeof(R) * (size_t)(m + 1)), *sx = (R *)xmalloc(sizeof(R) * (size_t)(m + 1));
+  for (int i = 0; i < nt; ++i) {
+    int k = 0;
+    for (int j = 0; j < m; ++j) {
+      if (offsets[j] == 0) continue;
+      if (conf[(size_t)j * nt + i] <= (R)0.15) continue;
+      const R o = (R)offsets[j];
+      sy[k] = (dy[(size_t)j * nt + i] - pshift[j]) / o;
+      sx[k] = dx[(size_t)j * nt + i] / o;
+      ++k;
+    }
+    vy[i] = k ? median_np(sy, k) : 0;
+    vx[i] = k ? median_np(sx, k) : 0;
+  }
+  free(sy); free(sx);
+}
+
+/* motion.cpp:564-601 TrajectorySnap: where >= 3 measurements lie within
+ * 1.5 px (|ddy| + |ddx|) of the fitted k v + h_k, those measurements move
+ * onto it.  In place; fewer than two measurements: no-op. */
+void OR_FN(trajectory_snap)(int m, const int *offsets, const R *pshift, R *dy, R *dx, const R *conf, int nt) {
+  if (m < 2) return;
+  R *vy = (R *)xmalloc(sizeof(R) * (size_t)nt), *vx = (R *)xmalloc(sizeof(R) * (size_t)nt);
+  R *agree = (R *)xmalloc(sizeof(R) * (size_t)nt);
+  OR_FN(fit_trajectory)(m, offsets, pshift, dy, dx, conf, nt, vy, vx);
+  const R tol = (R)1.5;
+  for (int j = 0; j < m; ++j) {
+    const R o = (R)offsets[j];
+    for (int i = 0; i < nt; ++i) {
+      const R py = o * vy[i] + pshift[j], px = o * vx[i];
+      const R dev = (R)fabs((double)(dy[(size_t)j * nt + i] - py)) + (R)fabs((double)(dx[(size_t)j * nt + i] - px));
+      if (dev <= tol) agree[i] += 1;
+    }
+  }
+  for (int j = 0; j < m; ++j) {
+    const R o = (R)offsets[j];
+    for (int i = 0; i < nt; ++i) {
+      if (agree[i] < (R)3) continue;
+      const R py = o * vy[i] + pshift[j], px = o * vx[i];
+      const R dev = (R)fabs((double)(dy[(size_t)j * nt + i] - py)) + (R)fabs((double)(dx[(size_t)j * nt + i] - px));
+      if (dev <= tol) {
+        dy[(size_t)j * nt + i] = py;
+        dx[(size_t)j * nt + i] = px;
+      }
+    }
+  }
+  free(vy); free(vx); free(agree);
+}
+
+/* variational.cpp:558-891 VariationalRefineJoint: Y and chi both free; the
+ * exact data term (S - Y - Re[chi c])^2, per neighbour nu wt (S_w - Y -
+ * Re[chi c_w])^2 with wt lagged per stage, the anchor nu_a w_a (|Y - Y_a|^2 +
+ * |chi - chi_a|^2), Charbonnier priors on grad Y and the chroma priors of
+ * the 2-D solve (the oriented ones too), the diffusivities scaled by
+ * 1 + nu sum wt.  y, chi: the start on entry, the result on return.
+ * Neighbour planes as in variational_refine_nbr; a_* may be NULL (no anchor).
+ * stage_iters (optional) receives the CG iterations per stage. */
+R OR_FN(joint_refine)(const R *s, const R *carrier_r, int h, int w, const or_cfg *cfg, R *y, R *chi_r, int nn,
+                      const R *sw, const R *cw_r, const R *conf, R nu_in, R eps_t, const R *a_luma,
+                      const R *a_chroma_r, const R *a_weight, R nu_a_in, int *stage_iters) {
+  const size_t n = (size_t)h * w;
+  const C *car = (const C *)carrier_r, *cw = (const C *)cw_r, *a_chroma = (const C *)a_chroma_r;
+  C *chi = (C *)chi_r;
+  const R eps = (R)cfg->charbonnier_eps, eps_c = (R)cfg->chroma_eps, k = (R)cfg->structure_coupling;
+  const R lambda_c = (R)cfg->lambda_c;
+  const R aniso = OR_FN(resolve_chroma_aniso)(chi_r, h, w, cfg->is_pal, (R)cfg->chroma_aniso);
+  R mu_h = lambda_c * aniso, mu_v = lambda_c, mu_d = lambda_c * (R)cfg->diag_prior * (R)0.5;
+  {
+    const R tot0 = lambda_c * (1 + aniso);
+    const R sc = mu_d > 0 ? tot0 / (tot0 + 2 * mu_d) : 1;
+    mu_h *= sc; mu_v *= sc; mu_d *= sc;
+  }
+  if (nn > 16) nn = 16;
+  const R nu = nn > 0 ? nu_in : 0;
+  const R nu_a = a_weight ? nu_a_in : 0;
+  const int irls_outer = cfg->irls_outer > 1 ? cfg->irls_outer : 1;
+  const int cg_total = cfg->fast ? cfg->cg_iterations * 2 / 3 : cfg->cg_iterations;
+  const int n_inner = cg_total / irls_outer > 1 ? cg_total / irls_outer : 1;
+  const R cg_tol = cfg->cg_tol > 0 ? (R)cfg->cg_tol : (cfg->fast ? (R)0.10 : (R)0.02);
+  const R e2 = eps * eps, ec2 = eps_c * eps_c, et2 = eps_t * eps_t;
+
+  R *rp[14];
+  for (int i = 0; i < 14; ++i) rp[i] = (R *)xmalloc(sizeof(R) * n);
+  R *wx = rp[0], *wy = rp[1], *wcx = rp[2], *wcy = rp[3], *wd1 = rp[4], *wd2 = rp[5];
+  R *gxY = rp[6], *gyY = rp[7], *t1 = rp[8], *t2 = rp[9], *gY = rp[10], *gYn = rp[11], *dY = rp[12], *tr = rp[13];
+  C *cp[8];
+  for (int i = 0; i < 8; ++i) cp[i] = (C *)xmalloc(sizeof(C) * n);
+  C *c1 = cp[0], *c2 = cp[1], *c3 = cp[2], *gC = cp[3], *gCn = cp[4], *dC = cp[5], *c4 = cp[6], *c5 = cp[7];
+  R *wt = (R *)xmalloc(sizeof(R) * n * (size_t)(nn > 0 ? nn : 1));
+
+  /* :615-683 grad(Y, chi) */
+#define JGRAD(YY, CC, GY, GC)                                                                          \
+  do {                                                                                                 \
+    for (size_t i = 0; i < n; ++i) {                                                                   \
+      const R r0 = s[i] - (YY)[i] - re_prod((CC)[i], car[i]);                                          \
+      const R f = (R)-2 * r0;                                                                          \
+      (GY)[i] = f;                                                                                     \
+      (GC)[i].re = f * car[i].re;                                                                      \
+      (GC)[i].im = f * -car[i].im;                                                                     \
+    }                                                                                                  \
+    for (int j = 0; j < nn; ++j) {                                                                     \
+      const R *swj = sw + (size_t)j * n, *wtj = wt + (size_t)j * n;                                    \
+      const C *cwj = cw + (size_t)j * n;                                                               \
+      for (size_t i = 0; i < n; ++i) {                                                                 \
+        const R rk = swj[i] - (YY)[i] - re_prod((CC)[i], cwj[i]);                                      \
+        (GY)[i] += (R)-2 * nu * wtj[i] * rk;                                                           \
+        const R f = (R)-2 * nu * wtj[i] * rk;                                                          \
+        (GC)[i].re += f * cwj[i].re;                                                                   \
+        (GC)[i].im += f * -cwj[i].im;                                                                  \
+      }                                                                                                \
+    }                                                                                                  \
+    if (nu_a > 0) {                                                                                    \
+      for (size_t i = 0; i < n; ++i) {                                                                 \
+        const R f = (R)2 * nu_a * a_weight[i];                                                         \
+        (GY)[i] += f * ((YY)[i] - a_luma[i]);                                                          \
+        (GC)[i].re += f * ((CC)[i].re - a_chroma[i].re);                                               \
+        (GC)[i].im += f * ((CC)[i].im - a_chroma[i].im);                                               \
+      }                                                                                                \
+    }                                                                                                  \
+    OR_FN(dx)((YY), h, w, t1);                                                                         \
+    for (size_t i = 0; i < n; ++i) t1[i] *= wx[i];                                                     \
+    OR_FN(dxt)(t1, h, w, t2);                                                                          \
+    for (size_t i = 0; i < n; ++i) (GY)[i] += (R)2 * t2[i];                                            \
+    OR_FN(dy)((YY), h, w, t1);                                                                         \
+    for (size_t i = 0; i < n; ++i) t1[i] *= wy[i];                                                     \
+    OR_FN(dyt)(t1, h, w, t2);                                                                          \
+    for (size_t i = 0; i < n; ++i) (GY)[i] += (R)2 * t2[i];                                            \
+    cdx((CC), h, w, c1);                                                                               \
+    for (size_t i = 0; i < n; ++i) { c1[i].re *= wcx[i]; c1[i].im *= wcx[i]; }                          \
+    cdxt(c1, h, w, c3);                                                                                \
+    for (size_t i = 0; i < n; ++i) { (GC)[i].re += (R)2 * mu_h * c3[i].re; (GC)[i].im += (R)2 * mu_h * c3[i].im; } \
+    cdy((CC), h, w, c2);                                                                               \
+    for (size_t i = 0; i < n; ++i) { c2[i].re *= wcy[i]; c2[i].im *= wcy[i]; }                          \
+    cdyt(c2, h, w, c3);                                                                                \
+    for (size_t i = 0; i < n; ++i) { (GC)[i].re += (R)2 * mu_v * c3[i].re; (GC)[i].im += (R)2 * mu_v * c3[i].im; } \
+    if (mu_d > 0) {                                                                                    \
+      cd1((CC), h, w, c1);                                                                             \
+      for (size_t i = 0; i < n; ++i) { c1[i].re *= wd1[i]; c1[i].im *= wd1[i]; }                        \
+      cd1t(c1, h, w, c3);                                                                              \
+      for (size_t i = 0; i < n; ++i) { (GC)[i].re += (R)2 * mu_d * c3[i].re; (GC)[i].im += (R)2 * mu_d * c3[i].im; } \
+      cd2((CC), h, w, c1);                                                                             \
+      for (size_t i = 0; i < n; ++i) { c1[i].re *= wd2[i]; c1[i].im *= wd2[i]; }                        \
+      cd2t(c1, h, w, c3);                                                                              \
+      for (size_t i = 0; i < n; ++i) { (GC)[i].re += (R)2 * mu_d * c3[i].re; (GC)[i].im += (R)2 * mu_d * c3[i].im; } \
+    }                                                                                                  \
+  } while (0)
+
+  for (int outer = 0; outer < irls_outer; ++outer) {
+    /* :773-858 the stage's weights, lagged on (Y, chi) */
+    OR_FN(dx)(y, h, w, gxY);
+    OR_FN(dy)(y, h, w, gyY);
+    cdx(chi, h, w, c1);
+    cdy(chi, h, w, c2);
+    for (size_t i = 0; i < n; ++i) {
+      wx[i] = eps / (R)sqrt((double)(gxY[i] * gxY[i] + e2));
+      wy[i] = eps / (R)sqrt((double)(gyY[i] * gyY[i] + e2));
+      const R mx = cabsr(c1[i]), my = cabsr(c2[i]);
+      wcx[i] = eps_c / (R)sqrt((double)(mx * mx + k * gxY[i] * gxY[i] + ec2));
+      wcy[i] = eps_c / (R)sqrt((double)(my * my + k * gyY[i] * gyY[i] + ec2));
+    }
+    if (mu_d > 0) {
+      cd1(chi, h, w, c1);
+      cd2(chi, h, w, c2);
+      for (size_t i = 0; i < n; ++i) {
+        const R a1 = cabsr(c1[i]) / eps_c, a2 = cabsr(c2[i]) / eps_c;
+        wd1[i] = (R)1 / (R)sqrt((double)(1 + a1 * a1));
+        wd2[i] = (R)1 / (R)sqrt((double)(1 + a2 * a2));
+      }
+    }
+    if (nu > 0) {
+      for (int j = 0; j < nn; ++j) {
+        const R *swj = sw + (size_t)j * n, *cfj = conf + (size_t)j * n;
+        const C *cwj = cw + (size_t)j * n;
+        R *wtj = wt + (size_t)j * n;
+        for (size_t i = 0; i < n; ++i) {
+          const R rk = swj[i] - y[i] - re_prod(chi[i], cwj[i]);
+          wtj[i] = cfj[i] * et2 / (rk * rk + et2);
+        }
+      }
+      for (size_t i = 0; i < n; ++i) tr[i] = 1;
+      for (int j = 0; j < nn; ++j)
+        for (size_t i = 0; i < n; ++i) tr[i] += nu * wt[(size_t)j * n + i];
+      for (size_t i = 0; i < n; ++i) {
+        const R ki = tr[i];
+        wx[i] *= ki; wy[i] *= ki; wcx[i] *= ki; wcy[i] *= ki;
+        if (mu_d > 0) { wd1[i] *= ki; wd2[i] *= ki; }
+      }
+    }
+    JGRAD(y, chi, gY, gC);
+    for (size_t i = 0; i < n; ++i) { dY[i] = -gY[i]; dC[i].re = -gC[i].re; dC[i].im = -gC[i].im; }
+    double gg = 0;
+    { double a = 0, b = 0; for (size_t i = 0; i < n; ++i) a += (double)gY[i] * gY[i]; b = sum_abs2(gC, n); gg = a + b; }
+    const double gg0 = gg;
+    int it = 0;
+    for (; it < n_inner; ++it) {
+      /* :687-755 curv along (dY, dC) */
+      double hh = 0;
+      for (size_t i = 0; i < n; ++i) { const R v = dY[i] + re_prod(dC[i], car[i]); hh += (double)v * v; }
+      for (int j = 0; j < nn; ++j) {
+        const R *wtj = wt + (size_t)j * n;
+        const C *cwj = cw + (size_t)j * n;
+        double acc = 0;
+        for (size_t i = 0; i < n; ++i) {
+          const R dk = dY[i] + re_prod(dC[i], cwj[i]);
+          acc += (double)wtj[i] * dk * dk;
+        }
+        hh += (double)nu * acc;
+      }
+      if (nu_a > 0) {
+        double acc = 0;
+        for (size_t i = 0; i < n; ++i) acc += (double)a_weight[i] * ((double)dY[i] * dY[i] + (double)cabs2(dC[i]));
+        hh += (double)nu_a * acc;
+      }
+      OR_FN(dx)(dY, h, w, t1);
+      OR_FN(dy)(dY, h, w, t2);
+      cdx(dC, h, w, c4);
+      cdy(dC, h, w, c5);
+      double spatial = 0;
+      for (size_t i = 0; i < n; ++i) {
+        spatial += (double)wx[i] * t1[i] * t1[i];
+        spatial += (double)wy[i] * t2[i] * t2[i];
+        spatial += (double)mu_h * wcx[i] * cabs2(c4[i]);
+        spatial += (double)mu_v * wcy[i] * cabs2(c5[i]);
+      }
+      if (mu_d > 0) {
+        cd1(dC, h, w, c4);
+        cd2(dC, h, w, c5);
+        double ad = 0;
+        for (size_t i = 0; i < n; ++i) {
+          ad += (double)wd1[i] * cabs2(c4[i]);
+          ad += (double)wd2[i] * cabs2(c5[i]);
+        }
+        spatial += mu_d * ad;
+      }
+      hh += spatial;
+      if (hh <= 1e-12) break;
+      double gd = 0;
+      { double a = 0; for (size_t i = 0; i < n; ++i) a += (double)gY[i] * dY[i]; gd = a + dot_real(gC, dC, n); }
+      const R alpha = (R)(-0.5 * gd / hh);
+      for (size_t i = 0; i < n; ++i) {
+        y[i] += alpha * dY[i];
+        chi[i].re += alpha * dC[i].re;
+        chi[i].im += alpha * dC[i].im;
+      }
+      JGRAD(y, chi, gYn, gCn);
+      double ggn = 0;
+      { double a = 0; for (size_t i = 0; i < n; ++i) a += (double)gYn[i] * gYn[i]; ggn = a + sum_abs2(gCn, n); }
+      const R beta = (R)(ggn / (gg > 1e-30 ? gg : 1e-30));
+      for (size_t i = 0; i < n; ++i) {
+        dY[i] = -gYn[i] + beta * dY[i];
+        dC[i].re = -gCn[i].re + beta * dC[i].re;
+        dC[i].im = -gCn[i].im + beta * dC[i].im;
+      }
+      memcpy(gY, gYn, sizeof(R) * n);
+      memcpy(gC, gCn, sizeof(C) * n);
+      gg = ggn;
+      if (gg < (double)cg_tol * cg_tol * gg0) { ++it; break; }
+      if (gg < 1e-10 * (double)n) { ++it; break; }
+    }
+    if (stage_iters) stage_iters[outer] = it;
+  }
+#undef JGRAD
+  for (int i = 0; i < 14; ++i) free(rp[i]);
+  for (int i = 0; i < 8; ++i) free(cp[i]);
+  free(wt);
+  return aniso;
+}

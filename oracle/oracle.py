@@ -65,6 +65,25 @@ class Cfg(C.Structure):
         return cls(**d)
 
 
+class SeqCfg(C.Structure):
+    """The HvdConfig fields of the window driver (sequence.cpp) beyond the
+    solver's; defaults are hvd-core's (engine/hvd_config.h)."""
+    _fields_ = [("base", Cfg)] + [(n, C.c_double) for n in (
+        "temporal_strength", "temporal_eps", "nr_anchor", "nr_eps", "coherence_gate", "odd_gate_floor")] + \
+        [(n, C.c_int) for n in ("passes", "nr_radius", "mc_tile", "mc_search", "bidirectional", "extended_temporal",
+                                "trajectory_fit", "psi_init", "output_fidelity", "parallel_across_fields")]
+
+    @classmethod
+    def make(cls, **kw) -> "SeqCfg":
+        names = {f[0] for f in Cfg._fields_}
+        base = Cfg.make(**{k: v for k, v in kw.items() if k in names})
+        d = dict(temporal_strength=0.0, temporal_eps=0.0, nr_anchor=1.0, nr_eps=0.0, coherence_gate=0.6,
+                 odd_gate_floor=0.35, passes=2, nr_radius=2, mc_tile=32, mc_search=16, bidirectional=1,
+                 extended_temporal=1, trajectory_fit=1, psi_init=0, output_fidelity=1, parallel_across_fields=1)
+        d.update({k: v for k, v in kw.items() if k not in names})
+        return cls(base=base, **d)
+
+
 def _p(a):
     return a.ctypes.data_as(C.c_void_p)
 
@@ -368,6 +387,61 @@ class Oracle:
         self.f("estimate_motion")(_p(y_ref), _p(y_cur), h, w, tile, search, _p(dy), _p(dx), _p(cf))
         return dy, dx, cf
 
+    # -- 3-D sequence pieces --------------------------------------------------
+    def estimate_noise_ire(self, s):
+        s = self.arr(s)
+        self.f("estimate_noise_ire").restype = self.R
+        return float(self.f("estimate_noise_ire")(_p(s), s.shape[0], s.shape[1]))
+
+    def verify_motion(self, y_ref, y_cur, pdy, pdx, tile=32):
+        y_ref, y_cur, pdy, pdx = self.arr(y_ref), self.arr(y_cur), self.arr(pdy), self.arr(pdx)
+        h, w = y_ref.shape
+        dy, dx, cf = (np.zeros_like(pdy) for _ in range(3))
+        self.f("verify_motion")(_p(y_ref), _p(y_cur), h, w, _p(pdy), _p(pdx), tile, _p(dy), _p(dx), _p(cf))
+        return dy, dx, cf
+
+    def trajectory_snap(self, offsets, pshift, dy, dx, conf):
+        """dy, dx, conf: [m, th, tw]; returns snapped copies of dy, dx."""
+        dy, dx, conf = self.arr(dy).copy(), self.arr(dx).copy(), self.arr(conf)
+        m = dy.shape[0]
+        off = np.ascontiguousarray(offsets, np.int32)
+        ps = self.arr(pshift)
+        self.f("trajectory_snap")(m, _p(off), _p(ps), _p(dy), _p(dx), _p(conf), dy[0].size)
+        return dy, dx
+
+    def fit_trajectory(self, offsets, pshift, dy, dx, conf):
+        dy, dx, conf = self.arr(dy), self.arr(dx), self.arr(conf)
+        vy, vx = np.zeros_like(dy[0]), np.zeros_like(dy[0])
+        self.f("fit_trajectory")(dy.shape[0], _p(np.ascontiguousarray(offsets, np.int32)), _p(self.arr(pshift)),
+                                 _p(dy), _p(dx), _p(conf), dy[0].size, _p(vy), _p(vx))
+        return vy, vx
+
+    def joint_refine(self, s, carrier, y0, chi0, cfg: Cfg, neighbours=(), nu=0.0, eps_t=1.0, anchor=None,
+                     nu_a=0.0):
+        """VariationalRefineJoint -> (y, chi, aniso, stage iterations);
+        neighbours: (S_w, c_w, conf) triples; anchor: (Y_a, chi_a, w_a)."""
+        s = self.arr(s)
+        h, w = s.shape
+        y = self.arr(y0).copy()
+        chi = self.cbuf(chi0)
+        nn = len(neighbours)
+        if nn:
+            sw = np.ascontiguousarray(np.stack([self.arr(n[0]) for n in neighbours]))
+            cw = np.ascontiguousarray(np.stack([self.cbuf(n[1]) for n in neighbours]))
+            cf = np.ascontiguousarray(np.stack([self.arr(n[2]) for n in neighbours]))
+        else:
+            sw = cw = cf = np.zeros(1, self.dt)
+        if anchor is not None:
+            al, ac, aw = self.arr(anchor[0]), self.cbuf(anchor[1]), self.arr(anchor[2])
+        iters = np.zeros(max(cfg.irls_outer, 1), np.int32)
+        fn = self.f("joint_refine")
+        fn.restype = self.R
+        aniso = fn(_p(s), _p(self.cbuf(carrier)), h, w, C.byref(cfg), _p(y), _p(chi), nn, _p(sw), _p(cw), _p(cf),
+                   self.R(nu), self.R(eps_t), _p(al) if anchor is not None else None,
+                   _p(ac) if anchor is not None else None, _p(aw) if anchor is not None else None, self.R(nu_a),
+                   _p(iters))
+        return y, self.uncbuf(chi), float(aniso), iters
+
     def block_match_sad_u16(self, ref, cur, bh=8, bw=16, sy=4, sx=16):
         ref = np.ascontiguousarray(ref, np.uint16)
         cur = np.ascontiguousarray(cur, np.uint16)
@@ -387,7 +461,10 @@ class Reference:
         path = REFLIB_OMP if openmp else REFLIB
         if not path.exists():
             raise FileNotFoundError(str(path))
-        self.lib = C.CDLL(str(path))
+        # lazy binding: engine/sequence.cpp's DecodeFieldWindow names
+        # HolographicInit, which needs FFTW and is not in the library; no
+        # function exported here reaches it
+        self.lib = C.CDLL(str(path), mode=os.RTLD_LAZY)
         self.lib.ref_burst_amplitude_ire.restype = C.c_float
         self.lib.ref_acc_gain.restype = C.c_float
 
@@ -537,6 +614,77 @@ class Reference:
         dy, dx, cf = (np.zeros((th, tw), np.float32) for _ in range(3))
         self.lib.ref_estimate_motion(_p(y_ref), _p(y_cur), h, w, tile, search, _p(dy), _p(dx), _p(cf))
         return dy, dx, cf
+
+
+def _ref_extras():
+    """Methods of Reference for the window driver's pieces."""
+
+    def estimate_noise_ire(self, s):
+        s = self.f32(s)
+        self.lib.ref_estimate_noise_ire.restype = C.c_float
+        return float(self.lib.ref_estimate_noise_ire(_p(s), s.shape[0], s.shape[1]))
+
+    def verify_motion(self, y_ref, y_cur, pdy, pdx, tile=32):
+        y_ref, y_cur, pdy, pdx = self.f32(y_ref), self.f32(y_cur), self.f32(pdy), self.f32(pdx)
+        h, w = y_ref.shape
+        dy, dx, cf = (np.zeros_like(pdy) for _ in range(3))
+        self.lib.ref_verify_motion(_p(y_ref), _p(y_cur), h, w, _p(pdy), _p(pdx), pdy.shape[0], pdy.shape[1], tile,
+                                   _p(dy), _p(dx), _p(cf))
+        return dy, dx, cf
+
+    def trajectory_snap(self, offsets, pshift, dy, dx, conf):
+        dy, dx, conf = self.f32(dy).copy(), self.f32(dx).copy(), self.f32(conf)
+        self.lib.ref_trajectory_snap(dy.shape[0], _p(np.ascontiguousarray(offsets, np.int32)), _p(self.f32(pshift)),
+                                     _p(dy), _p(dx), _p(conf), dy.shape[1], dy.shape[2])
+        return dy, dx
+
+    def joint_refine(self, s, carrier, y0, chi0, cfg: Cfg, neighbours=(), nu=0.0, eps_t=1.0, anchor=None,
+                     nu_a=0.0):
+        s = self.f32(s)
+        h, w = s.shape
+        y = self.f32(y0).copy()
+        chi = self.cbuf32(chi0)
+        nn = len(neighbours)
+        if nn:
+            sw = np.ascontiguousarray(np.stack([self.f32(n[0]) for n in neighbours]))
+            cw = np.ascontiguousarray(np.stack([self.cbuf32(n[1]) for n in neighbours]))
+            cf = np.ascontiguousarray(np.stack([self.f32(n[2]) for n in neighbours]))
+        else:
+            sw = cw = cf = np.zeros(1, np.float32)
+        if anchor is not None:
+            al, ac, aw = self.f32(anchor[0]), self.cbuf32(anchor[1]), self.f32(anchor[2])
+        self.lib.ref_joint_refine(_p(s), _p(self.cbuf32(carrier)), h, w, C.byref(cfg), nn, _p(sw), _p(cw), _p(cf),
+                                  C.c_float(nu), C.c_float(eps_t), _p(al) if anchor is not None else None,
+                                  _p(ac) if anchor is not None else None, _p(aw) if anchor is not None else None,
+                                  C.c_float(nu_a), _p(y), _p(chi))
+        return y, chi[:, 0::2] + 1j * chi[:, 1::2]
+
+    def decode_window_with_inits(self, fields, parities, inits_chi, cfg: "SeqCfg", g=None, inits_y=None):
+        """hvd-core's DecodeFieldWindowWithInits: fields = [(S, carrier)],
+        -> (Y planes, chi planes, diagnostics dict)."""
+        nf = len(fields)
+        h, w = fields[0][0].shape
+        s = np.ascontiguousarray(np.stack([self.f32(f[0]) for f in fields]))
+        car = np.ascontiguousarray(np.stack([self.cbuf32(f[1]) for f in fields]))
+        chi0 = np.ascontiguousarray(np.stack([self.cbuf32(c) for c in inits_chi]))
+        y0 = None if inits_y is None else np.ascontiguousarray(np.stack([self.f32(y) for y in inits_y]))
+        par = np.ascontiguousarray(parities, np.int32)
+        oy = np.zeros((nf, h, w), np.float32)
+        oc = np.zeros((nf, h, 2 * w), np.float32)
+        diag = np.zeros(5, np.float32)
+        geom = Geom() if g is None else Geom.of(g)
+        n = self.lib.ref_decode_window_with_inits(nf, h, w, _p(s), _p(car), _p(par), _p(chi0),
+                                                  None if y0 is None else _p(y0), C.byref(geom), C.byref(cfg),
+                                                  _p(oy), _p(oc), _p(diag))
+        assert n == nf
+        keys = ("resolved_strength", "ambiguity_ire", "sigma_ire", "temporal_eps", "nr_eps")
+        return oy, oc[..., 0::2] + 1j * oc[..., 1::2], dict(zip(keys, diag.tolist()))
+
+    for fn in (estimate_noise_ire, verify_motion, trajectory_snap, joint_refine, decode_window_with_inits):
+        setattr(Reference, fn.__name__, fn)
+
+
+_ref_extras()
 
 
 def threads_hint() -> int:

@@ -8,9 +8,12 @@
 // Plane / FieldGeometry / HvdConfig types and calls its functions, so the
 // oracle and the GPU path can be compared with the real engine.
 //
-// engine/fft2d.cpp (and with it HolographicInit, DecodeFrame and the field
-// driver) needs FFTW3, which this image does not have; those functions are
-// not exported here.
+// engine/fft2d.cpp (and with it HolographicInit, DecodeFrame and
+// DecodeFieldWindow) needs FFTW3, which this image does not have; those
+// functions are not exported here.  engine/sequence.cpp is compiled for its
+// test seam DecodeFieldWindowWithInits (the whole window driver with the
+// inits supplied), which never reaches HolographicInit: that one symbol is
+// left unresolved in the library and nothing here calls it.
 
 #include <cstdint>
 #include <cstring>
@@ -23,6 +26,7 @@
 #include "engine/temporal.h"
 #include "engine/ntsc_geometry.h"
 #include "engine/plane.h"
+#include "engine/sequence.h"
 #include "engine/variational.h"
 
 namespace {
@@ -41,6 +45,51 @@ struct ShimCfg {
   int fast, is_pal;
   double lambda_y;
 };
+
+// The HvdConfig fields the window driver reads beyond the solver's.
+struct ShimSeqCfg {
+  ShimCfg base;
+  double temporal_strength, temporal_eps, nr_anchor, nr_eps, coherence_gate, odd_gate_floor;
+  int passes, nr_radius, mc_tile, mc_search;
+  int bidirectional, extended_temporal, trajectory_fit, psi_init, output_fidelity, parallel_across_fields;
+};
+
+hvd::HvdConfig ToCfg(const ShimCfg* sc) {
+  hvd::HvdConfig cfg;
+  cfg.lambda_c = static_cast<float>(sc->lambda_c);
+  cfg.chroma_aniso = static_cast<float>(sc->chroma_aniso);
+  cfg.charbonnier_eps = static_cast<float>(sc->charbonnier_eps);
+  cfg.chroma_eps = static_cast<float>(sc->chroma_eps);
+  cfg.structure_coupling = static_cast<float>(sc->structure_coupling);
+  cfg.diag_prior = static_cast<float>(sc->diag_prior);
+  cfg.cg_iterations = sc->cg_iterations;
+  cfg.irls_outer = sc->irls_outer;
+  cfg.cg_tol = static_cast<float>(sc->cg_tol);
+  cfg.fast = sc->fast != 0;
+  cfg.is_pal = sc->is_pal != 0;
+  return cfg;
+}
+
+hvd::HvdConfig ToSeqCfg(const ShimSeqCfg* q) {
+  hvd::HvdConfig cfg = ToCfg(&q->base);
+  cfg.temporal_strength = static_cast<float>(q->temporal_strength);
+  cfg.temporal_eps = static_cast<float>(q->temporal_eps);
+  cfg.nr_anchor = static_cast<float>(q->nr_anchor);
+  cfg.nr_eps = static_cast<float>(q->nr_eps);
+  cfg.coherence_gate = static_cast<float>(q->coherence_gate);
+  cfg.odd_gate_floor = static_cast<float>(q->odd_gate_floor);
+  cfg.passes = q->passes;
+  cfg.nr_radius = q->nr_radius;
+  cfg.mc_tile = q->mc_tile;
+  cfg.mc_search = q->mc_search;
+  cfg.bidirectional = q->bidirectional != 0;
+  cfg.extended_temporal = q->extended_temporal != 0;
+  cfg.trajectory_fit = q->trajectory_fit != 0;
+  cfg.psi_init = q->psi_init != 0;
+  cfg.output_fidelity = q->output_fidelity != 0;
+  cfg.parallel_across_fields = q->parallel_across_fields != 0;
+  return cfg;
+}
 
 hvd::FieldGeometry ToGeom(const ShimGeom* s) {
   hvd::FieldGeometry g;
@@ -257,6 +306,101 @@ void ref_motion_compensate_prev(const float* comp, const float* car, int h, int 
   std::memcpy(out_comp, r.composite.data(), sizeof(float) * r.composite.size());
   std::memcpy(out_car, r.carrier.data(), sizeof(float) * 2 * r.carrier.size());
   std::memcpy(out_conf, r.confidence.data(), sizeof(float) * r.confidence.size());
+}
+
+float ref_estimate_noise_ire(const float* s, int h, int w) { return hvd::EstimateNoiseIre(ToPlane(s, h, w)); }
+
+void ref_verify_motion(const float* y_ref, const float* y_cur, int h, int w, const float* pdy, const float* pdx,
+                       int th, int tw, int tile, float* dy, float* dx, float* conf) {
+  const hvd::MotionField m = hvd::VerifyMotion(ToPlane(y_ref, h, w), ToPlane(y_cur, h, w), ToPlane(pdy, th, tw),
+                                               ToPlane(pdx, th, tw), tile);
+  std::memcpy(dy, m.dy.data(), sizeof(float) * m.dy.size());
+  std::memcpy(dx, m.dx.data(), sizeof(float) * m.dx.size());
+  std::memcpy(conf, m.confidence.data(), sizeof(float) * m.confidence.size());
+}
+
+// TrajectorySnap over m measurements (offset, parity shift, th x tw dy, dx,
+// confidence planes back to back); dy, dx in place.
+void ref_trajectory_snap(int m, const int* offsets, const float* pshift, float* dy, float* dx, const float* conf,
+                         int th, int tw) {
+  std::vector<hvd::OffsetMotion> pm(m);
+  const size_t nt = static_cast<size_t>(th) * tw;
+  for (int j = 0; j < m; ++j) {
+    pm[j].offset = offsets[j];
+    pm[j].parity_shift = pshift[j];
+    pm[j].field.dy = ToPlane(dy + j * nt, th, tw);
+    pm[j].field.dx = ToPlane(dx + j * nt, th, tw);
+    pm[j].field.confidence = ToPlane(conf + j * nt, th, tw);
+  }
+  hvd::TrajectorySnap(&pm);
+  for (int j = 0; j < m; ++j) {
+    std::memcpy(dy + j * nt, pm[j].field.dy.data(), sizeof(float) * nt);
+    std::memcpy(dx + j * nt, pm[j].field.dx.data(), sizeof(float) * nt);
+  }
+}
+
+// VariationalRefineJoint with nn neighbours and an optional anchor (a_*
+// null: none).  y, chi: the start in, the result out.
+void ref_joint_refine(const float* s, const float* carrier, int h, int w, const ShimCfg* sc, int nn,
+                      const float* sw, const float* cw, const float* conf, float nu, float eps_t,
+                      const float* a_luma, const float* a_chroma, const float* a_weight, float nu_a, float* y,
+                      float* chi) {
+  hvd::HvdConfig cfg = ToCfg(sc);
+  cfg.temporal_strength = nu;
+  cfg.temporal_eps = eps_t;
+  cfg.nr_anchor = nu_a;
+  std::vector<hvd::NeighborTerm> nb(nn);
+  const size_t np = static_cast<size_t>(h) * w;
+  for (int j = 0; j < nn; ++j) {
+    nb[j].composite = ToPlane(sw + j * np, h, w);
+    nb[j].carrier = ToCPlane(cw + j * 2 * np, h, w);
+    nb[j].confidence = ToPlane(conf + j * np, h, w);
+  }
+  hvd::AnchorTerm anchor;
+  if (a_weight) {
+    anchor.luma = ToPlane(a_luma, h, w);
+    anchor.chroma = ToCPlane(a_chroma, h, w);
+    anchor.weight = ToPlane(a_weight, h, w);
+  }
+  const hvd::RefineResult r = hvd::VariationalRefineJoint(ToPlane(s, h, w), ToCPlane(carrier, h, w),
+                                                          ToPlane(y, h, w), ToCPlane(chi, h, w), cfg, nb,
+                                                          a_weight ? &anchor : nullptr);
+  std::memcpy(chi, r.chroma.data(), sizeof(float) * 2 * r.chroma.size());
+  std::memcpy(y, r.luma.data(), sizeof(float) * r.luma.size());
+}
+
+// DecodeFieldWindowWithInits over nf fields of h x w (s, carrier, chi0 and,
+// when init_y is non-null, Y0 planes back to back; parity per field).
+// diag receives resolved_strength, ambiguity_ire, sigma_ire, temporal_eps,
+// nr_eps.  Returns the number of fields decoded.
+int ref_decode_window_with_inits(int nf, int h, int w, const float* s, const float* carrier, const int* parity,
+                                 const float* init_chi, const float* init_y, const ShimGeom* sg,
+                                 const ShimSeqCfg* q, float* out_y, float* out_chi, float* diag) {
+  const size_t np = static_cast<size_t>(h) * w;
+  std::vector<hvd::FieldObs> fields(nf);
+  std::vector<hvd::DecodedField> inits(nf);
+  for (int j = 0; j < nf; ++j) {
+    fields[j].s = ToPlane(s + j * np, h, w);
+    fields[j].carrier = ToCPlane(carrier + j * 2 * np, h, w);
+    fields[j].parity = parity[j];
+    inits[j].chroma = ToCPlane(init_chi + j * 2 * np, h, w);
+    if (init_y) inits[j].luma = ToPlane(init_y + j * np, h, w);
+  }
+  hvd::SequenceDiagnostics d;
+  const std::vector<hvd::DecodedField> out =
+      hvd::DecodeFieldWindowWithInits(fields, inits, ToGeom(sg), ToSeqCfg(q), &d);
+  for (size_t j = 0; j < out.size(); ++j) {
+    std::memcpy(out_y + j * np, out[j].luma.data(), sizeof(float) * np);
+    std::memcpy(out_chi + j * 2 * np, out[j].chroma.data(), sizeof(float) * 2 * np);
+  }
+  if (diag) {
+    diag[0] = d.resolved_strength;
+    diag[1] = d.ambiguity_ire;
+    diag[2] = d.sigma_ire;
+    diag[3] = d.temporal_eps;
+    diag[4] = d.nr_eps;
+  }
+  return static_cast<int>(out.size());
 }
 
 }  // extern "C"
