@@ -1005,7 +1005,7 @@ GroupLayout group_layout(Carve cv, const DevField* F, int n, int is_pal) {
 // Workspace of the per-pixel solve (pixsolve.hip) for fields F, field i with
 // nn[i] neighbours, carved after `cv`.
 struct PxLayout {
-  std::vector<size_t> chi1, d, g, w, wd, part;
+  std::vector<size_t> chi1, d, g, w, wd, part, y1, dyj, gyj;
   std::vector<std::vector<size_t>> nb;
   std::vector<int> cp;  // chunk prefix
   std::vector<int> sp;  // aniso sample prefix
@@ -1017,10 +1017,11 @@ size_t px_desc_bytes(int n) {
          align256(sizeof(FieldCG) * n) + align256(sizeof(int) * (n + 1));
 }
 
-PxLayout px_layout(Carve cv, const std::vector<DevField>& F, const std::vector<int>& nn, bool diag, int is_pal) {
+PxLayout px_layout(Carve cv, const std::vector<DevField>& F, const std::vector<int>& nn, bool diag, int is_pal,
+                   bool joint = false) {
   const int n = (int)F.size();
   PxLayout L;
-  for (auto* v : {&L.chi1, &L.d, &L.g, &L.w, &L.wd, &L.part}) v->resize(n);
+  for (auto* v : {&L.chi1, &L.d, &L.g, &L.w, &L.wd, &L.part, &L.y1, &L.dyj, &L.gyj}) v->resize(n);
   L.nb.resize(n);
   L.cp.assign(n + 1, 0);
   L.sp.assign(n + 1, 0);
@@ -1036,6 +1037,11 @@ PxLayout px_layout(Carve cv, const std::vector<DevField>& F, const std::vector<i
     L.wd[i] = diag ? cv.take(sizeof(float2) * np) : 0;
     for (int j = 0; j < (nn.empty() ? 0 : nn[i]); ++j) L.nb[i].push_back(cv.take(sizeof(float4) * np));
     L.part[i] = cv.take(sizeof(double) * 4 * nc);
+    if (joint) {
+      L.y1[i] = cv.take(sizeof(float) * np);
+      L.dyj[i] = cv.take(sizeof(float) * np * 2);
+      L.gyj[i] = cv.take(sizeof(float) * np);
+    }
   }
   L.px = cv.take(sizeof(PxState) * n);
   L.st = cv.take(sizeof(FieldCG) * n);
@@ -1317,9 +1323,18 @@ namespace {
 
 // Shared by the plane-level refine and the field driver: carve the solver
 // workspace after `cv` and run.
+// One field's extra planes in the joint solve (VariationalRefineJoint).
+struct PxJointField {
+  float* y;  // Y0 in, Y out
+  const float* a_luma;
+  const float2* a_chroma;
+  const float* a_weight;  // null: no anchor
+};
+
 void refine_pixel(hvd_gpu_ctx* ctx, const std::vector<DevField>& F,
                   const std::vector<std::vector<hvd_gpu_neighbor>>& N, const hvd_solver_params& p, float nu,
-                  float eps_t, hipStream_t st, Carve cv, RefineOut out);
+                  float eps_t, hipStream_t st, Carve cv, RefineOut out,
+                  const std::vector<PxJointField>* joint = nullptr, float nu_a = 0.f);
 
 void refine_planes(hvd_gpu_ctx* ctx, std::vector<DevField>& F, const hvd_solver_params& p, hipStream_t st,
                    Carve cv, RefineOut out) {
@@ -1591,14 +1606,16 @@ void refine_entry(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hv
 // eps_t.  All fields advance together, one wave per chunk of kPxChunk pixels.
 void refine_pixel(hvd_gpu_ctx* ctx, const std::vector<DevField>& F,
                   const std::vector<std::vector<hvd_gpu_neighbor>>& N, const hvd_solver_params& p, float nu,
-                  float eps_t, hipStream_t st, Carve cv, RefineOut out) {
+                  float eps_t, hipStream_t st, Carve cv, RefineOut out, const std::vector<PxJointField>* joint,
+                  float nu_a) {
   const int nf = (int)F.size();
   if (nf == 0) return;
   std::vector<int> nn(nf, 0);
   if (!N.empty())
     for (int i = 0; i < nf; ++i) nn[i] = (int)N[i].size();
   const bool diag = p.diag_prior > 0.0;
-  const PxLayout L = px_layout(cv, F, nn, diag, p.is_pal);
+  const bool jt = joint != nullptr;
+  const PxLayout L = px_layout(cv, F, nn, diag, p.is_pal, jt);
   REQUIRE(ctx, L.end <= ctx->ws_cap, "internal: pixel-solve workspace not sized");
   char* ws = ctx->ws;
   const int outer = std::max(1, p.irls_outer);
@@ -1631,6 +1648,19 @@ void refine_pixel(hvd_gpu_ctx* ctx, const std::vector<DevField>& F,
     }
     q.nu = nn[i] > 0 ? nu : 0.f;
     q.et2 = eps_t * eps_t;
+    if (jt) {
+      const PxJointField& J = (*joint)[i];
+      q.joint = 1;
+      q.yv[0] = J.y;
+      q.yv[1] = (float*)(ws + L.y1[i]);
+      q.dyv[0] = (float*)(ws + L.dyj[i]);
+      q.dyv[1] = q.dyv[0] + np;
+      q.gy = (float*)(ws + L.gyj[i]);
+      q.a_luma = J.a_luma;
+      q.a_chroma = J.a_chroma;
+      q.a_weight = J.a_weight;
+      q.nu_a = J.a_weight ? nu_a : 0.f;
+    }
     q.nchunks = L.cp[i + 1] - L.cp[i];
     q.part = (double*)(ws + L.part[i]);
     q.px = dpx + i;
@@ -1660,19 +1690,23 @@ void refine_pixel(hvd_gpu_ctx* ctx, const std::vector<DevField>& F,
   for (int i = 0; i < nf; ++i) any_nb = any_nb || nn[i] > 0;
   if (any_nb) Lq.kernel((const void*)k_px_prep, grid, blk, 0, dP, dcp, nf);
   int cur = 0;
+  const void* kw = jt ? (const void*)k_pj_weights : (const void*)k_px_weights;
+  const void* kg = jt ? (const void*)k_pj_grad0 : (const void*)k_px_grad0;
+  const void* kd = jt ? (const void*)k_pj_dir : (const void*)k_px_dir;
+  const void* ks = jt ? (const void*)k_pj_step : (const void*)k_px_step;
   for (int o = 0; o < outer; ++o) {
-    Lq.kernel((const void*)k_px_weights, grid, blk, 0, dP, dcp, nf, cur, o);
-    Lq.kernel((const void*)k_px_grad0, grid, blk, 0, dP, dcp, nf, cur);
+    Lq.kernel(kw, grid, blk, 0, dP, dcp, nf, cur, o);
+    Lq.kernel(kg, grid, blk, 0, dP, dcp, nf, cur);
     Lq.mark(o, false);
     for (int it = 0; it < n_inner; ++it) {
-      Lq.kernel((const void*)k_px_dir, grid, blk, 0, dP, dcp, nf, it, o);
-      Lq.kernel((const void*)k_px_step, grid, blk, 0, dP, dcp, nf, it, cur, o);
+      Lq.kernel(kd, grid, blk, 0, dP, dcp, nf, it, o);
+      Lq.kernel(ks, grid, blk, 0, dP, dcp, nf, it, cur, o);
     }
     Lq.mark(o, true);
     Lq.loop_iters.push_back(n_inner);
     cur ^= n_inner & 1;
   }
-  Lq.kernel((const void*)k_px_finalize, grid, blk, 0, dP, dcp, nf, cur);
+  Lq.kernel(jt ? (const void*)k_pj_finalize : (const void*)k_px_finalize, grid, blk, 0, dP, dcp, nf, cur);
   run_launches(ctx, Lq, ctx->graphs && !ctx->no_alloc, st);
   arm(ctx, sl, st);
   if (out.stage_iters || out.aniso) {
@@ -1918,6 +1952,111 @@ int hvd_gpu_variational_refine_temporal(hvd_gpu_ctx* ctx, const hvd_gpu_plane* p
     for (int j = 0; j < nn; ++j)
       REQUIRE(ctx, nbrs[j].composite && nbrs[j].carrier && nbrs[j].confidence, "neighbour %d: null plane", j);
     refine_temporal(ctx, *plane, nbrs, nn, *p, temporal_strength, temporal_eps, stage_iters, S(stream));
+  });
+}
+
+}  // extern "C"
+
+namespace {
+
+// The batched per-pixel solves: VariationalRefine with neighbours or
+// VariationalRefineJoint (anchors non-null or joint set), n planes, plane i
+// with nn[i] neighbours from nbrs (back to back).
+void pixel_batch_entry(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hvd_gpu_neighbor* nbrs,
+                       const int* nn, const hvd_gpu_anchor* anchors, bool joint, const hvd_solver_params* p,
+                       float nu, float eps_t, float nu_a, int* stage_iters, void* stream) {
+  REQUIRE(ctx, n >= 0 && (n == 0 || planes), "null argument");
+  check_params(ctx, p);
+  if (n == 0) return;
+  REQUIRE(ctx, n <= 65535, "%d planes in one call (at most 65535)", n);
+  std::vector<std::vector<hvd_gpu_neighbor>> N(n);
+  int off = 0;
+  bool any = false;
+  for (int i = 0; i < n; ++i) {
+    check_plane(ctx, planes[i], i);
+    const int k = nn ? nn[i] : 0;
+    REQUIRE(ctx, k >= 0 && k <= kMaxNeighbors, "plane %d: neighbour count %d outside 0..%d", i, k, kMaxNeighbors);
+    REQUIRE(ctx, k == 0 || nbrs, "null neighbour array");
+    for (int j = 0; j < k; ++j) {
+      const hvd_gpu_neighbor& b = nbrs[off + j];
+      REQUIRE(ctx, b.composite && b.carrier && b.confidence, "plane %d neighbour %d: null plane", i, j);
+      N[i].push_back(b);
+    }
+    off += k;
+    any = any || k > 0;
+    if (joint) REQUIRE(ctx, planes[i].y, "plane %d: the joint solve needs Y0 in planes[i].y", i);
+    if (anchors && anchors[i].weight)
+      REQUIRE(ctx, anchors[i].luma && anchors[i].chroma, "plane %d: incomplete anchor", i);
+  }
+  if (any && nu > 0.f)
+    REQUIRE(ctx, eps_t > 0.f, "temporal_eps must be positive (resolve it first, as sequence.cpp:445-446 does)");
+  hipStream_t st = S(stream);
+  Carve cv;
+  std::vector<size_t> o_row(n), o_vsw(n), o_col(n);
+  std::vector<DevField> F(n);
+  for (int i = 0; i < n; ++i) {
+    o_row[i] = cv.take(sizeof(float2) * planes[i].lines);
+    o_vsw[i] = cv.take(planes[i].lines);
+    o_col[i] = cv.take(sizeof(float2) * planes[i].width);
+    DevField f{};
+    f.h = planes[i].lines;
+    f.w = planes[i].width;
+    f.s = planes[i].s;
+    f.carrier = reinterpret_cast<const float2*>(planes[i].carrier);
+    f.chi[0] = reinterpret_cast<float2*>(planes[i].chi);
+    f.y = joint ? nullptr : planes[i].y;
+    F[i] = f;
+  }
+  std::vector<int> nnv(n);
+  for (int i = 0; i < n; ++i) nnv[i] = (int)N[i].size();
+  ensure_ws(ctx, px_layout(cv, F, nnv, p->diag_prior > 0.0, p->is_pal, joint).end, st);
+  char* ws = ctx->ws;
+  std::vector<PrepDesc> prep;
+  for (int i = 0; i < n; ++i)
+    if (!planes[i].carrier) {
+      prep.push_back(plane_tables(planes[i], (float2*)(ws + o_row[i]), (int8_t*)(ws + o_vsw[i]),
+                                  (float2*)(ws + o_col[i])));
+      F[i].rowph = (const float2*)(ws + o_row[i]);
+      F[i].colph = (const float2*)(ws + o_col[i]);
+      F[i].vsw = planes[i].vswitch ? (const int8_t*)(ws + o_vsw[i]) : nullptr;
+    }
+  run_prepare(ctx, prep, false, st);
+  std::vector<PxJointField> J;
+  if (joint) {
+    J.resize(n);
+    for (int i = 0; i < n; ++i) {
+      J[i].y = planes[i].y;
+      if (anchors && anchors[i].weight) {
+        J[i].a_luma = anchors[i].luma;
+        J[i].a_chroma = reinterpret_cast<const float2*>(anchors[i].chroma);
+        J[i].a_weight = anchors[i].weight;
+      }
+    }
+  }
+  refine_pixel(ctx, F, N, *p, nu, eps_t, st, cv, RefineOut{stage_iters, nullptr}, joint ? &J : nullptr, nu_a);
+}
+
+}  // namespace
+
+extern "C" {
+
+int hvd_gpu_variational_refine_temporal_batch(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n,
+                                              const hvd_gpu_neighbor* nbrs, const int* nn,
+                                              const hvd_solver_params* p, float temporal_strength,
+                                              float temporal_eps, int* stage_iters, void* stream) {
+  return guarded(ctx, [&] {
+    pixel_batch_entry(ctx, planes, n, nbrs, nn, nullptr, false, p, temporal_strength, temporal_eps, 0.f,
+                      stage_iters, stream);
+  });
+}
+
+int hvd_gpu_variational_refine_joint(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n,
+                                     const hvd_gpu_neighbor* nbrs, const int* nn, const hvd_gpu_anchor* anchors,
+                                     const hvd_solver_params* p, float temporal_strength, float temporal_eps,
+                                     float nr_anchor, int* stage_iters, void* stream) {
+  return guarded(ctx, [&] {
+    pixel_batch_entry(ctx, planes, n, nbrs, nn, anchors, true, p, temporal_strength, temporal_eps, nr_anchor,
+                      stage_iters, stream);
   });
 }
 
@@ -2761,6 +2900,73 @@ int hvd_gpu_weave_frame_s16(hvd_gpu_ctx* ctx, const int16_t* src, const hvd_fram
                        o);
     LAUNCH_CHECK(ctx);
     arm(ctx, sl, st);
+  });
+}
+
+}  // extern "C"
+
+namespace {
+
+// The woven picture's extent (hvd_chroma_decoder_stage.cpp:328-363 raster,
+// :336-353 single field).
+WovenDesc woven_geometry(const hvd_frame_params& fp, int full_raster, int field) {
+  WovenDesc D{};
+  const int fw = fp.frame_width, fh = fp.frame_height, f1 = fp.field1_lines;
+  const int a0 = full_raster ? 0 : std::max(0, fp.active_video_start);
+  const int a1 = full_raster ? fw : (fp.active_video_end > a0 ? fp.active_video_end : fw);
+  const int y0 = full_raster ? 0 : std::max(0, fp.first_active_frame_line);
+  const int y1 = full_raster ? fh : (fp.last_active_frame_line > y0 ? fp.last_active_frame_line : fh);
+  D.a0 = a0;
+  D.width = std::max(0, a1 - a0);
+  D.y0 = y0;
+  D.field = (field == 0 || field == 1) ? field : -1;
+  D.f1 = f1;
+  D.fw = fw;
+  D.fh = fh;
+  if (D.field >= 0) {
+    // field lines fl with y0 <= 2 fl + field < y1, fl < that field's line count
+    const int nfl = D.field == 0 ? f1 : fh - f1;
+    int lo = 0;
+    while (2 * lo + D.field < y0) ++lo;
+    int hi = lo;
+    while (hi < nfl && 2 * hi + D.field < y1) ++hi;
+    D.fl0 = lo;
+    D.height = std::max(0, hi - lo);
+  } else {
+    D.height = std::max(0, y1 - y0);
+  }
+  if (D.width == 0 || D.height == 0) D.width = D.height = 0;
+  D.black = (double)fp.black_level;
+  D.range = (double)(fp.white_level - fp.black_level);  // a float difference, as at :554
+  return D;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hvd_gpu_woven_size(const hvd_frame_params* fp, int full_raster, int field, int* width, int* height) {
+  if (!fp || !width || !height || fp->frame_width <= 0 || fp->frame_height <= 0) return HVD_GPU_ERR_INVALID;
+  const WovenDesc D = woven_geometry(*fp, full_raster, field);
+  *width = D.width;
+  *height = D.height;
+  return HVD_GPU_OK;
+}
+
+int hvd_gpu_woven_picture(hvd_gpu_ctx* ctx, const hvd_yc_frame* yc, const hvd_frame_params* fp, int full_raster,
+                          int field, double* y, double* u, double* v, uint8_t* rgb24, void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, yc && fp && yc->luma && yc->u_plane && yc->v_plane, "null argument");
+    REQUIRE(ctx, fp->frame_width > 0 && fp->frame_height > 0, "empty frame");
+    REQUIRE(ctx, fp->field1_lines >= 0 && fp->field1_lines <= fp->frame_height, "field1_lines outside the frame");
+    if (rgb24)
+      REQUIRE(ctx, fp->white_level > fp->black_level && fp->white_level > fp->blanking_level,
+              "white level must exceed black and blanking for the RGB rendering");
+    const WovenDesc D = woven_geometry(*fp, full_raster, field);
+    if (D.width == 0 || D.height == 0 || !(y || u || v || rgb24)) return;
+    hipLaunchKernelGGL(k_woven_rgb, dim3((D.width + 255) / 256, D.height), dim3(256), 0, S(stream), D, yc->luma,
+                       yc->u_plane, yc->v_plane, y, u, v, rgb24);
+    LAUNCH_CHECK(ctx);
   });
 }
 

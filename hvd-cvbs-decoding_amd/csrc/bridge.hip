@@ -246,4 +246,49 @@ __global__ void k_chroma_plane(int mode, const float* s, const float* yl, const 
   }
 }
 
+// The woven active picture of a field-sequential YcFrameS16 and its RGB24
+// rendering: ReorderToWoven (hvd_chroma_decoder_stage.cpp:316-413; raster
+// rows alternate fields, or one field's lines in the single-field view) and
+// YuvToRgb8 (:79-88) over the luma excursion black..white
+// (WovenToPreviewImage / WriteWovenAsRgb24, :500-638), in double as there.
+// One thread per output pixel; grid (ceil(width / 256), height).
+__global__ void k_woven_rgb(WovenDesc D, const int16_t* luma, const double* u, const double* v, double* yo,
+                            double* uo, double* vo, uint8_t* rgb) {
+#pragma clang fp contract(off)
+  const int row = blockIdx.y;
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= D.width || row >= D.height) return;
+  int flat_line;
+  if (D.field == 0 || D.field == 1) {
+    const int fl = D.fl0 + row;
+    flat_line = D.field == 0 ? fl : D.f1 + fl;
+  } else {
+    const int frame_line = D.y0 + row;
+    const int field_line = frame_line / 2;
+    flat_line = (frame_line % 2 == 0) ? field_line : D.f1 + field_line;
+  }
+  const int flat_col = D.a0 + col;
+  double yv = 0.0, uv = 0.0, vv = 0.0;
+  if (flat_line >= 0 && flat_line < D.fh && flat_col >= 0 && flat_col < D.fw) {
+    const size_t src = (size_t)flat_line * D.fw + flat_col;
+    yv = (double)luma[src];
+    uv = u[src];
+    vv = v[src];
+  }
+  const size_t dst = (size_t)row * D.width + col;
+  if (yo) yo[dst] = yv;
+  if (uo) uo[dst] = uv;
+  if (vo) vo[dst] = vv;
+  if (rgb) {
+    const double ny = (yv - D.black) / D.range;
+    const double nu = uv / D.range;
+    const double nv = vv / D.range;
+    const double c3[3] = {ny + 1.13983 * nv, ny - 0.39465 * nu - 0.58060 * nv, ny + 2.03211 * nu};
+    for (int k = 0; k < 3; ++k) {
+      const double cl = fmin(fmax(c3[k], 0.0), 1.0);
+      rgb[dst * 3 + k] = (uint8_t)(cl * 255.0 + 0.5);
+    }
+  }
+}
+
 }  // namespace hvdk

@@ -177,6 +177,17 @@ struct PxField {
   double* part;             // 4 * nchunks: |g|^2 by parity, curv(d), Re<g, d>
   PxState* px;
   const FieldCG* st;        // mu_h, mu_v, mu_d, lam_y, eps, eps_c, k, exits, n_inner
+  // the joint solve (VariationalRefineJoint): Y a free unknown beside chi.
+  // nb[j] then holds {S_w, c_w, wt} (not the differences), and the anchor
+  // (luma, chroma, weight) is optional
+  int joint;
+  float nu_a;
+  float* yv[2];             // Y ping-pong (yv[0] the caller's: Y0 in, Y out)
+  float* dyv[2];            // dY ping-pong
+  float* gy;                // dE/dY
+  const float* a_luma;
+  const float2* a_chroma;
+  const float* a_weight;
 };
 
 __global__ void k_px_prep(const PxField* F, const int* cp, int nf);
@@ -185,6 +196,11 @@ __global__ void k_px_grad0(const PxField* F, const int* cp, int nf, int cur);
 __global__ void k_px_dir(const PxField* F, const int* cp, int nf, int it, int stage);
 __global__ void k_px_step(const PxField* F, const int* cp, int nf, int it, int cur, int stage);
 __global__ void k_px_finalize(const PxField* F, const int* cp, int nf, int cur);
+__global__ void k_pj_weights(const PxField* F, const int* cp, int nf, int cur, int stage);
+__global__ void k_pj_grad0(const PxField* F, const int* cp, int nf, int cur);
+__global__ void k_pj_dir(const PxField* F, const int* cp, int nf, int it, int stage);
+__global__ void k_pj_step(const PxField* F, const int* cp, int nf, int it, int cur, int stage);
+__global__ void k_pj_finalize(const PxField* F, const int* cp, int nf, int cur);
 
 // warp.hip
 __global__ void k_vpp_snap(int th, int tw, const float* mdy, const float* mdx, float* sy, float* sx);
@@ -199,7 +215,17 @@ __global__ void k_coh_terms(size_t n, const float2* z1, const float2* z2, float*
                             float* m2);
 __global__ void k_coh_out(size_t n, const float* bre, const float* bim, const float* b1, const float* b2, float* out);
 
+// The woven active picture (hvd_chroma_decoder_stage.cpp:316-413): output
+// width x height; columns from a0; rows from frame line y0 (raster view) or,
+// in the single-field view (field 0 / 1), that field's lines from fl0.
+struct WovenDesc {
+  int width, height, a0, y0, field, fl0, f1, fw, fh, pad_;
+  double black, range;  // luma excursion black .. white of the RGB rendering
+};
+
 // bridge.hip
+__global__ void k_woven_rgb(WovenDesc D, const int16_t* luma, const double* u, const double* v, double* yo,
+                            double* uo, double* vo, uint8_t* rgb);
 __global__ void k_split_fields(const BridgeFrameParams* FP, const int16_t* const* frames, float zero,
                                float* const* fields);
 __global__ void k_window_decisions(const BridgeFrameParams* FP, const float* const* fields, int nframes,

@@ -164,7 +164,9 @@ __global__ void __launch_bounds__(64) k_px_prep(const PxField* __restrict__ F, c
     const float s = P.f.s[i];
     for (int j = 0; j < P.nn; ++j) {
       const float2 cw = P.cw[j][i];
-      P.nb[j][i] = make_float4(P.sw[j][i] - s, car.x - cw.x, car.y - cw.y, 0.f);
+      // the joint solve keeps the neighbour's own composite and carrier
+      P.nb[j][i] = P.joint ? make_float4(P.sw[j][i], cw.x, cw.y, 0.f)
+                           : make_float4(P.sw[j][i] - s, car.x - cw.x, car.y - cw.y, 0.f);
     }
   }
 }
@@ -421,6 +423,356 @@ __global__ void __launch_bounds__(64) k_px_finalize(const PxField* __restrict__ 
       const float2 car = px_carrier(f, y, x);
       f.y[i] = f.s[i] - (chi.x * car.x - chi.y * car.y);
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The joint solve (variational.cpp:558-891, VariationalRefineJoint): Y is a
+// free unknown beside chi.  The objective is the exact data term
+// (S - Y - Re[chi c])^2, per neighbour nu wt (S_w - Y - Re[chi c_w])^2, the
+// anchor nu_a w_a (|Y - Y_a|^2 + |chi - chi_a|^2), Charbonnier priors on
+// grad Y and the 2-D chroma priors (the oriented ones too); the weights are
+// lagged per stage and scaled by 1 + nu sum wt.  The same launch structure,
+// reductions and exit bookkeeping as the chi-only kernels above.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct PjSample {
+  float y;
+  float2 chi;
+};
+
+__device__ inline PjSample pj_sample(const PxField& P, const float* yv, const float2* chi, const float* dy,
+                                     const float2* dc, float a, int y, int x) {
+#pragma clang fp contract(off)
+  const size_t i = (size_t)y * P.f.w + x;
+  PjSample q;
+  q.y = yv[i];
+  q.chi = chi[i];
+  if (dy) {
+    q.y += a * dy[i];
+    const float2 d = dc[i];
+    q.chi.x += a * d.x;
+    q.chi.y += a * d.y;
+  }
+  return q;
+}
+
+// (dE/dY, dE/dchi) at pixel (y, x) (:615-683).
+__device__ inline void pj_grad(const PxField& P, const FieldCG& s, const float* yv, const float2* chi,
+                               const float* dyv, const float2* dcv, float a, int y, int x, float* gy_out,
+                               float2* gc_out, PjSample* at) {
+#pragma clang fp contract(off)
+  const DevField& f = P.f;
+  const int xl = max(x - 1, 0), xr = min(x + 1, f.w - 1), yu = max(y - 1, 0), yd = min(y + 1, f.h - 1);
+  const PjSample q0 = pj_sample(P, yv, chi, dyv, dcv, a, y, x), ql = pj_sample(P, yv, chi, dyv, dcv, a, y, xl),
+                 qr = pj_sample(P, yv, chi, dyv, dcv, a, y, xr), qu = pj_sample(P, yv, chi, dyv, dcv, a, yu, x),
+                 qd = pj_sample(P, yv, chi, dyv, dcv, a, yd, x);
+  if (at) *at = q0;
+  const size_t i = (size_t)y * f.w + x;
+  const float2 c0 = px_carrier(f, y, x);
+  const float r0 = f.s[i] - q0.y - (q0.chi.x * c0.x - q0.chi.y * c0.y);
+  const float fr = -2.f * r0;
+  float gy = fr;
+  float2 gc = make_float2(fr * c0.x, fr * -c0.y);
+  for (int j = 0; j < P.nn; ++j) {
+    const float4 nb = P.nb[j][i];  // S_w, c_w, wt
+    const float rk = nb.x - q0.y - (q0.chi.x * nb.y - q0.chi.y * nb.z);
+    const float fk = -2.f * P.nu * nb.w * rk;
+    gy += fk;
+    gc.x += fk * nb.y;
+    gc.y += fk * -nb.z;
+  }
+  if (P.a_weight && P.nu_a > 0.f) {
+    const float fa = 2.f * P.nu_a * P.a_weight[i];
+    const float2 ac = P.a_chroma[i];
+    gy += fa * (q0.y - P.a_luma[i]);
+    gc.x += fa * (q0.chi.x - ac.x);
+    gc.y += fa * (q0.chi.y - ac.y);
+  }
+  const float4 w0 = P.wts[i];
+  const float4 wl = x > 0 ? P.wts[i - 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const float4 wu = y > 0 ? P.wts[i - f.w] : make_float4(0.f, 0.f, 0.f, 0.f);
+  // DxT(wx Dx Y), then DyT(wy Dy Y), each added as hvd-core adds them
+  gy += 2.f * (wl.x * (q0.y - ql.y) - w0.x * (qr.y - q0.y));
+  gy += 2.f * (wu.y * (q0.y - qu.y) - w0.y * (qd.y - q0.y));
+  const float2 cx0 = cscale(w0.z, csub(qr.chi, q0.chi)), cxl = cscale(wl.z, csub(q0.chi, ql.chi));
+  const float2 cy0 = cscale(w0.w, csub(qd.chi, q0.chi)), cyu = cscale(wu.w, csub(q0.chi, qu.chi));
+  gc.x += 2.f * s.mu_h * (cxl.x - cx0.x);
+  gc.y += 2.f * s.mu_h * (cxl.y - cx0.y);
+  gc.x += 2.f * s.mu_v * (cyu.x - cy0.x);
+  gc.y += 2.f * s.mu_v * (cyu.y - cy0.y);
+  if (P.wd) {
+    const float2 wd0 = P.wd[i];
+    const float2 pdr = pj_sample(P, yv, chi, dyv, dcv, a, yd, xr).chi, pdl = pj_sample(P, yv, chi, dyv, dcv, a, yd, xl).chi;
+    const float2 pul = pj_sample(P, yv, chi, dyv, dcv, a, yu, xl).chi, pur = pj_sample(P, yv, chi, dyv, dcv, a, yu, xr).chi;
+    const float wd1_ul = (y > 0 && x > 0) ? P.wd[i - f.w - 1].x : 0.f;
+    const float wd2_ur = (y > 0 && x + 1 < f.w) ? P.wd[i - f.w + 1].y : 0.f;
+    const float2 a1 = cscale(wd0.x, csub(pdr, q0.chi)), a1u = cscale(wd1_ul, csub(q0.chi, pul));
+    const float2 a2 = cscale(wd0.y, csub(pdl, q0.chi)), a2u = cscale(wd2_ur, csub(q0.chi, pur));
+    gc.x += 2.f * s.mu_d * (-a1.x + a1u.x);
+    gc.y += 2.f * s.mu_d * (-a1.y + a1u.y);
+    gc.x += 2.f * s.mu_d * (-a2.x + a2u.x);
+    gc.y += 2.f * s.mu_d * (-a2.y + a2u.y);
+  }
+  *gy_out = gy;
+  *gc_out = gc;
+}
+
+}  // namespace
+
+// The joint solve's stage weights (:773-858): Charbonnier on grad Y, the
+// coupled chroma diffusivities, the diagonal ones, the neighbours' robust
+// weights on S_w - Y - Re[chi c_w] and the prior scale 1 + nu sum wt.
+__global__ void __launch_bounds__(64) k_pj_weights(const PxField* __restrict__ F, const int* __restrict__ cp, int nf,
+                                                   int cur, int stage) {
+#pragma clang fp contract(off)
+  const PxCtx c = px_ctx(cp, nf);
+  const PxField& P = F[c.fi];
+  const FieldCG s = *P.st;
+  if (c.chunk == 0 && px_lane() == 0) {
+    P.px->done[0] = 0;
+    P.px->done[1] = 0;
+    if (stage < 16) P.px->iters[stage] = s.n_inner;
+  }
+  const DevField& f = P.f;
+  const float2* chi = f.chi[cur];
+  const float* yv = P.yv[cur];
+  const size_t n = (size_t)f.h * f.w;
+  const float e2 = s.eps * s.eps, ec2 = s.eps_c * s.eps_c;
+  for (int k = 0; k < kPxPerLane; ++k) {
+    const size_t i = (size_t)c.chunk * kPxChunk + (size_t)k * 64 + px_lane();
+    if (i >= n) break;
+    const int y = (int)(i / f.w), x = (int)(i % f.w);
+    const bool hr = x + 1 < f.w, hd = y + 1 < f.h, hl = x > 0;
+    const float y0 = yv[i];
+    const float2 q0 = chi[i];
+    const float gx = hr ? yv[i + 1] - y0 : 0.f;
+    const float gyv = hd ? yv[i + f.w] - y0 : 0.f;
+    const float2 dcx = hr ? csub(chi[i + 1], q0) : make_float2(0.f, 0.f);
+    const float2 dcy = hd ? csub(chi[i + f.w], q0) : make_float2(0.f, 0.f);
+    const float mx = sqrtf(cnorm(dcx)), my = sqrtf(cnorm(dcy));
+    float4 w;
+    w.x = hr ? s.eps / sqrtf(gx * gx + e2) : 0.f;
+    w.y = hd ? s.eps / sqrtf(gyv * gyv + e2) : 0.f;
+    w.z = hr ? s.eps_c / sqrtf(mx * mx + s.k * gx * gx + ec2) : 0.f;
+    w.w = hd ? s.eps_c / sqrtf(my * my + s.k * gyv * gyv + ec2) : 0.f;
+    float2 wd = make_float2(0.f, 0.f);
+    if (P.wd) {
+      if (hd && hr) {
+        const float a1 = sqrtf(cnorm(csub(chi[i + f.w + 1], q0))) / s.eps_c;
+        wd.x = 1.f / sqrtf(1.f + a1 * a1);
+      }
+      if (hd && hl) {
+        const float a2 = sqrtf(cnorm(csub(chi[i + f.w - 1], q0))) / s.eps_c;
+        wd.y = 1.f / sqrtf(1.f + a2 * a2);
+      }
+    }
+    if (P.nn > 0 && P.nu > 0.f) {
+      float ks = 1.f;
+      for (int j = 0; j < P.nn; ++j) {
+        float4 nb = P.nb[j][i];
+        const float rk = nb.x - y0 - (q0.x * nb.y - q0.y * nb.z);
+        nb.w = P.conf[j][i] * P.et2 / (rk * rk + P.et2);
+        P.nb[j][i] = nb;
+        ks += P.nu * nb.w;
+      }
+      w.x *= ks;
+      w.y *= ks;
+      w.z *= ks;
+      w.w *= ks;
+      wd.x *= ks;
+      wd.y *= ks;
+    }
+    P.wts[i] = w;
+    if (P.wd) P.wd[i] = wd;
+  }
+}
+
+__global__ void __launch_bounds__(64) k_pj_grad0(const PxField* __restrict__ F, const int* __restrict__ cp, int nf,
+                                                 int cur) {
+  const PxCtx c = px_ctx(cp, nf);
+  const PxField& P = F[c.fi];
+  const FieldCG s = *P.st;
+  const size_t n = (size_t)P.f.h * P.f.w;
+  double gg = 0.0;
+  for (int k = 0; k < kPxPerLane; ++k) {
+    const size_t i = (size_t)c.chunk * kPxChunk + (size_t)k * 64 + px_lane();
+    if (i >= n) break;
+    float gy;
+    float2 gc;
+    pj_grad(P, s, P.yv[cur], P.f.chi[cur], nullptr, nullptr, 0.f, (int)(i / P.f.w), (int)(i % P.f.w), &gy, &gc,
+            nullptr);
+    P.gy[i] = gy;
+    P.g[i] = gc;
+    gg += (double)gy * gy + ((double)gc.x * gc.x + (double)gc.y * gc.y);
+  }
+  gg = px_wave_sum(gg);
+  if (px_lane() == 0) P.part[c.chunk] = gg;
+}
+
+// First launch of joint iteration `it`: beta and the tolerance exits, the
+// direction (dY, dC) = -(gY, gC) + beta (dY, dC)_prev, and the partials of
+// its curvature (:687-755) and of <g, d>.
+__global__ void __launch_bounds__(64) k_pj_dir(const PxField* __restrict__ F, const int* __restrict__ cp, int nf,
+                                               int it, int stage) {
+#pragma clang fp contract(off)
+  const PxCtx c = px_ctx(cp, nf);
+  const PxField& P = F[c.fi];
+  const FieldCG s = *P.st;
+  const int nc = P.nchunks;
+  const bool done_in = P.px->done[0] != 0;
+  bool stop = false;
+  float beta = 0.f;
+  if (!done_in) {
+    const double gg = px_sum(P.part + (size_t)(it & 1) * nc, nc);
+    if (it == 0) {
+      if (c.chunk == 0 && px_lane() == 0) P.px->gg0 = gg;
+    } else {
+      const double prev = px_sum(P.part + (size_t)((it - 1) & 1) * nc, nc);
+      beta = (float)(gg / (prev > 1e-30 ? prev : 1e-30));
+      const double gg0 = P.px->gg0;
+      stop = gg < s.tol2 * gg0 || gg < s.gg_floor * s.npix;
+    }
+  }
+  if (c.chunk == 0 && px_lane() == 0) {
+    P.px->done[1] = (done_in || stop) ? 1 : 0;
+    if (!done_in && stop && stage < 16) P.px->iters[stage] = it;
+  }
+  if (done_in || stop) return;
+  const DevField& f = P.f;
+  const size_t n = (size_t)f.h * f.w;
+  const float2* dcp = f.d[(it + 1) & 1];
+  const float* dyp = P.dyv[(it + 1) & 1];
+  float2* dcn = f.d[it & 1];
+  float* dyn = P.dyv[it & 1];
+  auto dir_c = [&](size_t j) {
+    const float2 g = P.g[j];
+    if (it == 0) return make_float2(-g.x, -g.y);
+    const float2 q = dcp[j];
+    return make_float2(-g.x + beta * q.x, -g.y + beta * q.y);
+  };
+  auto dir_y = [&](size_t j) {
+    const float g = P.gy[j];
+    if (it == 0) return -g;
+    return -g + beta * dyp[j];
+  };
+  double hh = 0.0, hsp = 0.0, hd2 = 0.0, gd = 0.0;
+  for (int k = 0; k < kPxPerLane; ++k) {
+    const size_t i = (size_t)c.chunk * kPxChunk + (size_t)k * 64 + px_lane();
+    if (i >= n) break;
+    const int y = (int)(i / f.w), x = (int)(i % f.w);
+    const bool hr = x + 1 < f.w, hdn = y + 1 < f.h;
+    const float dY = dir_y(i);
+    const float2 dC = dir_c(i);
+    dyn[i] = dY;
+    dcn[i] = dC;
+    const float2 c0 = px_carrier(f, y, x);
+    const float v = dY + (dC.x * c0.x - dC.y * c0.y);
+    hh += (double)v * v;
+    for (int j = 0; j < P.nn; ++j) {
+      const float4 nb = P.nb[j][i];
+      const float dk = dY + (dC.x * nb.y - dC.y * nb.z);
+      hh += (double)P.nu * ((double)nb.w * dk * dk);
+    }
+    if (P.a_weight && P.nu_a > 0.f)
+      hh += (double)P.nu_a * ((double)P.a_weight[i] * ((double)dY * dY + (double)cnorm(dC)));
+    float t1 = 0.f, t2 = 0.f;
+    float2 cx = make_float2(0.f, 0.f), cy = make_float2(0.f, 0.f);
+    if (hr) {
+      t1 = dir_y(i + 1) - dY;
+      cx = csub(dir_c(i + 1), dC);
+    }
+    if (hdn) {
+      t2 = dir_y(i + f.w) - dY;
+      cy = csub(dir_c(i + f.w), dC);
+    }
+    const float4 w = P.wts[i];
+    hsp += (double)w.x * t1 * t1;
+    hsp += (double)w.y * t2 * t2;
+    hsp += (double)s.mu_h * w.z * cnorm(cx);
+    hsp += (double)s.mu_v * w.w * cnorm(cy);
+    if (P.wd && hdn) {
+      const float2 wd = P.wd[i];
+      if (hr) hd2 += (double)wd.x * cnorm(csub(dir_c(i + f.w + 1), dC));
+      if (x > 0) hd2 += (double)wd.y * cnorm(csub(dir_c(i + f.w - 1), dC));
+    }
+    const float gY = P.gy[i];
+    const float2 gC = P.g[i];
+    gd += (double)gY * dY + ((double)gC.x * dC.x + (double)gC.y * dC.y);
+  }
+  hh = px_wave_sum(hh + hsp + (double)s.mu_d * hd2);
+  gd = px_wave_sum(gd);
+  if (px_lane() == 0) {
+    P.part[(size_t)2 * nc + c.chunk] = hh;
+    P.part[(size_t)3 * nc + c.chunk] = gd;
+  }
+}
+
+// Second launch of joint iteration `it`: alpha or the curvature exit, the
+// step (Y, chi) += alpha (dY, dC) into the other buffers, the gradient there.
+__global__ void __launch_bounds__(64) k_pj_step(const PxField* __restrict__ F, const int* __restrict__ cp, int nf,
+                                                int it, int cur, int stage) {
+#pragma clang fp contract(off)
+  const PxCtx c = px_ctx(cp, nf);
+  const PxField& P = F[c.fi];
+  const FieldCG s = *P.st;
+  const int nc = P.nchunks;
+  const DevField& f = P.f;
+  const size_t n = (size_t)f.h * f.w;
+  const int src = cur ^ (it & 1), dst = cur ^ ((it + 1) & 1);
+  bool stopped = P.px->done[1] != 0;
+  double hh = 0.0, gd = 0.0;
+  if (!stopped) {
+    hh = px_sum(P.part + (size_t)2 * nc, nc);
+    gd = px_sum(P.part + (size_t)3 * nc, nc);
+    if (hh <= s.hh_floor) {
+      stopped = true;
+      if (c.chunk == 0 && px_lane() == 0 && stage < 16) P.px->iters[stage] = it;
+    }
+  }
+  if (c.chunk == 0 && px_lane() == 0) P.px->done[0] = stopped ? 1 : 0;
+  if (stopped) {
+    for (int k = 0; k < kPxPerLane; ++k) {
+      const size_t i = (size_t)c.chunk * kPxChunk + (size_t)k * 64 + px_lane();
+      if (i >= n) break;
+      f.chi[dst][i] = f.chi[src][i];
+      P.yv[dst][i] = P.yv[src][i];
+    }
+    return;
+  }
+  const float alpha = (float)(-0.5 * gd / hh);
+  double gg = 0.0;
+  for (int k = 0; k < kPxPerLane; ++k) {
+    const size_t i = (size_t)c.chunk * kPxChunk + (size_t)k * 64 + px_lane();
+    if (i >= n) break;
+    float gy;
+    float2 gc;
+    PjSample q;
+    pj_grad(P, s, P.yv[src], f.chi[src], P.dyv[it & 1], f.d[it & 1], alpha, (int)(i / f.w), (int)(i % f.w), &gy,
+            &gc, &q);
+    P.yv[dst][i] = q.y;
+    f.chi[dst][i] = q.chi;
+    P.gy[i] = gy;
+    P.g[i] = gc;
+    gg += (double)gy * gy + ((double)gc.x * gc.x + (double)gc.y * gc.y);
+  }
+  gg = px_wave_sum(gg);
+  if (px_lane() == 0) P.part[(size_t)((it + 1) & 1) * nc + c.chunk] = gg;
+}
+
+// (Y, chi) into the caller's buffers when they ended in the workspace copy.
+__global__ void __launch_bounds__(64) k_pj_finalize(const PxField* __restrict__ F, const int* __restrict__ cp, int nf,
+                                                    int cur) {
+  const PxCtx c = px_ctx(cp, nf);
+  const PxField& P = F[c.fi];
+  if (!cur) return;
+  const size_t n = (size_t)P.f.h * P.f.w;
+  for (int k = 0; k < kPxPerLane; ++k) {
+    const size_t i = (size_t)c.chunk * kPxChunk + (size_t)k * 64 + px_lane();
+    if (i >= n) break;
+    P.f.chi[0][i] = P.f.chi[1][i];
+    P.yv[0][i] = P.yv[1][i];
   }
 }
 

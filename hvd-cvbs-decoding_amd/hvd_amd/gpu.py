@@ -28,7 +28,8 @@ ABI_SYMBOLS = (
     "hvd_gpu_set_solver_tuning", "hvd_gpu_autotune_solver", "hvd_gpu_set_graphs",
     "hvd_gpu_variational_refine_temporal", "hvd_gpu_vectors_per_pixel", "hvd_gpu_warp_by_tiles",
     "hvd_gpu_warp_bilinear", "hvd_gpu_upsample_confidence", "hvd_gpu_complex_coherence",
-    "hvd_gpu_motion_compensate_prev", "hvd_gpu_variational_refine_noalloc",
+    "hvd_gpu_motion_compensate_prev", "hvd_gpu_variational_refine_noalloc", "hvd_gpu_woven_size",
+    "hvd_gpu_woven_picture", "hvd_gpu_variational_refine_temporal_batch", "hvd_gpu_variational_refine_joint",
 )
 
 
@@ -87,6 +88,11 @@ class Neighbor(C.Structure):
     _fields_ = [("composite", C.c_void_p), ("carrier", C.c_void_p), ("confidence", C.c_void_p)]
 
 
+class Anchor(C.Structure):
+    """hvd::AnchorTerm (variational.h:60-66) as device pointers."""
+    _fields_ = [("luma", C.c_void_p), ("chroma", C.c_void_p), ("weight", C.c_void_p)]
+
+
 class FrameParams(C.Structure):
     """hvd::FrameParams (frame_bridge.h:40-101)."""
     _fields_ = [(n, C.c_int) for n in (
@@ -140,6 +146,16 @@ def load() -> C.CDLL:
                                                    C.POINTER(C.c_int), C.POINTER(C.c_float), vp]
         lib.hvd_gpu_variational_refine_noalloc.argtypes = [vp, C.POINTER(Plane), i, C.POINTER(SolverParams),
                                                            C.POINTER(C.c_int), C.POINTER(C.c_float), vp]
+        lib.hvd_gpu_woven_size.argtypes = [C.POINTER(FrameParams), i, i, C.POINTER(i), C.POINTER(i)]
+        lib.hvd_gpu_woven_picture.argtypes = [vp, C.POINTER(YcFrame), C.POINTER(FrameParams), i, i, vp, vp, vp, vp,
+                                              vp]
+        lib.hvd_gpu_variational_refine_temporal_batch.argtypes = [vp, C.POINTER(Plane), i, C.POINTER(Neighbor),
+                                                                  C.POINTER(C.c_int), C.POINTER(SolverParams),
+                                                                  C.c_float, C.c_float, vp, vp]
+        lib.hvd_gpu_variational_refine_joint.argtypes = [vp, C.POINTER(Plane), i, C.POINTER(Neighbor),
+                                                         C.POINTER(C.c_int), C.POINTER(Anchor),
+                                                         C.POINTER(SolverParams), C.c_float, C.c_float, C.c_float,
+                                                         vp, vp]
         lib.hvd_gpu_decode_fields_u16.argtypes = [vp, pp, i, C.POINTER(Geometry), C.c_float, C.c_float,
                                                   C.POINTER(SolverParams), C.c_float, pp, pp, pp, vp]
         lib.hvd_gpu_decode_fields_ire.argtypes = [vp, pp, i, C.POINTER(Geometry), C.POINTER(SolverParams),
@@ -332,6 +348,47 @@ class Engine:
         del keep
         if want_stats:
             return np.array(iters[:], dtype=np.int32)
+        return None
+
+    def _batch_args(self, planes, neighbours):
+        n = len(planes)
+        P = (Plane * n)(*planes)
+        flat = [nb for per in neighbours for nb in per]
+        N = (Neighbor * max(len(flat), 1))(*[Neighbor(a.data_ptr(), b.data_ptr(), c.data_ptr()) for a, b, c in flat])
+        nn = (C.c_int * n)(*[len(per) for per in neighbours])
+        return P, N, nn, flat
+
+    def variational_refine_temporal_batch(self, planes, neighbours, params: SolverParams, temporal_strength,
+                                          temporal_eps, want_stats=False):
+        """The batched solve with neighbours: neighbours[i] lists plane i's
+        (composite, carrier, confidence) device tensors."""
+        P, N, nn, keep = self._batch_args(planes, neighbours)
+        outer = max(params.irls_outer, 1)
+        iters = (C.c_int * (len(planes) * outer))() if want_stats else None
+        self._check(self.lib.hvd_gpu_variational_refine_temporal_batch(
+            self.ctx, P, len(planes), N, nn, C.byref(params), float(temporal_strength), float(temporal_eps), iters,
+            _stream()), "variational_refine_temporal_batch")
+        del keep
+        if want_stats:
+            return np.array(iters[:], dtype=np.int32).reshape(len(planes), outer)
+        return None
+
+    def variational_refine_joint(self, planes, neighbours, anchors, params: SolverParams, temporal_strength,
+                                 temporal_eps, nr_anchor, want_stats=False):
+        """VariationalRefineJoint of a batch: planes[i].y holds Y0 and
+        receives Y; anchors[i] is (luma, chroma, weight) device tensors or
+        None."""
+        P, N, nn, keep = self._batch_args(planes, neighbours)
+        A = (Anchor * len(planes))(*[Anchor(None, None, None) if a is None else
+                                     Anchor(a[0].data_ptr(), a[1].data_ptr(), a[2].data_ptr()) for a in anchors])
+        outer = max(params.irls_outer, 1)
+        iters = (C.c_int * (len(planes) * outer))() if want_stats else None
+        self._check(self.lib.hvd_gpu_variational_refine_joint(
+            self.ctx, P, len(planes), N, nn, A, C.byref(params), float(temporal_strength), float(temporal_eps),
+            float(nr_anchor), iters, _stream()), "variational_refine_joint")
+        del keep
+        if want_stats:
+            return np.array(iters[:], dtype=np.int32).reshape(len(planes), outer)
         return None
 
     # ---- temporal neighbours' warps (temporal.cpp) ----------------------------
@@ -549,6 +606,25 @@ class Engine:
         return o
 
     # ---- motion ---------------------------------------------------------------
+    def woven_picture(self, luma, u, v, fp: FrameParams, full_raster=False, field=-1):
+        """ReorderToWoven + RGB24 of a field-sequential YcFrameS16 (device
+        tensors luma int16, u / v float64, frame_height x frame_width) ->
+        (y, u, v float64, rgb uint8 [h, w, 3]) device tensors."""
+        import torch
+        w, h = C.c_int(), C.c_int()
+        self._check(self.lib.hvd_gpu_woven_size(C.byref(fp), int(full_raster), int(field), C.byref(w), C.byref(h)),
+                    "woven_size")
+        dev = luma.device
+        yo = torch.zeros((h.value, w.value), dtype=torch.float64, device=dev)
+        uo, vo = torch.zeros_like(yo), torch.zeros_like(yo)
+        rgb = torch.zeros((h.value, w.value, 3), dtype=torch.uint8, device=dev)
+        yc = YcFrame(luma.data_ptr(), None, u.data_ptr(), v.data_ptr())
+        if h.value and w.value:
+            self._check(self.lib.hvd_gpu_woven_picture(self.ctx, C.byref(yc), C.byref(fp), int(full_raster),
+                                                       int(field), yo.data_ptr(), uo.data_ptr(), vo.data_ptr(),
+                                                       rgb.data_ptr(), _stream()), "woven_picture")
+        return yo, uo, vo, rgb
+
     def estimate_motion(self, refs, curs, tile=32, search=8):
         """hvd-core EstimateMotion for pairs of (h, w) float32 planes: returns
         (dy list, dx list, confidence list), each ceil(h/tile) x ceil(w/tile)."""

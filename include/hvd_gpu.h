@@ -282,6 +282,38 @@ int hvd_gpu_variational_refine_temporal(hvd_gpu_ctx* ctx, const hvd_gpu_plane* p
                                         int nn, const hvd_solver_params* p, float temporal_strength,
                                         float temporal_eps, int* stage_iters, void* stream);
 
+/* VariationalRefine with temporal neighbours (variational.cpp:198-556) for a
+ * batch: plane i with nn[i] (<= 16) neighbours taken in order from nbrs
+ * (back to back; nn NULL: none).  Fields advance together in one launch per
+ * CG step; otherwise as hvd_gpu_variational_refine_temporal of each plane
+ * (a plane without neighbours gets the 2-D solve).  stage_iters: optional
+ * HOST output, n * irls_outer ints (makes the call synchronous). */
+int hvd_gpu_variational_refine_temporal_batch(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n,
+                                              const hvd_gpu_neighbor* nbrs, const int* nn,
+                                              const hvd_solver_params* p, float temporal_strength,
+                                              float temporal_eps, int* stage_iters, void* stream);
+
+/* A synthesised reference the joint solve is anchored to (hvd::AnchorTerm,
+ * variational.h:60-66): luma, interleaved chroma and per-pixel weight, each
+ * lines x width of its plane.  weight NULL: no anchor. */
+typedef struct {
+  const float* luma;
+  const float* chroma;
+  const float* weight;
+} hvd_gpu_anchor;
+
+/* VariationalRefineJoint (variational.cpp:558-891) for a batch: Y and chi
+ * both free.  planes[i].y holds Y0 on entry and Y on return, planes[i].chi
+ * chi0 and chi.  Neighbours as in hvd_gpu_variational_refine_temporal_batch
+ * (strength temporal_strength, robust scale temporal_eps); anchors (NULL, or
+ * n entries) weigh nr_anchor (|Y - Y_a|^2 + |chi - chi_a|^2).  stage_iters:
+ * optional HOST output, n * irls_outer ints. */
+int hvd_gpu_variational_refine_joint(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n,
+                                     const hvd_gpu_neighbor* nbrs, const int* nn, const hvd_gpu_anchor* anchors,
+                                     const hvd_solver_params* p, float temporal_strength, float temporal_eps,
+                                     float nr_anchor, int* stage_iters, void* stream);
+
+
 /* ---- the temporal neighbours' warps (temporal.h / temporal.cpp) -------- */
 /* Motion fields are hvd-core's per-tile MotionField (dy, dx, confidence of
  * tiles_h x tiles_w tiles of `tile` pixels, as hvd_gpu_estimate_motion
@@ -426,6 +458,24 @@ int hvd_gpu_block_match_sad_u16(hvd_gpu_ctx* ctx, const uint16_t* const* ref, co
  * for a separate chroma channel (DecodeYcFrameBuffer, :276-297). */
 int hvd_gpu_split_frame_fields(hvd_gpu_ctx* ctx, const int16_t* frame, const hvd_frame_params* fp, float zero_code,
                                float* field1, float* field2, void* stream);
+
+/* The woven active picture's size (ReorderToWoven,
+ * orc-plugin_hvd/src/hvd_chroma_decoder_stage.cpp:316-413): the active
+ * window (or, full_raster != 0, the whole frame) in raster order, or with
+ * field 0 / 1 that field's lines only (any other value: both fields
+ * woven).  Host only; width = height = 0 when the picture is empty. */
+int hvd_gpu_woven_size(const hvd_frame_params* fp, int full_raster, int field, int* width, int* height);
+
+/* ReorderToWoven of a field-sequential hvd::YcFrameS16 (its planes
+ * frame_width x frame_height, DEVICE pointers; chroma unused) into width x
+ * height doubles y, u, v (hvd_chroma_decoder_stage.cpp:316-413), and its
+ * RGB24 rendering rgb24 (width x height x 3 bytes, R G B): YuvToRgb8 (:79-88)
+ * on Y and U/V normalised over the luma excursion black..white
+ * (WovenToPreviewImage / WriteWovenAsRgb24, :500-638), clamped to [0, 1],
+ * x 255 rounded.  Any output may be NULL.  The RGB rendering needs white
+ * above black and blanking (the stage returns no image otherwise). */
+int hvd_gpu_woven_picture(hvd_gpu_ctx* ctx, const hvd_yc_frame* yc, const hvd_frame_params* fp, int full_raster,
+                          int field, double* y, double* u, double* v, uint8_t* rgb24, void* stream);
 
 /* DetectFieldParity (frame_bridge.cpp:102-149) on two field1_lines x
  * frame_width IRE blocks: *vote (HOST) = +1 field 1 top, -1 field 1 bottom,
