@@ -21,6 +21,7 @@
 
 #include "hvd_gpu.h"
 #include "hvd_kernels.h"
+#include "window.h"
 
 using namespace hvdk;
 
@@ -85,6 +86,10 @@ struct hvd_gpu_ctx {
   // nothing may allocate device or pinned memory
   char* scratch = nullptr;
   bool no_alloc = false;
+  // device memory of the 3-D window driver (window.hip), grow-only and
+  // apart from ws, which the component calls it makes may reallocate
+  char* win = nullptr;
+  size_t win_cap = 0;
   Slot na_slot;  // descriptor region of a no-allocation call (no event)
   Slot slots[kSlots];
   int next_slot = 0;
@@ -1159,6 +1164,7 @@ void hvd_gpu_destroy(hvd_gpu_ctx* ctx) {
   }
   drop_graphs(ctx);
   if (ctx->ws) (void)hipFree(ctx->ws);
+  if (ctx->win) (void)hipFree(ctx->win);
   clear_profile(ctx);
   delete ctx;
 }
@@ -2592,6 +2598,88 @@ int hvd_gpu_estimate_motion(hvd_gpu_ctx* ctx, const float* const* y_ref, const f
   });
 }
 
+int hvd_gpu_decode_window(hvd_gpu_ctx* ctx, const hvd_gpu_plane* fields, int n, const int* parities, int is_pal,
+                          int y_given, const hvd_solver_params* p, const hvd_window_params* wp,
+                          hvd_window_diag* diag, void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, !ctx->no_alloc, "internal: window inside a no-allocation call");
+    if (p) check_params(ctx, p);
+    const int rc = hvd_window::drive(ctx, fields, n, parities, is_pal, y_given, p, wp, diag, S(stream));
+    if (rc != HVD_GPU_OK) throw Fail{rc};
+  });
+}
+
+int hvd_gpu_verify_motion(hvd_gpu_ctx* ctx, const float* const* y_ref, const float* const* y_cur, int n, int h,
+                          int w, const float* const* pdy, const float* const* pdx, int tile, float* const* dy,
+                          float* const* dx, float* const* conf, void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, n >= 0 && (n == 0 || (y_ref && y_cur && pdy && pdx && dy && dx && conf)), "null argument");
+    REQUIRE(ctx, h >= 1 && w >= 1, "empty plane");
+    REQUIRE(ctx, tile >= 1 && tile <= 256, "tile %d outside 1..256", tile);
+    if (n == 0) return;
+    hipStream_t st = S(stream);
+    const int th = (h + tile - 1) / tile, tw = (w + tile - 1) / tile, nt = th * tw;
+    const size_t np = (size_t)h * w;
+    Carve cv;
+    const size_t o_tmp = cv.take(sizeof(float) * n * 2 * np), o_P = cv.take(sizeof(float) * n * 2 * np);
+    const size_t o_se = cv.take(sizeof(float) * n * nt);
+    const size_t o_v = cv.take(sizeof(float) * n * nt * 11);
+    char* ws = ensure_ws(ctx, cv.off, st);
+    float* tmp = (float*)(ws + o_tmp);
+    float* P = (float*)(ws + o_P);
+    float* se = (float*)(ws + o_se);
+    float* v = (float*)(ws + o_v);
+    auto vec = [&](int k) { return v + (size_t)k * n * nt; };
+    float *py = vec(0), *px = vec(1), *bp = vec(2), *bz = vec(3), *jy = vec(4), *jx = vec(5), *best = vec(6);
+    float *ody = vec(7), *odx = vec(8);
+    std::vector<MotionDesc> D(n);
+    for (int i = 0; i < n; ++i) {
+      REQUIRE(ctx, y_ref[i] && y_cur[i] && pdy[i] && pdx[i] && dy[i] && dx[i] && conf[i], "pair %d: null pointer", i);
+      D[i] = MotionDesc{y_ref[i], y_cur[i]};
+      HIPCHK(ctx, hipMemcpyAsync(py + (size_t)i * nt, pdy[i], sizeof(float) * nt, hipMemcpyDeviceToDevice, st));
+      HIPCHK(ctx, hipMemcpyAsync(px + (size_t)i * nt, pdx[i], sizeof(float) * nt, hipMemcpyDeviceToDevice, st));
+    }
+    std::vector<float*> cptr(conf, conf + n);
+    const std::vector<int> r0 = {0};
+    Stage stg;
+    const size_t oD = stg.put(D), oR0 = stg.put(r0), oC = stg.put(cptr);
+    Slot* sl;
+    char* base = upload(ctx, stg, st, &sl);
+    const int* dr0 = (const int*)(base + oR0);
+    // motion.cpp:490-525: both planes blurred r = 2; the rounded prediction
+    // (k_me_ssd rounds a relative base) and the zero vector, one SSD each
+    hipLaunchKernelGGL(k_me_blur_cols, dim3(blocks_for((size_t)n * 2 * w)), dim3(256), 0, st,
+                       (const MotionDesc*)(base + oD), n, h, w, 2, tmp);
+    LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_me_blur_rows, dim3(blocks_for((size_t)n * 2 * h)), dim3(256), 0, st, n, h, w, 2,
+                       (const float*)tmp, P);
+    LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_me_ssd, dim3(blocks_for((size_t)n * nt)), dim3(256), 0, st, n, h, w, tile, (const float*)P,
+                       dr0, dr0, 1, 1, (const float*)py, (const float*)px, se);
+    LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_me_select, dim3(blocks_for((size_t)n * nt)), dim3(256), 0, st, n, nt, dr0, dr0, 1, 1,
+                       (const float*)se, (const float*)py, (const float*)px, jy, jx, bp, (float*)nullptr);
+    LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_me_ssd, dim3(blocks_for((size_t)n * nt)), dim3(256), 0, st, n, h, w, tile, (const float*)P,
+                       dr0, dr0, 1, 1, (const float*)nullptr, (const float*)nullptr, se);
+    LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_me_select, dim3(blocks_for((size_t)n * nt)), dim3(256), 0, st, n, nt, dr0, dr0, 1, 1,
+                       (const float*)se, (const float*)nullptr, (const float*)nullptr, jy, jx, bz, (float*)nullptr);
+    LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_verify_pick, dim3(blocks_for((size_t)n * nt)), dim3(256), 0, st, n * nt, (const float*)py,
+                       (const float*)px, (const float*)bp, (const float*)bz, ody, odx, best);
+    LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_me_confidence, dim3(n), dim3(256), 0, st, h, w, tile, (const float*)P, (const float*)best,
+                       (float* const*)(base + oC), vec(9) /* scratch: 2 n nt floats from vec(9) */);
+    LAUNCH_CHECK(ctx);
+    for (int i = 0; i < n; ++i) {
+      HIPCHK(ctx, hipMemcpyAsync(dy[i], ody + (size_t)i * nt, sizeof(float) * nt, hipMemcpyDeviceToDevice, st));
+      HIPCHK(ctx, hipMemcpyAsync(dx[i], odx + (size_t)i * nt, sizeof(float) * nt, hipMemcpyDeviceToDevice, st));
+    }
+    arm(ctx, sl, st);
+  });
+}
+
 // ---- frame-bridge and component entry points of the C++ host API ----------
 
 int hvd_gpu_split_frame_fields(hvd_gpu_ctx* ctx, const int16_t* frame, const hvd_frame_params* fp, float zero_code,
@@ -3005,3 +3093,33 @@ int hvd_gpu_autotune_solver(hvd_gpu_ctx* ctx, int lines, int width, int is_pal, 
 }
 
 }  // extern "C"
+
+// ---- services of the window driver (window.h) --------------------------------
+namespace hvd_window {
+
+char* pool(hvd_gpu_ctx* ctx, size_t bytes, hipStream_t st) {
+  if (bytes <= ctx->win_cap) return ctx->win;
+  if (hipStreamSynchronize(st) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+    set_err(ctx, "synchronisation before the window allocation failed");
+    return nullptr;
+  }
+  if (ctx->win) (void)hipFree(ctx->win);
+  ctx->win = nullptr;
+  ctx->win_cap = 0;
+  const size_t want = bytes + bytes / 4;
+  void* q = nullptr;
+  if (hipMalloc(&q, want) != hipSuccess) {
+    set_err(ctx, "hipMalloc(%zu) for the window failed", want);
+    return nullptr;
+  }
+  ctx->win = static_cast<char*>(q);
+  ctx->win_cap = want;
+  return ctx->win;
+}
+
+int fail(hvd_gpu_ctx* ctx, int code, const char* msg) {
+  set_err(ctx, "%s", msg);
+  return code;
+}
+
+}  // namespace hvd_window

@@ -1,0 +1,55 @@
+// window.h -- the 3-D window driver's kernels (window.hip) and its host entry
+// point, called by hvd_gpu_decode_window in abi.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hvd_gpu.h"
+
+namespace hvdk {
+
+__global__ void k_verify_pick(int n, const float* pdy, const float* pdx, const float* bp, const float* bz, float* dy,
+                              float* dx, float* best);
+__global__ void k_noise_sigma(const float* const* s, int h, int w, float* d_all, float* sigma);
+__global__ void k_amb_bands(const float* const* s, const float* const* car, int stride, int h, int w, float nr_eps,
+                            int nbands, double* acc, int* cnt);
+__global__ void k_fidelity(size_t n, const float* s, const float2* chi, const float2* car, float* y);
+__global__ void k_traj_predict(int m, int nt, const float* const* dy, const float* const* dx, const float* ho,
+                               const int* off, float* vy, float* vx);
+__global__ void k_traj_extrapolate(int nt, const float* vy, const float* vx, float o, float ho, float* pdy,
+                                   float* pdx);
+__global__ void k_traj_snap(int m, int nt, float* const* dy, float* const* dx, const float* const* conf,
+                            const float* ho, const int* off);
+__global__ void k_env_terms(int h, int w, const float* Y, const float2* chi, float* gmag, float* lapmag);
+__global__ void k_smooth_rows(int h, int w, const float* src, float* out);
+__global__ void k_env_finish(int n, const float* const* vg_in, float* const* vgrad, float* const* lap_io,
+                             float* const* sample);
+__global__ void k_split_cplx(size_t n, const float2* z, float* re, float* im);
+__global__ void k_join_cplx(size_t n, const float* re, const float* im, float2* z);
+__global__ void k_conf_gates(size_t n, float* conf, const float* gamma, float a, const float* vgrad,
+                             const float* lapsm, float e2, float floor_max);
+__global__ void k_synth_init(size_t n, const float* Y, const float2* chi, float* aY, float* aCr, float* aCi,
+                             float* aW);
+__global__ void k_synth_accum(size_t n, const float* Y, const float2* chi, const float* conf, const float* Yw,
+                              const float* wre, const float* wim, float eps_b, float* aY, float* aCr, float* aCi,
+                              float* aW);
+__global__ void k_synth_finish(size_t n, const float* s, const float2* car, const float* aY, const float* aCr,
+                               const float* aCi, const float* aW, float* luma, float2* chroma, float* resid);
+__global__ void k_synth_weight(size_t n, const float* r, float eps_b, float* weight);
+__global__ void k_psi(size_t n, int nn, const float* s, const float2* car, const float* const* sw,
+                      const float2* const* cw, const float* const* conf, float2* chi);
+
+}  // namespace hvdk
+
+// Services of abi.hip the driver uses.
+namespace hvd_window {
+// Grow-only device memory of the driver, separate from the component calls'
+// workspace; null (with the context's error set) when it cannot be had.
+char* pool(hvd_gpu_ctx* ctx, size_t bytes, hipStream_t st);
+// Records a failure message on the context; returns code.
+int fail(hvd_gpu_ctx* ctx, int code, const char* msg);
+// The window driver (window.hip): 0 or an HVD_GPU_ERR_* code.
+int drive(hvd_gpu_ctx* ctx, const hvd_gpu_plane* fields, int n, const int* parities, int is_pal, int y_given,
+          const hvd_solver_params* p, const hvd_window_params* wp, hvd_window_diag* diag, hipStream_t st);
+}  // namespace hvd_window

@@ -1,0 +1,1122 @@
+// window.hip -- the 3-D window driver on the GPU: hvd-core's
+// DecodeFieldWindowWithInits (engine/sequence.cpp:371-1114) over a window of
+// equally sized fields, with its noise self-calibration, adaptive temporal
+// strength, per-pass pair motion (full search, fast mode's trajectory
+// prediction audited by VerifyMotion, the consensus snap), the
+// motion-compensated neighbour equations with the coherence and odd-offset
+// envelope gates, the PSI init, SynthReference and the anchored joint pass,
+// and the output-fidelity luma.
+//
+// Every per-pixel, per-tile and per-band computation runs on the device: the
+// kernels below plus the library's component entry points (EstimateMotion,
+// VerifyMotion, MotionCompensatePrev, the warps, ComplexCoherence, BoxBlur and
+// the batched solves), all on the caller's stream.  The host keeps the
+// scalar decisions hvd-core makes on a handful of values (the median of the
+// per-field noise, the p95 of the band energies, the strength map) and the
+// order of work: fields of one colour of the Gauss-Seidel sweep (sequence.cpp:
+// 1046-1098; fields spaced reach + 1 apart read none of each other's state)
+// are solved as one batch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "hvd_gpu.h"
+#include "hvd_kernels.h"
+#include "window.h"
+
+namespace hvdk {
+
+namespace {
+
+constexpr int kSelBlock = 1024;
+
+// Order-preserving map of a float onto an unsigned key (negatives reversed).
+__device__ inline unsigned fkey(float f) {
+  const unsigned u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ inline float funkey(unsigned k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+// The k-th smallest of v[0 .. n) (std::nth_element's element k), any sign.
+// One block of kSelBlock threads; hist: 16 x 256 unsigned in LDS.
+__device__ float select_kth(const float* v, int n, int k, unsigned* hist) {
+  __shared__ unsigned tot[256];
+  __shared__ unsigned s_prefix, s_k;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (threadIdx.x == 0) {
+    s_prefix = 0u;
+    s_k = (unsigned)k;
+  }
+  __syncthreads();
+  for (int pass = 3; pass >= 0; --pass) {
+    for (int i = threadIdx.x; i < 16 * 256; i += kSelBlock) hist[i] = 0u;
+    __syncthreads();
+    const unsigned prefix = s_prefix, kk = s_k;
+    const int hi_shift = (pass + 1) * 8;
+    unsigned* h = hist + wv * 256;
+    for (int i = threadIdx.x; i < n; i += kSelBlock) {
+      const unsigned u = fkey(v[i]);
+      if (hi_shift >= 32 || (u >> hi_shift) == prefix) atomicAdd(&h[(u >> (pass * 8)) & 255u], 1u);
+    }
+    __syncthreads();
+    unsigned cnt = 0u, incl = 0u;
+    if (threadIdx.x < 256) {
+      for (int w = 0; w < 16; ++w) cnt += hist[w * 256 + threadIdx.x];
+      incl = cnt;
+      for (int o = 1; o < 64; o <<= 1) {
+        const unsigned y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+      }
+      tot[threadIdx.x] = incl;
+    }
+    __syncthreads();
+    if (threadIdx.x < 256) {
+      for (int w = 0; w < wv; ++w) incl += tot[w * 64 + 63];
+      const unsigned excl = incl - cnt;
+      if (excl <= kk && kk < incl) {
+        s_k = kk - excl;
+        s_prefix = (prefix << 8) | (unsigned)threadIdx.x;
+      }
+    }
+    __syncthreads();
+  }
+  return funkey(s_prefix);
+}
+
+// numpy median (MedianOf, sequence.cpp:28-38) of v[0 .. n) in one block.
+__device__ float median_np(const float* v, int n, unsigned* hist) {
+#pragma clang fp contract(off)
+  if (n <= 0) return 0.f;
+  const float m = select_kth(v, n, n / 2, hist);
+  if (n % 2) return m;
+  __syncthreads();
+  const float lo = select_kth(v, n, n / 2 - 1, hist);
+  return 0.5f * (m + lo);
+}
+
+__device__ inline float2 c_sub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ inline float c_abs(float2 a) { return hypotf(a.x, a.y); }
+__device__ inline float re_of(float2 a, float2 b) {
+#pragma clang fp contract(off)
+  return a.x * b.x - a.y * b.y;
+}
+
+}  // namespace
+
+// VerifyMotion's pick (motion.cpp:508-522): the zero vector on ties, else the
+// full float prediction.
+__global__ void k_verify_pick(int n, const float* pdy, const float* pdx, const float* bp, const float* bz, float* dy,
+                              float* dx, float* best) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (bz[i] <= bp[i]) {
+    dy[i] = 0.f;
+    dx[i] = 0.f;
+    best[i] = bz[i];
+  } else {
+    dy[i] = pdy[i];
+    dx[i] = pdx[i];
+    best[i] = bp[i];
+  }
+}
+
+// EstimateNoiseIre (ntsc_geometry.cpp:49-71), one block per field: the
+// second differences at the 4-sample period into d, their median (element
+// n/2), |d - median| and its element n/4, scaled to sigma.
+__global__ void __launch_bounds__(kSelBlock) k_noise_sigma(const float* const* s, int h, int w, float* d_all,
+                                                           float* sigma) {
+#pragma clang fp contract(off)
+  __shared__ unsigned hist[16 * 256];
+  const int fi = blockIdx.x;
+  const int m = w - 8, n = h * m;
+  if (m <= 0 || h <= 0) {
+    if (threadIdx.x == 0) sigma[fi] = 0.f;
+    return;
+  }
+  const float* S = s[fi];
+  float* d = d_all + (size_t)fi * n;
+  for (int i = threadIdx.x; i < n; i += kSelBlock) {
+    const int y = i / m, x = i % m;
+    const float* r = S + (size_t)y * w + x;
+    d[i] = r[0] - 2.0f * r[4] + r[8];
+  }
+  __syncthreads();
+  const float med = select_kth(d, n, n / 2, hist);
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += kSelBlock) d[i] = fabsf(d[i] - med);
+  __syncthreads();
+  const float q = select_kth(d, n, n / 4, hist);
+  if (threadIdx.x == 0) sigma[fi] = q / 0.3186f / sqrtf(6.0f);
+}
+
+// The adaptive-strength scan's band energies (sequence.cpp:560-630): block
+// (pair, band) over the decimated samples of that band's 8 sample rows; a
+// sample counts where the box-4 means of the pair agree within nr_eps.
+__global__ void __launch_bounds__(256) k_amb_bands(const float* const* s, const float* const* car, int stride, int h,
+                                                   int w, float nr_eps, int nbands, double* acc, int* cnt) {
+#pragma clang fp contract(off)
+  __shared__ double sd[256];
+  __shared__ int sn[256];
+  const int pr = blockIdx.x / nbands, band = blockIdx.x % nbands;
+  const float *Sj = s[pr], *Sk = s[pr + stride];
+  const float2 *cj = (const float2*)car[pr], *ck = (const float2*)car[pr + stride];
+  const int nx = (w - 7 - 4 + 3) / 4;  // x = 4, 8, ... while x + 7 < w
+  const int rows = 8;
+  double e_sum = 0.0;
+  int n_sum = 0;
+  const float kw[7] = {1.f, 2.f, 3.f, 4.f, 3.f, 2.f, 1.f};
+  for (int t = threadIdx.x; t < rows * nx; t += 256) {
+    const int yi = band * 8 + t / nx;
+    const int y = 2 + 3 * yi;
+    if (y >= h - 2) continue;
+    const int x = 4 + 4 * (t % nx);
+    float mj = 0.f, mk = 0.f;
+    for (int k4 = 0; k4 < 4; ++k4) {
+      mj += Sj[(size_t)y * w + x + k4];
+      mk += Sk[(size_t)y * w + x + k4];
+    }
+    if (fabsf(mj - mk) * 0.25f > nr_eps) continue;
+    float2 dj = make_float2(0.f, 0.f), dk = make_float2(0.f, 0.f);
+    for (int k = 0; k < 7; ++k) {
+      const size_t i = (size_t)y * w + x + k;
+      const float aj = kw[k] * Sj[i], ak = kw[k] * Sk[i];
+      const float2 c1 = cj[i], c2 = ck[i];
+      dj.x += aj * c1.x;
+      dj.y += aj * -c1.y;
+      dk.x += ak * c2.x;
+      dk.y += ak * -c2.y;
+    }
+    dj = make_float2(dj.x * (1.0f / 16.0f), dj.y * (1.0f / 16.0f));
+    dk = make_float2(dk.x * (1.0f / 16.0f), dk.y * (1.0f / 16.0f));
+    const float2 e = c_sub(dj, dk);
+    e_sum += 0.25 * (double)(e.x * e.x + e.y * e.y);
+    n_sum += 1;
+  }
+  sd[threadIdx.x] = e_sum;
+  sn[threadIdx.x] = n_sum;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      sd[threadIdx.x] += sd[threadIdx.x + o];
+      sn[threadIdx.x] += sn[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    acc[blockIdx.x] = sd[0];
+    cnt[blockIdx.x] = sn[0];
+  }
+}
+
+// Y = S - Re[chi c] (the init's luma when none is given; the output-fidelity
+// luma, sequence.cpp:1102-1110).
+__global__ void k_fidelity(size_t n, const float* s, const float2* chi, const float2* car, float* y) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    y[i] = s[i] - re_of(chi[i], car[i]);
+}
+
+// Field j's per-tile trajectory prediction (sequence.cpp:702-724): the numpy
+// median over the measured offsets of (dy - h_o) / o and dx / o.  dy, dx:
+// m planes of nt tiles; ho, off: m entries.
+__global__ void k_traj_predict(int m, int nt, const float* const* dy, const float* const* dx, const float* ho,
+                               const int* off, float* vy, float* vx) {
+#pragma clang fp contract(off)
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  float sy[kMaxNeighbors], sx[kMaxNeighbors];
+  for (int j = 0; j < m; ++j) {
+    const float o = (float)off[j];
+    sy[j] = (dy[j][t] - ho[j]) / o;
+    sx[j] = dx[j][t] / o;
+  }
+  // insertion sorts (m <= 16)
+  for (int a = 1; a < m; ++a)
+    for (int b = a; b > 0 && sy[b] < sy[b - 1]; --b) {
+      const float z = sy[b];
+      sy[b] = sy[b - 1];
+      sy[b - 1] = z;
+    }
+  for (int a = 1; a < m; ++a)
+    for (int b = a; b > 0 && sx[b] < sx[b - 1]; --b) {
+      const float z = sx[b];
+      sx[b] = sx[b - 1];
+      sx[b - 1] = z;
+    }
+  float my = sy[m / 2], mx = sx[m / 2];
+  if (m % 2 == 0) {
+    my = 0.5f * (my + sy[m / 2 - 1]);
+    mx = 0.5f * (mx + sx[m / 2 - 1]);
+  }
+  vy[t] = my;
+  vx[t] = mx;
+}
+
+// The predicted vector of offset o: o v + h_o.
+__global__ void k_traj_extrapolate(int nt, const float* vy, const float* vx, float o, float ho, float* pdy,
+                                   float* pdx) {
+#pragma clang fp contract(off)
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  pdy[t] = o * vy[t] + ho;
+  pdx[t] = o * vx[t];
+}
+
+// TrajectorySnap (motion.cpp:527-601) over one field's m measurements: the
+// fit (median of confident (dy - h)/o, dx/o), the consensus count of those
+// within 1.5 px, and the snap where >= 3 agree.  In place on dy, dx.
+__global__ void k_traj_snap(int m, int nt, float* const* dy, float* const* dx, const float* const* conf,
+                            const float* ho, const int* off) {
+#pragma clang fp contract(off)
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt || m < 2) return;
+  float sy[kMaxNeighbors], sx[kMaxNeighbors];
+  int k = 0;
+  for (int j = 0; j < m; ++j) {
+    if (off[j] == 0 || conf[j][t] <= 0.15f) continue;
+    const float o = (float)off[j];
+    sy[k] = (dy[j][t] - ho[j]) / o;
+    sx[k] = dx[j][t] / o;
+    ++k;
+  }
+  float vy = 0.f, vx = 0.f;
+  if (k > 0) {
+    for (int a = 1; a < k; ++a)
+      for (int b = a; b > 0 && sy[b] < sy[b - 1]; --b) {
+        const float z = sy[b];
+        sy[b] = sy[b - 1];
+        sy[b - 1] = z;
+      }
+    for (int a = 1; a < k; ++a)
+      for (int b = a; b > 0 && sx[b] < sx[b - 1]; --b) {
+        const float z = sx[b];
+        sx[b] = sx[b - 1];
+        sx[b - 1] = z;
+      }
+    vy = sy[k / 2];
+    vx = sx[k / 2];
+    if (k % 2 == 0) {
+      vy = 0.5f * (vy + sy[k / 2 - 1]);
+      vx = 0.5f * (vx + sx[k / 2 - 1]);
+    }
+  }
+  float agree = 0.f;
+  for (int j = 0; j < m; ++j) {
+    const float o = (float)off[j];
+    const float py = o * vy + ho[j], px = o * vx;
+    const float dev = fabsf(dy[j][t] - py) + fabsf(dx[j][t] - px);
+    if (dev <= 1.5f) agree += 1.f;
+  }
+  if (agree < 3.f) return;
+  for (int j = 0; j < m; ++j) {
+    const float o = (float)off[j];
+    const float py = o * vy + ho[j], px = o * vx;
+    const float dev = fabsf(dy[j][t] - py) + fabsf(dx[j][t] - px);
+    if (dev <= 1.5f) {
+      dy[j][t] = py;
+      dx[j][t] = px;
+    }
+  }
+}
+
+// The envelope's per-pixel terms (sequence.cpp:817-852): the one-sided max
+// vertical difference energy and the second-difference energy of the
+// decoded baseband (Y, chi).
+__global__ void k_env_terms(int h, int w, const float* Y, const float2* chi, float* gmag, float* lapmag) {
+#pragma clang fp contract(off)
+  const size_t n = (size_t)h * w;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const int y = (int)(i / w), x = (int)(i % w);
+    const size_t im = (size_t)max(0, y - 1) * w + x, ip = (size_t)min(h - 1, y + 1) * w + x;
+    const float y0 = Y[i], ym = Y[im], yp = Y[ip];
+    const float2 c0 = chi[i], cm = chi[im], cp = chi[ip];
+    const float dyu = y0 - ym, dyd = yp - y0;
+    const float dcu = c_abs(c_sub(c0, cm)), dcd = c_abs(c_sub(cp, c0));
+    const float mu = dyu * dyu + dcu * dcu;
+    const float md = dyd * dyd + dcd * dcd;
+    gmag[i] = fmaxf(mu, md);
+    const float ly = y0 - 0.5f * (ym + yp);
+    const float2 hs = make_float2(cm.x + cp.x, cm.y + cp.y);
+    const float lc = c_abs(c_sub(c0, make_float2(0.5f * hs.x, 0.5f * hs.y)));
+    lapmag[i] = ly * ly + lc * lc;
+  }
+}
+
+// The 5-tap sliding RMS along each row (sequence.cpp:855-888), in hvd-core's
+// order of adds and subtracts; one thread per row.
+__global__ void k_smooth_rows(int h, int w, const float* src, float* out) {
+#pragma clang fp contract(off)
+  const int y = blockIdx.x * blockDim.x + threadIdx.x;
+  if (y >= h) return;
+  const float* r = src + (size_t)y * w;
+  float* o = out + (size_t)y * w;
+  float acc = 0.f;
+  for (int x = 0; x < min(3, w); ++x) acc += r[x];
+  for (int x = 0; x < w; ++x) {
+    const int lo = x - 2, hi = x + 2;
+    if (x > 0) {
+      if (hi < w) acc += r[hi];
+      if (lo - 1 >= 0) acc -= r[lo - 1];
+    }
+    const int n_taps = min(hi, w - 1) - max(lo, 0) + 1;
+    o[x] = sqrtf(acc / (float)n_taps);
+  }
+}
+
+// The envelope's noise floor and extremum factor (sequence.cpp:889-939), one
+// block per field: the median of every seventh vgrad sample, then per pixel
+// vgrad <- sqrt(max(0, vgrad^2 - (1.5 median)^2)) and lapsm = 1 - t^2 with
+// t = clamp(smoothed second difference / vgrad, 0, 1).
+__global__ void __launch_bounds__(kSelBlock) k_env_finish(int n, const float* const* vg_in, float* const* vgrad,
+                                                          float* const* lap_io, float* const* sample) {
+#pragma clang fp contract(off)
+  __shared__ unsigned hist[16 * 256];
+  const int fi = blockIdx.x;
+  const float* v = vg_in[fi];
+  float* smp = sample[fi];
+  const int ns = (n + 6) / 7;
+  for (int i = threadIdx.x; i < ns; i += kSelBlock) smp[i] = v[(size_t)i * 7];
+  __syncthreads();
+  const float med = median_np(smp, ns, hist);
+  const float fl2 = 2.25f * med * med;
+  float* vg = vgrad[fi];
+  float* lp = lap_io[fi];
+  for (int i = threadIdx.x; i < n; i += kSelBlock) {
+    const float g0 = v[i];
+    const float g = sqrtf(fmaxf(0.0f, g0 * g0 - fl2));
+    vg[i] = g;
+    const float t = g > 1e-6f ? fminf(fmaxf(lp[i] / g, 0.0f), 1.0f) : 0.0f;
+    lp[i] = 1.0f - t * t;
+  }
+}
+
+// Complex plane <-> two real planes (the warps take real planes).
+__global__ void k_split_cplx(size_t n, const float2* z, float* re, float* im) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    re[i] = z[i].x;
+    im[i] = z[i].y;
+  }
+}
+__global__ void k_join_cplx(size_t n, const float* re, const float* im, float2* z) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    z[i] = make_float2(re[i], im[i]);
+}
+
+// The neighbour gates on its confidence (sequence.cpp:965-1015): the InSAR
+// coherence (1 - a) + a gamma when gamma is given, and for an odd offset
+// max(floor lapsm, e2 / (e2 + vgrad^2)) when vgrad is given.
+__global__ void k_conf_gates(size_t n, float* conf, const float* gamma, float a, const float* vgrad,
+                             const float* lapsm, float e2, float floor_max) {
+#pragma clang fp contract(off)
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    float c = conf[i];
+    if (gamma) c *= (1.0f - a) + a * gamma[i];
+    if (vgrad) {
+      const float b = vgrad[i];
+      const float fl = floor_max * lapsm[i];
+      c *= fmaxf(fl, e2 / (e2 + b * b));
+    }
+    conf[i] = c;
+  }
+}
+
+// SynthReference (sequence.cpp:52-150): the accumulators start at the
+// field's own decode; each neighbour adds w (Yw, cw) with the Geman-McClure
+// weight w = conf eps^2 / (|dY|^2 + |dchi|^2 + eps^2).
+__global__ void k_synth_init(size_t n, const float* Y, const float2* chi, float* aY, float* aCr, float* aCi,
+                             float* aW) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    aY[i] = Y[i];
+    aCr[i] = chi[i].x;
+    aCi[i] = chi[i].y;
+    aW[i] = 1.0f;
+  }
+}
+__global__ void k_synth_accum(size_t n, const float* Y, const float2* chi, const float* conf, const float* Yw,
+                              const float* wre, const float* wim, float eps_b, float* aY, float* aCr, float* aCi,
+                              float* aW) {
+#pragma clang fp contract(off)
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const float dy = Yw[i] - Y[i];
+    const float2 dc = c_sub(make_float2(wre[i], wim[i]), chi[i]);
+    const float d2 = dy * dy + (dc.x * dc.x + dc.y * dc.y);
+    const float w = conf[i] * eps_b * eps_b / (d2 + eps_b * eps_b);
+    aY[i] += w * Yw[i];
+    aCr[i] += w * wre[i];
+    aCi[i] += w * wim[i];
+    aW[i] += w;
+  }
+}
+__global__ void k_synth_finish(size_t n, const float* s, const float2* car, const float* aY, const float* aCr,
+                               const float* aCi, const float* aW, float* luma, float2* chroma, float* resid) {
+#pragma clang fp contract(off)
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const float l = aY[i] / aW[i];
+    const float2 c = make_float2(aCr[i] / aW[i], aCi[i] / aW[i]);
+    luma[i] = l;
+    chroma[i] = c;
+    const float s_hat = l + re_of(c, car[i]);
+    resid[i] = fabsf(s[i] - s_hat);
+  }
+}
+__global__ void k_synth_weight(size_t n, const float* r, float eps_b, float* weight) {
+#pragma clang fp contract(off)
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    weight[i] = eps_b * eps_b / (r[i] * r[i] + eps_b * eps_b);
+}
+
+// PsiClosedForm (sequence.cpp:160-226): per pixel the 3 x 3 weighted normal
+// equations over the field's own observation and its neighbours', solved by
+// Cramer in FP64, blended with the fallback chi by the neighbour weight.
+__global__ void k_psi(size_t n, int nn, const float* s, const float2* car, const float* const* sw,
+                      const float2* const* cw, const float* const* conf, float2* chi) {
+#pragma clang fp contract(off)
+  const float kEpsT = 6.0f;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    double a11 = 0, a12 = 0, a13 = 0, a22 = 0, a23 = 0, a33 = 0, b1 = 0, b2 = 0, b3 = 0, wsum = 0;
+    const float2 fb = chi[i];
+    auto add = [&](float Sk, float2 ck, float wk) {
+      const double rc = ck.x, ic = -ck.y;
+      a11 += wk;
+      a12 += wk * rc;
+      a13 += wk * ic;
+      a22 += wk * rc * rc;
+      a23 += wk * rc * ic;
+      a33 += wk * ic * ic;
+      const float wsk = wk * Sk;
+      b1 += wsk;
+      b2 += wsk * rc;
+      b3 += wsk * ic;
+    };
+    const float2 c0 = car[i];
+    add(s[i], c0, 1.0f);
+    for (int j = 0; j < nn; ++j) {
+      const float2 cj = cw[j][i];
+      const float2 dc = c_sub(c0, cj);
+      const float rt = (sw[j][i] - s[i]) + re_of(fb, dc);
+      const float wk = conf[j][i] * kEpsT * kEpsT / (rt * rt + kEpsT * kEpsT);
+      add(sw[j][i], cj, wk);
+      wsum += wk;
+    }
+    const double det = a11 * (a22 * a33 - a23 * a23) - a12 * (a12 * a33 - a23 * a13) + a13 * (a12 * a23 - a22 * a13);
+    const double a11c = fmax(a11, 1e-9);
+    const bool ok = det > 1e-3 * a11c * a11c * a11c * 0.01;
+    const double d = ok ? det : 1.0;
+    const double q1 = (a11 * (b2 * a33 - b3 * a23) - b1 * (a12 * a33 - a23 * a13) + a13 * (a12 * b3 - b2 * a13)) / d;
+    const double q2 = (a11 * (a22 * b3 - a23 * b2) - a12 * (a12 * b3 - b2 * a13) + b1 * (a12 * a23 - a22 * a13)) / d;
+    const float t = ok ? fminf(fmaxf((float)(wsum / 2.0), 0.0f), 1.0f) : 0.0f;
+    const float pr = (float)q1, pi = (float)q2;
+    chi[i] = make_float2(t * pr + (1.0f - t) * fb.x, t * pi + (1.0f - t) * fb.y);
+  }
+}
+
+}  // namespace hvdk
+
+// ---------------------------------------------------------------------------
+// The host side of the driver.
+// ---------------------------------------------------------------------------
+namespace hvd_window {
+
+namespace {
+
+using namespace hvdk;
+
+inline unsigned grid_for(size_t n) { return (unsigned)std::min<size_t>((n + 255) / 256, 8192); }
+
+struct Carver {
+  size_t off = 0;
+  size_t take(size_t b) {
+    const size_t o = off;
+    off += (b + 255) & ~size_t(255);
+    return o;
+  }
+};
+
+// numpy median of host floats (MedianOf, sequence.cpp:28-38).
+float median_host(std::vector<float> v) {
+  if (v.empty()) return 0.f;
+  std::sort(v.begin(), v.end());
+  const size_t k = v.size() / 2;
+  float m = v[k];
+  if (v.size() % 2 == 0) m = 0.5F * (m + v[k - 1]);
+  return m;
+}
+
+struct Ctx {
+  hvd_gpu_ctx* ctx;
+  hipStream_t st;
+  int rc = 0;
+  bool ok(int r) {
+    if (r != HVD_GPU_OK && rc == 0) rc = r;
+    return rc == 0;
+  }
+  bool launched() {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess && rc == 0) rc = fail(ctx, HVD_GPU_ERR_HIP, hipGetErrorString(e));
+    return rc == 0;
+  }
+};
+
+// One motion field on the device (tiles).
+struct MF {
+  float *dy, *dx, *conf;
+};
+
+}  // namespace
+
+int drive(hvd_gpu_ctx* ctx, const hvd_gpu_plane* F, int nf, const int* par, int is_pal, int y_given,
+          const hvd_solver_params* p_in, const hvd_window_params* wp, hvd_window_diag* diag, hipStream_t st) {
+  if (!F || !par || !p_in || !wp || nf < 1) return fail(ctx, HVD_GPU_ERR_INVALID, "null argument or empty window");
+  const int h = F[0].lines, w = F[0].width;
+  for (int j = 0; j < nf; ++j) {
+    if (F[j].lines != h || F[j].width != w) return fail(ctx, HVD_GPU_ERR_INVALID, "all fields of a window share one shape");
+    if (!F[j].s || !F[j].carrier || !F[j].chi || !F[j].y)
+      return fail(ctx, HVD_GPU_ERR_INVALID, "each field needs s, an explicit carrier, chi and y planes");
+  }
+  if (h < 4 || w < 4) return fail(ctx, HVD_GPU_ERR_INVALID, "fields must be at least 4 x 4");
+  hvd_window_params c = *wp;
+  if (c.mc_tile < 1 || c.mc_tile > 256) return fail(ctx, HVD_GPU_ERR_INVALID, "mc_tile outside 1..256");
+  if (c.mc_search < 0 || c.mc_search > 64) return fail(ctx, HVD_GPU_ERR_INVALID, "mc_search outside 0..64");
+  if (c.nr_radius < 0 || c.nr_radius > 16) return fail(ctx, HVD_GPU_ERR_INVALID, "nr_radius outside 0..16");
+  hvd_solver_params p = *p_in;
+  p.is_pal = is_pal ? 1 : 0;
+  Ctx X{ctx, st};
+  const size_t np = (size_t)h * w;
+  const int tile = c.mc_tile;
+  const int th = (h + tile - 1) / tile, tw = (w + tile - 1) / tile, nt = th * tw;
+
+  // offsets (sequence.cpp:452-476) and the PAL defaults
+  std::vector<int> offs;
+  if (c.extended_temporal)
+    offs = is_pal ? (c.bidirectional ? std::vector<int>{-4, -3, -2, -1, 1, 2, 3, 4} : std::vector<int>{-4, -3, -2, -1})
+                  : (c.bidirectional ? std::vector<int>{-3, -2, -1, 1, 2, 3} : std::vector<int>{-3, -2, -1});
+  else
+    offs = c.bidirectional ? std::vector<int>{-2, -1, 1, 2} : std::vector<int>{-2, -1};
+  const int no = (int)offs.size();
+  if (is_pal) {
+    if ((float)c.coherence_gate == 0.6F) c.coherence_gate = 1.0;
+    if (c.nr_radius == 2) c.nr_radius = 4;
+  }
+  // the motion cache's offsets: the neighbour offsets first (their slot is
+  // their index in offs), then the anchor's radius beyond them
+  std::vector<int> coffs = offs;
+  for (int o = -c.nr_radius; o <= c.nr_radius; ++o)
+    if (o != 0 && std::find(coffs.begin(), coffs.end(), o) == coffs.end()) coffs.push_back(o);
+  const int nco = (int)coffs.size();
+  const int amb_stride = is_pal ? 4 : 2;
+  const int ny_amb = h > 4 ? (h - 4 + 2) / 3 : 0;
+  const int nbands = (ny_amb + 7) / 8;
+  const int npairs = std::max(0, nf - amb_stride);
+
+  // ---- device memory -------------------------------------------------------
+  Carver cv;
+  const size_t o_sig = cv.take(sizeof(float) * nf);
+  const size_t o_d = cv.take(sizeof(float) * nf * (size_t)h * std::max(1, w - 8));
+  const size_t o_acc = cv.take(sizeof(double) * (size_t)std::max(1, npairs * nbands));
+  const size_t o_cnt = cv.take(sizeof(int) * (size_t)std::max(1, npairs * nbands));
+  const size_t o_mc = cv.take(sizeof(float) * 3 * nt * (size_t)nf * nco);     // motion cache
+  const size_t o_pm = cv.take(sizeof(float) * 2 * nt * (size_t)nf * no);      // snapped copies
+  const size_t o_pred = cv.take(sizeof(float) * 4 * nt * (size_t)nf);
+  const size_t o_nb = cv.take(sizeof(float) * 4 * np * (size_t)nf * no);      // comp, car (2), conf
+  const size_t o_env = cv.take(sizeof(float) * 2 * np * (size_t)nf);          // vgrad, lapsm
+  const size_t o_envt = cv.take(sizeof(float) * 2 * np * (size_t)nf);         // smoothed terms
+  const size_t o_envs = cv.take(sizeof(float) * (np / 7 + 1) * (size_t)nf);   // median samples
+  const size_t o_anc = cv.take(sizeof(float) * 4 * np * (size_t)nf);          // anchor luma, chroma (2), weight
+  const size_t o_tmp = cv.take(sizeof(float) * 12 * np);
+  const size_t o_smf = cv.take(sizeof(float) * 3 * nt);                       // SynthReference's own estimate
+  const size_t o_ptr = cv.take(sizeof(void*) * 64 * (size_t)(nf + 4));       // pointer tables
+  const size_t o_small = cv.take(sizeof(float) * 64 * (size_t)(nf + 4));
+  char* base = pool(ctx, cv.off, st);
+  if (!base) return HVD_GPU_ERR_NOMEM;
+  float* sig = (float*)(base + o_sig);
+  float* dnoise = (float*)(base + o_d);
+  double* bacc = (double*)(base + o_acc);
+  int* bcnt = (int*)(base + o_cnt);
+  auto mcache = [&](int j, int oi) {
+    float* q = (float*)(base + o_mc) + ((size_t)j * nco + oi) * 3 * nt;
+    return MF{q, q + nt, q + 2 * nt};
+  };
+  auto pmf = [&](int j, int oi) {
+    float* q = (float*)(base + o_pm) + ((size_t)j * no + oi) * 2 * nt;
+    return MF{q, q + nt, nullptr};
+  };
+  float* pred = (float*)(base + o_pred);
+  auto nbp = [&](int j, int oi) { return (float*)(base + o_nb) + ((size_t)j * no + oi) * 4 * np; };
+  auto envp = [&](int j) { return (float*)(base + o_env) + (size_t)j * 2 * np; };
+  auto envt = [&](int j) { return (float*)(base + o_envt) + (size_t)j * 2 * np; };
+  auto envs = [&](int j) { return (float*)(base + o_envs) + (size_t)j * (np / 7 + 1); };
+  auto ancp = [&](int j) { return (float*)(base + o_anc) + (size_t)j * 4 * np; };
+  float* tmp = (float*)(base + o_tmp);
+  auto T = [&](int k) { return tmp + (size_t)k * np; };
+  float* smf = (float*)(base + o_smf);
+  void** ptrs = (void**)(base + o_ptr);
+  float* small = (float*)(base + o_small);
+  auto cfl = [](const float* q) { return (const float2*)q; };
+
+  // ---- noise self-calibration (sequence.cpp:439-449) ------------------------
+  {
+    std::vector<const float*> sp(nf);
+    for (int j = 0; j < nf; ++j) sp[j] = F[j].s;
+    if (hipMemcpyAsync(ptrs, sp.data(), sizeof(void*) * nf, hipMemcpyHostToDevice, st) != hipSuccess)
+      return fail(ctx, HVD_GPU_ERR_HIP, "pointer upload failed");
+    hipLaunchKernelGGL(k_noise_sigma, dim3(nf), dim3(1024), 0, st, (const float* const*)ptrs, h, w, dnoise, sig);
+    if (!X.launched()) return X.rc;
+  }
+  std::vector<float> hsig(nf);
+  if (hipMemcpyAsync(hsig.data(), sig, sizeof(float) * nf, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return fail(ctx, HVD_GPU_ERR_HIP, "noise readback failed");
+  const float sigma = median_host(hsig);
+  float temporal_eps = (float)c.temporal_eps, nr_eps = (float)c.nr_eps;
+  if (temporal_eps <= 0.0F) temporal_eps = std::clamp(7.0F * sigma, 4.0F, 20.0F);
+  if (nr_eps <= 0.0F) nr_eps = std::clamp(3.0F * sigma, 3.0F, 12.0F);
+
+  // ---- the working state: Y and chi in the callers' planes -----------------
+  if (!y_given)
+    for (int j = 0; j < nf; ++j) {
+      hipLaunchKernelGGL(k_fidelity, dim3(grid_for(np)), dim3(256), 0, st, np, F[j].s, cfl(F[j].chi),
+                         cfl(F[j].carrier), F[j].y);
+      if (!X.launched()) return X.rc;
+    }
+
+  // ---- adaptive strength (sequence.cpp:526-652) ------------------------------
+  float strength = (float)c.temporal_strength;
+  float amb = 0.F;
+  if (strength == 0.0F && p.cg_iterations > 0 && nf > amb_stride) {
+    if (npairs > 0 && nbands > 0) {
+      std::vector<const float*> sp(nf), cp(nf);
+      for (int j = 0; j < nf; ++j) {
+        sp[j] = F[j].s;
+        cp[j] = F[j].carrier;
+      }
+      if (hipMemcpyAsync(ptrs, sp.data(), sizeof(void*) * nf, hipMemcpyHostToDevice, st) != hipSuccess ||
+          hipMemcpyAsync(ptrs + nf, cp.data(), sizeof(void*) * nf, hipMemcpyHostToDevice, st) != hipSuccess)
+        return fail(ctx, HVD_GPU_ERR_HIP, "pointer upload failed");
+      hipLaunchKernelGGL(k_amb_bands, dim3(npairs * nbands), dim3(256), 0, st, (const float* const*)ptrs,
+                         (const float* const*)(ptrs + nf), amb_stride, h, w, nr_eps, nbands, bacc, bcnt);
+      if (!X.launched()) return X.rc;
+    }
+    std::vector<double> ha((size_t)npairs * nbands);
+    std::vector<int> hn((size_t)npairs * nbands);
+    if (!ha.empty() &&
+        (hipMemcpyAsync(ha.data(), bacc, sizeof(double) * ha.size(), hipMemcpyDeviceToHost, st) != hipSuccess ||
+         hipMemcpyAsync(hn.data(), bcnt, sizeof(int) * hn.size(), hipMemcpyDeviceToHost, st) != hipSuccess ||
+         hipStreamSynchronize(st) != hipSuccess))
+      return fail(ctx, HVD_GPU_ERR_HIP, "band readback failed");
+    std::vector<float> ambs;
+    for (int pr = 0; pr < npairs; ++pr) {
+      double acc = 0.0;
+      long n = 0;
+      std::vector<float> brms;
+      for (int b = 0; b < nbands; ++b) {
+        acc += ha[(size_t)pr * nbands + b];
+        n += hn[(size_t)pr * nbands + b];
+        if (hn[(size_t)pr * nbands + b] >= 8)
+          brms.push_back((float)(ha[(size_t)pr * nbands + b] / hn[(size_t)pr * nbands + b]));
+      }
+      if (n > 0) {
+        if (!brms.empty()) {
+          const size_t k95 = std::min(brms.size() - 1, (size_t)(0.95 * brms.size()));
+          std::nth_element(brms.begin(), brms.begin() + k95, brms.end());
+          ambs.push_back(std::sqrt(brms[k95]));
+        } else {
+          ambs.push_back(std::sqrt((float)(acc / n)));
+        }
+      }
+    }
+    amb = median_host(ambs);
+    const float sigma_d = nr_eps > 0.0F ? nr_eps / 3.0F : 1.0F;
+    const float noise_floor = sigma_d * 0.29F;
+    amb = std::sqrt(std::max(0.0F, amb * amb - noise_floor * noise_floor));
+    strength = std::clamp(0.5F * amb - 0.05F, 0.0F, 1.5F);
+  }
+  if (strength < 0.0F) strength = 0.0F;
+  if (diag) {
+    diag->resolved_strength = std::max(0.0F, strength);
+    diag->ambiguity_ire = amb;
+    diag->sigma_ire = sigma;
+    diag->temporal_eps = temporal_eps;
+    diag->nr_eps = nr_eps;
+  }
+  const bool has_temporal = strength > 0.0F && p.cg_iterations > 0;
+  const int n_passes = has_temporal ? std::max(1, c.passes) : 1;
+  const bool fast = p.fast != 0;
+
+  // ---- the sweep -------------------------------------------------------------
+  std::vector<char> cached((size_t)nf * nco, 0);  // motion cache validity
+  std::vector<char> cached_fast((size_t)nf * nco, 0);
+  auto oidx = [&](int o) { return (int)(std::find(coffs.begin(), coffs.end(), o) - coffs.begin()); };
+
+  for (int pss = 0; pss < n_passes; ++pss) {
+    std::vector<char>& cache = fast ? cached_fast : cached;
+    if (!fast) std::fill(cache.begin(), cache.end(), 0);
+    // the groups solved as one batch (sequence.cpp:1046-1098)
+    std::vector<std::vector<int>> groups;
+    if (!has_temporal) {
+      groups.push_back({});
+      for (int j = 0; j < nf; ++j) groups.back().push_back(j);
+    } else if (fast && c.parallel_across_fields) {
+      int max_off = 0;
+      for (int o : offs) max_off = std::max(max_off, std::abs(o));
+      const int reach = std::max(max_off, c.nr_anchor > 0.0 ? c.nr_radius : 0);
+      const int stride = reach + 1;
+      for (int color = 0; color < stride; ++color) {
+        groups.push_back({});
+        for (int j = color; j < nf; j += stride) groups.back().push_back(j);
+        if (groups.back().empty()) groups.pop_back();
+      }
+    } else {
+      for (int j = 0; j < nf; ++j) groups.push_back({j});
+    }
+    const bool anchored = pss >= 1 && c.nr_anchor > 0.0 && nf > 1;
+    for (const std::vector<int>& G : groups) {
+      const int ng = (int)G.size();
+      std::vector<std::vector<int>> present(ng);  // offset indices with motion, per field
+      if (has_temporal) {
+        // (a) full searches: uncached offsets in the searched set
+        std::vector<const float*> refs, curs;
+        std::vector<float*> ody, odx, ocf;
+        std::vector<std::pair<int, int>> filled;
+        for (int gi = 0; gi < ng; ++gi) {
+          const int j = G[gi];
+          for (int oi = 0; oi < no; ++oi) {
+            const int o = offs[oi], k = j + o;
+            if (k < 0 || k >= nf) continue;
+            const bool in_full = !fast || o == -1 || o == 1 || o == 2;
+            if (cache[(size_t)j * nco + oi]) {
+              present[gi].push_back(oi);
+            } else if (!c.trajectory_fit || in_full) {
+              const MF m = mcache(j, oi);
+              refs.push_back(F[k].y);
+              curs.push_back(F[j].y);
+              ody.push_back(m.dy);
+              odx.push_back(m.dx);
+              ocf.push_back(m.conf);
+              filled.push_back({j, oi});
+              present[gi].push_back(oi);
+            }
+          }
+        }
+        if (!refs.empty() &&
+            !X.ok(hvd_gpu_estimate_motion(ctx, refs.data(), curs.data(), (int)refs.size(), h, w, tile, c.mc_search,
+                                          ody.data(), odx.data(), ocf.data(), st)))
+          return X.rc;
+        for (auto& f : filled) cache[(size_t)f.first * nco + f.second] = 1;
+        // (b) fast mode: predict the remaining offsets from the measured
+        // ones and audit them with VerifyMotion (sequence.cpp:696-748)
+        if (fast && c.trajectory_fit) {
+          std::vector<const float*> vr, vc, vpy, vpx;
+          std::vector<float*> vdy, vdx, vcf;
+          std::vector<std::pair<int, int>> vfilled;
+          for (int gi = 0; gi < ng; ++gi) {
+            const int j = G[gi];
+            if (present[gi].empty()) continue;
+            std::sort(present[gi].begin(), present[gi].end(), [&](int a, int b) { return offs[a] < offs[b]; });
+            const int m = (int)present[gi].size();
+            std::vector<const float*> pdy(m), pdx(m);
+            std::vector<float> ho(m);
+            std::vector<int> off(m);
+            for (int q = 0; q < m; ++q) {
+              const int oi = present[gi][q];
+              const MF mf = mcache(j, oi);
+              pdy[q] = mf.dy;
+              pdx[q] = mf.dx;
+              ho[q] = ((float)par[j + offs[oi]] - (float)par[j]) / 2.0F;
+              off[q] = offs[oi];
+            }
+            float* vy = pred + (size_t)j * 4 * nt;
+            float* vx = vy + nt;
+            // pointer tables and scalars for this field
+            void** pt = ptrs + (size_t)(j + 2) * 64;
+            float* sm = small + (size_t)(j + 2) * 64;
+            if (hipMemcpyAsync(pt, pdy.data(), sizeof(void*) * m, hipMemcpyHostToDevice, st) != hipSuccess ||
+                hipMemcpyAsync(pt + 16, pdx.data(), sizeof(void*) * m, hipMemcpyHostToDevice, st) != hipSuccess ||
+                hipMemcpyAsync(sm, ho.data(), sizeof(float) * m, hipMemcpyHostToDevice, st) != hipSuccess ||
+                hipMemcpyAsync(sm + 16, off.data(), sizeof(int) * m, hipMemcpyHostToDevice, st) != hipSuccess)
+              return fail(ctx, HVD_GPU_ERR_HIP, "table upload failed");
+            hipLaunchKernelGGL(k_traj_predict, dim3((nt + 63) / 64), dim3(64), 0, st, m, nt,
+                               (const float* const*)pt, (const float* const*)(pt + 16), (const float*)sm,
+                               (const int*)(sm + 16), vy, vx);
+            if (!X.launched()) return X.rc;
+            // synchronous uploads above used host vectors that go out of
+            // scope: wait for them before the next field reuses its tables
+            if (hipStreamSynchronize(st) != hipSuccess) return fail(ctx, HVD_GPU_ERR_HIP, "sync failed");
+            for (int oi = 0; oi < no; ++oi) {
+              const int o = offs[oi], k = j + o;
+              if (k < 0 || k >= nf) continue;
+              if (std::find(present[gi].begin(), present[gi].end(), oi) != present[gi].end()) continue;
+              if (cache[(size_t)j * nco + oi]) {
+                present[gi].push_back(oi);
+                continue;
+              }
+              const float hoo = ((float)par[k] - (float)par[j]) / 2.0F;
+              // one prediction buffer per pending offset, in T(10 .. 11)
+              if ((vfilled.size() + 1) * 2 * nt > 2 * np)
+                return fail(ctx, HVD_GPU_ERR_INVALID, "too many predicted offsets for the window's scratch");
+              float* pbuf = T(10) + (size_t)vfilled.size() * 2 * nt;
+              hipLaunchKernelGGL(k_traj_extrapolate, dim3((nt + 63) / 64), dim3(64), 0, st, nt, (const float*)vy,
+                                 (const float*)vx, (float)o, hoo, pbuf, pbuf + nt);
+              if (!X.launched()) return X.rc;
+              const MF mf = mcache(j, oi);
+              vr.push_back(F[k].y);
+              vc.push_back(F[j].y);
+              vpy.push_back(pbuf);
+              vpx.push_back(pbuf + nt);
+              vdy.push_back(mf.dy);
+              vdx.push_back(mf.dx);
+              vcf.push_back(mf.conf);
+              vfilled.push_back({j, oi});
+              present[gi].push_back(oi);
+            }
+          }
+          if (!vr.empty()) {
+            if (!X.ok(hvd_gpu_verify_motion(ctx, vr.data(), vc.data(), (int)vr.size(), h, w, vpy.data(), vpx.data(),
+                                            tile, vdy.data(), vdx.data(), vcf.data(), st)))
+              return X.rc;
+            for (auto& f : vfilled) cache[(size_t)f.first * nco + f.second] = 1;
+          }
+        }
+        // (c) the snapped working copies (sequence.cpp:749-777)
+        for (int gi = 0; gi < ng; ++gi) {
+          const int j = G[gi];
+          std::sort(present[gi].begin(), present[gi].end(), [&](int a, int b) { return offs[a] < offs[b]; });
+          for (int oi : present[gi]) {
+            const MF src = mcache(j, oi), dst = pmf(j, oi);
+            if (hipMemcpyAsync(dst.dy, src.dy, sizeof(float) * 2 * nt, hipMemcpyDeviceToDevice, st) != hipSuccess)
+              return fail(ctx, HVD_GPU_ERR_HIP, "copy failed");
+          }
+          const int m = (int)present[gi].size();
+          if (c.trajectory_fit && m >= 2) {
+            std::vector<float*> dy(m), dx(m);
+            std::vector<const float*> cf(m);
+            std::vector<float> ho(m);
+            std::vector<int> off(m);
+            for (int q = 0; q < m; ++q) {
+              const int oi = present[gi][q];
+              dy[q] = pmf(j, oi).dy;
+              dx[q] = pmf(j, oi).dx;
+              cf[q] = mcache(j, oi).conf;
+              ho[q] = ((float)par[j + offs[oi]] - (float)par[j]) / 2.0F;
+              off[q] = offs[oi];
+            }
+            void** pt = ptrs + (size_t)(j + 2) * 64;
+            float* sm = small + (size_t)(j + 2) * 64;
+            if (hipMemcpyAsync(pt, dy.data(), sizeof(void*) * m, hipMemcpyHostToDevice, st) != hipSuccess ||
+                hipMemcpyAsync(pt + 16, dx.data(), sizeof(void*) * m, hipMemcpyHostToDevice, st) != hipSuccess ||
+                hipMemcpyAsync(pt + 32, cf.data(), sizeof(void*) * m, hipMemcpyHostToDevice, st) != hipSuccess ||
+                hipMemcpyAsync(sm, ho.data(), sizeof(float) * m, hipMemcpyHostToDevice, st) != hipSuccess ||
+                hipMemcpyAsync(sm + 16, off.data(), sizeof(int) * m, hipMemcpyHostToDevice, st) != hipSuccess)
+              return fail(ctx, HVD_GPU_ERR_HIP, "table upload failed");
+            hipLaunchKernelGGL(k_traj_snap, dim3((nt + 63) / 64), dim3(64), 0, st, m, nt, (float* const*)pt,
+                               (float* const*)(pt + 16), (const float* const*)(pt + 32), (const float*)sm,
+                               (const int*)(sm + 16));
+            if (!X.launched()) return X.rc;
+            if (hipStreamSynchronize(st) != hipSuccess) return fail(ctx, HVD_GPU_ERR_HIP, "sync failed");
+          }
+        }
+      }
+      // (d) the odd-offset envelopes (sequence.cpp:790-940)
+      std::vector<char> has_env(ng, 0);
+      if (has_temporal) {
+        std::vector<const float*> vin;
+        std::vector<float*> vout, lio, smp;
+        for (int gi = 0; gi < ng; ++gi) {
+          const int j = G[gi];
+          bool any_odd = false;
+          for (int o : offs)
+            if (j + o >= 0 && j + o < nf && (o & 1)) any_odd = true;
+          if (!any_odd) continue;
+          has_env[gi] = 1;
+          float* sm = envt(j);  // [smoothed gmag | smoothed lapmag]
+          hipLaunchKernelGGL(k_env_terms, dim3(grid_for(np)), dim3(256), 0, st, h, w, (const float*)F[j].y,
+                             cfl(F[j].chi), T(0), T(1));
+          hipLaunchKernelGGL(k_smooth_rows, dim3((h + 63) / 64), dim3(64), 0, st, h, w, (const float*)T(0), sm);
+          hipLaunchKernelGGL(k_smooth_rows, dim3((h + 63) / 64), dim3(64), 0, st, h, w, (const float*)T(1), sm + np);
+          if (!X.launched()) return X.rc;
+          vin.push_back(sm);
+          vout.push_back(envp(j));
+          lio.push_back(sm + np);
+          smp.push_back(envs(j));
+        }
+        if (!vin.empty()) {
+          const int m = (int)vin.size();
+          if (m > 64) return fail(ctx, HVD_GPU_ERR_INVALID, "more than 64 fields in one colour");
+          void** pt = ptrs;  // the first 64 x 4 slots
+          if (hipMemcpyAsync(pt, vin.data(), sizeof(void*) * m, hipMemcpyHostToDevice, st) != hipSuccess ||
+              hipMemcpyAsync(pt + 64, vout.data(), sizeof(void*) * m, hipMemcpyHostToDevice, st) != hipSuccess ||
+              hipMemcpyAsync(pt + 128, lio.data(), sizeof(void*) * m, hipMemcpyHostToDevice, st) != hipSuccess ||
+              hipMemcpyAsync(pt + 192, smp.data(), sizeof(void*) * m, hipMemcpyHostToDevice, st) != hipSuccess)
+            return fail(ctx, HVD_GPU_ERR_HIP, "table upload failed");
+          hipLaunchKernelGGL(k_env_finish, dim3(m), dim3(1024), 0, st, (int)np, (const float* const*)pt,
+                             (float* const*)(pt + 64), (float* const*)(pt + 128), (float* const*)(pt + 192));
+          if (!X.launched()) return X.rc;
+          if (hipStreamSynchronize(st) != hipSuccess) return fail(ctx, HVD_GPU_ERR_HIP, "sync failed");
+        }
+      }
+      // (e) neighbour equations (sequence.cpp:941-1021)
+      std::vector<std::vector<hvd_gpu_neighbor>> nbrs(ng);
+      if (has_temporal) {
+        const bool gate_on = c.coherence_gate > 0.0 && !(fast && pss == 0);
+        for (int gi = 0; gi < ng; ++gi) {
+          const int j = G[gi];
+          for (int oi = 0; oi < no; ++oi) {
+            const int o = offs[oi], k = j + o;
+            if (k < 0 || k >= nf) continue;
+            const MF mo = pmf(j, oi);
+            const float* mconf = mcache(j, oi).conf;
+            float* nb = nbp(j, oi);
+            float *comp = nb, *car = nb + np, *conf = nb + 3 * np;
+            if (!X.ok(hvd_gpu_motion_compensate_prev(ctx, F[k].s, F[k].carrier, h, w, mo.dy, mo.dx, mconf, th, tw,
+                                                     tile, fast ? 1 : 0, comp, car, conf, st)))
+              return X.rc;
+            const float* gamma = nullptr;
+            if (gate_on) {
+              float *vdy = T(0), *vdx = T(1), *re = T(2), *im = T(3), *wre = T(4), *wim = T(5), *cw = T(6),
+                    *gm = T(8);
+              if (!X.ok(hvd_gpu_vectors_per_pixel(ctx, mo.dy, mo.dx, th, tw, tile, h, w, vdy, vdx, st))) return X.rc;
+              hipLaunchKernelGGL(k_split_cplx, dim3(grid_for(np)), dim3(256), 0, st, np, cfl(F[k].chi), re, im);
+              if (!X.launched()) return X.rc;
+              if (!X.ok(hvd_gpu_warp_bilinear(ctx, re, h, w, vdy, vdx, 0.0f, h, w, wre, st)) ||
+                  !X.ok(hvd_gpu_warp_bilinear(ctx, im, h, w, vdy, vdx, 0.0f, h, w, wim, st)))
+                return X.rc;
+              hipLaunchKernelGGL(k_join_cplx, dim3(grid_for(np)), dim3(256), 0, st, np, (const float*)wre,
+                                 (const float*)wim, (float2*)cw);
+              if (!X.launched()) return X.rc;
+              if (!X.ok(hvd_gpu_complex_coherence(ctx, F[j].chi, cw, h, w, 6, gm, st))) return X.rc;
+              gamma = gm;
+            }
+            const bool odd_gate = (o & 1) && has_env[gi];
+            if (gamma || odd_gate) {
+              const float e2 = temporal_eps * temporal_eps;
+              const float fmax = std::clamp((float)c.odd_gate_floor, 0.0F, 1.0F);
+              hipLaunchKernelGGL(k_conf_gates, dim3(grid_for(np)), dim3(256), 0, st, np, conf, gamma,
+                                 (float)c.coherence_gate, odd_gate ? (const float*)envp(j) : nullptr,
+                                 odd_gate ? (const float*)envt(j) + np : nullptr, e2, fmax);
+              if (!X.launched()) return X.rc;
+            }
+            nbrs[gi].push_back(hvd_gpu_neighbor{comp, car, conf});
+          }
+        }
+      }
+      // (f) the PSI init (sequence.cpp:1023-1025)
+      if (pss == 0 && c.psi_init)
+        for (int gi = 0; gi < ng; ++gi) {
+          const int j = G[gi];
+          const int m = (int)nbrs[gi].size();
+          if (m == 0) continue;
+          std::vector<const void*> sw(m), cw(m), cf(m);
+          for (int q = 0; q < m; ++q) {
+            sw[q] = nbrs[gi][q].composite;
+            cw[q] = nbrs[gi][q].carrier;
+            cf[q] = nbrs[gi][q].confidence;
+          }
+          void** pt = ptrs + (size_t)(j + 2) * 64;
+          if (hipMemcpyAsync(pt, sw.data(), sizeof(void*) * m, hipMemcpyHostToDevice, st) != hipSuccess ||
+              hipMemcpyAsync(pt + 16, cw.data(), sizeof(void*) * m, hipMemcpyHostToDevice, st) != hipSuccess ||
+              hipMemcpyAsync(pt + 32, cf.data(), sizeof(void*) * m, hipMemcpyHostToDevice, st) != hipSuccess)
+            return fail(ctx, HVD_GPU_ERR_HIP, "table upload failed");
+          hipLaunchKernelGGL(k_psi, dim3(grid_for(np)), dim3(256), 0, st, np, m, F[j].s, cfl(F[j].carrier),
+                             (const float* const*)pt, (const float2* const*)(pt + 16), (const float* const*)(pt + 32),
+                             (float2*)F[j].chi);
+          if (!X.launched()) return X.rc;
+          if (hipStreamSynchronize(st) != hipSuccess) return fail(ctx, HVD_GPU_ERR_HIP, "sync failed");
+        }
+      // (g) SynthReference anchors (sequence.cpp:52-150, :1026-1034)
+      std::vector<hvd_gpu_anchor> anchors;
+      if (anchored) {
+        anchors.resize(ng);
+        for (int gi = 0; gi < ng; ++gi) {
+          const int j = G[gi];
+          float *aY = T(0), *aCr = T(1), *aCi = T(2), *aW = T(3), *cup = T(4), *vdy = T(5), *vdx = T(6), *Yw = T(7),
+                *re = T(8), *im = T(9), *wre = T(10), *wim = T(11);
+          hipLaunchKernelGGL(k_synth_init, dim3(grid_for(np)), dim3(256), 0, st, np, (const float*)F[j].y,
+                             cfl(F[j].chi), aY, aCr, aCi, aW);
+          if (!X.launched()) return X.rc;
+          for (int k = j - c.nr_radius; k <= j + c.nr_radius; ++k) {
+            if (k == j || k < 0 || k >= nf) continue;
+            MF mo{smf, smf + nt, smf + 2 * nt};
+            const int oi = oidx(k - j);
+            if (fast && cached_fast[(size_t)j * nco + oi]) {
+              mo = mcache(j, oi);
+            } else {
+              MF dst = fast ? mcache(j, oi) : mo;
+              const float* r1[1] = {F[k].y};
+              const float* c1[1] = {F[j].y};
+              float* d1[1] = {dst.dy};
+              float* x1[1] = {dst.dx};
+              float* f1[1] = {dst.conf};
+              if (!X.ok(hvd_gpu_estimate_motion(ctx, r1, c1, 1, h, w, tile, c.mc_search, d1, x1, f1, st)))
+                return X.rc;
+              if (fast) cached_fast[(size_t)j * nco + oi] = 1;
+              mo = dst;
+            }
+            const float row_off = ((float)par[j] - (float)par[k]) / 2.0F;
+            if (!X.ok(hvd_gpu_upsample_confidence(ctx, mo.conf, th, tw, tile, h, w, 0, cup, st)) ||
+                !X.ok(hvd_gpu_vectors_per_pixel(ctx, mo.dy, mo.dx, th, tw, tile, h, w, vdy, vdx, st)) ||
+                !X.ok(hvd_gpu_warp_bilinear(ctx, F[k].y, h, w, vdy, vdx, row_off, h, w, Yw, st)))
+              return X.rc;
+            hipLaunchKernelGGL(k_split_cplx, dim3(grid_for(np)), dim3(256), 0, st, np, cfl(F[k].chi), re, im);
+            if (!X.launched()) return X.rc;
+            if (!X.ok(hvd_gpu_warp_bilinear(ctx, re, h, w, vdy, vdx, row_off, h, w, wre, st)) ||
+                !X.ok(hvd_gpu_warp_bilinear(ctx, im, h, w, vdy, vdx, row_off, h, w, wim, st)))
+              return X.rc;
+            hipLaunchKernelGGL(k_synth_accum, dim3(grid_for(np)), dim3(256), 0, st, np, (const float*)F[j].y,
+                               cfl(F[j].chi), (const float*)cup, (const float*)Yw, (const float*)wre,
+                               (const float*)wim, nr_eps, aY, aCr, aCi, aW);
+            if (!X.launched()) return X.rc;
+          }
+          float* an = ancp(j);
+          float *al = an, *ac = an + np, *aw = an + 3 * np;
+          float* resid = T(4);
+          float* rb = T(5);
+          hipLaunchKernelGGL(k_synth_finish, dim3(grid_for(np)), dim3(256), 0, st, np, F[j].s, cfl(F[j].carrier),
+                             (const float*)aY, (const float*)aCr, (const float*)aCi, (const float*)aW, al,
+                             (float2*)ac, resid);
+          if (!X.launched()) return X.rc;
+          if (!X.ok(hvd_gpu_box_blur(ctx, resid, h, w, 1, rb, st))) return X.rc;
+          hipLaunchKernelGGL(k_synth_weight, dim3(grid_for(np)), dim3(256), 0, st, np, (const float*)rb, nr_eps, aw);
+          if (!X.launched()) return X.rc;
+          anchors[gi] = hvd_gpu_anchor{al, ac, aw};
+        }
+      }
+      // (h) the solves of the group (sequence.cpp:1026-1040)
+      std::vector<hvd_gpu_plane> planes(ng);
+      std::vector<hvd_gpu_neighbor> flat;
+      std::vector<int> nn(ng);
+      for (int gi = 0; gi < ng; ++gi) {
+        planes[gi] = F[G[gi]];
+        planes[gi].theta = nullptr;
+        planes[gi].vswitch = nullptr;
+        nn[gi] = (int)nbrs[gi].size();
+        flat.insert(flat.end(), nbrs[gi].begin(), nbrs[gi].end());
+      }
+      if (anchored) {
+        if (!X.ok(hvd_gpu_variational_refine_joint(ctx, planes.data(), ng, flat.data(), nn.data(), anchors.data(), &p,
+                                                   strength, temporal_eps, (float)c.nr_anchor, nullptr, st)))
+          return X.rc;
+      } else if (has_temporal) {
+        if (!X.ok(hvd_gpu_variational_refine_temporal_batch(ctx, planes.data(), ng, flat.data(), nn.data(), &p,
+                                                            strength, temporal_eps, nullptr, st)))
+          return X.rc;
+      } else {
+        if (!X.ok(hvd_gpu_variational_refine(ctx, planes.data(), ng, &p, nullptr, nullptr, st))) return X.rc;
+      }
+    }
+  }
+  // ---- output luma (sequence.cpp:1100-1114) ----------------------------------
+  if (c.output_fidelity)
+    for (int j = 0; j < nf; ++j) {
+      hipLaunchKernelGGL(k_fidelity, dim3(grid_for(np)), dim3(256), 0, st, np, F[j].s, cfl(F[j].chi),
+                         cfl(F[j].carrier), F[j].y);
+      if (!X.launched()) return X.rc;
+    }
+  if (hipStreamSynchronize(st) != hipSuccess) return fail(ctx, HVD_GPU_ERR_HIP, "window sync failed");
+  return X.rc;
+}
+
+}  // namespace hvd_window

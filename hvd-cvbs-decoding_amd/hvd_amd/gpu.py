@@ -30,6 +30,7 @@ ABI_SYMBOLS = (
     "hvd_gpu_warp_bilinear", "hvd_gpu_upsample_confidence", "hvd_gpu_complex_coherence",
     "hvd_gpu_motion_compensate_prev", "hvd_gpu_variational_refine_noalloc", "hvd_gpu_woven_size",
     "hvd_gpu_woven_picture", "hvd_gpu_variational_refine_temporal_batch", "hvd_gpu_variational_refine_joint",
+    "hvd_gpu_verify_motion", "hvd_gpu_decode_window",
 )
 
 
@@ -86,6 +87,26 @@ class Plane(C.Structure):
 class Neighbor(C.Structure):
     """hvd::NeighborTerm (variational.h:54-58) as device pointers."""
     _fields_ = [("composite", C.c_void_p), ("carrier", C.c_void_p), ("confidence", C.c_void_p)]
+
+
+class WindowParams(C.Structure):
+    """hvd_window_params: the window driver's HvdConfig knobs (hvd_config.h defaults)."""
+    _fields_ = [(n, C.c_double) for n in ("temporal_strength", "temporal_eps", "nr_anchor", "nr_eps",
+                                          "coherence_gate", "odd_gate_floor")] + \
+        [(n, C.c_int) for n in ("passes", "nr_radius", "mc_tile", "mc_search", "bidirectional", "extended_temporal",
+                                "trajectory_fit", "psi_init", "output_fidelity", "parallel_across_fields")]
+
+    @classmethod
+    def make(cls, **kw) -> "WindowParams":
+        d = dict(temporal_strength=0.0, temporal_eps=0.0, nr_anchor=1.0, nr_eps=0.0, coherence_gate=0.6,
+                 odd_gate_floor=0.35, passes=2, nr_radius=2, mc_tile=32, mc_search=16, bidirectional=1,
+                 extended_temporal=1, trajectory_fit=1, psi_init=0, output_fidelity=1, parallel_across_fields=1)
+        d.update(kw)
+        return cls(**d)
+
+
+class WindowDiag(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("resolved_strength", "ambiguity_ire", "sigma_ire", "temporal_eps", "nr_eps")]
 
 
 class Anchor(C.Structure):
@@ -156,6 +177,10 @@ def load() -> C.CDLL:
                                                          C.POINTER(C.c_int), C.POINTER(Anchor),
                                                          C.POINTER(SolverParams), C.c_float, C.c_float, C.c_float,
                                                          vp, vp]
+        lib.hvd_gpu_verify_motion.argtypes = [vp, pp, pp, i, i, i, pp, pp, i, pp, pp, pp, vp]
+        lib.hvd_gpu_decode_window.argtypes = [vp, C.POINTER(Plane), i, C.POINTER(C.c_int), i, i,
+                                              C.POINTER(SolverParams), C.POINTER(WindowParams),
+                                              C.POINTER(WindowDiag), vp]
         lib.hvd_gpu_decode_fields_u16.argtypes = [vp, pp, i, C.POINTER(Geometry), C.c_float, C.c_float,
                                                   C.POINTER(SolverParams), C.c_float, pp, pp, pp, vp]
         lib.hvd_gpu_decode_fields_ire.argtypes = [vp, pp, i, C.POINTER(Geometry), C.POINTER(SolverParams),
@@ -390,6 +415,30 @@ class Engine:
         if want_stats:
             return np.array(iters[:], dtype=np.int32).reshape(len(planes), outer)
         return None
+
+    def verify_motion(self, refs, curs, pdys, pdxs, tile=32):
+        import torch
+        n = len(refs)
+        h, w = refs[0].shape
+        th, tw = -(-h // tile), -(-w // tile)
+        outs = [[torch.zeros((th, tw), dtype=torch.float32, device=refs[0].device) for _ in range(n)]
+                for _ in range(3)]
+        self._check(self.lib.hvd_gpu_verify_motion(self.ctx, _ptrs(refs), _ptrs(curs), n, h, w, _ptrs(pdys),
+                                                   _ptrs(pdxs), tile, _ptrs(outs[0]), _ptrs(outs[1]),
+                                                   _ptrs(outs[2]), _stream()), "verify_motion")
+        return outs
+
+    def decode_window(self, planes, parities, is_pal, params: SolverParams, wparams: WindowParams, y_given=False):
+        """hvd-core's DecodeFieldWindowWithInits on the device: planes with s,
+        an explicit carrier, chi (chi0 in / chi out) and y (Y0 in when
+        y_given / Y out).  Returns the diagnostics dict."""
+        n = len(planes)
+        P = (Plane * n)(*planes)
+        par = (C.c_int * n)(*[int(v) for v in parities])
+        d = WindowDiag()
+        self._check(self.lib.hvd_gpu_decode_window(self.ctx, P, n, par, int(is_pal), int(y_given), C.byref(params),
+                                                   C.byref(wparams), C.byref(d), _stream()), "decode_window")
+        return {k: float(getattr(d, k)) for k, _ in WindowDiag._fields_}
 
     # ---- temporal neighbours' warps (temporal.cpp) ----------------------------
     def vectors_per_pixel(self, mdy, mdx, tile, h, w):

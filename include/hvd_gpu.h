@@ -91,6 +91,36 @@ typedef struct {
   float* y;   /* lines * width: luma out, may be NULL */
 } hvd_gpu_plane;
 
+/* The HvdConfig knobs of hvd-core's window driver (engine/hvd_config.h,
+ * sequence.cpp:371-1114) beyond the solver's. */
+typedef struct {
+  double temporal_strength; /* < 0: 3-D off; 0: adaptive (measured); > 0: fixed */
+  double temporal_eps;      /* <= 0: 7 sigma, clamped to [4, 20] */
+  double nr_anchor;         /* anchor weight of the passes >= 1; 0: no anchor */
+  double nr_eps;            /* <= 0: 3 sigma, clamped to [3, 12] */
+  double coherence_gate;
+  double odd_gate_floor;
+  int passes;
+  int nr_radius;
+  int mc_tile;
+  int mc_search;
+  int bidirectional;
+  int extended_temporal;
+  int trajectory_fit;
+  int psi_init;
+  int output_fidelity;
+  int parallel_across_fields;
+} hvd_window_params;
+
+/* hvd::SequenceDiagnostics (sequence.h:73-79): what the driver decided. */
+typedef struct {
+  float resolved_strength;
+  float ambiguity_ire;
+  float sigma_ire;
+  float temporal_eps;
+  float nr_eps;
+} hvd_window_diag;
+
 /* hvd::FrameParams (frame_bridge.h:40-101): a field-sequential frame's
  * geometry and levels in the 10-bit sample domain.  extra_sample_lines lists,
  * once per extra sample, the frame-flat lines that carry one more sample
@@ -238,6 +268,31 @@ int hvd_gpu_holographic_init(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int 
  * asking for either makes the call synchronous. */
 int hvd_gpu_variational_refine(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int n, const hvd_solver_params* p,
                                int* stage_iters, float* aniso, void* stream);
+
+/* VerifyMotion (motion.cpp:490-525) for n pairs of h x w planes: per tile
+ * the rounded prediction (pdy, pdx: th x tw floats) against the zero
+ * vector, one SSD each on the r = 2 blurred planes; the winner (zero on
+ * ties; the prediction keeps its float value) and its motion confidence. */
+int hvd_gpu_verify_motion(hvd_gpu_ctx* ctx, const float* const* y_ref, const float* const* y_cur, int n, int h,
+                          int w, const float* const* pdy, const float* const* pdx, int tile, float* const* dy,
+                          float* const* dx, float* const* conf, void* stream);
+
+/* hvd-core's window driver, DecodeFieldWindowWithInits
+ * (engine/sequence.cpp:371-1114, sequence.h:104-135), over n fields of one
+ * shape: fields[j].s, an explicit carrier, chi0 in fields[j].chi and, when
+ * y_given, Y0 in fields[j].y (else Y0 = S - Re[chi0 c]); on return chi and
+ * Y (S - Re[chi c] with output_fidelity, else the anchored luma) in the same
+ * planes.  parities: n entries (0 top, 1 bottom).  is_pal selects the PAL
+ * offsets and defaults (sequence.cpp:452-476) and the aniso grouping.  The
+ * noise self-calibration, the adaptive strength, per-pass pair motion with
+ * fast mode's prediction + VerifyMotion and the trajectory snap, the
+ * motion-compensated neighbours with the coherence and odd-offset gates,
+ * the PSI init, SynthReference and the joint pass all run on the device;
+ * fields of one colour of the Gauss-Seidel sweep are solved as one batch.
+ * diag: optional HOST output.  Synchronous. */
+int hvd_gpu_decode_window(hvd_gpu_ctx* ctx, const hvd_gpu_plane* fields, int n, const int* parities, int is_pal,
+                          int y_given, const hvd_solver_params* p, const hvd_window_params* wp,
+                          hvd_window_diag* diag, void* stream);
 
 /* Device scratch of hvd_gpu_variational_refine_noalloc: 64 MiB per GPU, held
  * from the library's first context on that GPU until the process ends. */
