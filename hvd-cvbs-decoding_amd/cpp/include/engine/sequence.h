@@ -1,6 +1,7 @@
 // engine/sequence.h -- the field-granularity decode of the hvd-core C++ API
-// (hvd-core/src/engine/sequence.h), 2-D path: every field of a window is
-// initialised and solved in one GPU batch.
+// (hvd-core/src/engine/sequence.h): a window's inits run as one GPU batch,
+// then the library's window driver (2-D, and 3-D with its motion, gates and
+// anchored passes) on the device.
 #ifndef HVD_AMD_ENGINE_SEQUENCE_H_
 #define HVD_AMD_ENGINE_SEQUENCE_H_
 
@@ -39,11 +40,21 @@ struct SequenceDiagnostics {
   float nr_eps = 0.0F;
 };
 
-// The window decode (sequence.cpp:423-1114), 2-D: temporal strength < 0, or
-// 0 with no more fields than the same-parity stride (2 NTSC, 4 PAL), or
-// cg_iterations == 0.  Anything that would engage 3-D returns an empty vector.
+// The window decode (sequence.cpp:270-281, :423-1114): HolographicInit per
+// field, then noise self-calibration, the adaptive or fixed temporal
+// strength, the Gauss-Seidel passes with pair motion, the gated
+// motion-compensated neighbour equations, the PSI init, the SynthReference
+// anchor and joint pass, and the output-fidelity luma.  All fields must
+// share one shape when the window can engage 3-D.  `fft` is not used (the
+// inits run on the device).  Failure: an empty vector (GpuLastError()).
 std::vector<DecodedField> DecodeFieldWindow(const std::vector<FieldObs>& fields, const FieldGeometry& g,
                                             const HvdConfig& cfg, Fft2d* fft, SequenceDiagnostics* diag = nullptr);
+
+// The same driver from the caller's inits (sequence.h:104-135, :139-142):
+// inits index-aligned with fields; an empty luma means Y0 = S - Re[chi0 c].
+std::vector<DecodedField> DecodeFieldWindowWithInits(const std::vector<FieldObs>& fields,
+                                                     const std::vector<DecodedField>& inits, const FieldGeometry& g,
+                                                     const HvdConfig& cfg, SequenceDiagnostics* diag = nullptr);
 
 }  // namespace hvd
 

@@ -81,7 +81,9 @@ bool ResolveField1IsBottom(const FieldInput& block1, const FieldInput& block2, c
                            const HvdConfig& cfg);
 
 // Field-granularity decode of a window of frames; one YcFrameS16 per core
-// frame [core_begin, core_end) (frame_bridge.cpp:722-947), 2-D path.
+// frame [core_begin, core_end) (frame_bridge.cpp:722-947): the fields of all
+// frames decode as one window, 3-D when cfg's temporal strength engages it
+// (the stage passes a negative strength when enable_temporal is off).
 std::vector<YcFrameS16> DecodeFrameSequenceWindow(const std::vector<const int16_t*>& frames, int core_begin,
                                                   int core_end, const FrameParams& fp, const HvdConfig& cfg,
                                                   HvdEngine& engine, int64_t base_frame_id = 0);

@@ -228,9 +228,10 @@ std::vector<YcFrameS16> DecodeFrameSequenceWindow(const std::vector<const int16_
     const hvd_frame_params a = gpu::ToAbi(fp);
     const hvd_solver_params sp = gpu::ToAbi(cfg, fp.uses_vswitch());
     const hvd_bridge_params bp = gpu::BridgeOf(cfg);
-    ss.check(hvd_gpu_decode_frame_window_s16(ss.ctx(), dptr.data(), nframes, core_begin, core_end, &a, &sp, &bp,
-                                             outs.data(), nullptr, nullptr),
-             "hvd_gpu_decode_frame_window_s16");
+    const hvd_window_params wp = gpu::WindowOf(cfg);
+    ss.check(hvd_gpu_decode_frame_window_ex_s16(ss.ctx(), dptr.data(), nframes, core_begin, core_end, &a, &sp, &bp,
+                                                &wp, outs.data(), nullptr, nullptr, nullptr),
+             "hvd_gpu_decode_frame_window_ex_s16");
     std::vector<YcFrameS16> out;
     for (int k = 0; k < ncore; ++k) out.push_back(Output(fp, nflat, lum[k], chr[k], u[k], v[k]));
     return out;

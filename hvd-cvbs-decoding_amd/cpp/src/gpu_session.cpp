@@ -3,6 +3,9 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <map>
+#include <mutex>
+#include <utility>
 
 #include "engine/gpu_status.h"
 #include "frame_bridge.h"
@@ -59,6 +62,86 @@ void Session::autotune(int lines, int width, bool pal, int nfields) {
 Session& ThreadSession() {
   thread_local Session s;
   return s;
+}
+
+namespace {
+
+struct BlockCache {
+  std::mutex mu;
+  std::map<std::pair<int, size_t>, std::vector<void*>> free;  // (device, class) -> blocks
+  std::map<int, size_t> held;                                // cached bytes per device
+};
+
+BlockCache& Cache() {
+  static BlockCache* c = new BlockCache;  // outlives every thread's buffers
+  return *c;
+}
+
+size_t SizeClass(size_t bytes) {
+  if (bytes <= 4096) return 4096;
+  size_t p2 = 4096;
+  while (p2 < bytes) p2 <<= 1;
+  const size_t step = p2 / 16;  // classes of 1/8 of the power of two below
+  return (bytes + step - 1) / step * step;
+}
+
+constexpr size_t kMaxCached = size_t(8) << 30;
+
+}  // namespace
+
+void* CachedMalloc(size_t bytes, size_t* cap, int* device) {
+  int dev = 0;
+  HipCheck(hipGetDevice(&dev), "hipGetDevice");
+  *device = dev;
+  const size_t c = SizeClass(bytes);
+  BlockCache& bc = Cache();
+  {
+    std::lock_guard<std::mutex> lk(bc.mu);
+    auto it = bc.free.find({dev, c});
+    if (it != bc.free.end() && !it->second.empty()) {
+      void* p = it->second.back();
+      it->second.pop_back();
+      bc.held[dev] -= c;
+      *cap = c;
+      return p;
+    }
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, c);
+  if (e != hipSuccess) {
+    // release what the cache holds on this device and try once more
+    std::vector<void*> drop;
+    {
+      std::lock_guard<std::mutex> lk(bc.mu);
+      for (auto& kv : bc.free)
+        if (kv.first.first == dev) {
+          drop.insert(drop.end(), kv.second.begin(), kv.second.end());
+          kv.second.clear();
+        }
+      bc.held[dev] = 0;
+    }
+    (void)hipDeviceSynchronize();
+    for (void* q : drop) (void)hipFree(q);
+    e = hipMalloc(&p, c);
+  }
+  HipCheck(e, "hipMalloc");
+  *cap = c;
+  return p;
+}
+
+void CachedFree(void* p, size_t cap, int dev) {
+  if (!p) return;
+  BlockCache& bc = Cache();
+  {
+    std::lock_guard<std::mutex> lk(bc.mu);
+    if (bc.held[dev] + cap <= kMaxCached) {
+      bc.free[{dev, cap}].push_back(p);
+      bc.held[dev] += cap;
+      return;
+    }
+  }
+  // hipFree acts on the block's own device whichever is current
+  (void)hipFree(p);
 }
 
 void HipCheck(hipError_t e, const char* what) {
@@ -135,6 +218,27 @@ hvd_bridge_params BridgeOf(const HvdConfig& cfg) {
   b.chroma_phase_deg = cfg.chroma_phase_deg;
   b.temporal_strength = cfg.temporal_strength;
   return b;
+}
+
+hvd_window_params WindowOf(const HvdConfig& cfg) {
+  hvd_window_params w{};
+  w.temporal_strength = cfg.temporal_strength;
+  w.temporal_eps = cfg.temporal_eps;
+  w.nr_anchor = cfg.nr_anchor;
+  w.nr_eps = cfg.nr_eps;
+  w.coherence_gate = cfg.coherence_gate;
+  w.odd_gate_floor = cfg.odd_gate_floor;
+  w.passes = cfg.passes;
+  w.nr_radius = cfg.nr_radius;
+  w.mc_tile = cfg.mc_tile;
+  w.mc_search = cfg.mc_search;
+  w.bidirectional = cfg.bidirectional ? 1 : 0;
+  w.extended_temporal = cfg.extended_temporal ? 1 : 0;
+  w.trajectory_fit = cfg.trajectory_fit ? 1 : 0;
+  w.psi_init = cfg.psi_init ? 1 : 0;
+  w.output_fidelity = cfg.output_fidelity ? 1 : 0;
+  w.parallel_across_fields = cfg.parallel_across_fields ? 1 : 0;
+  return w;
 }
 
 bool WantsTemporal(const HvdConfig& cfg, int nfields, bool pal) {

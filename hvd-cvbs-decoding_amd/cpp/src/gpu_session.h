@@ -58,6 +58,16 @@ void HipCheck(hipError_t e, const char* what);
 // Records a failure for GpuLastError() and reports it on stderr.
 void Report(const char* api, const std::exception& e);
 
+// Device memory of the C++ API's buffers, cached per device: a released
+// block goes back to a size-classed free list (sizes rounded up to 1/8 of
+// their power of two) and the next request of that class on that device
+// takes it, so a steady stream of calls of one shape allocates nothing after
+// the first.  All the API's work runs on the device's null stream, which
+// orders a block's reuse after the work that used it.  At most 8 GiB per
+// device stays cached; beyond that a released block is freed.
+void* CachedMalloc(size_t bytes, size_t* cap, int* device);
+void CachedFree(void* p, size_t cap, int device);
+
 template <typename T>
 class DevBuf {
  public:
@@ -65,21 +75,28 @@ class DevBuf {
   ~DevBuf() { release(); }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+  DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_), cap_(o.cap_), dev_(o.dev_) {
+    o.p_ = nullptr;
+    o.n_ = 0;
+    o.cap_ = 0;
+  }
   DevBuf& operator=(DevBuf&& o) noexcept {
     if (this != &o) {
       release();
       p_ = o.p_;
       n_ = o.n_;
+      cap_ = o.cap_;
+      dev_ = o.dev_;
       o.p_ = nullptr;
       o.n_ = 0;
+      o.cap_ = 0;
     }
     return *this;
   }
 
   void resize(size_t n) {
     release();
-    if (n) HipCheck(hipMalloc(reinterpret_cast<void**>(&p_), n * sizeof(T)), "hipMalloc");
+    if (n) p_ = static_cast<T*>(CachedMalloc(n * sizeof(T), &cap_, &dev_));
     n_ = n;
   }
   T* get() const { return p_; }
@@ -93,12 +110,15 @@ class DevBuf {
 
  private:
   void release() {
-    if (p_) (void)hipFree(p_);
+    if (p_) CachedFree(p_, cap_, dev_);
     p_ = nullptr;
     n_ = 0;
+    cap_ = 0;
   }
   T* p_ = nullptr;
   size_t n_ = 0;
+  size_t cap_ = 0;
+  int dev_ = 0;
 };
 
 template <typename T>
@@ -112,6 +132,7 @@ hvd_field_geometry ToAbi(const FieldGeometry& g);
 hvd_solver_params ToAbi(const HvdConfig& cfg, bool is_pal);
 hvd_frame_params ToAbi(const FrameParams& fp);
 hvd_bridge_params BridgeOf(const HvdConfig& cfg);
+hvd_window_params WindowOf(const HvdConfig& cfg);
 
 // Would hvd-core engage its 3-D temporal path for this window?
 // (sequence.cpp:526-527, :652-661: strength > 0, or AUTO (0) with more

@@ -1,5 +1,6 @@
-// sequence.cpp -- PrepareFieldObs and the 2-D DecodeFieldWindow over the GPU
-// library: the whole window's inits and solves run as one batch each.
+// sequence.cpp -- PrepareFieldObs, DecodeFieldWindow and
+// DecodeFieldWindowWithInits over the GPU library: the window's inits run as
+// one batch, then the library's window driver (2-D and 3-D).
 #include <algorithm>
 
 #include "engine/engine.h"
@@ -47,17 +48,32 @@ FieldObs PrepareFieldObs(const FieldInput& field, const FieldGeometry& g, const 
 
 namespace {
 
-std::vector<DecodedField> WindowOn(Session& ss, const std::vector<FieldObs>& fields, const FieldGeometry& g,
+// The window decode on the device.  inits == nullptr: every field's init is
+// HolographicInit's (sequence.cpp:270-281); else the caller's chi0 and,
+// where given, Y0 (DecodeFieldWindowWithInits).  A window of one field
+// shape runs the library's window driver (sequence.cpp:423-1114: noise
+// calibration, adaptive strength, the 3-D passes, output fidelity); fields
+// of differing shapes, which hvd-core does not accept in one window, can
+// only take the 2-D path and run as one solve batch.
+std::vector<DecodedField> WindowOn(Session& ss, const std::vector<FieldObs>& fields,
+                                   const std::vector<DecodedField>* inits, const FieldGeometry& g,
                                    const HvdConfig& cfg, SequenceDiagnostics* diag) {
   const int nf = static_cast<int>(fields.size());
-  if (gpu::WantsTemporal(cfg, nf, g.uses_vswitch()))
-    throw gpu::Error("this window would engage 3-D temporal decoding, which this library does not run; "
-                     "set temporal_strength < 0 for the 2-D path");
+  const bool pal = g.uses_vswitch();
   if (diag) *diag = SequenceDiagnostics{};
   std::vector<DecodedField> out(nf);
   if (nf == 0) return out;
+  if (inits && static_cast<int>(inits->size()) != nf)
+    throw gpu::Error("inits must be index-aligned with fields");
+  bool one_shape = true;
+  for (int j = 1; j < nf; ++j)
+    one_shape = one_shape && fields[j].s.height() == fields[0].s.height() &&
+                fields[j].s.width() == fields[0].s.width();
+  if (!one_shape && (inits || gpu::WantsTemporal(cfg, nf, pal)))
+    throw gpu::Error("all fields of a window must share one shape");
   std::vector<DevBuf<float>> s(nf), car(nf), chi(nf), y(nf);
   std::vector<hvd_gpu_plane> planes(nf);
+  std::vector<int> parities(nf), y_given(nf, 1);
   for (int j = 0; j < nf; ++j) {
     const FieldObs& f = fields[j];
     if (f.carrier.height() != f.s.height() || f.carrier.width() != f.s.width())
@@ -67,6 +83,20 @@ std::vector<DecodedField> WindowOn(Session& ss, const std::vector<FieldObs>& fie
     car[j] = gpu::Upload(reinterpret_cast<const float*>(f.carrier.data()), f.carrier.size() * 2);
     chi[j].resize(f.s.size() * 2);
     y[j].resize(f.s.size());
+    if (inits) {
+      const DecodedField& in = (*inits)[j];
+      if (in.chroma.height() != f.s.height() || in.chroma.width() != f.s.width())
+        throw gpu::Error("init " + std::to_string(j) + ": chroma size differs from S");
+      chi[j].upload(reinterpret_cast<const float*>(in.chroma.data()), f.s.size() * 2);
+      if (in.luma.empty()) {
+        y_given[j] = 0;
+      } else {
+        if (in.luma.height() != f.s.height() || in.luma.width() != f.s.width())
+          throw gpu::Error("init " + std::to_string(j) + ": luma size differs from S");
+        y[j].upload(in.luma.data(), f.s.size());
+      }
+    }
+    parities[j] = f.parity;
     hvd_gpu_plane& p = planes[j];
     p = hvd_gpu_plane{};
     p.lines = f.s.height();
@@ -79,14 +109,30 @@ std::vector<DecodedField> WindowOn(Session& ss, const std::vector<FieldObs>& fie
     p.chi = chi[j].get();
     p.y = y[j].get();
   }
-  ss.autotune(planes[0].lines, planes[0].width, g.uses_vswitch(), nf);
-  // sequence.cpp:423-431 inits, then the decoupled per-field solve with the
-  // standard flag from the geometry (:446) and data-fidelity luma (:1102-1114)
-  ss.check(hvd_gpu_holographic_init(ss.ctx(), planes.data(), nf, cfg.symmetry_variant ? 1 : 0, nullptr),
-           "hvd_gpu_holographic_init");
-  const hvd_solver_params sp = gpu::ToAbi(cfg, g.uses_vswitch());
-  ss.check(hvd_gpu_variational_refine(ss.ctx(), planes.data(), nf, &sp, nullptr, nullptr, nullptr),
-           "hvd_gpu_variational_refine");
+  ss.autotune(planes[0].lines, planes[0].width, pal, nf);
+  // sequence.cpp:270-281: the inits, Y0 with them
+  if (!inits)
+    ss.check(hvd_gpu_holographic_init(ss.ctx(), planes.data(), nf, cfg.symmetry_variant ? 1 : 0, nullptr),
+             "hvd_gpu_holographic_init");
+  const hvd_solver_params sp = gpu::ToAbi(cfg, pal);
+  if (one_shape) {
+    const hvd_window_params wp = gpu::WindowOf(cfg);
+    hvd_window_diag d{};
+    ss.check(hvd_gpu_decode_window(ss.ctx(), planes.data(), nf, parities.data(), pal ? 1 : 0, y_given.data(), &sp,
+                                   &wp, &d, nullptr),
+             "hvd_gpu_decode_window");
+    if (diag) {
+      diag->resolved_strength = d.resolved_strength;
+      diag->ambiguity_ire = d.ambiguity_ire;
+      diag->sigma_ire = d.sigma_ire;
+      diag->temporal_eps = d.temporal_eps;
+      diag->nr_eps = d.nr_eps;
+    }
+  } else {
+    // the decoupled per-field solve with data-fidelity luma (:1046, :1102-1114)
+    ss.check(hvd_gpu_variational_refine(ss.ctx(), planes.data(), nf, &sp, nullptr, nullptr, nullptr),
+             "hvd_gpu_variational_refine");
+  }
   for (int j = 0; j < nf; ++j) {
     const Plane& sj = fields[j].s;
     out[j].luma = Plane(sj.height(), sj.width());
@@ -102,9 +148,20 @@ std::vector<DecodedField> WindowOn(Session& ss, const std::vector<FieldObs>& fie
 std::vector<DecodedField> DecodeFieldWindow(const std::vector<FieldObs>& fields, const FieldGeometry& g,
                                             const HvdConfig& cfg, Fft2d* /*fft*/, SequenceDiagnostics* diag) {
   try {
-    return WindowOn(gpu::ThreadSession(), fields, g, cfg, diag);
+    return WindowOn(gpu::ThreadSession(), fields, nullptr, g, cfg, diag);
   } catch (const std::exception& e) {
     gpu::Report("DecodeFieldWindow", e);
+    return {};
+  }
+}
+
+std::vector<DecodedField> DecodeFieldWindowWithInits(const std::vector<FieldObs>& fields,
+                                                     const std::vector<DecodedField>& inits, const FieldGeometry& g,
+                                                     const HvdConfig& cfg, SequenceDiagnostics* diag) {
+  try {
+    return WindowOn(gpu::ThreadSession(), fields, &inits, g, cfg, diag);
+  } catch (const std::exception& e) {
+    gpu::Report("DecodeFieldWindowWithInits", e);
     return {};
   }
 }
@@ -113,7 +170,7 @@ std::vector<DecodedField> HvdEngine::DecodeSequenceWindow(const std::vector<Fiel
                                                           const FieldGeometry& g, const HvdConfig& cfg,
                                                           SequenceDiagnostics* diag) {
   try {
-    return WindowOn(session(), fields, g, cfg, diag);
+    return WindowOn(session(), fields, nullptr, g, cfg, diag);
   } catch (const std::exception& e) {
     gpu::Report("HvdEngine::DecodeSequenceWindow", e);
     return {};

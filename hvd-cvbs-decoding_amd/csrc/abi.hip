@@ -90,6 +90,11 @@ struct hvd_gpu_ctx {
   // apart from ws, which the component calls it makes may reallocate
   char* win = nullptr;
   size_t win_cap = 0;
+  // the 3-D frame window's field planes (lines x width per field: S, carrier,
+  // Y, chi and the carrier tables), grow-only: they must outlive the window
+  // driver's component calls for the same reason
+  char* fwin = nullptr;
+  size_t fwin_cap = 0;
   Slot na_slot;  // descriptor region of a no-allocation call (no event)
   Slot slots[kSlots];
   int next_slot = 0;
@@ -1165,6 +1170,7 @@ void hvd_gpu_destroy(hvd_gpu_ctx* ctx) {
   drop_graphs(ctx);
   if (ctx->ws) (void)hipFree(ctx->ws);
   if (ctx->win) (void)hipFree(ctx->win);
+  if (ctx->fwin) (void)hipFree(ctx->fwin);
   clear_profile(ctx);
   delete ctx;
 }
@@ -2097,7 +2103,7 @@ namespace {
 // Workspace layout of the field driver, carved after `cv`; `end` covers the
 // solver's carve too, so one ensure_ws(end) sizes the whole call.
 struct DecodeLayout {
-  std::vector<size_t> ire, th, vs, s, row, vsa, col, spec, var;
+  std::vector<size_t> ire, th, vs, s, row, vsa, col, spec, var, tha;
   Carve solver;  // where refine_planes starts carving
   size_t end = 0;
 };
@@ -2105,7 +2111,7 @@ struct DecodeLayout {
 DecodeLayout decode_layout(const hvd_gpu_ctx* ctx, int n, const hvd_field_geometry* geom, bool from_raw, Carve cv,
                            bool diag) {
   DecodeLayout L;
-  for (auto* v : {&L.ire, &L.th, &L.vs, &L.s, &L.row, &L.vsa, &L.col, &L.spec, &L.var}) v->resize(n);
+  for (auto* v : {&L.ire, &L.th, &L.vs, &L.s, &L.row, &L.vsa, &L.col, &L.spec, &L.var, &L.tha}) v->resize(n);
   std::vector<DevField> shapes(n), shapes_pal, shapes_ntsc;
   for (int i = 0; i < n; ++i) {
     const hvd_field_geometry& g = geom[i];
@@ -2121,6 +2127,7 @@ DecodeLayout decode_layout(const hvd_gpu_ctx* ctx, int n, const hvd_field_geomet
     L.col[i] = cv.take(sizeof(float2) * width);
     L.spec[i] = cv.take(sizeof(float2) * (size_t)(lines + 2 * pad) * width);
     L.var[i] = cv.take(sizeof(float2) * np * 2);
+    L.tha[i] = cv.take(sizeof(float) * lines);
     shapes[i].h = lines;
     shapes[i].w = width;
     (uses_vswitch(g) ? shapes_pal : shapes_ntsc).push_back(shapes[i]);
@@ -2144,10 +2151,16 @@ void check_fields(hvd_gpu_ctx* ctx, int n, const hvd_field_geometry* geom) {
 
 // The field driver proper.  The workspace must already hold L.end bytes.
 // dec_out (optional) receives, per field, the pointers the frame weave reads.
+// carriers (optional, lines x width per field): the fields' carrier planes
+// are materialised there with hvd-core's float MakeCarrier / MakeCarrierPal
+// arithmetic (ntsc_geometry.cpp:29-47, :75-93) and the inits read them;
+// else the inits and the solve use the exact factored carrier tables.
+// solve = false stops after the inits.
 void decode_fields_run(hvd_gpu_ctx* ctx, const uint16_t* const* raw, const float* const* field_ire, int n,
                        const hvd_field_geometry* geom, float blanking, float white, const hvd_solver_params* p,
                        float chroma_phase_deg, float* const* y_out, float* const* chi_out, float* const* theta_out,
-                       hipStream_t st, const DecodeLayout& L, std::vector<BridgeField>* dec_out) {
+                       hipStream_t st, const DecodeLayout& L, std::vector<BridgeField>* dec_out,
+                       bool solve = true, float2* const* carriers = nullptr) {
   char* ws = ctx->ws;
   REQUIRE(ctx, ctx->ws_cap >= L.end, "internal: workspace not sized");
   // (1) codes -> IRE, every field in one launch
@@ -2203,9 +2216,17 @@ void decode_fields_run(hvd_gpu_ctx* ctx, const uint16_t* const* raw, const float
     d.signed_by_parity = uses_vswitch(g);
     d.phase_offset = chroma_phase_deg == 0.0f ? 0.0f : -chroma_phase_deg * 3.14159265358979323846f / 180.0f;
     d.w0 = phase_per_sample(g);
+    d.th_active = carriers ? (float*)(ws + L.tha[i]) : nullptr;
     prep[i] = d;
   }
   run_prepare(ctx, prep, true, st);
+  if (carriers)
+    for (int i = 0; i < n; ++i) {
+      hipLaunchKernelGGL(k_make_carrier, dim3((prep[i].width + 255) / 256, prep[i].lines), dim3(256), 0, st,
+                         (const float*)prep[i].th_active, (const int8_t*)prep[i].vsw_active, prep[i].width,
+                         prep[i].a0, (float)prep[i].w0, carriers[i]);
+      LAUNCH_CHECK(ctx);
+    }
   if (theta_out)
     for (int i = 0; i < n; ++i)
       if (theta_out[i])
@@ -2224,6 +2245,7 @@ void decode_fields_run(hvd_gpu_ctx* ctx, const uint16_t* const* raw, const float
     f.rowph = prep[i].rowph;
     f.vsw = prep[i].vsw_active;
     f.colph = prep[i].colph;
+    f.carrier = carriers ? carriers[i] : nullptr;
     f.spec = (float2*)(ws + L.spec[i]);
     f.var[0] = (float2*)(ws + L.var[i]);
     f.var[1] = f.var[0] + (size_t)f.h * f.w;
@@ -2236,6 +2258,13 @@ void decode_fields_run(hvd_gpu_ctx* ctx, const uint16_t* const* raw, const float
     IF[i] = f;
   }
   run_init(ctx, IF, ps, st);
+  if (dec_out) {
+    dec_out->resize(n);
+    for (int i = 0; i < n; ++i)
+      (*dec_out)[i] = BridgeField{IF[i].h, IF[i].w, y_out[i], reinterpret_cast<const float2*>(chi_out[i]),
+                                  IF[i].rowph, IF[i].colph, IF[i].vsw, IF[i].carrier};
+  }
+  if (!solve) return;
   // (5) solve (sequence.cpp:1046: always, with is_pal from the geometry)
   std::vector<DevField> F(n);
   bool any_pal = false, any_ntsc = false;
@@ -2269,12 +2298,6 @@ void decode_fields_run(hvd_gpu_ctx* ctx, const uint16_t* const* raw, const float
     pp.is_pal = any_pal ? 1 : 0;
     refine_planes(ctx, F, pp, st, c, RefineOut{});
   }
-  if (dec_out) {
-    dec_out->resize(n);
-    for (int i = 0; i < n; ++i)
-      (*dec_out)[i] = BridgeField{IF[i].h, IF[i].w, y_out[i], reinterpret_cast<const float2*>(chi_out[i]),
-                                  IF[i].rowph, IF[i].colph, IF[i].vsw};
-  }
 }
 
 int decode_fields(hvd_gpu_ctx* ctx, const uint16_t* const* raw, const float* const* field_ire, int n,
@@ -2298,6 +2321,207 @@ int decode_fields(hvd_gpu_ctx* ctx, const uint16_t* const* raw, const float* con
   });
 }
 
+// HvdConfig's defaults of the window driver's knobs (hvd_config.h:336-418).
+hvd_window_params default_window_params() {
+  hvd_window_params w{};
+  w.temporal_strength = 0.0;
+  w.temporal_eps = 0.0;
+  w.nr_anchor = 1.0;
+  w.nr_eps = 0.0;
+  w.coherence_gate = 0.6;
+  w.odd_gate_floor = 0.35;
+  w.passes = 2;
+  w.nr_radius = 2;
+  w.mc_tile = 32;
+  w.mc_search = 16;
+  w.bidirectional = 1;
+  w.extended_temporal = 1;
+  w.trajectory_fit = 1;
+  w.psi_init = 0;
+  w.output_fidelity = 1;
+  w.parallel_across_fields = 1;
+  return w;
+}
+
+char* ensure_fwin(hvd_gpu_ctx* ctx, size_t bytes, hipStream_t st) {
+  if (bytes <= ctx->fwin_cap) return ctx->fwin;
+  HIPCHK(ctx, hipStreamSynchronize(st));
+  HIPCHK(ctx, hipDeviceSynchronize());
+  if (ctx->fwin) HIPCHK(ctx, hipFree(ctx->fwin));
+  ctx->fwin = nullptr;
+  ctx->fwin_cap = 0;
+  const size_t want = bytes + bytes / 4;
+  void* q = nullptr;
+  if (hipMalloc(&q, want) != hipSuccess) {
+    set_err(ctx, "hipMalloc(%zu) for the frame window failed", want);
+    throw Fail{HVD_GPU_ERR_NOMEM};
+  }
+  ctx->fwin = static_cast<char*>(q);
+  ctx->fwin_cap = want;
+  return ctx->fwin;
+}
+
+// DecodeFrameSequenceWindow (frame_bridge.cpp:722-947) on the device.  The
+// 2-D window runs the field driver's batch; a window that may engage 3-D
+// (sequence.cpp:526-652: a positive strength, or the adaptive 0 with more
+// fields than the ambiguity stride) runs the window driver on the fields'
+// holographic inits, with the parities of the measured field order.
+void frame_window(hvd_gpu_ctx* ctx, const int16_t* const* frames, int nframes, int core_begin, int core_end,
+                  const hvd_frame_params* fp, const hvd_solver_params* p, const hvd_bridge_params* b,
+                  const hvd_window_params* wp_in, const hvd_yc_frame* out, float* acc_gain, hvd_window_diag* diag,
+                  hipStream_t st) {
+  REQUIRE(ctx, fp && p && b && frames && out, "null argument");
+  REQUIRE(ctx, nframes >= 1 && core_begin >= 0 && core_end <= nframes && core_begin < core_end,
+          "bad window: %d frames, core [%d, %d)", nframes, core_begin, core_end);
+  check_params(ctx, p);
+  check_frame(ctx, *fp);
+  const int fw = fp->frame_width, fh = fp->frame_height, f1 = fp->field1_lines;
+  const hvd_field_geometry g = frame_field_geometry(*fp);
+  check_geom(ctx, g, 0);
+  check_active(ctx, g, 0);
+  const bool pal = uses_vswitch(g);
+  const int amb_stride = pal ? 4 : 2;
+  const int nf = 2 * nframes, ncore = core_end - core_begin;
+  const bool three_d = p->cg_iterations > 0 &&
+                       (b->temporal_strength > 0.0f || (b->temporal_strength == 0.0f && nf > amb_stride));
+  hvd_window_params wp = wp_in ? *wp_in : default_window_params();
+  wp.temporal_strength = b->temporal_strength;
+  if (diag) *diag = hvd_window_diag{};
+  for (int t = 0; t < nframes; ++t) REQUIRE(ctx, frames[t], "frame %d: null pointer", t);
+  for (int k = 0; k < ncore; ++k)
+    REQUIRE(ctx, out[k].luma && out[k].chroma && out[k].u_plane && out[k].v_plane, "output %d: null plane", k);
+  const int lines = last_line(g) - g.first_active_field_line, width = g.active_video_end - g.active_video_start;
+  std::vector<hvd_field_geometry> geoms(nf, g);
+  check_fields(ctx, nf, geoms.data());
+  Carve cv;
+  const size_t o_fields = cv.take(sizeof(float) * (size_t)nf * f1 * fw);
+  const size_t o_amps = cv.take(sizeof(float) * nf);
+  const size_t o_dec = cv.take(sizeof(int) * 2 + sizeof(float) * 4);
+  const size_t o_y = cv.take(sizeof(float) * (size_t)nf * lines * width);
+  const size_t o_chi = cv.take(sizeof(float2) * (size_t)nf * lines * width);
+  const DecodeLayout L = decode_layout(ctx, nf, geoms.data(), false, cv, p->diag_prior > 0.0);
+  char* ws = ensure_ws(ctx, L.end, st);
+  float* fields = (float*)(ws + o_fields);
+  float* amps = (float*)(ws + o_amps);
+  int* f1_bottom = (int*)(ws + o_dec);
+  int* vote = f1_bottom + 1;
+  float* gain = (float*)(ws + o_dec + 2 * sizeof(int));
+  const BridgeFrameParams bp = bridge_params(*fp);
+  std::vector<const int16_t*> fr(frames, frames + nframes);
+  std::vector<const float*> fptr(nf);
+  std::vector<float*> fout(nf);
+  for (int j = 0; j < nf; ++j) fout[j] = fields + (size_t)j * f1 * fw;
+  for (int j = 0; j < nf; ++j) fptr[j] = fout[j];
+  std::vector<LockinGeom> LG(nf, lockin_geom(g));
+  std::vector<BridgeOut> bo(ncore);
+  for (int k = 0; k < ncore; ++k) bo[k] = BridgeOut{out[k].luma, out[k].chroma, out[k].u_plane, out[k].v_plane};
+  Stage stg;
+  const size_t oP = stg.put(&bp, 1), oF = stg.put(fr), oFP = stg.put(fptr), oG = stg.put(LG), oFO = stg.put(fout);
+  Slot* sl;
+  char* base = upload(ctx, stg, st, &sl);
+  const BridgeFrameParams* dP = (const BridgeFrameParams*)(base + oP);
+  // (1) split every frame into its two IRE fields
+  hipLaunchKernelGGL(k_split_fields, dim3((fw + 255) / 256, f1, nframes), dim3(256), 0, st, dP,
+                     (const int16_t* const*)(base + oF), fp->blanking_level, (float* const*)(base + oFO));
+  LAUNCH_CHECK(ctx);
+  // (2) burst amplitude per field, window ACC and field order
+  hipLaunchKernelGGL(k_burst_amp, dim3(nf), dim3(256), 0, st, (const LockinGeom*)(base + oG),
+                     (const float* const*)(base + oFP), amps);
+  LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_window_decisions, dim3(1), dim3(256), 0, st, dP, (const float* const*)(base + oFP), nframes,
+                     (const float*)amps, b->acc, b->chroma_gain, g.standard == 1 ? 21.43f : 20.0f, b->field_order,
+                     f1_bottom, gain, vote);
+  LAUNCH_CHECK(ctx);
+  arm(ctx, sl, st);
+  // (3) the fields' decode: Y and chi of every field, and the per-field
+  // pointers the weave reads
+  std::vector<BridgeField> dec;
+  const int* w_f1_bottom = f1_bottom;
+  const float* w_gain = gain;
+  if (!three_d) {
+    std::vector<float*> ys(nf), chis(nf);
+    for (int j = 0; j < nf; ++j) {
+      ys[j] = (float*)(ws + o_y) + (size_t)j * lines * width;
+      chis[j] = (float*)(ws + o_chi) + (size_t)j * lines * width * 2;
+    }
+    decode_fields_run(ctx, nullptr, fptr.data(), nf, geoms.data(), 0.0f, 100.0f, p, b->chroma_phase_deg, ys.data(),
+                      chis.data(), nullptr, st, L, &dec);
+  } else {
+    // the parities of the measured order (frame_bridge.cpp:766-774): block 1
+    // of every frame is the top field unless field 1 is the bottom one
+    int fb = 0;
+    HIPCHK(ctx, hipMemcpyAsync(&fb, f1_bottom, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    std::vector<int> par(nf);
+    for (int j = 0; j < nf; ++j) par[j] = (j % 2 == 0) == (fb == 0) ? 0 : 1;
+    // the planes the driver works on live in their own pool: its component
+    // calls may reallocate the workspace.  The carriers are hvd-core's float
+    // MakeCarrier planes (PrepareFieldObs, sequence.cpp:229-268), which the
+    // inits, the driver and the weave all read.
+    const size_t np = (size_t)lines * width;
+    Carve sc;
+    const size_t s_s = sc.take(sizeof(float) * nf * np), s_car = sc.take(sizeof(float2) * nf * np),
+                 s_y = sc.take(sizeof(float) * nf * np), s_chi = sc.take(sizeof(float2) * nf * np),
+                 s_dec = sc.take(sizeof(int) * 2 + sizeof(float) * 4);
+    char* fwn = ensure_fwin(ctx, sc.off, st);
+    float* Ss = (float*)(fwn + s_s);
+    float2* Car = (float2*)(fwn + s_car);
+    std::vector<float*> ys(nf), chis(nf);
+    std::vector<float2*> cars(nf);
+    for (int j = 0; j < nf; ++j) {
+      ys[j] = (float*)(fwn + s_y) + (size_t)j * np;
+      chis[j] = (float*)(fwn + s_chi) + (size_t)j * np * 2;
+      cars[j] = Car + (size_t)j * np;
+    }
+    std::vector<BridgeField> d0;
+    decode_fields_run(ctx, nullptr, fptr.data(), nf, geoms.data(), 0.0f, 100.0f, p, b->chroma_phase_deg, ys.data(),
+                      chis.data(), nullptr, st, L, &d0, /*solve=*/false, cars.data());
+    for (int j = 0; j < nf; ++j)
+      HIPCHK(ctx, hipMemcpyAsync(Ss + (size_t)j * np, ws + L.s[j], sizeof(float) * np, hipMemcpyDeviceToDevice, st));
+    HIPCHK(ctx, hipMemcpyAsync(fwn + s_dec, ws + o_dec, sizeof(int) * 2 + sizeof(float) * 4,
+                               hipMemcpyDeviceToDevice, st));
+    std::vector<hvd_gpu_plane> planes(nf);
+    std::vector<int> y_given(nf, 1);
+    for (int j = 0; j < nf; ++j) {
+      hvd_gpu_plane& q = planes[j];
+      q = hvd_gpu_plane{};
+      q.lines = lines;
+      q.width = width;
+      q.s = Ss + (size_t)j * np;
+      q.carrier = reinterpret_cast<const float*>(cars[j]);
+      q.active_video_start = g.active_video_start;
+      q.phase_per_sample = phase_per_sample(g);
+      q.sample_rate = g.sample_rate;
+      q.chi = chis[j];
+      q.y = ys[j];
+    }
+    const int rc = hvd_window::drive(ctx, planes.data(), nf, par.data(), pal ? 1 : 0, y_given.data(), p, &wp, diag,
+                                     st);
+    if (rc != HVD_GPU_OK) throw Fail{rc};
+    dec.resize(nf);
+    for (int j = 0; j < nf; ++j)
+      dec[j] = BridgeField{lines, width, ys[j], reinterpret_cast<const float2*>(chis[j]), nullptr, nullptr, nullptr,
+                           cars[j]};
+    w_f1_bottom = (const int*)(fwn + s_dec);
+    w_gain = (const float*)(fwn + s_dec + 2 * sizeof(int));
+  }
+  // (4) weave + requantise the core frames
+  Stage stg2;
+  const size_t oD = stg2.put(dec), oF2 = stg2.put(fr), oP2 = stg2.put(&bp, 1), oO2 = stg2.put(bo);
+  Slot* sl2;
+  char* base2 = upload(ctx, stg2, st, &sl2);
+  hipLaunchKernelGGL(k_weave_requant, dim3((fw + 255) / 256, fh, ncore), dim3(256), 0, st,
+                     (const BridgeFrameParams*)(base2 + oP2), (const int16_t* const*)(base2 + oF2), core_begin,
+                     (const BridgeField*)(base2 + oD), w_f1_bottom, w_gain, b->monochrome,
+                     (const BridgeOut*)(base2 + oO2));
+  LAUNCH_CHECK(ctx);
+  arm(ctx, sl2, st);
+  if (acc_gain) {
+    HIPCHK(ctx, hipMemcpyAsync(acc_gain, w_gain, sizeof(float), hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx, hipStreamSynchronize(st));
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -2307,97 +2531,17 @@ int hvd_gpu_decode_frame_window_s16(hvd_gpu_ctx* ctx, const int16_t* const* fram
                                     const hvd_bridge_params* b, const hvd_yc_frame* out, float* acc_gain,
                                     void* stream) {
   return guarded(ctx, [&] {
-    REQUIRE(ctx, fp && p && b && frames && out, "null argument");
-    REQUIRE(ctx, nframes >= 1 && core_begin >= 0 && core_end <= nframes && core_begin < core_end,
-            "bad window: %d frames, core [%d, %d)", nframes, core_begin, core_end);
-    check_params(ctx, p);
-    check_frame(ctx, *fp);
-    const int fw = fp->frame_width, fh = fp->frame_height, f1 = fp->field1_lines;
-    const int amb_stride = (fp->standard == 1 || fp->standard == 2) ? 4 : 2;
-    const bool three_d = p->cg_iterations > 0 &&
-                         (b->temporal_strength > 0.0f || (b->temporal_strength == 0.0f && 2 * nframes > amb_stride));
-    if (three_d) {
-      set_err(ctx, "3-D decoding (temporal strength %g) is not built; pass a negative strength for 2-D",
-              (double)b->temporal_strength);
-      throw Fail{HVD_GPU_ERR_UNSUPPORTED};
-    }
-    const hvd_field_geometry g = frame_field_geometry(*fp);
-    check_geom(ctx, g, 0);
-    check_active(ctx, g, 0);
-    const int nf = 2 * nframes, ncore = core_end - core_begin;
-    for (int t = 0; t < nframes; ++t) REQUIRE(ctx, frames[t], "frame %d: null pointer", t);
-    for (int k = 0; k < ncore; ++k)
-      REQUIRE(ctx, out[k].luma && out[k].chroma && out[k].u_plane && out[k].v_plane, "output %d: null plane", k);
-    const int lines = last_line(g) - g.first_active_field_line, width = g.active_video_end - g.active_video_start;
-    std::vector<hvd_field_geometry> geoms(nf, g);
-    check_fields(ctx, nf, geoms.data());
-    hipStream_t st = S(stream);
-    Carve cv;
-    const size_t o_fields = cv.take(sizeof(float) * (size_t)nf * f1 * fw);
-    const size_t o_amps = cv.take(sizeof(float) * nf);
-    const size_t o_dec = cv.take(sizeof(int) * 2 + sizeof(float) * 4);
-    const size_t o_y = cv.take(sizeof(float) * (size_t)nf * lines * width);
-    const size_t o_chi = cv.take(sizeof(float2) * (size_t)nf * lines * width);
-    const DecodeLayout L = decode_layout(ctx, nf, geoms.data(), false, cv, p->diag_prior > 0.0);
-    char* ws = ensure_ws(ctx, L.end, st);
-    float* fields = (float*)(ws + o_fields);
-    float* amps = (float*)(ws + o_amps);
-    int* f1_bottom = (int*)(ws + o_dec);
-    int* vote = f1_bottom + 1;
-    float* gain = (float*)(ws + o_dec + 2 * sizeof(int));
-    const BridgeFrameParams bp = bridge_params(*fp);
-    std::vector<const int16_t*> fr(frames, frames + nframes);
-    std::vector<const float*> fptr(nf);
-    std::vector<float*> fout(nf);
-    for (int j = 0; j < nf; ++j) fout[j] = fields + (size_t)j * f1 * fw;
-    for (int j = 0; j < nf; ++j) fptr[j] = fout[j];
-    std::vector<LockinGeom> LG(nf, lockin_geom(g));
-    std::vector<BridgeOut> bo(ncore);
-    for (int k = 0; k < ncore; ++k) bo[k] = BridgeOut{out[k].luma, out[k].chroma, out[k].u_plane, out[k].v_plane};
-    Stage stg;
-    const size_t oP = stg.put(&bp, 1), oF = stg.put(fr), oFP = stg.put(fptr), oG = stg.put(LG), oO = stg.put(bo),
-                 oFO = stg.put(fout);
-    Slot* sl;
-    char* base = upload(ctx, stg, st, &sl);
-    const BridgeFrameParams* dP = (const BridgeFrameParams*)(base + oP);
-    // (1) split every frame into its two IRE fields
-    hipLaunchKernelGGL(k_split_fields, dim3((fw + 255) / 256, f1, nframes), dim3(256), 0, st, dP,
-                       (const int16_t* const*)(base + oF), fp->blanking_level, (float* const*)(base + oFO));
-    LAUNCH_CHECK(ctx);
-    // (2) burst amplitude per field, window ACC and field order
-    hipLaunchKernelGGL(k_burst_amp, dim3(nf), dim3(256), 0, st, (const LockinGeom*)(base + oG),
-                       (const float* const*)(base + oFP), amps);
-    LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_window_decisions, dim3(1), dim3(256), 0, st, dP, (const float* const*)(base + oFP), nframes,
-                       (const float*)amps, b->acc, b->chroma_gain, g.standard == 1 ? 21.43f : 20.0f, b->field_order,
-                       f1_bottom, gain, vote);
-    LAUNCH_CHECK(ctx);
-    arm(ctx, sl, st);
-    // (3) the field driver on all fields of the window
-    std::vector<float*> ys(nf), chis(nf);
-    for (int j = 0; j < nf; ++j) {
-      ys[j] = (float*)(ws + o_y) + (size_t)j * lines * width;
-      chis[j] = (float*)(ws + o_chi) + (size_t)j * lines * width * 2;
-    }
-    std::vector<BridgeField> dec;
-    decode_fields_run(ctx, nullptr, fptr.data(), nf, geoms.data(), 0.0f, 100.0f, p, b->chroma_phase_deg, ys.data(),
-                      chis.data(), nullptr, st, L, &dec);
-    // (4) weave + requantise the core frames
-    Stage stg2;
-    const size_t oD = stg2.put(dec), oF2 = stg2.put(fr), oP2 = stg2.put(&bp, 1), oO2 = stg2.put(bo);
-    Slot* sl2;
-    char* base2 = upload(ctx, stg2, st, &sl2);
-    hipLaunchKernelGGL(k_weave_requant, dim3((fw + 255) / 256, fh, ncore), dim3(256), 0, st,
-                       (const BridgeFrameParams*)(base2 + oP2), (const int16_t* const*)(base2 + oF2), core_begin,
-                       (const BridgeField*)(base2 + oD), (const int*)f1_bottom, (const float*)gain, b->monochrome,
-                       (const BridgeOut*)(base2 + oO2));
-    LAUNCH_CHECK(ctx);
-    arm(ctx, sl2, st);
-    (void)oO;
-    if (acc_gain) {
-      HIPCHK(ctx, hipMemcpyAsync(acc_gain, gain, sizeof(float), hipMemcpyDeviceToHost, st));
-      HIPCHK(ctx, hipStreamSynchronize(st));
-    }
+    frame_window(ctx, frames, nframes, core_begin, core_end, fp, p, b, nullptr, out, acc_gain, nullptr, S(stream));
+  });
+}
+
+int hvd_gpu_decode_frame_window_ex_s16(hvd_gpu_ctx* ctx, const int16_t* const* frames, int nframes, int core_begin,
+                                       int core_end, const hvd_frame_params* fp, const hvd_solver_params* p,
+                                       const hvd_bridge_params* b, const hvd_window_params* wp,
+                                       const hvd_yc_frame* out, float* acc_gain, hvd_window_diag* diag,
+                                       void* stream) {
+  return guarded(ctx, [&] {
+    frame_window(ctx, frames, nframes, core_begin, core_end, fp, p, b, wp, out, acc_gain, diag, S(stream));
   });
 }
 
@@ -2599,7 +2743,7 @@ int hvd_gpu_estimate_motion(hvd_gpu_ctx* ctx, const float* const* y_ref, const f
 }
 
 int hvd_gpu_decode_window(hvd_gpu_ctx* ctx, const hvd_gpu_plane* fields, int n, const int* parities, int is_pal,
-                          int y_given, const hvd_solver_params* p, const hvd_window_params* wp,
+                          const int* y_given, const hvd_solver_params* p, const hvd_window_params* wp,
                           hvd_window_diag* diag, void* stream) {
   return guarded(ctx, [&] {
     REQUIRE(ctx, !ctx->no_alloc, "internal: window inside a no-allocation call");

@@ -163,8 +163,13 @@ __global__ void k_weave_requant(const BridgeFrameParams* FP, const int16_t* cons
       const size_t i = (size_t)fl * d.width + c;
       const float y_ire = d.y[i];
       const float2 chi = monochrome ? make_float2(0.f, 0.f) : d.chi[i];
-      float2 car = cmul(d.rowph[fl], d.colph[c]);
-      if (d.vsw && d.vsw[fl] < 0) car.x = -car.x;
+      float2 car;
+      if (d.carrier) {
+        car = d.carrier[i];
+      } else {
+        car = cmul(d.rowph[fl], d.colph[c]);
+        if (d.vsw && d.vsw[fl] < 0) car.x = -car.x;
+      }
       const float scale = (fp.white_level - fp.blanking_level) / 100.0f;
       lum = clamp_u10((fp.blanking_level + (y_ire * scale)));
       const float re = ((chi.x * car.x) - (chi.y * car.y));
@@ -222,6 +227,7 @@ __global__ void k_weave_woven(const BridgeFrameParams* FP, const int16_t* src, i
 // whose vswitch < 0.  Grid: (ceil(width / 256), lines).
 __global__ void k_make_carrier(const float* theta, const int8_t* vswitch, int width, int a0, float w0,
                                float2* carrier) {
+#pragma clang fp contract(off)
   const int yy = blockIdx.y;
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= width) return;

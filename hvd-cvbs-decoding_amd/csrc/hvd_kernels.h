@@ -37,6 +37,7 @@ struct PrepDesc {
   int std_phase, signed_by_parity;
   float phase_offset;        // -chroma_phase_deg * pi / 180
   double w0;
+  float* th_active;          // lines or null: the trimmed float theta of the active lines
 };
 
 struct FftPlan {
@@ -96,6 +97,7 @@ struct BridgeField {
   const float2* rowph;
   const float2* colph;
   const int8_t* vsw;
+  const float2* carrier;  // explicit carrier plane (lines x width) or null: the tables
 };
 
 struct BridgeOut {

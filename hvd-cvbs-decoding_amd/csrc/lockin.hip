@@ -338,11 +338,13 @@ __global__ void k_prepare(const PrepDesc* P) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < p.lines) {
     const int8_t s = p.vswitch_field ? p.vswitch_field[p.fal + i] : (int8_t)1;
-    double t = (double)p.theta_field[p.fal + i];
+    float tf = p.theta_field[p.fal + i];
     if (p.phase_offset != 0.0f) {
       const float off = p.phase_offset;  // already -deg * pi / 180
-      t = (double)(p.theta_field[p.fal + i] + ((p.signed_by_parity && s < 0) ? -off : off));
+      tf = tf + ((p.signed_by_parity && s < 0) ? -off : off);
     }
+    if (p.th_active) p.th_active[i] = tf;
+    const double t = (double)tf;
     double sn, cs;
     sincos(t, &sn, &cs);
     p.rowph[i] = make_float2((float)cs, (float)sn);

@@ -50,6 +50,6 @@ char* pool(hvd_gpu_ctx* ctx, size_t bytes, hipStream_t st);
 // Records a failure message on the context; returns code.
 int fail(hvd_gpu_ctx* ctx, int code, const char* msg);
 // The window driver (window.hip): 0 or an HVD_GPU_ERR_* code.
-int drive(hvd_gpu_ctx* ctx, const hvd_gpu_plane* fields, int n, const int* parities, int is_pal, int y_given,
+int drive(hvd_gpu_ctx* ctx, const hvd_gpu_plane* fields, int n, const int* parities, int is_pal, const int* y_given,
           const hvd_solver_params* p, const hvd_window_params* wp, hvd_window_diag* diag, hipStream_t st);
 }  // namespace hvd_window

@@ -569,7 +569,7 @@ struct MF {
 
 }  // namespace
 
-int drive(hvd_gpu_ctx* ctx, const hvd_gpu_plane* F, int nf, const int* par, int is_pal, int y_given,
+int drive(hvd_gpu_ctx* ctx, const hvd_gpu_plane* F, int nf, const int* par, int is_pal, const int* y_given,
           const hvd_solver_params* p_in, const hvd_window_params* wp, hvd_window_diag* diag, hipStream_t st) {
   if (!F || !par || !p_in || !wp || nf < 1) return fail(ctx, HVD_GPU_ERR_INVALID, "null argument or empty window");
   const int h = F[0].lines, w = F[0].width;
@@ -677,8 +677,8 @@ int drive(hvd_gpu_ctx* ctx, const hvd_gpu_plane* F, int nf, const int* par, int 
   if (nr_eps <= 0.0F) nr_eps = std::clamp(3.0F * sigma, 3.0F, 12.0F);
 
   // ---- the working state: Y and chi in the callers' planes -----------------
-  if (!y_given)
-    for (int j = 0; j < nf; ++j) {
+  for (int j = 0; j < nf; ++j)
+    if (!y_given || !y_given[j]) {
       hipLaunchKernelGGL(k_fidelity, dim3(grid_for(np)), dim3(256), 0, st, np, F[j].s, cfl(F[j].chi),
                          cfl(F[j].carrier), F[j].y);
       if (!X.launched()) return X.rc;

@@ -30,7 +30,7 @@ ABI_SYMBOLS = (
     "hvd_gpu_warp_bilinear", "hvd_gpu_upsample_confidence", "hvd_gpu_complex_coherence",
     "hvd_gpu_motion_compensate_prev", "hvd_gpu_variational_refine_noalloc", "hvd_gpu_woven_size",
     "hvd_gpu_woven_picture", "hvd_gpu_variational_refine_temporal_batch", "hvd_gpu_variational_refine_joint",
-    "hvd_gpu_verify_motion", "hvd_gpu_decode_window",
+    "hvd_gpu_verify_motion", "hvd_gpu_decode_window", "hvd_gpu_decode_frame_window_ex_s16",
 )
 
 
@@ -178,7 +178,7 @@ def load() -> C.CDLL:
                                                          C.POINTER(SolverParams), C.c_float, C.c_float, C.c_float,
                                                          vp, vp]
         lib.hvd_gpu_verify_motion.argtypes = [vp, pp, pp, i, i, i, pp, pp, i, pp, pp, pp, vp]
-        lib.hvd_gpu_decode_window.argtypes = [vp, C.POINTER(Plane), i, C.POINTER(C.c_int), i, i,
+        lib.hvd_gpu_decode_window.argtypes = [vp, C.POINTER(Plane), i, C.POINTER(C.c_int), i, C.POINTER(C.c_int),
                                               C.POINTER(SolverParams), C.POINTER(WindowParams),
                                               C.POINTER(WindowDiag), vp]
         lib.hvd_gpu_decode_fields_u16.argtypes = [vp, pp, i, C.POINTER(Geometry), C.c_float, C.c_float,
@@ -190,6 +190,10 @@ def load() -> C.CDLL:
         lib.hvd_gpu_decode_frame_window_s16.argtypes = [vp, pp, i, i, i, C.POINTER(FrameParams),
                                                         C.POINTER(SolverParams), C.POINTER(BridgeParams),
                                                         C.POINTER(YcFrame), C.POINTER(C.c_float), vp]
+        lib.hvd_gpu_decode_frame_window_ex_s16.argtypes = [vp, pp, i, i, i, C.POINTER(FrameParams),
+                                                           C.POINTER(SolverParams), C.POINTER(BridgeParams),
+                                                           C.POINTER(WindowParams), C.POINTER(YcFrame),
+                                                           C.POINTER(C.c_float), C.POINTER(WindowDiag), vp]
         lib.hvd_gpu_split_frame_fields.argtypes = [vp, vp, C.POINTER(FrameParams), C.c_float, vp, vp, vp]
         lib.hvd_gpu_field_parity_vote.argtypes = [vp, vp, vp, C.POINTER(FrameParams), C.POINTER(C.c_int), vp]
         lib.hvd_gpu_make_carrier.argtypes = [vp, vp, vp, i, C.POINTER(Geometry), vp, vp]
@@ -428,15 +432,16 @@ class Engine:
                                                    _ptrs(outs[2]), _stream()), "verify_motion")
         return outs
 
-    def decode_window(self, planes, parities, is_pal, params: SolverParams, wparams: WindowParams, y_given=False):
+    def decode_window(self, planes, parities, is_pal, params: SolverParams, wparams: WindowParams, y_given=None):
         """hvd-core's DecodeFieldWindowWithInits on the device: planes with s,
-        an explicit carrier, chi (chi0 in / chi out) and y (Y0 in when
-        y_given / Y out).  Returns the diagnostics dict."""
+        an explicit carrier, chi (chi0 in / chi out) and y (Y0 in where
+        y_given[j] / Y out).  Returns the diagnostics dict."""
         n = len(planes)
         P = (Plane * n)(*planes)
         par = (C.c_int * n)(*[int(v) for v in parities])
+        yg = None if y_given is None else (C.c_int * n)(*[int(bool(v)) for v in y_given])
         d = WindowDiag()
-        self._check(self.lib.hvd_gpu_decode_window(self.ctx, P, n, par, int(is_pal), int(y_given), C.byref(params),
+        self._check(self.lib.hvd_gpu_decode_window(self.ctx, P, n, par, int(is_pal), yg, C.byref(params),
                                                    C.byref(wparams), C.byref(d), _stream()), "decode_window")
         return {k: float(getattr(d, k)) for k, _ in WindowDiag._fields_}
 
@@ -564,10 +569,12 @@ class Engine:
 
     # ---- frame bridge ---------------------------------------------------------
     def decode_frame_window_s16(self, frames, core_begin, core_end, fp: FrameParams, params: SolverParams,
-                                bridge: BridgeParams, want_gain=False):
+                                bridge: BridgeParams, want_gain=False, wparams: "WindowParams" = None,
+                                diag: dict = None):
         """frames: list of flat int16 device tensors (field-sequential frames).
         Returns a list of (luma, chroma, u, v) device tensors per core frame and
-        the applied chroma gain (None unless want_gain)."""
+        the applied chroma gain (None unless want_gain).  With wparams (or a
+        diag dict to fill) the _ex entry point runs with those window knobs."""
         import torch
         dev = frames[0].device
         n_px = fp.frame_width * fp.frame_height
@@ -579,6 +586,15 @@ class Engine:
             descs.append(YcFrame(*[t.data_ptr() for t in o]))
         arr = (YcFrame * max(len(descs), 1))(*descs)
         gain = C.c_float(0.0)
+        if wparams is not None or diag is not None:
+            d = WindowDiag()
+            self._check(self.lib.hvd_gpu_decode_frame_window_ex_s16(
+                self.ctx, _ptrs(frames), len(frames), core_begin, core_end, C.byref(fp), C.byref(params),
+                C.byref(bridge), C.byref(wparams) if wparams is not None else None, arr,
+                C.byref(gain) if want_gain else None, C.byref(d), _stream()), "decode_frame_window_ex_s16")
+            if diag is not None:
+                diag.update({k: float(getattr(d, k)) for k, _ in WindowDiag._fields_})
+            return outs, (gain.value if want_gain else None)
         self._check(self.lib.hvd_gpu_decode_frame_window_s16(
             self.ctx, _ptrs(frames), len(frames), core_begin, core_end, C.byref(fp), C.byref(params),
             C.byref(bridge), arr, C.byref(gain) if want_gain else None, _stream()), "decode_frame_window_s16")

@@ -279,8 +279,9 @@ int hvd_gpu_verify_motion(hvd_gpu_ctx* ctx, const float* const* y_ref, const flo
 
 /* hvd-core's window driver, DecodeFieldWindowWithInits
  * (engine/sequence.cpp:371-1114, sequence.h:104-135), over n fields of one
- * shape: fields[j].s, an explicit carrier, chi0 in fields[j].chi and, when
- * y_given, Y0 in fields[j].y (else Y0 = S - Re[chi0 c]); on return chi and
+ * shape: fields[j].s, an explicit carrier, chi0 in fields[j].chi and, where
+ * y_given[j] is non-zero, Y0 in fields[j].y (else Y0 = S - Re[chi0 c], as
+ * for an empty DecodedField::luma; y_given may be null); on return chi and
  * Y (S - Re[chi c] with output_fidelity, else the anchored luma) in the same
  * planes.  parities: n entries (0 top, 1 bottom).  is_pal selects the PAL
  * offsets and defaults (sequence.cpp:452-476) and the aniso grouping.  The
@@ -291,7 +292,7 @@ int hvd_gpu_verify_motion(hvd_gpu_ctx* ctx, const float* const* y_ref, const flo
  * fields of one colour of the Gauss-Seidel sweep are solved as one batch.
  * diag: optional HOST output.  Synchronous. */
 int hvd_gpu_decode_window(hvd_gpu_ctx* ctx, const hvd_gpu_plane* fields, int n, const int* parities, int is_pal,
-                          int y_given, const hvd_solver_params* p, const hvd_window_params* wp,
+                          const int* y_given, const hvd_solver_params* p, const hvd_window_params* wp,
                           hvd_window_diag* diag, void* stream);
 
 /* Device scratch of hvd_gpu_variational_refine_noalloc: 64 MiB per GPU, held
@@ -462,21 +463,33 @@ int hvd_gpu_decode_fields_ire(hvd_gpu_ctx* ctx, const float* const* field_ire, i
                               const hvd_solver_params* p, float chroma_phase_deg, float* const* y_out,
                               float* const* chi_out, float* const* theta_out, void* stream);
 
-/* DecodeFrameSequenceWindow, 2-D path (frame_bridge.cpp:722-947): frames[t]
+/* DecodeFrameSequenceWindow (frame_bridge.cpp:722-947): frames[t]
  * (t < nframes) are field-sequential int16 10-bit frames (device); the call
  * splits them into fields (SplitFrameFields :61-99, PAL line offsets
  * FrameLineOffset frame_bridge.h:106-112), measures the window ACC gain
- * (:776-795) and field order (:759-765), runs the field driver on all
- * 2 * nframes fields in one batch, and weaves and requantises the core
- * frames [core_begin, core_end) into out[0 .. core_end - core_begin)
- * (:867-945).  3-D decoding (temporal strength >= 0 with more fields than
- * the ambiguity stride, and cg_iterations > 0) is HVD_GPU_ERR_UNSUPPORTED.
- * `frames`, `out` are HOST arrays; acc_gain (HOST, optional) receives
- * chroma_gain x ACC and makes the call synchronous. */
+ * (:776-795) and field order (:759-765), decodes all 2 * nframes fields,
+ * and weaves and requantises the core frames [core_begin, core_end) into
+ * out[0 .. core_end - core_begin) (:867-945).  A 2-D window (temporal
+ * strength < 0, or 0 with no more fields than the ambiguity stride, or
+ * cg_iterations == 0) runs the field driver's batch; any other runs the
+ * window driver (hvd_gpu_decode_window) on the fields' holographic inits
+ * with HvdConfig's default window knobs (see the _ex variant).  `frames`,
+ * `out` are HOST arrays; acc_gain (HOST, optional) receives chroma_gain x
+ * ACC and makes the call synchronous. */
 int hvd_gpu_decode_frame_window_s16(hvd_gpu_ctx* ctx, const int16_t* const* frames, int nframes, int core_begin,
                                     int core_end, const hvd_frame_params* fp, const hvd_solver_params* p,
                                     const hvd_bridge_params* b, const hvd_yc_frame* out, float* acc_gain,
                                     void* stream);
+
+/* The same with the window driver's knobs (HvdConfig's 3-D fields; null:
+ * their defaults) and its decisions (diag, HOST, optional: zeros for a 2-D
+ * window).  The strength is b->temporal_strength; wp->temporal_strength is
+ * not read. */
+int hvd_gpu_decode_frame_window_ex_s16(hvd_gpu_ctx* ctx, const int16_t* const* frames, int nframes, int core_begin,
+                                       int core_end, const hvd_frame_params* fp, const hvd_solver_params* p,
+                                       const hvd_bridge_params* b, const hvd_window_params* wp,
+                                       const hvd_yc_frame* out, float* acc_gain, hvd_window_diag* diag,
+                                       void* stream);
 
 /* ---- motion ------------------------------------------------------------ */
 

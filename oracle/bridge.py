@@ -12,6 +12,8 @@ from __future__ import annotations
 
 import dataclasses
 
+import math
+
 import numpy as np
 
 
@@ -108,7 +110,19 @@ def decode_frame_sequence_window(orc, frames, core_begin, core_end, fp: FramePar
     """Returns (list of (luma, chroma, u, v) flat planes per core frame, gain,
     field 1 is bottom)."""
     g = field_geometry(fp, Geometry)
-    fw, fh, f1 = fp.frame_width, fp.frame_height, fp.field1_lines
+    pairs, f1_bottom, gain = _window_decisions(orc, frames, fp, g, chroma_gain, acc, field_order)
+    dec = []
+    for top, bot in pairs:
+        for fld in (top, bot):
+            r = orc.decode_field(fld, g, cfg, chroma_phase_deg)
+            fal = g.first_active_field_line
+            vs = r["vswitch"][fal:fal + g.lines] if g.uses_vswitch else None
+            car = orc.make_carrier(r["theta"][fal:fal + g.lines].astype(np.float32), vs, g)
+            dec.append((r["y"], r["chi"], car))
+    return _weave(frames, dec, core_begin, core_end, fp, g, gain, f1_bottom, monochrome), float(gain), bool(f1_bottom)
+
+
+def _window_decisions(orc, frames, fp, g, chroma_gain, acc, field_order):
     pairs = [split_frame_fields(fr, fp) for fr in frames]
     amps = []
     for top, bot in pairs:
@@ -122,15 +136,60 @@ def decode_frame_sequence_window(orc, frames, core_begin, core_end, fp: FramePar
     if acc:
         med = sorted(amps)[len(amps) // 2]
         gain = orc.acc_gain(med, 21.43 if g.standard == 1 else 20.0)
-    gain = np.float32(np.float32(chroma_gain) * np.float32(gain))
-    dec = []
+    return pairs, bool(f1_bottom), np.float32(np.float32(chroma_gain) * np.float32(gain))
+
+
+def decode_frame_sequence_window_3d(orc64, orc32, frames, core_begin, core_end, fp: FrameParams, wcfg, Geometry, *,
+                                    chroma_gain=1.0, acc=True, field_order=0, monochrome=False,
+                                    chroma_phase_deg=0.0, diag=None, make_carrier=None, holographic_init=None):
+    """DecodeFrameSequenceWindow (frame_bridge.cpp:722-947) with the window
+    driver: PrepareFieldObs per field (sequence.cpp:229-268; lock-in in
+    FP64, hvd-core's float MakeCarrier), parities from the measured field
+    order (:759-774), HolographicInit (FP64, stored as complex64) and the
+    FP32 window driver restatement (oracle/sequence3d.py, bit-exact with
+    hvd-core's own DecodeFieldWindowWithInits).  wcfg: a
+    sequence3d.WindowCfg.  make_carrier(theta, vswitch, g) replaces
+    hvd-core's MakeCarrier when given (a checker that feeds the oracle the
+    carrier planes of the implementation under test: the 3-D passes turn
+    sub-ulp differences of sinf/cosf into rare decision flips);
+    holographic_init(s, carrier, sample_rate) -> (Y0, chi0) likewise
+    replaces the oracle's FP64 init.  Returns (frames, gain, field 1 is
+    bottom)."""
+    import sequence3d as S3
+    g = field_geometry(fp, Geometry)
+    pairs, f1_bottom, gain = _window_decisions(orc64, frames, fp, g, chroma_gain, acc, field_order)
+    fal = g.first_active_field_line
+    fields, parities, inits, inits_y, cars = [], [], [], [], []
+    p0 = 1 if f1_bottom else 0
     for top, bot in pairs:
-        for fld in (top, bot):
-            r = orc.decode_field(fld, g, cfg, chroma_phase_deg)
-            fal = g.first_active_field_line
-            vs = r["vswitch"][fal:fal + g.lines] if g.uses_vswitch else None
-            car = orc.make_carrier(r["theta"][fal:fal + g.lines].astype(np.float32), vs, g)
-            dec.append((r["y"], r["chi"], car))
+        for k, fld in enumerate((top, bot)):
+            s, _, th, vs = orc64.prepare_field(fld, g)  # untrimmed theta: the trim below is hvd-core's float one
+            vs_act = vs[fal:fal + g.lines] if g.uses_vswitch else None
+            th_act = th[fal:fal + g.lines].astype(np.float32)
+            if chroma_phase_deg != 0.0:
+                # ApplyChromaPhase (ntsc_geometry.h:265-276), float adds
+                off = np.float32(-np.float32(chroma_phase_deg) * np.float32(math.pi) / np.float32(180.0))
+                sgn = np.where((vs_act if vs_act is not None else np.ones_like(th_act)) < 0, -off, off)
+                th_act = (th_act + sgn.astype(np.float32)).astype(np.float32)
+            car = (make_carrier or orc32.make_carrier)(th_act, vs_act, g).astype(np.complex64)
+            if holographic_init is not None:
+                y0, chi0 = holographic_init(s.astype(np.float32), car, g.sample_rate)
+            else:
+                y0, chi0 = orc64.holographic_init(s, car.astype(np.complex128), g.sample_rate)
+            fields.append((s.astype(np.float32), car))
+            inits.append(chi0.astype(np.complex64))
+            inits_y.append(y0.astype(np.float32))
+            parities.append(p0 if k == 0 else 1 - p0)
+            cars.append(car)
+    ys, chis = S3.decode_window_with_inits(orc32, fields, parities, inits, wcfg, is_pal=g.uses_vswitch,
+                                          inits_y=inits_y, diag=diag)
+    dec = [(y, c.astype(np.complex128), car.astype(np.complex128)) for y, c, car in zip(ys, chis, cars)]
+    return _weave(frames, dec, core_begin, core_end, fp, g, gain, f1_bottom, monochrome), float(gain), f1_bottom
+
+
+def _weave(frames, dec, core_begin, core_end, fp, g, gain, f1_bottom, monochrome):
+    """The core frames' weave and requantisation (frame_bridge.cpp:867-945)."""
+    fw, fh, f1 = fp.frame_width, fp.frame_height, fp.field1_lines
     scale = np.float32((fp.white_level - fp.blanking_level) / 100.0)
     p0 = 1 if f1_bottom else 0
     outs = []
@@ -163,7 +222,7 @@ def decode_frame_sequence_window(orc, frames, core_begin, core_end, fp: FramePar
             v[idx] = float(gain) * ch.real * float(scale)
             u[idx] = float(gain) * (-ch.imag) * float(scale)
         outs.append((luma, chroma, u, v))
-    return outs, float(gain), bool(f1_bottom)
+    return outs
 
 
 # ---- the stage's woven picture and RGB24 export ---------------------------

@@ -9,6 +9,14 @@
 //   api_driver frames <ntsc|pal> <nframes> <core_begin> <core_end> <in.s16> <out>
 //       DecodeFrameSequenceWindow; out: per core frame luma, chroma (s16),
 //       u, v (f64).
+//   api_driver frames3d <ntsc|pal> <nframes> <core_begin> <core_end> <strength> <in.s16> <out>
+//       DecodeFrameSequenceWindow with the 3-D driver (cg_iterations 20,
+//       fast mode, the given temporal strength); out as for frames.
+//   api_driver window3d <ntsc|pal> <h> <w> <nfields> <strength> <dir>
+//       DecodeFieldWindowWithInits on dir/s, dir/car, dir/chi0 (f32,
+//       complex interleaved, fields one after another) and dir/par (int32
+//       parities), cg_iterations 30; out dir/out: per field luma then chi,
+//       then the five SequenceDiagnostics floats.
 //   api_driver frame <ntsc|pal> <in.s16> <out>          DecodeFrameBuffer
 //   api_driver ycframe <ntsc|pal> <luma.s16> <chroma.s16> <out>  DecodeYcFrameBuffer
 //   api_driver refine3d <ntsc|pal> <h> <w> <nn> <nu> <eps_t> <dir>
@@ -167,6 +175,57 @@ int main(int argc, char** argv) {
     if (out.empty()) return Fail();
     Out o(argv[7]);
     for (const hvd::YcFrameS16& y : out) PutFrame(o, y);
+    return 0;
+  }
+  if (mode == "frames3d" && argc == 9) {
+    hvd::HvdConfig cfg;
+    cfg.cg_iterations = 20;
+    cfg.irls_outer = 4;
+    cfg.fast = true;
+    cfg.temporal_strength = (float)std::atof(argv[6]);
+    const int nframes = std::atoi(argv[3]);
+    const std::vector<int16_t> in = ReadAll<int16_t>(argv[7], frame_n * nframes);
+    std::vector<const int16_t*> frames;
+    for (int t = 0; t < nframes; ++t) frames.push_back(in.data() + frame_n * t);
+    const std::vector<hvd::YcFrameS16> out =
+        hvd::DecodeFrameSequenceWindow(frames, std::atoi(argv[4]), std::atoi(argv[5]), fp, cfg, engine);
+    if (out.empty()) return Fail();
+    Out o(argv[8]);
+    for (const hvd::YcFrameS16& y : out) PutFrame(o, y);
+    return 0;
+  }
+  if (mode == "window3d" && argc == 8) {
+    const int h = std::atoi(argv[3]), w = std::atoi(argv[4]), nf = std::atoi(argv[5]);
+    const size_t n = (size_t)h * w;
+    const std::string dir = argv[7];
+    const std::vector<float> s = ReadAll<float>((dir + "/s").c_str(), nf * n);
+    const std::vector<float> car = ReadAll<float>((dir + "/car").c_str(), 2 * nf * n);
+    const std::vector<float> chi0 = ReadAll<float>((dir + "/chi0").c_str(), 2 * nf * n);
+    const std::vector<int32_t> par = ReadAll<int32_t>((dir + "/par").c_str(), nf);
+    std::vector<hvd::FieldObs> obs(nf);
+    std::vector<hvd::DecodedField> inits(nf);
+    for (int j = 0; j < nf; ++j) {
+      obs[j].s = hvd::Plane(h, w);
+      std::memcpy(obs[j].s.data(), s.data() + j * n, sizeof(float) * n);
+      obs[j].carrier = hvd::ComplexPlane(h, w);
+      std::memcpy(reinterpret_cast<float*>(obs[j].carrier.data()), car.data() + 2 * j * n, sizeof(float) * 2 * n);
+      obs[j].parity = par[j];
+      inits[j].chroma = hvd::ComplexPlane(h, w);
+      std::memcpy(reinterpret_cast<float*>(inits[j].chroma.data()), chi0.data() + 2 * j * n, sizeof(float) * 2 * n);
+    }
+    hvd::HvdConfig cfg;
+    cfg.cg_iterations = 30;
+    cfg.temporal_strength = (float)std::atof(argv[6]);
+    hvd::SequenceDiagnostics d;
+    const std::vector<hvd::DecodedField> dec = hvd::DecodeFieldWindowWithInits(obs, inits, g, cfg, &d);
+    if (static_cast<int>(dec.size()) != nf) return Fail();
+    Out o((dir + "/out").c_str());
+    for (const hvd::DecodedField& f : dec) {
+      o.put(f.luma.data(), n);
+      o.put(reinterpret_cast<const float*>(f.chroma.data()), 2 * n);
+    }
+    const float dv[5] = {d.resolved_strength, d.ambiguity_ire, d.sigma_ire, d.temporal_eps, d.nr_eps};
+    o.put(dv, 5);
     return 0;
   }
   if (mode == "frame" && argc == 5) {

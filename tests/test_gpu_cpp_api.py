@@ -285,3 +285,62 @@ def test_motion_compensate_prev_and_coherence(tmp_path, orc32, fast):
     np.testing.assert_array_equal(ocar[..., 0] + 1j * ocar[..., 1], rcar.astype(np.complex64))
     np.testing.assert_array_equal(ocf, rcf.astype(np.float32))
     np.testing.assert_array_equal(g, orc32.complex_coherence(z1, z2, 3).astype(np.float32))
+
+
+@pytest.mark.parametrize("std,strength", [("ntsc", 1.0), ("pal", 0.8), ("ntsc", 0.0)])
+def test_decode_field_window_with_inits(tmp_path, orc32, std, strength):
+    """DecodeFieldWindowWithInits (sequence.h:139-142) through the C++ API vs
+    the oracle's window driver (bit-exact with hvd-core's own) on a moving
+    interlaced window: decisions equal, Y and chi within TOL_IRE."""
+    import sequence3d as S3
+    from sequence_cases import window
+    pal = std == "pal"
+    fields, parities, inits, _ = window(lines=48, width=128, nf=10 if pal else 8, motion=(0, 1), seed=17, pal=pal)
+    h, w = fields[0][0].shape
+    np.concatenate([s.reshape(-1) for s, _ in fields]).astype(np.float32).tofile(tmp_path / "s")
+    np.concatenate([np.stack([c.real, c.imag], -1).reshape(-1) for _, c in fields]).astype(
+        np.float32).tofile(tmp_path / "car")
+    np.concatenate([np.stack([c.real, c.imag], -1).reshape(-1) for c in inits]).astype(
+        np.float32).tofile(tmp_path / "chi0")
+    np.asarray(parities, np.int32).tofile(tmp_path / "par")
+    run(["window3d", std, h, w, len(fields), strength, tmp_path])
+    out = np.fromfile(tmp_path / "out", np.float32)
+    d_ref = {}
+    ry, rc = S3.decode_window_with_inits(orc32, fields, parities, inits,
+                                        S3.WindowCfg(cg_iterations=30, temporal_strength=strength), is_pal=pal,
+                                        diag=d_ref)
+    n = h * w
+    for j in range(len(fields)):
+        blk = out[j * 3 * n:(j + 1) * 3 * n]
+        assert np.abs(blk[:n].reshape(h, w) - ry[j]).max() <= TOL_IRE
+        c = blk[n:].reshape(h, w, 2)
+        assert np.abs(c[..., 0] + 1j * c[..., 1] - rc[j]).max() <= TOL_IRE
+    dv = out[len(fields) * 3 * n:]
+    for v, k in zip(dv, ("resolved_strength", "ambiguity_ire", "sigma_ire", "temporal_eps", "nr_eps")):
+        assert abs(v - d_ref[k]) <= 1e-4 * max(1.0, abs(d_ref[k])), k
+
+
+@pytest.mark.parametrize("std", ["ntsc", "pal"])
+def test_decode_frame_sequence_window_3d(tmp_path, orc64, orc32, std):
+    """DecodeFrameSequenceWindow with 3-D on (strength 1) through the C++ API
+    vs the oracle's restatement (FP64 lock-in and inits, FP32 window driver)
+    on the library's float carriers and inits (test_gpu_parity.test_frame_window_3d
+    gives the reason and the bar against the oracle's own)."""
+    import sequence3d as S3
+    from hvd_amd import gpu
+    from test_gpu_parity import gpu_carrier, gpu_init
+    fpd, fp_o, g = frame_geom(std)
+    frames = synth.frames_s16(fpd, 3, seed=97)
+    np.concatenate(frames).astype(np.int16).tofile(tmp_path / "in.s16")
+    run(["frames3d", std, 3, 1, 3, 1.0, tmp_path / "in.s16", tmp_path / "out.bin"])
+    got = _read_frames(tmp_path / "out.bin", fpd, 2)
+    eng = gpu.Engine(0)
+    want, gain, _ = OB.decode_frame_sequence_window_3d(
+        orc64, orc32, frames, 1, 3, fp_o, S3.WindowCfg(cg_iterations=20, temporal_strength=1.0), synth.Geometry,
+        make_carrier=gpu_carrier(eng), holographic_init=gpu_init(eng))
+    scale = (fpd["white_level"] - fpd["blanking_level"]) / 100.0
+    for (gl, gc, gu, gv), (wl, wc, wu, wv) in zip(got, want):
+        _codes_close(gl, wl)
+        _codes_close(gc, wc)
+        assert np.abs(gu - wu).max() <= TOL_IRE * scale * max(gain, 1.0)
+        assert np.abs(gv - wv).max() <= TOL_IRE * scale * max(gain, 1.0)
