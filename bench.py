@@ -74,6 +74,25 @@ def traffic_from_profile():
     return float(m.group(1)) if m else None
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_cores() -> int:
+    """Cores this process may use (its affinity mask), at most the 16 of the GPU's host share."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
 def cpu_baseline(torch, gpu, synth, eng, n_fields=6):
     """hvd-core's own BurstLockinPhase + VariationalRefine (compiled from the
     reference sources with OpenMP, oracle/_ref) on n config-0 fields.  The
@@ -131,10 +150,44 @@ def cpu_baseline(torch, gpu, synth, eng, n_fields=6):
         c = dchi.cpu().numpy()
         errs.append(max(float(np.abs(dy.cpu().numpy() - y_ref).max()),
                         float(np.abs(c[..., 0] + 1j * c[..., 1] - chi_ref).max())))
+    # mode (ii): parallel_across_fields (sequence.cpp:1079-1085), every field
+    # single-threaded, one field per core
+    across = cpu_baseline_across_fields(ref, g, fields, ss, chi0s, cfg)
     return {"value": round(n_fields / dt, 3), "unit": "fields/s", "cores": cores, "kind": "reference",
             "sample": f"{n_fields} NTSC 910x263 fields: hvd-core BurstLockinPhase + VariationalRefine "
                       f"(IRLS 4 x CG 25), OpenMP {cores} threads, one field at a time; init excluded (FFTW absent)",
-            "gpu_vs_reference_max_abs_err_ire": float(max(errs))}
+            "gpu_vs_reference_max_abs_err_ire": float(max(errs)),
+            "across_fields": across, "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "affinity_cores": host_cores()}
+
+
+def cpu_baseline_across_fields(ref, g, fields, ss, chi0s, cfg):
+    """hvd-core's second CPU mode: fields decoded concurrently, each on one
+    thread (OpenMP teams of one), as many at once as the host share has
+    cores; the same fields repeated to fill them."""
+    import ctypes
+    from concurrent.futures import ThreadPoolExecutor
+    try:
+        gomp = ctypes.CDLL("libgomp.so.1")
+    except OSError:
+        return None
+    workers = host_cores()
+    fal = g.first_active_field_line
+    jobs = [(fields[k % len(fields)], ss[k % len(ss)], chi0s[k % len(chi0s)]) for k in range(workers)]
+
+    def one(job):
+        gomp.omp_set_num_threads(1)
+        f, s, chi0 = job
+        th = ref.burst_lockin_phase(f, g)
+        car = ref.make_carrier(th[fal:fal + g.lines], None, g)
+        ref.variational_refine(s, car, chi0, cfg)
+
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        t0 = time.perf_counter()
+        list(ex.map(one, jobs))
+        dt = time.perf_counter() - t0
+    return {"value": round(len(jobs) / dt, 3), "unit": "fields/s", "cores": workers, "kind": "reference",
+            "sample": f"{len(jobs)} NTSC fields, {workers} at once, each single-threaded (parallel_across_fields)"}
 
 
 def training_bench(torch, gpu, synth, eng, steps=200, n_fields=16):
@@ -244,8 +297,11 @@ def streaming_bench(torch, gpu, synth, eng, n_windows=64, window=8, unique=16):
 def temporal_bench(torch, gpu, eng, reps=10):
     """VariationalRefine with two temporal neighbours (the 3-D solve) on an
     NTSC field's active grid, 263 x 758, the headline's solver settings
-    (4 IRLS stages x 25 CG iterations, every iteration run).  Fields/s and
-    the time of one CG iteration (six launches)."""
+    (4 IRLS stages x 25 CG iterations, every iteration run).  Fields/s; the
+    CG iterations' own time from HIP events around each stage's dir + step
+    launches (the rest of the call, per-stage weights and the fixed set-up,
+    reported apart); their algorithmic bytes: the 2-D 64 B per pixel plus,
+    per neighbour, its composite, carrier and temporal weight read (16 B)."""
     h, w, nn = 263, 758, 2
     r, c = np.mgrid[0:h, 0:w].astype(np.float64)
     car = np.exp(1j * (0.5 * np.pi * c + 0.3 * r))
@@ -275,13 +331,109 @@ def temporal_bench(torch, gpu, eng, reps=10):
 
     iters = int(one(True).sum())  # iterations the solve actually runs (it is deterministic)
     torch.cuda.synchronize()
+    eng.set_profiling(True)
     t0 = time.perf_counter()
     for _ in range(reps):
         one()
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / reps
+    cg_ms, launched, _ = eng.cg_timing()
+    eng.set_profiling(False)
+    us_it = 1e3 * cg_ms / max(launched, 1)
+    bytes_it = (CG_BYTES_PER_PX_ITER + 16 * nn) * h * w
     return {"fields_per_s": round(1.0 / dt, 1), "ms_per_field": round(1e3 * dt, 3),
-            "us_per_cg_iteration": round(1e6 * dt / max(iters, 1), 2), "neighbours": nn, "cg_iterations": iters}
+            "us_per_cg_iteration": round(us_it, 2),
+            "ms_per_field_outside_cg_iterations": round(1e3 * dt - cg_ms / reps, 3),
+            "cg_algorithmic_gbs": round(bytes_it / (us_it * 1e-6) / 1e9, 1),
+            "neighbours": nn, "cg_iterations": iters}
+
+
+WINDOW_FRAMES, WINDOW_CORE = 8, (1, 7)  # HvdConfig chunk_frames 6 + chunk_overlap 1 on each side
+
+
+def window3d_frames(synth, seed=61):
+    """A window of 8 NTSC frames of one scene (field-sequential int16 10-bit,
+    0.5 IRE noise).  The block matching searches every candidate whatever the
+    content, so a static scene costs what a moving one does."""
+    fpd = synth.FRAME_NTSC
+    return fpd, synth.frames_s16(fpd, WINDOW_FRAMES, seed=seed)
+
+
+def window3d_bench(torch, gpu, synth, eng, reps=3):
+    """DecodeFrameSequenceWindow with the 3-D window driver on 8 NTSC frames
+    (16 fields; HvdConfig's chunk of 6 core frames plus one overlap frame on
+    each side), temporal strength 1: noise calibration, two Gauss-Seidel
+    passes in fast mode (pair motion, predicted + verified long offsets,
+    trajectory snap, motion-compensated neighbours, coherence and odd-offset
+    gates, SynthReference anchor + joint solve from pass 1), output-fidelity
+    luma, weave and requantise.  HvdConfig's solver defaults (cg_iterations
+    2) and cg_iterations 40.  Core frames/s per call, the frames already in
+    HBM."""
+    fpd, frames = window3d_frames(synth)
+    dfr = [torch.from_numpy(f).to("cuda:0") for f in frames]
+    fp = gpu.FrameParams(**{k: v for k, v in fpd.items() if k != "extra_sample_lines"})
+    out = {}
+    for cg in (2, 40):
+        sp = gpu.SolverParams.make(cg_iterations=cg)
+        bp = gpu.BridgeParams.make(temporal_strength=1.0)
+        wp = gpu.WindowParams.make()
+        d = {}
+        eng.decode_frame_window_s16(dfr, WINDOW_CORE[0], WINDOW_CORE[1], fp, sp, bp, wparams=wp, diag=d)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            eng.decode_frame_window_s16(dfr, WINDOW_CORE[0], WINDOW_CORE[1], fp, sp, bp, wparams=wp)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        ncore = WINDOW_CORE[1] - WINDOW_CORE[0]
+        out[f"cg{cg}"] = {"core_frames_per_s": round(ncore / dt, 2), "window_fields_per_s": round(2 * WINDOW_FRAMES / dt, 2),
+                          "ms_per_window": round(1e3 * dt, 2), "resolved_strength": round(d["resolved_strength"], 3),
+                          "sigma_ire": round(d["sigma_ire"], 3)}
+    return out
+
+
+def window3d_cpu_baseline(torch, gpu, synth, eng):
+    """hvd-core's own DecodeFieldWindowWithInits (oracle/_ref, OpenMP) on the
+    same 16 fields with HvdConfig's defaults (cg_iterations 2), temporal
+    strength 1.  HolographicInit needs FFTW, absent here: the inits come
+    from the library (hvd_gpu_holographic_init on hvd-core's float carriers)
+    and the CPU time is the driver's alone."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+    import bridge as OB
+    if not oracle.REFLIB_OMP.exists():
+        return None
+    ref = oracle.Reference(openmp=True)
+    fpd, frames = window3d_frames(synth)
+    fp_o = OB.FrameParams(**fpd)
+    g = OB.field_geometry(fp_o, synth.Geometry)
+    fal = g.first_active_field_line
+    fields, inits, ys, par = [], [], [], []
+    for fr in frames:
+        for k, fld in enumerate(OB.split_frame_fields(fr, fp_o)):
+            th = ref.burst_lockin_phase(fld, g)
+            car = ref.make_carrier(th[fal:fal + g.lines], None, g).astype(np.complex64)
+            s = np.ascontiguousarray(fld[fal:fal + g.lines, g.active_video_start:g.active_video_end], np.float32)
+            ds = torch.from_numpy(s).to("cuda:0")
+            dchi = torch.zeros((g.lines, g.width, 2), device="cuda:0")
+            dy = torch.zeros_like(ds)
+            dcar = torch.from_numpy(np.stack([car.real, car.imag], -1).astype(np.float32)).to("cuda:0")
+            eng.holographic_init([eng.plane(ds, dchi, dy, carrier=dcar, sample_rate=g.sample_rate)])
+            c = dchi.cpu().numpy()
+            fields.append((s, car))
+            inits.append((c[..., 0] + 1j * c[..., 1]).astype(np.complex64))
+            ys.append(dy.cpu().numpy())
+            par.append(k)
+    cfg = oracle.SeqCfg.make(cg_iterations=2, temporal_strength=1.0)
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    ref.decode_window_with_inits(fields, par, inits, cfg, inits_y=ys)
+    dt = time.perf_counter() - t0
+    ncore = WINDOW_CORE[1] - WINDOW_CORE[0]
+    return {"value": round(ncore / dt, 3), "unit": "core frames/s", "cores": cores, "kind": "reference",
+            "sample": f"one window of {len(fields)} NTSC fields: hvd-core DecodeFieldWindowWithInits "
+                      f"(HvdConfig defaults, strength 1), OpenMP {cores} threads; inits and lock-in excluded",
+            "s_per_window": round(dt, 3)}
 
 
 def main():
@@ -427,6 +579,9 @@ def main():
         result["training_16ntsc_adam200"] = training_bench(torch, gpu, synth, eng)
         log(f"[bench] training: {result['training_16ntsc_adam200']}")
     if not args.only_batched:
+        log("[bench] 3-D window: DecodeFrameSequenceWindow, 8 NTSC frames")
+        result["frame_window_3d_ntsc8"] = window3d_bench(torch, gpu, synth, eng)
+        log(f"[bench] 3-D window: {result['frame_window_3d_ntsc8']}")
         log("[bench] temporal: NTSC-sized field with 2 temporal neighbours")
         result["temporal_2nbr_ntsc758x263"] = temporal_bench(torch, gpu, eng)
         log(f"[bench] temporal: {result['temporal_2nbr_ntsc758x263']}")
@@ -434,6 +589,9 @@ def main():
         log("[bench] CPU baseline (hvd-core reference build)")
         cb = cpu_baseline(torch, gpu, synth, eng)
         result["cpu_baseline"] = cb
+        if not args.only_batched:
+            log("[bench] CPU baseline of the 3-D window (hvd-core reference build)")
+            result["frame_window_3d_ntsc8"]["cpu_baseline"] = window3d_cpu_baseline(torch, gpu, synth, eng)
         if cb:
             log(f"[bench] CPU: {cb['value']} fields/s on {cb['cores']} threads; GPU vs hvd-core "
                 f"max |d| {cb['gpu_vs_reference_max_abs_err_ire']:.2e} IRE")

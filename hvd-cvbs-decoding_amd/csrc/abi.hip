@@ -1689,7 +1689,7 @@ void refine_pixel(hvd_gpu_ctx* ctx, const std::vector<DevField>& F,
   const int* dsp = (const int*)(base + oS);
   float* gx = (float*)(ws + L.gx);
   float* gy = (float*)(ws + L.gy);
-  const dim3 grid(L.cp[nf]), blk(64);
+  const dim3 grid(L.cp[nf]), blk(64 * kPxWaves);
   LaunchList Lq;
   Lq.kernel((const void*)k_refine_begin, dim3((nf + 63) / 64), dim3(64), 0, dstate, (const FieldCG*)(base + oI), nf);
   if (!(p.chroma_aniso > 0.0) && L.sp[nf] > 0) {
@@ -2678,11 +2678,7 @@ int hvd_gpu_estimate_motion(hvd_gpu_ctx* ctx, const float* const* y_ref, const f
     const int* dr0 = (const int*)(base + oR0);
     const int nrc = (int)rc.size(), nr3 = (int)r3.size();
     // (1) BoxBlur r = 2 of both planes (:398-399)
-    hipLaunchKernelGGL(k_me_blur_cols, dim3(blocks_for((size_t)n * 2 * w)), dim3(256), 0, st,
-                       (const MotionDesc*)(base + oD), n, h, w, 2, tmp);
-    LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_me_blur_rows, dim3(blocks_for((size_t)n * 2 * h)), dim3(256), 0, st, n, h, w, 2,
-                       (const float*)tmp, P);
+    launch_me_blur(st, (const MotionDesc*)(base + oD), n, h, w, 2, tmp, P);
     LAUNCH_CHECK(ctx);
     // (2) 4x-decimated exhaustive coarse search (:405-414)
     hipLaunchKernelGGL(k_me_decimate, dim3(blocks_for((size_t)n * 2 * hh * ww)), dim3(256), 0, st, n, h, w, f,
@@ -2792,11 +2788,7 @@ int hvd_gpu_verify_motion(hvd_gpu_ctx* ctx, const float* const* y_ref, const flo
     const int* dr0 = (const int*)(base + oR0);
     // motion.cpp:490-525: both planes blurred r = 2; the rounded prediction
     // (k_me_ssd rounds a relative base) and the zero vector, one SSD each
-    hipLaunchKernelGGL(k_me_blur_cols, dim3(blocks_for((size_t)n * 2 * w)), dim3(256), 0, st,
-                       (const MotionDesc*)(base + oD), n, h, w, 2, tmp);
-    LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_me_blur_rows, dim3(blocks_for((size_t)n * 2 * h)), dim3(256), 0, st, n, h, w, 2,
-                       (const float*)tmp, P);
+    launch_me_blur(st, (const MotionDesc*)(base + oD), n, h, w, 2, tmp, P);
     LAUNCH_CHECK(ctx);
     hipLaunchKernelGGL(k_me_ssd, dim3(blocks_for((size_t)n * nt)), dim3(256), 0, st, n, h, w, tile, (const float*)P,
                        dr0, dr0, 1, 1, (const float*)py, (const float*)px, se);
@@ -2906,7 +2898,8 @@ int hvd_gpu_tridiag_smooth(hvd_gpu_ctx* ctx, const float* d, const float* a, int
 int hvd_gpu_box_blur(hvd_gpu_ctx* ctx, const float* in, int h, int w, int r, float* out, void* stream) {
   return guarded(ctx, [&] {
     REQUIRE(ctx, in && out, "null argument");
-    REQUIRE(ctx, h >= 1 && w >= 1 && r >= 0, "bad blur %d x %d, radius %d", h, w, r);
+    REQUIRE(ctx, h >= 1 && w >= 1 && r >= 0 && r <= kBlurMaxR, "bad blur %d x %d, radius %d (at most %d)", h, w, r,
+            kBlurMaxR);
     hipStream_t st = S(stream);
     const size_t np = (size_t)h * w;
     Carve cv;
@@ -2917,11 +2910,7 @@ int hvd_gpu_box_blur(hvd_gpu_ctx* ctx, const float* in, int h, int w, int r, flo
     const size_t oD = stg.put(&md, 1);
     Slot* sl;
     char* base = upload(ctx, stg, st, &sl);
-    hipLaunchKernelGGL(k_me_blur_cols, dim3((2 * w + 255) / 256), dim3(256), 0, st, (const MotionDesc*)(base + oD), 1,
-                       h, w, r, (float*)(ws + o_tmp));
-    LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_me_blur_rows, dim3((2 * h + 255) / 256), dim3(256), 0, st, 1, h, w, r,
-                       (const float*)(ws + o_tmp), (float*)(ws + o_out));
+    launch_me_blur(st, (const MotionDesc*)(base + oD), 1, h, w, r, (float*)(ws + o_tmp), (float*)(ws + o_out));
     LAUNCH_CHECK(ctx);
     arm(ctx, sl, st);
     HIPCHK(ctx, hipMemcpyAsync(out, ws + o_out, sizeof(float) * np, hipMemcpyDeviceToDevice, st));
@@ -2957,11 +2946,7 @@ void blur_planes(hvd_gpu_ctx* ctx, const std::vector<MotionDesc>& pairs, int h, 
   Slot* sl;
   char* base = upload(ctx, stg, st, &sl);
   const int np = (int)pairs.size();
-  hipLaunchKernelGGL(k_me_blur_cols, dim3(blocks_for((size_t)np * 2 * w)), dim3(256), 0, st,
-                     (const MotionDesc*)(base + oD), np, h, w, r, tmp);
-  LAUNCH_CHECK(ctx);
-  hipLaunchKernelGGL(k_me_blur_rows, dim3(blocks_for((size_t)np * 2 * h)), dim3(256), 0, st, np, h, w, r,
-                     (const float*)tmp, out);
+  launch_me_blur(st, (const MotionDesc*)(base + oD), np, h, w, r, tmp, out);
   LAUNCH_CHECK(ctx);
   arm(ctx, sl, st);
 }
@@ -3049,7 +3034,8 @@ int hvd_gpu_complex_coherence(hvd_gpu_ctx* ctx, const float* z1, const float* z2
                               void* stream) {
   return guarded(ctx, [&] {
     REQUIRE(ctx, z1 && z2 && out, "null argument");
-    REQUIRE(ctx, h >= 1 && w >= 1 && r >= 0, "bad coherence %d x %d, radius %d", h, w, r);
+    REQUIRE(ctx, h >= 1 && w >= 1 && r >= 0 && r <= kBlurMaxR, "bad coherence %d x %d, radius %d (at most %d)", h, w,
+            r, kBlurMaxR);
     hipStream_t st = S(stream);
     const size_t n = (size_t)h * w;
     Carve cv;

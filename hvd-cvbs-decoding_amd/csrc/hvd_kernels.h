@@ -154,8 +154,9 @@ __global__ void k_refine_finalize(const DevField* F, const FieldCG* st, StripGri
 // diag_prior > 0 or NeighborTerm, variational.h:54-58), fields of a batch
 // each with their own neighbours.
 constexpr int kMaxNeighbors = 16;
-constexpr int kPxPerLane = 8;
-constexpr int kPxChunk = 64 * kPxPerLane;  // pixels per wave
+constexpr int kPxWaves = 4;                            // waves per block
+constexpr int kPxPerLane = 2;                          // pixels per lane
+constexpr int kPxChunk = 64 * kPxWaves * kPxPerLane;  // pixels per block
 
 // Per-field CG state of the per-pixel solve beside its FieldCG.
 struct PxState {
@@ -272,8 +273,17 @@ __global__ void k_zero_d(double* a, int n);
 
 // motion.hip
 __global__ void k_sad_bm(const SadDesc* D, const int* blk_prefix, int nf);
+// The box blur's LDS chunks: kBlurC rows (columns) per chunk plus the
+// window's reach on either side, radius at most kBlurMaxR (row stride kBlurS,
+// odd, >= kBlurC + 2 kBlurMaxR + 1).
+constexpr int kBlurC = 64, kBlurMaxR = 32, kBlurS = 131;
+// Rows up to kBlurMaxW samples take the along-x pass's parallel prefix sum.
+constexpr int kBlurMaxW = 2048;
 __global__ void k_me_blur_cols(const MotionDesc* D, int npairs, int h, int w, int r, float* tmp);
 __global__ void k_me_blur_rows(int npairs, int h, int w, int r, const float* tmp, float* out);
+// Both passes of BoxBlur (motion.cpp:21-55) over npairs * 2 planes (D), the
+// along-y pass into tmp, the result into out (planes one after another).
+void launch_me_blur(hipStream_t st, const MotionDesc* D, int npairs, int h, int w, int r, float* tmp, float* out);
 __global__ void k_me_decimate(int npairs, int h, int w, int f, const float* in, float* out);
 __global__ void k_me_ssd(int npairs, int h, int w, int tile, const float* planes, const int* dys, const int* dxs,
                          int ndy, int ndx, const float* base_dy, const float* base_dx, float* se);

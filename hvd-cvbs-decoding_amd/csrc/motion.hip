@@ -106,42 +106,155 @@ __device__ inline const float* mplane(const MotionDesc& d, int which) {
 
 }  // namespace
 
-// Box blur pass 1 (motion.cpp:21-36): along y for every column, FP64 prefix.
-// Thread per (pair, plane, column).
-__global__ void k_me_blur_cols(const MotionDesc* D, int npairs, int h, int w, int r, float* tmp) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= npairs * 2 * w) return;
-  const int x = idx % w, pl = (idx / w) % 2, pi = idx / (2 * w);
-  const float* a = mplane(D[pi], pl);
-  float* o = tmp + ((size_t)pi * 2 + pl) * h * w;
+// Box blur pass 1 (motion.cpp:21-36, BoxBlur1DRows): along y for every
+// column, FP64 running sums c_hi (the window's far edge) and c_lo (its near
+// edge) added in hvd-core's order, out = (c_hi - c_lo) * (1 / (2r + 1)).
+// Block: 64 consecutive columns of one plane, one thread per column; the
+// rows come through LDS in chunks of kBlurC (plus the window's reach on
+// either side) so a chunk's loads are all in flight at once.
+// Grid: (ceil(w / 64), npairs * 2).
+__global__ void __launch_bounds__(64) k_me_blur_cols(const MotionDesc* D, int npairs, int h, int w, int r,
+                                                     float* tmp) {
+  __shared__ float sm[kBlurS][64];
+  const int t = threadIdx.x, x = blockIdx.x * 64 + t, pp = blockIdx.y;
+  const float* a = mplane(D[pp >> 1], pp & 1);
+  float* o = tmp + (size_t)pp * h * w;
   const float norm = 1.0f / (float)(2 * r + 1);
-  // prefix sums in FP64 over a sliding window of the column
-  double c_lo = 0.0, c_hi = 0.0;  // c[y0], c[y1]
-  int i_lo = 0, i_hi = 0;
-  for (int y = 0; y < h; ++y) {
-    const int y0 = max(y - r, 0), y1 = min(y + r + 1, h);
-    while (i_hi < y1) { c_hi += (double)a[(size_t)i_hi * w + x]; ++i_hi; }
-    while (i_lo < y0) { c_lo += (double)a[(size_t)i_lo * w + x]; ++i_lo; }
-    o[(size_t)y * w + x] = (float)((c_hi - c_lo) * (double)norm);
+  const int L = r + 1, SW = kBlurC + 2 * r + 1;
+  double c_lo = 0.0, c_hi = 0.0;
+  for (int y0 = 0; y0 < h; y0 += kBlurC) {
+    __syncthreads();
+    // the chunk's loads in batches of 16, all in flight before their stores
+    for (int k0 = 0; k0 < SW; k0 += 16) {
+      float v[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int y = y0 - L + k0 + u;
+        v[u] = (k0 + u < SW && x < w && y >= 0 && y < h) ? a[(size_t)y * w + x] : 0.0f;
+      }
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+        if (k0 + u < SW) sm[k0 + u][t] = v[u];
+    }
+    __syncthreads();
+    if (x >= w) continue;
+    const int yend = min(y0 + kBlurC, h);
+    int y = y0;
+    if (y == 0) {
+      const int y1 = min(r + 1, h);
+      for (int i = 0; i < y1; ++i) c_hi += (double)sm[i + L][t];
+      o[x] = (float)((c_hi - c_lo) * (double)norm);
+      ++y;
+    }
+    // rows beyond the plane hold 0 in LDS: adding it leaves the sums (never
+    // -0) as they are
+#pragma unroll 8
+    for (; y < yend; ++y) {
+      c_hi += (double)sm[y + r - y0 + L][t];
+      c_lo += (double)sm[y - r - 1 - y0 + L][t];
+      o[(size_t)y * w + x] = (float)((c_hi - c_lo) * (double)norm);
+    }
   }
 }
 
-// Box blur pass 2 (motion.cpp:38-55): along x for every row.
-__global__ void k_me_blur_rows(int npairs, int h, int w, int r, const float* tmp, float* out) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= npairs * 2 * h) return;
-  const int y = idx % h, pp = idx / h;  // pp = pair * 2 + plane
-  const float* a = tmp + (size_t)pp * h * w + (size_t)y * w;
-  float* o = out + (size_t)pp * h * w + (size_t)y * w;
+// Box blur pass 2 (motion.cpp:38-55, BoxBlur1DCols): along x for every row,
+// out[x] = (c[x1] - c[x0]) * (1 / (2r + 1)) with c the FP64 running sum of
+// the row.  One wave per row.
+//
+// hvd-core adds the row left to right.  When no partial sum of the row can
+// round, any order of the adds gives the same c, and the wave forms c as a
+// parallel prefix sum.  That holds when every input is a multiple of 2^e
+// (e the exponent of the smallest ulp among the row's nonzero floats) and
+// w max|a| < 2^(53 + e): every partial sum, of any subset, is then an
+// integer multiple of 2^e below 2^53 of them, which a double holds exactly.
+// The wave checks this on the row first; a row that fails it (a nonzero
+// sample below about 2^-12 of the row's largest, in practice) is summed left
+// to right by lane 0, as hvd-core does, from a copy of the row in LDS.
+// Grid: rows = npairs * 2 * h blocks.
+__global__ void __launch_bounds__(64) k_me_blur_rows(int npairs, int h, int w, int r, const float* tmp, float* out) {
+  __shared__ double pc[kBlurMaxW + 1];
+  const int lane = threadIdx.x;
+  const size_t row = blockIdx.x;
+  if (row >= (size_t)npairs * 2 * h) return;
+  const float* a = tmp + row * w;
+  float* o = out + row * w;
   const float norm = 1.0f / (float)(2 * r + 1);
-  double c_lo = 0.0, c_hi = 0.0;
-  int i_lo = 0, i_hi = 0;
-  for (int x = 0; x < w; ++x) {
-    const int x0 = max(x - r, 0), x1 = min(x + r + 1, w);
-    while (i_hi < x1) { c_hi += (double)a[i_hi]; ++i_hi; }
-    while (i_lo < x0) { c_lo += (double)a[i_lo]; ++i_lo; }
-    o[x] = (float)((c_hi - c_lo) * (double)norm);
+  float mx = 0.0f;
+  int emin = 1 << 20;
+  for (int x = lane; x < w; x += 64) {
+    const float v = a[x];
+    mx = fmaxf(mx, fabsf(v));
+    if (v != 0.0f) emin = min(emin, max((int)((__float_as_uint(v) >> 23) & 255u), 1) - 150);
   }
+  for (int off = 32; off > 0; off >>= 1) {
+    mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+    emin = min(emin, __shfl_xor(emin, off, 64));
+  }
+  const bool exact = w <= kBlurMaxW && (emin == (1 << 20) || (double)w * (double)mx < ldexp(1.0, 53 + emin));
+  if (exact) {
+    // pc[k] = a[0] + ... + a[k - 1]: lanes take 8 consecutive samples, a
+    // scan of the lanes' sums, a carry between chunks of 512
+    double carry = 0.0;
+    if (lane == 0) pc[0] = 0.0;
+    for (int c0 = 0; c0 < w; c0 += 512) {
+      double loc[8], sacc = 0.0;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int x = c0 + 8 * lane + k;
+        sacc += x < w ? (double)a[x] : 0.0;
+        loc[k] = sacc;
+      }
+      double incl = sacc;
+      for (int off = 1; off < 64; off <<= 1) {
+        const double y = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += y;
+      }
+      const double base = carry + (incl - sacc);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int x = c0 + 8 * lane + k;
+        if (x < w) pc[x + 1] = base + loc[k];
+      }
+      carry += __shfl(incl, 63, 64);
+    }
+    __syncthreads();
+    for (int x = lane; x < w; x += 64) {
+      const int x1 = min(x + r + 1, w), x0 = max(x - r, 0);
+      o[x] = (float)((pc[x1] - pc[x0]) * (double)norm);
+    }
+  } else if (w <= kBlurMaxW) {
+    float* sa = reinterpret_cast<float*>(pc);  // the row, then its result
+    float* so = sa + kBlurMaxW;
+    for (int x = lane; x < w; x += 64) sa[x] = a[x];
+    __syncthreads();
+    if (lane == 0) {
+      double c_lo = 0.0, c_hi = 0.0;
+      for (int i = 0; i < min(r + 1, w); ++i) c_hi += (double)sa[i];
+      so[0] = (float)((c_hi - c_lo) * (double)norm);
+      for (int x = 1; x < w; ++x) {
+        if (x + r < w) c_hi += (double)sa[x + r];
+        if (x - r - 1 >= 0) c_lo += (double)sa[x - r - 1];
+        so[x] = (float)((c_hi - c_lo) * (double)norm);
+      }
+    }
+    __syncthreads();
+    for (int x = lane; x < w; x += 64) o[x] = so[x];
+  } else if (lane == 0) {
+    double c_lo = 0.0, c_hi = 0.0;
+    int i_lo = 0, i_hi = 0;
+    for (int x = 0; x < w; ++x) {
+      const int x0 = max(x - r, 0), x1 = min(x + r + 1, w);
+      while (i_hi < x1) c_hi += (double)a[i_hi++];
+      while (i_lo < x0) c_lo += (double)a[i_lo++];
+      o[x] = (float)((c_hi - c_lo) * (double)norm);
+    }
+  }
+}
+
+void launch_me_blur(hipStream_t st, const MotionDesc* D, int npairs, int h, int w, int r, float* tmp, float* out) {
+  hipLaunchKernelGGL(k_me_blur_cols, dim3((w + 63) / 64, npairs * 2), dim3(64), 0, st, D, npairs, h, w, r, tmp);
+  const size_t rows = (size_t)npairs * 2 * h;
+  hipLaunchKernelGGL(k_me_blur_rows, dim3((unsigned)rows), dim3(64), 0, st, npairs, h, w, r, (const float*)tmp, out);
 }
 
 // f x f block average, remainder cropped (motion.cpp:59-78).

@@ -6,23 +6,25 @@
 // weights, prior scale), for a batch of fields of uneven size, each with its
 // own neighbours.
 //
-// Work split: one 64-lane wave per chunk of kPxChunk consecutive pixels of one
-// field (lane l takes pixels base + l + 64 k, so every load of a step is
-// coalesced); the stencil's neighbours are gathered from the L2.  Two
-// launches per CG iteration:
+// Work split: one block of kPxWaves waves per chunk of kPxChunk consecutive
+// pixels of one field, wave w taking the chunk's w-th quarter (lane l takes
+// pixels base + l + 64 k, so every load of a step is coalesced); the
+// stencil's neighbours are gathered from the L2.  Two launches per CG
+// iteration:
 //   k_px_dir  : beta and the tolerance exits from the previous step's |g|^2
 //               partials; d = -g + beta d_prev; partials of curv(d), Re<g, d>;
 //   k_px_step : alpha (or the curvature exit) from those partials; chi' =
 //               chi + alpha d formed on the fly at the 3x3 stencil; chi' and
 //               g = grad(chi') written; partials of |g|^2.
-// Every wave of a field sums the field's chunk partials of the previous
-// launch in one fixed order (lane l adds chunks l, l + 64, ..., then a
-// butterfly, which leaves the same bits in every lane), so all waves take the
-// same alpha, beta and exits.  A launch never reads state that the same
-// launch writes: the stop flag is double-buffered (k_px_dir reads done[0] and
-// writes done[1], k_px_step the reverse), gg0 is written by the first
-// iteration's k_px_dir and read from the second on, and the |g|^2 partials
-// alternate between two slots by iteration parity.
+// A launch's sums (|g|^2, the curvature, <g, d>) go out as one partial per
+// block (its waves' sums added in wave order); the next launch's blocks each
+// add the field's partials in one fixed order (wave 0, batched loads, the
+// result shared through LDS), so every block of a field takes the same
+// alpha, beta and exits, whatever the batch.  A launch never reads state
+// that the same launch writes: the stop flag is double-buffered (k_px_dir
+// reads done[0] and writes done[1], k_px_step the reverse), gg0 is written by
+// the first iteration's k_px_dir and read from the second on, and the |g|^2
+// partials alternate between two slots by iteration parity.
 #include "hvd_device.h"
 #include "hvd_kernels.h"
 
@@ -50,11 +52,63 @@ __device__ inline double px_wave_sum(double v) {
   return v;
 }
 
-// Sum of part[0 .. n) in the fixed order every wave of the field uses.
-__device__ inline double px_sum(const double* part, int n) {
-  double v = 0.0;
-  for (int i = px_lane(); i < n; i += 64) v += part[i];
-  return px_wave_sum(v);
+// Pixel k of this lane: wave w of the block takes the block's w-th quarter.
+__device__ inline size_t px_index(const PxCtx& c, int k) {
+  return (size_t)c.chunk * kPxChunk + (size_t)(threadIdx.x >> 6) * (64 * kPxPerLane) + (size_t)k * 64 + px_lane();
+}
+
+// The block's share of one or two field sums: the waves' sums added in wave
+// order into part[slot][chunk].  Called by every thread of the block, on
+// uniform control flow.
+__device__ void px_store(const PxField& P, int chunk, int nv, double v0, double v1, int s0, int s1) {
+  __shared__ double red[kPxWaves][2];
+  const int wv = threadIdx.x >> 6;
+  v0 = px_wave_sum(v0);
+  v1 = px_wave_sum(v1);
+  if (px_lane() == 0) {
+    red[wv][0] = v0;
+    red[wv][1] = v1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b0 = red[0][0], b1 = red[0][1];
+    for (int k = 1; k < kPxWaves; ++k) {
+      b0 += red[k][0];
+      b1 += red[k][1];
+    }
+    const int nc = P.nchunks;
+    P.part[(size_t)s0 * nc + chunk] = b0;
+    if (nv > 1) P.part[(size_t)s1 * nc + chunk] = b1;
+  }
+}
+
+// The field totals of one or two partial slots an earlier launch stored:
+// wave 0 adds the chunk partials in one fixed order (lane l adds l, l + 64,
+// ..., eight loads in flight at a time, then the butterfly) and hands the
+// totals to the block through LDS, so every block of the field gets the same
+// bits.  Called by every thread of the block, on uniform control flow.
+__device__ void px_totals(const PxField& P, int nv, int s0, int s1, double* t0, double* t1) {
+  __shared__ double tot[2];
+  const int nc = P.nchunks;
+  if (threadIdx.x < 64) {
+    for (int q = 0; q < nv; ++q) {
+      const double* part = P.part + (size_t)(q == 0 ? s0 : s1) * nc;
+      double acc = 0.0;
+      for (int i0 = px_lane(); i0 < nc; i0 += 64 * 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = i0 + 64 * u < nc ? part[i0 + 64 * u] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (i0 + 64 * u < nc) acc += v[u];
+      }
+      acc = px_wave_sum(acc);
+      if (px_lane() == 0) tot[q] = acc;
+    }
+  }
+  __syncthreads();
+  *t0 = tot[0];
+  if (nv > 1) *t1 = tot[1];
 }
 
 __device__ inline float2 px_carrier(const DevField& f, int y, int x) {
@@ -150,14 +204,14 @@ __device__ inline float2 px_grad(const PxField& P, const FieldCG& s, const float
 }  // namespace
 
 // dc = c - c_w and S_w - S per neighbour (:255-268), fixed for the solve.
-__global__ void __launch_bounds__(64) k_px_prep(const PxField* __restrict__ F, const int* __restrict__ cp, int nf) {
+__global__ void __launch_bounds__(256) k_px_prep(const PxField* __restrict__ F, const int* __restrict__ cp, int nf) {
 #pragma clang fp contract(off)
   const PxCtx c = px_ctx(cp, nf);
   const PxField& P = F[c.fi];
   if (P.nn == 0) return;
   const size_t n = (size_t)P.f.h * P.f.w;
   for (int k = 0; k < kPxPerLane; ++k) {
-    const size_t i = (size_t)c.chunk * kPxChunk + (size_t)k * 64 + px_lane();
+    const size_t i = px_index(c, k);
     if (i >= n) break;
     const int y = (int)(i / P.f.w), x = (int)(i % P.f.w);
     const float2 car = px_carrier(P.f, y, x);
@@ -175,13 +229,13 @@ __global__ void __launch_bounds__(64) k_px_prep(const PxField* __restrict__ F, c
 // leave the plane), the diagonal ones when the oriented prior is on, the
 // neighbours' robust weights and the prior scale 1 + nu sum_j wt_j |dc_j|^2.
 // The field's first wave also opens the stage's CG state.
-__global__ void __launch_bounds__(64) k_px_weights(const PxField* __restrict__ F, const int* __restrict__ cp, int nf,
+__global__ void __launch_bounds__(256) k_px_weights(const PxField* __restrict__ F, const int* __restrict__ cp, int nf,
                                                    int cur, int stage) {
 #pragma clang fp contract(off)
   const PxCtx c = px_ctx(cp, nf);
   const PxField& P = F[c.fi];
   const FieldCG s = *P.st;
-  if (c.chunk == 0 && px_lane() == 0) {
+  if (c.chunk == 0 && threadIdx.x == 0) {
     P.px->done[0] = 0;
     P.px->done[1] = 0;
     if (stage < 16) P.px->iters[stage] = s.n_inner;
@@ -191,7 +245,7 @@ __global__ void __launch_bounds__(64) k_px_weights(const PxField* __restrict__ F
   const size_t n = (size_t)f.h * f.w;
   const float e2 = s.eps * s.eps, ec2 = s.eps_c * s.eps_c;
   for (int k = 0; k < kPxPerLane; ++k) {
-    const size_t i = (size_t)c.chunk * kPxChunk + (size_t)k * 64 + px_lane();
+    const size_t i = px_index(c, k);
     if (i >= n) break;
     const int y = (int)(i / f.w), x = (int)(i % f.w);
     const bool hr = x + 1 < f.w, hd = y + 1 < f.h, hl = x > 0;
@@ -242,7 +296,7 @@ __global__ void __launch_bounds__(64) k_px_weights(const PxField* __restrict__ F
 }
 
 // The stage-start gradient g = grad(chi) and its |g|^2 partials (slot 0).
-__global__ void __launch_bounds__(64) k_px_grad0(const PxField* __restrict__ F, const int* __restrict__ cp, int nf,
+__global__ void __launch_bounds__(256) k_px_grad0(const PxField* __restrict__ F, const int* __restrict__ cp, int nf,
                                                  int cur) {
   const PxCtx c = px_ctx(cp, nf);
   const PxField& P = F[c.fi];
@@ -250,14 +304,13 @@ __global__ void __launch_bounds__(64) k_px_grad0(const PxField* __restrict__ F, 
   const size_t n = (size_t)P.f.h * P.f.w;
   double gg = 0.0;
   for (int k = 0; k < kPxPerLane; ++k) {
-    const size_t i = (size_t)c.chunk * kPxChunk + (size_t)k * 64 + px_lane();
+    const size_t i = px_index(c, k);
     if (i >= n) break;
     const float2 g = px_grad(P, s, P.f.chi[cur], nullptr, 0.f, (int)(i / P.f.w), (int)(i % P.f.w), nullptr);
     P.g[i] = g;
     gg += (double)g.x * g.x + (double)g.y * g.y;
   }
-  gg = px_wave_sum(gg);
-  if (px_lane() == 0) P.part[c.chunk] = gg;
+  px_store(P, c.chunk, 1, gg, 0.0, 0, 0);
 }
 
 // First launch of CG iteration `it` (:539-547 of iteration it - 1, then
@@ -265,28 +318,27 @@ __global__ void __launch_bounds__(64) k_px_grad0(const PxField* __restrict__ F, 
 // chi (slot it & 1) -- and, from it = 1 on, of the previous one (slot
 // (it - 1) & 1) -- into beta and the tolerance exits; then d = -g + beta d_prev
 // (d = -g at it = 0) into d[it & 1] and the partials of curv(d) and Re<g, d>.
-__global__ void __launch_bounds__(64) k_px_dir(const PxField* __restrict__ F, const int* __restrict__ cp, int nf,
+__global__ void __launch_bounds__(256) k_px_dir(const PxField* __restrict__ F, const int* __restrict__ cp, int nf,
                                                int it, int stage) {
 #pragma clang fp contract(off)
   const PxCtx c = px_ctx(cp, nf);
   const PxField& P = F[c.fi];
   const FieldCG s = *P.st;
-  const int nc = P.nchunks;
   const bool done_in = P.px->done[0] != 0;
   bool stop = false;
   float beta = 0.f;
   if (!done_in) {
-    const double gg = px_sum(P.part + (size_t)(it & 1) * nc, nc);
+    double gg = 0.0, prev = 0.0;
+    px_totals(P, it == 0 ? 1 : 2, it & 1, (it - 1) & 1, &gg, &prev);
     if (it == 0) {
-      if (c.chunk == 0 && px_lane() == 0) P.px->gg0 = gg;
+      if (c.chunk == 0 && threadIdx.x == 0) P.px->gg0 = gg;
     } else {
-      const double prev = px_sum(P.part + (size_t)((it - 1) & 1) * nc, nc);
       beta = (float)(gg / (prev > 1e-30 ? prev : 1e-30));
       const double gg0 = P.px->gg0;
       stop = gg < s.tol2 * gg0 || gg < s.gg_floor * s.npix;
     }
   }
-  if (c.chunk == 0 && px_lane() == 0) {
+  if (c.chunk == 0 && threadIdx.x == 0) {
     P.px->done[1] = (done_in || stop) ? 1 : 0;
     if (!done_in && stop && stage < 16) P.px->iters[stage] = it;
   }
@@ -303,7 +355,7 @@ __global__ void __launch_bounds__(64) k_px_dir(const PxField* __restrict__ F, co
     return make_float2(-g.x + beta * q.x, -g.y + beta * q.y);
   };
   for (int k = 0; k < kPxPerLane; ++k) {
-    const size_t i = (size_t)c.chunk * kPxChunk + (size_t)k * 64 + px_lane();
+    const size_t i = px_index(c, k);
     if (i >= n) break;
     const int y = (int)(i / f.w), x = (int)(i % f.w);
     const bool hr = x + 1 < f.w, hdn = y + 1 < f.h;
@@ -345,12 +397,7 @@ __global__ void __launch_bounds__(64) k_px_dir(const PxField* __restrict__ F, co
     const float2 g = P.g[i];
     gd += (double)g.x * d0.x + (double)g.y * d0.y;
   }
-  hh = px_wave_sum(hh + (double)s.mu_d * hd2);
-  gd = px_wave_sum(gd);
-  if (px_lane() == 0) {
-    P.part[(size_t)2 * nc + c.chunk] = hh;
-    P.part[(size_t)3 * nc + c.chunk] = gd;
-  }
+  px_store(P, c.chunk, 2, hh + (double)s.mu_d * hd2, gd, 2, 3);
 }
 
 // Second launch of CG iteration `it` (:528-538): alpha = -gd / (2 hh) from the
@@ -358,13 +405,12 @@ __global__ void __launch_bounds__(64) k_px_dir(const PxField* __restrict__ F, co
 // chi[cur ^ ((it + 1) & 1)], g = grad(chi') and its |g|^2 partials into slot
 // (it + 1) & 1.  A stopped field copies chi forward instead, so after the
 // stage chi sits in chi[cur ^ (n_inner & 1)] whatever happened.
-__global__ void __launch_bounds__(64) k_px_step(const PxField* __restrict__ F, const int* __restrict__ cp, int nf,
+__global__ void __launch_bounds__(256) k_px_step(const PxField* __restrict__ F, const int* __restrict__ cp, int nf,
                                                 int it, int cur, int stage) {
 #pragma clang fp contract(off)
   const PxCtx c = px_ctx(cp, nf);
   const PxField& P = F[c.fi];
   const FieldCG s = *P.st;
-  const int nc = P.nchunks;
   const DevField& f = P.f;
   const size_t n = (size_t)f.h * f.w;
   const float2* chi_in = f.chi[cur ^ (it & 1)];
@@ -372,17 +418,16 @@ __global__ void __launch_bounds__(64) k_px_step(const PxField* __restrict__ F, c
   bool stopped = P.px->done[1] != 0;
   double hh = 0.0, gd = 0.0;
   if (!stopped) {
-    hh = px_sum(P.part + (size_t)2 * nc, nc);
-    gd = px_sum(P.part + (size_t)3 * nc, nc);
+    px_totals(P, 2, 2, 3, &hh, &gd);
     if (hh <= s.hh_floor) {
       stopped = true;
-      if (c.chunk == 0 && px_lane() == 0 && stage < 16) P.px->iters[stage] = it;
+      if (c.chunk == 0 && threadIdx.x == 0 && stage < 16) P.px->iters[stage] = it;
     }
   }
-  if (c.chunk == 0 && px_lane() == 0) P.px->done[0] = stopped ? 1 : 0;
+  if (c.chunk == 0 && threadIdx.x == 0) P.px->done[0] = stopped ? 1 : 0;
   if (stopped) {
     for (int k = 0; k < kPxPerLane; ++k) {
-      const size_t i = (size_t)c.chunk * kPxChunk + (size_t)k * 64 + px_lane();
+      const size_t i = px_index(c, k);
       if (i >= n) break;
       chi_out[i] = chi_in[i];
     }
@@ -392,7 +437,7 @@ __global__ void __launch_bounds__(64) k_px_step(const PxField* __restrict__ F, c
   const float2* d = f.d[it & 1];
   double gg = 0.0;
   for (int k = 0; k < kPxPerLane; ++k) {
-    const size_t i = (size_t)c.chunk * kPxChunk + (size_t)k * 64 + px_lane();
+    const size_t i = px_index(c, k);
     if (i >= n) break;
     float2 cn;
     const float2 g = px_grad(P, s, chi_in, d, alpha, (int)(i / f.w), (int)(i % f.w), &cn);
@@ -400,13 +445,12 @@ __global__ void __launch_bounds__(64) k_px_step(const PxField* __restrict__ F, c
     P.g[i] = g;
     gg += (double)g.x * g.x + (double)g.y * g.y;
   }
-  gg = px_wave_sum(gg);
-  if (px_lane() == 0) P.part[(size_t)((it + 1) & 1) * nc + c.chunk] = gg;
+  px_store(P, c.chunk, 1, gg, 0.0, (it + 1) & 1, 0);
 }
 
 // Y = S - Re[chi c] (:552-555) from chi[cur], and chi copied into the
 // caller's buffer chi[0] when it ended in the workspace copy.
-__global__ void __launch_bounds__(64) k_px_finalize(const PxField* __restrict__ F, const int* __restrict__ cp, int nf,
+__global__ void __launch_bounds__(256) k_px_finalize(const PxField* __restrict__ F, const int* __restrict__ cp, int nf,
                                                     int cur) {
 #pragma clang fp contract(off)
   const PxCtx c = px_ctx(cp, nf);
@@ -414,7 +458,7 @@ __global__ void __launch_bounds__(64) k_px_finalize(const PxField* __restrict__ 
   const DevField& f = P.f;
   const size_t n = (size_t)f.h * f.w;
   for (int k = 0; k < kPxPerLane; ++k) {
-    const size_t i = (size_t)c.chunk * kPxChunk + (size_t)k * 64 + px_lane();
+    const size_t i = px_index(c, k);
     if (i >= n) break;
     const float2 chi = f.chi[cur][i];
     if (cur) f.chi[0][i] = chi;
@@ -524,13 +568,13 @@ __device__ inline void pj_grad(const PxField& P, const FieldCG& s, const float* 
 // The joint solve's stage weights (:773-858): Charbonnier on grad Y, the
 // coupled chroma diffusivities, the diagonal ones, the neighbours' robust
 // weights on S_w - Y - Re[chi c_w] and the prior scale 1 + nu sum wt.
-__global__ void __launch_bounds__(64) k_pj_weights(const PxField* __restrict__ F, const int* __restrict__ cp, int nf,
+__global__ void __launch_bounds__(256) k_pj_weights(const PxField* __restrict__ F, const int* __restrict__ cp, int nf,
                                                    int cur, int stage) {
 #pragma clang fp contract(off)
   const PxCtx c = px_ctx(cp, nf);
   const PxField& P = F[c.fi];
   const FieldCG s = *P.st;
-  if (c.chunk == 0 && px_lane() == 0) {
+  if (c.chunk == 0 && threadIdx.x == 0) {
     P.px->done[0] = 0;
     P.px->done[1] = 0;
     if (stage < 16) P.px->iters[stage] = s.n_inner;
@@ -541,7 +585,7 @@ __global__ void __launch_bounds__(64) k_pj_weights(const PxField* __restrict__ F
   const size_t n = (size_t)f.h * f.w;
   const float e2 = s.eps * s.eps, ec2 = s.eps_c * s.eps_c;
   for (int k = 0; k < kPxPerLane; ++k) {
-    const size_t i = (size_t)c.chunk * kPxChunk + (size_t)k * 64 + px_lane();
+    const size_t i = px_index(c, k);
     if (i >= n) break;
     const int y = (int)(i / f.w), x = (int)(i % f.w);
     const bool hr = x + 1 < f.w, hd = y + 1 < f.h, hl = x > 0;
@@ -589,7 +633,7 @@ __global__ void __launch_bounds__(64) k_pj_weights(const PxField* __restrict__ F
   }
 }
 
-__global__ void __launch_bounds__(64) k_pj_grad0(const PxField* __restrict__ F, const int* __restrict__ cp, int nf,
+__global__ void __launch_bounds__(256) k_pj_grad0(const PxField* __restrict__ F, const int* __restrict__ cp, int nf,
                                                  int cur) {
   const PxCtx c = px_ctx(cp, nf);
   const PxField& P = F[c.fi];
@@ -597,7 +641,7 @@ __global__ void __launch_bounds__(64) k_pj_grad0(const PxField* __restrict__ F, 
   const size_t n = (size_t)P.f.h * P.f.w;
   double gg = 0.0;
   for (int k = 0; k < kPxPerLane; ++k) {
-    const size_t i = (size_t)c.chunk * kPxChunk + (size_t)k * 64 + px_lane();
+    const size_t i = px_index(c, k);
     if (i >= n) break;
     float gy;
     float2 gc;
@@ -607,35 +651,33 @@ __global__ void __launch_bounds__(64) k_pj_grad0(const PxField* __restrict__ F, 
     P.g[i] = gc;
     gg += (double)gy * gy + ((double)gc.x * gc.x + (double)gc.y * gc.y);
   }
-  gg = px_wave_sum(gg);
-  if (px_lane() == 0) P.part[c.chunk] = gg;
+  px_store(P, c.chunk, 1, gg, 0.0, 0, 0);
 }
 
 // First launch of joint iteration `it`: beta and the tolerance exits, the
 // direction (dY, dC) = -(gY, gC) + beta (dY, dC)_prev, and the partials of
 // its curvature (:687-755) and of <g, d>.
-__global__ void __launch_bounds__(64) k_pj_dir(const PxField* __restrict__ F, const int* __restrict__ cp, int nf,
+__global__ void __launch_bounds__(256) k_pj_dir(const PxField* __restrict__ F, const int* __restrict__ cp, int nf,
                                                int it, int stage) {
 #pragma clang fp contract(off)
   const PxCtx c = px_ctx(cp, nf);
   const PxField& P = F[c.fi];
   const FieldCG s = *P.st;
-  const int nc = P.nchunks;
   const bool done_in = P.px->done[0] != 0;
   bool stop = false;
   float beta = 0.f;
   if (!done_in) {
-    const double gg = px_sum(P.part + (size_t)(it & 1) * nc, nc);
+    double gg = 0.0, prev = 0.0;
+    px_totals(P, it == 0 ? 1 : 2, it & 1, (it - 1) & 1, &gg, &prev);
     if (it == 0) {
-      if (c.chunk == 0 && px_lane() == 0) P.px->gg0 = gg;
+      if (c.chunk == 0 && threadIdx.x == 0) P.px->gg0 = gg;
     } else {
-      const double prev = px_sum(P.part + (size_t)((it - 1) & 1) * nc, nc);
       beta = (float)(gg / (prev > 1e-30 ? prev : 1e-30));
       const double gg0 = P.px->gg0;
       stop = gg < s.tol2 * gg0 || gg < s.gg_floor * s.npix;
     }
   }
-  if (c.chunk == 0 && px_lane() == 0) {
+  if (c.chunk == 0 && threadIdx.x == 0) {
     P.px->done[1] = (done_in || stop) ? 1 : 0;
     if (!done_in && stop && stage < 16) P.px->iters[stage] = it;
   }
@@ -659,7 +701,7 @@ __global__ void __launch_bounds__(64) k_pj_dir(const PxField* __restrict__ F, co
   };
   double hh = 0.0, hsp = 0.0, hd2 = 0.0, gd = 0.0;
   for (int k = 0; k < kPxPerLane; ++k) {
-    const size_t i = (size_t)c.chunk * kPxChunk + (size_t)k * 64 + px_lane();
+    const size_t i = px_index(c, k);
     if (i >= n) break;
     const int y = (int)(i / f.w), x = (int)(i % f.w);
     const bool hr = x + 1 < f.w, hdn = y + 1 < f.h;
@@ -701,40 +743,33 @@ __global__ void __launch_bounds__(64) k_pj_dir(const PxField* __restrict__ F, co
     const float2 gC = P.g[i];
     gd += (double)gY * dY + ((double)gC.x * dC.x + (double)gC.y * dC.y);
   }
-  hh = px_wave_sum(hh + hsp + (double)s.mu_d * hd2);
-  gd = px_wave_sum(gd);
-  if (px_lane() == 0) {
-    P.part[(size_t)2 * nc + c.chunk] = hh;
-    P.part[(size_t)3 * nc + c.chunk] = gd;
-  }
+  px_store(P, c.chunk, 2, hh + hsp + (double)s.mu_d * hd2, gd, 2, 3);
 }
 
 // Second launch of joint iteration `it`: alpha or the curvature exit, the
 // step (Y, chi) += alpha (dY, dC) into the other buffers, the gradient there.
-__global__ void __launch_bounds__(64) k_pj_step(const PxField* __restrict__ F, const int* __restrict__ cp, int nf,
+__global__ void __launch_bounds__(256) k_pj_step(const PxField* __restrict__ F, const int* __restrict__ cp, int nf,
                                                 int it, int cur, int stage) {
 #pragma clang fp contract(off)
   const PxCtx c = px_ctx(cp, nf);
   const PxField& P = F[c.fi];
   const FieldCG s = *P.st;
-  const int nc = P.nchunks;
   const DevField& f = P.f;
   const size_t n = (size_t)f.h * f.w;
   const int src = cur ^ (it & 1), dst = cur ^ ((it + 1) & 1);
   bool stopped = P.px->done[1] != 0;
   double hh = 0.0, gd = 0.0;
   if (!stopped) {
-    hh = px_sum(P.part + (size_t)2 * nc, nc);
-    gd = px_sum(P.part + (size_t)3 * nc, nc);
+    px_totals(P, 2, 2, 3, &hh, &gd);
     if (hh <= s.hh_floor) {
       stopped = true;
-      if (c.chunk == 0 && px_lane() == 0 && stage < 16) P.px->iters[stage] = it;
+      if (c.chunk == 0 && threadIdx.x == 0 && stage < 16) P.px->iters[stage] = it;
     }
   }
-  if (c.chunk == 0 && px_lane() == 0) P.px->done[0] = stopped ? 1 : 0;
+  if (c.chunk == 0 && threadIdx.x == 0) P.px->done[0] = stopped ? 1 : 0;
   if (stopped) {
     for (int k = 0; k < kPxPerLane; ++k) {
-      const size_t i = (size_t)c.chunk * kPxChunk + (size_t)k * 64 + px_lane();
+      const size_t i = px_index(c, k);
       if (i >= n) break;
       f.chi[dst][i] = f.chi[src][i];
       P.yv[dst][i] = P.yv[src][i];
@@ -744,7 +779,7 @@ __global__ void __launch_bounds__(64) k_pj_step(const PxField* __restrict__ F, c
   const float alpha = (float)(-0.5 * gd / hh);
   double gg = 0.0;
   for (int k = 0; k < kPxPerLane; ++k) {
-    const size_t i = (size_t)c.chunk * kPxChunk + (size_t)k * 64 + px_lane();
+    const size_t i = px_index(c, k);
     if (i >= n) break;
     float gy;
     float2 gc;
@@ -757,19 +792,18 @@ __global__ void __launch_bounds__(64) k_pj_step(const PxField* __restrict__ F, c
     P.g[i] = gc;
     gg += (double)gy * gy + ((double)gc.x * gc.x + (double)gc.y * gc.y);
   }
-  gg = px_wave_sum(gg);
-  if (px_lane() == 0) P.part[(size_t)((it + 1) & 1) * nc + c.chunk] = gg;
+  px_store(P, c.chunk, 1, gg, 0.0, (it + 1) & 1, 0);
 }
 
 // (Y, chi) into the caller's buffers when they ended in the workspace copy.
-__global__ void __launch_bounds__(64) k_pj_finalize(const PxField* __restrict__ F, const int* __restrict__ cp, int nf,
+__global__ void __launch_bounds__(256) k_pj_finalize(const PxField* __restrict__ F, const int* __restrict__ cp, int nf,
                                                     int cur) {
   const PxCtx c = px_ctx(cp, nf);
   const PxField& P = F[c.fi];
   if (!cur) return;
   const size_t n = (size_t)P.f.h * P.f.w;
   for (int k = 0; k < kPxPerLane; ++k) {
-    const size_t i = (size_t)c.chunk * kPxChunk + (size_t)k * 64 + px_lane();
+    const size_t i = px_index(c, k);
     if (i >= n) break;
     P.f.chi[0][i] = P.f.chi[1][i];
     P.yv[0][i] = P.yv[1][i];

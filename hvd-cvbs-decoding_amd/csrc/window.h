@@ -22,7 +22,7 @@ __global__ void k_traj_extrapolate(int nt, const float* vy, const float* vx, flo
 __global__ void k_traj_snap(int m, int nt, float* const* dy, float* const* dx, const float* const* conf,
                             const float* ho, const int* off);
 __global__ void k_env_terms(int h, int w, const float* Y, const float2* chi, float* gmag, float* lapmag);
-__global__ void k_smooth_rows(int h, int w, const float* src, float* out);
+__global__ void k_smooth_rows(int h, int w, const float* const* src, float* const* out);
 __global__ void k_env_finish(int n, const float* const* vg_in, float* const* vgrad, float* const* lap_io,
                              float* const* sample);
 __global__ void k_split_cplx(size_t n, const float2* z, float* re, float* im);

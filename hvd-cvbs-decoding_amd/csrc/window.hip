@@ -348,23 +348,69 @@ __global__ void k_env_terms(int h, int w, const float* Y, const float2* chi, flo
 }
 
 // The 5-tap sliding RMS along each row (sequence.cpp:855-888), in hvd-core's
-// order of adds and subtracts; one thread per row.
-__global__ void k_smooth_rows(int h, int w, const float* src, float* out) {
+// order of adds and subtracts (a float running sum: the first three samples,
+// then per step the sample entering at x + 2 and the one leaving at x - 3;
+// outside the row 0 is added, which leaves the sum, never -0, as it is), for
+// a batch of planes: plane b is src[b] -> out[b].  Block: 64 rows of one
+// plane, one thread per row running only the sum; the row segments come
+// through LDS in chunks of 64 columns and the chunk's RMS is formed and
+// stored column by column (coalesced).  Grid: (ceil(h / 64), planes).
+__global__ void __launch_bounds__(64) k_smooth_rows(int h, int w, const float* const* src, float* const* out) {
 #pragma clang fp contract(off)
-  const int y = blockIdx.x * blockDim.x + threadIdx.x;
-  if (y >= h) return;
-  const float* r = src + (size_t)y * w;
-  float* o = out + (size_t)y * w;
+  constexpr int C = 64, L = 3, SW = C + 5;
+  __shared__ float sm[64][SW + 2];
+  __shared__ float sa[64][C + 1];
+  const int t = threadIdx.x;
+  const float* in = src[blockIdx.y];
+  float* o = out[blockIdx.y];
+  const int row0 = blockIdx.x * 64, y = row0 + t;
   float acc = 0.f;
-  for (int x = 0; x < min(3, w); ++x) acc += r[x];
-  for (int x = 0; x < w; ++x) {
-    const int lo = x - 2, hi = x + 2;
-    if (x > 0) {
-      if (hi < w) acc += r[hi];
-      if (lo - 1 >= 0) acc -= r[lo - 1];
+  for (int x0 = 0; x0 < w; x0 += C) {
+    __syncthreads();
+    // row j's segment: thread t loads columns t and t + 64 of it; rows in
+    // batches of 8, every load of a batch in flight before its stores
+    for (int j0 = 0; j0 < 64; j0 += 8) {
+      float v[8][2];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int rj = row0 + j0 + u;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const int k = t + 64 * q, col = x0 - L + k;
+          v[u][q] = (k < SW && rj < h && col >= 0 && col < w) ? in[(size_t)rj * w + col] : 0.f;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+          if (t + 64 * q < SW) sm[j0 + u][t + 64 * q] = v[u][q];
     }
-    const int n_taps = min(hi, w - 1) - max(lo, 0) + 1;
-    o[x] = sqrtf(acc / (float)n_taps);
+    __syncthreads();
+    const int xend = min(x0 + C, w);
+    if (y < h) {
+      int x = x0;
+      if (x == 0) {
+        for (int i = 0; i < min(3, w); ++i) acc += sm[t][i + L];
+        sa[t][0] = acc;
+        ++x;
+      }
+#pragma unroll 8
+      for (; x < xend; ++x) {
+        acc += sm[t][x + 2 - x0 + L];
+        acc -= sm[t][x - 3 - x0 + L];
+        sa[t][x - x0] = acc;
+      }
+    }
+    __syncthreads();
+    const int x = x0 + t;
+    if (x < w) {
+      const float n_taps = (float)(min(x + 2, w - 1) - max(x - 2, 0) + 1);
+      for (int j = 0; j < 64; ++j) {
+        const int rj = row0 + j;
+        if (rj < h) o[(size_t)rj * w + x] = sqrtf(sa[j][t] / n_taps);
+      }
+    }
   }
 }
 
@@ -625,6 +671,7 @@ int drive(hvd_gpu_ctx* ctx, const hvd_gpu_plane* F, int nf, const int* par, int 
   const size_t o_nb = cv.take(sizeof(float) * 4 * np * (size_t)nf * no);      // comp, car (2), conf
   const size_t o_env = cv.take(sizeof(float) * 2 * np * (size_t)nf);          // vgrad, lapsm
   const size_t o_envt = cv.take(sizeof(float) * 2 * np * (size_t)nf);         // smoothed terms
+  const size_t o_envr = cv.take(sizeof(float) * 2 * np * (size_t)nf);         // raw terms
   const size_t o_envs = cv.take(sizeof(float) * (np / 7 + 1) * (size_t)nf);   // median samples
   const size_t o_anc = cv.take(sizeof(float) * 4 * np * (size_t)nf);          // anchor luma, chroma (2), weight
   const size_t o_tmp = cv.take(sizeof(float) * 12 * np);
@@ -649,6 +696,7 @@ int drive(hvd_gpu_ctx* ctx, const hvd_gpu_plane* F, int nf, const int* par, int 
   auto nbp = [&](int j, int oi) { return (float*)(base + o_nb) + ((size_t)j * no + oi) * 4 * np; };
   auto envp = [&](int j) { return (float*)(base + o_env) + (size_t)j * 2 * np; };
   auto envt = [&](int j) { return (float*)(base + o_envt) + (size_t)j * 2 * np; };
+  auto envr = [&](int j) { return (float*)(base + o_envr) + (size_t)j * 2 * np; };
   auto envs = [&](int j) { return (float*)(base + o_envs) + (size_t)j * (np / 7 + 1); };
   auto ancp = [&](int j) { return (float*)(base + o_anc) + (size_t)j * 4 * np; };
   float* tmp = (float*)(base + o_tmp);
@@ -923,8 +971,8 @@ int drive(hvd_gpu_ctx* ctx, const hvd_gpu_plane* F, int nf, const int* par, int 
       // (d) the odd-offset envelopes (sequence.cpp:790-940)
       std::vector<char> has_env(ng, 0);
       if (has_temporal) {
-        std::vector<const float*> vin;
-        std::vector<float*> vout, lio, smp;
+        std::vector<const float*> vin, rin;
+        std::vector<float*> vout, lio, smp, rout;
         for (int gi = 0; gi < ng; ++gi) {
           const int j = G[gi];
           bool any_odd = false;
@@ -933,11 +981,14 @@ int drive(hvd_gpu_ctx* ctx, const hvd_gpu_plane* F, int nf, const int* par, int 
           if (!any_odd) continue;
           has_env[gi] = 1;
           float* sm = envt(j);  // [smoothed gmag | smoothed lapmag]
+          float* raw = envr(j);  // [gmag | lapmag]
           hipLaunchKernelGGL(k_env_terms, dim3(grid_for(np)), dim3(256), 0, st, h, w, (const float*)F[j].y,
-                             cfl(F[j].chi), T(0), T(1));
-          hipLaunchKernelGGL(k_smooth_rows, dim3((h + 63) / 64), dim3(64), 0, st, h, w, (const float*)T(0), sm);
-          hipLaunchKernelGGL(k_smooth_rows, dim3((h + 63) / 64), dim3(64), 0, st, h, w, (const float*)T(1), sm + np);
+                             cfl(F[j].chi), raw, raw + np);
           if (!X.launched()) return X.rc;
+          rin.push_back(raw);
+          rin.push_back(raw + np);
+          rout.push_back(sm);
+          rout.push_back(sm + np);
           vin.push_back(sm);
           vout.push_back(envp(j));
           lio.push_back(sm + np);
@@ -947,6 +998,13 @@ int drive(hvd_gpu_ctx* ctx, const hvd_gpu_plane* F, int nf, const int* par, int 
           const int m = (int)vin.size();
           if (m > 64) return fail(ctx, HVD_GPU_ERR_INVALID, "more than 64 fields in one colour");
           void** pt = ptrs;  // the first 64 x 4 slots
+          // the smoothing of every plane of the colour in one launch
+          if (hipMemcpyAsync(pt, rin.data(), sizeof(void*) * 2 * m, hipMemcpyHostToDevice, st) != hipSuccess ||
+              hipMemcpyAsync(pt + 128, rout.data(), sizeof(void*) * 2 * m, hipMemcpyHostToDevice, st) != hipSuccess)
+            return fail(ctx, HVD_GPU_ERR_HIP, "table upload failed");
+          hipLaunchKernelGGL(k_smooth_rows, dim3((h + 63) / 64, 2 * m), dim3(64), 0, st, h, w,
+                             (const float* const*)pt, (float* const*)(pt + 128));
+          if (!X.launched()) return X.rc;
           if (hipMemcpyAsync(pt, vin.data(), sizeof(void*) * m, hipMemcpyHostToDevice, st) != hipSuccess ||
               hipMemcpyAsync(pt + 64, vout.data(), sizeof(void*) * m, hipMemcpyHostToDevice, st) != hipSuccess ||
               hipMemcpyAsync(pt + 128, lio.data(), sizeof(void*) * m, hipMemcpyHostToDevice, st) != hipSuccess ||

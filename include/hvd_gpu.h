@@ -402,8 +402,8 @@ int hvd_gpu_upsample_confidence(hvd_gpu_ctx* ctx, const float* conf, int tiles_h
                                 int out_w, int fast, float* out, void* stream);
 
 /* ComplexCoherence (temporal.h:68, temporal.cpp:169-196): |<z1 conj z2>| /
- * (sqrt(<|z1|^2><|z2|^2>) + 1e-6) with box averages of radius r, in [0, 1];
- * z1, z2 interleaved complex h x w. */
+ * (sqrt(<|z1|^2><|z2|^2>) + 1e-6) with box averages of radius r (at most
+ * 32), in [0, 1]; z1, z2 interleaved complex h x w. */
 int hvd_gpu_complex_coherence(hvd_gpu_ctx* ctx, const float* z1, const float* z2, int h, int w, int r, float* out,
                               void* stream);
 
@@ -563,7 +563,7 @@ int hvd_gpu_tridiag_smooth(hvd_gpu_ctx* ctx, const float* d, const float* a, int
                            void* stream);
 
 /* BoxBlur (engine/motion.cpp:21-55): separable (2r + 1)-box mean with
- * edge-truncated windows and FP64 running sums, h x w floats. */
+ * edge-truncated windows and FP64 running sums, h x w floats, r <= 32. */
 int hvd_gpu_box_blur(hvd_gpu_ctx* ctx, const float* in, int h, int w, int r, float* out, void* stream);
 
 /* The chroma plane of hvd::FrameYc: mode 0 C = S - Y (DecodeFrame,

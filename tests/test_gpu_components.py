@@ -83,12 +83,31 @@ def test_tridiag_smooth(engine, orc64):
         np.testing.assert_allclose(got, want, rtol=1e-6, atol=1e-6)
 
 
-@pytest.mark.parametrize("hw,r", [((242, 760), 2), ((7, 5), 3), ((1, 9), 1)])
+@pytest.mark.parametrize("hw,r", [((242, 760), 2), ((7, 5), 3), ((1, 9), 1), ((3, 2100), 4), ((5, 1500), 8)])
 def test_box_blur(engine, orc32, hw, r):
     rng = np.random.default_rng(11)
     a = rng.normal(50, 20, size=hw).astype(np.float32)
     got = engine.box_blur(dev(a), r).cpu().numpy()
     np.testing.assert_array_equal(got, orc32.box_blur(a, r))
+
+
+def test_box_blur_rows_that_round(engine, orc32):
+    """Rows whose running sums round (the along-x pass's left-to-right
+    fallback): tiny samples beside large ones, huge cancelling samples, a
+    zero row, denormals; every row must still match hvd-core's order bit for
+    bit."""
+    rng = np.random.default_rng(12)
+    h, w = 8, 700
+    a = rng.normal(50, 20, size=(h, w)).astype(np.float32)
+    a[0, ::7] = 1e-9                      # below 2^-12 of the row's largest
+    a[1, :] = 0.0                         # all zero
+    a[2, 10], a[2, 11], a[2, 300] = 1e30, 1.0, -1e30  # cancellation after rounding
+    a[3, ::3] = np.float32(1e-40)          # denormals
+    a[4, :] = rng.normal(0, 1e-6, w).astype(np.float32)  # a tiny row (still exact)
+    a[5, ::2] *= -1
+    for r in (1, 2, 5):
+        got = engine.box_blur(dev(a), r).cpu().numpy()
+        np.testing.assert_array_equal(got, orc32.box_blur(a, r))
 
 
 def test_chroma_plane(engine):
