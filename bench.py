@@ -341,11 +341,39 @@ def temporal_bench(torch, gpu, eng, reps=10):
     eng.set_profiling(False)
     us_it = 1e3 * cg_ms / max(launched, 1)
     bytes_it = (CG_BYTES_PER_PX_ITER + 16 * nn) * h * w
+    # the same solve for a batch of 16 such fields in one call (the window
+    # driver solves a colour of its Gauss-Seidel sweep as one batch)
+    nb_f = 16
+    planes, outs = [], []
+    for _ in range(nb_f):
+        c = torch.empty_like(dchi0)
+        planes.append(gpu.Engine.plane(ds, c, torch.empty_like(ds), carrier=dcar))
+        outs.append(c)
+
+    def batch():
+        for c in outs:
+            c.copy_(dchi0)
+        eng.variational_refine_temporal_batch(planes, [nbrs] * nb_f, p, 0.5, 2.0)
+
+    batch()
+    torch.cuda.synchronize()
+    eng.set_profiling(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batch()
+    torch.cuda.synchronize()
+    dtb = (time.perf_counter() - t0) / reps
+    cg_ms_b, launched_b, _ = eng.cg_timing()
+    eng.set_profiling(False)
+    us_it_b = 1e3 * cg_ms_b / max(launched_b, 1)
     return {"fields_per_s": round(1.0 / dt, 1), "ms_per_field": round(1e3 * dt, 3),
             "us_per_cg_iteration": round(us_it, 2),
             "ms_per_field_outside_cg_iterations": round(1e3 * dt - cg_ms / reps, 3),
             "cg_algorithmic_gbs": round(bytes_it / (us_it * 1e-6) / 1e9, 1),
-            "neighbours": nn, "cg_iterations": iters}
+            "neighbours": nn, "cg_iterations": iters,
+            "batch16": {"fields_per_s": round(nb_f / dtb, 1), "us_per_cg_iteration": round(us_it_b, 2),
+                        "cg_algorithmic_gbs": round(nb_f * bytes_it / (us_it_b * 1e-6) / 1e9, 1),
+                        "frac_of_hbm_peak": round(nb_f * bytes_it / (us_it_b * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}}
 
 
 WINDOW_FRAMES, WINDOW_CORE = 8, (1, 7)  # HvdConfig chunk_frames 6 + chunk_overlap 1 on each side

@@ -9,6 +9,31 @@
 
 namespace hvdk {
 
+// One neighbour equation of a colour group (window.hip, the batched form of
+// MotionCompensatePrev plus the coherence and odd-offset gates).
+struct NbJob {
+  const float* mdy;     // tile motion (snapped copy), th x tw
+  const float* mdx;
+  const float* mconf;   // tile confidence, th x tw
+  const float* s_k;     // neighbour composite, carrier, chroma (h x w)
+  const float2* car_k;
+  const float2* chi_k;
+  const float2* chi_j;  // the field's own chroma (the coherence gate)
+  float* comp;          // outputs: warped composite, carrier, confidence
+  float2* car;
+  float* conf;
+  float* sq;            // scratch: squared tile confidence, snapped vectors and map (th x tw each)
+  float* sy;
+  float* sx;
+  float* sc;
+  float* cn;            // slow mode: the nearest tile's squared confidence (h x w)
+  float* cb;            //            and its blur
+  float* coh;           // the gate: the four coherence terms (4 h x w)
+  float* cb4;           //           and their blur
+  const float* vgrad;   // the odd-offset envelope or null
+  const float* lapsm;
+};
+
 __global__ void k_verify_pick(int n, const float* pdy, const float* pdx, const float* bp, const float* bz, float* dy,
                               float* dx, float* best);
 __global__ void k_noise_sigma(const float* const* s, int h, int w, float* d_all, float* sigma);
@@ -39,6 +64,11 @@ __global__ void k_synth_finish(size_t n, const float* s, const float2* car, cons
 __global__ void k_synth_weight(size_t n, const float* r, float eps_b, float* weight);
 __global__ void k_psi(size_t n, int nn, const float* s, const float2* car, const float* const* sw,
                       const float2* const* cw, const float* const* conf, float2* chi);
+
+__global__ void k_nb_square(int nt, const NbJob* J);
+__global__ void k_nb_snap(int th, int tw, int fast, const NbJob* J);
+__global__ void k_nb_warp(int h, int w, int th, int tw, int tile, int fast, int gate, const NbJob* J);
+__global__ void k_nb_gates(int h, int w, int fast, int gate, float a, float e2, float floor_max, const NbJob* J);
 
 }  // namespace hvdk
 

@@ -26,6 +26,7 @@
 
 #include "hvd_gpu.h"
 #include "hvd_kernels.h"
+#include "warp_ops.h"
 #include "window.h"
 
 namespace hvdk {
@@ -561,6 +562,92 @@ __global__ void k_psi(size_t n, int nn, const float* s, const float2* car, const
   }
 }
 
+// ---- the neighbour equations of a colour, batched -------------------------
+// One job per (field j, offset o) with its neighbour k = j + o: hvd-core's
+// MotionCompensatePrev of S_k and c_k by the snapped tile motion (temporal.cpp:
+// 198-249), the coherence gate on chi_j against the bilinear warp of chi_k
+// (:140-196, sequence.cpp:965-995) and the odd-offset envelope gate
+// (:996-1015), every job of the colour in the same launches.  The arithmetic
+// is the single-plane kernels' (warp_ops.h).
+
+// (1) per tile: the squared confidence (fast mode's map).
+__global__ void k_nb_square(int nt, const NbJob* J) {
+  const NbJob& j = J[blockIdx.y];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nt) j.sq[i] = j.mconf[i] * j.mconf[i];
+}
+
+// (2) per tile: the outlier snap of the vectors (and, in fast mode, of the
+// squared confidence map taken as both components, as hvd-core does).
+__global__ void k_nb_snap(int th, int tw, int fast, const NbJob* J) {
+  const NbJob& j = J[blockIdx.y];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= th * tw) return;
+  float oy, ox;
+  wops::vpp_snap(th, tw, j.mdy, j.mdx, i, oy, ox);
+  j.sy[i] = oy;
+  j.sx[i] = ox;
+  if (fast) {
+    float cy, cx;
+    wops::vpp_snap(th, tw, j.sq, j.sq, i, cy, cx);
+    j.sc[i] = cy;
+  }
+}
+
+// (3) per pixel: the vectors, the warped composite and carrier, the
+// confidence (fast: the interpolated map; else the nearest tile's square for
+// the radius-8 blur), and with the gate the coherence terms of chi_j against
+// the warped chi_k.
+__global__ void k_nb_warp(int h, int w, int th, int tw, int tile, int fast, int gate, const NbJob* J) {
+#pragma clang fp contract(off)
+  const NbJob& j = J[blockIdx.y];
+  const size_t n = (size_t)h * w;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int y = (int)(i / w), x = (int)(i % w);
+  const float dy = wops::vpp_interp(th, tw, tile, j.sy, y, x), dx = wops::vpp_interp(th, tw, tile, j.sx, y, x);
+  const size_t src = wops::warp_tiles_src(h, w, y, x, dy, dx);
+  j.comp[i] = j.s_k[src];
+  j.car[i] = j.car_k[src];
+  if (fast) {
+    j.conf[i] = wops::vpp_interp(th, tw, tile, j.sc, y, x);
+  } else {
+    const float cf = wops::tile_at(j.mconf, tw, min(y / tile, th - 1), min(x / tile, tw - 1));
+    j.cn[i] = cf * cf;
+  }
+  if (gate) {
+    const float2 cw = make_float2(wops::warp_bilinear_c(h, w, j.chi_k, 0, dy, dx, 0.0f, y, x),
+                                  wops::warp_bilinear_c(h, w, j.chi_k, 1, dy, dx, 0.0f, y, x));
+    float a, b, c, d;
+    wops::coh_terms(j.chi_j[i], cw, a, b, c, d);
+    j.coh[i] = a;
+    j.coh[n + i] = b;
+    j.coh[2 * n + i] = c;
+    j.coh[3 * n + i] = d;
+  }
+}
+
+// (4) per pixel: the confidence after the gates (k_conf_gates' arithmetic),
+// from the blurred confidence (slow mode) and the blurred coherence terms.
+__global__ void k_nb_gates(int h, int w, int fast, int gate, float a, float e2, float floor_max, const NbJob* J) {
+#pragma clang fp contract(off)
+  const NbJob& j = J[blockIdx.y];
+  const size_t n = (size_t)h * w;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float c = fast ? j.conf[i] : j.cb[i];
+  if (gate) {
+    const float g = wops::coh_out(j.cb4[i], j.cb4[n + i], j.cb4[2 * n + i], j.cb4[3 * n + i]);
+    c *= (1.0f - a) + a * g;
+  }
+  if (j.vgrad) {
+    const float b = j.vgrad[i];
+    const float fl = floor_max * j.lapsm[i];
+    c *= fmaxf(fl, e2 / (e2 + b * b));
+  }
+  j.conf[i] = c;
+}
+
 }  // namespace hvdk
 
 // ---------------------------------------------------------------------------
@@ -573,6 +660,8 @@ namespace {
 using namespace hvdk;
 
 inline unsigned grid_for(size_t n) { return (unsigned)std::min<size_t>((n + 255) / 256, 8192); }
+
+constexpr int kJobChunk = 32;  // neighbour equations batched per launch
 
 struct Carver {
   size_t off = 0;
@@ -676,6 +765,11 @@ int drive(hvd_gpu_ctx* ctx, const hvd_gpu_plane* F, int nf, const int* par, int 
   const size_t o_anc = cv.take(sizeof(float) * 4 * np * (size_t)nf);          // anchor luma, chroma (2), weight
   const size_t o_tmp = cv.take(sizeof(float) * 12 * np);
   const size_t o_smf = cv.take(sizeof(float) * 3 * nt);                       // SynthReference's own estimate
+  // the batched neighbour equations: up to kJobChunk jobs at a time
+  const size_t job_floats = 4 * (size_t)nt + 14 * np;
+  const size_t o_jobs = cv.take(sizeof(NbJob) * kJobChunk);
+  const size_t o_jmd = cv.take(sizeof(MotionDesc) * (3 * kJobChunk + 2));
+  const size_t o_jscr = cv.take(sizeof(float) * job_floats * kJobChunk);
   const size_t o_ptr = cv.take(sizeof(void*) * 64 * (size_t)(nf + 4));       // pointer tables
   const size_t o_small = cv.take(sizeof(float) * 64 * (size_t)(nf + 4));
   char* base = pool(ctx, cv.off, st);
@@ -1016,49 +1110,94 @@ int drive(hvd_gpu_ctx* ctx, const hvd_gpu_plane* F, int nf, const int* par, int 
           if (hipStreamSynchronize(st) != hipSuccess) return fail(ctx, HVD_GPU_ERR_HIP, "sync failed");
         }
       }
-      // (e) neighbour equations (sequence.cpp:941-1021)
+      // (e) neighbour equations (sequence.cpp:941-1021), the colour's jobs
+      // batched kJobChunk at a time
       std::vector<std::vector<hvd_gpu_neighbor>> nbrs(ng);
       if (has_temporal) {
         const bool gate_on = c.coherence_gate > 0.0 && !(fast && pss == 0);
+        std::vector<NbJob> jobs;
         for (int gi = 0; gi < ng; ++gi) {
           const int j = G[gi];
           for (int oi = 0; oi < no; ++oi) {
             const int o = offs[oi], k = j + o;
             if (k < 0 || k >= nf) continue;
             const MF mo = pmf(j, oi);
-            const float* mconf = mcache(j, oi).conf;
             float* nb = nbp(j, oi);
-            float *comp = nb, *car = nb + np, *conf = nb + 3 * np;
-            if (!X.ok(hvd_gpu_motion_compensate_prev(ctx, F[k].s, F[k].carrier, h, w, mo.dy, mo.dx, mconf, th, tw,
-                                                     tile, fast ? 1 : 0, comp, car, conf, st)))
-              return X.rc;
-            const float* gamma = nullptr;
-            if (gate_on) {
-              float *vdy = T(0), *vdx = T(1), *re = T(2), *im = T(3), *wre = T(4), *wim = T(5), *cw = T(6),
-                    *gm = T(8);
-              if (!X.ok(hvd_gpu_vectors_per_pixel(ctx, mo.dy, mo.dx, th, tw, tile, h, w, vdy, vdx, st))) return X.rc;
-              hipLaunchKernelGGL(k_split_cplx, dim3(grid_for(np)), dim3(256), 0, st, np, cfl(F[k].chi), re, im);
-              if (!X.launched()) return X.rc;
-              if (!X.ok(hvd_gpu_warp_bilinear(ctx, re, h, w, vdy, vdx, 0.0f, h, w, wre, st)) ||
-                  !X.ok(hvd_gpu_warp_bilinear(ctx, im, h, w, vdy, vdx, 0.0f, h, w, wim, st)))
-                return X.rc;
-              hipLaunchKernelGGL(k_join_cplx, dim3(grid_for(np)), dim3(256), 0, st, np, (const float*)wre,
-                                 (const float*)wim, (float2*)cw);
-              if (!X.launched()) return X.rc;
-              if (!X.ok(hvd_gpu_complex_coherence(ctx, F[j].chi, cw, h, w, 6, gm, st))) return X.rc;
-              gamma = gm;
+            NbJob q{};
+            q.mdy = mo.dy;
+            q.mdx = mo.dx;
+            q.mconf = mcache(j, oi).conf;
+            q.s_k = F[k].s;
+            q.car_k = cfl(F[k].carrier);
+            q.chi_k = cfl(F[k].chi);
+            q.chi_j = cfl(F[j].chi);
+            q.comp = nb;
+            q.car = (float2*)(nb + np);
+            q.conf = nb + 3 * np;
+            if ((o & 1) && has_env[gi]) {
+              q.vgrad = envp(j);
+              q.lapsm = envt(j) + np;
             }
-            const bool odd_gate = (o & 1) && has_env[gi];
-            if (gamma || odd_gate) {
-              const float e2 = temporal_eps * temporal_eps;
-              const float fmax = std::clamp((float)c.odd_gate_floor, 0.0F, 1.0F);
-              hipLaunchKernelGGL(k_conf_gates, dim3(grid_for(np)), dim3(256), 0, st, np, conf, gamma,
-                                 (float)c.coherence_gate, odd_gate ? (const float*)envp(j) : nullptr,
-                                 odd_gate ? (const float*)envt(j) + np : nullptr, e2, fmax);
-              if (!X.launched()) return X.rc;
-            }
-            nbrs[gi].push_back(hvd_gpu_neighbor{comp, car, conf});
+            jobs.push_back(q);
+            nbrs[gi].push_back(hvd_gpu_neighbor{nb, nb + np, nb + 3 * np});
           }
+        }
+        const float e2 = temporal_eps * temporal_eps;
+        const float fmax = std::clamp((float)c.odd_gate_floor, 0.0F, 1.0F);
+        NbJob* djobs = (NbJob*)(base + o_jobs);
+        MotionDesc* dmd = (MotionDesc*)(base + o_jmd);
+        float* scr = (float*)(base + o_jscr);
+        for (size_t q0 = 0; q0 < jobs.size(); q0 += kJobChunk) {
+          const int m = (int)std::min<size_t>(kJobChunk, jobs.size() - q0);
+          std::vector<NbJob> chunk(jobs.begin() + q0, jobs.begin() + q0 + m);
+          // scratch: per job 4 tile maps; the blurs read and write planes in
+          // job order (coherence: 4 per job; slow confidence: 1 per job)
+          float* cn_all = scr;                                   // m + 1 planes
+          float* cb_all = cn_all + (size_t)(m + 1) * np;         // m + 1 planes
+          float* coh_all = cb_all + (size_t)(m + 1) * np;        // 4 m planes
+          float* cb4_all = coh_all + (size_t)4 * m * np;         // 4 m planes
+          float* tmp_all = cb4_all + (size_t)4 * m * np;         // 4 m planes (blur scratch)
+          float* tiles = tmp_all + (size_t)4 * m * np;           // 4 m tile maps
+          for (int q = 0; q < m; ++q) {
+            NbJob& jb = chunk[q];
+            jb.sq = tiles + (size_t)4 * q * nt;
+            jb.sy = jb.sq + nt;
+            jb.sx = jb.sy + nt;
+            jb.sc = jb.sx + nt;
+            jb.cn = cn_all + (size_t)q * np;
+            jb.cb = cb_all + (size_t)q * np;
+            jb.coh = coh_all + (size_t)4 * q * np;
+            jb.cb4 = cb4_all + (size_t)4 * q * np;
+          }
+          std::vector<MotionDesc> md;
+          if (gate_on)
+            for (int q = 0; q < m; ++q) {
+              md.push_back(MotionDesc{chunk[q].coh, chunk[q].coh + np});
+              md.push_back(MotionDesc{chunk[q].coh + 2 * np, chunk[q].coh + 3 * np});
+            }
+          const size_t md_coh = md.size();
+          if (!fast)
+            for (int q = 0; q < m; q += 2)
+              md.push_back(MotionDesc{chunk[q].cn, q + 1 < m ? chunk[q + 1].cn : chunk[q].cn});
+          if (hipMemcpyAsync(djobs, chunk.data(), sizeof(NbJob) * m, hipMemcpyHostToDevice, st) != hipSuccess ||
+              (!md.empty() &&
+               hipMemcpyAsync(dmd, md.data(), sizeof(MotionDesc) * md.size(), hipMemcpyHostToDevice, st) != hipSuccess))
+            return fail(ctx, HVD_GPU_ERR_HIP, "job upload failed");
+          const dim3 gt((nt + 63) / 64, m), gp((unsigned)((np + 255) / 256), m);
+          if (fast) hipLaunchKernelGGL(k_nb_square, gt, dim3(64), 0, st, nt, (const NbJob*)djobs);
+          hipLaunchKernelGGL(k_nb_snap, gt, dim3(64), 0, st, th, tw, fast ? 1 : 0, (const NbJob*)djobs);
+          hipLaunchKernelGGL(k_nb_warp, gp, dim3(256), 0, st, h, w, th, tw, tile, fast ? 1 : 0, gate_on ? 1 : 0,
+                             (const NbJob*)djobs);
+          if (!X.launched()) return X.rc;
+          // the confidence's radius-8 blur (slow mode: temporal.cpp:217-233)
+          // and the coherence terms' radius-6 blur (:169-196)
+          if (!fast) launch_me_blur(st, dmd + md_coh, (m + 1) / 2, h, w, 8, tmp_all, cb_all);
+          if (gate_on) launch_me_blur(st, dmd, 2 * m, h, w, 6, tmp_all, cb4_all);
+          hipLaunchKernelGGL(k_nb_gates, gp, dim3(256), 0, st, h, w, fast ? 1 : 0, gate_on ? 1 : 0,
+                             (float)c.coherence_gate, e2, fmax, (const NbJob*)djobs);
+          if (!X.launched()) return X.rc;
+          // the job tables and scratch are reused by the next chunk
+          if (hipStreamSynchronize(st) != hipSuccess) return fail(ctx, HVD_GPU_ERR_HIP, "sync failed");
         }
       }
       // (f) the PSI init (sequence.cpp:1023-1025)
