@@ -72,6 +72,11 @@ inline float IreToSample(float ire, float blanking_level, float white_level) {
   return blanking_level + ire * CodesPerIre(blanking_level, white_level);
 }
 
+// Robust per-field noise (IRE) from the stride-4 horizontal second difference
+// (ntsc_geometry.h:278-283, ntsc_geometry.cpp:49-71), computed on the GPU;
+// 0 for planes of 8 columns or fewer (and on failure, see GpuLastError()).
+float EstimateNoiseIre(const Plane& s);
+
 // Explicit carrier planes, theta.size() x active_width (computed on the GPU).
 ComplexPlane MakeCarrier(const std::vector<float>& theta, const FieldGeometry& g);
 ComplexPlane MakeCarrierPal(const std::vector<float>& theta, const std::vector<int8_t>& vswitch,

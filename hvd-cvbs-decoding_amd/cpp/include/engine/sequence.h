@@ -50,6 +50,19 @@ struct SequenceDiagnostics {
 std::vector<DecodedField> DecodeFieldWindow(const std::vector<FieldObs>& fields, const FieldGeometry& g,
                                             const HvdConfig& cfg, Fft2d* fft, SequenceDiagnostics* diag = nullptr);
 
+// Drizzle stacking of fields j0 and j0 + 1 onto a (2 lines scale) x width
+// grid (sequence.h:105-130, sequence.cpp:283-421): window neighbours matched
+// by EstimateMotion (cfg.mc_tile, cfg.mc_search) within 2 cfg.nr_radius
+// fields, Geman-McClure weights at cfg.nr_eps (3 when unset), blended with
+// the woven interpolation where coverage is thin.  On the GPU; failure:
+// empty planes (GpuLastError()).
+struct DrizzleResult {
+  Plane luma;
+  ComplexPlane chroma;
+};
+DrizzleResult DrizzleFrame(int j0, const std::vector<Plane>& Ys, const std::vector<ComplexPlane>& chis,
+                           const std::vector<int>& parities, const HvdConfig& cfg, int scale = 2);
+
 // The same driver from the caller's inits (sequence.h:104-135, :139-142):
 // inits index-aligned with fields; an empty luma means Y0 = S - Re[chi0 c].
 std::vector<DecodedField> DecodeFieldWindowWithInits(const std::vector<FieldObs>& fields,

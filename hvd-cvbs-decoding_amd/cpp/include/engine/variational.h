@@ -30,6 +30,24 @@ struct NeighborTerm {
 RefineResult VariationalRefine(const Plane& s, const ComplexPlane& carrier, const ComplexPlane& chi0,
                                const HvdConfig& cfg, const std::vector<NeighborTerm>& neighbors = {});
 
+// The anchor of the joint pass (variational.h:80-90): a synthesized reference
+// (Y_a, chi_a) and its per-pixel weight.
+struct AnchorTerm {
+  Plane luma;
+  ComplexPlane chroma;
+  Plane weight;
+};
+
+// The pass-2+ solver with Y and chi both free (variational.h:92-109,
+// variational.cpp:558-891): the data, neighbour (strength
+// cfg.temporal_strength, robust scale cfg.temporal_eps, which must be
+// positive when neighbours are given) and anchor (weight cfg.nr_anchor) terms
+// and the spatial priors; y0 and chi0 start the solve.  On the GPU
+// (hvd_gpu_variational_refine_joint); failure: empty planes.
+RefineResult VariationalRefineJoint(const Plane& s, const ComplexPlane& carrier, const Plane& y0,
+                                    const ComplexPlane& chi0, const HvdConfig& cfg,
+                                    const std::vector<NeighborTerm>& neighbors, const AnchorTerm* anchor);
+
 }  // namespace hvd
 
 #endif  // HVD_AMD_ENGINE_VARIATIONAL_H_

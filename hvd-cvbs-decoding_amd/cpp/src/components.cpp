@@ -285,6 +285,180 @@ MotionField EstimateMotion(const Plane& y_ref, const Plane& y_cur, int tile, int
   }
 }
 
+RefineResult VariationalRefineJoint(const Plane& s, const ComplexPlane& carrier, const Plane& y0,
+                                    const ComplexPlane& chi0, const HvdConfig& cfg,
+                                    const std::vector<NeighborTerm>& neighbors, const AnchorTerm* anchor) {
+  try {
+    auto same = [&](int h, int w) { return h == s.height() && w == s.width(); };
+    if (!same(carrier.height(), carrier.width()) || !same(chi0.height(), chi0.width()) ||
+        !same(y0.height(), y0.width()))
+      throw gpu::Error("carrier / y0 / chi0 size differs from S");
+    for (const NeighborTerm& nb : neighbors)
+      if (!same(nb.composite.height(), nb.composite.width()) || !same(nb.carrier.height(), nb.carrier.width()) ||
+          !same(nb.confidence.height(), nb.confidence.width()))
+        throw gpu::Error("neighbour plane size differs from S");
+    if (anchor && (!same(anchor->luma.height(), anchor->luma.width()) ||
+                   !same(anchor->chroma.height(), anchor->chroma.width()) ||
+                   !same(anchor->weight.height(), anchor->weight.width())))
+      throw gpu::Error("anchor plane size differs from S");
+    RefineResult out;
+    out.luma = Plane(s.height(), s.width());
+    out.chroma = ComplexPlane(s.height(), s.width());
+    if (s.empty()) return out;
+    Session& ss = gpu::ThreadSession();
+    DevBuf<float> ds = gpu::Upload(s.data(), s.size());
+    DevBuf<float> dc = gpu::Upload(reinterpret_cast<const float*>(carrier.data()), carrier.size() * 2);
+    DevBuf<float> chi = gpu::Upload(reinterpret_cast<const float*>(chi0.data()), chi0.size() * 2);
+    DevBuf<float> y = gpu::Upload(y0.data(), y0.size());
+    FieldGeometry g;
+    const hvd_gpu_plane p = PlaneOf(s, ds, dc, g, chi.get(), y.get());
+    const hvd_solver_params sp = gpu::ToAbi(cfg, cfg.is_pal);
+    std::vector<DevBuf<float>> bufs;
+    std::vector<hvd_gpu_neighbor> nb(neighbors.size());
+    bufs.reserve(3 * neighbors.size() + 3);
+    for (size_t j = 0; j < neighbors.size(); ++j) {
+      const NeighborTerm& t = neighbors[j];
+      bufs.push_back(gpu::Upload(t.composite.data(), t.composite.size()));
+      nb[j].composite = bufs.back().get();
+      bufs.push_back(gpu::Upload(reinterpret_cast<const float*>(t.carrier.data()), t.carrier.size() * 2));
+      nb[j].carrier = bufs.back().get();
+      bufs.push_back(gpu::Upload(t.confidence.data(), t.confidence.size()));
+      nb[j].confidence = bufs.back().get();
+    }
+    hvd_gpu_anchor an{};
+    if (anchor) {
+      bufs.push_back(gpu::Upload(anchor->luma.data(), anchor->luma.size()));
+      an.luma = bufs.back().get();
+      bufs.push_back(gpu::Upload(reinterpret_cast<const float*>(anchor->chroma.data()), anchor->chroma.size() * 2));
+      an.chroma = bufs.back().get();
+      bufs.push_back(gpu::Upload(anchor->weight.data(), anchor->weight.size()));
+      an.weight = bufs.back().get();
+    }
+    const int nn = (int)nb.size();
+    // nu = temporal_strength with neighbours, nu_a = nr_anchor with an
+    // anchor (variational.cpp:581-582)
+    ss.check(hvd_gpu_variational_refine_joint(ss.ctx(), &p, 1, nb.empty() ? nullptr : nb.data(), &nn,
+                                              anchor ? &an : nullptr, &sp, cfg.temporal_strength, cfg.temporal_eps,
+                                              anchor ? cfg.nr_anchor : 0.0F, nullptr, nullptr),
+             "hvd_gpu_variational_refine_joint");
+    y.download(out.luma.data(), s.size());
+    chi.download(reinterpret_cast<float*>(out.chroma.data()), s.size() * 2);
+    return out;
+  } catch (const std::exception& e) {
+    gpu::Report("VariationalRefineJoint", e);
+    return {};
+  }
+}
+
+MotionField VerifyMotion(const Plane& y_ref, const Plane& y_cur, const Plane& pdy, const Plane& pdx, int tile) {
+  try {
+    if (y_ref.height() != y_cur.height() || y_ref.width() != y_cur.width() || y_ref.empty())
+      throw gpu::Error("reference and current planes differ in size or are empty");
+    if (tile < 1) throw gpu::Error("tile must be positive");
+    const int th = (y_ref.height() + tile - 1) / tile, tw = (y_ref.width() + tile - 1) / tile;
+    if (pdy.height() != th || pdy.width() != tw || pdx.height() != th || pdx.width() != tw)
+      throw gpu::Error("prediction planes are not on the tile grid");
+    Session& s = gpu::ThreadSession();
+    DevBuf<float> a = gpu::Upload(y_ref.data(), y_ref.size()), b = gpu::Upload(y_cur.data(), y_cur.size());
+    DevBuf<float> py = gpu::Upload(pdy.data(), pdy.size()), px = gpu::Upload(pdx.data(), pdx.size());
+    DevBuf<float> dy((size_t)th * tw), dx((size_t)th * tw), cf((size_t)th * tw);
+    const float* ra[1] = {a.get()};
+    const float* rb[1] = {b.get()};
+    const float* rpy[1] = {py.get()};
+    const float* rpx[1] = {px.get()};
+    float* ody[1] = {dy.get()};
+    float* odx[1] = {dx.get()};
+    float* ocf[1] = {cf.get()};
+    s.check(hvd_gpu_verify_motion(s.ctx(), ra, rb, 1, y_ref.height(), y_ref.width(), rpy, rpx, tile, ody, odx, ocf,
+                                  nullptr),
+            "hvd_gpu_verify_motion");
+    MotionField m;
+    m.tile = tile;
+    m.dy = Plane(th, tw);
+    m.dx = Plane(th, tw);
+    m.confidence = Plane(th, tw);
+    dy.download(m.dy.data(), m.dy.size());
+    dx.download(m.dx.data(), m.dx.size());
+    cf.download(m.confidence.data(), m.confidence.size());
+    return m;
+  } catch (const std::exception& e) {
+    gpu::Report("VerifyMotion", e);
+    return {};
+  }
+}
+
+namespace {
+
+// The measurements' planes on the device for the trajectory entry points.
+struct TrajectoryBufs {
+  std::vector<DevBuf<float>> dy, dx, cf;
+  std::vector<int> off;
+  std::vector<float> shift;
+  int th = 0, tw = 0;
+  explicit TrajectoryBufs(const std::vector<OffsetMotion>& ms) {
+    th = ms.front().field.dy.height();
+    tw = ms.front().field.dy.width();
+    for (const OffsetMotion& m : ms) {
+      const MotionField& f = m.field;
+      if (f.dy.height() != th || f.dy.width() != tw || f.dx.height() != th || f.dx.width() != tw ||
+          f.confidence.height() != th || f.confidence.width() != tw)
+        throw gpu::Error("measurements on different tile grids");
+      dy.push_back(gpu::Upload(f.dy.data(), f.dy.size()));
+      dx.push_back(gpu::Upload(f.dx.data(), f.dx.size()));
+      cf.push_back(gpu::Upload(f.confidence.data(), f.confidence.size()));
+      off.push_back(m.offset);
+      shift.push_back(m.parity_shift);
+    }
+  }
+  template <typename P>
+  std::vector<P> ptrs(std::vector<DevBuf<float>>& v) const {
+    std::vector<P> p;
+    for (auto& b : v) p.push_back(b.get());
+    return p;
+  }
+};
+
+}  // namespace
+
+void FitTrajectory(const std::vector<OffsetMotion>& measurements, Plane* vy, Plane* vx) {
+  if (measurements.empty() || !vy || !vx) return;
+  try {
+    TrajectoryBufs B(measurements);
+    Session& s = gpu::ThreadSession();
+    DevBuf<float> oy((size_t)B.th * B.tw), ox((size_t)B.th * B.tw);
+    auto pdy = B.ptrs<const float*>(B.dy), pdx = B.ptrs<const float*>(B.dx), pcf = B.ptrs<const float*>(B.cf);
+    s.check(hvd_gpu_trajectory_fit(s.ctx(), (int)measurements.size(), B.off.data(), B.shift.data(), pdy.data(),
+                                   pdx.data(), pcf.data(), B.th, B.tw, oy.get(), ox.get(), nullptr),
+            "hvd_gpu_trajectory_fit");
+    *vy = Plane(B.th, B.tw);
+    *vx = Plane(B.th, B.tw);
+    oy.download(vy->data(), vy->size());
+    ox.download(vx->data(), vx->size());
+  } catch (const std::exception& e) {
+    gpu::Report("FitTrajectory", e);
+  }
+}
+
+void TrajectorySnap(std::vector<OffsetMotion>* measurements) {
+  if (!measurements || measurements->size() < 2) return;
+  try {
+    TrajectoryBufs B(*measurements);
+    Session& s = gpu::ThreadSession();
+    auto pdy = B.ptrs<float*>(B.dy), pdx = B.ptrs<float*>(B.dx);
+    auto pcf = B.ptrs<const float*>(B.cf);
+    s.check(hvd_gpu_trajectory_snap(s.ctx(), (int)measurements->size(), B.off.data(), B.shift.data(), pdy.data(),
+                                    pdx.data(), pcf.data(), B.th, B.tw, nullptr),
+            "hvd_gpu_trajectory_snap");
+    for (size_t j = 0; j < measurements->size(); ++j) {
+      MotionField& f = (*measurements)[j].field;
+      B.dy[j].download(f.dy.data(), f.dy.size());
+      B.dx[j].download(f.dx.data(), f.dx.size());
+    }
+  } catch (const std::exception& e) {
+    gpu::Report("TrajectorySnap", e);
+  }
+}
+
 Plane BoxBlur(Plane a, int r) {
   try {
     if (a.empty() || r <= 0) return a;

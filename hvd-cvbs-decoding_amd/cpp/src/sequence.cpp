@@ -166,6 +166,42 @@ std::vector<DecodedField> DecodeFieldWindowWithInits(const std::vector<FieldObs>
   }
 }
 
+DrizzleResult DrizzleFrame(int j0, const std::vector<Plane>& Ys, const std::vector<ComplexPlane>& chis,
+                           const std::vector<int>& parities, const HvdConfig& cfg, int scale) {
+  try {
+    const int n = static_cast<int>(Ys.size());
+    if (j0 < 0 || j0 + 1 >= n || chis.size() != Ys.size() || parities.size() != Ys.size())
+      throw gpu::Error("j0, j0 + 1 must index a window of equally many Y, chi and parities");
+    const int L = Ys[j0].height(), W = Ys[j0].width();
+    for (int k = 0; k < n; ++k)
+      if (Ys[k].height() != L || Ys[k].width() != W || chis[k].height() != L || chis[k].width() != W)
+        throw gpu::Error("all fields of the window must share one shape");
+    Session& ss = gpu::ThreadSession();
+    std::vector<DevBuf<float>> dy(n), dc(n);
+    std::vector<const float*> py(n), pc(n);
+    for (int k = 0; k < n; ++k) {
+      dy[k] = gpu::Upload(Ys[k].data(), Ys[k].size());
+      dc[k] = gpu::Upload(reinterpret_cast<const float*>(chis[k].data()), chis[k].size() * 2);
+      py[k] = dy[k].get();
+      pc[k] = dc[k].get();
+    }
+    const size_t nfine = (size_t)2 * L * scale * W;
+    DevBuf<float> oy(nfine), oc(2 * nfine);
+    ss.check(hvd_gpu_drizzle_frame(ss.ctx(), j0, n, py.data(), pc.data(), parities.data(), L, W, cfg.mc_tile,
+                                   cfg.mc_search, cfg.nr_radius, cfg.nr_eps, scale, oy.get(), oc.get(), nullptr),
+             "hvd_gpu_drizzle_frame");
+    DrizzleResult r;
+    r.luma = Plane(2 * L * scale, W);
+    r.chroma = ComplexPlane(2 * L * scale, W);
+    oy.download(r.luma.data(), nfine);
+    oc.download(reinterpret_cast<float*>(r.chroma.data()), 2 * nfine);
+    return r;
+  } catch (const std::exception& e) {
+    gpu::Report("DrizzleFrame", e);
+    return {};
+  }
+}
+
 std::vector<DecodedField> HvdEngine::DecodeSequenceWindow(const std::vector<FieldObs>& fields,
                                                           const FieldGeometry& g, const HvdConfig& cfg,
                                                           SequenceDiagnostics* diag) {

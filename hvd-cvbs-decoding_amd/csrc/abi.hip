@@ -1144,7 +1144,7 @@ int hvd_gpu_create(int device, hvd_gpu_ctx** out) {
     HIPCHK(ctx, hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
     ctx->max_dyn_lds = lds;
     for (const void* k : {(const void*)k_fft_rows_fwd, (const void*)k_fft_cols_fwd, (const void*)k_fft_cols_inv,
-                          (const void*)k_fft_rows_inv})
+                          (const void*)k_fft_rows_inv, (const void*)k_fft2_rows, (const void*)k_fft2_cols})
       HIPCHK(ctx, hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
     HIPCHK(ctx, hipFuncSetAttribute((const void*)k_aniso_select, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     std::max(0, lds - kSelStaticLds)));
@@ -2738,6 +2738,75 @@ int hvd_gpu_estimate_motion(hvd_gpu_ctx* ctx, const float* const* y_ref, const f
   });
 }
 
+namespace {
+
+// The device tables of the trajectory entry points: m pointers each of dy,
+// dx, conf, then m shifts and m offsets.
+char* trajectory_tables(hvd_gpu_ctx* ctx, int m, const int* offsets, const float* shifts, const void* const* dy,
+                        const void* const* dx, const float* const* conf, hipStream_t st, Slot** sl, size_t* o) {
+  REQUIRE(ctx, m >= 0 && m <= kMaxNeighbors, "%d measurements (at most %d)", m, kMaxNeighbors);
+  REQUIRE(ctx, m == 0 || (offsets && shifts && dy && dx && conf), "null argument");
+  for (int j = 0; j < m; ++j) REQUIRE(ctx, dy[j] && dx[j] && conf[j], "measurement %d: null plane", j);
+  Stage stg;
+  o[0] = stg.put(dy, m);
+  o[1] = stg.put(dx, m);
+  o[2] = stg.put(conf, m);
+  o[3] = stg.put(shifts, m);
+  o[4] = stg.put(offsets, m);
+  return upload(ctx, stg, st, sl);
+}
+
+}  // namespace
+
+int hvd_gpu_trajectory_fit(hvd_gpu_ctx* ctx, int m, const int* offsets, const float* parity_shifts,
+                           const float* const* dy, const float* const* dx, const float* const* conf, int tiles_h,
+                           int tiles_w, float* vy, float* vx, void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, vy && vx && tiles_h >= 1 && tiles_w >= 1, "bad output grid");
+    hipStream_t st = S(stream);
+    const int nt = tiles_h * tiles_w;
+    Slot* sl;
+    size_t o[5];
+    char* base = trajectory_tables(ctx, m, offsets, parity_shifts, (const void* const*)dy, (const void* const*)dx,
+                                   conf, st, &sl, o);
+    hipLaunchKernelGGL(k_traj_fit, dim3((nt + 63) / 64), dim3(64), 0, st, m, nt, (const float* const*)(base + o[0]),
+                       (const float* const*)(base + o[1]), (const float* const*)(base + o[2]),
+                       (const float*)(base + o[3]), (const int*)(base + o[4]), vy, vx);
+    LAUNCH_CHECK(ctx);
+    arm(ctx, sl, st);
+  });
+}
+
+int hvd_gpu_trajectory_snap(hvd_gpu_ctx* ctx, int m, const int* offsets, const float* parity_shifts, float* const* dy,
+                            float* const* dx, const float* const* conf, int tiles_h, int tiles_w, void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, tiles_h >= 1 && tiles_w >= 1, "bad tile grid");
+    if (m < 2) return;
+    hipStream_t st = S(stream);
+    const int nt = tiles_h * tiles_w;
+    Slot* sl;
+    size_t o[5];
+    char* base = trajectory_tables(ctx, m, offsets, parity_shifts, (const void* const*)dy, (const void* const*)dx,
+                                   conf, st, &sl, o);
+    hipLaunchKernelGGL(k_traj_snap, dim3((nt + 63) / 64), dim3(64), 0, st, m, nt, (float* const*)(base + o[0]),
+                       (float* const*)(base + o[1]), (const float* const*)(base + o[2]), (const float*)(base + o[3]),
+                       (const int*)(base + o[4]));
+    LAUNCH_CHECK(ctx);
+    arm(ctx, sl, st);
+  });
+}
+
+int hvd_gpu_drizzle_frame(hvd_gpu_ctx* ctx, int j0, int n, const float* const* ys, const float* const* chis,
+                          const int* parities, int lines, int width, int tile, int search, int nr_radius,
+                          float nr_eps, int scale, float* out_luma, float* out_chroma, void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, !ctx->no_alloc, "internal: drizzle inside a no-allocation call");
+    const int rc = hvd_window::drizzle(ctx, j0, n, ys, chis, parities, lines, width, tile, search, nr_radius, nr_eps,
+                                       scale, out_luma, out_chroma, S(stream));
+    if (rc != HVD_GPU_OK) throw Fail{rc};
+  });
+}
+
 int hvd_gpu_decode_window(hvd_gpu_ctx* ctx, const hvd_gpu_plane* fields, int n, const int* parities, int is_pal,
                           const int* y_given, const hvd_solver_params* p, const hvd_window_params* wp,
                           hvd_window_diag* diag, void* stream) {
@@ -2892,6 +2961,75 @@ int hvd_gpu_tridiag_smooth(hvd_gpu_ctx* ctx, const float* d, const float* a, int
     char* ws = ensure_ws(ctx, cv.off, st);
     hipLaunchKernelGGL(k_tridiag, dim3(1), dim3(64), 0, st, d, a, n, lam, (double*)(ws + o), out);
     LAUNCH_CHECK(ctx);
+  });
+}
+
+int hvd_gpu_difference(hvd_gpu_ctx* ctx, int op, int is_complex, const float* a, const float* weight, int h, int w,
+                       float* d, void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, op >= HVD_DIFF_DX && op <= HVD_DIFF_D2T, "unknown difference operator %d", op);
+    REQUIRE(ctx, h >= 0 && w >= 0, "bad plane %d x %d", h, w);
+    if ((size_t)h * w == 0) return;
+    REQUIRE(ctx, a && d && a != d, "null or aliased plane");
+    REQUIRE(ctx, !weight || op == HVD_DIFF_DX || op == HVD_DIFF_DY || op == HVD_DIFF_D1 || op == HVD_DIFF_D2,
+            "a weight applies to Dx, Dy, D1 and D2 only");
+    hipStream_t st = S(stream);
+    hipLaunchKernelGGL(k_difference, dim3(blocks_for((size_t)h * w)), dim3(256), 0, st, op, is_complex ? 2 : 1, a,
+                       weight, h, w, d);
+    LAUNCH_CHECK(ctx);
+  });
+}
+
+int hvd_gpu_estimate_noise_ire(hvd_gpu_ctx* ctx, const float* s, int h, int w, float* sigma, void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, s && sigma, "null argument");
+    REQUIRE(ctx, h >= 0 && w >= 0, "bad plane %d x %d", h, w);
+    if (w <= 8 || h <= 0) {
+      *sigma = 0.0f;
+      return;
+    }
+    hipStream_t st = S(stream);
+    Carve cv;
+    const size_t o_d = cv.take(sizeof(float) * (size_t)h * (w - 8)), o_sig = cv.take(sizeof(float));
+    char* ws = ensure_ws(ctx, cv.off, st);
+    const float* tab[1] = {s};
+    Stage stg;
+    const size_t oT = stg.put(tab, 1);
+    Slot* sl;
+    char* base = upload(ctx, stg, st, &sl);
+    hipLaunchKernelGGL(k_noise_sigma, dim3(1), dim3(1024), 0, st, (const float* const*)(base + oT), h, w,
+                       (float*)(ws + o_d), (float*)(ws + o_sig));
+    LAUNCH_CHECK(ctx);
+    arm(ctx, sl, st);
+    HIPCHK(ctx, hipMemcpyAsync(sigma, ws + o_sig, sizeof(float), hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx, hipStreamSynchronize(st));
+  });
+}
+
+int hvd_gpu_fft2d(hvd_gpu_ctx* ctx, const float* in, int h, int w, int inverse, float* out, void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, in && out, "null argument");
+    REQUIRE(ctx, h >= 1 && w >= 1 && h <= 1024 && w <= 4096, "FFT plane %d x %d outside 1..1024 x 1..4096", h, w);
+    hipStream_t st = S(stream);
+    PlanSet ps;
+    const int pr = ps.get(ctx, w), pc = ps.get(ctx, h);
+    Carve cv;
+    const size_t o_roots = cv.take(sizeof(float2) * ps.roots.size());
+    char* ws = ensure_ws(ctx, cv.off, st);
+    float2* roots = (float2*)(ws + o_roots);
+    HIPCHK(ctx, hipMemcpyAsync(roots, ps.roots.data(), sizeof(float2) * ps.roots.size(), hipMemcpyHostToDevice, st));
+    const size_t lds_r = sizeof(float2) * 3 * (size_t)w, lds_c = sizeof(float2) * (2 * kColGroup + 1) * (size_t)h;
+    REQUIRE(ctx, (int)std::max(lds_r, lds_c) <= ctx->max_dyn_lds, "FFT plane %d x %d needs more LDS than a block has",
+            h, w);
+    const int sign = inverse ? 1 : -1;
+    hipLaunchKernelGGL(k_fft2_rows, dim3(h), dim3(256), lds_r, st, (const float2*)in, (float2*)out, h, w,
+                       ps.plans[pr], (const float2*)roots, sign);
+    LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_fft2_cols, dim3((w + kColGroup - 1) / kColGroup), dim3(256), lds_c, st, (float2*)out, h, w,
+                       ps.plans[pc], (const float2*)roots, sign, inverse ? (float)(1.0 / ((double)h * w)) : 1.0f);
+    LAUNCH_CHECK(ctx);
+    // the roots are read by the launches above before a later call reuses ws
+    HIPCHK(ctx, hipStreamSynchronize(st));
   });
 }
 

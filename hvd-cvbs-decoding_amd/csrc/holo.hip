@@ -471,4 +471,47 @@ __global__ void __launch_bounds__(256) k_holo_blend(const InitField* F, const in
   }
 }
 
+// ---- the numpy-convention 2-D DFT of engine/fft2d.h (fft2d.cpp:88-118) ----
+// The 2-D DFT's row pass: one block per row, the row's FFT in LDS (sign -1
+// forward, +1 inverse).  LDS: 3 n float2.
+__global__ void __launch_bounds__(256) k_fft2_rows(const float2* in, float2* out, int h, int w, FftPlan P,
+                                                   const float2* roots, int sign) {
+  extern __shared__ float2 lds[];
+  const int y = blockIdx.x;
+  float2* a = lds;
+  float2* b = lds + P.n;
+  float2* rt = lds + 2 * P.n;
+  for (int j = threadIdx.x; j < P.n; j += blockDim.x) rt[j] = roots[P.root_off + j];
+  for (int x = threadIdx.x; x < w; x += blockDim.x) a[x] = in[(size_t)y * w + x];
+  __syncthreads();
+  const float2* r = fft_lds(a, b, rt, P, sign, 1);
+  for (int x = threadIdx.x; x < w; x += blockDim.x) out[(size_t)y * w + x] = r[x];
+}
+
+// The column pass, in place, kColGroup columns per block, each output times
+// `scale` (1 forward, 1 / (h w) inverse).  LDS: (2 kColGroup + 1) n float2.
+__global__ void __launch_bounds__(256) k_fft2_cols(float2* io, int h, int w, FftPlan P, const float2* roots, int sign,
+                                                   float scale) {
+  extern __shared__ float2 lds[];
+  const int x0 = blockIdx.x * kColGroup;
+  const int nc = min(kColGroup, w - x0);
+  float2* a = lds;
+  float2* b = lds + P.n * kColGroup;
+  float2* rt = lds + 2 * P.n * kColGroup;
+  for (int j = threadIdx.x; j < P.n; j += blockDim.x) rt[j] = roots[P.root_off + j];
+  for (int e = threadIdx.x; e < P.n * kColGroup; e += blockDim.x) {
+    const int c = e % kColGroup, y = e / kColGroup;
+    a[e] = c < nc ? io[(size_t)y * w + x0 + c] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  const float2* r = fft_lds(a, b, rt, P, sign, kColGroup);
+  for (int e = threadIdx.x; e < P.n * kColGroup; e += blockDim.x) {
+    const int c = e % kColGroup, y = e / kColGroup;
+    if (c < nc) {
+      const float2 v = r[e];
+      io[(size_t)y * w + x0 + c] = scale == 1.0f ? v : make_float2(v.x * scale, v.y * scale);
+    }
+  }
+}
+
 }  // namespace hvdk

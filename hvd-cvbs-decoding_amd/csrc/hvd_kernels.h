@@ -132,6 +132,9 @@ __global__ void k_fft_cols_inv(const InitField* F, const FftPlan* plans, const f
                                int nf);
 __global__ void k_fft_rows_inv(const InitField* F, const FftPlan* plans, const float2* roots, const int* row_prefix,
                                int nf);
+__global__ void k_fft2_rows(const float2* in, float2* out, int h, int w, FftPlan P, const float2* roots, int sign);
+__global__ void k_fft2_cols(float2* io, int h, int w, FftPlan P, const float2* roots, int sign, float scale);
+__global__ void k_difference(int op, int nc, const float* a, const float* wt, int h, int w, float* d);
 __global__ void k_holo_blend(const InitField* F, const int* tp, int nf);
 
 // solve.hip

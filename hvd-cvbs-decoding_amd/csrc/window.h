@@ -65,6 +65,8 @@ __global__ void k_synth_weight(size_t n, const float* r, float eps_b, float* wei
 __global__ void k_psi(size_t n, int nn, const float* s, const float2* car, const float* const* sw,
                       const float2* const* cw, const float* const* conf, float2* chi);
 
+__global__ void k_traj_fit(int m, int nt, const float* const* dy, const float* const* dx, const float* const* conf,
+                           const float* ho, const int* off, float* vy, float* vx);
 __global__ void k_nb_square(int nt, const NbJob* J);
 __global__ void k_nb_snap(int th, int tw, int fast, const NbJob* J);
 __global__ void k_nb_warp(int h, int w, int th, int tw, int tile, int fast, int gate, const NbJob* J);
@@ -79,6 +81,10 @@ namespace hvd_window {
 char* pool(hvd_gpu_ctx* ctx, size_t bytes, hipStream_t st);
 // Records a failure message on the context; returns code.
 int fail(hvd_gpu_ctx* ctx, int code, const char* msg);
+// DrizzleFrame (drizzle.hip): 0 or an HVD_GPU_ERR_* code.
+int drizzle(hvd_gpu_ctx* ctx, int j0, int n, const float* const* Ys, const float* const* chis, const int* par,
+            int L, int W, int tile, int search, int nr_radius, float nr_eps, int scale, float* out_luma,
+            float* out_chroma, hipStream_t st);
 // The window driver (window.hip): 0 or an HVD_GPU_ERR_* code.
 int drive(hvd_gpu_ctx* ctx, const hvd_gpu_plane* fields, int n, const int* parities, int is_pal, const int* y_given,
           const hvd_solver_params* p, const hvd_window_params* wp, hvd_window_diag* diag, hipStream_t st);

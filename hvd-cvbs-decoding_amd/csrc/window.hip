@@ -268,14 +268,14 @@ __global__ void k_traj_extrapolate(int nt, const float* vy, const float* vx, flo
   pdx[t] = o * vx[t];
 }
 
-// TrajectorySnap (motion.cpp:527-601) over one field's m measurements: the
-// fit (median of confident (dy - h)/o, dx/o), the consensus count of those
-// within 1.5 px, and the snap where >= 3 agree.  In place on dy, dx.
-__global__ void k_traj_snap(int m, int nt, float* const* dy, float* const* dx, const float* const* conf,
-                            const float* ho, const int* off) {
+namespace {
+
+// FitTrajectory (motion.cpp:532-562) at tile t: the numpy median of the
+// confident (conf > 0.15, offset != 0) (dy - h) / o and dx / o; 0 where no
+// measurement qualifies.
+__device__ inline void traj_fit(int m, int t, const float* const* dy, const float* const* dx,
+                                const float* const* conf, const float* ho, const int* off, float& vy, float& vx) {
 #pragma clang fp contract(off)
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nt || m < 2) return;
   float sy[kMaxNeighbors], sx[kMaxNeighbors];
   int k = 0;
   for (int j = 0; j < m; ++j) {
@@ -285,27 +285,52 @@ __global__ void k_traj_snap(int m, int nt, float* const* dy, float* const* dx, c
     sx[k] = dx[j][t] / o;
     ++k;
   }
-  float vy = 0.f, vx = 0.f;
-  if (k > 0) {
-    for (int a = 1; a < k; ++a)
-      for (int b = a; b > 0 && sy[b] < sy[b - 1]; --b) {
-        const float z = sy[b];
-        sy[b] = sy[b - 1];
-        sy[b - 1] = z;
-      }
-    for (int a = 1; a < k; ++a)
-      for (int b = a; b > 0 && sx[b] < sx[b - 1]; --b) {
-        const float z = sx[b];
-        sx[b] = sx[b - 1];
-        sx[b - 1] = z;
-      }
-    vy = sy[k / 2];
-    vx = sx[k / 2];
-    if (k % 2 == 0) {
-      vy = 0.5f * (vy + sy[k / 2 - 1]);
-      vx = 0.5f * (vx + sx[k / 2 - 1]);
+  vy = 0.f;
+  vx = 0.f;
+  if (k == 0) return;
+  for (int a = 1; a < k; ++a)
+    for (int b = a; b > 0 && sy[b] < sy[b - 1]; --b) {
+      const float z = sy[b];
+      sy[b] = sy[b - 1];
+      sy[b - 1] = z;
     }
+  for (int a = 1; a < k; ++a)
+    for (int b = a; b > 0 && sx[b] < sx[b - 1]; --b) {
+      const float z = sx[b];
+      sx[b] = sx[b - 1];
+      sx[b - 1] = z;
+    }
+  vy = sy[k / 2];
+  vx = sx[k / 2];
+  if (k % 2 == 0) {
+    vy = 0.5f * (vy + sy[k / 2 - 1]);
+    vx = 0.5f * (vx + sx[k / 2 - 1]);
   }
+}
+
+}  // namespace
+
+// FitTrajectory over m measurements of nt tiles into vy, vx.
+__global__ void k_traj_fit(int m, int nt, const float* const* dy, const float* const* dx, const float* const* conf,
+                           const float* ho, const int* off, float* vy, float* vx) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  float a = 0.f, b = 0.f;
+  if (m > 0) traj_fit(m, t, dy, dx, conf, ho, off, a, b);
+  vy[t] = a;
+  vx[t] = b;
+}
+
+// TrajectorySnap (motion.cpp:564-601) over one field's m measurements: the
+// fit, the consensus count of those within 1.5 px, and the snap where >= 3
+// agree.  In place on dy, dx.
+__global__ void k_traj_snap(int m, int nt, float* const* dy, float* const* dx, const float* const* conf,
+                            const float* ho, const int* off) {
+#pragma clang fp contract(off)
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt || m < 2) return;
+  float vy, vx;
+  traj_fit(m, t, (const float* const*)dy, (const float* const*)dx, conf, ho, off, vy, vx);
   float agree = 0.f;
   for (int j = 0; j < m; ++j) {
     const float o = (float)off[j];

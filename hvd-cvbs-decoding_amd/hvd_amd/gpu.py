@@ -31,6 +31,8 @@ ABI_SYMBOLS = (
     "hvd_gpu_motion_compensate_prev", "hvd_gpu_variational_refine_noalloc", "hvd_gpu_woven_size",
     "hvd_gpu_woven_picture", "hvd_gpu_variational_refine_temporal_batch", "hvd_gpu_variational_refine_joint",
     "hvd_gpu_verify_motion", "hvd_gpu_decode_window", "hvd_gpu_decode_frame_window_ex_s16",
+    "hvd_gpu_difference", "hvd_gpu_estimate_noise_ire", "hvd_gpu_fft2d", "hvd_gpu_trajectory_fit",
+    "hvd_gpu_trajectory_snap", "hvd_gpu_drizzle_frame",
 )
 
 
@@ -178,6 +180,14 @@ def load() -> C.CDLL:
                                                          C.POINTER(SolverParams), C.c_float, C.c_float, C.c_float,
                                                          vp, vp]
         lib.hvd_gpu_verify_motion.argtypes = [vp, pp, pp, i, i, i, pp, pp, i, pp, pp, pp, vp]
+        lib.hvd_gpu_difference.argtypes = [vp, i, i, vp, vp, i, i, vp, vp]
+        lib.hvd_gpu_estimate_noise_ire.argtypes = [vp, vp, i, i, C.POINTER(C.c_float), vp]
+        lib.hvd_gpu_fft2d.argtypes = [vp, vp, i, i, i, vp, vp]
+        lib.hvd_gpu_trajectory_fit.argtypes = [vp, i, C.POINTER(C.c_int), C.POINTER(C.c_float), pp, pp, pp, i, i,
+                                               vp, vp, vp]
+        lib.hvd_gpu_trajectory_snap.argtypes = [vp, i, C.POINTER(C.c_int), C.POINTER(C.c_float), pp, pp, pp, i, i, vp]
+        lib.hvd_gpu_drizzle_frame.argtypes = [vp, i, i, pp, pp, C.POINTER(C.c_int), i, i, i, i, i, C.c_float, i, vp,
+                                              vp, vp]
         lib.hvd_gpu_decode_window.argtypes = [vp, C.POINTER(Plane), i, C.POINTER(C.c_int), i, C.POINTER(C.c_int),
                                               C.POINTER(SolverParams), C.POINTER(WindowParams),
                                               C.POINTER(WindowDiag), vp]
@@ -419,6 +429,65 @@ class Engine:
         if want_stats:
             return np.array(iters[:], dtype=np.int32).reshape(len(planes), outer)
         return None
+
+    # ---- gradients.h, EstimateNoiseIre, Fft2d -----------------------------------
+    DIFF_OPS = {"Dx": 0, "Dy": 1, "DxT": 2, "DyT": 3, "D1": 4, "D1T": 5, "D2": 6, "D2T": 7}
+
+    def difference(self, op, a, weight=None):
+        """gradients.h's op on a (h, w) float32 or (h, w, 2) complex device tensor."""
+        import torch
+        cplx = a.dim() == 3
+        h, w = a.shape[0], a.shape[1]
+        out = torch.empty_like(a)
+        self._check(self.lib.hvd_gpu_difference(self.ctx, self.DIFF_OPS[op], int(cplx), a.data_ptr(),
+                                                None if weight is None else weight.data_ptr(), h, w, out.data_ptr(),
+                                                _stream()), "difference")
+        return out
+
+    def estimate_noise_ire(self, s):
+        v = C.c_float(0.0)
+        self._check(self.lib.hvd_gpu_estimate_noise_ire(self.ctx, s.data_ptr(), s.shape[0], s.shape[1], C.byref(v),
+                                                        _stream()), "estimate_noise_ire")
+        return v.value
+
+    def fft2d(self, z, inverse=False):
+        """(h, w, 2) float32 device tensor -> its numpy-convention 2-D DFT."""
+        import torch
+        out = torch.empty_like(z)
+        self._check(self.lib.hvd_gpu_fft2d(self.ctx, z.data_ptr(), z.shape[0], z.shape[1], int(inverse),
+                                           out.data_ptr(), _stream()), "fft2d")
+        return out
+
+    def trajectory_fit(self, offsets, shifts, dys, dxs, confs):
+        import torch
+        m = len(offsets)
+        th, tw = dys[0].shape
+        vy = torch.empty((th, tw), dtype=torch.float32, device=dys[0].device)
+        vx = torch.empty_like(vy)
+        self._check(self.lib.hvd_gpu_trajectory_fit(self.ctx, m, (C.c_int * m)(*offsets), (C.c_float * m)(*shifts),
+                                                    _ptrs(dys), _ptrs(dxs), _ptrs(confs), th, tw, vy.data_ptr(),
+                                                    vx.data_ptr(), _stream()), "trajectory_fit")
+        return vy, vx
+
+    def trajectory_snap(self, offsets, shifts, dys, dxs, confs):
+        """In place on the dys, dxs device tensors."""
+        m = len(offsets)
+        th, tw = dys[0].shape
+        self._check(self.lib.hvd_gpu_trajectory_snap(self.ctx, m, (C.c_int * m)(*offsets), (C.c_float * m)(*shifts),
+                                                     _ptrs(dys), _ptrs(dxs), _ptrs(confs), th, tw, _stream()),
+                    "trajectory_snap")
+
+    def drizzle_frame(self, j0, ys, chis, parities, mc_tile=32, mc_search=16, nr_radius=2, nr_eps=0.0, scale=2):
+        import torch
+        n = len(ys)
+        h, w = ys[0].shape
+        oy = torch.empty((2 * h * scale, w), dtype=torch.float32, device=ys[0].device)
+        oc = torch.empty((2 * h * scale, w, 2), dtype=torch.float32, device=ys[0].device)
+        self._check(self.lib.hvd_gpu_drizzle_frame(self.ctx, j0, n, _ptrs(ys), _ptrs(chis),
+                                                   (C.c_int * n)(*[int(v) for v in parities]), h, w, mc_tile,
+                                                   mc_search, nr_radius, nr_eps, scale, oy.data_ptr(), oc.data_ptr(),
+                                                   _stream()), "drizzle_frame")
+        return oy, oc
 
     def verify_motion(self, refs, curs, pdys, pdxs, tile=32):
         import torch

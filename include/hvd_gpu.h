@@ -277,6 +277,34 @@ int hvd_gpu_verify_motion(hvd_gpu_ctx* ctx, const float* const* y_ref, const flo
                           int w, const float* const* pdy, const float* const* pdx, int tile, float* const* dy,
                           float* const* dx, float* const* conf, void* stream);
 
+/* FitTrajectory (motion.h:89-95, motion.cpp:532-562) of m <= 16 pairwise
+ * tile motion fields of one field (dy, dx, conf: DEVICE, tiles_h x
+ * tiles_w each; offsets and parity_shifts: HOST, m entries): per tile the
+ * median over the confident (conf > 0.15) nonzero offsets of (dy - h) / o
+ * and dx / o, 0 where none qualifies, into vy, vx (DEVICE). */
+int hvd_gpu_trajectory_fit(hvd_gpu_ctx* ctx, int m, const int* offsets, const float* parity_shifts,
+                           const float* const* dy, const float* const* dx, const float* const* conf, int tiles_h,
+                           int tiles_w, float* vy, float* vx, void* stream);
+
+/* TrajectorySnap (motion.h:97-106, motion.cpp:564-601): the fit above, then
+ * on tiles where at least three measurements lie within 1.5 px (L1) of o v
+ * + h those measurements snap onto it; dy, dx in place.  m < 2: no change. */
+int hvd_gpu_trajectory_snap(hvd_gpu_ctx* ctx, int m, const int* offsets, const float* parity_shifts, float* const* dy,
+                            float* const* dx, const float* const* conf, int tiles_h, int tiles_w, void* stream);
+
+/* DrizzleFrame (sequence.h:105-130, sequence.cpp:283-421): fields j0 and
+ * j0 + 1 of a window of n decoded fields (Ys: lines x width floats, chis:
+ * interleaved complex; DEVICE; parities HOST) stacked onto a (2 lines scale)
+ * x width fine grid from their neighbours k in [j0 - 2 nr_radius, j0 + 2 +
+ * 2 nr_radius), each matched to the target field by EstimateMotion (tile,
+ * search), robustly weighted (scale nr_eps, 3 when <= 0) and blended with the
+ * woven interpolation.  out_luma (2 lines scale x width) and out_chroma
+ * (interleaved) DEVICE.  The fine pixels' sums are formed in hvd-core's
+ * deposit order.  Synchronous. */
+int hvd_gpu_drizzle_frame(hvd_gpu_ctx* ctx, int j0, int n, const float* const* ys, const float* const* chis,
+                          const int* parities, int lines, int width, int tile, int search, int nr_radius,
+                          float nr_eps, int scale, float* out_luma, float* out_chroma, void* stream);
+
 /* hvd-core's window driver, DecodeFieldWindowWithInits
  * (engine/sequence.cpp:371-1114, sequence.h:104-135), over n fields of one
  * shape: fields[j].s, an explicit carrier, chi0 in fields[j].chi and, where
@@ -561,6 +589,36 @@ int hvd_gpu_make_carrier(hvd_gpu_ctx* ctx, const float* theta, const int8_t* vsw
  * (diag(a) + lam L) x = a d with Neumann ends; n floats each. */
 int hvd_gpu_tridiag_smooth(hvd_gpu_ctx* ctx, const float* d, const float* a, int n, float lam, float* out,
                            void* stream);
+
+/* The forward differences of engine/gradients.h with Neumann boundaries and
+ * their exact adjoints, d = op(a) on an h x w plane: real (is_complex 0) or
+ * interleaved complex.  op: HVD_DIFF_* below.  weight (h x w floats, may be
+ * NULL; Dx, Dy, D1, D2 only) multiplies the difference: the *WeightedInto
+ * forms (gradients.h:290-381).  a, weight, d are DEVICE pointers; d must not
+ * alias a. */
+enum {
+  HVD_DIFF_DX = 0,  /* Dx / DxInto (gradients.h:26-42, :112-123) */
+  HVD_DIFF_DY = 1,  /* Dy / DyInto (:44-63, :125-138) */
+  HVD_DIFF_DXT = 2, /* DxT / DxTInto (:65-83, :140-159) */
+  HVD_DIFF_DYT = 3, /* DyT / DyTInto (:85-104, :161-182) */
+  HVD_DIFF_D1 = 4,  /* D1Into (:196-214) */
+  HVD_DIFF_D1T = 5, /* D1TInto (:216-236) */
+  HVD_DIFF_D2 = 6,  /* D2Into (:238-256) */
+  HVD_DIFF_D2T = 7  /* D2TInto (:258-278) */
+};
+int hvd_gpu_difference(hvd_gpu_ctx* ctx, int op, int is_complex, const float* a, const float* weight, int h, int w,
+                       float* d, void* stream);
+
+/* EstimateNoiseIre (ntsc_geometry.h:278-283, ntsc_geometry.cpp:49-71) of an
+ * h x w composite plane (DEVICE): the stride-4 second differences' median
+ * absolute deviation, as sigma in IRE, into *sigma (HOST).  Synchronous. */
+int hvd_gpu_estimate_noise_ire(hvd_gpu_ctx* ctx, const float* s, int h, int w, float* sigma, void* stream);
+
+/* Fft2d::Forward / Inverse (fft2d.h, fft2d.cpp:88-118): the numpy-convention
+ * 2-D DFT of an h x w interleaved complex plane (DEVICE), forward with sign
+ * -1 unnormalised, inverse with sign +1 divided by h w, into out (may equal
+ * in).  h <= 1024, w <= 4096. */
+int hvd_gpu_fft2d(hvd_gpu_ctx* ctx, const float* in, int h, int w, int inverse, float* out, void* stream);
 
 /* BoxBlur (engine/motion.cpp:21-55): separable (2r + 1)-box mean with
  * edge-truncated windows and FP64 running sums, h x w floats, r <= 32. */

@@ -680,7 +680,21 @@ def _ref_extras():
         keys = ("resolved_strength", "ambiguity_ire", "sigma_ire", "temporal_eps", "nr_eps")
         return oy, oc[..., 0::2] + 1j * oc[..., 1::2], dict(zip(keys, diag.tolist()))
 
-    for fn in (estimate_noise_ire, verify_motion, trajectory_snap, joint_refine, decode_window_with_inits):
+    def drizzle_frame(self, j0, ys, chis, parities, mc_tile=32, mc_search=16, nr_radius=2, nr_eps=0.0, scale=2):
+        """hvd-core's DrizzleFrame: (luma, chroma) on the (2 h scale) x w grid."""
+        n = len(ys)
+        h, w = ys[0].shape
+        y = np.ascontiguousarray(np.stack([self.f32(a) for a in ys]))
+        c = np.ascontiguousarray(np.stack([self.cbuf32(a) for a in chis]))
+        par = np.ascontiguousarray(parities, np.int32)
+        oy = np.zeros((2 * h * scale, w), np.float32)
+        oc = np.zeros((2 * h * scale, 2 * w), np.float32)
+        self.lib.ref_drizzle_frame(j0, n, h, w, _p(y), _p(c), _p(par), mc_tile, mc_search, nr_radius,
+                                   C.c_float(nr_eps), scale, _p(oy), _p(oc))
+        return oy, oc[..., 0::2] + 1j * oc[..., 1::2]
+
+    for fn in (estimate_noise_ire, verify_motion, trajectory_snap, joint_refine, decode_window_with_inits,
+               drizzle_frame):
         setattr(Reference, fn.__name__, fn)
 
 

@@ -403,4 +403,27 @@ int ref_decode_window_with_inits(int nf, int h, int w, const float* s, const flo
   return static_cast<int>(out.size());
 }
 
+// DrizzleFrame of fields j0, j0 + 1 of n decoded fields (Y planes, chi
+// planes one after another); out: (2 h scale) x w luma and chroma.
+void ref_drizzle_frame(int j0, int n, int h, int w, const float* ys, const float* chis, const int* parity,
+                       int mc_tile, int mc_search, int nr_radius, float nr_eps, int scale, float* out_y,
+                       float* out_chi) {
+  const size_t np = static_cast<size_t>(h) * w;
+  std::vector<hvd::Plane> Y(n);
+  std::vector<hvd::ComplexPlane> C(n);
+  std::vector<int> par(parity, parity + n);
+  for (int k = 0; k < n; ++k) {
+    Y[k] = ToPlane(ys + k * np, h, w);
+    C[k] = ToCPlane(chis + k * 2 * np, h, w);
+  }
+  hvd::HvdConfig cfg;
+  cfg.mc_tile = mc_tile;
+  cfg.mc_search = mc_search;
+  cfg.nr_radius = nr_radius;
+  cfg.nr_eps = nr_eps;
+  const hvd::DrizzleResult r = hvd::DrizzleFrame(j0, Y, C, par, cfg, scale);
+  std::memcpy(out_y, r.luma.data(), sizeof(float) * r.luma.size());
+  std::memcpy(out_chi, r.chroma.data(), sizeof(float) * 2 * r.chroma.size());
+}
+
 }  // extern "C"

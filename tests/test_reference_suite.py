@@ -14,7 +14,9 @@ import pytest
 ROOT = pathlib.Path(__file__).resolve().parents[1]
 BIN = ROOT / "oracle" / "_ref" / "tests"
 NAMES = ["lockin_test", "pal_lockin_test", "gain_chain_test", "chroma_reference_test", "field_order_test",
-         "frame_bridge_test", "motion_test"]
+         "frame_bridge_test", "motion_test", "gradients_test", "lossless_identity_test", "temporal_test",
+         "sequence_test", "colour_test", "variational_v2_test", "motion_v2_test", "pal_aniso_group_test",
+         "preview_aspect_test"]
 
 
 @pytest.mark.gpu
