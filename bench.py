@@ -246,7 +246,7 @@ def streaming_bench(torch, gpu, synth, eng, n_windows=64, window=8, unique=16):
     """512 mixed fields (3 NTSC : 1 PAL) in 8-field motion windows: per
     window, integer-SAD block matching of its 7 consecutive pairs and the
     batched field decode (lock-in, FFT init, IRLS 4 x CG <= 40 to relative
-    residual 1e-6).  Noise 0.2 / 1 / 3 IRE, chroma up to 60 IRE, one window
+    residual 1e-6).  Noise 0.2 / 1 / 3 IRE, chroma up to 75 IRE, one window
     in five with fast motion (<= 12 px/field).  `unique` distinct windows are
     generated and streamed round-robin from HBM."""
     rng = np.random.default_rng(11)
@@ -256,7 +256,7 @@ def streaming_bench(torch, gpu, synth, eng, n_windows=64, window=8, unique=16):
         gg = synth.PAL if pal else synth.NTSC
         fast = k % 5 == 4
         raws, _ = synth.motion_sequence(9000 + k, n_fields=window, g=gg, noise_ire=float([0.2, 1.0, 3.0][k % 3]),
-                                        max_rect_v=12 if fast else 8, saturation=float(rng.uniform(30.0, 60.0)))
+                                        max_rect_v=12 if fast else 8, saturation=float(rng.uniform(30.0, 75.0)))
         d = [torch.from_numpy(r.view(np.int16)).to("cuda:0") for r in raws]
         ys = [torch.empty((gg.lines, gg.width), device="cuda:0") for _ in d]
         chis = [torch.empty((gg.lines, gg.width, 2), device="cuda:0") for _ in d]
@@ -464,6 +464,54 @@ def window3d_cpu_baseline(torch, gpu, synth, eng):
             "s_per_window": round(dt, 3)}
 
 
+def plugin_path_bench(torch, gpu, synth, eng, reps=5):
+    """DecodeFrameSequenceWindow through libhvd_core.so as the decode-orc
+    plugin calls it per chunk (hvd_chroma_decoder_stage.cpp:700-760): host
+    int16 frames in, YcFrameS16 (luma, chroma, U, V) out, host copies
+    included, one HvdEngine across calls, HvdConfig defaults (cg_iterations
+    2).  lib/hvd_window_bench runs it in a child process.  Two chunk shapes
+    on the 3-D window's NTSC frames: the temporal switch off (6 frames, no
+    overlap) and on (6 core frames + 1 overlap frame per side, strength 1).
+    Beside each, the C ABI call (hvd_gpu_decode_frame_window_ex_s16) on the
+    same frames already in HBM with the same settings."""
+    import subprocess
+    import tempfile
+    exe = ROOT / "hvd-cvbs-decoding_amd" / "lib" / "hvd_window_bench"
+    if not exe.exists():
+        return None
+    fpd, frames = window3d_frames(synth)
+    kv = [f"{k}={','.join(str(e) for e in v) if k == 'extra_sample_lines' else v}" for k, v in fpd.items()]
+    fp = gpu.FrameParams(**{k: v for k, v in fpd.items() if k != "extra_sample_lines"})
+    dfr = [torch.from_numpy(f).to("cuda:0") for f in frames]
+    out = {}
+    with tempfile.TemporaryDirectory() as td:
+        path = pathlib.Path(td) / "frames.s16"
+        np.concatenate([np.asarray(f, np.int16).reshape(-1) for f in frames]).tofile(path)
+        for name, nfr, (cb, ce), temporal in (("temporal_off_chunk6", 6, (0, 6), "off"),
+                                              ("temporal_on_chunk6_overlap1", WINDOW_FRAMES, WINDOW_CORE, "1")):
+            r = subprocess.run([str(exe), str(path), str(nfr), str(cb), str(ce), temporal, str(reps)] + kv,
+                               capture_output=True, text=True, timeout=600)
+            if r.returncode != 0:
+                raise RuntimeError(f"hvd_window_bench failed ({r.returncode}): {r.stderr.strip()[-400:]}")
+            cpp = json.loads(r.stdout.strip().splitlines()[-1])
+            sp = gpu.SolverParams.make()
+            bp = gpu.BridgeParams.make(temporal_strength=-1.0 if temporal == "off" else float(temporal))
+            wp = gpu.WindowParams.make()
+            eng.decode_frame_window_s16(dfr[:nfr], cb, ce, fp, sp, bp, wparams=wp)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                eng.decode_frame_window_s16(dfr[:nfr], cb, ce, fp, sp, bp, wparams=wp)
+            torch.cuda.synchronize()
+            abi_ms = 1e3 * (time.perf_counter() - t0) / reps
+            out[name] = {"frames": nfr, "core_frames": ce - cb, "cpp_api_ms_per_call": cpp["ms_per_call"],
+                         "cpp_api_ms_best_call": cpp["ms_best_call"], "cpp_api_ms_first_call": cpp["ms_first_call"],
+                         "cpp_api_core_frames_per_s": cpp["core_frames_per_s"],
+                         "c_abi_resident_ms_per_call": round(abi_ms, 3),
+                         "cpp_over_c_abi": round(cpp["ms_per_call"] / abi_ms, 3)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -610,6 +658,9 @@ def main():
         log("[bench] 3-D window: DecodeFrameSequenceWindow, 8 NTSC frames")
         result["frame_window_3d_ntsc8"] = window3d_bench(torch, gpu, synth, eng)
         log(f"[bench] 3-D window: {result['frame_window_3d_ntsc8']}")
+        log("[bench] plugin path: DecodeFrameSequenceWindow through libhvd_core.so on host frames")
+        result["plugin_path_ntsc"] = plugin_path_bench(torch, gpu, synth, eng)
+        log(f"[bench] plugin path: {result['plugin_path_ntsc']}")
         log("[bench] temporal: NTSC-sized field with 2 temporal neighbours")
         result["temporal_2nbr_ntsc758x263"] = temporal_bench(torch, gpu, eng)
         log(f"[bench] temporal: {result['temporal_2nbr_ntsc758x263']}")

@@ -4,6 +4,10 @@
 #include "frame_bridge.h"
 
 #include <algorithm>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
 
 #include "woven.h"
 
@@ -75,6 +79,79 @@ YcFrameS16 Output(const FrameParams& fp, size_t n, const DevBuf<int16_t>& lum, c
   chr.download(out.chroma.data(), n);
   u.download(out.u_plane.data(), n);
   v.download(out.v_plane.data(), n);
+  return out;
+}
+
+// Staging bytes of one output frame of n samples: U and V (double), luma and
+// chroma (int16), rounded up so every frame's U plane starts 64-byte aligned.
+size_t StagedFrameBytes(size_t n) { return (n * (2 * sizeof(double) + 2 * sizeof(int16_t)) + 63) & ~size_t(63); }
+
+// The outputs of a frame window through page-locked staging (`staging`
+// holds ncore frames of U, V, luma and chroma planes): every frame's four
+// copies are queued on the null stream behind the decode with an event after
+// them, and one host thread per frame waits for its event and builds that
+// frame's YcFrameS16 from the staging area while later frames' copies are in
+// flight.
+std::vector<YcFrameS16> StagedOutputs(const FrameParams& fp, size_t n, const std::vector<DevBuf<int16_t>>& lum,
+                                      const std::vector<DevBuf<int16_t>>& chr, const std::vector<DevBuf<double>>& u,
+                                      const std::vector<DevBuf<double>>& v, char* staging) {
+  const int ncore = static_cast<int>(lum.size());
+  const size_t fb = StagedFrameBytes(n);
+  std::vector<hipEvent_t> ev(ncore, nullptr);
+  auto release = [&] {
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+  };
+  try {
+    for (int k = 0; k < ncore; ++k) {
+      char* b = staging + fb * k;
+      gpu::HipCheck(hipMemcpyAsync(b, u[k].get(), n * sizeof(double), hipMemcpyDeviceToHost, nullptr),
+                    "hipMemcpyAsync D2H");
+      gpu::HipCheck(hipMemcpyAsync(b + n * sizeof(double), v[k].get(), n * sizeof(double), hipMemcpyDeviceToHost,
+                                   nullptr), "hipMemcpyAsync D2H");
+      gpu::HipCheck(hipMemcpyAsync(b + 2 * n * sizeof(double), lum[k].get(), n * sizeof(int16_t),
+                                   hipMemcpyDeviceToHost, nullptr), "hipMemcpyAsync D2H");
+      gpu::HipCheck(hipMemcpyAsync(b + 2 * n * sizeof(double) + n * sizeof(int16_t), chr[k].get(),
+                                   n * sizeof(int16_t), hipMemcpyDeviceToHost, nullptr), "hipMemcpyAsync D2H");
+      gpu::HipCheck(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming), "hipEventCreateWithFlags");
+      gpu::HipCheck(hipEventRecord(ev[k], nullptr), "hipEventRecord");
+    }
+  } catch (...) {
+    (void)hipDeviceSynchronize();
+    release();
+    throw;
+  }
+  std::vector<YcFrameS16> out(ncore);
+  std::vector<std::string> err(ncore);
+  std::vector<std::thread> th;
+  th.reserve(ncore);
+  for (int k = 0; k < ncore; ++k)
+    th.emplace_back([&, k] {
+      const hipError_t e = hipEventSynchronize(ev[k]);
+      if (e != hipSuccess) {
+        err[k] = std::string("hipEventSynchronize: ") + hipGetErrorString(e);
+        return;
+      }
+      try {
+        const char* b = staging + fb * k;
+        const double* pu = reinterpret_cast<const double*>(b);
+        const int16_t* pl = reinterpret_cast<const int16_t*>(b + 2 * n * sizeof(double));
+        YcFrameS16& o = out[k];
+        o.width = fp.frame_width;
+        o.height = fp.frame_height;
+        o.chroma_dc = fp.chroma_dc;
+        o.u_plane.assign(pu, pu + n);
+        o.v_plane.assign(pu + n, pu + 2 * n);
+        o.luma.assign(pl, pl + n);
+        o.chroma.assign(pl + n, pl + 2 * n);
+      } catch (const std::exception& x) {
+        err[k] = x.what();
+      }
+    });
+  for (std::thread& t : th) t.join();
+  release();
+  for (const std::string& e : err)
+    if (!e.empty()) throw gpu::Error(e);
   return out;
 }
 
@@ -206,13 +283,24 @@ std::vector<YcFrameS16> DecodeFrameSequenceWindow(const std::vector<const int16_
       throw gpu::Error("selective 3-D decoding is not run by this library");
     Session& ss = engine.session();
     const size_t nsrc = FrameSamples(fp), nflat = (size_t)fp.frame_width * fp.frame_height;
+    const int ncore = core_end - core_begin;
+    // Transfers go through the session's page-locked staging area: the frames
+    // in (each frame's host copy overlaps the previous frame's DMA), then each
+    // core frame's U, V, luma and chroma planes out.
+    const size_t in_bytes = nsrc * sizeof(int16_t);
+    const size_t out_bytes = StagedFrameBytes(nflat);
+    const size_t in_total = (in_bytes * nframes + 63) & ~size_t(63);
+    char* pin = ss.pinned(in_total + out_bytes * ncore);
     std::vector<DevBuf<int16_t>> df;
     std::vector<const int16_t*> dptr;
-    for (const int16_t* fr : frames) {
-      df.push_back(gpu::Upload(fr, nsrc));
+    for (int t = 0; t < nframes; ++t) {
+      df.emplace_back(nsrc);
       dptr.push_back(df.back().get());
+      char* st = pin + in_bytes * t;
+      std::memcpy(st, frames[t], in_bytes);
+      gpu::HipCheck(hipMemcpyAsync(df.back().get(), st, in_bytes, hipMemcpyHostToDevice, nullptr),
+                    "hipMemcpyAsync H2D");
     }
-    const int ncore = core_end - core_begin;
     std::vector<DevBuf<int16_t>> lum, chr;
     std::vector<DevBuf<double>> u, v;
     std::vector<hvd_yc_frame> outs(ncore);
@@ -232,9 +320,7 @@ std::vector<YcFrameS16> DecodeFrameSequenceWindow(const std::vector<const int16_
     ss.check(hvd_gpu_decode_frame_window_ex_s16(ss.ctx(), dptr.data(), nframes, core_begin, core_end, &a, &sp, &bp,
                                                 &wp, outs.data(), nullptr, nullptr, nullptr),
              "hvd_gpu_decode_frame_window_ex_s16");
-    std::vector<YcFrameS16> out;
-    for (int k = 0; k < ncore; ++k) out.push_back(Output(fp, nflat, lum[k], chr[k], u[k], v[k]));
-    return out;
+    return StagedOutputs(fp, nflat, lum, chr, u, v, pin + in_total);
   } catch (const std::exception& e) {
     gpu::Report("DecodeFrameSequenceWindow", e);
     return {};

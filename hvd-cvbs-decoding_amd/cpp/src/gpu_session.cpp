@@ -22,7 +22,21 @@ void ClearGpuError() { g_last_error.clear(); }
 namespace gpu {
 
 Session::~Session() {
+  if (pinned_) (void)hipHostFree(pinned_);
   if (ctx_) hvd_gpu_destroy(ctx_);
+}
+
+char* Session::pinned(size_t bytes) {
+  if (bytes > pinned_cap_) {
+    if (pinned_) HipCheck(hipHostFree(pinned_), "hipHostFree");
+    pinned_ = nullptr;
+    pinned_cap_ = 0;
+    void* p = nullptr;
+    HipCheck(hipHostMalloc(&p, bytes, hipHostMallocDefault), "hipHostMalloc");
+    pinned_ = static_cast<char*>(p);
+    pinned_cap_ = bytes;
+  }
+  return pinned_;
 }
 
 hvd_gpu_ctx* Session::ctx() {

@@ -44,10 +44,17 @@ class Session {
   // library's autotuner for that batch first (hvd_gpu_autotune_solver).
   // Failures of the tuner are reported and otherwise ignored.
   void autotune(int lines, int width, bool pal, int nfields);
+  // Page-locked host memory for staged transfers (hipHostMalloc), at least
+  // `bytes`, owned by the session and grown only when a call needs more (the
+  // previous contents are not kept).  Callers finish their copies through it
+  // before they return.
+  char* pinned(size_t bytes);
 
  private:
   hvd_gpu_ctx* ctx_ = nullptr;
   std::set<std::array<int, 4>> tuned_;
+  char* pinned_ = nullptr;
+  size_t pinned_cap_ = 0;
 };
 
 // The session free functions (BurstLockinPhase, HolographicInit, ...) use.

@@ -14,18 +14,13 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "strip_geom.h"
+
 namespace hvdk {
 
 constexpr int kTileW = 64;  // one wavefront per tile row (init kernels)
 constexpr int kTileH = 4;
 constexpr int kBlock = kTileW * kTileH;  // 256 threads
-// Output columns of a solver strip: 64 lanes less two halo lanes on the left
-// and three on the right.  The fused CG iteration recomputes the gradient of
-// chi (lanes 1 .. 62, each needs both neighbours), moves chi there, and forms
-// the new gradient on lanes 2 .. 61: the outputs' curvature terms need it one
-// column past the last output.
-constexpr int kStripCols = 59;
-constexpr int kStripHalo = 2;  // lane of the strip's first output column
 // Launch geometry of the solver kernels: strip prefix over the batch, and
 // the strips per field when every field has the same size (else 0).
 struct StripGrid {
@@ -33,19 +28,9 @@ struct StripGrid {
   int nf, nb, rows, spf;
 };
 
-// Rows of carrier factors a solver strip of `rows` lines holds, one per lane
-// (two halo rows above and three below); at most 64.
-constexpr int carrier_span(int rows) { return rows + 5; }
-constexpr int kMaxStripRows = 32;
 // FP64 partial sums per solver strip and CG launch (|g|^2, curv(g),
 // curv(g, d), Re<g, d>).
 constexpr int kCgPartials = 4;
-static_assert(carrier_span(kMaxStripRows) <= 64, "strip carrier rows must fit one wave");
-
-// Solver strips of `rows` lines covering an h x w plane.
-inline int solver_strips(int h, int w, int rows) {
-  return ((w + kStripCols - 1) / kStripCols) * ((h + rows - 1) / rows);
-}
 
 // One field of a batch, as every per-pixel kernel sees it.
 struct DevField {
@@ -174,25 +159,6 @@ __device__ inline float2 carrier_at(const DevField& f, int y, int x) {
   float2 v = cmul(f.rowph[y], f.colph[x]);
   if (f.vsw && f.vsw[y] < 0) v.x = -v.x;
   return v;
-}
-
-// Block index -> logical tile, grouping consecutive logical tiles on one XCD
-// (blocks are dispatched round-robin over the 8 XCDs, each with its own L2),
-// so a tile and its vertical neighbours tend to share an L2.
-__device__ inline int xcd_remap(int b, int nb) {
-  const int per = nb / 8, rem = nb % 8;
-  const int xcd = b % 8, idx = b / 8;
-  return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
-}
-
-// Field owning logical tile t: tile_prefix has n+1 entries.
-__device__ inline int field_of_tile(const int* tile_prefix, int n, int t) {
-  int lo = 0, hi = n;  // invariant: tile_prefix[lo] <= t < tile_prefix[hi]
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (tile_prefix[mid] <= t) lo = mid; else hi = mid;
-  }
-  return lo;
 }
 
 // Deterministic block-wide FP64 sum; result valid in thread 0.

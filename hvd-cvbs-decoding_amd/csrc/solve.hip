@@ -54,8 +54,8 @@ __device__ inline StripCtx strip_ctx(const DevField* F, const StripGrid& sg) {
   const DevField f = F[c.f];  // a private copy: stores cannot alias it
   c.lt = sg.spf > 0 ? t - c.f * sg.spf : t - sg.tp[c.f];
   const int lane = lane_id();
-  c.x = (c.lt % f.tiles_x) * kStripCols + lane - kStripHalo;
-  c.y0 = (c.lt / f.tiles_x) * sg.rows;
+  c.x = strip_col(c.lt, f.tiles_x, lane);
+  c.y0 = strip_row0(c.lt, f.tiles_x, sg.rows);
   c.valid = c.x >= 0 && c.x < f.w;
   c.out = lane >= kStripHalo && lane < kStripHalo + kStripCols && c.x < f.w;
   return c;
@@ -77,7 +77,7 @@ struct Carrier {
   // lanes) the value is unspecified but the access stays in bounds.
   __device__ inline float2 at(int y) const { return plane ? load(y) : factored(y); }
   __device__ inline float2 load(int y) const {
-    return gld(plane, (size_t)min(max(y, 0), h - 1) * w + min(max(x, 0), w - 1));
+    return gld(plane, (size_t)clamped_index(y, x, h, w));
   }
   __device__ inline float2 factored(int y) const {
 #pragma clang fp contract(off)
@@ -330,8 +330,7 @@ template <bool Start, bool Upd, bool Adj, bool FuseW>
 __device__ inline IterRaw iter_fetch(const DevField& f, const Carrier& k, int x, const float2* chi,
                                      const float2* dprev, int y) {
   IterRaw r;
-  const int yy = min(max(y, 0), f.h - 1), xx = min(max(x, 0), f.w - 1);
-  const unsigned i = (unsigned)(yy * f.w + xx);
+  const unsigned i = clamped_index(y, x, f.h, f.w);
   r.chi = gld32(chi, i);
   r.s = Adj ? 0.f : gld32(f.s, i);
   r.dp = Upd ? gld32(dprev, i) : make_float2(0.f, 0.f);
@@ -625,7 +624,7 @@ __device__ inline void cg_strip(const DevField& f, const FieldCG& s, const CgDyn
   fyu = flux(w0.y, w0.w, r0.p, r1.p);
   // lines 0 .. jl lie inside the plane (jl <= rows: the walk forms g' one
   // line past the strip's last output row)
-  const int jl = min(rows, f.h - 1 - c.y0);
+  const int jl = down_walk_last(c.y0, rows, f.h);
   int j = 0;
   for (; j + 2 <= jl; j += 3) {
     step(j, r0, r1, r2, w0, w1, w2, p0, p1, oa, ob);
@@ -681,7 +680,7 @@ __device__ inline void cg_strip_up(const DevField& f, const FieldCG& s, const Cg
   Flux fdn;   // flux of chi' through the lower edge of the current line
   Flux fdo;   // flux of chi through the lower edge of the row above the line
   double gg = 0.0, A = 0.0, B = 0.0, D = 0.0;
-  const int T = min(c.y0 + rows, f.h);
+  const int T = up_walk_top(c.y0, rows, f.h);
   // Line j of the walk (row y = T - j).  Slots as in cg_strip: dn / up / cur
   // rotate over the rows below / at / above the line (`above` is filled
   // here), the weights with them, the rows before the move (o1: row y - 1,
@@ -973,7 +972,7 @@ __device__ inline void cg_iter(const DevField* __restrict__ F, FieldCG* __restri
   if (q.done) return;
   double* out = partials + (it & 1) * slot;
   // strips on odd strip rows walk bottom-up (uniform per wave)
-  if ((c.lt / f.tiles_x) & 1) {
+  if (strip_walks_up(c.lt, f.tiles_x)) {
     if (it > 0) cg_strip_up<true, Adj>(f, s, q, c, sg.rows, out);
     else cg_strip_up<false, Adj>(f, s, q, c, sg.rows, out);
   } else {

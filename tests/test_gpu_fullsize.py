@@ -11,7 +11,8 @@ Properties: every output finite; the lossless split Y + Re[chi c] = S within
 1e-3 IRE (lossless_identity_test.cpp:73-74, with c the carrier the decode
 uses); a field's result is the same bits whether it is decoded
 in the batch or alone (same strip height); sampled fields within TOL_IRE of
-the FP64 oracle.
+the FP64 oracle; every field of the mixed batch against the FP64 oracle
+(per-field max |dY|, |dU|, |dV| printed).
 """
 import numpy as np
 import pytest
@@ -100,17 +101,31 @@ def test_mixed_32_32_full(eng, orc64, orc32):
     X = [c.cpu().numpy() for c in chis]
     T = [t.cpu().numpy() for t in ths]
     assert all(np.isfinite(y).all() and np.isfinite(x).all() for y, x in zip(Y, X))
-    worst = max(lossless_err(orc32, raws[k], gs[k], Y[k], X[k], T[k]) for k in (0, 1, 30, 63))
+    worst = max(lossless_err(orc32, raws[k], gs[k], Y[k], X[k], T[k]) for k in range(64))
     print(f"mixed batch lossless split: max {worst:.2e} IRE", flush=True)
     assert worst <= 1e-3
-    for k in (2, 33):
-        want = orc64.decode_field(synth.ire_from_raw(raws[k]), gs[k], O.Cfg.make(is_pal=int(gs[k].uses_vswitch),
-                                                                             **CFG2))
-        ey = np.abs(Y[k] - want["y"]).max()
-        ec = np.abs(X[k][..., 0] + 1j * X[k][..., 1] - want["chi"]).max()
-        print(f"mixed field {k} ({'PAL' if gs[k].uses_vswitch else 'NTSC'}) vs oracle: {ey:.2e}, {ec:.2e}",
-              flush=True)
-        assert ey <= TOL_IRE and ec <= TOL_IRE
+    # every field against the FP64 oracle: Y, U = -Im chi, V = Re chi
+    import concurrent.futures as cf
+    import os
+
+    def oracle_field(k):
+        cfg = O.Cfg.make(is_pal=int(gs[k].uses_vswitch), **CFG2)
+        return orc64.decode_field(synth.ire_from_raw(raws[k]), gs[k], cfg)
+
+    with cf.ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        wants = list(ex.map(oracle_field, range(64)))
+    errs = []
+    for k, want in enumerate(wants):
+        ey = float(np.abs(Y[k] - want["y"]).max())
+        eu = float(np.abs(X[k][..., 1] - want["chi"].imag).max())
+        ev = float(np.abs(X[k][..., 0] - want["chi"].real).max())
+        errs.append((ey, eu, ev))
+        print(f"field {k:2d} ({'PAL ' if gs[k].uses_vswitch else 'NTSC'}): max |dY| {ey:.2e} |dU| {eu:.2e} "
+              f"|dV| {ev:.2e} IRE", flush=True)
+    E = np.array(errs)
+    print(f"all 64 fields: worst |dY| {E[:, 0].max():.2e} |dU| {E[:, 1].max():.2e} |dV| {E[:, 2].max():.2e} IRE",
+          flush=True)
+    assert E.max() <= TOL_IRE, np.unravel_index(np.argmax(E), E.shape)
 
 
 def test_largest_plane(eng, orc64):

@@ -121,3 +121,31 @@ def test_temporal_argument_errors(engine):
         run_gpu(engine, s, car, chi0, nbrs, CFG, 1.0, 0.0)
     with pytest.raises(gpu.HvdGpuError, match="neighbour count"):
         run_gpu(engine, s, car, chi0, nbrs * 17, CFG, 1.0, 1.0)
+
+
+@pytest.mark.parametrize("diag_prior", [0.0, 0.5])
+def test_stationary_start_exits_on_curvature(engine, orc64, diag_prior):
+    """A scene that starts at its minimum (flat composite 37.5 IRE, zero
+    chroma, neighbours that agree): the gradient and so the first direction
+    are zero, hh = 0 and every IRLS stage takes the curvature exit before its
+    first step (variational.cpp's `hh <= 1e-12` break).  The 2-D and 3-D
+    solves leave chi untouched and count the iterations as the oracle does."""
+    h, w = 40, 64
+    kw = dict(CFG, diag_prior=diag_prior)
+    s = np.full((h, w), 37.5)
+    r, c = np.mgrid[0:h, 0:w]
+    car = np.exp(1j * (0.5 * np.pi * c + 0.3 * r))
+    chi0 = np.zeros((h, w), np.complex128)
+    nbrs = [(s.copy(), car.copy(), np.full((h, w), 0.8)) for _ in range(2)]
+    y_ref, c_ref, _, it_ref = orc64.variational_refine(s, car, chi0, O.Cfg.make(**kw))
+    assert not it_ref.any() and not np.abs(c_ref).any()
+    y6, c6, _, it6 = orc64.variational_refine_nbr(s, car, chi0, O.Cfg.make(**kw), nbrs, 1.0, 1.0)
+    assert not it6.any() and not np.abs(c6).any()
+    dchi, dy = cdev(chi0), torch.zeros(s.shape, device=DEV)
+    it, _ = engine.variational_refine([gpu.Engine.plane(dev(s), dchi, dy, carrier=cdev(car))],
+                                      gpu.SolverParams.make(**kw), want_stats=True)
+    assert np.array_equal(it[0], it_ref)
+    assert not chost(dchi).any() and np.array_equal(dy.cpu().numpy(), y_ref.astype(np.float32))
+    y, cc, it = run_gpu(engine, s, car, chi0, nbrs, kw, 1.0, 1.0)
+    assert np.array_equal(np.asarray(it).reshape(-1), it6.reshape(-1))
+    assert not cc.any() and np.array_equal(y, y6.astype(np.float32))
