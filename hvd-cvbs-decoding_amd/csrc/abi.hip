@@ -99,6 +99,7 @@ struct hvd_gpu_ctx {
   Slot slots[kSlots];
   int next_slot = 0;
   std::map<int, std::vector<float2>> roots;
+  std::map<int, std::vector<float2>> rader_tabs;  // per prime radix
   int max_dyn_lds = 0;
   // profiling (diagnostics): event pairs around every CG loop and pinned
   // copies of the per-field solver state, read by hvd_gpu_cg_timing().
@@ -350,25 +351,115 @@ const std::vector<float2>& root_table(hvd_gpu_ctx* ctx, int n) {
   return ctx->roots.emplace(n, std::move(r)).first->second;
 }
 
+// Smallest generator of the multiplicative group mod the prime q.
+int primitive_root(int q) {
+  std::vector<int> pf;
+  int m = q - 1;
+  for (int d = 2; d * d <= m; ++d)
+    if (m % d == 0) {
+      pf.push_back(d);
+      while (m % d == 0) m /= d;
+    }
+  if (m > 1) pf.push_back(m);
+  for (int g = 2; g < q; ++g) {
+    bool ok = true;
+    for (int f : pf) {
+      long long r = 1, b = g;
+      for (int e = (q - 1) / f; e; e >>= 1, b = b * b % q)
+        if (e & 1) r = r * b % q;
+      if (r == 1) ok = false;
+    }
+    if (ok) return g;
+  }
+  return 1;
+}
+
+// Rader tables of prime R (L = R - 1): the L roots of L, bhat for the forward
+// and the inverse direction (bhat = DFT_L(b) / L with b_l = w^{g^-l},
+// w = e^{-+2 pi i / R}), and the permutation {g^l, g^-l} mod R stored as int
+// pairs in float2 slots; all in FP64, rounded once.
+const std::vector<float2>& rader_table(hvd_gpu_ctx* ctx, int R) {
+  auto it = ctx->rader_tabs.find(R);
+  if (it != ctx->rader_tabs.end()) return it->second;
+  const int L = R - 1, g = primitive_root(R);
+  std::vector<int> gp(L), gm(L);
+  long long x = 1;
+  for (int l = 0; l < L; ++l, x = x * g % R) gp[l] = (int)x;
+  for (int l = 0; l < L; ++l) gm[l] = gp[(L - l) % L];  // g^-l = g^(L - l)
+  std::vector<float2> t(4 * (size_t)L);
+  for (int j = 0; j < L; ++j) {
+    const double a = -2.0 * kPi * (double)j / (double)L;
+    t[j] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  for (int dir = 0; dir < 2; ++dir) {
+    const double sgn = dir == 0 ? -1.0 : 1.0;
+    for (int k = 0; k < L; ++k) {
+      double re = 0.0, im = 0.0;
+      for (int l = 0; l < L; ++l) {
+        // b_l e^{-2 pi i k l / L}, phases reduced exactly before scaling
+        const double a = sgn * 2.0 * kPi * (double)gm[l] / (double)R -
+                         2.0 * kPi * (double)((long long)k * l % L) / (double)L;
+        re += std::cos(a);
+        im += std::sin(a);
+      }
+      t[(size_t)(1 + dir) * L + k] = make_float2((float)(re / L), (float)(im / L));
+    }
+  }
+  for (int l = 0; l < L; ++l) {
+    int2 pr{gp[l], gm[l]};
+    std::memcpy(&t[(size_t)3 * L + l], &pr, sizeof(pr));
+  }
+  return ctx->rader_tabs.emplace(R, std::move(t)).first->second;
+}
+
+// FFT plans of one call, keyed by (length, interleaved sequences per block),
+// with their root and Rader tables appended to one buffer.  A prime stage
+// R >= kRaderMin runs by Rader's algorithm when its tables fit the block's
+// LDS, else as the direct pair form.
 struct PlanSet {
   std::vector<FftPlan> plans;
   std::vector<float2> roots;
-  std::map<int, int> index;
-  int get(hvd_gpu_ctx* ctx, int n) {
-    auto it = index.find(n);
+  std::map<std::pair<int, int>, int> index;
+  int get(hvd_gpu_ctx* ctx, int n, int ncols) {
+    auto it = index.find({n, ncols});
     if (it != index.end()) return it->second;
     FftPlan p{};
     p.n = n;
     const std::vector<int> f = factorize(n);
     REQUIRE(ctx, (int)f.size() <= kMaxRadix, "FFT length %d has too many factors", n);
     p.nst = (int)f.size();
-    for (size_t i = 0; i < f.size(); ++i) p.radix[i] = f[i];
+    for (size_t i = 0; i < f.size(); ++i) {
+      p.radix[i] = f[i];
+      p.rader_of[i] = -1;
+    }
     p.root_off = (int)roots.size();
     const std::vector<float2>& r = root_table(ctx, n);
     roots.insert(roots.end(), r.begin(), r.end());
+    FftPlan pr = p;  // with Rader stages
+    for (size_t i = 0; i < f.size(); ++i) {
+      const int R = f[i];
+      if (R < kRaderMin || pr.nrader == kMaxRader) continue;
+      const std::vector<int> fi = factorize(R - 1);
+      if ((int)fi.size() > kMaxInnerRadix) continue;
+      FftRader& rd = pr.rader[pr.nrader];
+      rd.L = R - 1;
+      rd.nst = (int)fi.size();
+      for (size_t j = 0; j < fi.size(); ++j) rd.radix[j] = fi[j];
+      pr.rader_of[i] = (signed char)pr.nrader++;
+      pr.rader_fixed = std::max(pr.rader_fixed, 3 * rd.L);
+      pr.rader_per_col = std::max(pr.rader_per_col, 2 * (n / R));
+    }
+    if (pr.nrader > 0 && sizeof(float2) * (size_t)fft_lds_slots(pr, ncols) <= (size_t)ctx->max_dyn_lds) {
+      for (int k = 0; k < pr.nrader; ++k) {
+        pr.rader[k].tab = (int)roots.size();
+        const std::vector<float2>& t = rader_table(ctx, pr.rader[k].L + 1);
+        roots.insert(roots.end(), t.begin(), t.end());
+      }
+      p = pr;
+    }
     plans.push_back(p);
     const int id = (int)plans.size() - 1;
-    index[n] = id;
+    index[{n, ncols}] = id;
     return id;
   }
 };
@@ -524,41 +615,59 @@ void run_prepare(hvd_gpu_ctx* ctx, const std::vector<PrepDesc>& P, bool crop, hi
   arm(ctx, sl, st);
 }
 
-size_t fft_rows_lds(int n) { return sizeof(float2) * 3 * (size_t)n; }
-size_t fft_cols_lds(int n) { return sizeof(float2) * ((size_t)2 * n * kColGroup + n); }
-
 void run_init(hvd_gpu_ctx* ctx, std::vector<InitField>& F, PlanSet& ps, hipStream_t st) {
   const int nf = (int)F.size();
-  std::vector<int> rows_fwd(nf + 1, 0), grp(nf + 1, 0), rows_inv(nf + 1, 0), tiles(nf + 1, 0);
-  int max_row = 1, max_col = 1;
+  // per FFT pass and transform class: the fields, the running count of
+  // blocks, the LDS a block needs
+  struct PassList {
+    std::vector<int> fields, prefix{0};
+    size_t lds = 0, o_fields = 0, o_prefix = 0;
+  };
+  PassList L[4][3];
+  std::vector<int> tiles(nf + 1, 0);
   for (int i = 0; i < nf; ++i) {
     const InitField& f = F[i];
-    rows_fwd[i + 1] = rows_fwd[i] + f.hp;
-    grp[i + 1] = grp[i] + (f.w + kColGroup - 1) / kColGroup;
-    rows_inv[i + 1] = rows_inv[i] + f.h;
+    const FftPlan& pr = ps.plans[f.row_plan];
+    const FftPlan& pc = ps.plans[f.col_plan];
+    const int rc = fft_init_class(pr, true), cc = fft_init_class(pc, false);
+    const int rows_fwd = (f.hp + kRowGroup - 1) / kRowGroup, cols = (f.w + kColGroup - 1) / kColGroup,
+              rows_inv = (f.h + kRowGroup - 1) / kRowGroup;
+    const size_t lds_r = sizeof(float2) * (size_t)fft_lds_slots(pr, kRowGroup),
+                 lds_c = sizeof(float2) * (size_t)fft_lds_slots(pc, kColGroup),
+                 lds_ci = lds_c + sizeof(float) * (size_t)f.hp;  // + the Gaussian's row factors
+    REQUIRE(ctx, lds_r <= (size_t)ctx->max_dyn_lds && lds_ci <= (size_t)ctx->max_dyn_lds,
+            "FFT sizes %d x %d exceed the LDS budget (%zu / %zu bytes per block)", f.hp, f.w, lds_ci, lds_r);
+    const int cls[4] = {rc, cc, cc, rc}, blocks[4] = {rows_fwd, cols, cols, rows_inv};
+    const size_t lds[4] = {lds_r, lds_c, lds_ci, lds_r};
+    for (int p = 0; p < 4; ++p) {
+      PassList& pl = L[p][cls[p]];
+      pl.fields.push_back(i);
+      pl.prefix.push_back(pl.prefix.back() + blocks[p]);
+      pl.lds = std::max(pl.lds, lds[p]);
+    }
     tiles[i + 1] = tiles[i] + ((f.w + kBlendW - 1) / kBlendW) * ((f.h + kBlendH - 1) / kBlendH);
-    max_row = std::max(max_row, f.w);
-    max_col = std::max(max_col, f.hp);
   }
-  const size_t lds_r = fft_rows_lds(max_row), lds_c = fft_cols_lds(max_col);
-  REQUIRE(ctx, (int)lds_r <= ctx->max_dyn_lds && (int)lds_c <= ctx->max_dyn_lds,
-          "FFT sizes %d x %d exceed the LDS budget", max_col, max_row);
   Stage stg;
-  const size_t oF = stg.put(F), oP = stg.put(ps.plans), oR = stg.put(ps.roots);
-  const size_t o1 = stg.put(rows_fwd), o2 = stg.put(grp), o3 = stg.put(rows_inv), o4 = stg.put(tiles);
+  const size_t oF = stg.put(F), oP = stg.put(ps.plans), oR = stg.put(ps.roots), o4 = stg.put(tiles);
+  for (auto& pass : L)
+    for (PassList& pl : pass)
+      if (!pl.fields.empty()) {
+        pl.o_fields = stg.put(pl.fields);
+        pl.o_prefix = stg.put(pl.prefix);
+      }
   Slot* sl;
   char* base = upload(ctx, stg, st, &sl);
   const InitField* dF = (const InitField*)(base + oF);
   const FftPlan* dP = (const FftPlan*)(base + oP);
   const float2* dR = (const float2*)(base + oR);
-  hipLaunchKernelGGL(k_fft_rows_fwd, dim3(rows_fwd[nf]), dim3(256), lds_r, st, dF, dP, dR, (const int*)(base + o1), nf);
-  LAUNCH_CHECK(ctx);
-  hipLaunchKernelGGL(k_fft_cols_fwd, dim3(grp[nf]), dim3(256), lds_c, st, dF, dP, dR, (const int*)(base + o2), nf);
-  LAUNCH_CHECK(ctx);
-  hipLaunchKernelGGL(k_fft_cols_inv, dim3(grp[nf]), dim3(256), lds_c, st, dF, dP, dR, (const int*)(base + o2), nf);
-  LAUNCH_CHECK(ctx);
-  hipLaunchKernelGGL(k_fft_rows_inv, dim3(rows_inv[nf]), dim3(256), lds_r, st, dF, dP, dR, (const int*)(base + o3), nf);
-  LAUNCH_CHECK(ctx);
+  for (int p = 0; p < 4; ++p)
+    for (int c = 0; c < 3; ++c) {
+      const PassList& pl = L[p][c];
+      if (pl.fields.empty()) continue;
+      launch_fft_pass(p, c, pl.prefix.back(), pl.lds, st, dF, dP, dR, (const int*)(base + pl.o_prefix),
+                      (const int*)(base + pl.o_fields), (int)pl.fields.size());
+      LAUNCH_CHECK(ctx);
+    }
   hipLaunchKernelGGL(k_holo_blend, dim3(tiles[nf]), dim3(256), 0, st, dF, (const int*)(base + o4), nf);
   LAUNCH_CHECK(ctx);
   arm(ctx, sl, st);
@@ -1143,9 +1252,9 @@ int hvd_gpu_create(int device, hvd_gpu_ctx** out) {
     int lds = 0;
     HIPCHK(ctx, hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
     ctx->max_dyn_lds = lds;
-    for (const void* k : {(const void*)k_fft_rows_fwd, (const void*)k_fft_cols_fwd, (const void*)k_fft_cols_inv,
-                          (const void*)k_fft_rows_inv, (const void*)k_fft2_rows, (const void*)k_fft2_cols})
+    for (const void* k : {(const void*)k_fft2_rows, (const void*)k_fft2_cols})
       HIPCHK(ctx, hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+    HIPCHK(ctx, fft_allow_lds(lds));
     HIPCHK(ctx, hipFuncSetAttribute((const void*)k_aniso_select, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     std::max(0, lds - kSelStaticLds)));
     HIPCHK(ctx, scratch_address(device, &ctx->scratch));
@@ -1318,8 +1427,8 @@ int hvd_gpu_holographic_init(hvd_gpu_ctx* ctx, const hvd_gpu_plane* planes, int 
       for (int v = 0; v < nvar; ++v) f.var[v] = (float2*)(ws + o_var[i]) + (size_t)v * f.h * f.w;
       f.chi0 = reinterpret_cast<float2*>(pl.chi);
       f.y0 = pl.y;
-      f.row_plan = ps.get(ctx, f.w);
-      f.col_plan = ps.get(ctx, f.hp);
+      f.row_plan = ps.get(ctx, f.w, kRowGroup);
+      f.col_plan = ps.get(ctx, f.hp, kColGroup);
       f.nvar = nvar;
       f.sample_rate = (float)pl.sample_rate;
       F[i] = f;
@@ -2251,8 +2360,8 @@ void decode_fields_run(hvd_gpu_ctx* ctx, const uint16_t* const* raw, const float
     f.var[1] = f.var[0] + (size_t)f.h * f.w;
     f.chi0 = reinterpret_cast<float2*>(chi_out[i]);
     f.y0 = y_out[i];
-    f.row_plan = ps.get(ctx, f.w);
-    f.col_plan = ps.get(ctx, f.hp);
+    f.row_plan = ps.get(ctx, f.w, kRowGroup);
+    f.col_plan = ps.get(ctx, f.hp, kColGroup);
     f.nvar = 2;
     f.sample_rate = (float)geom[i].sample_rate;
     IF[i] = f;
@@ -3012,13 +3121,14 @@ int hvd_gpu_fft2d(hvd_gpu_ctx* ctx, const float* in, int h, int w, int inverse, 
     REQUIRE(ctx, h >= 1 && w >= 1 && h <= 1024 && w <= 4096, "FFT plane %d x %d outside 1..1024 x 1..4096", h, w);
     hipStream_t st = S(stream);
     PlanSet ps;
-    const int pr = ps.get(ctx, w), pc = ps.get(ctx, h);
+    const int pr = ps.get(ctx, w, 1), pc = ps.get(ctx, h, kColGroup);
     Carve cv;
     const size_t o_roots = cv.take(sizeof(float2) * ps.roots.size());
     char* ws = ensure_ws(ctx, cv.off, st);
     float2* roots = (float2*)(ws + o_roots);
     HIPCHK(ctx, hipMemcpyAsync(roots, ps.roots.data(), sizeof(float2) * ps.roots.size(), hipMemcpyHostToDevice, st));
-    const size_t lds_r = sizeof(float2) * 3 * (size_t)w, lds_c = sizeof(float2) * (2 * kColGroup + 1) * (size_t)h;
+    const size_t lds_r = sizeof(float2) * (size_t)fft_lds_slots(ps.plans[pr], 1),
+                 lds_c = sizeof(float2) * (size_t)fft_lds_slots(ps.plans[pc], kColGroup);
     REQUIRE(ctx, (int)std::max(lds_r, lds_c) <= ctx->max_dyn_lds, "FFT plane %d x %d needs more LDS than a block has",
             h, w);
     const int sign = inverse ? 1 : -1;

@@ -11,6 +11,7 @@ namespace hvdk {
 
 constexpr int kMaxLines = 640;   // lines per field the lock-in handles in LDS
 constexpr int kColGroup = 8;     // columns per block in the column FFTs
+constexpr int kRowGroup = 2;     // rows per block in the init's row FFTs
 constexpr int kBlendW = 32;      // holographic blend tile
 constexpr int kBlendH = 8;
 constexpr int kMaxRadix = 16;    // stages per 1-D FFT plan
@@ -40,12 +41,38 @@ struct PrepDesc {
   float* th_active;          // lines or null: the trimmed float theta of the active lines
 };
 
+// A prime stage of radix R >= kRaderMin runs by Rader's algorithm: its R - 1
+// non-zero inputs, permuted by powers of a generator g of (Z/R)*, are
+// cyclically convolved with the permuted roots by a length-(R - 1) transform
+// of small radices.
+constexpr int kRaderMin = 37;
+constexpr int kMaxRader = 2;       // Rader stages per plan (37 * 41 * 43 > 4096)
+constexpr int kMaxInnerRadix = 8;  // stages of the length-(R - 1) transform
+
+struct FftRader {
+  int L, nst;                  // L = R - 1 and its stage count
+  int radix[kMaxInnerRadix];   // small radices and primes < kRaderMin
+  int tab;  // float2 offset in the root buffer of [L roots of L | bhat fwd L | bhat inv L | perm L]
+  int pad_;
+};
+
 struct FftPlan {
   int n, nst;
   int radix[kMaxRadix];
   int root_off;  // offset of this size's n roots in the shared root buffer
-  int pad_;
+  int nrader;
+  signed char rader_of[kMaxRadix];  // -1, or the index into rader[] of stage s
+  FftRader rader[kMaxRader];
+  // float2 slots of LDS the Rader stages need beyond the two buffers and the
+  // n roots: rader_fixed + rader_per_col x (interleaved sequences)
+  int rader_fixed, rader_per_col;
 };
+
+// LDS (float2 slots) of an in-LDS transform of `ncols` interleaved sequences
+// of plan P: two buffers, the n roots, the Rader tables.
+__host__ __device__ inline int fft_lds_slots(const FftPlan& P, int ncols) {
+  return 2 * P.n * ncols + P.n + P.rader_fixed + P.rader_per_col * ncols;
+}
 
 struct InitField {
   int h, w, hp, pad;
@@ -124,14 +151,15 @@ __global__ void k_prepare(const PrepDesc* P);
 __global__ void k_crop(const PrepDesc* P);
 
 // holo.hip
-__global__ void k_fft_rows_fwd(const InitField* F, const FftPlan* plans, const float2* roots, const int* row_prefix,
-                               int nf);
-__global__ void k_fft_cols_fwd(const InitField* F, const FftPlan* plans, const float2* roots, const int* grp_prefix,
-                               int nf);
-__global__ void k_fft_cols_inv(const InitField* F, const FftPlan* plans, const float2* roots, const int* grp_prefix,
-                               int nf);
-__global__ void k_fft_rows_inv(const InitField* F, const FftPlan* plans, const float2* roots, const int* row_prefix,
-                               int nf);
+// The init's four FFT passes (0 rows forward, 1 columns forward, 2 columns
+// inverse, 3 rows inverse) over the fields listed in `fields` (device array,
+// nf entries; `prefix` the running count of blocks, kRowGroup rows or
+// kColGroup columns per block) that share the transform class `cls`
+// (fft_init_class of their row or column plan).
+int fft_init_class(const FftPlan& p, bool rows);
+void launch_fft_pass(int pass, int cls, int nblocks, size_t lds, hipStream_t st, const InitField* F,
+                     const FftPlan* plans, const float2* roots, const int* prefix, const int* fields, int nf);
+hipError_t fft_allow_lds(int bytes);  // dynamic LDS limit of every pass kernel
 __global__ void k_fft2_rows(const float2* in, float2* out, int h, int w, FftPlan P, const float2* roots, int sign);
 __global__ void k_fft2_cols(float2* io, int h, int w, FftPlan P, const float2* roots, int sign, float scale);
 __global__ void k_difference(int op, int nc, const float* a, const float* wt, int h, int w, float* d);

@@ -39,7 +39,7 @@ ABI_SYMBOLS = (
 def build(arch: str = "gfx950") -> pathlib.Path:
     """Compile, from scratch, libhvd_gpu.so (hipcc, the kernels and the C ABI)
     and libhvd_core.so (g++, the hvd-core C++ API over it), in-tree."""
-    subprocess.run(["make", "-s", "-B", "-C", str(PKG), f"ARCH={arch}"], check=True)
+    subprocess.run(["make", "-s", "-B", "-j8", "-C", str(PKG), f"ARCH={arch}"], check=True)
     return LIB_PATH
 
 

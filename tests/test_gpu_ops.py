@@ -128,7 +128,11 @@ def test_estimate_noise_ire(engine, orc32, hw):
         assert 0.5 < got < 0.9
 
 
-@pytest.mark.parametrize("hw", [(274, 760), (320, 922), (17, 31), (1, 5), (64, 64)])
+@pytest.mark.parametrize("hw", [(274, 760), (320, 922), (17, 31), (1, 5), (64, 64),
+                                # Rader stages: 137 and 461 (one per transform), 37 * 41 and
+                                # 37 * 37 (two), 509 in columns; 4093's tables exceed a block's
+                                # LDS, so it runs as the direct pair form
+                                (922, 74), (1018, 137), (2, 1517), (3, 1369), (74, 111), (3, 4093)])
 def test_fft2d(engine, hw):
     rng = np.random.default_rng(6)
     z = (rng.normal(0, 1, hw) + 1j * rng.normal(0, 1, hw)).astype(np.complex64)

@@ -132,6 +132,59 @@ def test_holographic_init(engine, orc64, std):
     assert np.abs(y0 + (chi0 * car).real - s).max() < 1e-3
 
 
+@pytest.mark.parametrize("hw", [(53, 41), (21, 137), (242, 922), (7, 74), (288, 760)])
+def test_holographic_init_any_size(engine, orc64, hw):
+    """Planes of other sizes than the two standards take the run-time plans:
+    Rader stages (padded heights 85 = 5 * 17, 53 = 53; widths 41, 137, 922),
+    pair-form primes and mixed radices, in one batch with a standard plane."""
+    rng = np.random.default_rng(17)
+    g = synth.NTSC
+    planes, refs, outs = [], [], []
+    for h, w in [hw, (g.lines, g.width)]:
+        x = np.arange(w)
+        ph = 0.5 * np.pi * x[None, :] + np.pi * np.arange(h)[:, None]
+        s = (40 + 10 * rng.standard_normal((h, w)) + 20 * np.cos(ph + 0.3)).astype(np.float32)
+        car = np.exp(1j * ph).astype(np.complex64).astype(np.complex128)
+        refs.append((s, car) + orc64.holographic_init(s, car, g.sample_rate))
+        dchi = torch.zeros((h, w, 2), device=DEV)
+        dy = torch.zeros((h, w), device=DEV)
+        outs.append((dchi, dy))
+        planes.append(gpu.Engine.plane(dev(s), dchi, dy, carrier=cplx_dev(car.astype(np.complex64)),
+                                       sample_rate=g.sample_rate))
+    engine.holographic_init(planes)
+    for (s, car, y0_ref, chi0_ref), (dchi, dy) in zip(refs, outs):
+        assert np.abs(cplx_host(dchi) - chi0_ref).max() <= TOL_IRE
+        assert np.abs(dy.cpu().numpy() - y0_ref).max() <= TOL_IRE
+
+
+def test_holographic_init_batch_invariant(engine, orc64):
+    """A field's init has the same bits alone and in a batch with fields of
+    other sizes and transform classes."""
+    rng = np.random.default_rng(23)
+    fields = []
+    for h, w in [(288, 922), (242, 760), (53, 41), (288, 922), (21, 137)]:
+        x = np.arange(w)
+        ph = 0.5 * np.pi * x[None, :] + np.pi * np.arange(h)[:, None]
+        s = (40 + 10 * rng.standard_normal((h, w)) + 20 * np.cos(ph + 0.3)).astype(np.float32)
+        fields.append((s, np.exp(1j * ph).astype(np.complex64)))
+
+    def run(idx):
+        outs, planes = [], []
+        for i in idx:
+            s, car = fields[i]
+            dchi = torch.zeros(s.shape + (2,), device=DEV)
+            dy = torch.zeros(s.shape, device=DEV)
+            outs.append((dchi, dy))
+            planes.append(gpu.Engine.plane(dev(s), dchi, dy, carrier=cplx_dev(car), sample_rate=synth.PAL.sample_rate))
+        engine.holographic_init(planes)
+        return [(c.cpu().numpy(), y.cpu().numpy()) for c, y in outs]
+
+    batch = run(range(len(fields)))
+    for i in range(len(fields)):
+        (c1, y1), = run([i])
+        assert np.array_equal(c1, batch[i][0]) and np.array_equal(y1, batch[i][1]), i
+
+
 def test_holographic_init_symmetry_variant(engine, orc64):
     g = synth.NTSC
     s, car, th, vs = _prepared(orc64, g, synth.ntsc_field(12)[0])
