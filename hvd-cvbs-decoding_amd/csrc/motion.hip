@@ -52,6 +52,53 @@ __global__ void __launch_bounds__(256) k_sad_bm(const SadDesc* D, const int* blk
   const int yl = min(d.bh, d.h - y0), xl = min(d.bw, d.w - (x0 + mx0));
   const int ncx = 2 * d.sx + 1, ncand = ncx * (2 * d.sy + 1);
   unsigned long long best = ~0ull;
+  if (xl == d.bw && (d.bw & 1) == 0) {
+    // Full macroblock of even width: a lane takes four neighbouring
+    // candidates (dy, dx0 .. dx0 + 3), dx0 = -sx + 4 g, which share one read
+    // of their window rows as 32-bit pairs.  Their columns start at
+    // ox_j = ox0 - j with ox0 = mx0 + 2 sx - 4 g even, so candidate j's sample
+    // pairs are the dwords from u16 index ox0 - 4 shifted by 4 - j samples:
+    // whole dwords for even j, byte-aligned across two dwords for odd j.  The
+    // block's rows are read as dwords every lane of the wave shares.
+    const int ngx = (ncx + 3) / 4, ntask = ngx * (2 * d.sy + 1), nw = d.bw / 2;
+    for (int task = lane; task < ntask; task += 64) {
+      const int iy = task / ngx, g = task % ngx;
+      const int dy = iy - d.sy, dx0 = 4 * g - d.sx;
+      const int oy = d.sy - dy, s = mx0 + 2 * d.sx - 4 * g - 4;  // even, >= 0 once g <= sx / 2
+      unsigned acc[4] = {0u, 0u, 0u, 0u};
+      for (int yy = 0; yy < yl; ++yy) {
+        const uint16_t* wr = &win[yy + oy][0];
+        const uint32_t* cw = reinterpret_cast<const uint32_t*>(&cur[yy][mx0]);
+#pragma unroll 4
+        for (int k = 0; k < nw; ++k) {
+          const uint32_t cv = cw[k];
+          // window dwords k .. k + 2 from u16 index s: i0 + 5 <= ox0 + bw - 1 <= ww - 1;
+          // s >= -4, and an index below 0 belongs only to the candidates
+          // dx > sx of the last group, which are dropped (read as 0)
+          const int i0 = s + 2 * k;
+          const uint32_t w0 = i0 >= 0 ? *reinterpret_cast<const uint32_t*>(wr + i0) : 0u;
+          const uint32_t w1 = i0 + 2 >= 0 ? *reinterpret_cast<const uint32_t*>(wr + i0 + 2) : 0u;
+          const uint32_t w2 = *reinterpret_cast<const uint32_t*>(wr + i0 + 4);
+          const uint32_t p[4] = {w2, __builtin_amdgcn_alignbyte(w2, w1, 2), w1,
+                                 __builtin_amdgcn_alignbyte(w1, w0, 2)};
+          // v_sad_u16 on gfx950: |a[15:0] - b[15:0]| + |a[31:16] - b[31:16]|
+          // (unsigned) + acc, both samples of a pair in one instruction
+#pragma unroll
+          for (int j = 0; j < 4; ++j) acc[j] = __builtin_amdgcn_sad_u16(cv, p[j], acc[j]);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int dx = dx0 + j;
+        if (dx > d.sx) break;
+        const int c = iy * ncx + dx + d.sx;
+        const unsigned m = (unsigned)(abs(dy) + abs(dx));
+        const unsigned long long key =
+            ((unsigned long long)acc[j] << 32) | ((unsigned long long)m << 16) | (unsigned)c;
+        best = key < best ? key : best;
+      }
+    }
+  } else
   for (int c = lane; c < ncand; c += 64) {
     const int dy = c / ncx - d.sy, dx = c % ncx - d.sx;
     // ref(y - dy, x - dx) lives at win[(y - y0) + sy - dy][(x - x0) + sx - dx]

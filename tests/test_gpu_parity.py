@@ -340,7 +340,8 @@ def test_sad_block_match_sequence(engine, orc64):
 
 
 @pytest.mark.parametrize("shape,blk,srch", [((17, 23), (8, 16), (4, 16)), ((8, 16), (8, 16), (0, 0)),
-                                             ((40, 70), (16, 32), (16, 32)), ((33, 5), (4, 4), (2, 3))])
+                                             ((40, 70), (16, 32), (16, 32)), ((33, 5), (4, 4), (2, 3)),
+                                             ((29, 45), (8, 5), (3, 6)), ((24, 96), (8, 2), (1, 1))])
 def test_sad_block_match_edges(engine, orc64, shape, blk, srch):
     rng = np.random.default_rng(5)
     ref_p = rng.integers(0, 65536, size=shape, dtype=np.uint16)
