@@ -2732,6 +2732,20 @@ std::vector<int> range_from_centre(int lo, int hi) {
 
 inline unsigned blocks_for(size_t n) { return (unsigned)((n + 255) / 256); }
 
+// SSD of every (pair, tile, candidate): a thread each for small tiles, a wave
+// each (parallel products, the same sequential sum) from 16 x 16 up.
+void launch_me_ssd(hipStream_t st, int n, int h, int w, int tile, const float* P, const int* dys, const int* dxs,
+                   int ndy, int ndx, const float* bdy, const float* bdx, float* se) {
+  const int th = (h + tile - 1) / tile, tw = (w + tile - 1) / tile;
+  const size_t items = (size_t)n * th * tw * ndy * ndx;
+  if (tile >= 16)
+    hipLaunchKernelGGL(k_me_ssd_wave, dim3((unsigned)((items + 3) / 4)), dim3(256), 0, st, n, h, w, tile, P, dys, dxs,
+                       ndy, ndx, bdy, bdx, se);
+  else
+    hipLaunchKernelGGL(k_me_ssd, dim3(blocks_for(items)), dim3(256), 0, st, n, h, w, tile, P, dys, dxs, ndy, ndx,
+                       bdy, bdx, se);
+}
+
 }  // namespace
 
 extern "C" {
@@ -2793,8 +2807,7 @@ int hvd_gpu_estimate_motion(hvd_gpu_ctx* ctx, const float* const* y_ref, const f
     hipLaunchKernelGGL(k_me_decimate, dim3(blocks_for((size_t)n * 2 * hh * ww)), dim3(256), 0, st, n, h, w, f,
                        (const float*)P, Pd);
     LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_me_ssd, dim3(blocks_for((size_t)n * nct * nrc * nrc)), dim3(256), 0, st, n, hh, ww, ct,
-                       (const float*)Pd, drc, drc, nrc, nrc, (const float*)nullptr, (const float*)nullptr, se);
+    launch_me_ssd(st, n, hh, ww, ct, Pd, drc, drc, nrc, nrc, nullptr, nullptr, se);
     LAUNCH_CHECK(ctx);
     hipLaunchKernelGGL(k_me_select, dim3(blocks_for((size_t)n * nct)), dim3(256), 0, st, n, nct, drc, drc, nrc, nrc,
                        (const float*)se, (const float*)nullptr, (const float*)nullptr, cdy, cdx, cbest, (float*)nullptr);
@@ -2804,8 +2817,7 @@ int hvd_gpu_estimate_motion(hvd_gpu_ctx* ctx, const float* const* y_ref, const f
                        (const float*)cdy, (const float*)cdx, bdy, bdx);
     LAUNCH_CHECK(ctx);
     // (4) +/-3 refinement around the coarse vector (:431-435)
-    hipLaunchKernelGGL(k_me_ssd, dim3(blocks_for((size_t)n * nt * nr3 * nr3)), dim3(256), 0, st, n, h, w, tile,
-                       (const float*)P, dr3, dr3, nr3, nr3, (const float*)bdy, (const float*)bdx, se);
+    launch_me_ssd(st, n, h, w, tile, P, dr3, dr3, nr3, nr3, bdy, bdx, se);
     LAUNCH_CHECK(ctx);
     hipLaunchKernelGGL(k_me_select, dim3(blocks_for((size_t)n * nt)), dim3(256), 0, st, n, nt, dr3, dr3, nr3, nr3,
                        (const float*)se, (const float*)bdy, (const float*)bdx, mdy, mdx, best, (float*)nullptr);
@@ -2816,16 +2828,14 @@ int hvd_gpu_estimate_motion(hvd_gpu_ctx* ctx, const float* const* y_ref, const f
       hipLaunchKernelGGL(k_me_offset, dim3(blocks_for((size_t)n * nt)), dim3(256), 0, st, n * nt, (const float*)mdy,
                          (const float*)mdx, offs[k][0], offs[k][1], tdy, tdx);
       LAUNCH_CHECK(ctx);
-      hipLaunchKernelGGL(k_me_ssd, dim3(blocks_for((size_t)n * nt)), dim3(256), 0, st, n, h, w, tile,
-                         (const float*)P, dr0, dr0, 1, 1, (const float*)tdy, (const float*)tdx, se);
+      launch_me_ssd(st, n, h, w, tile, P, dr0, dr0, 1, 1, tdy, tdx, se);
       LAUNCH_CHECK(ctx);
       hipLaunchKernelGGL(k_me_select, dim3(blocks_for((size_t)n * nt)), dim3(256), 0, st, n, nt, dr0, dr0, 1, 1,
                          (const float*)se, (const float*)tdy, (const float*)tdx, jy, jx, cm[k], (float*)nullptr);
       LAUNCH_CHECK(ctx);
     }
     // (6) explicit zero-motion evaluation (:458-460)
-    hipLaunchKernelGGL(k_me_ssd, dim3(blocks_for((size_t)n * nt)), dim3(256), 0, st, n, h, w, tile, (const float*)P,
-                       dr0, dr0, 1, 1, (const float*)nullptr, (const float*)nullptr, se);
+    launch_me_ssd(st, n, h, w, tile, P, dr0, dr0, 1, 1, nullptr, nullptr, se);
     LAUNCH_CHECK(ctx);
     hipLaunchKernelGGL(k_me_select, dim3(blocks_for((size_t)n * nt)), dim3(256), 0, st, n, nt, dr0, dr0, 1, 1,
                        (const float*)se, (const float*)nullptr, (const float*)nullptr, jy, jx, best0, cost0);
@@ -2968,14 +2978,12 @@ int hvd_gpu_verify_motion(hvd_gpu_ctx* ctx, const float* const* y_ref, const flo
     // (k_me_ssd rounds a relative base) and the zero vector, one SSD each
     launch_me_blur(st, (const MotionDesc*)(base + oD), n, h, w, 2, tmp, P);
     LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_me_ssd, dim3(blocks_for((size_t)n * nt)), dim3(256), 0, st, n, h, w, tile, (const float*)P,
-                       dr0, dr0, 1, 1, (const float*)py, (const float*)px, se);
+    launch_me_ssd(st, n, h, w, tile, P, dr0, dr0, 1, 1, py, px, se);
     LAUNCH_CHECK(ctx);
     hipLaunchKernelGGL(k_me_select, dim3(blocks_for((size_t)n * nt)), dim3(256), 0, st, n, nt, dr0, dr0, 1, 1,
                        (const float*)se, (const float*)py, (const float*)px, jy, jx, bp, (float*)nullptr);
     LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_me_ssd, dim3(blocks_for((size_t)n * nt)), dim3(256), 0, st, n, h, w, tile, (const float*)P,
-                       dr0, dr0, 1, 1, (const float*)nullptr, (const float*)nullptr, se);
+    launch_me_ssd(st, n, h, w, tile, P, dr0, dr0, 1, 1, nullptr, nullptr, se);
     LAUNCH_CHECK(ctx);
     hipLaunchKernelGGL(k_me_select, dim3(blocks_for((size_t)n * nt)), dim3(256), 0, st, n, nt, dr0, dr0, 1, 1,
                        (const float*)se, (const float*)nullptr, (const float*)nullptr, jy, jx, bz, (float*)nullptr);
@@ -3151,17 +3159,18 @@ int hvd_gpu_box_blur(hvd_gpu_ctx* ctx, const float* in, int h, int w, int r, flo
     hipStream_t st = S(stream);
     const size_t np = (size_t)h * w;
     Carve cv;
-    const size_t o_tmp = cv.take(sizeof(float) * 2 * np), o_out = cv.take(sizeof(float) * 2 * np);
+    const size_t o_tmp = cv.take(sizeof(float) * np);
     char* ws = ensure_ws(ctx, cv.off, st);
     const MotionDesc md{in, in};
     Stage stg;
     const size_t oD = stg.put(&md, 1);
     Slot* sl;
     char* base = upload(ctx, stg, st, &sl);
-    launch_me_blur(st, (const MotionDesc*)(base + oD), 1, h, w, r, (float*)(ws + o_tmp), (float*)(ws + o_out));
+    // the column pass reads `in` whole before the row pass writes `out`, so
+    // in place is fine
+    launch_me_blur_planes(st, (const MotionDesc*)(base + oD), 1, h, w, r, (float*)(ws + o_tmp), out);
     LAUNCH_CHECK(ctx);
     arm(ctx, sl, st);
-    HIPCHK(ctx, hipMemcpyAsync(out, ws + o_out, sizeof(float) * np, hipMemcpyDeviceToDevice, st));
   });
 }
 
@@ -3213,13 +3222,18 @@ void upsample_confidence(hvd_gpu_ctx* ctx, const float* conf, int th, int tw, in
     vectors_per_pixel(ctx, c2, c2, th, tw, tile, out_h, out_w, out, nullptr, sy, sx, st);
     return;
   }
-  float* px = (float*)(ws + cv.take(sizeof(float) * 2 * np));  // the plane and a zero partner
-  float* tmp = (float*)(ws + cv.take(sizeof(float) * 2 * np));
-  float* bl = (float*)(ws + cv.take(sizeof(float) * 2 * np));
+  float* px = (float*)(ws + cv.take(sizeof(float) * np));
+  float* tmp = (float*)(ws + cv.take(sizeof(float) * np));
   hipLaunchKernelGGL(k_conf_nearest, dim3(blocks_for(np)), dim3(256), 0, st, th, tw, tile, out_h, out_w, conf, px);
   LAUNCH_CHECK(ctx);
-  blur_planes(ctx, {MotionDesc{px, px}}, out_h, out_w, 8, tmp, bl, st);
-  HIPCHK(ctx, hipMemcpyAsync(out, bl, sizeof(float) * np, hipMemcpyDeviceToDevice, st));
+  Stage stg;
+  const MotionDesc md{px, px};
+  const size_t oD = stg.put(&md, 1);
+  Slot* sl;
+  char* base = upload(ctx, stg, st, &sl);
+  launch_me_blur_planes(st, (const MotionDesc*)(base + oD), 1, out_h, out_w, 8, tmp, out);  // one plane, into out
+  LAUNCH_CHECK(ctx);
+  arm(ctx, sl, st);
 }
 
 }  // namespace

@@ -315,7 +315,11 @@ __global__ void k_me_blur_rows(int npairs, int h, int w, int r, const float* tmp
 // Both passes of BoxBlur (motion.cpp:21-55) over npairs * 2 planes (D), the
 // along-y pass into tmp, the result into out (planes one after another).
 void launch_me_blur(hipStream_t st, const MotionDesc* D, int npairs, int h, int w, int r, float* tmp, float* out);
+void launch_me_blur_planes(hipStream_t st, const MotionDesc* D, int nplanes, int h, int w, int r, float* tmp,
+                           float* out);
 __global__ void k_me_decimate(int npairs, int h, int w, int f, const float* in, float* out);
+__global__ void k_me_ssd_wave(int npairs, int h, int w, int tile, const float* planes, const int* dys,
+                              const int* dxs, int ndy, int ndx, const float* base_dy, const float* base_dx, float* se);
 __global__ void k_me_ssd(int npairs, int h, int w, int tile, const float* planes, const int* dys, const int* dxs,
                          int ndy, int ndx, const float* base_dy, const float* base_dx, float* se);
 __global__ void k_me_select(int npairs, int nt, const int* dys, const int* dxs, int ndy, int ndx, const float* se,

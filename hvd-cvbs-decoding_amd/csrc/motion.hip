@@ -158,8 +158,11 @@ __device__ inline const float* mplane(const MotionDesc& d, int which) {
 // edge) added in hvd-core's order, out = (c_hi - c_lo) * (1 / (2r + 1)).
 // Block: 64 consecutive columns of one plane, one thread per column; the
 // rows come through LDS in chunks of kBlurC (plus the window's reach on
-// either side) so a chunk's loads are all in flight at once.
+// either side).  Radii up to kBlurPrefR: a chunk's rows are loaded into
+// registers all at once, the next chunk's before the current one is summed;
+// larger radii load their chunks in batches of 16.
 // Grid: (ceil(w / 64), npairs * 2).
+constexpr int kBlurPrefR = 8, kBlurPrefRows = 32 + 2 * kBlurPrefR + 1;
 __global__ void __launch_bounds__(64) k_me_blur_cols(const MotionDesc* D, int npairs, int h, int w, int r,
                                                      float* tmp) {
   __shared__ float sm[kBlurS][64];
@@ -167,25 +170,43 @@ __global__ void __launch_bounds__(64) k_me_blur_cols(const MotionDesc* D, int np
   const float* a = mplane(D[pp >> 1], pp & 1);
   float* o = tmp + (size_t)pp * h * w;
   const float norm = 1.0f / (float)(2 * r + 1);
-  const int L = r + 1, SW = kBlurC + 2 * r + 1;
+  const bool pref = r <= kBlurPrefR;
+  const int CH = pref ? 32 : kBlurC;  // rows per chunk
+  const int L = r + 1, SW = CH + 2 * r + 1;
+  float v[kBlurPrefRows];
+  auto fetch = [&](int y0) {
+#pragma unroll
+    for (int u = 0; u < kBlurPrefRows; ++u) {
+      const int y = y0 - L + u;
+      v[u] = (u < SW && x < w && y >= 0 && y < h) ? a[(size_t)y * w + x] : 0.0f;
+    }
+  };
+  if (pref) fetch(0);
   double c_lo = 0.0, c_hi = 0.0;
-  for (int y0 = 0; y0 < h; y0 += kBlurC) {
+  for (int y0 = 0; y0 < h; y0 += CH) {
     __syncthreads();
-    // the chunk's loads in batches of 16, all in flight before their stores
-    for (int k0 = 0; k0 < SW; k0 += 16) {
-      float v[16];
+    if (pref) {
 #pragma unroll
-      for (int u = 0; u < 16; ++u) {
-        const int y = y0 - L + k0 + u;
-        v[u] = (k0 + u < SW && x < w && y >= 0 && y < h) ? a[(size_t)y * w + x] : 0.0f;
+      for (int u = 0; u < kBlurPrefRows; ++u)
+        if (u < SW) sm[u][t] = v[u];
+      if (y0 + CH < h) fetch(y0 + CH);
+    } else {
+      // the chunk's loads in batches of 16, all in flight before their stores
+      for (int k0 = 0; k0 < SW; k0 += 16) {
+        float b[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          const int y = y0 - L + k0 + u;
+          b[u] = (k0 + u < SW && x < w && y >= 0 && y < h) ? a[(size_t)y * w + x] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+          if (k0 + u < SW) sm[k0 + u][t] = b[u];
       }
-#pragma unroll
-      for (int u = 0; u < 16; ++u)
-        if (k0 + u < SW) sm[k0 + u][t] = v[u];
     }
     __syncthreads();
     if (x >= w) continue;
-    const int yend = min(y0 + kBlurC, h);
+    const int yend = min(y0 + CH, h);
     int y = y0;
     if (y == 0) {
       const int y1 = min(r + 1, h);
@@ -299,8 +320,16 @@ __global__ void __launch_bounds__(64) k_me_blur_rows(int npairs, int h, int w, i
 }
 
 void launch_me_blur(hipStream_t st, const MotionDesc* D, int npairs, int h, int w, int r, float* tmp, float* out) {
-  hipLaunchKernelGGL(k_me_blur_cols, dim3((w + 63) / 64, npairs * 2), dim3(64), 0, st, D, npairs, h, w, r, tmp);
-  const size_t rows = (size_t)npairs * 2 * h;
+  launch_me_blur_planes(st, D, 2 * npairs, h, w, r, tmp, out);
+}
+
+// The first `nplanes` planes of D's pairs (plane p: pair p / 2, ref then cur)
+// into out (nplanes planes after one another), tmp holding as many.
+void launch_me_blur_planes(hipStream_t st, const MotionDesc* D, int nplanes, int h, int w, int r, float* tmp,
+                           float* out) {
+  const int npairs = (nplanes + 1) / 2;
+  hipLaunchKernelGGL(k_me_blur_cols, dim3((w + 63) / 64, nplanes), dim3(64), 0, st, D, npairs, h, w, r, tmp);
+  const size_t rows = (size_t)nplanes * h;
   hipLaunchKernelGGL(k_me_blur_rows, dim3((unsigned)rows), dim3(64), 0, st, npairs, h, w, r, (const float*)tmp, out);
 }
 
@@ -348,6 +377,55 @@ __global__ void k_me_ssd(int npairs, int h, int w, int tile, const float* planes
     }
   }
   se[idx] = acc;
+}
+
+// The same SSDs with one wave per (pair, tile, candidate), for full-size
+// tiles: the lanes form the squared differences of kMeChunk pixels at a time
+// into LDS, in raster order, and lane 0 adds them one after another, so the
+// sum has the bits of the sequential loop above while the loads and products
+// run in parallel.  Four waves per block, each with its own LDS row.
+constexpr int kMeChunk = 1024;
+__global__ void __launch_bounds__(256) k_me_ssd_wave(int npairs, int h, int w, int tile, const float* planes,
+                                                     const int* dys, const int* dxs, int ndy, int ndx,
+                                                     const float* base_dy, const float* base_dx, float* se) {
+  __shared__ float sq[4][kMeChunk];
+  const int th = (h + tile - 1) / tile, tw = (w + tile - 1) / tile;
+  const int nc = ndy * ndx, nt = th * tw;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long long idx = (long long)blockIdx.x * 4 + wv;
+  if (idx >= (long long)npairs * nt * nc) return;  // whole waves leave; no block barrier below
+  const int c = (int)(idx % nc), t = (int)((idx / nc) % nt), pi = (int)(idx / ((long long)nc * nt));
+  const int ty = t / tw, tx = t % tw;
+  int sy = dys[c / ndx], sx = dxs[c % ndx];
+  if (base_dy) {
+    sy += (int)lroundf(base_dy[(size_t)pi * nt + t]);
+    sx += (int)lroundf(base_dx[(size_t)pi * nt + t]);
+  }
+  const float* B = planes + ((size_t)pi * 2 + 0) * h * w;
+  const float* A = planes + ((size_t)pi * 2 + 1) * h * w;
+  const int y0 = ty * tile, x0 = tx * tile;
+  const int y_end = min(y0 + tile, h), x_end = min(x0 + tile, w);
+  const int cw = x_end - x0, rows = max(1, kMeChunk / cw);
+  float* q = sq[wv];
+  float acc = 0.0f;
+  for (int ya = y0; ya < y_end; ya += rows) {
+    const int nr = min(rows, y_end - ya), n = nr * cw;
+    for (int e = lane; e < n; e += 64) {
+      const int y = ya + e / cw, x = x0 + e % cw;
+      const int yy = min(max(y - sy, 0), h - 1), xx = min(max(x - sx, 0), w - 1);
+      const float diff = (A[(size_t)y * w + x] - B[(size_t)yy * w + xx]);
+      q[e] = (diff * diff);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane == 0)
+      for (int e = 0; e < n; ++e) acc = (acc + q[e]);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  if (lane == 0) se[idx] = acc;
 }
 
 // In-order candidate selection (motion.cpp:233-252): cost = se * bias,

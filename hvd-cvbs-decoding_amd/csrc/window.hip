@@ -377,44 +377,49 @@ __global__ void k_env_terms(int h, int w, const float* Y, const float2* chi, flo
 // order of adds and subtracts (a float running sum: the first three samples,
 // then per step the sample entering at x + 2 and the one leaving at x - 3;
 // outside the row 0 is added, which leaves the sum, never -0, as it is), for
-// a batch of planes: plane b is src[b] -> out[b].  Block: 64 rows of one
-// plane, one thread per row running only the sum; the row segments come
-// through LDS in chunks of 64 columns and the chunk's RMS is formed and
-// stored column by column (coalesced).  Grid: (ceil(h / 64), planes).
+// a batch of planes: plane b is src[b] -> out[b].  Block: kSmR rows of one
+// plane, threads 0 .. kSmR - 1 running one row's sum each; the row segments
+// come through LDS in chunks of 64 columns, every thread loading two columns
+// of every row with all of a chunk's loads in flight at once, and the next
+// chunk's loads issued before the current chunk is summed; the chunk's RMS is
+// formed and stored column by column (coalesced).  Grid: (ceil(h / kSmR),
+// planes).
+constexpr int kSmR = 16;
 __global__ void __launch_bounds__(64) k_smooth_rows(int h, int w, const float* const* src, float* const* out) {
 #pragma clang fp contract(off)
   constexpr int C = 64, L = 3, SW = C + 5;
-  __shared__ float sm[64][SW + 2];
-  __shared__ float sa[64][C + 1];
+  __shared__ float sm[kSmR][SW + 2];
+  __shared__ float sa[kSmR][C + 1];
   const int t = threadIdx.x;
   const float* in = src[blockIdx.y];
   float* o = out[blockIdx.y];
-  const int row0 = blockIdx.x * 64, y = row0 + t;
+  const int row0 = blockIdx.x * kSmR, y = row0 + t;
   float acc = 0.f;
+  // row j's segment of the chunk at x0: thread t holds columns t and t + 64
+  float v[kSmR][2];
+  auto fetch = [&](int x0) {
+#pragma unroll
+    for (int u = 0; u < kSmR; ++u) {
+      const int rj = row0 + u;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int k = t + 64 * q, col = x0 - L + k;
+        v[u][q] = (k < SW && rj < h && col >= 0 && col < w) ? in[(size_t)rj * w + col] : 0.f;
+      }
+    }
+  };
+  fetch(0);
   for (int x0 = 0; x0 < w; x0 += C) {
     __syncthreads();
-    // row j's segment: thread t loads columns t and t + 64 of it; rows in
-    // batches of 8, every load of a batch in flight before its stores
-    for (int j0 = 0; j0 < 64; j0 += 8) {
-      float v[8][2];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int rj = row0 + j0 + u;
+    for (int u = 0; u < kSmR; ++u)
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-          const int k = t + 64 * q, col = x0 - L + k;
-          v[u][q] = (k < SW && rj < h && col >= 0 && col < w) ? in[(size_t)rj * w + col] : 0.f;
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u)
-#pragma unroll
-        for (int q = 0; q < 2; ++q)
-          if (t + 64 * q < SW) sm[j0 + u][t + 64 * q] = v[u][q];
-    }
+      for (int q = 0; q < 2; ++q)
+        if (t + 64 * q < SW) sm[u][t + 64 * q] = v[u][q];
+    if (x0 + C < w) fetch(x0 + C);
     __syncthreads();
     const int xend = min(x0 + C, w);
-    if (y < h) {
+    if (t < kSmR && y < h) {
       int x = x0;
       if (x == 0) {
         for (int i = 0; i < min(3, w); ++i) acc += sm[t][i + L];
@@ -432,7 +437,7 @@ __global__ void __launch_bounds__(64) k_smooth_rows(int h, int w, const float* c
     const int x = x0 + t;
     if (x < w) {
       const float n_taps = (float)(min(x + 2, w - 1) - max(x - 2, 0) + 1);
-      for (int j = 0; j < 64; ++j) {
+      for (int j = 0; j < kSmR; ++j) {
         const int rj = row0 + j;
         if (rj < h) o[(size_t)rj * w + x] = sqrtf(sa[j][t] / n_taps);
       }
@@ -1121,7 +1126,7 @@ int drive(hvd_gpu_ctx* ctx, const hvd_gpu_plane* F, int nf, const int* par, int 
           if (hipMemcpyAsync(pt, rin.data(), sizeof(void*) * 2 * m, hipMemcpyHostToDevice, st) != hipSuccess ||
               hipMemcpyAsync(pt + 128, rout.data(), sizeof(void*) * 2 * m, hipMemcpyHostToDevice, st) != hipSuccess)
             return fail(ctx, HVD_GPU_ERR_HIP, "table upload failed");
-          hipLaunchKernelGGL(k_smooth_rows, dim3((h + 63) / 64, 2 * m), dim3(64), 0, st, h, w,
+          hipLaunchKernelGGL(k_smooth_rows, dim3((h + kSmR - 1) / kSmR, 2 * m), dim3(64), 0, st, h, w,
                              (const float* const*)pt, (float* const*)(pt + 128));
           if (!X.launched()) return X.rc;
           if (hipMemcpyAsync(pt, vin.data(), sizeof(void*) * m, hipMemcpyHostToDevice, st) != hipSuccess ||
