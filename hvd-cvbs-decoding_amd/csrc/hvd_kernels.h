@@ -304,10 +304,8 @@ __global__ void k_zero_d(double* a, int n);
 
 // motion.hip
 __global__ void k_sad_bm(const SadDesc* D, const int* blk_prefix, int nf);
-// The box blur's LDS chunks: kBlurC rows (columns) per chunk plus the
-// window's reach on either side, radius at most kBlurMaxR (row stride kBlurS,
-// odd, >= kBlurC + 2 kBlurMaxR + 1).
-constexpr int kBlurC = 64, kBlurMaxR = 32, kBlurS = 131;
+// The box blur's largest radius.
+constexpr int kBlurMaxR = 32;
 // Rows up to kBlurMaxW samples take the along-x pass's parallel prefix sum.
 constexpr int kBlurMaxW = 2048;
 __global__ void k_me_blur_cols(const MotionDesc* D, int npairs, int h, int w, int r, float* tmp);

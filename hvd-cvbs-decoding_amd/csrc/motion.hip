@@ -12,6 +12,8 @@
 // LDS copy of the reference search window; each lane evaluates candidates
 // lane, lane + 64, ... of its wavefront's macroblock and the (cost, |v|,
 // index) keys are min-reduced across the wave.
+#include <algorithm>
+
 #include "hvd_device.h"
 #include "hvd_kernels.h"
 
@@ -156,72 +158,75 @@ __device__ inline const float* mplane(const MotionDesc& d, int which) {
 // Box blur pass 1 (motion.cpp:21-36, BoxBlur1DRows): along y for every
 // column, FP64 running sums c_hi (the window's far edge) and c_lo (its near
 // edge) added in hvd-core's order, out = (c_hi - c_lo) * (1 / (2r + 1)).
-// Block: 64 consecutive columns of one plane, one thread per column; the
-// rows come through LDS in chunks of kBlurC (plus the window's reach on
-// either side).  Radii up to kBlurPrefR: a chunk's rows are loaded into
-// registers all at once, the next chunk's before the current one is summed;
-// larger radii load their chunks in batches of 16.
-// Grid: (ceil(w / 64), npairs * 2).
-constexpr int kBlurPrefR = 8, kBlurPrefRows = 32 + 2 * kBlurPrefR + 1;
-__global__ void __launch_bounds__(64) k_me_blur_cols(const MotionDesc* D, int npairs, int h, int w, int r,
-                                                     float* tmp) {
-  __shared__ float sm[kBlurS][64];
-  const int t = threadIdx.x, x = blockIdx.x * 64 + t, pp = blockIdx.y;
+//
+// When no partial sum of the column can round, c_hi and c_lo are exact, so
+// c_hi - c_lo at row y is the exact sum of the window around y, which any
+// order of adds forms with the same bits.  That holds when every sample is a
+// multiple of 2^e (e the exponent of the smallest ulp among the column's
+// nonzero floats) and h max|a| < 2^(53 + e) (the certificate of the row pass
+// below).  The block's waves take the row segments of 64 columns: each
+// checks its segment, the column's verdict is combined through LDS, and a
+// certified column's segments are formed in parallel, each from its own
+// running window sum; a column that fails it (a nonzero sample below about
+// 2^-20 of the column's largest, in practice) is summed top to bottom by the
+// first wave, as hvd-core does.
+// Block: 64 consecutive columns of one plane x (blockDim / 64) segments.
+// Grid: (ceil(w / 64), planes).
+constexpr int kBlurSegs = 8;
+__global__ void __launch_bounds__(64 * kBlurSegs) k_me_blur_cols(const MotionDesc* D, int npairs, int h, int w, int r,
+                                                                 float* tmp) {
+  __shared__ float smx[kBlurSegs][64];
+  __shared__ int sem[kBlurSegs][64];
+  const int t = threadIdx.x & 63, seg = threadIdx.x >> 6, nseg = blockDim.x >> 6;
+  const int x = blockIdx.x * 64 + t, pp = blockIdx.y;
+  const bool col = x < w;
   const float* a = mplane(D[pp >> 1], pp & 1);
   float* o = tmp + (size_t)pp * h * w;
   const float norm = 1.0f / (float)(2 * r + 1);
-  const bool pref = r <= kBlurPrefR;
-  const int CH = pref ? 32 : kBlurC;  // rows per chunk
-  const int L = r + 1, SW = CH + 2 * r + 1;
-  float v[kBlurPrefRows];
-  auto fetch = [&](int y0) {
+  const int ya = (int)((long long)seg * h / nseg), yb = (int)((long long)(seg + 1) * h / nseg);
+  float mx = 0.0f;
+  int emin = 1 << 20;
+  for (int y0 = ya; col && y0 < yb; y0 += 16) {
+    float v[16];
 #pragma unroll
-    for (int u = 0; u < kBlurPrefRows; ++u) {
-      const int y = y0 - L + u;
-      v[u] = (u < SW && x < w && y >= 0 && y < h) ? a[(size_t)y * w + x] : 0.0f;
+    for (int u = 0; u < 16; ++u) v[u] = y0 + u < yb ? a[(size_t)(y0 + u) * w + x] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      mx = fmaxf(mx, fabsf(v[u]));
+      if (v[u] != 0.0f) emin = min(emin, max((int)((__float_as_uint(v[u]) >> 23) & 255u), 1) - 150);
     }
-  };
-  if (pref) fetch(0);
+  }
+  smx[seg][t] = mx;
+  sem[seg][t] = emin;
+  __syncthreads();
+  if (!col) return;  // no barriers below
+  for (int k = 0; k < nseg; ++k) {
+    mx = fmaxf(mx, smx[k][t]);
+    emin = min(emin, sem[k][t]);
+  }
+  const bool exact = emin == (1 << 20) || (double)h * (double)mx < ldexp(1.0, 53 + emin);
+  if (exact) {
+    double s = 0.0;  // the window sum of row ya (a subset sum: exact)
+    for (int k = max(ya - r, 0); k <= min(ya + r, h - 1); ++k) s += (double)a[(size_t)k * w + x];
+    for (int y = ya; y < yb; ++y) {
+      o[(size_t)y * w + x] = (float)(s * (double)norm);
+      const int kin = y + r + 1, kout = y - r;
+      if (kin < h) s += (double)a[(size_t)kin * w + x];
+      if (kout >= 0) s -= (double)a[(size_t)kout * w + x];
+    }
+    return;
+  }
+  if (seg != 0) return;
+  // rows beyond the plane add 0, which leaves the sums (never -0) as they are
   double c_lo = 0.0, c_hi = 0.0;
-  for (int y0 = 0; y0 < h; y0 += CH) {
-    __syncthreads();
-    if (pref) {
-#pragma unroll
-      for (int u = 0; u < kBlurPrefRows; ++u)
-        if (u < SW) sm[u][t] = v[u];
-      if (y0 + CH < h) fetch(y0 + CH);
-    } else {
-      // the chunk's loads in batches of 16, all in flight before their stores
-      for (int k0 = 0; k0 < SW; k0 += 16) {
-        float b[16];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) {
-          const int y = y0 - L + k0 + u;
-          b[u] = (k0 + u < SW && x < w && y >= 0 && y < h) ? a[(size_t)y * w + x] : 0.0f;
-        }
-#pragma unroll
-        for (int u = 0; u < 16; ++u)
-          if (k0 + u < SW) sm[k0 + u][t] = b[u];
-      }
-    }
-    __syncthreads();
-    if (x >= w) continue;
-    const int yend = min(y0 + CH, h);
-    int y = y0;
-    if (y == 0) {
-      const int y1 = min(r + 1, h);
-      for (int i = 0; i < y1; ++i) c_hi += (double)sm[i + L][t];
-      o[x] = (float)((c_hi - c_lo) * (double)norm);
-      ++y;
-    }
-    // rows beyond the plane hold 0 in LDS: adding it leaves the sums (never
-    // -0) as they are
-#pragma unroll 8
-    for (; y < yend; ++y) {
-      c_hi += (double)sm[y + r - y0 + L][t];
-      c_lo += (double)sm[y - r - 1 - y0 + L][t];
-      o[(size_t)y * w + x] = (float)((c_hi - c_lo) * (double)norm);
-    }
+  const int y1 = min(r + 1, h);
+  for (int i = 0; i < y1; ++i) c_hi += (double)a[(size_t)i * w + x];
+  o[x] = (float)((c_hi - c_lo) * (double)norm);
+  for (int y = 1; y < h; ++y) {
+    const int kin = y + r, kout = y - r - 1;
+    c_hi += kin < h ? (double)a[(size_t)kin * w + x] : 0.0;
+    c_lo += kout >= 0 ? (double)a[(size_t)kout * w + x] : 0.0;
+    o[(size_t)y * w + x] = (float)((c_hi - c_lo) * (double)norm);
   }
 }
 
@@ -328,7 +333,8 @@ void launch_me_blur(hipStream_t st, const MotionDesc* D, int npairs, int h, int 
 void launch_me_blur_planes(hipStream_t st, const MotionDesc* D, int nplanes, int h, int w, int r, float* tmp,
                            float* out) {
   const int npairs = (nplanes + 1) / 2;
-  hipLaunchKernelGGL(k_me_blur_cols, dim3((w + 63) / 64, nplanes), dim3(64), 0, st, D, npairs, h, w, r, tmp);
+  const int nseg = std::min(kBlurSegs, std::max(1, h / 32));  // row segments of a certified column
+  hipLaunchKernelGGL(k_me_blur_cols, dim3((w + 63) / 64, nplanes), dim3(64 * nseg), 0, st, D, npairs, h, w, r, tmp);
   const size_t rows = (size_t)nplanes * h;
   hipLaunchKernelGGL(k_me_blur_rows, dim3((unsigned)rows), dim3(64), 0, st, npairs, h, w, r, (const float*)tmp, out);
 }

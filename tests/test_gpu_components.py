@@ -110,6 +110,25 @@ def test_box_blur_rows_that_round(engine, orc32):
         np.testing.assert_array_equal(got, orc32.box_blur(a, r))
 
 
+def test_box_blur_columns_that_round(engine, orc32):
+    """Columns whose running sums round take the along-y pass's top-to-bottom
+    fallback, the others its row segments: the same cases down the columns
+    of a tall plane (several segments), beside ordinary columns; every output
+    must match hvd-core's order bit for bit."""
+    rng = np.random.default_rng(14)
+    a = rng.normal(50, 20, size=(300, 70)).astype(np.float32)
+    a[::7, 0] = 1e-9
+    a[:, 1] = 0.0
+    a[10, 2], a[11, 2], a[200, 2] = 1e30, 1.0, -1e30
+    a[::3, 3] = np.float32(1e-40)
+    a[:, 4] = rng.normal(0, 1e-6, 300).astype(np.float32)
+    a[::2, 5] *= -1
+    a[:, 6] = 2.0 ** np.arange(300) % 7  # small integers: exact in any order
+    for r in (1, 2, 8, 31):
+        got = engine.box_blur(dev(a), r).cpu().numpy()
+        np.testing.assert_array_equal(got, orc32.box_blur(a, r))
+
+
 def test_chroma_plane(engine):
     rng = np.random.default_rng(13)
     s = rng.normal(size=(9, 31)).astype(np.float32)
