@@ -882,6 +882,10 @@ void run_launches(hvd_gpu_ctx* ctx, LaunchList& L, bool graph, hipStream_t st) {
     }
 }
 
+// A field's standard for the AUTO aniso grouping: its own (DevField::pal,
+// set where one solve mixes standards) or the solve's.
+int field_pal(const DevField& f, int is_pal) { return f.pal ? f.pal - 1 : is_pal; }
+
 // Per-field CG state of a solve with these parameters (floors: hvd-core's).
 FieldCG field_cg_init(const DevField& f, const hvd_solver_params& p, int n_inner, double tol) {
   FieldCG s{};
@@ -895,7 +899,7 @@ FieldCG field_cg_init(const DevField& f, const hvd_solver_params& p, int n_inner
   s.k = (float)p.structure_coupling;
   s.aniso_forced = (float)p.chroma_aniso;
   s.diag = (float)p.diag_prior;
-  s.is_pal = p.is_pal;
+  s.is_pal = field_pal(f, p.is_pal);
   s.n_inner = n_inner;
   return s;
 }
@@ -1056,23 +1060,29 @@ std::vector<std::pair<int, int>> solve_groups(const std::vector<DevField>& F, si
 }
 
 // The launch shape for a batch: the fixed tuning, else the autotuned entry for
-// its field shape (when every field has that width and within two lines of
-// that height, as the two fields of a frame do): all of it for the same
-// field count, else the strip height of the entry with the nearest count (its
-// group size belongs to that count; the default applies); else the built-in
-// policy.
+// the shape of its largest fields (the fields of that width within two lines
+// of that height and of its standard, as the two fields of a frame are; in a
+// mixed batch, the standard whose fields are larger): all of it for the same
+// count of such fields, else the strip height of the entry with the nearest
+// count (its group size belongs to that count; the default applies); else the
+// built-in policy.
 SolverTuning tuning_for(const hvd_gpu_ctx* ctx, const std::vector<DevField>& F, int is_pal) {
   SolverTuning t = ctx->fixed;
   if ((t.rows > 0 && t.group_px > 0) || F.empty() || ctx->tuned.empty()) return t;
+  size_t i0 = 0;
+  for (size_t i = 1; i < F.size(); ++i)
+    if ((long)F[i].h * F[i].w > (long)F[i0].h * F[i0].w) i0 = i;
+  const DevField& f0 = F[i0];
+  const int pal0 = field_pal(f0, is_pal) ? 1 : 0;
+  int n = 0;
   for (const DevField& f : F)
-    if (f.w != F[0].w || std::abs(f.h - F[0].h) > 2) return t;
+    if (f.w == f0.w && std::abs(f.h - f0.h) <= 2 && (field_pal(f, is_pal) ? 1 : 0) == pal0) ++n;
   auto it = ctx->tuned.end();
   for (int dh : {0, -1, 1, -2, 2}) {
-    it = ctx->tuned.find({F[0].h + dh, F[0].w, is_pal ? 1 : 0});
+    it = ctx->tuned.find({f0.h + dh, f0.w, pal0});
     if (it != ctx->tuned.end()) break;
   }
   if (it == ctx->tuned.end() || it->second.empty()) return t;
-  const int n = (int)F.size();
   const auto exact = it->second.find(n);
   const SolverTuning* best = exact != it->second.end() ? &exact->second : nullptr;
   double bd = 1e300;
@@ -1110,7 +1120,7 @@ GroupLayout group_layout(Carve cv, const DevField* F, int n, int is_pal) {
     L.d[i] = cv.take(sizeof(float2) * np * 2);
     L.w[i] = cv.take(sizeof(float4) * np);
     strips += (size_t)solver_strips(F[i].h, F[i].w, kMinStripRows);
-    L.sp[i + 1] = L.sp[i] + aniso_samples(F[i].h, F[i].w, is_pal);
+    L.sp[i + 1] = L.sp[i] + aniso_samples(F[i].h, F[i].w, field_pal(F[i], is_pal));
   }
   L.part = cv.take(sizeof(double) * 2 * kCgPartials * strips);  // two launch slots
   L.gx = cv.take(sizeof(float) * (L.sp[n] + 1));
@@ -1148,7 +1158,7 @@ PxLayout px_layout(Carve cv, const std::vector<DevField>& F, const std::vector<i
     const size_t np = (size_t)F[i].h * F[i].w;
     const int nc = (int)((np + kPxChunk - 1) / kPxChunk);
     L.cp[i + 1] = L.cp[i] + nc;
-    L.sp[i + 1] = L.sp[i] + aniso_samples(F[i].h, F[i].w, is_pal);
+    L.sp[i + 1] = L.sp[i] + aniso_samples(F[i].h, F[i].w, field_pal(F[i], is_pal));
     L.chi1[i] = cv.take(sizeof(float2) * np);
     L.d[i] = cv.take(sizeof(float2) * np * 2);
     L.g[i] = cv.take(sizeof(float2) * np);
@@ -2211,6 +2221,13 @@ namespace {
 
 // Workspace layout of the field driver, carved after `cv`; `end` covers the
 // solver's carve too, so one ensure_ws(end) sizes the whole call.
+// The order of a decode batch's solve: PAL-family fields first, then NTSC,
+// each in batch order, so the solve groups (runs of consecutive fields) mix
+// the standards at one boundary at most.  Each field keeps its own buffers.
+void solve_order(std::vector<DevField>& F) {
+  std::stable_partition(F.begin(), F.end(), [](const DevField& f) { return f.pal == 2; });
+}
+
 struct DecodeLayout {
   std::vector<size_t> ire, th, vs, s, row, vsa, col, spec, var, tha;
   Carve solver;  // where refine_planes starts carving
@@ -2221,7 +2238,7 @@ DecodeLayout decode_layout(const hvd_gpu_ctx* ctx, int n, const hvd_field_geomet
                            bool diag) {
   DecodeLayout L;
   for (auto* v : {&L.ire, &L.th, &L.vs, &L.s, &L.row, &L.vsa, &L.col, &L.spec, &L.var, &L.tha}) v->resize(n);
-  std::vector<DevField> shapes(n), shapes_pal, shapes_ntsc;
+  std::vector<DevField> shapes(n);
   for (int i = 0; i < n; ++i) {
     const hvd_field_geometry& g = geom[i];
     const int lines = last_line(g) - g.first_active_field_line, width = g.active_video_end - g.active_video_start;
@@ -2239,11 +2256,12 @@ DecodeLayout decode_layout(const hvd_gpu_ctx* ctx, int n, const hvd_field_geomet
     L.tha[i] = cv.take(sizeof(float) * lines);
     shapes[i].h = lines;
     shapes[i].w = width;
-    (uses_vswitch(g) ? shapes_pal : shapes_ntsc).push_back(shapes[i]);
+    shapes[i].pal = 1 + (uses_vswitch(g) ? 1 : 0);
   }
   L.solver = cv;
-  // the solve runs per standard (see decode_fields_run), each from L.solver
-  L.end = std::max(refine_ws_end(ctx, cv, shapes_pal, 1, diag), refine_ws_end(ctx, cv, shapes_ntsc, 0, diag));
+  // one solve over the batch (see decode_fields_run), from L.solver
+  solve_order(shapes);
+  L.end = refine_ws_end(ctx, cv, shapes, 0, diag);
   return L;
 }
 
@@ -2387,26 +2405,17 @@ void decode_fields_run(hvd_gpu_ctx* ctx, const uint16_t* const* raw, const float
     f.colph = IF[i].colph;
     f.chi[0] = IF[i].chi0;
     f.y = y_out[i];
+    f.pal = 1 + (uses_vswitch(geom[i]) ? 1 : 0);  // the AUTO aniso grouping per field
     F[i] = f;
     (uses_vswitch(geom[i]) ? any_pal : any_ntsc) = true;
   }
+  // one solve over the whole batch, NTSC and PAL fields side by side (in
+  // solve_order, as decode_layout sized it)
+  solve_order(F);
   hvd_solver_params pp = *p;
-  if (any_pal && any_ntsc) {
-    // Mixed batch: the AUTO aniso line grouping differs per standard, so
-    // solve each standard as its own batch on the same stream.
-    std::vector<DevField> Fp, Fn;
-    for (int i = 0; i < n; ++i) (uses_vswitch(geom[i]) ? Fp : Fn).push_back(F[i]);
-    Carve c1 = L.solver;
-    pp.is_pal = 1;
-    refine_planes(ctx, Fp, pp, st, c1, RefineOut{});
-    Carve c2 = L.solver;
-    pp.is_pal = 0;
-    refine_planes(ctx, Fn, pp, st, c2, RefineOut{});
-  } else {
-    Carve c = L.solver;
-    pp.is_pal = any_pal ? 1 : 0;
-    refine_planes(ctx, F, pp, st, c, RefineOut{});
-  }
+  pp.is_pal = any_pal && !any_ntsc ? 1 : 0;
+  Carve c = L.solver;
+  refine_planes(ctx, F, pp, st, c, RefineOut{});
 }
 
 int decode_fields(hvd_gpu_ctx* ctx, const uint16_t* const* raw, const float* const* field_ire, int n,

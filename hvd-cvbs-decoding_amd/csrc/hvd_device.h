@@ -51,6 +51,8 @@ struct DevField {
   // the gradient, so the CG solves Hessian . chi = rhs
   const float2* rhs;
   float2* save;        // when set: chi at the start of each IRLS stage (stage-major planes)
+  int pal;             // 1 + (PAL-family field: the AUTO aniso's 4-line grouping), or 0: the solve's setting
+  int pad_;
 };
 
 // The scalar state of one field's CG loop after a launch (variational.cpp:
