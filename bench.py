@@ -525,6 +525,8 @@ def main():
                     help="fix the solver's strip height and solve-group pixels instead of autotuning")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 512 MiB device-to-device copy and exit (reference point for PMC byte counters)")
+    ap.add_argument("--only-line", choices=("window3d", "temporal"), default="",
+                    help="run one secondary line alone and print it (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU layout is described in "
@@ -544,6 +546,11 @@ def main():
         log(f"[bench] copied {n} bytes device to device")
         return
     eng = gpu.Engine(0)
+    if args.only_line:
+        line = window3d_bench(torch, gpu, synth, eng) if args.only_line == "window3d" else temporal_bench(torch, gpu, eng)
+        eng.close()
+        print(json.dumps({args.only_line: line}), flush=True)
+        return
     g = synth.NTSC
     px_field = g.lines * g.width
 
