@@ -153,12 +153,15 @@ def cpu_baseline(torch, gpu, synth, eng, n_fields=6):
     # mode (ii): parallel_across_fields (sequence.cpp:1079-1085), every field
     # single-threaded, one field per core
     across = cpu_baseline_across_fields(ref, g, fields, ss, chi0s, cfg)
-    return {"value": round(n_fields / dt, 3), "unit": "fields/s", "cores": cores, "kind": "reference",
-            "sample": f"{n_fields} NTSC 910x263 fields: hvd-core BurstLockinPhase + VariationalRefine "
-                      f"(IRLS 4 x CG 25), OpenMP {cores} threads, one field at a time; init excluded (FFTW absent)",
-            "gpu_vs_reference_max_abs_err_ire": float(max(errs)),
-            "across_fields": across, "cpu_model": cpu_model(), "nproc": os.cpu_count(),
-            "affinity_cores": host_cores()}
+    one_at_a_time = {"value": round(n_fields / dt, 3), "unit": "fields/s", "cores": cores, "kind": "reference",
+                     "sample": f"{n_fields} NTSC 910x263 fields: hvd-core BurstLockinPhase + VariationalRefine "
+                               f"(IRLS 4 x CG 25), OpenMP {cores} threads, one field at a time; "
+                               f"init excluded (FFTW absent)"}
+    # the baseline is the better of hvd-core's two CPU modes; both are kept
+    best = across if across and across["value"] > one_at_a_time["value"] else one_at_a_time
+    return dict(best, mode="across_fields" if best is across else "one_field_at_a_time",
+                gpu_vs_reference_max_abs_err_ire=float(max(errs)), one_field_at_a_time=one_at_a_time,
+                across_fields=across, cpu_model=cpu_model(), nproc=os.cpu_count(), affinity_cores=host_cores())
 
 
 def cpu_baseline_across_fields(ref, g, fields, ss, chi0s, cfg):
@@ -187,7 +190,9 @@ def cpu_baseline_across_fields(ref, g, fields, ss, chi0s, cfg):
         list(ex.map(one, jobs))
         dt = time.perf_counter() - t0
     return {"value": round(len(jobs) / dt, 3), "unit": "fields/s", "cores": workers, "kind": "reference",
-            "sample": f"{len(jobs)} NTSC fields, {workers} at once, each single-threaded (parallel_across_fields)"}
+            "sample": f"{len(jobs)} NTSC 910x263 fields: hvd-core BurstLockinPhase + VariationalRefine "
+                      f"(IRLS 4 x CG 25), {workers} at once, each on one thread (parallel_across_fields); "
+                      f"init excluded (FFTW absent)"}
 
 
 def training_bench(torch, gpu, synth, eng, steps=200, n_fields=16):
