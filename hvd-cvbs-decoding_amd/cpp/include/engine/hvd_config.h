@@ -19,7 +19,7 @@ struct HvdConfig {
   float charbonnier_eps = 0.5F;
   float chroma_eps = 1.0F;
   float structure_coupling = 0.25F;
-  float diag_prior = 0.0F;    // > 0 is not run by the GPU solve
+  float diag_prior = 0.0F;
   int cg_iterations = 2;
   int irls_outer = 4;
   float cg_tol = 0.0F;        // 0 => auto (0.02 slow / 0.10 fast)

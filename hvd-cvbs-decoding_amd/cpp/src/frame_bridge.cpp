@@ -279,8 +279,6 @@ std::vector<YcFrameS16> DecodeFrameSequenceWindow(const std::vector<const int16_
   if (nframes == 0 || core_begin < 0 || core_end > nframes || core_begin >= core_end) return {};
   try {
     CheckFrame(fp);
-    if (cfg.selective_3d && cfg.cg_iterations > 0)
-      throw gpu::Error("selective 3-D decoding is not run by this library");
     Session& ss = engine.session();
     const size_t nsrc = FrameSamples(fp), nflat = (size_t)fp.frame_width * fp.frame_height;
     const int ncore = core_end - core_begin;
@@ -316,7 +314,10 @@ std::vector<YcFrameS16> DecodeFrameSequenceWindow(const std::vector<const int16_
     const hvd_frame_params a = gpu::ToAbi(fp);
     const hvd_solver_params sp = gpu::ToAbi(cfg, fp.uses_vswitch());
     const hvd_bridge_params bp = gpu::BridgeOf(cfg);
-    const hvd_window_params wp = gpu::WindowOf(cfg);
+    // the selective window is this entry's alone (frame_bridge.cpp:800-805)
+    hvd_window_params wp = gpu::WindowOf(cfg);
+    wp.selective_3d = cfg.selective_3d ? 1 : 0;
+    wp.selective_max_area = cfg.selective_max_area;
     ss.check(hvd_gpu_decode_frame_window_ex_s16(ss.ctx(), dptr.data(), nframes, core_begin, core_end, &a, &sp, &bp,
                                                 &wp, outs.data(), nullptr, nullptr, nullptr),
              "hvd_gpu_decode_frame_window_ex_s16");

@@ -95,6 +95,8 @@ struct hvd_gpu_ctx {
   // driver's component calls for the same reason
   char* fwin = nullptr;
   size_t fwin_cap = 0;
+  char* sel = nullptr;  // the selective window's inits, score scratch and crops
+  size_t sel_cap = 0;
   Slot na_slot;  // descriptor region of a no-allocation call (no event)
   Slot slots[kSlots];
   int next_slot = 0;
@@ -1248,7 +1250,7 @@ void clear_profile(hvd_gpu_ctx* ctx) {
 // ============================================================================
 extern "C" {
 
-int hvd_gpu_abi_version(void) { return 102; }
+int hvd_gpu_abi_version(void) { return 103; }
 
 int hvd_gpu_create(int device, hvd_gpu_ctx** out) {
   if (!out) return HVD_GPU_ERR_INVALID;
@@ -1290,6 +1292,7 @@ void hvd_gpu_destroy(hvd_gpu_ctx* ctx) {
   if (ctx->ws) (void)hipFree(ctx->ws);
   if (ctx->win) (void)hipFree(ctx->win);
   if (ctx->fwin) (void)hipFree(ctx->fwin);
+  if (ctx->sel) (void)hipFree(ctx->sel);
   clear_profile(ctx);
   delete ctx;
 }
@@ -2458,6 +2461,8 @@ hvd_window_params default_window_params() {
   w.psi_init = 0;
   w.output_fidelity = 1;
   w.parallel_across_fields = 1;
+  w.selective_3d = 0;
+  w.selective_max_area = 0.45f;
   return w;
 }
 
@@ -2482,8 +2487,9 @@ char* ensure_fwin(hvd_gpu_ctx* ctx, size_t bytes, hipStream_t st) {
 // DecodeFrameSequenceWindow (frame_bridge.cpp:722-947) on the device.  The
 // 2-D window runs the field driver's batch; a window that may engage 3-D
 // (sequence.cpp:526-652: a positive strength, or the adaptive 0 with more
-// fields than the ambiguity stride) runs the window driver on the fields'
-// holographic inits, with the parities of the measured field order.
+// fields than the ambiguity stride) or asks for the selective window runs
+// the window driver on the fields' holographic inits, with the parities of
+// the measured field order.
 void frame_window(hvd_gpu_ctx* ctx, const int16_t* const* frames, int nframes, int core_begin, int core_end,
                   const hvd_frame_params* fp, const hvd_solver_params* p, const hvd_bridge_params* b,
                   const hvd_window_params* wp_in, const hvd_yc_frame* out, float* acc_gain, hvd_window_diag* diag,
@@ -2504,6 +2510,8 @@ void frame_window(hvd_gpu_ctx* ctx, const int16_t* const* frames, int nframes, i
                        (b->temporal_strength > 0.0f || (b->temporal_strength == 0.0f && nf > amb_stride));
   hvd_window_params wp = wp_in ? *wp_in : default_window_params();
   wp.temporal_strength = b->temporal_strength;
+  // DecodeWindowSelective (frame_bridge.cpp:800-805)
+  const bool selective = wp.selective_3d && p->cg_iterations > 0 && b->temporal_strength >= 0.0f;
   if (diag) *diag = hvd_window_diag{};
   for (int t = 0; t < nframes; ++t) REQUIRE(ctx, frames[t], "frame %d: null pointer", t);
   for (int k = 0; k < ncore; ++k)
@@ -2556,7 +2564,7 @@ void frame_window(hvd_gpu_ctx* ctx, const int16_t* const* frames, int nframes, i
   std::vector<BridgeField> dec;
   const int* w_f1_bottom = f1_bottom;
   const float* w_gain = gain;
-  if (!three_d) {
+  if (!three_d && !selective) {
     std::vector<float*> ys(nf), chis(nf);
     for (int j = 0; j < nf; ++j) {
       ys[j] = (float*)(ws + o_y) + (size_t)j * lines * width;
@@ -2613,8 +2621,10 @@ void frame_window(hvd_gpu_ctx* ctx, const int16_t* const* frames, int nframes, i
       q.chi = chis[j];
       q.y = ys[j];
     }
-    const int rc = hvd_window::drive(ctx, planes.data(), nf, par.data(), pal ? 1 : 0, y_given.data(), p, &wp, diag,
-                                     st);
+    const int rc = selective ? hvd_window::drive_selective(ctx, planes.data(), nf, par.data(), pal ? 1 : 0,
+                                                           y_given.data(), p, &wp, diag, st)
+                             : hvd_window::drive(ctx, planes.data(), nf, par.data(), pal ? 1 : 0, y_given.data(), p,
+                                                 &wp, diag, st);
     if (rc != HVD_GPU_OK) throw Fail{rc};
     dec.resize(nf);
     for (int j = 0; j < nf; ++j)
@@ -2941,8 +2951,38 @@ int hvd_gpu_decode_window(hvd_gpu_ctx* ctx, const hvd_gpu_plane* fields, int n, 
   return guarded(ctx, [&] {
     REQUIRE(ctx, !ctx->no_alloc, "internal: window inside a no-allocation call");
     if (p) check_params(ctx, p);
-    const int rc = hvd_window::drive(ctx, fields, n, parities, is_pal, y_given, p, wp, diag, S(stream));
+    if (diag) *diag = hvd_window_diag{};
+    const bool selective = p && wp && wp->selective_3d && p->cg_iterations > 0 && wp->temporal_strength >= 0.0;
+    const int rc = selective
+                       ? hvd_window::drive_selective(ctx, fields, n, parities, is_pal, y_given, p, wp, diag, S(stream))
+                       : hvd_window::drive(ctx, fields, n, parities, is_pal, y_given, p, wp, diag, S(stream));
     if (rc != HVD_GPU_OK) throw Fail{rc};
+  });
+}
+
+int hvd_gpu_ambiguous_boxes(hvd_gpu_ctx* ctx, const float* y, const float* chi, int lines, int width,
+                            float max_area, int* boxes, int cap, int* nboxes, float* scores, void* stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx, y && chi && nboxes && (cap == 0 || boxes), "null argument");
+    REQUIRE(ctx, lines >= 1 && width >= 1, "empty plane");
+    REQUIRE(ctx, !ctx->no_alloc, "internal: selective boxes inside a no-allocation call");
+    hipStream_t st = S(stream);
+    char* scr = hvd_window::sel_pool(ctx, hvd_window::sel_scratch_bytes(lines, width), st);
+    if (!scr) throw Fail{HVD_GPU_ERR_NOMEM};
+    std::vector<hvd_window::SelBox> bx;
+    bool everywhere = false;
+    std::vector<float> sc;
+    const int rc = hvd_window::ambiguous_boxes(ctx, y, chi, lines, width, max_area, scr, st, &bx, &everywhere,
+                                               scores ? &sc : nullptr);
+    if (rc != HVD_GPU_OK) throw Fail{rc};
+    *nboxes = everywhere ? -1 : (int)bx.size();
+    for (int k = 0; k < (int)bx.size() && k < cap; ++k) {
+      boxes[4 * k] = bx[k].fy0;
+      boxes[4 * k + 1] = bx[k].fy1;
+      boxes[4 * k + 2] = bx[k].x0;
+      boxes[4 * k + 3] = bx[k].x1;
+    }
+    if (scores) std::copy(sc.begin(), sc.end(), scores);
   });
 }
 
@@ -3516,6 +3556,25 @@ char* pool(hvd_gpu_ctx* ctx, size_t bytes, hipStream_t st) {
   ctx->win = static_cast<char*>(q);
   ctx->win_cap = want;
   return ctx->win;
+}
+
+char* sel_pool(hvd_gpu_ctx* ctx, size_t bytes, hipStream_t st) {
+  if (bytes <= ctx->sel_cap) return ctx->sel;
+  if (hipStreamSynchronize(st) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+    set_err(ctx, "synchronisation before the selective window's allocation failed");
+    return nullptr;
+  }
+  if (ctx->sel) (void)hipFree(ctx->sel);
+  ctx->sel = nullptr;
+  ctx->sel_cap = 0;
+  void* q = nullptr;
+  if (hipMalloc(&q, bytes) != hipSuccess) {
+    set_err(ctx, "hipMalloc(%zu) for the selective window failed", bytes);
+    return nullptr;
+  }
+  ctx->sel = static_cast<char*>(q);
+  ctx->sel_cap = bytes;
+  return ctx->sel;
 }
 
 int fail(hvd_gpu_ctx* ctx, int code, const char* msg) {

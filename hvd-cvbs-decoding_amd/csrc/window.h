@@ -1,9 +1,12 @@
-// window.h -- the 3-D window driver's kernels (window.hip) and its host entry
-// point, called by hvd_gpu_decode_window in abi.hip.
+// window.h -- the 3-D window driver's kernels (window.hip), its host entry
+// point and the selective window (selective.hip), called by
+// hvd_gpu_decode_window and the frame window in abi.hip.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <vector>
 
 #include "hvd_gpu.h"
 
@@ -88,4 +91,21 @@ int drizzle(hvd_gpu_ctx* ctx, int j0, int n, const float* const* Ys, const float
 // The window driver (window.hip): 0 or an HVD_GPU_ERR_* code.
 int drive(hvd_gpu_ctx* ctx, const hvd_gpu_plane* fields, int n, const int* parities, int is_pal, const int* y_given,
           const hvd_solver_params* p, const hvd_window_params* wp, hvd_window_diag* diag, hipStream_t st);
+// Grow-only device memory of the selective window, apart from the driver's
+// pool and the workspace (both of which the crops' decodes use).
+char* sel_pool(hvd_gpu_ctx* ctx, size_t bytes, hipStream_t st);
+// One box of the selective window, field coordinates [fy0, fy1) x [x0, x1).
+struct SelBox {
+  int fy0, fy1, x0, x1;
+};
+// Scratch bytes of ambiguous_boxes for a lines x width field.
+size_t sel_scratch_bytes(int h, int w);
+// AmbiguousBoxes (selective.hip) of a decoded field: the boxes, or
+// *everywhere (the whole window in 3-D); scores_out optional.  Synchronous.
+int ambiguous_boxes(hvd_gpu_ctx* ctx, const float* y, const float* chi, int h, int w, float max_area, char* scratch,
+                    hipStream_t st, std::vector<SelBox>* out, bool* everywhere, std::vector<float>* scores_out);
+// DecodeWindowSelective (selective.hip) over the driver's fields.
+int drive_selective(hvd_gpu_ctx* ctx, const hvd_gpu_plane* fields, int n, const int* parities, int is_pal,
+                    const int* y_given, const hvd_solver_params* p, const hvd_window_params* wp,
+                    hvd_window_diag* diag, hipStream_t st);
 }  // namespace hvd_window

@@ -32,7 +32,7 @@ ABI_SYMBOLS = (
     "hvd_gpu_woven_picture", "hvd_gpu_variational_refine_temporal_batch", "hvd_gpu_variational_refine_joint",
     "hvd_gpu_verify_motion", "hvd_gpu_decode_window", "hvd_gpu_decode_frame_window_ex_s16",
     "hvd_gpu_difference", "hvd_gpu_estimate_noise_ire", "hvd_gpu_fft2d", "hvd_gpu_trajectory_fit",
-    "hvd_gpu_trajectory_snap", "hvd_gpu_drizzle_frame",
+    "hvd_gpu_trajectory_snap", "hvd_gpu_drizzle_frame", "hvd_gpu_ambiguous_boxes",
 )
 
 
@@ -96,19 +96,22 @@ class WindowParams(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("temporal_strength", "temporal_eps", "nr_anchor", "nr_eps",
                                           "coherence_gate", "odd_gate_floor")] + \
         [(n, C.c_int) for n in ("passes", "nr_radius", "mc_tile", "mc_search", "bidirectional", "extended_temporal",
-                                "trajectory_fit", "psi_init", "output_fidelity", "parallel_across_fields")]
+                                "trajectory_fit", "psi_init", "output_fidelity", "parallel_across_fields",
+                                "selective_3d")] + [("selective_max_area", C.c_float)]
 
     @classmethod
     def make(cls, **kw) -> "WindowParams":
         d = dict(temporal_strength=0.0, temporal_eps=0.0, nr_anchor=1.0, nr_eps=0.0, coherence_gate=0.6,
                  odd_gate_floor=0.35, passes=2, nr_radius=2, mc_tile=32, mc_search=16, bidirectional=1,
-                 extended_temporal=1, trajectory_fit=1, psi_init=0, output_fidelity=1, parallel_across_fields=1)
+                 extended_temporal=1, trajectory_fit=1, psi_init=0, output_fidelity=1, parallel_across_fields=1,
+                 selective_3d=0, selective_max_area=0.45)
         d.update(kw)
         return cls(**d)
 
 
 class WindowDiag(C.Structure):
-    _fields_ = [(n, C.c_float) for n in ("resolved_strength", "ambiguity_ire", "sigma_ire", "temporal_eps", "nr_eps")]
+    _fields_ = [(n, C.c_float) for n in ("resolved_strength", "ambiguity_ire", "sigma_ire", "temporal_eps",
+                                         "nr_eps")] + [("selective_boxes", C.c_int)]
 
 
 class Anchor(C.Structure):
@@ -183,6 +186,7 @@ def load() -> C.CDLL:
         lib.hvd_gpu_difference.argtypes = [vp, i, i, vp, vp, i, i, vp, vp]
         lib.hvd_gpu_estimate_noise_ire.argtypes = [vp, vp, i, i, C.POINTER(C.c_float), vp]
         lib.hvd_gpu_fft2d.argtypes = [vp, vp, i, i, i, vp, vp]
+        lib.hvd_gpu_ambiguous_boxes.argtypes = [vp, vp, vp, i, i, C.c_float, C.POINTER(i), i, C.POINTER(i), vp, vp]
         lib.hvd_gpu_trajectory_fit.argtypes = [vp, i, C.POINTER(C.c_int), C.POINTER(C.c_float), pp, pp, pp, i, i,
                                                vp, vp, vp]
         lib.hvd_gpu_trajectory_snap.argtypes = [vp, i, C.POINTER(C.c_int), C.POINTER(C.c_float), pp, pp, pp, i, i, vp]
@@ -513,6 +517,24 @@ class Engine:
         self._check(self.lib.hvd_gpu_decode_window(self.ctx, P, n, par, int(is_pal), yg, C.byref(params),
                                                    C.byref(wparams), C.byref(d), _stream()), "decode_window")
         return {k: float(getattr(d, k)) for k, _ in WindowDiag._fields_}
+
+    def ambiguous_boxes(self, y, chi, max_area=0.45, with_scores=False):
+        """AmbiguousBoxes of one decoded field (y: (h, w), chi: (h, w, 2)
+        float32 on the device): None for ambiguity everywhere (the whole
+        window in 3-D), else the list of (fy0, fy1, x0, x1) boxes (empty: the
+        window stays 2-D); with_scores also returns the tile scores."""
+        h, w = y.shape
+        cap = 256
+        bx = (C.c_int * (4 * cap))()
+        nb = C.c_int()
+        th, tw = (h + 15) // 16, (w + 15) // 16
+        sc = np.zeros((th, tw), np.float32) if with_scores else None
+        self._check(self.lib.hvd_gpu_ambiguous_boxes(self.ctx, y.data_ptr(), chi.data_ptr(), h, w,
+                                                     C.c_float(max_area), bx, cap, C.byref(nb),
+                                                     sc.ctypes.data if with_scores else None, _stream()),
+                    "ambiguous_boxes")
+        boxes = None if nb.value < 0 else [tuple(bx[4 * k:4 * k + 4]) for k in range(min(nb.value, cap))]
+        return (boxes, sc) if with_scores else boxes
 
     # ---- temporal neighbours' warps (temporal.cpp) ----------------------------
     def vectors_per_pixel(self, mdy, mdx, tile, h, w):

@@ -110,6 +110,13 @@ typedef struct {
   int psi_init;
   int output_fidelity;
   int parallel_across_fields;
+  /* DecodeWindowSelective (frame_bridge.cpp:638-720), when non-zero with
+   * cg_iterations > 0 and temporal_strength >= 0: the window in 2-D, then the
+   * boxes of the first field's ambiguous tiles decoded again in 3-D and
+   * feather-blended in.  hvd-core runs it from DecodeFrameSequenceWindow
+   * only (HvdConfig::selective_3d, off by default). */
+  int selective_3d;
+  float selective_max_area; /* boxes above this fraction of the field: full 3-D */
 } hvd_window_params;
 
 /* hvd::SequenceDiagnostics (sequence.h:73-79): what the driver decided. */
@@ -119,6 +126,7 @@ typedef struct {
   float sigma_ire;
   float temporal_eps;
   float nr_eps;
+  int selective_boxes; /* selective 3-D: boxes decoded in 3-D; 0 none, -1 the whole window */
 } hvd_window_diag;
 
 /* hvd::FrameParams (frame_bridge.h:40-101): a field-sequential frame's
@@ -322,6 +330,22 @@ int hvd_gpu_drizzle_frame(hvd_gpu_ctx* ctx, int j0, int n, const float* const* y
 int hvd_gpu_decode_window(hvd_gpu_ctx* ctx, const hvd_gpu_plane* fields, int n, const int* parities, int is_pal,
                           const int* y_given, const hvd_solver_params* p, const hvd_window_params* wp,
                           hvd_window_diag* diag, void* stream);
+
+/* AmbiguousBoxes (frame_bridge.cpp:399-621) of one decoded field (y, chi:
+ * DEVICE, lines x width): per 16 x 16 tile the score sqrt(mean (Y -
+ * box(Y))^2 x mean (|chi| - box|chi|)^2) with count-normalised 8-sample box
+ * means, tiles at or above 4 x the median with >= 2 such neighbours (the
+ * outer ring excluded), bands of adjacent tile rows with >= 2 of them (at
+ * most 4, merged across the smallest gaps), one box per run of columns
+ * (gaps <= 2 tiles, single tiles dropped) with 16-row and 48-column
+ * margins.  *nboxes (HOST): 0 no box (the window stays 2-D); -1 ambiguity
+ * everywhere, or boxes above max_area of the field in total (the whole
+ * window in 3-D); else the count, the first `cap` of them in boxes (HOST,
+ * 4 ints each: first row, end row, first column, end column).  scores:
+ * optional HOST output, ceil(lines / 16) x ceil(width / 16) floats.
+ * Synchronous. */
+int hvd_gpu_ambiguous_boxes(hvd_gpu_ctx* ctx, const float* y, const float* chi, int lines, int width,
+                            float max_area, int* boxes, int cap, int* nboxes, float* scores, void* stream);
 
 /* Device scratch of hvd_gpu_variational_refine_noalloc: 64 MiB per GPU, held
  * from the library's first context on that GPU until the process ends. */

@@ -177,12 +177,16 @@ int main(int argc, char** argv) {
     for (const hvd::YcFrameS16& y : out) PutFrame(o, y);
     return 0;
   }
-  if (mode == "frames3d" && argc == 9) {
+  if (mode == "frames3d" && (argc == 9 || argc == 10)) {
     hvd::HvdConfig cfg;
     cfg.cg_iterations = 20;
     cfg.irls_outer = 4;
     cfg.fast = true;
     cfg.temporal_strength = (float)std::atof(argv[6]);
+    if (argc == 10) {  // the selective window, with this area cap
+      cfg.selective_3d = true;
+      cfg.selective_max_area = (float)std::atof(argv[9]);
+    }
     const int nframes = std::atoi(argv[3]);
     const std::vector<int16_t> in = ReadAll<int16_t>(argv[7], frame_n * nframes);
     std::vector<const int16_t*> frames;

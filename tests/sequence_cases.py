@@ -11,14 +11,21 @@ import numpy as np
 F = np.float32
 
 
-def window(lines=48, width=128, nf=8, noise=0.8, seed=7, ambiguous=True, motion=(0, 0), pal=False):
+def window(lines=48, width=128, nf=8, noise=0.8, seed=7, ambiguous=True, motion=(0, 0), pal=False,
+           amb_region=None):
+    """amb_region: (y0, y1, x0, x1) of the frame grid that alone carries the
+    ambiguous luma texture (default: all of it)."""
     rng = np.random.default_rng(seed)
     fl = 2 * lines
     pad = 24
     y, x = np.mgrid[0:fl + 2 * pad, 0:width + 2 * pad].astype(F)
     yf = 30 + 22 * np.sin(F(0.06) * (y + 2 * x))
     if ambiguous:
-        yf = yf + 14 * np.cos(F(math.pi / 2) * x + F(0.4) * y) * np.sin(F(0.15) * y)
+        tex = 14 * np.cos(F(math.pi / 2) * x + F(0.4) * y) * np.sin(F(0.15) * y)
+        if amb_region is not None:
+            y0, y1, x0, x1 = amb_region
+            tex = tex * ((y >= y0 + pad) & (y < y1 + pad) & (x >= x0 + pad) & (x < x1 + pad))
+        yf = yf + tex
     inside = (x > 40 + pad) & (x < 90 + pad) & (y > 24 + pad) & (y < 72 + pad)
     cf = np.where(inside, F(12) - 8j, F(3) + 2j).astype(np.complex64)
     fields, parities, inits, truth = [], [], [], []

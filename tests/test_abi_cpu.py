@@ -27,7 +27,7 @@ def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(str(gpu.LIB_PATH))
     for name in declared_symbols():
         assert hasattr(lib, name), name
-    assert lib.hvd_gpu_abi_version() == 102
+    assert lib.hvd_gpu_abi_version() == 103
 
 
 def test_create_without_device_fails_cleanly():

@@ -354,3 +354,37 @@ def test_decode_frame_sequence_window_3d(tmp_path, orc64, orc32, std):
         for a, b in ((gu, wu), (gv, wv)):
             e = np.abs(a - b) / (scale * max(gain, 1.0))
             assert (e > TOL_IRE).mean() <= 2e-4 and e.max() <= 0.15, ((e > TOL_IRE).mean(), e.max())
+
+
+@pytest.mark.parametrize("max_area", [0.45, 0.05])
+def test_decode_frame_sequence_window_selective(tmp_path, max_area):
+    """DecodeFrameSequenceWindow with HvdConfig::selective_3d through the C++
+    API (hvd-core refuses nothing here; the library used to return empty
+    frames) against the C ABI's selective frame window with the same knobs;
+    the numerics of the selective window itself are held to the oracle in
+    test_gpu_selective."""
+    from hvd_amd import gpu
+    import torch
+    from test_gpu_parity import _fp_gpu
+    fpd, _, _ = frame_geom("ntsc")
+    frames = synth.frames_s16(fpd, 3, seed=97)
+    np.concatenate(frames).astype(np.int16).tofile(tmp_path / "in.s16")
+    run(["frames3d", "ntsc", 3, 1, 3, 1.0, tmp_path / "in.s16", tmp_path / "out.bin", max_area])
+    got = _read_frames(tmp_path / "out.bin", fpd, 2)
+    eng = gpu.Engine(0)
+    d = {}
+    outs, _ = eng.decode_frame_window_s16(
+        [torch.from_numpy(f).to("cuda:0") for f in frames], 1, 3, _fp_gpu(fpd),
+        gpu.SolverParams.make(cg_iterations=20, irls_outer=4, fast=1), gpu.BridgeParams.make(temporal_strength=1.0),
+        wparams=gpu.WindowParams.make(selective_3d=1, selective_max_area=max_area), diag=d)
+    torch.cuda.synchronize()
+    print("selective frame window: boxes", d["selective_boxes"], flush=True)
+    scale = (fpd["white_level"] - fpd["blanking_level"]) / 100.0
+    for (gl, gc, gu, gv), o in zip(got, outs):
+        wl, wc, wu, wv = (t.cpu().numpy() for t in o)
+        _codes_close(gl, wl)
+        _codes_close(gc, wc)
+        for a, b in ((gu, wu), (gv, wv)):
+            e = np.abs(a - b) / scale
+            assert (e > TOL_IRE).mean() <= 2e-4 and e.max() <= 0.15, ((e > TOL_IRE).mean(), e.max())
+
