@@ -2880,18 +2880,20 @@ namespace {
 
 // The device tables of the trajectory entry points: m pointers each of dy,
 // dx, conf, then m shifts and m offsets.
-char* trajectory_tables(hvd_gpu_ctx* ctx, int m, const int* offsets, const float* shifts, const void* const* dy,
-                        const void* const* dx, const float* const* conf, hipStream_t st, Slot** sl, size_t* o) {
+hvdk::TrajTab trajectory_table(hvd_gpu_ctx* ctx, int m, const int* offsets, const float* shifts,
+                               float* const* dy, float* const* dx, const float* const* conf) {
   REQUIRE(ctx, m >= 0 && m <= kMaxNeighbors, "%d measurements (at most %d)", m, kMaxNeighbors);
   REQUIRE(ctx, m == 0 || (offsets && shifts && dy && dx && conf), "null argument");
-  for (int j = 0; j < m; ++j) REQUIRE(ctx, dy[j] && dx[j] && conf[j], "measurement %d: null plane", j);
-  Stage stg;
-  o[0] = stg.put(dy, m);
-  o[1] = stg.put(dx, m);
-  o[2] = stg.put(conf, m);
-  o[3] = stg.put(shifts, m);
-  o[4] = stg.put(offsets, m);
-  return upload(ctx, stg, st, sl);
+  hvdk::TrajTab T{};
+  for (int j = 0; j < m; ++j) {
+    REQUIRE(ctx, dy[j] && dx[j] && conf[j], "measurement %d: null plane", j);
+    T.dy[j] = dy[j];
+    T.dx[j] = dx[j];
+    T.conf[j] = conf[j];
+    T.ho[j] = shifts[j];
+    T.off[j] = offsets[j];
+  }
+  return T;
 }
 
 }  // namespace
@@ -2903,15 +2905,10 @@ int hvd_gpu_trajectory_fit(hvd_gpu_ctx* ctx, int m, const int* offsets, const fl
     REQUIRE(ctx, vy && vx && tiles_h >= 1 && tiles_w >= 1, "bad output grid");
     hipStream_t st = S(stream);
     const int nt = tiles_h * tiles_w;
-    Slot* sl;
-    size_t o[5];
-    char* base = trajectory_tables(ctx, m, offsets, parity_shifts, (const void* const*)dy, (const void* const*)dx,
-                                   conf, st, &sl, o);
-    hipLaunchKernelGGL(k_traj_fit, dim3((nt + 63) / 64), dim3(64), 0, st, m, nt, (const float* const*)(base + o[0]),
-                       (const float* const*)(base + o[1]), (const float* const*)(base + o[2]),
-                       (const float*)(base + o[3]), (const int*)(base + o[4]), vy, vx);
+    const hvdk::TrajTab T =
+        trajectory_table(ctx, m, offsets, parity_shifts, (float* const*)dy, (float* const*)dx, conf);
+    hipLaunchKernelGGL(k_traj_fit, dim3((nt + 63) / 64), dim3(64), 0, st, m, nt, T, vy, vx);
     LAUNCH_CHECK(ctx);
-    arm(ctx, sl, st);
   });
 }
 
@@ -2922,15 +2919,9 @@ int hvd_gpu_trajectory_snap(hvd_gpu_ctx* ctx, int m, const int* offsets, const f
     if (m < 2) return;
     hipStream_t st = S(stream);
     const int nt = tiles_h * tiles_w;
-    Slot* sl;
-    size_t o[5];
-    char* base = trajectory_tables(ctx, m, offsets, parity_shifts, (const void* const*)dy, (const void* const*)dx,
-                                   conf, st, &sl, o);
-    hipLaunchKernelGGL(k_traj_snap, dim3((nt + 63) / 64), dim3(64), 0, st, m, nt, (float* const*)(base + o[0]),
-                       (float* const*)(base + o[1]), (const float* const*)(base + o[2]), (const float*)(base + o[3]),
-                       (const int*)(base + o[4]));
+    const hvdk::TrajTab T = trajectory_table(ctx, m, offsets, parity_shifts, dy, dx, conf);
+    hipLaunchKernelGGL(k_traj_snap, dim3((nt + 63) / 64), dim3(64), 0, st, m, nt, T);
     LAUNCH_CHECK(ctx);
-    arm(ctx, sl, st);
   });
 }
 

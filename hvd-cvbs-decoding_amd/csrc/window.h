@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "hvd_gpu.h"
+#include "hvd_kernels.h"
 
 namespace hvdk {
 
@@ -37,22 +38,48 @@ struct NbJob {
   const float* lapsm;
 };
 
+// One field's pairwise motion measurements (at most kMaxNeighbors), passed
+// to the trajectory kernels by value: no table upload, nothing to wait for.
+struct TrajTab {
+  float* dy[kMaxNeighbors];
+  float* dx[kMaxNeighbors];
+  const float* conf[kMaxNeighbors];
+  float ho[kMaxNeighbors];  // parity shift of each measurement
+  int off[kMaxNeighbors];   // its field offset
+};
+// The envelope planes of one colour (at most kEnvMax fields), by value.
+constexpr int kEnvMax = 64;
+struct EnvRows {
+  const float* src[2 * kEnvMax];
+  float* out[2 * kEnvMax];
+};
+struct EnvFinish {
+  const float* vg_in[kEnvMax];
+  float* vgrad[kEnvMax];
+  float* lap[kEnvMax];
+  float* sample[kEnvMax];
+};
+// Tile-map copies (dy and dx, 2 nt floats each) of one colour, by value.
+constexpr int kCopyMax = 128;
+struct CopyTab {
+  const float* src[kCopyMax];
+  float* dst[kCopyMax];
+};
+
 __global__ void k_verify_pick(int n, const float* pdy, const float* pdx, const float* bp, const float* bz, float* dy,
                               float* dx, float* best);
 __global__ void k_noise_sigma(const float* const* s, int h, int w, float* d_all, float* sigma);
 __global__ void k_amb_bands(const float* const* s, const float* const* car, int stride, int h, int w, float nr_eps,
                             int nbands, double* acc, int* cnt);
 __global__ void k_fidelity(size_t n, const float* s, const float2* chi, const float2* car, float* y);
-__global__ void k_traj_predict(int m, int nt, const float* const* dy, const float* const* dx, const float* ho,
-                               const int* off, float* vy, float* vx);
+__global__ void k_traj_predict(int m, int nt, TrajTab T, float* vy, float* vx);
 __global__ void k_traj_extrapolate(int nt, const float* vy, const float* vx, float o, float ho, float* pdy,
                                    float* pdx);
-__global__ void k_traj_snap(int m, int nt, float* const* dy, float* const* dx, const float* const* conf,
-                            const float* ho, const int* off);
+__global__ void k_traj_snap(int m, int nt, TrajTab T);
 __global__ void k_env_terms(int h, int w, const float* Y, const float2* chi, float* gmag, float* lapmag);
-__global__ void k_smooth_rows(int h, int w, const float* const* src, float* const* out);
-__global__ void k_env_finish(int n, const float* const* vg_in, float* const* vgrad, float* const* lap_io,
-                             float* const* sample);
+__global__ void k_smooth_rows(int h, int w, EnvRows P);
+__global__ void k_env_finish(int n, EnvFinish P);
+__global__ void k_copy_tiles(int n, CopyTab P);
 __global__ void k_split_cplx(size_t n, const float2* z, float* re, float* im);
 __global__ void k_join_cplx(size_t n, const float* re, const float* im, float2* z);
 __global__ void k_conf_gates(size_t n, float* conf, const float* gamma, float a, const float* vgrad,
@@ -68,8 +95,7 @@ __global__ void k_synth_weight(size_t n, const float* r, float eps_b, float* wei
 __global__ void k_psi(size_t n, int nn, const float* s, const float2* car, const float* const* sw,
                       const float2* const* cw, const float* const* conf, float2* chi);
 
-__global__ void k_traj_fit(int m, int nt, const float* const* dy, const float* const* dx, const float* const* conf,
-                           const float* ho, const int* off, float* vy, float* vx);
+__global__ void k_traj_fit(int m, int nt, TrajTab T, float* vy, float* vx);
 __global__ void k_nb_square(int nt, const NbJob* J);
 __global__ void k_nb_snap(int th, int tw, int fast, const NbJob* J);
 __global__ void k_nb_warp(int h, int w, int th, int tw, int tile, int fast, int gate, const NbJob* J);

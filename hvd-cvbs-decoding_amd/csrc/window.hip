@@ -225,16 +225,15 @@ __global__ void k_fidelity(size_t n, const float* s, const float2* chi, const fl
 // Field j's per-tile trajectory prediction (sequence.cpp:702-724): the numpy
 // median over the measured offsets of (dy - h_o) / o and dx / o.  dy, dx:
 // m planes of nt tiles; ho, off: m entries.
-__global__ void k_traj_predict(int m, int nt, const float* const* dy, const float* const* dx, const float* ho,
-                               const int* off, float* vy, float* vx) {
+__global__ void k_traj_predict(int m, int nt, TrajTab T, float* vy, float* vx) {
 #pragma clang fp contract(off)
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nt) return;
   float sy[kMaxNeighbors], sx[kMaxNeighbors];
   for (int j = 0; j < m; ++j) {
-    const float o = (float)off[j];
-    sy[j] = (dy[j][t] - ho[j]) / o;
-    sx[j] = dx[j][t] / o;
+    const float o = (float)T.off[j];
+    sy[j] = (T.dy[j][t] - T.ho[j]) / o;
+    sx[j] = T.dx[j][t] / o;
   }
   // insertion sorts (m <= 16)
   for (int a = 1; a < m; ++a)
@@ -273,16 +272,15 @@ namespace {
 // FitTrajectory (motion.cpp:532-562) at tile t: the numpy median of the
 // confident (conf > 0.15, offset != 0) (dy - h) / o and dx / o; 0 where no
 // measurement qualifies.
-__device__ inline void traj_fit(int m, int t, const float* const* dy, const float* const* dx,
-                                const float* const* conf, const float* ho, const int* off, float& vy, float& vx) {
+__device__ inline void traj_fit(int m, int t, const TrajTab& T, float& vy, float& vx) {
 #pragma clang fp contract(off)
   float sy[kMaxNeighbors], sx[kMaxNeighbors];
   int k = 0;
   for (int j = 0; j < m; ++j) {
-    if (off[j] == 0 || conf[j][t] <= 0.15f) continue;
-    const float o = (float)off[j];
-    sy[k] = (dy[j][t] - ho[j]) / o;
-    sx[k] = dx[j][t] / o;
+    if (T.off[j] == 0 || T.conf[j][t] <= 0.15f) continue;
+    const float o = (float)T.off[j];
+    sy[k] = (T.dy[j][t] - T.ho[j]) / o;
+    sx[k] = T.dx[j][t] / o;
     ++k;
   }
   vy = 0.f;
@@ -311,12 +309,11 @@ __device__ inline void traj_fit(int m, int t, const float* const* dy, const floa
 }  // namespace
 
 // FitTrajectory over m measurements of nt tiles into vy, vx.
-__global__ void k_traj_fit(int m, int nt, const float* const* dy, const float* const* dx, const float* const* conf,
-                           const float* ho, const int* off, float* vy, float* vx) {
+__global__ void k_traj_fit(int m, int nt, TrajTab T, float* vy, float* vx) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nt) return;
   float a = 0.f, b = 0.f;
-  if (m > 0) traj_fit(m, t, dy, dx, conf, ho, off, a, b);
+  if (m > 0) traj_fit(m, t, T, a, b);
   vy[t] = a;
   vx[t] = b;
 }
@@ -324,30 +321,35 @@ __global__ void k_traj_fit(int m, int nt, const float* const* dy, const float* c
 // TrajectorySnap (motion.cpp:564-601) over one field's m measurements: the
 // fit, the consensus count of those within 1.5 px, and the snap where >= 3
 // agree.  In place on dy, dx.
-__global__ void k_traj_snap(int m, int nt, float* const* dy, float* const* dx, const float* const* conf,
-                            const float* ho, const int* off) {
+__global__ void k_traj_snap(int m, int nt, TrajTab T) {
 #pragma clang fp contract(off)
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nt || m < 2) return;
   float vy, vx;
-  traj_fit(m, t, (const float* const*)dy, (const float* const*)dx, conf, ho, off, vy, vx);
+  traj_fit(m, t, T, vy, vx);
   float agree = 0.f;
   for (int j = 0; j < m; ++j) {
-    const float o = (float)off[j];
-    const float py = o * vy + ho[j], px = o * vx;
-    const float dev = fabsf(dy[j][t] - py) + fabsf(dx[j][t] - px);
+    const float o = (float)T.off[j];
+    const float py = o * vy + T.ho[j], px = o * vx;
+    const float dev = fabsf(T.dy[j][t] - py) + fabsf(T.dx[j][t] - px);
     if (dev <= 1.5f) agree += 1.f;
   }
   if (agree < 3.f) return;
   for (int j = 0; j < m; ++j) {
-    const float o = (float)off[j];
-    const float py = o * vy + ho[j], px = o * vx;
-    const float dev = fabsf(dy[j][t] - py) + fabsf(dx[j][t] - px);
+    const float o = (float)T.off[j];
+    const float py = o * vy + T.ho[j], px = o * vx;
+    const float dev = fabsf(T.dy[j][t] - py) + fabsf(T.dx[j][t] - px);
     if (dev <= 1.5f) {
-      dy[j][t] = py;
-      dx[j][t] = px;
+      T.dy[j][t] = py;
+      T.dx[j][t] = px;
     }
   }
+}
+
+// Copies of n floats from P.src[b] to P.dst[b], block row b.
+__global__ void __launch_bounds__(256) k_copy_tiles(int n, CopyTab P) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) P.dst[blockIdx.y][i] = P.src[blockIdx.y][i];
 }
 
 // The envelope's per-pixel terms (sequence.cpp:817-852): the one-sided max
@@ -385,14 +387,14 @@ __global__ void k_env_terms(int h, int w, const float* Y, const float2* chi, flo
 // formed and stored column by column (coalesced).  Grid: (ceil(h / kSmR),
 // planes).
 constexpr int kSmR = 16;
-__global__ void __launch_bounds__(64) k_smooth_rows(int h, int w, const float* const* src, float* const* out) {
+__global__ void __launch_bounds__(64) k_smooth_rows(int h, int w, EnvRows P) {
 #pragma clang fp contract(off)
   constexpr int C = 64, L = 3, SW = C + 5;
   __shared__ float sm[kSmR][SW + 2];
   __shared__ float sa[kSmR][C + 1];
   const int t = threadIdx.x;
-  const float* in = src[blockIdx.y];
-  float* o = out[blockIdx.y];
+  const float* in = P.src[blockIdx.y];
+  float* o = P.out[blockIdx.y];
   const int row0 = blockIdx.x * kSmR, y = row0 + t;
   float acc = 0.f;
   // row j's segment of the chunk at x0: thread t holds columns t and t + 64
@@ -449,20 +451,19 @@ __global__ void __launch_bounds__(64) k_smooth_rows(int h, int w, const float* c
 // block per field: the median of every seventh vgrad sample, then per pixel
 // vgrad <- sqrt(max(0, vgrad^2 - (1.5 median)^2)) and lapsm = 1 - t^2 with
 // t = clamp(smoothed second difference / vgrad, 0, 1).
-__global__ void __launch_bounds__(kSelBlock) k_env_finish(int n, const float* const* vg_in, float* const* vgrad,
-                                                          float* const* lap_io, float* const* sample) {
+__global__ void __launch_bounds__(kSelBlock) k_env_finish(int n, EnvFinish P) {
 #pragma clang fp contract(off)
   __shared__ unsigned hist[16 * 256];
   const int fi = blockIdx.x;
-  const float* v = vg_in[fi];
-  float* smp = sample[fi];
+  const float* v = P.vg_in[fi];
+  float* smp = P.sample[fi];
   const int ns = (n + 6) / 7;
   for (int i = threadIdx.x; i < ns; i += kSelBlock) smp[i] = v[(size_t)i * 7];
   __syncthreads();
   const float med = median_np(smp, ns, hist);
   const float fl2 = 2.25f * med * med;
-  float* vg = vgrad[fi];
-  float* lp = lap_io[fi];
+  float* vg = P.vgrad[fi];
+  float* lp = P.lap[fi];
   for (int i = threadIdx.x; i < n; i += kSelBlock) {
     const float g0 = v[i];
     const float g = sqrtf(fmaxf(0.0f, g0 * g0 - fl2));
@@ -801,7 +802,6 @@ int drive(hvd_gpu_ctx* ctx, const hvd_gpu_plane* F, int nf, const int* par, int 
   const size_t o_jmd = cv.take(sizeof(MotionDesc) * (3 * kJobChunk + 2));
   const size_t o_jscr = cv.take(sizeof(float) * job_floats * kJobChunk);
   const size_t o_ptr = cv.take(sizeof(void*) * 64 * (size_t)(nf + 4));       // pointer tables
-  const size_t o_small = cv.take(sizeof(float) * 64 * (size_t)(nf + 4));
   char* base = pool(ctx, cv.off, st);
   if (!base) return HVD_GPU_ERR_NOMEM;
   float* sig = (float*)(base + o_sig);
@@ -827,7 +827,6 @@ int drive(hvd_gpu_ctx* ctx, const hvd_gpu_plane* F, int nf, const int* par, int 
   auto T = [&](int k) { return tmp + (size_t)k * np; };
   float* smf = (float*)(base + o_smf);
   void** ptrs = (void**)(base + o_ptr);
-  float* small = (float*)(base + o_small);
   auto cfl = [](const float* q) { return (const float2*)q; };
 
   // ---- noise self-calibration (sequence.cpp:439-449) ------------------------
@@ -990,34 +989,20 @@ int drive(hvd_gpu_ctx* ctx, const hvd_gpu_plane* F, int nf, const int* par, int 
             if (present[gi].empty()) continue;
             std::sort(present[gi].begin(), present[gi].end(), [&](int a, int b) { return offs[a] < offs[b]; });
             const int m = (int)present[gi].size();
-            std::vector<const float*> pdy(m), pdx(m);
-            std::vector<float> ho(m);
-            std::vector<int> off(m);
+            if (m > kMaxNeighbors) return fail(ctx, HVD_GPU_ERR_INVALID, "too many measured offsets");
+            TrajTab tt{};
             for (int q = 0; q < m; ++q) {
               const int oi = present[gi][q];
               const MF mf = mcache(j, oi);
-              pdy[q] = mf.dy;
-              pdx[q] = mf.dx;
-              ho[q] = ((float)par[j + offs[oi]] - (float)par[j]) / 2.0F;
-              off[q] = offs[oi];
+              tt.dy[q] = mf.dy;
+              tt.dx[q] = mf.dx;
+              tt.ho[q] = ((float)par[j + offs[oi]] - (float)par[j]) / 2.0F;
+              tt.off[q] = offs[oi];
             }
             float* vy = pred + (size_t)j * 4 * nt;
             float* vx = vy + nt;
-            // pointer tables and scalars for this field
-            void** pt = ptrs + (size_t)(j + 2) * 64;
-            float* sm = small + (size_t)(j + 2) * 64;
-            if (hipMemcpyAsync(pt, pdy.data(), sizeof(void*) * m, hipMemcpyHostToDevice, st) != hipSuccess ||
-                hipMemcpyAsync(pt + 16, pdx.data(), sizeof(void*) * m, hipMemcpyHostToDevice, st) != hipSuccess ||
-                hipMemcpyAsync(sm, ho.data(), sizeof(float) * m, hipMemcpyHostToDevice, st) != hipSuccess ||
-                hipMemcpyAsync(sm + 16, off.data(), sizeof(int) * m, hipMemcpyHostToDevice, st) != hipSuccess)
-              return fail(ctx, HVD_GPU_ERR_HIP, "table upload failed");
-            hipLaunchKernelGGL(k_traj_predict, dim3((nt + 63) / 64), dim3(64), 0, st, m, nt,
-                               (const float* const*)pt, (const float* const*)(pt + 16), (const float*)sm,
-                               (const int*)(sm + 16), vy, vx);
+            hipLaunchKernelGGL(k_traj_predict, dim3((nt + 63) / 64), dim3(64), 0, st, m, nt, tt, vy, vx);
             if (!X.launched()) return X.rc;
-            // synchronous uploads above used host vectors that go out of
-            // scope: wait for them before the next field reuses its tables
-            if (hipStreamSynchronize(st) != hipSuccess) return fail(ctx, HVD_GPU_ERR_HIP, "sync failed");
             for (int oi = 0; oi < no; ++oi) {
               const int o = offs[oi], k = j + o;
               if (k < 0 || k >= nf) continue;
@@ -1053,42 +1038,42 @@ int drive(hvd_gpu_ctx* ctx, const hvd_gpu_plane* F, int nf, const int* par, int 
             for (auto& f : vfilled) cache[(size_t)f.first * nco + f.second] = 1;
           }
         }
-        // (c) the snapped working copies (sequence.cpp:749-777)
+        // (c) the snapped working copies (sequence.cpp:749-777): every copy
+        // of the colour in one launch per kCopyMax, then each field's snap
+        CopyTab cp{};
+        int ncp = 0;
+        auto flush_copies = [&]() {
+          if (ncp == 0) return true;
+          hipLaunchKernelGGL(k_copy_tiles, dim3((2 * nt + 255) / 256, ncp), dim3(256), 0, st, 2 * nt, cp);
+          ncp = 0;
+          return X.launched();
+        };
         for (int gi = 0; gi < ng; ++gi) {
           const int j = G[gi];
           std::sort(present[gi].begin(), present[gi].end(), [&](int a, int b) { return offs[a] < offs[b]; });
           for (int oi : present[gi]) {
-            const MF src = mcache(j, oi), dst = pmf(j, oi);
-            if (hipMemcpyAsync(dst.dy, src.dy, sizeof(float) * 2 * nt, hipMemcpyDeviceToDevice, st) != hipSuccess)
-              return fail(ctx, HVD_GPU_ERR_HIP, "copy failed");
+            cp.src[ncp] = mcache(j, oi).dy;
+            cp.dst[ncp] = pmf(j, oi).dy;
+            if (++ncp == kCopyMax && !flush_copies()) return X.rc;
           }
+        }
+        if (!flush_copies()) return X.rc;
+        for (int gi = 0; gi < ng; ++gi) {
+          const int j = G[gi];
           const int m = (int)present[gi].size();
           if (c.trajectory_fit && m >= 2) {
-            std::vector<float*> dy(m), dx(m);
-            std::vector<const float*> cf(m);
-            std::vector<float> ho(m);
-            std::vector<int> off(m);
+            if (m > kMaxNeighbors) return fail(ctx, HVD_GPU_ERR_INVALID, "too many measured offsets");
+            TrajTab tt{};
             for (int q = 0; q < m; ++q) {
               const int oi = present[gi][q];
-              dy[q] = pmf(j, oi).dy;
-              dx[q] = pmf(j, oi).dx;
-              cf[q] = mcache(j, oi).conf;
-              ho[q] = ((float)par[j + offs[oi]] - (float)par[j]) / 2.0F;
-              off[q] = offs[oi];
+              tt.dy[q] = pmf(j, oi).dy;
+              tt.dx[q] = pmf(j, oi).dx;
+              tt.conf[q] = mcache(j, oi).conf;
+              tt.ho[q] = ((float)par[j + offs[oi]] - (float)par[j]) / 2.0F;
+              tt.off[q] = offs[oi];
             }
-            void** pt = ptrs + (size_t)(j + 2) * 64;
-            float* sm = small + (size_t)(j + 2) * 64;
-            if (hipMemcpyAsync(pt, dy.data(), sizeof(void*) * m, hipMemcpyHostToDevice, st) != hipSuccess ||
-                hipMemcpyAsync(pt + 16, dx.data(), sizeof(void*) * m, hipMemcpyHostToDevice, st) != hipSuccess ||
-                hipMemcpyAsync(pt + 32, cf.data(), sizeof(void*) * m, hipMemcpyHostToDevice, st) != hipSuccess ||
-                hipMemcpyAsync(sm, ho.data(), sizeof(float) * m, hipMemcpyHostToDevice, st) != hipSuccess ||
-                hipMemcpyAsync(sm + 16, off.data(), sizeof(int) * m, hipMemcpyHostToDevice, st) != hipSuccess)
-              return fail(ctx, HVD_GPU_ERR_HIP, "table upload failed");
-            hipLaunchKernelGGL(k_traj_snap, dim3((nt + 63) / 64), dim3(64), 0, st, m, nt, (float* const*)pt,
-                               (float* const*)(pt + 16), (const float* const*)(pt + 32), (const float*)sm,
-                               (const int*)(sm + 16));
+            hipLaunchKernelGGL(k_traj_snap, dim3((nt + 63) / 64), dim3(64), 0, st, m, nt, tt);
             if (!X.launched()) return X.rc;
-            if (hipStreamSynchronize(st) != hipSuccess) return fail(ctx, HVD_GPU_ERR_HIP, "sync failed");
           }
         }
       }
@@ -1120,24 +1105,24 @@ int drive(hvd_gpu_ctx* ctx, const hvd_gpu_plane* F, int nf, const int* par, int 
         }
         if (!vin.empty()) {
           const int m = (int)vin.size();
-          if (m > 64) return fail(ctx, HVD_GPU_ERR_INVALID, "more than 64 fields in one colour");
-          void** pt = ptrs;  // the first 64 x 4 slots
+          if (m > kEnvMax) return fail(ctx, HVD_GPU_ERR_INVALID, "more than 64 fields in one colour");
           // the smoothing of every plane of the colour in one launch
-          if (hipMemcpyAsync(pt, rin.data(), sizeof(void*) * 2 * m, hipMemcpyHostToDevice, st) != hipSuccess ||
-              hipMemcpyAsync(pt + 128, rout.data(), sizeof(void*) * 2 * m, hipMemcpyHostToDevice, st) != hipSuccess)
-            return fail(ctx, HVD_GPU_ERR_HIP, "table upload failed");
-          hipLaunchKernelGGL(k_smooth_rows, dim3((h + kSmR - 1) / kSmR, 2 * m), dim3(64), 0, st, h, w,
-                             (const float* const*)pt, (float* const*)(pt + 128));
+          EnvRows R{};
+          for (int q = 0; q < 2 * m; ++q) {
+            R.src[q] = rin[q];
+            R.out[q] = rout[q];
+          }
+          hipLaunchKernelGGL(k_smooth_rows, dim3((h + kSmR - 1) / kSmR, 2 * m), dim3(64), 0, st, h, w, R);
           if (!X.launched()) return X.rc;
-          if (hipMemcpyAsync(pt, vin.data(), sizeof(void*) * m, hipMemcpyHostToDevice, st) != hipSuccess ||
-              hipMemcpyAsync(pt + 64, vout.data(), sizeof(void*) * m, hipMemcpyHostToDevice, st) != hipSuccess ||
-              hipMemcpyAsync(pt + 128, lio.data(), sizeof(void*) * m, hipMemcpyHostToDevice, st) != hipSuccess ||
-              hipMemcpyAsync(pt + 192, smp.data(), sizeof(void*) * m, hipMemcpyHostToDevice, st) != hipSuccess)
-            return fail(ctx, HVD_GPU_ERR_HIP, "table upload failed");
-          hipLaunchKernelGGL(k_env_finish, dim3(m), dim3(1024), 0, st, (int)np, (const float* const*)pt,
-                             (float* const*)(pt + 64), (float* const*)(pt + 128), (float* const*)(pt + 192));
+          EnvFinish E{};
+          for (int q = 0; q < m; ++q) {
+            E.vg_in[q] = vin[q];
+            E.vgrad[q] = vout[q];
+            E.lap[q] = lio[q];
+            E.sample[q] = smp[q];
+          }
+          hipLaunchKernelGGL(k_env_finish, dim3(m), dim3(1024), 0, st, (int)np, E);
           if (!X.launched()) return X.rc;
-          if (hipStreamSynchronize(st) != hipSuccess) return fail(ctx, HVD_GPU_ERR_HIP, "sync failed");
         }
       }
       // (e) neighbour equations (sequence.cpp:941-1021), the colour's jobs
