@@ -153,6 +153,54 @@ __device__ inline const float* mplane(const MotionDesc& d, int which) {
   return which == 0 ? d.ref : d.cur;
 }
 
+// The window sums of one line (a column or a row: n samples at stride `st`
+// in a, outputs at stride `ost` in o) for positions [i0, i1), the first of
+// them given: with SEQ, hvd-core's two FP64 running sums (hi, the window's
+// far edge, gains a[i + r]; lo, its near edge, gains a[i - r - 1]; samples
+// past the ends add 0, which leaves the sums as they are) and out = (hi - lo)
+// / (2r + 1); without, one window sum (hi) that gains a[i + r] and loses
+// a[i - r - 1], exact by the line's certificate.  The next kBlurPre
+// positions' samples are loaded while the current ones are added, so the
+// chain of adds waits on no load.
+constexpr int kBlurPre = 16;
+template <bool SEQ>
+__device__ __forceinline__ void blur_line(const float* a, float* o, int n, size_t st, size_t ost, int r, double norm,
+                                          int i0, int i1, double hi, double lo) {
+#pragma clang fp contract(off)
+  o[(size_t)i0 * ost] = (float)((SEQ ? hi - lo : hi) * norm);
+  float vi[kBlurPre], vo[kBlurPre];
+  auto fetch = [&](int b, float* pi, float* po) {
+#pragma unroll
+    for (int u = 0; u < kBlurPre; ++u) {
+      const int i = b + u, ki = i + r, ko = i - r - 1;
+      pi[u] = (i < i1 && ki < n) ? a[(size_t)ki * st] : 0.0f;
+      po[u] = (i < i1 && ko >= 0) ? a[(size_t)ko * st] : 0.0f;
+    }
+  };
+  fetch(i0 + 1, vi, vo);
+  for (int b = i0 + 1; b < i1; b += kBlurPre) {
+    float ni[kBlurPre], no[kBlurPre];
+    if (b + kBlurPre < i1) fetch(b + kBlurPre, ni, no);
+#pragma unroll
+    for (int u = 0; u < kBlurPre; ++u) {
+      if (b + u >= i1) break;
+      if (SEQ) {
+        hi += (double)vi[u];
+        lo += (double)vo[u];
+      } else {
+        hi += (double)vi[u];
+        hi -= (double)vo[u];
+      }
+      o[(size_t)(b + u) * ost] = (float)((SEQ ? hi - lo : hi) * norm);
+    }
+#pragma unroll
+    for (int u = 0; u < kBlurPre; ++u) {
+      vi[u] = ni[u];
+      vo[u] = no[u];
+    }
+  }
+}
+
 }  // namespace
 
 // Box blur pass 1 (motion.cpp:21-36, BoxBlur1DRows): along y for every
@@ -206,28 +254,18 @@ __global__ void __launch_bounds__(64 * kBlurSegs) k_me_blur_cols(const MotionDes
   }
   const bool exact = emin == (1 << 20) || (double)h * (double)mx < ldexp(1.0, 53 + emin);
   if (exact) {
+    if (ya >= yb) return;
     double s = 0.0;  // the window sum of row ya (a subset sum: exact)
     for (int k = max(ya - r, 0); k <= min(ya + r, h - 1); ++k) s += (double)a[(size_t)k * w + x];
-    for (int y = ya; y < yb; ++y) {
-      o[(size_t)y * w + x] = (float)(s * (double)norm);
-      const int kin = y + r + 1, kout = y - r;
-      if (kin < h) s += (double)a[(size_t)kin * w + x];
-      if (kout >= 0) s -= (double)a[(size_t)kout * w + x];
-    }
+    blur_line<false>(a + x, o + x, h, w, w, r, (double)norm, ya, yb, s, 0.0);
     return;
   }
   if (seg != 0) return;
   // rows beyond the plane add 0, which leaves the sums (never -0) as they are
-  double c_lo = 0.0, c_hi = 0.0;
+  double c_hi = 0.0;
   const int y1 = min(r + 1, h);
   for (int i = 0; i < y1; ++i) c_hi += (double)a[(size_t)i * w + x];
-  o[x] = (float)((c_hi - c_lo) * (double)norm);
-  for (int y = 1; y < h; ++y) {
-    const int kin = y + r, kout = y - r - 1;
-    c_hi += kin < h ? (double)a[(size_t)kin * w + x] : 0.0;
-    c_lo += kout >= 0 ? (double)a[(size_t)kout * w + x] : 0.0;
-    o[(size_t)y * w + x] = (float)((c_hi - c_lo) * (double)norm);
-  }
+  blur_line<true>(a + x, o + x, h, w, w, r, (double)norm, 0, h, c_hi, 0.0);
 }
 
 // Box blur pass 2 (motion.cpp:38-55, BoxBlur1DCols): along x for every row,
@@ -295,20 +333,48 @@ __global__ void __launch_bounds__(64) k_me_blur_rows(int npairs, int h, int w, i
       const int x1 = min(x + r + 1, w), x0 = max(x - r, 0);
       o[x] = (float)((pc[x1] - pc[x0]) * (double)norm);
     }
+  } else if (12 * w + 8 <= 8 * (kBlurMaxW + 1)) {
+    // hvd-core's prefix sums c[k + 1] = c[k] + a[k] by lane 0 (the row's
+    // samples staged after them in LDS, the next kBlurPre loaded while the
+    // current ones are added), then every lane forms (c[x1] - c[x0]) / (2r + 1)
+    float* sa = reinterpret_cast<float*>(pc + w + 1);
+    for (int x = lane; x < w; x += 64) sa[x] = a[x];
+    __syncthreads();
+    if (lane == 0) {
+      double c = 0.0;
+      pc[0] = 0.0;
+      float v[kBlurPre];
+#pragma unroll
+      for (int u = 0; u < kBlurPre; ++u) v[u] = u < w ? sa[u] : 0.0f;
+      for (int b = 0; b < w; b += kBlurPre) {
+        float nv[kBlurPre];
+#pragma unroll
+        for (int u = 0; u < kBlurPre; ++u) nv[u] = b + kBlurPre + u < w ? sa[b + kBlurPre + u] : 0.0f;
+#pragma unroll
+        for (int u = 0; u < kBlurPre; ++u) {
+          if (b + u < w) {
+            c += (double)v[u];
+            pc[b + u + 1] = c;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kBlurPre; ++u) v[u] = nv[u];
+      }
+    }
+    __syncthreads();
+    for (int x = lane; x < w; x += 64) {
+      const int x1 = min(x + r + 1, w), x0 = max(x - r, 0);
+      o[x] = (float)((pc[x1] - pc[x0]) * (double)norm);
+    }
   } else if (w <= kBlurMaxW) {
     float* sa = reinterpret_cast<float*>(pc);  // the row, then its result
     float* so = sa + kBlurMaxW;
     for (int x = lane; x < w; x += 64) sa[x] = a[x];
     __syncthreads();
     if (lane == 0) {
-      double c_lo = 0.0, c_hi = 0.0;
+      double c_hi = 0.0;
       for (int i = 0; i < min(r + 1, w); ++i) c_hi += (double)sa[i];
-      so[0] = (float)((c_hi - c_lo) * (double)norm);
-      for (int x = 1; x < w; ++x) {
-        if (x + r < w) c_hi += (double)sa[x + r];
-        if (x - r - 1 >= 0) c_lo += (double)sa[x - r - 1];
-        so[x] = (float)((c_hi - c_lo) * (double)norm);
-      }
+      blur_line<true>(sa, so, w, 1, 1, r, (double)norm, 0, w, c_hi, 0.0);
     }
     __syncthreads();
     for (int x = lane; x < w; x += 64) o[x] = so[x];
