@@ -58,6 +58,7 @@ struct EnvFinish {
   float* vgrad[kEnvMax];
   float* lap[kEnvMax];
   float* sample[kEnvMax];
+  float* floor2;  // kEnvMax floats: each field's (1.5 median)^2
 };
 // Tile-map copies (dy and dx, 2 nt floats each) of one colour, by value.
 constexpr int kCopyMax = 128;
@@ -79,6 +80,7 @@ __global__ void k_traj_snap(int m, int nt, TrajTab T);
 __global__ void k_env_terms(int h, int w, const float* Y, const float2* chi, float* gmag, float* lapmag);
 __global__ void k_smooth_rows(int h, int w, EnvRows P);
 __global__ void k_env_finish(int n, EnvFinish P);
+__global__ void k_env_apply(int n, EnvFinish P);
 __global__ void k_copy_tiles(int n, CopyTab P);
 __global__ void k_split_cplx(size_t n, const float2* z, float* re, float* im);
 __global__ void k_join_cplx(size_t n, const float* re, const float* im, float2* z);

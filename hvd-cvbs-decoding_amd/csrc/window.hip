@@ -447,10 +447,10 @@ __global__ void __launch_bounds__(64) k_smooth_rows(int h, int w, EnvRows P) {
   }
 }
 
-// The envelope's noise floor and extremum factor (sequence.cpp:889-939), one
-// block per field: the median of every seventh vgrad sample, then per pixel
-// vgrad <- sqrt(max(0, vgrad^2 - (1.5 median)^2)) and lapsm = 1 - t^2 with
-// t = clamp(smoothed second difference / vgrad, 0, 1).
+// The envelope's noise floor (sequence.cpp:889-916), one block per field:
+// the median of every seventh vgrad sample, (1.5 median)^2 into floor2; then
+// (k_env_apply) per pixel vgrad <- sqrt(max(0, vgrad^2 - floor2)) and lapsm
+// = 1 - t^2 with t = clamp(smoothed second difference / vgrad, 0, 1).
 __global__ void __launch_bounds__(kSelBlock) k_env_finish(int n, EnvFinish P) {
 #pragma clang fp contract(off)
   __shared__ unsigned hist[16 * 256];
@@ -461,10 +461,19 @@ __global__ void __launch_bounds__(kSelBlock) k_env_finish(int n, EnvFinish P) {
   for (int i = threadIdx.x; i < ns; i += kSelBlock) smp[i] = v[(size_t)i * 7];
   __syncthreads();
   const float med = median_np(smp, ns, hist);
-  const float fl2 = 2.25f * med * med;
+  if (threadIdx.x == 0) P.floor2[fi] = 2.25f * med * med;
+}
+
+// The per-pixel part of the envelope (sequence.cpp:917-939) for every field
+// of the colour: grid (pixel blocks, fields).
+__global__ void __launch_bounds__(256) k_env_apply(int n, EnvFinish P) {
+#pragma clang fp contract(off)
+  const int fi = blockIdx.y;
+  const float fl2 = P.floor2[fi];
+  const float* v = P.vg_in[fi];
   float* vg = P.vgrad[fi];
   float* lp = P.lap[fi];
-  for (int i = threadIdx.x; i < n; i += kSelBlock) {
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
     const float g0 = v[i];
     const float g = sqrtf(fmaxf(0.0f, g0 * g0 - fl2));
     vg[i] = g;
@@ -802,6 +811,7 @@ int drive(hvd_gpu_ctx* ctx, const hvd_gpu_plane* F, int nf, const int* par, int 
   const size_t o_jmd = cv.take(sizeof(MotionDesc) * (3 * kJobChunk + 2));
   const size_t o_jscr = cv.take(sizeof(float) * job_floats * kJobChunk);
   const size_t o_ptr = cv.take(sizeof(void*) * 64 * (size_t)(nf + 4));       // pointer tables
+  const size_t o_envfl = cv.take(sizeof(float) * kEnvMax);                    // envelope noise floors
   char* base = pool(ctx, cv.off, st);
   if (!base) return HVD_GPU_ERR_NOMEM;
   float* sig = (float*)(base + o_sig);
@@ -827,6 +837,7 @@ int drive(hvd_gpu_ctx* ctx, const hvd_gpu_plane* F, int nf, const int* par, int 
   auto T = [&](int k) { return tmp + (size_t)k * np; };
   float* smf = (float*)(base + o_smf);
   void** ptrs = (void**)(base + o_ptr);
+  float* envfl = (float*)(base + o_envfl);
   auto cfl = [](const float* q) { return (const float2*)q; };
 
   // ---- noise self-calibration (sequence.cpp:439-449) ------------------------
@@ -1121,7 +1132,10 @@ int drive(hvd_gpu_ctx* ctx, const hvd_gpu_plane* F, int nf, const int* par, int 
             E.lap[q] = lio[q];
             E.sample[q] = smp[q];
           }
+          E.floor2 = envfl;
           hipLaunchKernelGGL(k_env_finish, dim3(m), dim3(1024), 0, st, (int)np, E);
+          hipLaunchKernelGGL(k_env_apply, dim3(std::min<size_t>((np + 255) / 256, 1024), m), dim3(256), 0, st,
+                             (int)np, E);
           if (!X.launched()) return X.rc;
         }
       }
