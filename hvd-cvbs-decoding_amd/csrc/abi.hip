@@ -1264,8 +1264,10 @@ int hvd_gpu_create(int device, hvd_gpu_ctx** out) {
     int lds = 0;
     HIPCHK(ctx, hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
     ctx->max_dyn_lds = lds;
-    for (const void* k : {(const void*)k_fft2_rows, (const void*)k_fft2_cols})
+    for (const void* k : {(const void*)k_fft2_rows, (const void*)k_fft2_cols, (const void*)k_me_conf_tiles})
       HIPCHK(ctx, hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+    REQUIRE(ctx, (size_t)lds >= kConfBandMaxLds, "the device has %d bytes of LDS per block (needs %zu)", lds,
+            kConfBandMaxLds);
     HIPCHK(ctx, fft_allow_lds(lds));
     HIPCHK(ctx, hipFuncSetAttribute((const void*)k_aniso_select, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     std::max(0, lds - kSelStaticLds)));
@@ -2865,8 +2867,8 @@ int hvd_gpu_estimate_motion(hvd_gpu_ctx* ctx, const float* const* y_ref, const f
                        (const float*)cost0, mdy, mdx, best, ody, odx);
     LAUNCH_CHECK(ctx);
     // (8) confidence (:321-387, :475)
-    hipLaunchKernelGGL(k_me_confidence, dim3(n), dim3(256), 0, st, h, w, tile, (const float*)P, (const float*)best,
-                       (float* const*)(base + oC), scratch);
+    launch_me_confidence(st, n, h, w, tile, (const float*)P, (const float*)best, (float* const*)(base + oC),
+                         scratch);
     LAUNCH_CHECK(ctx);
     for (int i = 0; i < n; ++i) {
       HIPCHK(ctx, hipMemcpyAsync(dy[i], ody + (size_t)i * nt, sizeof(float) * nt, hipMemcpyDeviceToDevice, st));
@@ -3031,8 +3033,8 @@ int hvd_gpu_verify_motion(hvd_gpu_ctx* ctx, const float* const* y_ref, const flo
     hipLaunchKernelGGL(k_verify_pick, dim3(blocks_for((size_t)n * nt)), dim3(256), 0, st, n * nt, (const float*)py,
                        (const float*)px, (const float*)bp, (const float*)bz, ody, odx, best);
     LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_me_confidence, dim3(n), dim3(256), 0, st, h, w, tile, (const float*)P, (const float*)best,
-                       (float* const*)(base + oC), vec(9) /* scratch: 2 n nt floats from vec(9) */);
+    launch_me_confidence(st, n, h, w, tile, (const float*)P, (const float*)best, (float* const*)(base + oC),
+                         vec(9) /* scratch: 2 n nt floats from vec(9) */);
     LAUNCH_CHECK(ctx);
     for (int i = 0; i < n; ++i) {
       HIPCHK(ctx, hipMemcpyAsync(dy[i], ody + (size_t)i * nt, sizeof(float) * nt, hipMemcpyDeviceToDevice, st));

@@ -330,6 +330,13 @@ __global__ void k_me_finish(int n, const float* cym, const float* cyp, const flo
                             const float* best0, const float* cost0, float* mdy, float* mdx, float* best, float* out_dy,
                             float* out_dx);
 __global__ void k_me_confidence(int h, int w, int tile, const float* planes, const float* best, float* const* conf_out,
-                                float* scratch);
+                                float* scratch, int have_tiles);
+__global__ void k_me_conf_tiles(int h, int w, int tile, const float* planes, const float* best, float* scratch);
+// MotionConfidence of n pairs: the per-tile moments from LDS-staged tile
+// rows when a band of tile x w floats fits kConfBandMaxLds, then the median
+// and scalings (motion.hip).
+constexpr size_t kConfBandMaxLds = 144 * 1024;
+void launch_me_confidence(hipStream_t st, int n, int h, int w, int tile, const float* planes, const float* best,
+                          float* const* conf_out, float* scratch);
 
 }  // namespace hvdk

@@ -390,6 +390,18 @@ __global__ void __launch_bounds__(64) k_me_blur_rows(int npairs, int h, int w, i
   }
 }
 
+void launch_me_confidence(hipStream_t st, int n, int h, int w, int tile, const float* planes, const float* best,
+                          float* const* conf_out, float* scratch) {
+  const size_t lds = sizeof(float) * (size_t)tile * w;
+  const bool band = lds <= kConfBandMaxLds;
+  if (band) {
+    const int th = (h + tile - 1) / tile;
+    hipLaunchKernelGGL(k_me_conf_tiles, dim3(th, n), dim3(256), lds, st, h, w, tile, planes, best, scratch);
+  }
+  hipLaunchKernelGGL(k_me_confidence, dim3(n), dim3(256), 0, st, h, w, tile, planes, best, conf_out, scratch,
+                     band ? 1 : 0);
+}
+
 void launch_me_blur(hipStream_t st, const MotionDesc* D, int npairs, int h, int w, int r, float* tmp, float* out) {
   launch_me_blur_planes(st, D, 2 * npairs, h, w, r, tmp, out);
 }
@@ -576,11 +588,54 @@ __global__ void k_me_finish(int n, const float* cym, const float* cyp, const flo
   out_dx[i] = (vx + sdx);
 }
 
+// MotionConfidence's per-tile moments for one tile row of one pair: the
+// band of `tile` sample rows is staged in LDS with coalesced loads, then one
+// thread per tile adds its samples in the reference's raster order (the
+// same adds as k_me_confidence's own loop, motion.cpp:330-350).  Grid: (tile
+// rows, pairs); dynamic LDS: tile * w floats.
+__global__ void __launch_bounds__(256) k_me_conf_tiles(int h, int w, int tile, const float* planes, const float* best,
+                                                       float* scratch) {
+  extern __shared__ float band[];
+  const int ty = blockIdx.x, pi = blockIdx.y;
+  const int th = (h + tile - 1) / tile, tw = (w + tile - 1) / tile, nt = th * tw;
+  const float* A = planes + ((size_t)pi * 2 + 1) * h * w;
+  const int y0 = ty * tile, nr = min(y0 + tile, h) - y0;
+  for (int i = threadIdx.x; i < nr * w; i += 256) band[i] = A[(size_t)y0 * w + i];
+  __syncthreads();
+  float* ratio = scratch + (size_t)pi * nt * 2;
+  float* conf = ratio + nt;
+  for (int tx = threadIdx.x; tx < tw; tx += 256) {
+    const int x0 = tx * tile, x1 = min(x0 + tile, w);
+    float sum = 0.0f, sum2 = 0.0f, np = 0.0f;
+    for (int y = 0; y < nr; ++y) {
+      const float* r = band + (size_t)y * w;
+#pragma unroll 8
+      for (int x = x0; x < x1; ++x) {
+        const float v = r[x];
+        sum = (sum + v);
+        sum2 = (sum2 + (v * v));
+        np = (np + 1.0f);
+      }
+    }
+    const int t = ty * tw + tx;
+    const float b = best[(size_t)pi * nt + t];
+    const float npc = fmaxf(1.0f, np);
+    const float norm = (1.0f / npc);
+    const float mean = (sum * norm);
+    const float var = ((sum2 * norm) - (mean * mean));
+    const float energy = ((var + 1.0f) * npc);
+    const float c = (1.0f - (b / energy));
+    conf[t] = fminf(fmaxf(c, 0.0f), 1.0f);
+    ratio[t] = (b / npc);
+  }
+}
+
 // MotionConfidence (motion.cpp:321-387).  One block per pair; per-tile sums
-// in the reference's raster order, the numpy median of best / npix, then the
-// outlier and global-validity scalings.  scratch: 2 * nt floats per pair.
+// in the reference's raster order (unless k_me_conf_tiles formed them:
+// have_tiles), the numpy median of best / npix, then the outlier and
+// global-validity scalings.  scratch: 2 * nt floats per pair.
 __global__ void __launch_bounds__(256) k_me_confidence(int h, int w, int tile, const float* planes, const float* best,
-                                                       float* const* conf_out, float* scratch) {
+                                                       float* const* conf_out, float* scratch, int have_tiles) {
   const int pi = blockIdx.x;
   const int th = (h + tile - 1) / tile, tw = (w + tile - 1) / tile, nt = th * tw;
   const float* A = planes + ((size_t)pi * 2 + 1) * h * w;
@@ -588,18 +643,46 @@ __global__ void __launch_bounds__(256) k_me_confidence(int h, int w, int tile, c
   float* ratio = scratch + (size_t)pi * nt * 2;
   float* conf = ratio + nt;
   __shared__ float s_lo, s_hi;
-  for (int t = threadIdx.x; t < nt; t += blockDim.x) {
+  for (int t = have_tiles ? nt : (int)threadIdx.x; t < nt; t += blockDim.x) {
     const int ty = t / tw, tx = t % tw;
     const int y0 = ty * tile, y1 = min((ty + 1) * tile, h);
     const int x0 = tx * tile, x1 = min((tx + 1) * tile, w);
     float sum = 0.0f, sum2 = 0.0f, np = 0.0f;
-    for (int y = y0; y < y1; ++y)
-      for (int x = x0; x < x1; ++x) {
-        const float v = A[(size_t)y * w + x];
-        sum = (sum + v);
-        sum2 = (sum2 + (v * v));
-        np = (np + 1.0f);
+    if (x1 - x0 == 32 && (w & 3) == 0) {
+      // full 32-sample tile rows as eight 16-byte loads, the next row's in
+      // flight while this one is added (the same adds in the same order)
+      const float4* row = reinterpret_cast<const float4*>(A + (size_t)y0 * w + x0);
+      const size_t pitch = (size_t)w / 4;
+      float4 cur[8], nxt[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) cur[k] = row[k];
+      for (int y = y0; y < y1; ++y) {
+        if (y + 1 < y1) {
+#pragma unroll
+          for (int k = 0; k < 8; ++k) nxt[k] = row[(size_t)(y + 1 - y0) * pitch + k];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const float q[4] = {cur[k].x, cur[k].y, cur[k].z, cur[k].w};
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            sum = (sum + q[c]);
+            sum2 = (sum2 + (q[c] * q[c]));
+            np = (np + 1.0f);
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) cur[k] = nxt[k];
       }
+    } else {
+      for (int y = y0; y < y1; ++y)
+        for (int x = x0; x < x1; ++x) {
+          const float v = A[(size_t)y * w + x];
+          sum = (sum + v);
+          sum2 = (sum2 + (v * v));
+          np = (np + 1.0f);
+        }
+    }
     const float npc = fmaxf(1.0f, np);
     const float norm = (1.0f / npc);
     const float mean = (sum * norm);
