@@ -62,7 +62,7 @@ def test_ambiguous_boxes_match_oracle(engine, name):
         assert got == _as_tuples(SEL.ambiguous_boxes(Y, chi, area))
 
 
-def _run_gpu(engine, fields, parities, inits, cfg, max_area):
+def _run_gpu(engine, fields, parities, inits, cfg, max_area, pal=False):
     planes, outs = [], []
     for (s, car), chi0 in zip(fields, inits):
         dchi, dy = cdev(chi0), torch.zeros(s.shape, device=DEV)
@@ -71,20 +71,22 @@ def _run_gpu(engine, fields, parities, inits, cfg, max_area):
     sp = gpu.SolverParams.make(cg_iterations=cfg.cg_iterations, irls_outer=cfg.irls_outer, cg_tol=cfg.cg_tol,
                                fast=int(cfg.fast))
     wp = gpu.WindowParams.make(temporal_strength=cfg.temporal_strength, selective_3d=1, selective_max_area=max_area)
-    d = engine.decode_window(planes, parities, False, sp, wp)
+    d = engine.decode_window(planes, parities, pal, sp, wp)
     torch.cuda.synchronize()
     return [y.cpu().numpy() for y, _ in outs], [chost(c) for _, c in outs], d
 
 
-@pytest.mark.parametrize("max_area", [0.75, 0.45])
-def test_selective_window_matches_oracle(engine, orc32, max_area):
+@pytest.mark.parametrize("pal,max_area", [(False, 0.75), (False, 0.45), (True, 0.75)])
+def test_selective_window_matches_oracle(engine, orc32, pal, max_area):
     from sequence_cases import window
-    fields, par, inits, _ = window(lines=128, width=384, nf=6, amb_region=(150, 220, 220, 330), seed=3)
+    fields, par, inits, _ = window(lines=128, width=384, nf=8 if pal else 6, amb_region=(150, 220, 220, 330), seed=3,
+                                   pal=pal)
     cfg = S3.WindowCfg(cg_iterations=20, temporal_strength=1.0)
     ry, rc, boxes = SEL.decode_window_selective(orc32, fields, par, inits, None, cfg,
-                                                lambda s, c: orc32.holographic_init(s, c, SR), max_area=max_area)
-    print("selective window", max_area, boxes, flush=True)
-    gy, gc, d = _run_gpu(engine, fields, par, inits, cfg, max_area)
+                                                lambda s, c: orc32.holographic_init(s, c, SR), is_pal=pal,
+                                                max_area=max_area)
+    print("selective window", "pal" if pal else "ntsc", max_area, boxes, flush=True)
+    gy, gc, d = _run_gpu(engine, fields, par, inits, cfg, max_area, pal)
     want = -1 if boxes == [SEL.Box()] else len(boxes)
     assert int(d["selective_boxes"]) == want
     for a, b in list(zip(gy, ry)) + list(zip(gc, rc)):
