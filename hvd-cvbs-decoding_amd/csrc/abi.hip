@@ -2759,7 +2759,10 @@ void launch_me_ssd(hipStream_t st, int n, int h, int w, int tile, const float* P
                    int ndy, int ndx, const float* bdy, const float* bdx, float* se) {
   const int th = (h + tile - 1) / tile, tw = (w + tile - 1) / tile;
   const size_t items = (size_t)n * th * tw * ndy * ndx;
-  if (tile >= 16)
+  if (tile >= 16 && tile <= kSsdTile && ndy * ndx >= 8 && tile + ndy - 1 <= kSsdWin && tile + ndx - 1 <= kSsdWin)
+    hipLaunchKernelGGL(k_me_ssd_tile, dim3((unsigned)((size_t)n * th * tw)), dim3(64), 0, st, n, h, w, tile, P, dys,
+                       dxs, ndy, ndx, bdy, bdx, se);
+  else if (tile >= 16)
     hipLaunchKernelGGL(k_me_ssd_wave, dim3((unsigned)((items + 3) / 4)), dim3(256), 0, st, n, h, w, tile, P, dys, dxs,
                        ndy, ndx, bdy, bdx, se);
   else

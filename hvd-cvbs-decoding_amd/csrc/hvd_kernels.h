@@ -318,6 +318,12 @@ void launch_me_blur_planes(hipStream_t st, const MotionDesc* D, int nplanes, int
 __global__ void k_me_decimate(int npairs, int h, int w, int f, const float* in, float* out);
 __global__ void k_me_ssd_wave(int npairs, int h, int w, int tile, const float* planes, const int* dys,
                               const int* dxs, int ndy, int ndx, const float* base_dy, const float* base_dx, float* se);
+// The staged SSD form (one wave per tile, a lane per candidate): tiles up to
+// kSsdTile, windows (tile plus the candidates' span) up to kSsdWin.
+constexpr int kSsdTile = 32;
+constexpr int kSsdWin = 48;
+__global__ void k_me_ssd_tile(int npairs, int h, int w, int tile, const float* planes, const int* dys, const int* dxs,
+                              int ndy, int ndx, const float* base_dy, const float* base_dx, float* se);
 __global__ void k_me_ssd(int npairs, int h, int w, int tile, const float* planes, const int* dys, const int* dxs,
                          int ndy, int ndx, const float* base_dy, const float* base_dx, float* se);
 __global__ void k_me_select(int npairs, int nt, const int* dys, const int* dxs, int ndy, int ndx, const float* se,

@@ -512,6 +512,79 @@ __global__ void __launch_bounds__(256) k_me_ssd_wave(int npairs, int h, int w, i
   if (lane == 0) se[idx] = acc;
 }
 
+// The SSDs of full-size tiles with many candidates (the refinement pass),
+// one wave per (pair, tile): the tile of A and the window of B that every
+// candidate reaches (rows and columns clamped to the plane, as BmPass clamps
+// them) are staged in LDS, then lane c sums candidate c's squared
+// differences in the tile's raster order (hvd-core's adds, motion.cpp:
+// 105-260), the candidates' chains side by side.  Candidate offsets span a
+// contiguous range (RangeFromCentre): the window is the tile grown by the
+// span; a window beyond kSsdWin is summed from global memory instead.
+// Grid: npairs * tiles blocks of 64.
+__global__ void __launch_bounds__(64) k_me_ssd_tile(int npairs, int h, int w, int tile, const float* planes,
+                                                    const int* dys, const int* dxs, int ndy, int ndx,
+                                                    const float* base_dy, const float* base_dx, float* se) {
+  __shared__ float sa[kSsdTile * kSsdTile];
+  __shared__ float sb[kSsdWin * kSsdWin];
+  const int th = (h + tile - 1) / tile, tw = (w + tile - 1) / tile, nt = th * tw, nc = ndy * ndx;
+  const int t = (int)(blockIdx.x % nt), pi = (int)(blockIdx.x / nt), lane = threadIdx.x;
+  if (pi >= npairs) return;
+  const int ty = t / tw, tx = t % tw;
+  const int y0 = ty * tile, x0 = tx * tile;
+  const int ch = min(y0 + tile, h) - y0, cw = min(x0 + tile, w) - x0;
+  int by = 0, bx = 0;
+  if (base_dy) {
+    by = (int)lroundf(base_dy[(size_t)pi * nt + t]);
+    bx = (int)lroundf(base_dx[(size_t)pi * nt + t]);
+  }
+  int dymin = dys[0], dymax = dys[0], dxmin = dxs[0], dxmax = dxs[0];
+  for (int i = 1; i < ndy; ++i) {
+    dymin = min(dymin, dys[i]);
+    dymax = max(dymax, dys[i]);
+  }
+  for (int i = 1; i < ndx; ++i) {
+    dxmin = min(dxmin, dxs[i]);
+    dxmax = max(dxmax, dxs[i]);
+  }
+  const int symax = dymax + by, sxmax = dxmax + bx;
+  const int wh = ch + dymax - dymin, ww = cw + dxmax - dxmin;
+  const float* B = planes + ((size_t)pi * 2 + 0) * h * w;
+  const float* A = planes + ((size_t)pi * 2 + 1) * h * w;
+  const bool staged = ch * cw <= kSsdTile * kSsdTile && wh <= kSsdWin && ww <= kSsdWin;
+  if (staged) {
+    for (int e = lane; e < ch * cw; e += 64) sa[e] = A[(size_t)(y0 + e / cw) * w + x0 + e % cw];
+    for (int e = lane; e < wh * ww; e += 64) {
+      const int yy = min(max(y0 - symax + e / ww, 0), h - 1), xx = min(max(x0 - sxmax + e % ww, 0), w - 1);
+      sb[e] = B[(size_t)yy * w + xx];
+    }
+  }
+  __syncthreads();
+  for (int c = lane; c < nc; c += 64) {
+    const int sy = dys[c / ndx] + by, sx = dxs[c % ndx] + bx;
+    float acc = 0.0f;
+    if (staged) {
+      const int ro = symax - sy, co = sxmax - sx;
+      for (int y = 0; y < ch; ++y) {
+        const float* ar = sa + y * cw;
+        const float* br = sb + (y + ro) * ww + co;
+#pragma unroll 8
+        for (int x = 0; x < cw; ++x) {
+          const float diff = (ar[x] - br[x]);
+          acc = (acc + (diff * diff));
+        }
+      }
+    } else {
+      for (int y = y0; y < y0 + ch; ++y)
+        for (int x = x0; x < x0 + cw; ++x) {
+          const int yy = min(max(y - sy, 0), h - 1), xx = min(max(x - sx, 0), w - 1);
+          const float diff = (A[(size_t)y * w + x] - B[(size_t)yy * w + xx]);
+          acc = (acc + (diff * diff));
+        }
+    }
+    se[((size_t)pi * nt + t) * nc + c] = acc;
+  }
+}
+
 // In-order candidate selection (motion.cpp:233-252): cost = se * bias,
 // strict '<'.  Thread per (pair, tile).  cost0 (may be null) receives the
 // SSD of the (0, 0) candidate.
